@@ -1,0 +1,177 @@
+/* gz_grav.h -- C ABI of the MI355X tree-gravity library (libgzgrav.so).
+ *
+ * Drop-in boundary for the shared-memory GIZMO tree (shmem/tree.h): Morton keys + sort, octree
+ * build + node moments, the cubic-spline-softened grouped and per-target tree walks, the exact
+ * direct sum, and the caller-side target ordering / acceleration write-back / kick
+ * (shmem/mfm.cc:679-905, 3699-3813). Plain pointers and sizes only.
+ *
+ * Conventions
+ *   - Every call returns gz_status; GZ_OK is 0. gz_last_error() describes the most recent failure
+ *     on the calling thread. Nothing calls exit() or abort() (the reference's CUDA path exits on a
+ *     launch error, shmem/grav_cuda.cu:239-243).
+ *   - Array arguments are DEVICE pointers (HBM) unless a name says host_. Calls are ordered on the
+ *     context's stream; functions that return host-visible results (counts, stats) synchronise it.
+ *   - Outputs are caller-allocated. Accelerations INCLUDE the G factor, as shmem::accel and
+ *     shmem::accel_grouped return them (tree.cc:580, :880); aold is in no-G units (tree.h:389-391).
+ *   - Softening is the cubic-spline kernel EXTENT h per particle (tree.h:52-67).
+ *   - Node ids are level-major (breadth-first): the children of a node are contiguous and in octant
+ *     order. The reference numbers nodes in build-task order; gz_tree_download also returns the
+ *     reference-style threaded next[] so either numbering walks the same tree.
+ */
+#ifndef GZ_GRAV_H
+#define GZ_GRAV_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+    GZ_OK = 0,
+    GZ_ERR_INVALID = 1,     /* bad argument (null pointer, size out of range)             */
+    GZ_ERR_HIP = 2,         /* a HIP runtime call failed                                  */
+    GZ_ERR_NOMEM = 3,       /* device allocation failed                                   */
+    GZ_ERR_UNSUPPORTED = 4, /* a reference option this build does not implement (zeta...) */
+    GZ_ERR_ARENA = 5,       /* node arena exhausted and growth was disabled               */
+    GZ_ERR_INTERNAL = 6     /* a device-side invariant check failed (reported, not faulted)*/
+} gz_status;
+
+typedef struct gz_ctx gz_ctx;   /* one per device + stream */
+typedef struct gz_tree gz_tree; /* device-resident tree, owned by the library */
+
+/* Message for the last failing call on this thread ("" if none). */
+const char* gz_last_error(void);
+const char* gz_status_string(gz_status s);
+
+/* ---- context ------------------------------------------------------------------------------- */
+/* stream: a hipStream_t, or NULL for a stream the context creates and owns. */
+gz_status gz_ctx_create(int device, void* stream, gz_ctx** out);
+gz_status gz_ctx_destroy(gz_ctx* ctx);
+gz_status gz_ctx_synchronize(gz_ctx* ctx);
+
+/* ---- particles (shmem::Particles, tree.h:36-50) --------------------------------------------- */
+typedef struct {
+    size_t n;
+    const double *x, *y, *z; /* positions, FP64                                        */
+    const double* m;         /* masses; m <= 0 marks a tombstone left out of the tree   */
+    const double* soft;      /* kernel extent per particle; NULL means 0                */
+    const double* zeta;      /* adaptive-softening zeta; must be NULL (GZ_ERR_UNSUPPORTED) */
+    const uint8_t* type;     /* GIZMO type; NULL = all gas. Only read for sink counts.   */
+} gz_particles;
+
+/* ---- build (shmem::build, tree.h:384-385, tree.cc:350-526) ---------------------------------- */
+typedef struct {
+    long long randomize_seed; /* >= 0: RANDOMIZE_GRAVTREE root shift (tree.cc:381-393); -1 off */
+    double shift[3];          /* SHMEM_TREE_SHIFT fractions (tree.cc:400-407); 0 = off         */
+    int max_nodes;            /* 0: grow the node arena as needed (the reference's retry,
+                                 tree.cc:438-462); > 0: fail with GZ_ERR_ARENA beyond this     */
+} gz_build_opts;
+
+/* Stage timings in ms, the reference's BuildTimes (tree.h:371) measured with HIP events. */
+typedef struct {
+    double bbox, keys, sort, recurse, links, pack, total;
+} gz_build_times;
+
+typedef struct {
+    long long n, n_live, nnodes, nleaves, root, depth, arena_grows;
+    double root_centre[3], root_side;
+    gz_build_times times;
+} gz_tree_info;
+
+/* Build a tree over p (device arrays). opts may be NULL (defaults: no randomisation). */
+gz_status gz_tree_build(gz_ctx* ctx, const gz_particles* p, const gz_build_opts* opts,
+                        gz_tree** out);
+gz_status gz_tree_free(gz_tree* tree);
+gz_status gz_tree_get_info(const gz_tree* tree, gz_tree_info* info);
+
+/* Copy tree arrays to HOST buffers (any may be NULL). Node arrays have nnodes entries.
+ * first = first child or -1; next = the reference's threaded successor (tree.cc:340-348);
+ * nchild = number of children; plo/phi = particle range in Morton order; wn_s = size + delta,
+ * wn_soft/wn_len = the float fields of the packed walk node (tree.h:195-204). */
+gz_status gz_tree_download_nodes(const gz_tree* tree, int* first, int* next, int* nchild,
+                                 int* plo, int* phi, int* parent, int* level, double* size,
+                                 double* mass, double* cx, double* cy, double* cz, double* soft,
+                                 double* delta, double* gcx, double* gcy, double* gcz,
+                                 double* wn_s, float* wn_soft, float* wn_len);
+/* orderbuf (n_live), rank (n; UINT32_MAX for tombstones), leaf_of (n; -1), skey (n_live). */
+gz_status gz_tree_download_particles(const gz_tree* tree, uint32_t* orderbuf, uint32_t* rank,
+                                     int* leaf_of, uint64_t* skey);
+/* Device pointers owned by the tree (valid until gz_tree_free). */
+gz_status gz_tree_device_arrays(const gz_tree* tree, const uint32_t** orderbuf,
+                                const uint32_t** rank, const int** leaf_of);
+
+/* ---- walks -------------------------------------------------------------------------------- */
+typedef enum {
+    GZ_PREC_FP64 = 0,    /* FP64 throughout: the parity walk                                 */
+    GZ_PREC_FP32_REL = 1 /* FP32 interactions on group-relative positions; FP64 group centre,
+                            node COM and final per-target sum                                  */
+} gz_precision;
+
+typedef enum {
+    GZ_CRIT_REFERENCE = 0, /* shmem open_node (tree.h:109-144)                                 */
+    GZ_CRIT_LEARNED = 1    /* per-group theta from the learned model, reference as fallback    */
+} gz_criterion;
+
+typedef struct {
+    int precision;   /* gz_precision                                                           */
+    int criterion;   /* gz_criterion                                                           */
+    int group_size;  /* batch <= 0 grouping: 0 = one group per leaf (the reference); G > 0 =
+                        targets under the highest node holding <= G particles                  */
+    int persistent;  /* 1 = persistent work-queue launch; 0 = one workgroup per work item      */
+    const void* model; /* gz_model* when criterion == GZ_CRIT_LEARNED                          */
+} gz_walk_opts;
+
+/* SHMEM_COUNT_WALK (tree.cc:671-674, :891-895): nodes tested, multipole interactions, pair
+ * interactions, summed over targets; plus launch figures measured on the device. */
+typedef struct {
+    long long nodes_visited, multipole, pairs, groups, items;
+    double walk_ms;    /* walk kernel duration, HIP events on the context stream */
+    double prep_ms;    /* grouping + group bounds */
+} gz_walk_counts;
+
+/* shmem::accel_grouped (tree.h:428-440, tree.cc:626-897) with tidal = lazy = jerk = NULL and
+ * sink_direct_radius = 0. targets: n_t particle indices (device); aold: n_t values or NULL;
+ * batch <= 0 groups targets by leaf (node-aligned), batch > 0 uses fixed windows.
+ * opts may be NULL (FP64, reference criterion, leaf groups). counts may be NULL. */
+gz_status gz_tree_accel_grouped(gz_ctx* ctx, const gz_tree* tree, const gz_particles* p,
+                                const uint32_t* targets, size_t n_t, double theta, double G,
+                                int batch, const double* aold, double* ax, double* ay, double* az,
+                                const gz_walk_opts* opts, gz_walk_counts* counts);
+
+/* shmem::accel (tree.h:392-394, tree.cc:528-582): per-target walk, a reached leaf is always
+ * direct-summed. */
+gz_status gz_tree_accel(gz_ctx* ctx, const gz_tree* tree, const gz_particles* p,
+                        const uint32_t* targets, size_t n_t, double theta, double G,
+                        const double* aold, double* ax, double* ay, double* az);
+
+/* shmem::accel_brute (tree.h:450-451, tree.cc:946-971): exact O(n_t * N) sum in FP64 with the
+ * walk's pair rule, in source index order; includes G. This is the error judge. The reference's
+ * device counterpart is shmem_cuda_accel_bruteforce (grav_cuda.cu:186-192), which omits G. */
+gz_status gz_direct_sum(gz_ctx* ctx, const gz_particles* p, const uint32_t* targets, size_t n_t,
+                        double G, double* ax, double* ay, double* az);
+
+/* ---- caller side: compute_gravity + kick (mfm.cc:679-905, 3699-3813) ---------------------- */
+/* Active set in the tree's Morton order (mfm.cc:700-718). active: n_active particle indices in
+ * any order (NULL with n_active == N means all active). out_targets: room for n_active. */
+gz_status gz_order_targets(gz_ctx* ctx, const gz_tree* tree, const uint32_t* active,
+                           size_t n_active, uint32_t* out_targets, size_t* n_out);
+/* aold[k] = err_tol_force_acc * |a_prev[targets[k]]| / G (mfm.cc:743-751); a_prev is N x 3
+ * interleaved (Vec3d per particle, the engine's a_grav). */
+gz_status gz_aold_from_accel(gz_ctx* ctx, const double* a_prev, const uint32_t* targets,
+                             size_t n_t, double err_tol_force_acc, double G, double* aold);
+/* a_grav[targets[k]] = (ax[k], ay[k], az[k]) (mfm.cc:828-831); a_grav N x 3 interleaved. */
+gz_status gz_scatter_accel(gz_ctx* ctx, const uint32_t* targets, size_t n_t, const double* ax,
+                           const double* ay, const double* az, double* a_grav);
+/* KDK kick of the active set (mfm.cc:3749-3767, gravity part): v += a * (close_dt + open_dt),
+ * with a = a_grav[i] (+ a_extra[i] when non-NULL), both N x 3 interleaved; dt per active slot;
+ * pending_half_kick[i] is consumed as close_dt and replaced by 0.5 * dt[k]. */
+gz_status gz_kick(gz_ctx* ctx, const uint32_t* active, size_t n_active, const double* a_grav,
+                  const double* a_extra, const double* dt, double* pending_half_kick, double* vx,
+                  double* vy, double* vz);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* GZ_GRAV_H */

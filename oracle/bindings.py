@@ -1,0 +1,370 @@
+"""ctypes bindings for the two CPU checkers (test infrastructure only).
+
+``Oracle`` wraps ``liboracle.so`` (the C restatement, symbol prefix ``gzo_``) and ``Reference``
+wraps ``_ref/libgzref.so`` (the reference's ``shmem/tree.cc``, prefix ``gzref_``). Both expose the
+same methods so a test can run one input through either and compare.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ORACLE_SO = os.path.join(HERE, "liboracle.so")
+REF_SO = os.path.join(HERE, "_ref", "libgzref.so")
+
+_dp = C.POINTER(C.c_double)
+_fp = C.POINTER(C.c_float)
+_ip = C.POINTER(C.c_int)
+_up = C.POINTER(C.c_uint32)
+_u64p = C.POINTER(C.c_uint64)
+_llp = C.POINTER(C.c_longlong)
+
+
+def _ptr(a, ctype):
+    if a is None:
+        return None
+    return a.ctypes.data_as(ctype)
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+@dataclass
+class HostParticles:
+    """Host SoA particle set (the reference's ``Particles``, tree.h:36-50)."""
+
+    x: np.ndarray
+    y: np.ndarray
+    z: np.ndarray
+    m: np.ndarray
+    soft: np.ndarray | None = None
+
+    def __post_init__(self):
+        self.x, self.y, self.z, self.m = (_f64(v) for v in (self.x, self.y, self.z, self.m))
+        if self.soft is not None:
+            self.soft = _f64(self.soft)
+
+    @property
+    def n(self) -> int:
+        return int(self.m.shape[0])
+
+
+class _GzoParticles(C.Structure):
+    _fields_ = [("n", C.c_size_t), ("x", _dp), ("y", _dp), ("z", _dp), ("m", _dp),
+                ("soft", _dp)]
+
+
+@dataclass
+class TreeArrays:
+    """Node and particle arrays of one built tree, as numpy arrays."""
+
+    nnodes: int
+    root: int
+    n_live: int
+    first: np.ndarray
+    next: np.ndarray
+    plo: np.ndarray
+    phi: np.ndarray
+    parent: np.ndarray
+    size: np.ndarray
+    mass: np.ndarray
+    cx: np.ndarray
+    cy: np.ndarray
+    cz: np.ndarray
+    soft: np.ndarray
+    delta: np.ndarray
+    gcx: np.ndarray
+    gcy: np.ndarray
+    gcz: np.ndarray
+    wn_s: np.ndarray
+    wn_soft: np.ndarray
+    wn_len: np.ndarray
+    orderbuf: np.ndarray
+    rank: np.ndarray
+    leaf_of: np.ndarray
+
+
+class _Lib:
+    prefix = ""
+
+    def __init__(self, path: str):
+        if not os.path.exists(path):
+            raise FileNotFoundError(f"{path} is not built (run `make -C oracle`)")
+        self.lib = C.CDLL(path)
+        self.path = path
+        L, p = self.lib, self.prefix
+        getattr(L, p + "morton").restype = C.c_uint64
+        getattr(L, p + "morton").argtypes = [C.c_uint32] * 3
+        getattr(L, p + "spline_force_over_r").restype = C.c_double
+        getattr(L, p + "spline_force_over_r").argtypes = [C.c_double, C.c_double]
+        getattr(L, p + "spline_potential").restype = C.c_double
+        getattr(L, p + "spline_potential").argtypes = [C.c_double, C.c_double]
+        getattr(L, p + "open_node").restype = C.c_int
+        getattr(L, p + "open_node").argtypes = [C.c_double] * 11
+        getattr(L, p + "tree_free").argtypes = [C.c_void_p]
+        getattr(L, p + "tree_sizes").argtypes = [C.c_void_p, _llp]
+        getattr(L, p + "tree_nodes").argtypes = [C.c_void_p] + [_ip] * 5 + [_dp] * 11 + [_fp, _fp]
+        getattr(L, p + "tree_particles").argtypes = [C.c_void_p, _up, _up, _ip]
+
+    # -- scalar kernels ------------------------------------------------------------------------
+    def morton(self, a: int, b: int, c: int) -> int:
+        return int(getattr(self.lib, self.prefix + "morton")(a, b, c))
+
+    def spline_force_over_r(self, r: float, h: float) -> float:
+        return getattr(self.lib, self.prefix + "spline_force_over_r")(r, h)
+
+    def spline_potential(self, r: float, h: float) -> float:
+        return getattr(self.lib, self.prefix + "spline_potential")(r, h)
+
+    def open_node(self, *args) -> bool:
+        return bool(getattr(self.lib, self.prefix + "open_node")(*[float(a) for a in args]))
+
+    # -- trees ---------------------------------------------------------------------------------
+    def _tree_arrays(self, handle) -> TreeArrays:
+        p = self.prefix
+        sz = np.zeros(3, dtype=np.int64)
+        getattr(self.lib, p + "tree_sizes")(handle, _ptr(sz, _llp))
+        nn, root, nl = (int(v) for v in sz)
+        ints = {k: np.zeros(nn, dtype=np.int32) for k in ("first", "next", "plo", "phi", "parent")}
+        dbls = {k: np.zeros(nn, dtype=np.float64) for k in
+                ("size", "mass", "cx", "cy", "cz", "soft", "delta", "gcx", "gcy", "gcz", "wn_s")}
+        flts = {k: np.zeros(nn, dtype=np.float32) for k in ("wn_soft", "wn_len")}
+        getattr(self.lib, p + "tree_nodes")(
+            handle, *[_ptr(ints[k], _ip) for k in ("first", "next", "plo", "phi", "parent")],
+            *[_ptr(dbls[k], _dp) for k in ("size", "mass", "cx", "cy", "cz", "soft", "delta",
+                                           "gcx", "gcy", "gcz", "wn_s")],
+            _ptr(flts["wn_soft"], _fp), _ptr(flts["wn_len"], _fp))
+        return TreeArrays(nnodes=nn, root=root, n_live=nl, orderbuf=None, rank=None,
+                          leaf_of=None, **ints, **dbls, **flts)
+
+
+class Oracle(_Lib):
+    """The C restatement, ``liboracle.so``."""
+
+    prefix = "gzo_"
+
+    def __init__(self, path: str = ORACLE_SO):
+        super().__init__(path)
+        L = self.lib
+        L.gzo_keys.argtypes = [C.POINTER(_GzoParticles), C.c_longlong, _u64p, _dp]
+        L.gzo_build.restype = C.c_void_p
+        L.gzo_build.argtypes = [C.POINTER(_GzoParticles), C.c_longlong]
+        L.gzo_tree_skeys.argtypes = [C.c_void_p, _u64p]
+        L.gzo_accel_grouped.argtypes = [C.c_void_p, C.POINTER(_GzoParticles), _up, C.c_size_t,
+                                        C.c_double, C.c_double, C.c_int, _dp, _dp, _dp, _dp, _llp]
+        L.gzo_accel.argtypes = [C.c_void_p, C.POINTER(_GzoParticles), _up, C.c_size_t,
+                                C.c_double, C.c_double, _dp, _dp, _dp, _dp]
+        L.gzo_accel_brute.argtypes = [C.POINTER(_GzoParticles), _up, C.c_size_t, C.c_double,
+                                      _dp, _dp, _dp]
+
+    @staticmethod
+    def _cp(P: HostParticles):
+        return _GzoParticles(P.n, _ptr(P.x, _dp), _ptr(P.y, _dp), _ptr(P.z, _dp),
+                             _ptr(P.m, _dp), _ptr(P.soft, _dp))
+
+    def keys(self, P: HostParticles, seed: int = -1):
+        keys = np.zeros(P.n, dtype=np.uint64)
+        root = np.zeros(4, dtype=np.float64)
+        cp = self._cp(P)
+        self.lib.gzo_keys(C.byref(cp), seed, _ptr(keys, _u64p), _ptr(root, _dp))
+        return keys, root
+
+    def build(self, P: HostParticles, seed: int = -1):
+        cp = self._cp(P)
+        return self.lib.gzo_build(C.byref(cp), seed)
+
+    def free(self, t):
+        self.lib.gzo_tree_free(t)
+
+    def tree(self, t, P: HostParticles) -> TreeArrays:
+        ta = self._tree_arrays(t)
+        ta.orderbuf = np.zeros(ta.n_live, dtype=np.uint32)
+        ta.rank = np.zeros(P.n, dtype=np.uint32)
+        ta.leaf_of = np.zeros(P.n, dtype=np.int32)
+        self.lib.gzo_tree_particles(t, _ptr(ta.orderbuf, _up), _ptr(ta.rank, _up),
+                                    _ptr(ta.leaf_of, _ip))
+        return ta
+
+    def skeys(self, t, n_live: int):
+        out = np.zeros(n_live, dtype=np.uint64)
+        self.lib.gzo_tree_skeys(t, _ptr(out, _u64p))
+        return out
+
+    def accel_grouped(self, t, P: HostParticles, targets, theta, G=1.0, batch=0, aold=None,
+                      want_counts=False):
+        tg = np.ascontiguousarray(targets, dtype=np.uint32)
+        nt = tg.shape[0]
+        ax, ay, az = (np.zeros(nt) for _ in range(3))
+        counts = np.zeros(3, dtype=np.int64)
+        ao = None if aold is None else _f64(aold)
+        cp = self._cp(P)
+        self.lib.gzo_accel_grouped(t, C.byref(cp), _ptr(tg, _up), nt, theta, G, batch,
+                                   _ptr(ao, _dp), _ptr(ax, _dp), _ptr(ay, _dp), _ptr(az, _dp),
+                                   _ptr(counts, _llp))
+        if want_counts:
+            return ax, ay, az, counts
+        return ax, ay, az
+
+    def accel(self, t, P: HostParticles, targets, theta, G=1.0, aold=None):
+        tg = np.ascontiguousarray(targets, dtype=np.uint32)
+        nt = tg.shape[0]
+        ax, ay, az = (np.zeros(nt) for _ in range(3))
+        ao = None if aold is None else _f64(aold)
+        cp = self._cp(P)
+        self.lib.gzo_accel(t, C.byref(cp), _ptr(tg, _up), nt, theta, G, _ptr(ao, _dp),
+                           _ptr(ax, _dp), _ptr(ay, _dp), _ptr(az, _dp))
+        return ax, ay, az
+
+    def accel_brute(self, P: HostParticles, targets, G=1.0):
+        tg = np.ascontiguousarray(targets, dtype=np.uint32)
+        nt = tg.shape[0]
+        ax, ay, az = (np.zeros(nt) for _ in range(3))
+        cp = self._cp(P)
+        self.lib.gzo_accel_brute(C.byref(cp), _ptr(tg, _up), nt, G, _ptr(ax, _dp),
+                                 _ptr(ay, _dp), _ptr(az, _dp))
+        return ax, ay, az
+
+
+class Reference(_Lib):
+    """The reference ``shmem/tree.cc``, compiled into ``_ref/libgzref.so``."""
+
+    prefix = "gzref_"
+
+    def __init__(self, path: str = REF_SO):
+        super().__init__(path)
+        L = self.lib
+        L.gzref_particles_new.restype = C.c_void_p
+        L.gzref_particles_new.argtypes = [C.c_size_t, _dp, _dp, _dp, _dp, _dp]
+        L.gzref_particles_free.argtypes = [C.c_void_p]
+        L.gzref_build.restype = C.c_void_p
+        L.gzref_build.argtypes = [C.c_void_p, C.c_longlong]
+        L.gzref_tree_times.argtypes = [C.c_void_p, _dp]
+        L.gzref_accel_grouped.argtypes = [C.c_void_p, C.c_void_p, _up, C.c_size_t, C.c_double,
+                                          C.c_double, C.c_int, _dp, _dp, _dp, _dp]
+        L.gzref_accel.argtypes = [C.c_void_p, C.c_void_p, _up, C.c_size_t, C.c_double,
+                                  C.c_double, _dp, _dp, _dp, _dp]
+        L.gzref_potential.argtypes = [C.c_void_p, C.c_void_p, _up, C.c_size_t, C.c_double,
+                                      C.c_double, _dp, _dp]
+        L.gzref_accel_brute.argtypes = [C.c_void_p, _up, C.c_size_t, C.c_double, _dp, _dp, _dp]
+        L.gzref_audit.restype = C.c_longlong
+        L.gzref_audit.argtypes = [C.c_void_p, C.c_void_p, C.c_char_p, C.c_size_t]
+        L.gzref_max_threads.restype = C.c_int
+        L.gzref_set_threads.argtypes = [C.c_int]
+        self._handles = {}
+
+    def particles(self, P: HostParticles):
+        return self.lib.gzref_particles_new(P.n, _ptr(P.x, _dp), _ptr(P.y, _dp), _ptr(P.z, _dp),
+                                            _ptr(P.m, _dp), _ptr(P.soft, _dp))
+
+    def free_particles(self, h):
+        self.lib.gzref_particles_free(h)
+
+    def build(self, rp, seed: int = -1):
+        return self.lib.gzref_build(rp, seed)
+
+    def free(self, t):
+        self.lib.gzref_tree_free(t)
+
+    def times(self, t):
+        out = np.zeros(7)
+        self.lib.gzref_tree_times(t, _ptr(out, _dp))
+        return dict(zip(("bbox", "keys", "sort", "recurse", "links", "pack", "total"), out))
+
+    def tree(self, t, n: int) -> TreeArrays:
+        ta = self._tree_arrays(t)
+        ta.orderbuf = np.zeros(ta.n_live, dtype=np.uint32)
+        ta.rank = np.zeros(n, dtype=np.uint32)
+        ta.leaf_of = np.zeros(n, dtype=np.int32)
+        self.lib.gzref_tree_particles(t, _ptr(ta.orderbuf, _up), _ptr(ta.rank, _up),
+                                      _ptr(ta.leaf_of, _ip))
+        return ta
+
+    def accel_grouped(self, t, rp, targets, theta, G=1.0, batch=0, aold=None):
+        tg = np.ascontiguousarray(targets, dtype=np.uint32)
+        nt = tg.shape[0]
+        ax, ay, az = (np.zeros(nt) for _ in range(3))
+        ao = None if aold is None else _f64(aold)
+        self.lib.gzref_accel_grouped(t, rp, _ptr(tg, _up), nt, theta, G, batch, _ptr(ao, _dp),
+                                     _ptr(ax, _dp), _ptr(ay, _dp), _ptr(az, _dp))
+        return ax, ay, az
+
+    def accel(self, t, rp, targets, theta, G=1.0, aold=None):
+        tg = np.ascontiguousarray(targets, dtype=np.uint32)
+        nt = tg.shape[0]
+        ax, ay, az = (np.zeros(nt) for _ in range(3))
+        ao = None if aold is None else _f64(aold)
+        self.lib.gzref_accel(t, rp, _ptr(tg, _up), nt, theta, G, _ptr(ao, _dp), _ptr(ax, _dp),
+                             _ptr(ay, _dp), _ptr(az, _dp))
+        return ax, ay, az
+
+    def potential(self, t, rp, targets, theta, G=1.0, aold=None):
+        tg = np.ascontiguousarray(targets, dtype=np.uint32)
+        out = np.zeros(tg.shape[0])
+        ao = None if aold is None else _f64(aold)
+        self.lib.gzref_potential(t, rp, _ptr(tg, _up), tg.shape[0], theta, G, _ptr(ao, _dp),
+                                 _ptr(out, _dp))
+        return out
+
+    def accel_brute(self, rp, targets, G=1.0):
+        tg = np.ascontiguousarray(targets, dtype=np.uint32)
+        nt = tg.shape[0]
+        ax, ay, az = (np.zeros(nt) for _ in range(3))
+        self.lib.gzref_accel_brute(rp, _ptr(tg, _up), nt, G, _ptr(ax, _dp), _ptr(ay, _dp),
+                                   _ptr(az, _dp))
+        return ax, ay, az
+
+    def audit(self, t, rp) -> tuple[int, str]:
+        buf = C.create_string_buffer(512)
+        bad = self.lib.gzref_audit(t, rp, buf, 512)
+        return int(bad), buf.value.decode()
+
+    def max_threads(self) -> int:
+        return int(self.lib.gzref_max_threads())
+
+    def set_threads(self, n: int):
+        self.lib.gzref_set_threads(int(n))
+
+
+def have_reference() -> bool:
+    return os.path.exists(REF_SO)
+
+
+# -------------------------------------------------------------------------------------------------
+# Canonical node order: DFS preorder with children in octant order. Two builds of the same tree
+# agree node-for-node in this order whatever order their builders allocated ids in.
+# -------------------------------------------------------------------------------------------------
+
+def preorder_from_threaded(first: np.ndarray, nxt: np.ndarray, root: int) -> np.ndarray:
+    """Preorder over a tree threaded the reference's way: the children of X are the chain
+    first[X], next[first[X]], ... up to (excluding) next[X] (tree.cc:340-348, 1004)."""
+    out = []
+    stack = [root]
+    while stack:
+        x = stack.pop()
+        out.append(x)
+        if first[x] < 0:
+            continue
+        kids = []
+        c = first[x]
+        while c != nxt[x]:
+            kids.append(c)
+            c = nxt[c]
+        stack.extend(reversed(kids))
+    return np.asarray(out, dtype=np.int64)
+
+
+def preorder_from_children(first: np.ndarray, nchild: np.ndarray, root: int) -> np.ndarray:
+    """Preorder over a tree whose children of X are first[X] .. first[X] + nchild[X] - 1."""
+    out = []
+    stack = [root]
+    while stack:
+        x = stack.pop()
+        out.append(x)
+        if nchild[x] > 0:
+            stack.extend(range(first[x] + nchild[x] - 1, first[x] - 1, -1))
+    return np.asarray(out, dtype=np.int64)

@@ -1,0 +1,486 @@
+/* CPU oracle for the tree-gravity path: a plain-C restatement of the reference shared-memory tree.
+ * TEST INFRASTRUCTURE ONLY (tests, smoke check, CPU-baseline leg of the benchmark); the product
+ * library never links it. How each function is pinned is listed in gz_oracle.h.
+ *
+ * Compiled with -ffp-contract=off (oracle/Makefile): every expression below is evaluated in the
+ * order written, with no fused multiply-add, so results are bit-comparable with the reference
+ * built the same way. Where the reference's floating-point expression order matters for bit
+ * equality (spline kernel, opening test, moment sums) it is kept as the reference writes it. */
+#include "gz_oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+#define LEAF_MAX 16  /* tree.h:357 */
+#define MAX_LEVEL 20 /* tree.h:358 */
+
+static inline double dmax(double a, double b) { return (a < b) ? b : a; } /* std::max */
+static inline double dmin(double a, double b) { return (b < a) ? b : a; } /* std::min */
+
+/* Spread the low 21 bits of v so bit k lands at bit 3k (tree.h:162-170), here as a loop over the
+ * bits rather than the reference's mask cascade; the two are compared in the tests. */
+static uint64_t spread_bits3(uint64_t v) {
+    uint64_t out = 0;
+    for (int k = 0; k < 21; ++k) out |= ((v >> k) & 1ull) << (3 * k);
+    return out;
+}
+
+/* Morton key, x in bit 0, y in bit 1, z in bit 2 (tree.h:171-173). */
+uint64_t gzo_morton(uint32_t a, uint32_t b, uint32_t c) {
+    return spread_bits3(a) | (spread_bits3(b) << 1) | (spread_bits3(c) << 2);
+}
+
+/* Cubic-spline force/r, h the kernel extent (tree.h:59-67). */
+double gzo_spline_force_over_r(double r, double h) {
+    const double hi = 1.0 / h;
+    const double hi3 = hi * hi * hi;
+    const double u = r * hi;
+    if (u >= 1.0) return 1.0 / (r * r * r);
+    if (u < 0.5) return hi3 * (10.666666666666667 + u * u * (32.0 * u - 38.4));
+    return hi3 * (21.333333333333332 - 48.0 * u + 38.4 * u * u - 10.666666666666667 * u * u * u -
+                  0.06666666666666667 / (u * u * u));
+}
+
+/* Cubic-spline potential (tree.h:72-80). */
+double gzo_spline_potential(double r, double h) {
+    const double hi = 1.0 / h;
+    const double u = r * hi;
+    if (u >= 1.0) return -1.0 / r;
+    if (u < 0.5) return hi * (-2.8 + u * u * (5.333333333333333 + u * u * (6.4 * u - 9.6)));
+    return hi * (-3.2 + 0.06666666666666667 / u +
+                 u * u * (10.666666666666667 + u * (-16.0 + u * (9.6 - 2.1333333333333333 * u))));
+}
+
+/* Node-opening decision (tree.h:109-144): softening overlap, Barnes-Hut when theta_sq > 0,
+ * relative criterion when aold > 0 (open everything when neither criterion is live), and the
+ * inside-node test. Returns 1 to open. */
+int gzo_open_node(double r_sq, double len, double l_open, double node_mass, double node_maxsoft,
+                  double soft_target, double aold, double theta_sq, double adx, double ady,
+                  double adz) {
+    const double reach_t = soft_target + 0.6 * len;
+    const double reach_n = node_maxsoft + 0.6 * len;
+    if (r_sq < reach_t * reach_t || r_sq < reach_n * reach_n) return 1;
+    if (theta_sq > 0.0 && l_open * l_open >= theta_sq * r_sq) return 1;
+    if (aold > 0) {
+        if (node_mass * len * len > r_sq * r_sq * aold) return 1;
+    } else if (theta_sq <= 0.0) {
+        return 1;
+    }
+    return (adx < 0.6 * len && ady < 0.6 * len && adz < 0.6 * len) ? 1 : 0;
+}
+
+/* ---------------------------------------------------------------------------------------------
+ * Keys (tree.cc:355-418)
+ * ------------------------------------------------------------------------------------------- */
+
+/* splitmix64 stream for RANDOMIZE_GRAVTREE (tree.cc:381-393). */
+typedef struct {
+    uint64_t s;
+} splitmix;
+static double splitmix_unit(splitmix* g) {
+    uint64_t z = (g->s += 0x9E3779B97F4A7C15ull);
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return (double)((z ^ (z >> 31)) >> 11) * (1.0 / 9007199254740992.0);
+}
+
+static void root_box(const gzo_particles* P, long long seed, double* root) {
+    double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+    for (size_t i = 0; i < P->n; ++i) {
+        if (!(P->m[i] > 0)) continue;
+        lo[0] = dmin(lo[0], P->x[i]); hi[0] = dmax(hi[0], P->x[i]);
+        lo[1] = dmin(lo[1], P->y[i]); hi[1] = dmax(hi[1], P->y[i]);
+        lo[2] = dmin(lo[2], P->z[i]); hi[2] = dmax(hi[2], P->z[i]);
+    }
+    double cx = 0.5 * (lo[0] + hi[0]), cy = 0.5 * (lo[1] + hi[1]), cz = 0.5 * (lo[2] + hi[2]);
+    double side = dmax(hi[0] - lo[0], dmax(hi[1] - lo[1], hi[2] - lo[2])) * 1.0000001;
+    if (side <= 0) side = 1.0;
+    if (seed >= 0) {
+        splitmix g = {(uint64_t)seed * 0x9E3779B97F4A7C15ull};
+        cx += side * (splitmix_unit(&g) - 0.5);
+        cy += side * (splitmix_unit(&g) - 0.5);
+        cz += side * (splitmix_unit(&g) - 0.5);
+        side *= 2.0;
+    }
+    root[0] = cx; root[1] = cy; root[2] = cz; root[3] = side;
+}
+
+static inline uint64_t key_of(double x, double y, double z, const double* root) {
+    const double scale = (double)((1u << MAX_LEVEL) - 1) / root[3];
+    const double half = 0.5 * root[3];
+    const uint32_t a = (uint32_t)((x - (root[0] - half)) * scale);
+    const uint32_t b = (uint32_t)((y - (root[1] - half)) * scale);
+    const uint32_t c = (uint32_t)((z - (root[2] - half)) * scale);
+    return gzo_morton(a, b, c);
+}
+
+void gzo_keys(const gzo_particles* P, long long seed, uint64_t* keys, double* root) {
+    root_box(P, seed, root);
+    for (size_t i = 0; i < P->n; ++i)
+        keys[i] = (P->m[i] > 0) ? key_of(P->x[i], P->y[i], P->z[i], root) : 0ull;
+}
+
+/* ---------------------------------------------------------------------------------------------
+ * Build (tree.cc:70-197, 340-348, 350-526)
+ * ------------------------------------------------------------------------------------------- */
+
+typedef struct {
+    int nnodes, cap, root;
+    size_t n, n_live;
+    int *first, *next, *plo, *phi, *parent;
+    double *size, *mass, *cx, *cy, *cz, *soft, *delta, *gcx, *gcy, *gcz, *s;
+    float *wsoft, *wlen;
+    uint32_t *orderbuf, *rank;
+    int* leaf_of;
+    uint64_t* skey;
+} otree;
+
+static void grow(otree* T) {
+    int cap = T->cap ? 2 * T->cap : 1024;
+#define RE(f, ty) T->f = (ty*)realloc(T->f, (size_t)cap * sizeof(ty))
+    RE(first, int); RE(next, int); RE(plo, int); RE(phi, int); RE(parent, int);
+    RE(size, double); RE(mass, double); RE(cx, double); RE(cy, double); RE(cz, double);
+    RE(soft, double); RE(delta, double); RE(gcx, double); RE(gcy, double); RE(gcz, double);
+    RE(s, double); RE(wsoft, float); RE(wlen, float);
+#undef RE
+    T->cap = cap;
+}
+
+typedef struct {
+    uint64_t key;
+    uint32_t idx;
+} keyed;
+
+static int keyed_cmp(const void* a, const void* b) {
+    const keyed* p = (const keyed*)a;
+    const keyed* q = (const keyed*)b;
+    if (p->key != q->key) return (p->key < q->key) ? -1 : 1;
+    return (p->idx < q->idx) ? -1 : (p->idx > q->idx);
+}
+
+/* One node over the sorted range [lo, hi) at `level`, geometric centre (gx,gy,gz), side `side`.
+ * Node ids are claimed in preorder. Children are the non-empty octants in octant order; their
+ * centres come from halving this node's box (tree.cc:101, 116-118), not from key bits. */
+static int make_node(otree* T, const gzo_particles* P, int lo, int hi, int level, double gx,
+                     double gy, double gz, double side) {
+    if (T->nnodes == T->cap) grow(T);
+    const int me = T->nnodes++;
+    T->size[me] = side; T->first[me] = -1; T->next[me] = -1;
+    T->plo[me] = lo; T->phi[me] = hi; T->parent[me] = -1;
+    T->gcx[me] = gx; T->gcy[me] = gy; T->gcz[me] = gz;
+
+    int kids[8], nk = 0;
+    if (!((hi - lo <= LEAF_MAX) || (level >= MAX_LEVEL))) {
+        const int shift = 3 * (MAX_LEVEL - level - 1);
+        const double half = side * 0.5, quarter = side * 0.25;
+        int at = lo;
+        for (int oct = 0; oct < 8; ++oct) {
+            const int start = at;
+            while (at < hi && (int)((T->skey[at] >> shift) & 7ull) == oct) ++at;
+            if (at == start) continue;
+            const double ox = gx + ((oct & 1) ? quarter : -quarter);
+            const double oy = gy + ((oct & 2) ? quarter : -quarter);
+            const double oz = gz + ((oct & 4) ? quarter : -quarter);
+            kids[nk++] = make_node(T, P, start, at, level + 1, ox, oy, oz, half);
+        }
+        for (int i = 0; i < nk; ++i) T->parent[kids[i]] = me;
+        T->first[me] = kids[0];
+        for (int i = 0; i + 1 < nk; ++i) T->next[kids[i]] = kids[i + 1];
+    }
+
+    /* Moments: a leaf sums its particles in sorted order; an interior node combines its
+     * children's final moments in octant order (tree.cc:153-190). */
+    double M = 0, sx = 0, sy = 0, sz = 0, smax = 0;
+    if (nk == 0) {
+        for (int i = lo; i < hi; ++i) {
+            const uint32_t p = T->orderbuf[i];
+            const double m = P->m[p];
+            M += m; sx += m * P->x[p]; sy += m * P->y[p]; sz += m * P->z[p];
+            if (P->soft && P->soft[p] > smax) smax = P->soft[p];
+            T->leaf_of[p] = me;
+        }
+    } else {
+        for (int i = 0; i < nk; ++i) {
+            const int k = kids[i];
+            const double m = T->mass[k];
+            M += m; sx += m * T->cx[k]; sy += m * T->cy[k]; sz += m * T->cz[k];
+            if (T->soft[k] > smax) smax = T->soft[k];
+        }
+    }
+    if (M > 0) { sx /= M; sy /= M; sz /= M; }
+    T->mass[me] = M; T->cx[me] = sx; T->cy[me] = sy; T->cz[me] = sz; T->soft[me] = smax;
+    const double dx = sx - gx, dy = sy - gy, dz = sz - gz;
+    T->delta[me] = sqrt(dx * dx + dy * dy + dz * dz);
+    return me;
+}
+
+/* next[] threading: the last child of a node continues at its parent's successor (tree.cc:340). */
+static void thread_links(otree* T, int node, int successor) {
+    T->next[node] = successor;
+    int c = T->first[node];
+    while (c >= 0) {
+        const int sib = T->next[c];
+        thread_links(T, c, sib >= 0 ? sib : successor);
+        c = sib;
+    }
+}
+
+void* gzo_build(const gzo_particles* P, long long seed) {
+    otree* T = (otree*)calloc(1, sizeof(otree));
+    const size_t n = P->n;
+    T->n = n;
+    double root[4];
+    uint64_t* keys = (uint64_t*)malloc((n ? n : 1) * sizeof(uint64_t));
+    gzo_keys(P, seed, keys, root);
+
+    /* live particles in index order, then sorted by key; ties keep index order */
+    size_t nl = 0;
+    keyed* kv = (keyed*)malloc((n ? n : 1) * sizeof(keyed));
+    for (size_t i = 0; i < n; ++i)
+        if (P->m[i] > 0) { kv[nl].key = keys[i]; kv[nl].idx = (uint32_t)i; ++nl; }
+    qsort(kv, nl, sizeof(keyed), keyed_cmp);
+    T->n_live = nl;
+    T->orderbuf = (uint32_t*)malloc((nl ? nl : 1) * sizeof(uint32_t));
+    T->skey = (uint64_t*)malloc((nl ? nl : 1) * sizeof(uint64_t));
+    for (size_t r = 0; r < nl; ++r) { T->orderbuf[r] = kv[r].idx; T->skey[r] = kv[r].key; }
+    free(kv);
+    free(keys);
+
+    T->leaf_of = (int*)malloc((n ? n : 1) * sizeof(int));
+    for (size_t i = 0; i < n; ++i) T->leaf_of[i] = -1;
+    T->root = make_node(T, P, 0, (int)nl, 0, root[0], root[1], root[2], root[3]);
+    thread_links(T, T->root, -1);
+
+    T->rank = (uint32_t*)malloc((n ? n : 1) * sizeof(uint32_t));
+    for (size_t i = 0; i < n; ++i) T->rank[i] = UINT32_MAX;
+    for (size_t r = 0; r < nl; ++r) T->rank[T->orderbuf[r]] = (uint32_t)r;
+
+    /* packed walk fields (tree.cc:507-523) */
+    for (int i = 0; i < T->nnodes; ++i) {
+        T->s[i] = T->size[i] + T->delta[i];
+        T->wsoft[i] = (float)T->soft[i];
+        T->wlen[i] = (float)T->size[i];
+    }
+    return T;
+}
+
+void gzo_tree_free(void* h) {
+    otree* T = (otree*)h;
+    if (!T) return;
+    free(T->first); free(T->next); free(T->plo); free(T->phi); free(T->parent);
+    free(T->size); free(T->mass); free(T->cx); free(T->cy); free(T->cz); free(T->soft);
+    free(T->delta); free(T->gcx); free(T->gcy); free(T->gcz); free(T->s); free(T->wsoft);
+    free(T->wlen); free(T->orderbuf); free(T->rank); free(T->leaf_of); free(T->skey);
+    free(T);
+}
+
+void gzo_tree_sizes(void* h, long long* out) {
+    otree* T = (otree*)h;
+    out[0] = T->nnodes; out[1] = T->root; out[2] = (long long)T->n_live;
+}
+
+void gzo_tree_nodes(void* h, int* first, int* next, int* plo, int* phi, int* parent,
+                    double* size, double* mass, double* cx, double* cy, double* cz, double* soft,
+                    double* delta, double* gcx, double* gcy, double* gcz, double* wn_s,
+                    float* wn_soft, float* wn_len) {
+    otree* T = (otree*)h;
+    const size_t nn = (size_t)T->nnodes;
+#define CP(dst, src, ty) if (dst) memcpy(dst, T->src, nn * sizeof(ty))
+    CP(first, first, int); CP(next, next, int); CP(plo, plo, int); CP(phi, phi, int);
+    CP(parent, parent, int); CP(size, size, double); CP(mass, mass, double);
+    CP(cx, cx, double); CP(cy, cy, double); CP(cz, cz, double); CP(soft, soft, double);
+    CP(delta, delta, double); CP(gcx, gcx, double); CP(gcy, gcy, double); CP(gcz, gcz, double);
+    CP(wn_s, s, double); CP(wn_soft, wsoft, float); CP(wn_len, wlen, float);
+#undef CP
+}
+
+void gzo_tree_particles(void* h, uint32_t* orderbuf, uint32_t* rank, int* leaf_of) {
+    otree* T = (otree*)h;
+    if (orderbuf) memcpy(orderbuf, T->orderbuf, T->n_live * sizeof(uint32_t));
+    if (rank) memcpy(rank, T->rank, T->n * sizeof(uint32_t));
+    if (leaf_of) memcpy(leaf_of, T->leaf_of, T->n * sizeof(int));
+}
+
+void gzo_tree_skeys(void* h, uint64_t* skey) {
+    otree* T = (otree*)h;
+    memcpy(skey, T->skey, T->n_live * sizeof(uint64_t));
+}
+
+/* ---------------------------------------------------------------------------------------------
+ * Walks (tree.cc:528-582, 626-897) and direct summation (tree.cc:946-971)
+ * ------------------------------------------------------------------------------------------- */
+
+static inline double soft_of(const gzo_particles* P, size_t i) { return P->soft ? P->soft[i] : 0.0; }
+
+/* Grouped walk (tree.cc:626-897) without tidal/jerk/lazy/sink terms. Groups are runs of targets
+ * sharing a leaf when batch <= 0, fixed windows of `batch` otherwise. Opening is decided once per
+ * group against its bounding box, largest softening and smallest aold. */
+void gzo_accel_grouped(void* h, const gzo_particles* P, const uint32_t* targets, size_t nt,
+                       double theta, double G, int batch, const double* aold, double* ax,
+                       double* ay, double* az, long long* counts) {
+    const otree* T = (const otree*)h;
+    const double theta2 = theta * theta;
+    size_t ngroups = 0;
+    size_t* gstart = (size_t*)malloc((nt + 2) * sizeof(size_t));
+    if (batch <= 0) {
+        gstart[ngroups++] = 0;
+        for (size_t i = 1; i < nt; ++i) {
+            const int li = T->leaf_of[targets[i]], lp = T->leaf_of[targets[i - 1]];
+            if (li != lp || li < 0) gstart[ngroups++] = i;
+        }
+        gstart[ngroups] = nt;
+        if (nt == 0) ngroups = 0;
+    } else {
+        for (size_t i = 0; i < nt; i += (size_t)batch) gstart[ngroups++] = i;
+        gstart[ngroups] = nt;
+    }
+    long long visited = 0, multipole = 0, pairs = 0;
+
+#pragma omp parallel for schedule(dynamic, 1) reduction(+ : visited, multipole, pairs)
+    for (long long b = 0; b < (long long)ngroups; ++b) {
+        const size_t lo = gstart[b], hi = gstart[b + 1];
+        const int nb = (int)(hi - lo);
+        double bx0 = 1e300, by0 = 1e300, bz0 = 1e300, bx1 = -1e300, by1 = -1e300, bz1 = -1e300;
+        double emax = 0;
+        for (size_t i = lo; i < hi; ++i) {
+            const uint32_t p = targets[i];
+            bx0 = dmin(bx0, P->x[p]); bx1 = dmax(bx1, P->x[p]);
+            by0 = dmin(by0, P->y[p]); by1 = dmax(by1, P->y[p]);
+            bz0 = dmin(bz0, P->z[p]); bz1 = dmax(bz1, P->z[p]);
+            if (P->soft) emax = dmax(emax, P->soft[p]);
+        }
+        double amin = aold ? 1e300 : 0.0;
+        if (aold) for (size_t i = lo; i < hi; ++i) amin = dmin(amin, aold[i]);
+        double* buf = (double*)malloc((size_t)nb * 7 * sizeof(double));
+        double *tx = buf, *ty = buf + nb, *tz = buf + 2 * nb, *te = buf + 3 * nb;
+        double *oax = buf + 4 * nb, *oay = buf + 5 * nb, *oaz = buf + 6 * nb;
+        for (int i = 0; i < nb; ++i) {
+            const uint32_t p = targets[lo + i];
+            tx[i] = P->x[p]; ty[i] = P->y[p]; tz[i] = P->z[p]; te[i] = soft_of(P, p);
+            oax[i] = 0; oay[i] = 0; oaz[i] = 0;
+        }
+        int node = T->root;
+        while (node >= 0) {
+            ++visited;
+            const double wcx = T->cx[node], wcy = T->cy[node], wcz = T->cz[node];
+            const double len = (double)T->wlen[node], wsoft = (double)T->wsoft[node];
+            const double dx = dmax(0.0, dmax(bx0 - wcx, wcx - bx1));
+            const double dy = dmax(0.0, dmax(by0 - wcy, wcy - by1));
+            const double dz = dmax(0.0, dmax(bz0 - wcz, wcz - bz1));
+            const double rmin2 = dx * dx + dy * dy + dz * dz;
+            const int accept = !gzo_open_node(rmin2, len, T->s[node], T->mass[node], wsoft, emax,
+                                              amin, theta2, dx, dy, dz);
+            if (accept || T->first[node] < 0) {
+                if (!accept) {
+                    for (int k = T->plo[node]; k < T->phi[node]; ++k) {
+                        const uint32_t q = T->orderbuf[k];
+                        const double qx = P->x[q], qy = P->y[q], qz = P->z[q], qm = P->m[q];
+                        const double qs = soft_of(P, q);
+                        for (int i = 0; i < nb; ++i) {
+                            if (targets[lo + i] == q) continue;
+                            const double ddx = qx - tx[i], ddy = qy - ty[i], ddz = qz - tz[i];
+                            const double r2 = ddx * ddx + ddy * ddy + ddz * ddz;
+                            if (r2 <= 0) continue;
+                            ++pairs;
+                            const double r = sqrt(r2);
+                            const double fac =
+                                qm * gzo_spline_force_over_r(r, dmax(dmax(te[i], qs), 1e-300));
+                            oax[i] += ddx * fac; oay[i] += ddy * fac; oaz[i] += ddz * fac;
+                        }
+                    }
+                } else {
+                    multipole += nb;
+                    for (int i = 0; i < nb; ++i) {
+                        const double e = dmax(te[i], wsoft);
+                        const double ox = wcx - tx[i], oy = wcy - ty[i], oz = wcz - tz[i];
+                        const double r2 = ox * ox + oy * oy + oz * oz;
+                        if (r2 <= 0) continue;
+                        const double f = T->mass[node] *
+                                         gzo_spline_force_over_r(sqrt(r2), dmax(e, 1e-300));
+                        oax[i] += ox * f; oay[i] += oy * f; oaz[i] += oz * f;
+                    }
+                }
+                node = T->next[node];
+            } else {
+                node = T->first[node];
+            }
+        }
+        for (int i = 0; i < nb; ++i) {
+            ax[lo + i] = G * oax[i]; ay[lo + i] = G * oay[i]; az[lo + i] = G * oaz[i];
+        }
+        free(buf);
+    }
+    free(gstart);
+    if (counts) { counts[0] = visited; counts[1] = multipole; counts[2] = pairs; }
+}
+
+/* Per-target walk (tree.cc:528-582): a leaf reached is always direct-summed. */
+void gzo_accel(void* h, const gzo_particles* P, const uint32_t* targets, size_t nt, double theta,
+               double G, const double* aold, double* ax, double* ay, double* az) {
+    const otree* T = (const otree*)h;
+    const double theta2 = theta * theta;
+#pragma omp parallel for schedule(dynamic, 16)
+    for (long long t = 0; t < (long long)nt; ++t) {
+        const uint32_t p = targets[t];
+        const double px = P->x[p], py = P->y[p], pz = P->z[p];
+        const double ep = soft_of(P, p);
+        const double at = aold ? aold[t] : 0.0;
+        double sx = 0, sy = 0, sz = 0;
+        int node = T->root;
+        while (node >= 0) {
+            const double ox = T->cx[node] - px, oy = T->cy[node] - py, oz = T->cz[node] - pz;
+            const double r2 = ox * ox + oy * oy + oz * oz;
+            if (T->first[node] < 0 ||
+                !gzo_open_node(r2, (double)T->wlen[node], T->s[node], T->mass[node],
+                               (double)T->wsoft[node], ep, at, theta2, fabs(ox), fabs(oy),
+                               fabs(oz))) {
+                if (T->first[node] < 0) {
+                    for (int k = T->plo[node]; k < T->phi[node]; ++k) {
+                        const uint32_t q = T->orderbuf[k];
+                        if (q == p) continue;
+                        const double dx = P->x[q] - px, dy = P->y[q] - py, dz = P->z[q] - pz;
+                        const double r = sqrt(dx * dx + dy * dy + dz * dz);
+                        if (r <= 0) continue;
+                        const double fac =
+                            P->m[q] * gzo_spline_force_over_r(r, dmax(dmax(ep, soft_of(P, q)), 1e-300));
+                        sx += dx * fac; sy += dy * fac; sz += dz * fac;
+                    }
+                } else {
+                    const double e = dmax(ep, (double)T->wsoft[node]);
+                    const double r = sqrt(r2);
+                    if (r > 0) {
+                        const double f = T->mass[node] * gzo_spline_force_over_r(r, dmax(e, 1e-300));
+                        sx += ox * f; sy += oy * f; sz += oz * f;
+                    }
+                }
+                node = T->next[node];
+            } else {
+                node = T->first[node];
+            }
+        }
+        ax[t] = G * sx; ay[t] = G * sy; az[t] = G * sz;
+    }
+}
+
+/* Exact FP64 sum over every particle with the walk's pair rule (tree.cc:946-971). */
+void gzo_accel_brute(const gzo_particles* P, const uint32_t* targets, size_t nt, double G,
+                     double* ax, double* ay, double* az) {
+#pragma omp parallel for schedule(static)
+    for (long long t = 0; t < (long long)nt; ++t) {
+        const uint32_t p = targets[t];
+        const double px = P->x[p], py = P->y[p], pz = P->z[p];
+        const double ep = soft_of(P, p);
+        double sx = 0, sy = 0, sz = 0;
+        for (size_t q = 0; q < P->n; ++q) {
+            if (q == p) continue;
+            const double dx = P->x[q] - px, dy = P->y[q] - py, dz = P->z[q] - pz;
+            const double r = sqrt(dx * dx + dy * dy + dz * dz);
+            if (r <= 0) continue;
+            const double fac =
+                P->m[q] * gzo_spline_force_over_r(r, dmax(dmax(ep, soft_of(P, q)), 1e-300));
+            sx += dx * fac; sy += dy * fac; sz += dz * fac;
+        }
+        ax[t] = G * sx; ay[t] = G * sy; az[t] = G * sz;
+    }
+}

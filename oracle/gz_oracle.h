@@ -1,0 +1,74 @@
+/* CPU oracle for the tree-gravity path: a plain-C restatement of the reference shared-memory tree
+ * (shmem/tree.h, shmem/tree.cc). TEST INFRASTRUCTURE ONLY: the tests, the smoke check and the
+ * CPU-baseline leg of the benchmark use it as the checker. The product library never links it.
+ *
+ * Pinning (each function is checked by something that does not share its failure modes):
+ *   gzo_morton            against the reference's morton (oracle/_ref) and an independent
+ *                         bit-by-bit interleave (system/peano.cc:317-328 has the same bit order)
+ *   gzo_build             topology, orderbuf, leaf_of and every node moment compared bit for bit
+ *                         with the reference build (oracle/_ref) in canonical preorder, plus the
+ *                         audit invariants (tree.cc:973-1119) and root mass == sum(m)
+ *   gzo_spline_force_over_r  against the reference kernel, continuity at u=1/2 and u=1, and the
+ *                         Newtonian limit for r >= h
+ *   gzo_open_node         against the reference open_node on random arguments
+ *   gzo_accel_grouped     bit for bit against the reference accel_grouped (both regimes), and
+ *                         theta -> 0 against gzo_accel_brute
+ *   gzo_accel             bit for bit against the reference accel
+ *   gzo_accel_brute       bit for bit against the reference accel_brute, Newton's third law, and
+ *                         a two-body closed form
+ */
+#ifndef GZ_ORACLE_H
+#define GZ_ORACLE_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Particle SoA (tree.h:36-50). soft may be NULL, meaning softening 0. */
+typedef struct {
+    size_t n;
+    const double *x, *y, *z, *m, *soft;
+} gzo_particles;
+
+uint64_t gzo_morton(uint32_t a, uint32_t b, uint32_t c);
+double gzo_spline_force_over_r(double r, double h);
+double gzo_spline_potential(double r, double h);
+int gzo_open_node(double r_sq, double len, double l_open, double node_mass, double node_maxsoft,
+                  double soft_target, double aold, double theta_sq, double adx, double ady,
+                  double adz);
+
+/* Root box + keys (tree.cc:355-418). keys has length n (0 for massless particles).
+ * root[4] = {cx, cy, cz, side}. */
+void gzo_keys(const gzo_particles* P, long long randomize_seed, uint64_t* keys, double* root);
+
+/* Full build (tree.cc:350-526). Node ids are DFS preorder in octant order, which is the
+ * reference's own numbering whenever its build runs serially. Returns an opaque handle. */
+void* gzo_build(const gzo_particles* P, long long randomize_seed);
+void gzo_tree_free(void* t);
+/* out[0]=nnodes, out[1]=root, out[2]=n_live */
+void gzo_tree_sizes(void* t, long long* out);
+void gzo_tree_nodes(void* t, int* first, int* next, int* plo, int* phi, int* parent,
+                    double* size, double* mass, double* cx, double* cy, double* cz,
+                    double* soft, double* delta, double* gcx, double* gcy, double* gcz,
+                    double* wn_s, float* wn_soft, float* wn_len);
+void gzo_tree_particles(void* t, uint32_t* orderbuf, uint32_t* rank, int* leaf_of);
+/* sorted keys (n_live) */
+void gzo_tree_skeys(void* t, uint64_t* skey);
+
+/* Walks. counts (may be NULL) receives {nodes_visited, multipole, pairs} as SHMEM_COUNT_WALK
+ * defines them (tree.cc:671-674). aold may be NULL. */
+void gzo_accel_grouped(void* t, const gzo_particles* P, const uint32_t* targets, size_t nt,
+                       double theta, double G, int batch, const double* aold, double* ax,
+                       double* ay, double* az, long long* counts);
+void gzo_accel(void* t, const gzo_particles* P, const uint32_t* targets, size_t nt, double theta,
+               double G, const double* aold, double* ax, double* ay, double* az);
+void gzo_accel_brute(const gzo_particles* P, const uint32_t* targets, size_t nt, double G,
+                     double* ax, double* ay, double* az);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

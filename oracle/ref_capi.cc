@@ -1,0 +1,186 @@
+// C entry points over the REFERENCE shared-memory tree (shmem/tree.h, shmem/tree.cc), compiled
+// from the reference sources where they lie (oracle/Makefile). Test infrastructure only: the
+// tests and the CPU-baseline leg of the benchmark load the resulting oracle/_ref/libgzref.so to
+// pin the C restatement (oracle/gz_oracle.c) and to time the reference CPU walk. Nothing in the
+// product library links or loads it.
+//
+// Every function here is a thin adapter: it moves plain arrays into shmem::Particles and calls the
+// reference function named in its comment, unchanged.
+
+#include "tree.h"
+
+#include <cstdint>
+#include <cstring>
+#include <vector>
+
+using namespace shmem;
+
+namespace {
+struct RefParticles {
+    Particles P;
+};
+struct RefTree {
+    Tree T;
+    BuildTimes bt;
+};
+}  // namespace
+
+extern "C" {
+
+// Particles (tree.h:36-50). soft may be NULL (empty vector => softening 0).
+void* gzref_particles_new(size_t n, const double* x, const double* y, const double* z,
+                          const double* m, const double* soft) {
+    auto* rp = new RefParticles();
+    rp->P.x.assign(x, x + n);
+    rp->P.y.assign(y, y + n);
+    rp->P.z.assign(z, z + n);
+    rp->P.m.assign(m, m + n);
+    if (soft) rp->P.soft.assign(soft, soft + n);
+    return rp;
+}
+
+void gzref_particles_free(void* p) { delete static_cast<RefParticles*>(p); }
+
+// shmem::build (tree.h:384, tree.cc:350-526). randomize_seed < 0 disables RANDOMIZE_GRAVTREE.
+void* gzref_build(void* particles, long long randomize_seed) {
+    auto* rp = static_cast<RefParticles*>(particles);
+    auto* rt = new RefTree();
+    rt->T = build(rp->P, &rt->bt, nullptr, false, randomize_seed);
+    return rt;
+}
+
+void gzref_tree_free(void* t) { delete static_cast<RefTree*>(t); }
+
+// Sizes: out[0]=nnodes, out[1]=root, out[2]=n_live (orderbuf size).
+void gzref_tree_sizes(void* t, long long* out) {
+    auto* rt = static_cast<RefTree*>(t);
+    out[0] = (long long)rt->T.nnodes();
+    out[1] = rt->T.root;
+    out[2] = (long long)rt->T.orderbuf.size();
+}
+
+// BuildTimes (tree.h:371): bbox, keys, sort, recurse, links, pack, total (ms).
+void gzref_tree_times(void* t, double* out7) {
+    auto* rt = static_cast<RefTree*>(t);
+    const BuildTimes& b = rt->bt;
+    const double v[7] = {b.bbox, b.keys, b.sort, b.recurse, b.links, b.pack, b.total};
+    std::memcpy(out7, v, sizeof v);
+}
+
+// Node arrays (tree.h:228-247), each of length nnodes; any pointer may be NULL.
+void gzref_tree_nodes(void* t, int* first, int* next, int* plo, int* phi, int* parent,
+                      double* size, double* mass, double* cx, double* cy, double* cz,
+                      double* soft, double* delta, double* gcx, double* gcy, double* gcz,
+                      double* wn_s, float* wn_soft, float* wn_len) {
+    auto* rt = static_cast<RefTree*>(t);
+    const Tree& T = rt->T;
+    const size_t nn = T.nnodes();
+    for (size_t i = 0; i < nn; ++i) {
+        if (first) first[i] = T.first[i];
+        if (next) next[i] = T.next[i];
+        if (plo) plo[i] = T.plo[i];
+        if (phi) phi[i] = T.phi[i];
+        if (parent) parent[i] = T.parent[i];
+        if (size) size[i] = T.size[i];
+        if (mass) mass[i] = T.mass[i];
+        if (cx) cx[i] = T.cx[i];
+        if (cy) cy[i] = T.cy[i];
+        if (cz) cz[i] = T.cz[i];
+        if (soft) soft[i] = T.soft[i];
+        if (delta) delta[i] = T.delta[i];
+        if (gcx) gcx[i] = T.gcx[i];
+        if (gcy) gcy[i] = T.gcy[i];
+        if (gcz) gcz[i] = T.gcz[i];
+        if (wn_s) wn_s[i] = T.wn[i].s;
+        if (wn_soft) wn_soft[i] = T.wn[i].soft;
+        if (wn_len) wn_len[i] = T.wn[i].len;
+    }
+}
+
+// orderbuf (n_live), rank (n), leaf_of (n); any pointer may be NULL.
+void gzref_tree_particles(void* t, uint32_t* orderbuf, uint32_t* rank, int* leaf_of) {
+    auto* rt = static_cast<RefTree*>(t);
+    const Tree& T = rt->T;
+    if (orderbuf) std::memcpy(orderbuf, T.orderbuf.data(), T.orderbuf.size() * sizeof(uint32_t));
+    if (rank) std::memcpy(rank, T.rank.data(), T.rank.size() * sizeof(uint32_t));
+    if (leaf_of) std::memcpy(leaf_of, T.leaf_of.data(), T.leaf_of.size() * sizeof(int));
+}
+
+// shmem::morton (tree.h:171-173), exposed so key tests can pin the interleave itself.
+uint64_t gzref_morton(uint32_t a, uint32_t b, uint32_t c) { return morton(a, b, c); }
+
+// shmem::spline_force_over_r (tree.h:59-67) and shmem::spline_potential (tree.h:72-80).
+double gzref_spline_force_over_r(double r, double h) { return spline_force_over_r(r, h); }
+double gzref_spline_potential(double r, double h) { return spline_potential(r, h); }
+
+// shmem::open_node (tree.h:109-144).
+int gzref_open_node(double r_sq, double len, double l_open, double node_mass,
+                    double node_maxsoft, double soft_target, double aold, double theta_sq,
+                    double adx, double ady, double adz) {
+    return open_node(r_sq, len, l_open, node_mass, node_maxsoft, soft_target, aold, theta_sq,
+                     adx, ady, adz) ? 1 : 0;
+}
+
+// shmem::accel_grouped (tree.h:428-440, tree.cc:626-897) with tidal/lazy/jerk/sink off.
+// aold may be NULL. Outputs have length nt.
+void gzref_accel_grouped(void* t, void* particles, const uint32_t* targets, size_t nt,
+                         double theta, double G, int batch, const double* aold,
+                         double* ax, double* ay, double* az) {
+    auto* rt = static_cast<RefTree*>(t);
+    auto* rp = static_cast<RefParticles*>(particles);
+    std::vector<uint32_t> tg(targets, targets + nt);
+    std::vector<double> vx, vy, vz;
+    accel_grouped(rt->T, rp->P, tg, theta, G, batch, vx, vy, vz, nullptr, aold);
+    std::memcpy(ax, vx.data(), nt * sizeof(double));
+    std::memcpy(ay, vy.data(), nt * sizeof(double));
+    std::memcpy(az, vz.data(), nt * sizeof(double));
+}
+
+// shmem::accel (tree.h:392-394, tree.cc:528-582), the per-target walk.
+void gzref_accel(void* t, void* particles, const uint32_t* targets, size_t nt, double theta,
+                 double G, const double* aold, double* ax, double* ay, double* az) {
+    auto* rt = static_cast<RefTree*>(t);
+    auto* rp = static_cast<RefParticles*>(particles);
+    std::vector<uint32_t> tg(targets, targets + nt);
+    std::vector<double> vx, vy, vz;
+    accel(rt->T, rp->P, tg, theta, G, vx, vy, vz, aold);
+    std::memcpy(ax, vx.data(), nt * sizeof(double));
+    std::memcpy(ay, vy.data(), nt * sizeof(double));
+    std::memcpy(az, vz.data(), nt * sizeof(double));
+}
+
+// shmem::potential (tree.h:446-447, tree.cc:899-944).
+void gzref_potential(void* t, void* particles, const uint32_t* targets, size_t nt, double theta,
+                     double G, const double* aold, double* phi) {
+    auto* rt = static_cast<RefTree*>(t);
+    auto* rp = static_cast<RefParticles*>(particles);
+    std::vector<uint32_t> tg(targets, targets + nt);
+    std::vector<double> v;
+    potential(rt->T, rp->P, tg, theta, G, v, aold);
+    std::memcpy(phi, v.data(), nt * sizeof(double));
+}
+
+// shmem::accel_brute (tree.h:450-451, tree.cc:946-971).
+void gzref_accel_brute(void* particles, const uint32_t* targets, size_t nt, double G,
+                       double* ax, double* ay, double* az) {
+    auto* rp = static_cast<RefParticles*>(particles);
+    std::vector<uint32_t> tg(targets, targets + nt);
+    std::vector<double> vx, vy, vz;
+    accel_brute(rp->P, tg, G, vx, vy, vz);
+    std::memcpy(ax, vx.data(), nt * sizeof(double));
+    std::memcpy(ay, vy.data(), nt * sizeof(double));
+    std::memcpy(az, vz.data(), nt * sizeof(double));
+}
+
+// shmem::audit_tree (tree.h:353-355, tree.cc:973-1119), gravity-only arguments.
+long long gzref_audit(void* t, void* particles, char* msg, size_t msglen) {
+    auto* rt = static_cast<RefTree*>(t);
+    auto* rp = static_cast<RefParticles*>(particles);
+    return audit_tree(rt->T, rp->P, nullptr, 0, nullptr, nullptr, nullptr, 0.0, nullptr, msg,
+                      msglen);
+}
+
+int gzref_max_threads(void) { return omp_get_max_threads(); }
+void gzref_set_threads(int n) { omp_set_num_threads(n); }
+
+}  // extern "C"
