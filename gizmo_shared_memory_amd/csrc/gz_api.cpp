@@ -1,0 +1,396 @@
+// C ABI of the tree-gravity library (include/gz_grav.h): argument checks, scratch management and
+// the order of launches. The device work itself lives in gz_build.hip, gz_walk.hip and
+// gz_caller.hip.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <new>
+#include <vector>
+
+#include "gz_internal.h"
+
+using gz::fail;
+
+namespace {
+
+gz_status check_ctx(gz_ctx* ctx) {
+    if (!ctx) return fail(GZ_ERR_INVALID, "null context");
+    GZ_HIP(hipSetDevice(ctx->device));
+    return GZ_OK;
+}
+
+gz_status check_particles(const gz_particles* p, const char* who) {
+    if (!p) return fail(GZ_ERR_INVALID, "%s: null particles", who);
+    if (p->n > 0 && (!p->x || !p->y || !p->z || !p->m))
+        return fail(GZ_ERR_INVALID, "%s: null position or mass array", who);
+    if (p->zeta) return fail(GZ_ERR_UNSUPPORTED, "%s: zeta corrections are not supported", who);
+    if (p->n >= 0x7FFFFFFFull) return fail(GZ_ERR_INVALID, "%s: n=%zu too large", who, p->n);
+    return GZ_OK;
+}
+
+float ev_ms(hipEvent_t a, hipEvent_t b) {
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, a, b) != hipSuccess) return 0.f;
+    return ms;
+}
+
+// Carve consecutive, 256-byte aligned pieces out of one scratch buffer.
+struct Carver {
+    size_t used = 0;
+    template <class T> size_t add(size_t count) {
+        const size_t off = used;
+        used += (count * sizeof(T) + 255) & ~size_t(255);
+        return off;
+    }
+};
+
+// The walk shared by gz_tree_accel_grouped and gz_tree_accel.
+gz_status walk(gz_ctx* ctx, const gz_tree* tree, const gz_particles* p, const uint32_t* targets,
+               size_t n_t, double theta, double G, int batch, int group_size, const double* aold,
+               const float* group_theta, int precision, int persistent, int leaf_always_open,
+               double* ax, double* ay, double* az, gz_walk_counts* counts, const char* who) {
+    GZ_TRY(check_ctx(ctx));
+    if (!tree) return fail(GZ_ERR_INVALID, "%s: null tree", who);
+    GZ_TRY(check_particles(p, who));
+    if (p->n != tree->n)
+        return fail(GZ_ERR_INVALID, "%s: particle count %zu differs from the tree's %zu", who,
+                    p->n, tree->n);
+    if (n_t > 0 && (!targets || !ax || !ay || !az))
+        return fail(GZ_ERR_INVALID, "%s: null target or output array", who);
+    if (n_t >= 0x7FFFFFFFull) return fail(GZ_ERR_INVALID, "%s: n_t=%zu too large", who, n_t);
+    if (precision != GZ_PREC_FP64 && precision != GZ_PREC_FP32_REL)
+        return fail(GZ_ERR_INVALID, "%s: unknown precision %d", who, precision);
+    if (group_size < 0 || group_size > 4096)
+        return fail(GZ_ERR_INVALID, "%s: group_size %d out of range [0, 4096]", who, group_size);
+    if (counts) std::memset(counts, 0, sizeof *counts);
+    if (n_t == 0) return GZ_OK;
+    hipStream_t st = ctx->stream;
+    hipEvent_t* ev = ctx->ev;
+
+    // scratch: per-target group keys/flags/ids, group starts, scan temp, counters
+    Carver c;
+    const size_t o_key = c.add<int>(n_t), o_flag = c.add<uint32_t>(n_t),
+                 o_gid = c.add<uint32_t>(n_t), o_gstart = c.add<int>(n_t + 1),
+                 o_misc = c.add<long long>(8), o_tmp = c.add<uint8_t>(gz::scan_tmp_bytes(n_t));
+    gz::DevMem& s = ctx->scratch[5];
+    GZ_TRY(s.reserve(c.used));
+    uint8_t* b = s.as<uint8_t>();
+    int* gkey = reinterpret_cast<int*>(b + o_key);
+    uint32_t* flag = reinterpret_cast<uint32_t*>(b + o_flag);
+    uint32_t* gid = reinterpret_cast<uint32_t*>(b + o_gid);
+    int* gstart = reinterpret_cast<int*>(b + o_gstart);
+    long long* misc = reinterpret_cast<long long*>(b + o_misc);
+    uint32_t* ngroups_dev = reinterpret_cast<uint32_t*>(misc + 4);
+    int* queue = reinterpret_cast<int*>(misc + 5);
+    int* status = reinterpret_cast<int*>(misc + 6);
+
+    GZ_HIP(hipEventRecord(ev[8], st));
+    GZ_TRY(gz::fill_zero_u32(reinterpret_cast<uint32_t*>(misc), 16, st));
+    GZ_TRY(gz::launch_group_prep(targets, n_t, tree->leaf_of, tree->parent, tree->wn, batch,
+                                 group_size, p->n, gkey, flag, gid, ngroups_dev, status, b + o_tmp,
+                                 st));
+    uint32_t prep[4] = {0, 0, 0, 0}; // misc + 4 .. misc + 6 as 32-bit words: ngroups, _, queue, _
+    GZ_HIP(hipMemcpyAsync(prep, ngroups_dev, sizeof prep, hipMemcpyDeviceToHost, st));
+    int prep_status = 0;
+    GZ_HIP(hipMemcpyAsync(&prep_status, status, sizeof prep_status, hipMemcpyDeviceToHost, st));
+    GZ_HIP(hipStreamSynchronize(st));
+    const uint32_t ngroups = prep[0];
+    if (prep_status & 2)
+        return fail(GZ_ERR_INVALID, "%s: a target index is outside the particle set (n=%zu)", who,
+                    p->n);
+    GZ_TRY(gz::launch_group_starts(n_t, flag, gid, gstart, st));
+
+    gz::DevMem& sg = ctx->scratch[6];
+    GZ_TRY(sg.reserve((size_t)ngroups * sizeof(gz::GroupInfo) + 256));
+    gz::GroupInfo* gi = sg.as<gz::GroupInfo>();
+    GZ_TRY(gz::launch_group_bounds((int)ngroups, n_t, gstart, targets, p, aold, group_theta,
+                                   theta * theta, gi, st));
+    // counters, queue and status start at zero (a fill kernel rather than a memset node, so the
+    // sequence stays valid under stream capture)
+    GZ_TRY(gz::fill_zero_u32(reinterpret_cast<uint32_t*>(misc), 16, st));
+    GZ_HIP(hipEventRecord(ev[9], st));
+
+    gz::WalkArgs a{};
+    a.wn = tree->wn;
+    a.src = tree->src;
+    a.src_soft = tree->src_soft;
+    a.targets = targets;
+    a.x = p->x; a.y = p->y; a.z = p->z; a.soft = p->soft;
+    a.groups = gi;
+    a.ngroups = (int)ngroups;
+    a.root = tree->root;
+    a.persistent = persistent;
+    a.leaf_always_open = leaf_always_open;
+    a.queue = queue;
+    a.status = status;
+    a.counts = misc;
+    a.G = G;
+    a.ax = ax; a.ay = ay; a.az = az;
+    GZ_TRY(gz::launch_walk(a, precision == GZ_PREC_FP64, counts != nullptr, st));
+    GZ_HIP(hipEventRecord(ev[10], st));
+
+    long long host_misc[8] = {0};
+    GZ_HIP(hipMemcpyAsync(host_misc, misc, sizeof host_misc, hipMemcpyDeviceToHost, st));
+    GZ_HIP(hipStreamSynchronize(st));
+    if (host_misc[6] & 1)
+        return fail(GZ_ERR_INTERNAL, "%s: a group's node stack overflowed its LDS capacity", who);
+    if (counts) {
+        counts->nodes_visited = host_misc[0];
+        counts->multipole = host_misc[1];
+        counts->pairs = host_misc[2];
+        counts->groups = ngroups;
+        counts->items = ngroups;
+        counts->prep_ms = ev_ms(ev[8], ev[9]);
+        counts->walk_ms = ev_ms(ev[9], ev[10]);
+    }
+    return GZ_OK;
+}
+
+} // namespace
+
+extern "C" {
+
+const char* gz_last_error(void) { return gz::last_error_cstr(); }
+
+const char* gz_status_string(gz_status s) {
+    switch (s) {
+    case GZ_OK: return "GZ_OK";
+    case GZ_ERR_INVALID: return "GZ_ERR_INVALID";
+    case GZ_ERR_HIP: return "GZ_ERR_HIP";
+    case GZ_ERR_NOMEM: return "GZ_ERR_NOMEM";
+    case GZ_ERR_UNSUPPORTED: return "GZ_ERR_UNSUPPORTED";
+    case GZ_ERR_ARENA: return "GZ_ERR_ARENA";
+    case GZ_ERR_INTERNAL: return "GZ_ERR_INTERNAL";
+    }
+    return "unknown gz_status";
+}
+
+gz_status gz_ctx_create(int device, void* stream, gz_ctx** out) {
+    gz::clear_error();
+    if (!out) return fail(GZ_ERR_INVALID, "gz_ctx_create: null out");
+    *out = nullptr;
+    int ndev = 0;
+    GZ_HIP(hipGetDeviceCount(&ndev));
+    if (device < 0 || device >= ndev)
+        return fail(GZ_ERR_INVALID, "gz_ctx_create: device %d of %d", device, ndev);
+    GZ_HIP(hipSetDevice(device));
+    gz_ctx* c = new (std::nothrow) gz_ctx();
+    if (!c) return fail(GZ_ERR_NOMEM, "gz_ctx_create: out of host memory");
+    c->device = device;
+    if (stream) {
+        c->stream = static_cast<hipStream_t>(stream);
+    } else {
+        hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+        if (e != hipSuccess) {
+            delete c;
+            return gz::hip_fail(e, "hipStreamCreateWithFlags", __FILE__, __LINE__);
+        }
+        c->own_stream = true;
+    }
+    for (auto& e : c->ev) {
+        hipError_t r = hipEventCreate(&e);
+        if (r != hipSuccess) {
+            gz_ctx_destroy(c);
+            return gz::hip_fail(r, "hipEventCreate", __FILE__, __LINE__);
+        }
+    }
+    *out = c;
+    return GZ_OK;
+}
+
+gz_status gz_ctx_destroy(gz_ctx* ctx) {
+    if (!ctx) return GZ_OK;
+    (void)hipSetDevice(ctx->device);
+    if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    for (auto& m : ctx->scratch) m.release();
+    for (auto& e : ctx->ev)
+        if (e) (void)hipEventDestroy(e);
+    if (ctx->own_stream && ctx->stream) (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return GZ_OK;
+}
+
+gz_status gz_ctx_synchronize(gz_ctx* ctx) {
+    GZ_TRY(check_ctx(ctx));
+    GZ_HIP(hipStreamSynchronize(ctx->stream));
+    return GZ_OK;
+}
+
+gz_status gz_tree_build(gz_ctx* ctx, const gz_particles* p, const gz_build_opts* opts,
+                        gz_tree** out) {
+    gz::clear_error();
+    if (!out) return fail(GZ_ERR_INVALID, "gz_tree_build: null out");
+    *out = nullptr;
+    GZ_TRY(check_ctx(ctx));
+    GZ_TRY(check_particles(p, "gz_tree_build"));
+    gz_tree* t = new (std::nothrow) gz_tree();
+    if (!t) return fail(GZ_ERR_NOMEM, "gz_tree_build: out of host memory");
+    t->ctx = ctx;
+    gz_status s = gz::build_tree(ctx, p, opts, t);
+    if (s != GZ_OK) {
+        (void)hipStreamSynchronize(ctx->stream);
+        gz::tree_release(t);
+        delete t;
+        return s;
+    }
+    *out = t;
+    return GZ_OK;
+}
+
+gz_status gz_tree_free(gz_tree* tree) {
+    if (!tree) return GZ_OK;
+    if (tree->ctx) {
+        (void)hipSetDevice(tree->ctx->device);
+        (void)hipStreamSynchronize(tree->ctx->stream);
+    }
+    gz::tree_release(tree);
+    delete tree;
+    return GZ_OK;
+}
+
+gz_status gz_tree_get_info(const gz_tree* tree, gz_tree_info* info) {
+    if (!tree || !info) return fail(GZ_ERR_INVALID, "gz_tree_get_info: null argument");
+    info->n = (long long)tree->n;
+    info->n_live = (long long)tree->n_live;
+    info->nnodes = tree->nnodes;
+    info->nleaves = tree->nleaves;
+    info->root = tree->root;
+    info->depth = tree->depth;
+    info->arena_grows = tree->arena_grows;
+    for (int k = 0; k < 3; ++k) info->root_centre[k] = tree->root_box[k];
+    info->root_side = tree->root_box[3];
+    info->times = tree->times;
+    return GZ_OK;
+}
+
+gz_status gz_tree_download_nodes(const gz_tree* tree, int* first, int* next, int* nchild,
+                                 int* plo, int* phi, int* parent, int* level, double* size,
+                                 double* mass, double* cx, double* cy, double* cz, double* soft,
+                                 double* delta, double* gcx, double* gcy, double* gcz,
+                                 double* wn_s, float* wn_soft, float* wn_len) {
+    if (!tree) return fail(GZ_ERR_INVALID, "gz_tree_download_nodes: null tree");
+    GZ_TRY(check_ctx(tree->ctx));
+    hipStream_t st = tree->ctx->stream;
+    const size_t nn = (size_t)tree->nnodes;
+    if (nn == 0) return GZ_OK;
+    std::vector<gz::WNode> w(nn);
+    GZ_HIP(hipMemcpyAsync(w.data(), tree->wn, nn * sizeof(gz::WNode), hipMemcpyDeviceToHost, st));
+    auto dl = [&](void* dst, const void* src, size_t bytes) -> gz_status {
+        if (dst) GZ_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, st));
+        return GZ_OK;
+    };
+    GZ_TRY(dl(next, tree->next, nn * sizeof(int)));
+    GZ_TRY(dl(parent, tree->parent, nn * sizeof(int)));
+    GZ_TRY(dl(size, tree->size, nn * sizeof(double)));
+    GZ_TRY(dl(soft, tree->soft, nn * sizeof(double)));
+    GZ_TRY(dl(delta, tree->delta, nn * sizeof(double)));
+    GZ_TRY(dl(gcx, tree->gcx, nn * sizeof(double)));
+    GZ_TRY(dl(gcy, tree->gcy, nn * sizeof(double)));
+    GZ_TRY(dl(gcz, tree->gcz, nn * sizeof(double)));
+    GZ_HIP(hipStreamSynchronize(st));
+    for (size_t i = 0; i < nn; ++i) {
+        const gz::WNode& q = w[i];
+        if (first) first[i] = q.first;
+        if (nchild) nchild[i] = q.nchild;
+        if (plo) plo[i] = q.plo;
+        if (phi) phi[i] = q.phi;
+        if (mass) mass[i] = q.mass;
+        if (cx) cx[i] = q.cx;
+        if (cy) cy[i] = q.cy;
+        if (cz) cz[i] = q.cz;
+        if (wn_s) wn_s[i] = q.s;
+        if (wn_soft) wn_soft[i] = q.soft;
+        if (wn_len) wn_len[i] = q.len;
+    }
+    if (level) {
+        for (size_t L = 0; L + 1 < tree->level_begin.size(); ++L)
+            for (int i = tree->level_begin[L]; i < tree->level_begin[L + 1]; ++i) level[i] = (int)L;
+    }
+    return GZ_OK;
+}
+
+gz_status gz_tree_download_particles(const gz_tree* tree, uint32_t* orderbuf, uint32_t* rank,
+                                     int* leaf_of, uint64_t* skey) {
+    if (!tree) return fail(GZ_ERR_INVALID, "gz_tree_download_particles: null tree");
+    GZ_TRY(check_ctx(tree->ctx));
+    hipStream_t st = tree->ctx->stream;
+    if (orderbuf && tree->n_live)
+        GZ_HIP(hipMemcpyAsync(orderbuf, tree->orderbuf, tree->n_live * sizeof(uint32_t),
+                              hipMemcpyDeviceToHost, st));
+    if (skey && tree->n_live)
+        GZ_HIP(hipMemcpyAsync(skey, tree->skey, tree->n_live * sizeof(uint64_t),
+                              hipMemcpyDeviceToHost, st));
+    if (rank && tree->n)
+        GZ_HIP(hipMemcpyAsync(rank, tree->rank, tree->n * sizeof(uint32_t), hipMemcpyDeviceToHost,
+                              st));
+    if (leaf_of && tree->n)
+        GZ_HIP(hipMemcpyAsync(leaf_of, tree->leaf_of, tree->n * sizeof(int),
+                              hipMemcpyDeviceToHost, st));
+    GZ_HIP(hipStreamSynchronize(st));
+    return GZ_OK;
+}
+
+gz_status gz_tree_device_arrays(const gz_tree* tree, const uint32_t** orderbuf,
+                                const uint32_t** rank, const int** leaf_of) {
+    if (!tree) return fail(GZ_ERR_INVALID, "gz_tree_device_arrays: null tree");
+    if (orderbuf) *orderbuf = tree->orderbuf;
+    if (rank) *rank = tree->rank;
+    if (leaf_of) *leaf_of = tree->leaf_of;
+    return GZ_OK;
+}
+
+gz_status gz_tree_update_sources(gz_ctx* ctx, gz_tree* tree, const gz_particles* p) {
+    gz::clear_error();
+    GZ_TRY(check_ctx(ctx));
+    if (!tree) return fail(GZ_ERR_INVALID, "gz_tree_update_sources: null tree");
+    GZ_TRY(check_particles(p, "gz_tree_update_sources"));
+    return gz::regather_sources(ctx, p, tree);
+}
+
+gz_status gz_tree_accel_grouped(gz_ctx* ctx, const gz_tree* tree, const gz_particles* p,
+                                const uint32_t* targets, size_t n_t, double theta, double G,
+                                int batch, const double* aold, double* ax, double* ay, double* az,
+                                const gz_walk_opts* opts, gz_walk_counts* counts) {
+    gz::clear_error();
+    int precision = GZ_PREC_FP64, group_size = 0, persistent = 0;
+    const float* group_theta = nullptr;
+    if (opts) {
+        precision = opts->precision;
+        group_size = opts->group_size;
+        persistent = opts->persistent;
+        if (opts->criterion == GZ_CRIT_LEARNED)
+            return fail(GZ_ERR_UNSUPPORTED,
+                        "gz_tree_accel_grouped: use gz_tree_accel_learned for the learned criterion");
+        if (opts->criterion != GZ_CRIT_REFERENCE)
+            return fail(GZ_ERR_INVALID, "gz_tree_accel_grouped: unknown criterion %d",
+                        opts->criterion);
+    }
+    if (batch > 0 && group_size > 0)
+        return fail(GZ_ERR_INVALID, "gz_tree_accel_grouped: batch > 0 and group_size > 0");
+    return walk(ctx, tree, p, targets, n_t, theta, G, batch, group_size, aold, group_theta,
+                precision, persistent, 0, ax, ay, az, counts, "gz_tree_accel_grouped");
+}
+
+gz_status gz_tree_accel(gz_ctx* ctx, const gz_tree* tree, const gz_particles* p,
+                        const uint32_t* targets, size_t n_t, double theta, double G,
+                        const double* aold, double* ax, double* ay, double* az) {
+    gz::clear_error();
+    // one group per target: the group box is the target's position, so the opening distance is
+    // |COM - x| per axis exactly as shmem::accel computes it (tree.cc:552-557)
+    return walk(ctx, tree, p, targets, n_t, theta, G, 1, 0, aold, nullptr, GZ_PREC_FP64, 0, 1, ax,
+                ay, az, nullptr, "gz_tree_accel");
+}
+
+gz_status gz_direct_sum(gz_ctx* ctx, const gz_particles* p, const uint32_t* targets, size_t n_t,
+                        double G, double* ax, double* ay, double* az) {
+    gz::clear_error();
+    GZ_TRY(check_ctx(ctx));
+    GZ_TRY(check_particles(p, "gz_direct_sum"));
+    if (n_t > 0 && (!targets || !ax || !ay || !az))
+        return fail(GZ_ERR_INVALID, "gz_direct_sum: null target or output array");
+    return gz::launch_direct(p, targets, n_t, G, ax, ay, az, ctx->stream);
+}
+
+} // extern "C"

@@ -1,0 +1,176 @@
+// The caller side of a force evaluation, on the device: active targets in Morton order, the
+// relative-criterion aold, the scatter of accelerations into the per-particle array and the
+// gravity part of the KDK kick. Follows compute_gravity (shmem/mfm.cc:698-751, 828-831) and the
+// kick (mfm.cc:3749-3767).
+#include <algorithm>
+#include <cmath>
+
+#include "gz_internal.h"
+
+namespace gz {
+
+constexpr int kT = 256;
+static inline unsigned blocks(size_t n) { return (unsigned)((n + kT - 1) / kT); }
+
+__global__ __launch_bounds__(kT) void k_mark_active(const uint32_t* __restrict__ active, size_t na,
+                                                    const uint32_t* __restrict__ rank, size_t n,
+                                                    uint32_t* __restrict__ flag) {
+    const size_t k = (size_t)blockIdx.x * kT + threadIdx.x;
+    if (k >= na) return;
+    const uint32_t i = active[k];
+    if (i >= n) return;
+    const uint32_t r = rank[i];
+    if (r != 0xFFFFFFFFu) flag[r] = 1u;
+}
+
+__global__ __launch_bounds__(kT) void k_compact_active(const uint32_t* __restrict__ orderbuf,
+                                                       size_t nl, const uint32_t* __restrict__ flag,
+                                                       const uint32_t* __restrict__ pos,
+                                                       uint32_t* __restrict__ out) {
+    const size_t r = (size_t)blockIdx.x * kT + threadIdx.x;
+    if (r < nl && flag[r]) out[pos[r]] = orderbuf[r];
+}
+
+__global__ __launch_bounds__(kT) void k_aold(const double* __restrict__ a_prev,
+                                             const uint32_t* __restrict__ targets, size_t nt,
+                                             double tol, double G, double* __restrict__ aold) {
+    const size_t k = (size_t)blockIdx.x * kT + threadIdx.x;
+    if (k >= nt) return;
+    const size_t i = targets[k];
+    const double ax = a_prev[3 * i], ay = a_prev[3 * i + 1], az = a_prev[3 * i + 2];
+    aold[k] = tol * sqrt(ax * ax + ay * ay + az * az) / G; // mfm.cc:750
+}
+
+__global__ __launch_bounds__(kT) void k_scatter(const uint32_t* __restrict__ targets, size_t nt,
+                                                const double* __restrict__ ax,
+                                                const double* __restrict__ ay,
+                                                const double* __restrict__ az,
+                                                double* __restrict__ a_grav) {
+    const size_t k = (size_t)blockIdx.x * kT + threadIdx.x;
+    if (k >= nt) return;
+    const size_t i = targets[k];
+    a_grav[3 * i] = ax[k];
+    a_grav[3 * i + 1] = ay[k];
+    a_grav[3 * i + 2] = az[k];
+}
+
+// Close the previous step's half kick, open the next one (mfm.cc:3716-3730, 3753-3766).
+__global__ __launch_bounds__(kT) void k_kick(const uint32_t* __restrict__ active, size_t na,
+                                             const double* __restrict__ a_grav,
+                                             const double* __restrict__ a_extra,
+                                             const double* __restrict__ dt,
+                                             double* __restrict__ pending, double* __restrict__ vx,
+                                             double* __restrict__ vy, double* __restrict__ vz) {
+    const size_t k = (size_t)blockIdx.x * kT + threadIdx.x;
+    if (k >= na) return;
+    const size_t i = active[k];
+    double a0 = a_grav[3 * i], a1 = a_grav[3 * i + 1], a2 = a_grav[3 * i + 2];
+    if (a_extra) {
+        a0 += a_extra[3 * i];
+        a1 += a_extra[3 * i + 1];
+        a2 += a_extra[3 * i + 2];
+    }
+    const double close_dt = pending[i];
+    const double open_dt = 0.5 * dt[k];
+    double v0 = vx[i], v1 = vy[i], v2 = vz[i];
+    v0 += a0 * close_dt; v1 += a1 * close_dt; v2 += a2 * close_dt;
+    v0 += a0 * open_dt;  v1 += a1 * open_dt;  v2 += a2 * open_dt;
+    vx[i] = v0; vy[i] = v1; vz[i] = v2;
+    pending[i] = open_dt;
+}
+
+} // namespace gz
+
+using gz::fail;
+
+extern "C" {
+
+gz_status gz_order_targets(gz_ctx* ctx, const gz_tree* tree, const uint32_t* active,
+                           size_t n_active, uint32_t* out_targets, size_t* n_out) {
+    gz::clear_error();
+    if (!ctx || !tree || !n_out) return fail(GZ_ERR_INVALID, "gz_order_targets: null argument");
+    GZ_HIP(hipSetDevice(ctx->device));
+    hipStream_t st = ctx->stream;
+    *n_out = 0;
+    const size_t nl = tree->n_live;
+    if (nl == 0) return GZ_OK;
+    if (!out_targets) return fail(GZ_ERR_INVALID, "gz_order_targets: null output");
+    if (!active || n_active >= tree->n) {
+        // everything active: the Morton order is orderbuf (mfm.cc:700-702)
+        GZ_HIP(hipMemcpyAsync(out_targets, tree->orderbuf, nl * sizeof(uint32_t),
+                              hipMemcpyDeviceToDevice, st));
+        GZ_HIP(hipStreamSynchronize(st));
+        *n_out = nl;
+        return GZ_OK;
+    }
+    // A subset: flag the active slots of the Morton order and compact orderbuf by the flags, the
+    // reference's filtering scan (mfm.cc:710-717). Particles outside the tree are dropped there.
+    gz::DevMem& s = ctx->scratch[7];
+    const size_t fb = ((nl + 64) * sizeof(uint32_t) + 255) & ~size_t(255);
+    GZ_TRY(s.reserve(2 * fb + gz::scan_tmp_bytes(nl) + 256));
+    uint8_t* b = s.as<uint8_t>();
+    uint32_t* flag = reinterpret_cast<uint32_t*>(b);
+    uint32_t* pos = reinterpret_cast<uint32_t*>(b + fb);
+    uint32_t* total = pos + nl;
+    void* tmp = b + 2 * fb;
+    GZ_TRY(gz::fill_zero_u32(flag, nl, st));
+    if (n_active)
+        gz::k_mark_active<<<gz::blocks(n_active), gz::kT, 0, st>>>(active, n_active, tree->rank,
+                                                                   tree->n, flag);
+    GZ_HIP(hipGetLastError());
+    GZ_TRY(gz::exclusive_scan_u32(flag, pos, nl, total, tmp, st));
+    gz::k_compact_active<<<gz::blocks(nl), gz::kT, 0, st>>>(tree->orderbuf, nl, flag, pos,
+                                                            out_targets);
+    GZ_HIP(hipGetLastError());
+    uint32_t h = 0;
+    GZ_HIP(hipMemcpyAsync(&h, total, sizeof h, hipMemcpyDeviceToHost, st));
+    GZ_HIP(hipStreamSynchronize(st));
+    *n_out = h;
+    return GZ_OK;
+}
+
+gz_status gz_aold_from_accel(gz_ctx* ctx, const double* a_prev, const uint32_t* targets,
+                             size_t n_t, double err_tol_force_acc, double G, double* aold) {
+    gz::clear_error();
+    if (!ctx) return fail(GZ_ERR_INVALID, "gz_aold_from_accel: null context");
+    if (n_t && (!a_prev || !targets || !aold))
+        return fail(GZ_ERR_INVALID, "gz_aold_from_accel: null array");
+    if (!(G != 0.0)) return fail(GZ_ERR_INVALID, "gz_aold_from_accel: G must be non-zero");
+    GZ_HIP(hipSetDevice(ctx->device));
+    if (n_t)
+        gz::k_aold<<<gz::blocks(n_t), gz::kT, 0, ctx->stream>>>(a_prev, targets, n_t,
+                                                               err_tol_force_acc, G, aold);
+    GZ_HIP(hipGetLastError());
+    return GZ_OK;
+}
+
+gz_status gz_scatter_accel(gz_ctx* ctx, const uint32_t* targets, size_t n_t, const double* ax,
+                           const double* ay, const double* az, double* a_grav) {
+    gz::clear_error();
+    if (!ctx) return fail(GZ_ERR_INVALID, "gz_scatter_accel: null context");
+    if (n_t && (!targets || !ax || !ay || !az || !a_grav))
+        return fail(GZ_ERR_INVALID, "gz_scatter_accel: null array");
+    GZ_HIP(hipSetDevice(ctx->device));
+    if (n_t)
+        gz::k_scatter<<<gz::blocks(n_t), gz::kT, 0, ctx->stream>>>(targets, n_t, ax, ay, az,
+                                                                  a_grav);
+    GZ_HIP(hipGetLastError());
+    return GZ_OK;
+}
+
+gz_status gz_kick(gz_ctx* ctx, const uint32_t* active, size_t n_active, const double* a_grav,
+                  const double* a_extra, const double* dt, double* pending_half_kick, double* vx,
+                  double* vy, double* vz) {
+    gz::clear_error();
+    if (!ctx) return fail(GZ_ERR_INVALID, "gz_kick: null context");
+    if (n_active && (!active || !a_grav || !dt || !pending_half_kick || !vx || !vy || !vz))
+        return fail(GZ_ERR_INVALID, "gz_kick: null array");
+    GZ_HIP(hipSetDevice(ctx->device));
+    if (n_active)
+        gz::k_kick<<<gz::blocks(n_active), gz::kT, 0, ctx->stream>>>(
+            active, n_active, a_grav, a_extra, dt, pending_half_kick, vx, vy, vz);
+    GZ_HIP(hipGetLastError());
+    return GZ_OK;
+}
+
+} // extern "C"

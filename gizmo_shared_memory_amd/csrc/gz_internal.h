@@ -1,0 +1,167 @@
+// Internal declarations shared by the device library's translation units.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "gz_grav.h"
+
+namespace gz {
+
+constexpr int kLeafMax = 16;  // particles per leaf (shmem/tree.h:357)
+constexpr int kMaxLevel = 20; // deepest level; keys carry 20 populated bits per axis (tree.h:358)
+constexpr int kWave = 64;     // CDNA wavefront width
+
+// Packed walk node: everything one opening test and one multipole interaction need, in one
+// 64-byte line (the reference's WNode, tree.h:195-204, with the threaded `next` replaced by the
+// child count: children are contiguous in this layout, so a walk expands them as a range).
+struct alignas(64) WNode {
+    double cx, cy, cz; // centre of mass
+    double s;          // size + delta: the conservative opening radius
+    double mass;
+    float soft;        // max softening below the node
+    float len;         // side length
+    int first;         // first child, or -1 for a leaf
+    int nchild;        // number of children (0 for a leaf)
+    int plo, phi;      // particle range in Morton order
+};
+static_assert(sizeof(WNode) == 64, "WNode must be one 64-byte line");
+
+// Opening bounds of one target group (tree.cc:683-702) and its slot range in the target list.
+struct GroupInfo {
+    double lo[3], hi[3]; // bounding box of the targets
+    double emax;         // largest target softening
+    double amin;         // smallest aold (0 when there is none)
+    double theta_sq;     // geometric opening parameter squared for this group
+    int tlo, thi;        // target slots [tlo, thi)
+};
+
+// Everything the walk kernel reads. Passed by value.
+struct WalkArgs {
+    const WNode* wn;
+    const double4* src;     // (x, y, z, m) per sorted slot
+    const double* src_soft; // softening per sorted slot
+    const uint32_t* targets;
+    const double *x, *y, *z, *soft; // target data, indexed by particle
+    const GroupInfo* groups;
+    int ngroups;
+    int root;
+    int persistent;
+    int leaf_always_open;   // 1: a reached leaf is always summed pair by pair (shmem::accel)
+    int* queue;             // persistent work counter (zeroed before the launch)
+    int* status;            // bit 0: a group's node stack overflowed
+    long long* counts;      // nodes visited, multipole, pairs (COUNT builds only)
+    double G;
+    double *ax, *ay, *az;
+};
+
+// ---- errors ---------------------------------------------------------------------------------
+void set_error(const char* fmt, ...);
+gz_status fail(gz_status s, const char* fmt, ...);
+gz_status hip_fail(hipError_t e, const char* what, const char* file, int line);
+
+#define GZ_HIP(call)                                                          \
+    do {                                                                      \
+        hipError_t gz_e_ = (call);                                            \
+        if (gz_e_ != hipSuccess) return ::gz::hip_fail(gz_e_, #call, __FILE__, __LINE__); \
+    } while (0)
+
+#define GZ_TRY(expr)                       \
+    do {                                   \
+        gz_status gz_s_ = (expr);          \
+        if (gz_s_ != GZ_OK) return gz_s_;  \
+    } while (0)
+
+// Grow-only device allocation.
+struct DevMem {
+    void* p = nullptr;
+    size_t bytes = 0;
+    gz_status reserve(size_t want);
+    void release();
+    template <class T> T* as() const { return static_cast<T*>(p); }
+};
+
+// ---- device-wide primitives (gz_prims.hip) --------------------------------------------------
+// Exclusive scan of n uint32 values (in may equal out). total_dev, when non-null, receives the
+// sum (device). tmp must hold scan_tmp_bytes(n).
+size_t scan_tmp_bytes(size_t n);
+gz_status exclusive_scan_u32(const uint32_t* in, uint32_t* out, size_t n, uint32_t* total_dev,
+                             void* tmp, hipStream_t st);
+
+// Stable LSD radix sort of (key, value) pairs by the low `bits` bits of the key, 8 bits per pass,
+// skipping passes whose digit is the same for every key. The *_alt buffers are scratch of the
+// same size; the pointers are swapped so that on return keys/vals hold the sorted pairs.
+gz_status radix_sort_pairs(uint64_t*& keys, uint32_t*& vals, uint64_t*& keys_alt,
+                           uint32_t*& vals_alt, size_t n, int bits, DevMem& scratch,
+                           hipStream_t st);
+
+// Zero n uint32 words with a kernel (no memset node, so the call is safe under stream capture).
+gz_status fill_zero_u32(uint32_t* a, size_t n, hipStream_t st);
+
+const char* last_error_cstr();
+void clear_error();
+
+// ---- walks (gz_walk.hip) ------------------------------------------------------------------------
+gz_status launch_group_prep(const uint32_t* targets, size_t nt, const int* leaf_of,
+                            const int* parent, const WNode* wn, int batch, int group_size,
+                            size_t n, int* gkey, uint32_t* flag, uint32_t* gid,
+                            uint32_t* ngroups_dev, int* status, void* scan_tmp, hipStream_t st);
+gz_status launch_group_starts(size_t nt, const uint32_t* flag, const uint32_t* gid, int* gstart,
+                              hipStream_t st);
+gz_status launch_group_bounds(int ngroups, size_t nt, const int* gstart, const uint32_t* targets,
+                              const gz_particles* p, const double* aold, const float* group_theta,
+                              double theta_sq, GroupInfo* gi, hipStream_t st);
+gz_status launch_walk(const WalkArgs& a, bool fp64, bool count, hipStream_t st);
+gz_status launch_direct(const gz_particles* p, const uint32_t* targets, size_t nt, double G,
+                        double* ax, double* ay, double* az, hipStream_t st);
+
+} // namespace gz
+
+// ---- handle types -----------------------------------------------------------------------------
+struct gz_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    bool own_stream = false;
+    gz::DevMem scratch[8];
+    hipEvent_t ev[16] = {};
+};
+
+struct gz_tree {
+    gz_ctx* ctx = nullptr;
+    size_t n = 0, n_live = 0;
+    int nnodes = 0, nleaves = 0, depth = 0, arena_grows = 0, root = 0;
+    int cap = 0;
+    double root_box[4] = {0, 0, 0, 0};
+    std::vector<int> level_begin; // node ids of level L are [level_begin[L], level_begin[L+1])
+    gz_build_times times = {};
+
+    // particles, Morton order
+    uint64_t* skey = nullptr;     // n_live sorted keys
+    uint32_t* orderbuf = nullptr; // n_live particle index per sorted slot
+    uint32_t* rank = nullptr;     // n: sorted slot per particle, UINT32_MAX if not in the tree
+    int* leaf_of = nullptr;       // n: leaf node per particle, -1 if not in the tree
+    double4* src = nullptr;       // n_live (x, y, z, m) in sorted order
+    double* src_soft = nullptr;   // n_live softening in sorted order
+
+    // nodes
+    gz::WNode* wn = nullptr;
+    int* next = nullptr;   // the reference's threaded successor
+    int* parent = nullptr;
+    int* nlo = nullptr;    // particle range during the build
+    int* nhi = nullptr;
+    double* gcx = nullptr; // geometric centre and side
+    double* gcy = nullptr;
+    double* gcz = nullptr;
+    double* size = nullptr;
+    double* delta = nullptr;
+    double* soft = nullptr; // FP64 max softening
+};
+
+namespace gz {
+gz_status build_tree(gz_ctx* ctx, const gz_particles* p, const gz_build_opts* opts, gz_tree* t);
+gz_status regather_sources(gz_ctx* ctx, const gz_particles* p, gz_tree* t);
+void tree_release(gz_tree* t);
+} // namespace gz

@@ -1,0 +1,576 @@
+// Tree walks on the device: target grouping, per-group opening bounds, the group-coherent walk
+// with an LDS interaction list (FP64 and FP32 group-relative), and the exact direct sum.
+//
+// Semantics follow shmem::accel_grouped (shmem/tree.cc:626-897) with tidal = jerk = lazy = NULL
+// and no sink-direct radius:
+//   * groups are runs of Morton-ordered targets that share a leaf (tree.cc:656-667), or fixed
+//     windows of `batch` targets, or (group_size > 0) runs under the highest node that holds at most
+//     group_size particles;
+//   * each group is tested against every node with the bounding box of its targets, their largest
+//     softening and their smallest aold (tree.cc:683-702), using the reference's open_node
+//     (tree.h:109-144) evaluated in FP64 exactly as written there;
+//   * an accepted node (leaves included) is a monopole, an opened leaf is summed pair by pair
+//     (tree.cc:779-874), both with the cubic-spline kernel of the larger softening (tree.h:59-67,
+//     tree.cc:39-53, 58-68).
+//
+// How it runs on CDNA: one wavefront (64 lanes) owns one group. The traversal is wave-coherent:
+// up to 64 candidate nodes are popped from an LDS stack at once, one per lane, their opening tests
+// run in parallel (the decision only depends on the group, so it is the same for every target),
+// accepted nodes and the particles of opened leaves are appended to an LDS interaction list, and
+// the children of opened interior nodes (contiguous in this tree's level-major numbering) are
+// pushed back. When the list fills, the whole wave evaluates it: lanes are (target, sub-slice)
+// pairs, each reading list entries as LDS broadcasts. The set of interactions each target receives
+// is exactly the reference's; only the order of the summation differs.
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+
+#include "gz_internal.h"
+
+namespace gz {
+
+constexpr int kWalkStack = 2048;  // node ids per wave
+constexpr int kListCap = 512;     // interaction-list entries per wave
+constexpr int kMaxChunk = 64;     // targets evaluated together (one wave)
+
+__device__ __forceinline__ double dmin_w(double a, double b) { return (b < a) ? b : a; }
+__device__ __forceinline__ double dmax_w(double a, double b) { return (a < b) ? b : a; }
+
+// ---- kernels of the pair force (tree.h:59-67), FP64 as the reference writes them --------------
+__device__ __forceinline__ double spline_force_over_r_d(double r, double h) {
+    const double h_inv = 1.0 / h;
+    const double h_inv3 = h_inv * h_inv * h_inv;
+    const double u = r * h_inv;
+    if (u >= 1.0) return 1.0 / (r * r * r);
+    if (u < 0.5) return h_inv3 * (10.666666666666667 + u * u * (32.0 * u - 38.4));
+    return h_inv3 * (21.333333333333332 - 48.0 * u + 38.4 * u * u - 10.666666666666667 * u * u * u -
+                     0.06666666666666667 / (u * u * u));
+}
+
+// open_node (tree.h:109-144), operand for operand.
+__device__ __forceinline__ bool open_node_d(double r_sq, double len, double l_open, double mass,
+                                            double node_soft, double soft_t, double aold,
+                                            double theta_sq, double adx, double ady, double adz) {
+    const double grow_t = soft_t + 0.6 * len, grow_n = node_soft + 0.6 * len;
+    if (r_sq < grow_t * grow_t || r_sq < grow_n * grow_n) return true;
+    if (theta_sq > 0.0 && l_open * l_open >= theta_sq * r_sq) return true;
+    if (aold > 0) {
+        if (mass * len * len > r_sq * r_sq * aold) return true;
+    } else if (theta_sq <= 0.0) {
+        return true;
+    }
+    return adx < 0.6 * len && ady < 0.6 * len && adz < 0.6 * len;
+}
+
+// ---- grouping ------------------------------------------------------------------------------------
+// Group key of each target slot: its leaf (batch <= 0, group_size == 0), the highest ancestor of the
+// leaf holding at most group_size particles (group_size > 0), or the window index (batch > 0).
+// A target outside the tree (leaf_of < 0) always starts a group of its own (tree.cc:663-664).
+__global__ __launch_bounds__(256) void k_group_flags(const uint32_t* __restrict__ targets,
+                                                     size_t nt, const int* __restrict__ leaf_of,
+                                                     const int* __restrict__ parent,
+                                                     const WNode* __restrict__ wn, int batch,
+                                                     int group_size, size_t n,
+                                                     int* __restrict__ gkey,
+                                                     int* __restrict__ status) {
+    const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= nt) return;
+    if (targets[i] >= n) { // reported to the caller before anything reads through it
+        atomicOr(status, 2);
+        gkey[i] = -1;
+        return;
+    }
+    int key;
+    if (batch > 0) {
+        key = (int)(i / (size_t)batch);
+    } else {
+        key = leaf_of[targets[i]];
+        if (key >= 0 && group_size > 0) {
+            int p = parent[key];
+            while (p >= 0 && wn[p].phi - wn[p].plo <= group_size) {
+                key = p;
+                p = parent[p];
+            }
+        }
+    }
+    gkey[i] = key;
+}
+
+__global__ __launch_bounds__(256) void k_group_flags2(size_t nt, int batch,
+                                                      const int* __restrict__ gkey,
+                                                      uint32_t* __restrict__ flag) {
+    const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= nt) return;
+    if (i == 0) {
+        flag[i] = 1u;
+        return;
+    }
+    const int k = gkey[i], kp = gkey[i - 1];
+    if (batch > 0) flag[i] = (k != kp) ? 1u : 0u;
+    else flag[i] = (k != kp || k < 0) ? 1u : 0u;
+}
+
+__global__ __launch_bounds__(256) void k_group_starts(size_t nt, const uint32_t* __restrict__ flag,
+                                                      const uint32_t* __restrict__ gid,
+                                                      int* __restrict__ gstart) {
+    const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= nt) return;
+    if (flag[i]) gstart[gid[i]] = (int)i;
+}
+
+// Per-group opening bounds (tree.cc:683-702): bounding box, largest softening, smallest aold.
+__global__ __launch_bounds__(256) void k_group_bounds(
+    int ngroups, size_t nt, const int* __restrict__ gstart, const uint32_t* __restrict__ targets,
+    const double* __restrict__ x, const double* __restrict__ y, const double* __restrict__ z,
+    const double* __restrict__ soft, const double* __restrict__ aold,
+    const float* __restrict__ group_theta, double theta_sq, GroupInfo* __restrict__ gi) {
+    const int g = blockIdx.x * 256 + threadIdx.x;
+    if (g >= ngroups) return;
+    const int lo = gstart[g];
+    const int hi = (g + 1 < ngroups) ? gstart[g + 1] : (int)nt;
+    double bx0 = 1e300, by0 = 1e300, bz0 = 1e300, bx1 = -1e300, by1 = -1e300, bz1 = -1e300;
+    double emax = 0;
+    for (int i = lo; i < hi; ++i) {
+        const uint32_t p = targets[i];
+        const double px = x[p], py = y[p], pz = z[p];
+        bx0 = dmin_w(bx0, px); bx1 = dmax_w(bx1, px);
+        by0 = dmin_w(by0, py); by1 = dmax_w(by1, py);
+        bz0 = dmin_w(bz0, pz); bz1 = dmax_w(bz1, pz);
+        if (soft) emax = dmax_w(emax, soft[p]);
+    }
+    double amin = aold ? 1e300 : 0.0;
+    if (aold)
+        for (int i = lo; i < hi; ++i) amin = dmin_w(amin, aold[i]);
+    GroupInfo o;
+    o.lo[0] = bx0; o.lo[1] = by0; o.lo[2] = bz0;
+    o.hi[0] = bx1; o.hi[1] = by1; o.hi[2] = bz1;
+    o.emax = emax;
+    o.amin = amin;
+    if (group_theta) {
+        const double th = (double)group_theta[g];
+        o.theta_sq = th * th;
+    } else {
+        o.theta_sq = theta_sq;
+    }
+    o.tlo = lo;
+    o.thi = hi;
+    gi[g] = o;
+}
+
+// ---- the walk --------------------------------------------------------------------------------------
+template <bool FP64> struct WalkTypes;
+template <> struct WalkTypes<true> {
+    using real = double;
+    using real4 = double4;
+};
+template <> struct WalkTypes<false> {
+    using real = float;
+    using real4 = float4;
+};
+
+__device__ __forceinline__ uint64_t lanemask_lt() {
+    const int lane = __lane_id();
+    return (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+}
+
+// Exclusive prefix sum over the wave and its total.
+__device__ __forceinline__ int wave_excl_scan(int v, int& total) {
+    int x = v;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const int y = __shfl_up(x, d);
+        if (__lane_id() >= d) x += y;
+    }
+    total = __shfl(x, 63);
+    return x - v;
+}
+
+// FP32 cubic-spline force/r for u < 1 (tree.h:59-67 in single precision), r = |d|, h > r.
+__device__ __forceinline__ float spline_inner_f(float r, float h) {
+    const float hinv = 1.0f / h;
+    const float hinv3 = hinv * hinv * hinv;
+    const float u = r * hinv;
+    if (u < 0.5f) return hinv3 * (10.666666666666667f + u * u * (32.0f * u - 38.4f));
+    return hinv3 * (21.333333333333332f - 48.0f * u + 38.4f * u * u -
+                    10.666666666666667f * u * u * u - 0.06666666666666667f / (u * u * u));
+}
+
+template <bool FP64, bool COUNT>
+__global__ __launch_bounds__(64) void k_walk(WalkArgs a) {
+    using real = typename WalkTypes<FP64>::real;
+    using real4 = typename WalkTypes<FP64>::real4;
+    __shared__ int s_stack[kWalkStack];
+    __shared__ real4 s_src[kListCap];
+    __shared__ real s_soft[kListCap];
+    __shared__ int s_lpre[65];
+    __shared__ int s_lplo[64];
+    __shared__ unsigned char s_kind[COUNT ? kListCap : 1];
+
+    const int lane = threadIdx.x;
+    long long n_visit = 0, n_multi = 0, n_pair = 0;
+
+    for (int item = 0;; ++item) {
+        // Item order: block b of XCD (b % 8) takes the b/8-th item of that XCD's contiguous share,
+        // so each XCD's L2 serves one compact stretch of Morton-ordered groups.
+        int g;
+        if (a.persistent) {
+            int v = 0;
+            if (lane == 0) v = atomicAdd(a.queue, 1);
+            g = __shfl(v, 0);
+        } else {
+            if (item > 0) break;
+            const int b = blockIdx.x;
+            const int nb = gridDim.x;
+            const int xcd = b & 7, slot = b >> 3;
+            const int per = nb >> 3, rem = nb & 7;
+            g = xcd * per + min(xcd, rem) + slot; // a bijection of [0, nb)
+        }
+        if (g >= a.ngroups) break;
+
+        const GroupInfo gi = a.groups[g];
+        const double gcx = 0.5 * (gi.lo[0] + gi.hi[0]);
+        const double gcy = 0.5 * (gi.lo[1] + gi.hi[1]);
+        const double gcz = 0.5 * (gi.lo[2] + gi.hi[2]);
+        const int nt_all = gi.thi - gi.tlo;
+
+        for (int c0 = 0; c0 < nt_all; c0 += kMaxChunk) {
+            const int tlo = gi.tlo + c0;
+            const int nt = min(kMaxChunk, nt_all - c0);
+            int T = 8;
+            while (T < nt) T <<= 1; // targets per sub-slice: 8, 16, 32 or 64
+            const int L = 64 / T;   // lanes per target
+            const int t = lane & (T - 1);
+            const int sub = lane / T;
+            const bool tvalid = t < nt;
+            real tx = 0, ty = 0, tz = 0, te = 0;
+            if (tvalid) {
+                const uint32_t p = a.targets[tlo + t];
+                const double px = a.x[p], py = a.y[p], pz = a.z[p];
+                if (FP64) {
+                    tx = (real)px; ty = (real)py; tz = (real)pz;
+                } else {
+                    tx = (real)(px - gcx); ty = (real)(py - gcy); tz = (real)(pz - gcz);
+                }
+                te = a.soft ? (real)a.soft[p] : (real)0;
+            }
+            double accx = 0, accy = 0, accz = 0;
+            int sp = 1;
+            if (lane == 0) s_stack[0] = a.root;
+            int lcount = 0;
+            bool overflow = false;
+            __syncthreads();
+
+            // Evaluate the list against every target of the chunk.
+            auto flush = [&]() {
+                __syncthreads();
+                if (FP64) {
+                    for (int j = sub; j < lcount; j += L) {
+                        const real4 s = s_src[j];
+                        const double es = (double)s_soft[j];
+                        const double dx = (double)s.x - (double)tx, dy = (double)s.y - (double)ty,
+                                     dz = (double)s.z - (double)tz;
+                        const double r2 = dx * dx + dy * dy + dz * dz;
+                        if (r2 <= 0) continue;
+                        if (COUNT && s_kind[j] && tvalid) ++n_pair;
+                        const double h = dmax_w(dmax_w((double)te, es), 1e-300);
+                        const double f = (double)s.w * spline_force_over_r_d(sqrt(r2), h);
+                        accx += dx * f;
+                        accy += dy * f;
+                        accz += dz * f;
+                    }
+                } else {
+                    float fx = 0.f, fy = 0.f, fz = 0.f;
+                    const float tef = (float)te;
+                    for (int j = sub; j < lcount; j += L) {
+                        const real4 s = s_src[j];
+                        const float es = (float)s_soft[j];
+                        const float dx = (float)s.x - (float)tx, dy = (float)s.y - (float)ty,
+                                    dz = (float)s.z - (float)tz;
+                        const float r2 = __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, dz * dz));
+                        const float h = fmaxf(tef, es);
+                        const float inv = __builtin_amdgcn_rsqf(r2);
+                        float g = inv * inv * inv;
+                        if (r2 < h * h) g = spline_inner_f(r2 * inv, h);
+                        float f = (float)s.w * g;
+                        f = (r2 > 0.f) ? f : 0.f;
+                        if (COUNT && s_kind[j] && tvalid && r2 > 0.f) ++n_pair;
+                        fx = __builtin_fmaf(dx, f, fx);
+                        fy = __builtin_fmaf(dy, f, fy);
+                        fz = __builtin_fmaf(dz, f, fz);
+                    }
+                    accx += (double)fx;
+                    accy += (double)fy;
+                    accz += (double)fz;
+                }
+                __syncthreads();
+                lcount = 0;
+            };
+
+            while (sp > 0 && !overflow) {
+                const int k = min(sp, 64);
+                const int base = sp - k;
+                const int node = (lane < k) ? s_stack[base + lane] : -1;
+                sp = base;
+                __syncthreads();
+                bool accept = false, open_leaf = false;
+                int nch = 0, first = -1, plo = 0, phi = 0;
+                WNode w;
+                if (node >= 0) {
+                    w = a.wn[node];
+                    const double dx = dmax_w(0.0, dmax_w(gi.lo[0] - w.cx, w.cx - gi.hi[0]));
+                    const double dy = dmax_w(0.0, dmax_w(gi.lo[1] - w.cy, w.cy - gi.hi[1]));
+                    const double dz = dmax_w(0.0, dmax_w(gi.lo[2] - w.cz, w.cz - gi.hi[2]));
+                    const double rmin2 = dx * dx + dy * dy + dz * dz;
+                    accept = !open_node_d(rmin2, (double)w.len, w.s, w.mass, (double)w.soft,
+                                          gi.emax, gi.amin, gi.theta_sq, dx, dy, dz);
+                    first = w.first;
+                    if (a.leaf_always_open && first < 0) accept = false; // tree.cc:554-570
+                    if (!accept) {
+                        if (first < 0) {
+                            open_leaf = true;
+                            plo = w.plo;
+                            phi = w.phi;
+                        } else {
+                            nch = w.nchild;
+                        }
+                    }
+                }
+                if (COUNT && c0 == 0) n_visit += (lane == 0) ? k : 0;
+
+                // children of opened interior nodes go back on the stack
+                int ctot;
+                const int coff = wave_excl_scan(nch, ctot);
+                if (sp + ctot > kWalkStack) {
+                    overflow = true;
+                    break;
+                }
+                for (int c = 0; c < nch; ++c) s_stack[sp + coff + c] = first + c;
+                sp += ctot;
+
+                // accepted nodes become monopole entries
+                const uint64_t mb = __ballot(accept);
+                const int mcount = __popcll(mb);
+                if (mcount > 0) {
+                    if (lcount + mcount > kListCap) flush();
+                    if (accept) {
+                        const int pos = lcount + __popcll(mb & lanemask_lt());
+                        real4 e;
+                        if (FP64) {
+                            e.x = (real)w.cx; e.y = (real)w.cy; e.z = (real)w.cz;
+                        } else {
+                            e.x = (real)(w.cx - gcx); e.y = (real)(w.cy - gcy);
+                            e.z = (real)(w.cz - gcz);
+                        }
+                        e.w = (real)w.mass;
+                        s_src[pos] = e;
+                        s_soft[pos] = (real)(double)w.soft;
+                        if (COUNT) s_kind[pos] = 0;
+                    }
+                    if (COUNT && tvalid && sub == 0) n_multi += mcount;
+                    lcount += mcount;
+                }
+
+                // particles of opened leaves become pair entries
+                const uint64_t lb = __ballot(open_leaf);
+                if (lb) {
+                    const int nleaf = __popcll(lb);
+                    int ptot;
+                    const int sz = open_leaf ? (phi - plo) : 0;
+                    const int poff = wave_excl_scan(sz, ptot);
+                    if (open_leaf) {
+                        const int li = __popcll(lb & lanemask_lt());
+                        s_lpre[li] = poff;
+                        s_lplo[li] = plo;
+                    }
+                    if (lane == 0) s_lpre[nleaf] = ptot;
+                    __syncthreads();
+                    for (int done = 0; done < ptot;) {
+                        if (lcount == kListCap) flush();
+                        const int ncopy = min(kListCap - lcount, ptot - done);
+                        for (int j = lane; j < ncopy; j += 64) {
+                            const int idx = done + j;
+                            int lo_ = 0, hi_ = nleaf; // last li with s_lpre[li] <= idx
+                            while (hi_ - lo_ > 1) {
+                                const int mid = (lo_ + hi_) >> 1;
+                                if (s_lpre[mid] <= idx) lo_ = mid;
+                                else hi_ = mid;
+                            }
+                            const int q = s_lplo[lo_] + (idx - s_lpre[lo_]);
+                            const double4 sp4 = a.src[q];
+                            real4 e;
+                            if (FP64) {
+                                e.x = (real)sp4.x; e.y = (real)sp4.y; e.z = (real)sp4.z;
+                            } else {
+                                e.x = (real)(sp4.x - gcx); e.y = (real)(sp4.y - gcy);
+                                e.z = (real)(sp4.z - gcz);
+                            }
+                            e.w = (real)sp4.w;
+                            s_src[lcount + j] = e;
+                            s_soft[lcount + j] = (real)a.src_soft[q];
+                            if (COUNT) s_kind[lcount + j] = 1;
+                        }
+                        lcount += ncopy;
+                        done += ncopy;
+                    }
+                }
+                __syncthreads();
+            }
+            if (overflow) {
+                if (lane == 0) atomicOr(a.status, 1);
+                __syncthreads();
+                continue;
+            }
+            if (lcount > 0) flush();
+
+            // combine the lanes of each target, then write (tree.cc:880)
+            for (int d = T; d < 64; d <<= 1) {
+                accx += __shfl_xor(accx, d);
+                accy += __shfl_xor(accy, d);
+                accz += __shfl_xor(accz, d);
+            }
+            if (tvalid && sub == 0) {
+                a.ax[tlo + t] = a.G * accx;
+                a.ay[tlo + t] = a.G * accy;
+                a.az[tlo + t] = a.G * accz;
+            }
+        }
+    }
+    if (COUNT) {
+        // wave totals, one atomic per wave
+        for (int d = 32; d >= 1; d >>= 1) {
+            n_visit += __shfl_xor(n_visit, d);
+            n_multi += __shfl_xor(n_multi, d);
+            n_pair += __shfl_xor(n_pair, d);
+        }
+        if (lane == 0) {
+            atomicAdd(reinterpret_cast<unsigned long long*>(&a.counts[0]),
+                      (unsigned long long)n_visit);
+            atomicAdd(reinterpret_cast<unsigned long long*>(&a.counts[1]),
+                      (unsigned long long)n_multi);
+            atomicAdd(reinterpret_cast<unsigned long long*>(&a.counts[2]),
+                      (unsigned long long)n_pair);
+        }
+    }
+}
+
+// ---- direct sum (tree.cc:946-971; the reference's device form is grav_cuda.cu:65-141) -----------
+// One target per thread, sources staged through LDS 256 at a time in index order, so each target
+// sums its sources in exactly the reference's order with the reference's FP64 expressions.
+constexpr int kDsThreads = 256;
+
+__global__ __launch_bounds__(kDsThreads) void k_direct_fp64(
+    const double* __restrict__ x, const double* __restrict__ y, const double* __restrict__ z,
+    const double* __restrict__ m, const double* __restrict__ soft, size_t n,
+    const uint32_t* __restrict__ targets, size_t nt, double G, double* __restrict__ ax,
+    double* __restrict__ ay, double* __restrict__ az) {
+    __shared__ double sx[kDsThreads], sy[kDsThreads], sz[kDsThreads], sm[kDsThreads],
+        ss[kDsThreads];
+    const size_t t = (size_t)blockIdx.x * kDsThreads + threadIdx.x;
+    const bool live = t < nt;
+    uint32_t p = 0;
+    double px = 0, py = 0, pz = 0, ep = 0;
+    bool bad = false;
+    if (live) {
+        p = targets[t];
+        if (p >= n) { // an index outside the particle set gets NaN, never a stray read
+            bad = true;
+            p = 0;
+        }
+        px = x[p]; py = y[p]; pz = z[p];
+        ep = soft ? soft[p] : 0.0;
+    }
+    double ax_ = 0, ay_ = 0, az_ = 0;
+    for (size_t base = 0; base < n; base += kDsThreads) {
+        const size_t k = base + threadIdx.x;
+        if (k < n) {
+            sx[threadIdx.x] = x[k]; sy[threadIdx.x] = y[k]; sz[threadIdx.x] = z[k];
+            sm[threadIdx.x] = m[k]; ss[threadIdx.x] = soft ? soft[k] : 0.0;
+        }
+        __syncthreads();
+        const int ntile = (int)min((size_t)kDsThreads, n - base);
+        if (live) {
+            for (int j = 0; j < ntile; ++j) {
+                if (base + (size_t)j == (size_t)p) continue;
+                const double dx = sx[j] - px, dy = sy[j] - py, dz = sz[j] - pz;
+                const double r = sqrt(dx * dx + dy * dy + dz * dz);
+                if (r <= 0) continue;
+                const double fac =
+                    sm[j] * spline_force_over_r_d(r, dmax_w(dmax_w(ep, ss[j]), 1e-300));
+                ax_ += dx * fac;
+                ay_ += dy * fac;
+                az_ += dz * fac;
+            }
+        }
+        __syncthreads();
+    }
+    if (live) {
+        const double nan = __builtin_nan("");
+        ax[t] = bad ? nan : G * ax_;
+        ay[t] = bad ? nan : G * ay_;
+        az[t] = bad ? nan : G * az_;
+    }
+}
+
+// ---- host launchers ----------------------------------------------------------------------------
+gz_status launch_walk(const WalkArgs& a, bool fp64, bool count, hipStream_t st) {
+    if (a.ngroups <= 0) return GZ_OK;
+    unsigned grid;
+    if (a.persistent) {
+        int dev = 0, ncu = 0;
+        GZ_HIP(hipGetDevice(&dev));
+        GZ_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+        grid = (unsigned)std::min<long long>((long long)ncu * 16, a.ngroups);
+    } else {
+        grid = (unsigned)a.ngroups;
+    }
+    if (fp64) {
+        if (count) k_walk<true, true><<<grid, 64, 0, st>>>(a);
+        else k_walk<true, false><<<grid, 64, 0, st>>>(a);
+    } else {
+        if (count) k_walk<false, true><<<grid, 64, 0, st>>>(a);
+        else k_walk<false, false><<<grid, 64, 0, st>>>(a);
+    }
+    GZ_HIP(hipGetLastError());
+    return GZ_OK;
+}
+
+gz_status launch_group_prep(const uint32_t* targets, size_t nt, const int* leaf_of,
+                            const int* parent, const WNode* wn, int batch, int group_size,
+                            size_t n, int* gkey, uint32_t* flag, uint32_t* gid,
+                            uint32_t* ngroups_dev, int* status, void* scan_tmp, hipStream_t st) {
+    const unsigned grid = (unsigned)((nt + 255) / 256);
+    k_group_flags<<<grid, 256, 0, st>>>(targets, nt, leaf_of, parent, wn, batch, group_size, n,
+                                        gkey, status);
+    k_group_flags2<<<grid, 256, 0, st>>>(nt, batch, gkey, flag);
+    GZ_HIP(hipGetLastError());
+    GZ_TRY(exclusive_scan_u32(flag, gid, nt, ngroups_dev, scan_tmp, st));
+    return GZ_OK;
+}
+
+gz_status launch_group_starts(size_t nt, const uint32_t* flag, const uint32_t* gid, int* gstart,
+                              hipStream_t st) {
+    k_group_starts<<<(unsigned)((nt + 255) / 256), 256, 0, st>>>(nt, flag, gid, gstart);
+    GZ_HIP(hipGetLastError());
+    return GZ_OK;
+}
+
+gz_status launch_group_bounds(int ngroups, size_t nt, const int* gstart, const uint32_t* targets,
+                              const gz_particles* p, const double* aold, const float* group_theta,
+                              double theta_sq, GroupInfo* gi, hipStream_t st) {
+    if (ngroups <= 0) return GZ_OK;
+    k_group_bounds<<<(unsigned)((ngroups + 255) / 256), 256, 0, st>>>(
+        ngroups, nt, gstart, targets, p->x, p->y, p->z, p->soft, aold, group_theta, theta_sq, gi);
+    GZ_HIP(hipGetLastError());
+    return GZ_OK;
+}
+
+gz_status launch_direct(const gz_particles* p, const uint32_t* targets, size_t nt, double G,
+                        double* ax, double* ay, double* az, hipStream_t st) {
+    if (nt == 0) return GZ_OK;
+    k_direct_fp64<<<(unsigned)((nt + kDsThreads - 1) / kDsThreads), kDsThreads, 0, st>>>(
+        p->x, p->y, p->z, p->m, p->soft, p->n, targets, nt, G, ax, ay, az);
+    GZ_HIP(hipGetLastError());
+    return GZ_OK;
+}
+
+} // namespace gz

@@ -1,0 +1,273 @@
+"""Host mirror of the shared-memory GIZMO tree call surface (shmem/tree.h:384-451) over the device
+library. Device memory and streams come from PyTorch; every computation runs in libgzgrav.so.
+
+    P = Particles(x, y, z, m, soft)          # torch tensors on a HIP device, float64
+    T = build(P)                             # shmem::build            (tree.h:384-385)
+    ax, ay, az = accel_grouped(T, P, targets, theta, G, batch=0, aold=None)
+                                             # shmem::accel_grouped    (tree.h:428-440)
+    ax, ay, az = accel(T, P, targets, theta, G, aold=None)
+                                             # shmem::accel            (tree.h:392-394)
+    ax, ay, az = accel_brute(P, targets, G)  # shmem::accel_brute      (tree.h:450-451)
+
+As in the reference, accelerations include G, ``aold`` is per target slot in no-G units, and the
+softening is the cubic-spline kernel extent. Errors raise ``GzError`` (the reference has no error
+channel; its CUDA path exits the process, shmem/grav_cuda.cu:239-243).
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass, field
+
+import numpy as np
+import torch
+
+from . import _lib as L
+
+
+def _ptr(t):
+    return None if t is None else C.c_void_p(t.data_ptr())
+
+
+def _dev_f64(a, device):
+    if isinstance(a, torch.Tensor):
+        return a.to(device=device, dtype=torch.float64).contiguous()
+    return torch.as_tensor(np.ascontiguousarray(a, dtype=np.float64), device=device)
+
+
+def _dev_u32(a, device):
+    """Particle indices as a contiguous uint32 device tensor."""
+    if isinstance(a, torch.Tensor):
+        if a.dtype == torch.uint32:
+            return a.to(device).contiguous()
+        if a.dtype != torch.int32:
+            a = a.to(torch.int64).to(torch.int32)
+        return a.to(device).contiguous().view(torch.uint32)
+    arr = np.ascontiguousarray(np.asarray(a, dtype=np.int64).astype(np.uint32).view(np.int32))
+    return torch.as_tensor(arr, device=device).view(torch.uint32)
+
+
+class Context:
+    """One gz_ctx per device, ordered on that device's current torch stream at creation."""
+
+    _by_device: dict = {}
+
+    def __init__(self, device: int = 0):
+        lib = L.load()
+        self.device = device
+        stream = torch.cuda.current_stream(device).cuda_stream
+        h = C.c_void_p()
+        L.check(lib.gz_ctx_create(device, C.c_void_p(stream), C.byref(h)), "gz_ctx_create")
+        self.handle = h
+
+    @classmethod
+    def get(cls, device: int = 0) -> "Context":
+        ctx = cls._by_device.get(device)
+        if ctx is None:
+            ctx = cls._by_device[device] = Context(device)
+        return ctx
+
+    def synchronize(self):
+        L.check(L.load().gz_ctx_synchronize(self.handle), "gz_ctx_synchronize")
+
+
+@dataclass
+class Particles:
+    """SoA particle set on the device (shmem::Particles, tree.h:36-50)."""
+
+    x: torch.Tensor
+    y: torch.Tensor
+    z: torch.Tensor
+    m: torch.Tensor
+    soft: torch.Tensor | None = None
+    device: str | torch.device = "cuda:0"
+    _c: L.gz_particles = field(default=None, repr=False)
+
+    def __post_init__(self):
+        dev = torch.device(self.device)
+        self.x, self.y, self.z, self.m = (_dev_f64(v, dev) for v in (self.x, self.y, self.z,
+                                                                        self.m))
+        if self.soft is not None:
+            self.soft = _dev_f64(self.soft, dev)
+        self.device = dev
+        self._c = L.gz_particles(self.n, self.x.data_ptr(), self.y.data_ptr(), self.z.data_ptr(),
+                                 self.m.data_ptr(),
+                                 None if self.soft is None else self.soft.data_ptr(), None, None)
+
+    @property
+    def n(self) -> int:
+        return int(self.m.shape[0])
+
+    @classmethod
+    def from_numpy(cls, x, y, z, m, soft=None, device="cuda:0"):
+        return cls(x, y, z, m, soft, device=device)
+
+    def cstruct(self):
+        return C.byref(self._c)
+
+
+class Tree:
+    """A device-resident tree (gz_tree). Node ids are level-major; see download()."""
+
+    def __init__(self, handle, ctx: Context, particles: Particles):
+        self.handle = handle
+        self.ctx = ctx
+        self.P = particles
+        info = L.gz_tree_info()
+        L.check(L.load().gz_tree_get_info(handle, C.byref(info)), "gz_tree_get_info")
+        self.info = info
+
+    def __del__(self):
+        h = getattr(self, "handle", None)
+        if h is not None and L._lib is not None:
+            L._lib.gz_tree_free(h)
+            self.handle = None
+
+    @property
+    def nnodes(self) -> int:
+        return int(self.info.nnodes)
+
+    @property
+    def n_live(self) -> int:
+        return int(self.info.n_live)
+
+    @property
+    def root(self) -> int:
+        return int(self.info.root)
+
+    def times(self) -> dict:
+        t = self.info.times
+        return {k: getattr(t, k) for k, _ in L.gz_build_times._fields_}
+
+    def device_orderbuf(self) -> torch.Tensor:
+        """A device copy of orderbuf: every live particle in Morton order."""
+        return order_targets(self, None)
+
+    def download(self) -> dict:
+        """Host copies of the node and particle arrays (numpy)."""
+        lib = L.load()
+        nn, n, nl = self.nnodes, self.P.n, self.n_live
+        ints = {k: np.zeros(nn, dtype=np.int32) for k in
+                ("first", "next", "nchild", "plo", "phi", "parent", "level")}
+        dbls = {k: np.zeros(nn, dtype=np.float64) for k in
+                ("size", "mass", "cx", "cy", "cz", "soft", "delta", "gcx", "gcy", "gcz", "wn_s")}
+        flts = {k: np.zeros(nn, dtype=np.float32) for k in ("wn_soft", "wn_len")}
+        ip = lambda a: a.ctypes.data_as(L._ip)  # noqa: E731
+        dp = lambda a: a.ctypes.data_as(L._dp)  # noqa: E731
+        fp = lambda a: a.ctypes.data_as(L._fp)  # noqa: E731
+        L.check(lib.gz_tree_download_nodes(
+            self.handle, *[ip(ints[k]) for k in ("first", "next", "nchild", "plo", "phi",
+                                                  "parent", "level")],
+            *[dp(dbls[k]) for k in ("size", "mass", "cx", "cy", "cz", "soft", "delta", "gcx",
+                                    "gcy", "gcz", "wn_s")],
+            fp(flts["wn_soft"]), fp(flts["wn_len"])), "gz_tree_download_nodes")
+        orderbuf = np.zeros(nl, dtype=np.uint32)
+        rank = np.zeros(n, dtype=np.uint32)
+        leaf_of = np.zeros(n, dtype=np.int32)
+        skey = np.zeros(nl, dtype=np.uint64)
+        L.check(lib.gz_tree_download_particles(
+            self.handle, orderbuf.ctypes.data_as(L._up), rank.ctypes.data_as(L._up),
+            ip(leaf_of), skey.ctypes.data_as(L._u64p)), "gz_tree_download_particles")
+        out = dict(nnodes=nn, root=self.root, n_live=nl, orderbuf=orderbuf, rank=rank,
+                   leaf_of=leaf_of, skey=skey)
+        out.update(ints)
+        out.update(dbls)
+        out.update(flts)
+        return out
+
+    def update_sources(self, P: Particles | None = None):
+        """Re-read source positions/masses/softenings into the tree's sorted copy."""
+        P = P or self.P
+        L.check(L.load().gz_tree_update_sources(self.ctx.handle, self.handle, P.cstruct()),
+                "gz_tree_update_sources")
+
+
+def build(P: Particles, randomize_seed: int = -1, shift=(0.0, 0.0, 0.0), max_nodes: int = 0,
+          ctx: Context | None = None) -> Tree:
+    """shmem::build (tree.h:384-385, tree.cc:350-526)."""
+    ctx = ctx or Context.get(P.device.index or 0)
+    opts = L.gz_build_opts(int(randomize_seed), (C.c_double * 3)(*shift), int(max_nodes))
+    h = C.c_void_p()
+    L.check(L.load().gz_tree_build(ctx.handle, P.cstruct(), C.byref(opts), C.byref(h)),
+            "gz_tree_build")
+    return Tree(h, ctx, P)
+
+
+def _outputs(n, device):
+    return tuple(torch.empty(n, dtype=torch.float64, device=device) for _ in range(3))
+
+
+def accel_grouped(T: Tree, P: Particles, targets, theta: float, G: float = 1.0, batch: int = 0,
+                  aold=None, *, precision: str = "fp64", group_size: int = 0,
+                  persistent: bool = False, counts: bool = False, out=None):
+    """shmem::accel_grouped (tree.h:428-440) without tidal/jerk/lazy/sink terms.
+
+    precision "fp64" is the parity walk; "fp32" evaluates interactions in FP32 on positions
+    relative to the group centre (opening decisions and the per-target sum stay FP64).
+    group_size > 0 groups targets under the highest node holding at most that many particles.
+    Returns (ax, ay, az) device tensors, plus a dict of walk counts when counts=True.
+    """
+    tg = _dev_u32(targets, P.device)
+    nt = int(tg.shape[0])
+    ax, ay, az = out if out is not None else _outputs(nt, P.device)
+    ao = None if aold is None else _dev_f64(aold, P.device)
+    prec = {"fp64": L.PREC_FP64, "fp32": L.PREC_FP32_REL}[precision]
+    opts = L.gz_walk_opts(prec, L.CRIT_REFERENCE, int(group_size), int(persistent), None)
+    cnt = L.gz_walk_counts()
+    L.check(L.load().gz_tree_accel_grouped(
+        T.ctx.handle, T.handle, P.cstruct(), _ptr(tg), nt, float(theta), float(G), int(batch),
+        _ptr(ao), _ptr(ax), _ptr(ay), _ptr(az), C.byref(opts),
+        C.byref(cnt) if counts else None), "gz_tree_accel_grouped")
+    if counts:
+        c = {k: getattr(cnt, k) for k, _ in L.gz_walk_counts._fields_}
+        return ax, ay, az, c
+    return ax, ay, az
+
+
+def accel(T: Tree, P: Particles, targets, theta: float, G: float = 1.0, aold=None):
+    """shmem::accel (tree.h:392-394): per-target walk, reached leaves always summed."""
+    tg = _dev_u32(targets, P.device)
+    nt = int(tg.shape[0])
+    ax, ay, az = _outputs(nt, P.device)
+    ao = None if aold is None else _dev_f64(aold, P.device)
+    L.check(L.load().gz_tree_accel(T.ctx.handle, T.handle, P.cstruct(), _ptr(tg), nt,
+                                   float(theta), float(G), _ptr(ao), _ptr(ax), _ptr(ay),
+                                   _ptr(az)), "gz_tree_accel")
+    return ax, ay, az
+
+
+def accel_brute(P: Particles, targets, G: float = 1.0, ctx: Context | None = None):
+    """shmem::accel_brute (tree.h:450-451): exact FP64 direct sum, includes G."""
+    ctx = ctx or Context.get(P.device.index or 0)
+    tg = _dev_u32(targets, P.device)
+    nt = int(tg.shape[0])
+    ax, ay, az = _outputs(nt, P.device)
+    L.check(L.load().gz_direct_sum(ctx.handle, P.cstruct(), _ptr(tg), nt, float(G), _ptr(ax),
+                                   _ptr(ay), _ptr(az)), "gz_direct_sum")
+    ctx.synchronize()
+    return ax, ay, az
+
+
+def order_targets(T: Tree, active=None) -> torch.Tensor:
+    """Active particles in the tree's Morton order (mfm.cc:698-718). None = all active."""
+    n_out = C.c_size_t(0)
+    out = torch.empty(max(T.n_live, 1), dtype=torch.int32, device=T.P.device)
+    if active is None:
+        L.check(L.load().gz_order_targets(T.ctx.handle, T.handle, None, T.P.n, _ptr(out),
+                                          C.byref(n_out)), "gz_order_targets")
+    else:
+        act = _dev_u32(active, T.P.device)
+        L.check(L.load().gz_order_targets(T.ctx.handle, T.handle, _ptr(act), int(act.shape[0]),
+                                          _ptr(out), C.byref(n_out)), "gz_order_targets")
+    return out[: n_out.value].view(torch.uint32)
+
+
+def aold_from_accel(T: Tree, a_prev: torch.Tensor, targets, err_tol_force_acc: float,
+                    G: float = 1.0) -> torch.Tensor:
+    """aold[k] = ErrTolForceAcc * |a_prev[targets[k]]| / G (mfm.cc:743-751); a_prev is (N, 3)."""
+    tg = _dev_u32(targets, T.P.device)
+    a_prev = a_prev.to(T.P.device, torch.float64).contiguous()
+    out = torch.empty(int(tg.shape[0]), dtype=torch.float64, device=T.P.device)
+    L.check(L.load().gz_aold_from_accel(T.ctx.handle, _ptr(a_prev), _ptr(tg), int(tg.shape[0]),
+                                        float(err_tol_force_acc), float(G), _ptr(out)),
+            "gz_aold_from_accel")
+    return out

@@ -102,6 +102,12 @@ gz_status gz_tree_download_particles(const gz_tree* tree, uint32_t* orderbuf, ui
 gz_status gz_tree_device_arrays(const gz_tree* tree, const uint32_t** orderbuf,
                                 const uint32_t** rank, const int** leaf_of);
 
+/* Re-read source positions, masses and softenings from p into the tree's Morton-ordered copy,
+ * for callers that move particles between a build and a walk. The reference's walk reads source
+ * particles from the live arrays (shmem/tree.cc:787-788); this library's walk reads the copy made
+ * at build time, so a caller that drifts particles calls this before walking. */
+gz_status gz_tree_update_sources(gz_ctx* ctx, gz_tree* tree, const gz_particles* p);
+
 /* ---- walks -------------------------------------------------------------------------------- */
 typedef enum {
     GZ_PREC_FP64 = 0,    /* FP64 throughout: the parity walk                                 */
