@@ -1,0 +1,265 @@
+#!/usr/bin/env python3
+"""Force-evaluation throughput of the device tree walk at 8M particles.
+
+Workload: the 8,388,608-particle Hernquist halo (a=1, 70%) + exponential disk (R_d=0.1, z_d=0.01,
+30%), equal masses, cubic-spline softening h=0.005, all particles active, G=1. One step is one full
+force evaluation: the grouped tree walk (target grouping, group bounds and the walk kernel) over all
+8M targets in Morton order, with the reference's default opening criterion: the relative criterion
+(ErrTolForceAcc = 0.0025) with aold from a bootstrap Barnes-Hut walk at theta = 0.7. The tree is
+built once before the timed region (build time is reported separately); inputs are resident in HBM.
+
+Reported beside the headline:
+  * every walk variant's evals/s, kernel time and interactions/target (SHMEM_COUNT_WALK's count);
+  * force error of each walk against the FP64 direct sum on 65,536 random probe particles;
+  * roofline of the walk kernel: 20 flops per interaction over the kernel's HIP-event time;
+  * cpu_baseline: the reference shmem CPU tree walk (oracle/_ref) timed on this host's cores on a
+    bounded sample of whole leaves of the same tree, or the C restatement when the reference build
+    is absent.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+METRIC = ("force evals/s at 8M part., learned+FP32 vs plain GPU walk, p99 rel err ≤1e-2")
+FLOPS_PER_INTERACTION = 20.0
+PEAK_FP32_TFLOPS = 157.3
+ERR_TOL_FORCE_ACC = 0.0025
+THETA_BOOT = 0.7
+
+
+def log(*a):
+    print("[bench]", *a, file=sys.stderr, flush=True)
+
+
+def rel_err(g, o):
+    g = np.stack(g, 1)
+    o = np.stack(o, 1)
+    return np.linalg.norm(g - o, axis=1) / np.maximum(np.linalg.norm(o, axis=1), 1e-300)
+
+
+def err_stats(e):
+    return {"median": float(np.median(e)), "p99": float(np.percentile(e, 99)),
+            "max": float(np.max(e))}
+
+
+def time_walk(gz, T, P, targets, aold, steps, warmup, **kw):
+    import torch
+    out = None
+    for _ in range(warmup):
+        out = gz.accel_grouped(T, P, targets, 0.0, 1.0, 0, aold=aold, timing=True, **kw)
+    torch.cuda.synchronize()
+    walk_ms = []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        out = gz.accel_grouped(T, P, targets, 0.0, 1.0, 0, aold=aold, timing=True, **kw)
+        walk_ms.append(out[3]["walk_ms"])
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / steps
+    return out, dt, float(np.mean(walk_ms))
+
+
+def cpu_baseline(x, y, z, m, soft, orderbuf, leaf_of, aold_all, seconds, threads):
+    """The reference walk (accel_grouped, batch 0, relative criterion) on whole leaves sampled
+    uniformly along the Morton order, timed on `threads` host cores."""
+    from oracle.bindings import HostParticles, Oracle, Reference, have_reference
+    hp = HostParticles(x, y, z, m, soft)
+    leaf_seq = leaf_of[orderbuf]
+    run_start = np.concatenate([[True], leaf_seq[1:] != leaf_seq[:-1]])
+    run_id = np.cumsum(run_start) - 1
+    n_runs = int(run_id[-1]) + 1
+
+    def sample(stride):
+        keep = (run_id % stride) == 0
+        return orderbuf[keep].astype(np.uint32), aold_all[keep]
+
+    if have_reference():
+        ref = Reference()
+        ref.set_threads(threads)
+        rp = ref.particles(hp)
+        t0 = time.perf_counter()
+        tr = ref.build(rp)
+        build_s = time.perf_counter() - t0
+
+        def walk(tg, ao):
+            return ref.accel_grouped(tr, rp, tg, 0.0, 1.0, 0, ao)
+        kind = "reference"
+        cleanup = lambda: (ref.free(tr), ref.free_particles(rp))  # noqa: E731
+    else:
+        orc = Oracle()
+        t0 = time.perf_counter()
+        to = orc.build(hp)
+        build_s = time.perf_counter() - t0
+
+        def walk(tg, ao):
+            return orc.accel_grouped(to, hp, tg, 0.0, 1.0, 0, ao)
+        kind = "port"
+        cleanup = lambda: orc.free(to)  # noqa: E731
+    try:
+        stride = 512
+        tg, ao = sample(stride)
+        t0 = time.perf_counter()
+        walk(tg, ao)
+        per = (time.perf_counter() - t0) / max(len(tg), 1)
+        want = seconds / max(per, 1e-12)
+        stride = int(max(1, min(n_runs, round(len(orderbuf) / max(want, 1.0)))))
+        tg, ao = sample(stride)
+        t0 = time.perf_counter()
+        walk(tg, ao)
+        dt = time.perf_counter() - t0
+    finally:
+        cleanup()
+    return {"value": len(tg) / dt, "unit": "force evals/s", "cores": threads, "kind": kind,
+            "sample": f"{len(tg)} targets = every {stride}th leaf of the {n_runs} leaves in Morton "
+                      f"order, relative criterion ErrTolForceAcc={ERR_TOL_FORCE_ACC}, aold from the "
+                      f"theta={THETA_BOOT} bootstrap; {dt:.2f} s walk; tree build {build_s:.2f} s",
+            "build_s": build_s}
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--n", type=int, default=8388608)
+    ap.add_argument("--probes", type=int, default=65536)
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--group-sizes", default="0,16,32,64")
+    ap.add_argument("--copy-mb", type=int, default=0,
+                    help="after the walks, run one plain device copy of this many MiB (a known "
+                         "traffic reference for hardware byte counters)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the walk shards "
+              f"by target groups; no multi-GPU figure is produced)", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+
+    from gizmo_shared_memory_amd import ics
+    from gizmo_shared_memory_amd import tree as gz
+
+    assert torch.cuda.is_available(), "bench.py needs a GPU"
+    t0 = time.perf_counter()
+    x, y, z, m, soft = ics.halo_disk(n=args.n)
+    log(f"generated halo+disk n={args.n} in {time.perf_counter() - t0:.1f} s")
+    P = gz.Particles(x, y, z, m, soft, device="cuda:0")
+    T = gz.build(P)
+    T = gz.build(P)  # second build is the timed one (first touches the allocator)
+    build_ms = T.times()["total"]
+    log(f"tree: nodes={T.nnodes} leaves={T.info.nleaves} depth={T.info.depth} "
+        f"build={build_ms:.1f} ms {T.times()}")
+    targets = T.device_orderbuf()
+    nt = int(targets.shape[0])
+
+    # bootstrap walk (reference: theta 0.7, no aold), then aold = ErrTolForceAcc |a| / G
+    boot = gz.accel_grouped(T, P, targets, THETA_BOOT, 1.0, 0, counts=True)
+    a_boot = torch.stack(boot[:3], 1)
+    aold = ERR_TOL_FORCE_ACC * torch.linalg.norm(a_boot, dim=1)
+    log(f"bootstrap fp64 walk: {boot[3]}")
+
+    # probes and the FP64 direct-sum judge
+    probes = ics.probes(nt, args.probes)
+    probe_t = targets.view(torch.int32)[torch.as_tensor(probes.astype(np.int64),
+                                                        device="cuda:0")].view(torch.uint32)
+    t0 = time.perf_counter()
+    ex = gz.accel_brute(P, probe_t, 1.0)
+    ex = [v.cpu().numpy() for v in ex]
+    log(f"direct sum on {len(probes)} probes: {time.perf_counter() - t0:.2f} s")
+
+    results = {}
+    # plain GPU walks with the reference criterion
+    for prec in ("fp64", "fp32"):
+        sizes = [0] if prec == "fp64" else [int(s) for s in args.group_sizes.split(",")]
+        best = None
+        for gs in sizes:
+            (ax, ay, az, c), dt, kms = time_walk(gz, T, P, targets, aold, 1, 1,
+                                                 precision=prec, group_size=gs)
+            log(f"tune {prec} group_size={gs}: {dt * 1e3:.2f} ms/step, kernel {kms:.2f} ms")
+            if best is None or dt < best[1]:
+                best = (gs, dt)
+        gs = best[0]
+        (ax, ay, az, c), dt, kms = time_walk(gz, T, P, targets, aold, args.steps, args.warmup,
+                                             precision=prec, group_size=gs)
+        cnt = gz.accel_grouped(T, P, targets, 0.0, 1.0, 0, aold=aold, precision=prec,
+                               group_size=gs, counts=True)[3]
+        inter = cnt["multipole"] + cnt["pairs"]
+        e = rel_err(tuple(v.cpu().numpy()[probes] for v in (ax, ay, az)), ex)
+        results[prec] = {
+            "group_size": gs, "ms_per_step": dt * 1e3, "kernel_ms": kms,
+            "evals_per_s": nt / dt, "interactions_per_target": inter / nt,
+            "nodes_visited_per_target": cnt["nodes_visited"] / nt, "groups": cnt["groups"],
+            "err": err_stats(e),
+            "achieved_tflops": FLOPS_PER_INTERACTION * inter / (kms * 1e-3) / 1e12,
+        }
+        log(f"{prec}: {results[prec]}")
+
+    if args.copy_mb > 0:
+        src = torch.ones(args.copy_mb << 18, dtype=torch.float32, device="cuda:0")
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        log(f"copy calibration: {args.copy_mb} MiB read + {args.copy_mb} MiB written")
+        del src, dst
+
+    head = results["fp32"]
+    ref_e = rel_err(tuple(v.cpu().numpy()[probes] for v in boot[:3]), ex)
+    cpu = None
+    if not args.no_cpu:
+        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+        td = T.download()
+        # targets are orderbuf, so aold is already indexed by Morton slot
+        cpu = cpu_baseline(x, y, z, m, soft, td["orderbuf"], td["leaf_of"], aold.cpu().numpy(),
+                           args.cpu_seconds, threads)
+        log(f"cpu baseline: {cpu}")
+
+    line = {
+        "metric": METRIC,
+        "value": head["evals_per_s"],
+        "unit": "force evals/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": head["ms_per_step"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": f"halo+disk N={args.n} (Hernquist a=1 70% + exp disk Rd=0.1 zd=0.01 30%),"
+                        f" h=0.005, all active",
+            "walk": f"FP32 group-relative, reference criterion (relative {ERR_TOL_FORCE_ACC}, "
+                    f"aold from theta={THETA_BOOT} bootstrap), group_size={head['group_size']}",
+            "learned_criterion": None,
+            "n_probes": len(probes),
+        },
+        "roofline": {"bound": "mfma", "achieved": head["achieved_tflops"],
+                     "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
+                     "frac": head["achieved_tflops"] / PEAK_FP32_TFLOPS, "traffic": None},
+        "cpu_baseline": None if cpu is None else {k: cpu[k] for k in
+                                                  ("value", "unit", "cores", "kind", "sample")},
+        "walks": results,
+        "bootstrap_fp64": {"err": err_stats(ref_e), **{k: boot[3][k] for k in
+                                                      ("nodes_visited", "multipole", "pairs")}},
+        "build_ms": build_ms,
+    }
+    if cpu is not None:
+        line["gpu_over_cpu"] = head["evals_per_s"] / cpu["value"]
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -60,7 +60,7 @@ class gz_tree_info(C.Structure):
 
 class gz_walk_opts(C.Structure):
     _fields_ = [("precision", C.c_int), ("criterion", C.c_int), ("group_size", C.c_int),
-                ("persistent", C.c_int), ("model", C.c_void_p)]
+                ("persistent", C.c_int), ("count", C.c_int), ("model", C.c_void_p)]
 
 
 class gz_walk_counts(C.Structure):

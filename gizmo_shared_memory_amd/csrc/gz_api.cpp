@@ -50,7 +50,8 @@ struct Carver {
 gz_status walk(gz_ctx* ctx, const gz_tree* tree, const gz_particles* p, const uint32_t* targets,
                size_t n_t, double theta, double G, int batch, int group_size, const double* aold,
                const float* group_theta, int precision, int persistent, int leaf_always_open,
-               double* ax, double* ay, double* az, gz_walk_counts* counts, const char* who) {
+               int count, double* ax, double* ay, double* az, gz_walk_counts* counts,
+               const char* who) {
     GZ_TRY(check_ctx(ctx));
     if (!tree) return fail(GZ_ERR_INVALID, "%s: null tree", who);
     GZ_TRY(check_particles(p, who));
@@ -128,7 +129,7 @@ gz_status walk(gz_ctx* ctx, const gz_tree* tree, const gz_particles* p, const ui
     a.counts = misc;
     a.G = G;
     a.ax = ax; a.ay = ay; a.az = az;
-    GZ_TRY(gz::launch_walk(a, precision == GZ_PREC_FP64, counts != nullptr, st));
+    GZ_TRY(gz::launch_walk(a, precision == GZ_PREC_FP64, count && counts != nullptr, st));
     GZ_HIP(hipEventRecord(ev[10], st));
 
     long long host_misc[8] = {0};
@@ -354,12 +355,13 @@ gz_status gz_tree_accel_grouped(gz_ctx* ctx, const gz_tree* tree, const gz_parti
                                 int batch, const double* aold, double* ax, double* ay, double* az,
                                 const gz_walk_opts* opts, gz_walk_counts* counts) {
     gz::clear_error();
-    int precision = GZ_PREC_FP64, group_size = 0, persistent = 0;
+    int precision = GZ_PREC_FP64, group_size = 0, persistent = 0, count = 0;
     const float* group_theta = nullptr;
     if (opts) {
         precision = opts->precision;
         group_size = opts->group_size;
         persistent = opts->persistent;
+        count = opts->count;
         if (opts->criterion == GZ_CRIT_LEARNED)
             return fail(GZ_ERR_UNSUPPORTED,
                         "gz_tree_accel_grouped: use gz_tree_accel_learned for the learned criterion");
@@ -370,7 +372,7 @@ gz_status gz_tree_accel_grouped(gz_ctx* ctx, const gz_tree* tree, const gz_parti
     if (batch > 0 && group_size > 0)
         return fail(GZ_ERR_INVALID, "gz_tree_accel_grouped: batch > 0 and group_size > 0");
     return walk(ctx, tree, p, targets, n_t, theta, G, batch, group_size, aold, group_theta,
-                precision, persistent, 0, ax, ay, az, counts, "gz_tree_accel_grouped");
+                precision, persistent, 0, count, ax, ay, az, counts, "gz_tree_accel_grouped");
 }
 
 gz_status gz_tree_accel(gz_ctx* ctx, const gz_tree* tree, const gz_particles* p,
@@ -379,8 +381,8 @@ gz_status gz_tree_accel(gz_ctx* ctx, const gz_tree* tree, const gz_particles* p,
     gz::clear_error();
     // one group per target: the group box is the target's position, so the opening distance is
     // |COM - x| per axis exactly as shmem::accel computes it (tree.cc:552-557)
-    return walk(ctx, tree, p, targets, n_t, theta, G, 1, 0, aold, nullptr, GZ_PREC_FP64, 0, 1, ax,
-                ay, az, nullptr, "gz_tree_accel");
+    return walk(ctx, tree, p, targets, n_t, theta, G, 1, 0, aold, nullptr, GZ_PREC_FP64, 0, 1, 0,
+                ax, ay, az, nullptr, "gz_tree_accel");
 }
 
 gz_status gz_direct_sum(gz_ctx* ctx, const gz_particles* p, const uint32_t* targets, size_t n_t,

@@ -198,26 +198,31 @@ def _outputs(n, device):
 
 def accel_grouped(T: Tree, P: Particles, targets, theta: float, G: float = 1.0, batch: int = 0,
                   aold=None, *, precision: str = "fp64", group_size: int = 0,
-                  persistent: bool = False, counts: bool = False, out=None):
+                  persistent: bool = False, counts: bool = False, timing: bool = False,
+                  out=None):
     """shmem::accel_grouped (tree.h:428-440) without tidal/jerk/lazy/sink terms.
 
     precision "fp64" is the parity walk; "fp32" evaluates interactions in FP32 on positions
     relative to the group centre (opening decisions and the per-target sum stay FP64).
     group_size > 0 groups targets under the highest node holding at most that many particles.
-    Returns (ax, ay, az) device tensors, plus a dict of walk counts when counts=True.
+    Returns (ax, ay, az) device tensors, plus a dict of walk counts when counts=True (node
+    visits and interactions, SHMEM_COUNT_WALK's definition) or timing=True (walk and prep ms only,
+    from the kernel variant without counters).
     """
     tg = _dev_u32(targets, P.device)
     nt = int(tg.shape[0])
     ax, ay, az = out if out is not None else _outputs(nt, P.device)
     ao = None if aold is None else _dev_f64(aold, P.device)
     prec = {"fp64": L.PREC_FP64, "fp32": L.PREC_FP32_REL}[precision]
-    opts = L.gz_walk_opts(prec, L.CRIT_REFERENCE, int(group_size), int(persistent), None)
+    opts = L.gz_walk_opts(prec, L.CRIT_REFERENCE, int(group_size), int(persistent), int(counts),
+                          None)
     cnt = L.gz_walk_counts()
+    want = counts or timing
     L.check(L.load().gz_tree_accel_grouped(
         T.ctx.handle, T.handle, P.cstruct(), _ptr(tg), nt, float(theta), float(G), int(batch),
         _ptr(ao), _ptr(ax), _ptr(ay), _ptr(az), C.byref(opts),
-        C.byref(cnt) if counts else None), "gz_tree_accel_grouped")
-    if counts:
+        C.byref(cnt) if want else None), "gz_tree_accel_grouped")
+    if want:
         c = {k: getattr(cnt, k) for k, _ in L.gz_walk_counts._fields_}
         return ax, ay, az, c
     return ax, ay, az

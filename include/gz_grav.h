@@ -126,6 +126,8 @@ typedef struct {
     int group_size;  /* batch <= 0 grouping: 0 = one group per leaf (the reference); G > 0 =
                         targets under the highest node holding <= G particles                  */
     int persistent;  /* 1 = persistent work-queue launch; 0 = one workgroup per work item      */
+    int count;       /* 1 = count node visits and interactions into gz_walk_counts (a slower
+                        kernel variant); timings are reported whenever counts is non-NULL     */
     const void* model; /* gz_model* when criterion == GZ_CRIT_LEARNED                          */
 } gz_walk_opts;
 
