@@ -60,13 +60,15 @@ class gz_tree_info(C.Structure):
 
 class gz_walk_opts(C.Structure):
     _fields_ = [("precision", C.c_int), ("criterion", C.c_int), ("group_size", C.c_int),
-                ("persistent", C.c_int), ("count", C.c_int), ("model", C.c_void_p)]
+                ("persistent", C.c_int), ("count", C.c_int), ("stack_limit", C.c_int),
+                ("model", C.c_void_p)]
 
 
 class gz_walk_counts(C.Structure):
     _fields_ = [("nodes_visited", C.c_longlong), ("multipole", C.c_longlong),
                 ("pairs", C.c_longlong), ("groups", C.c_longlong), ("items", C.c_longlong),
-                ("walk_ms", C.c_double), ("prep_ms", C.c_double)]
+                ("walk_ms", C.c_double), ("prep_ms", C.c_double), ("max_stack", C.c_longlong),
+                ("retries", C.c_longlong)]
 
 
 _PROTOS = {

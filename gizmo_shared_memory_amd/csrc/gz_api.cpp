@@ -50,8 +50,8 @@ struct Carver {
 gz_status walk(gz_ctx* ctx, const gz_tree* tree, const gz_particles* p, const uint32_t* targets,
                size_t n_t, double theta, double G, int batch, int group_size, const double* aold,
                const float* group_theta, int precision, int persistent, int leaf_always_open,
-               int count, double* ax, double* ay, double* az, gz_walk_counts* counts,
-               const char* who) {
+               int count, int stack_limit, int pack, double* ax, double* ay, double* az,
+               gz_walk_counts* counts, const char* who) {
     GZ_TRY(check_ctx(ctx));
     if (!tree) return fail(GZ_ERR_INVALID, "%s: null tree", who);
     GZ_TRY(check_particles(p, who));
@@ -74,7 +74,10 @@ gz_status walk(gz_ctx* ctx, const gz_tree* tree, const gz_particles* p, const ui
     Carver c;
     const size_t o_key = c.add<int>(n_t), o_flag = c.add<uint32_t>(n_t),
                  o_gid = c.add<uint32_t>(n_t), o_gstart = c.add<int>(n_t + 1),
-                 o_misc = c.add<long long>(8), o_tmp = c.add<uint8_t>(gz::scan_tmp_bytes(n_t));
+                 o_retry = c.add<int>(n_t), o_pk = c.add<int>(n_t), o_mflag = c.add<uint32_t>(n_t),
+                 o_mid = c.add<uint32_t>(n_t), o_gstart2 = c.add<int>(n_t + 1),
+                 o_misc = c.add<long long>(8),
+                 o_tmp = c.add<uint8_t>(gz::scan_tmp_bytes(n_t));
     gz::DevMem& s = ctx->scratch[5];
     GZ_TRY(s.reserve(c.used));
     uint8_t* b = s.as<uint8_t>();
@@ -86,22 +89,36 @@ gz_status walk(gz_ctx* ctx, const gz_tree* tree, const gz_particles* p, const ui
     uint32_t* ngroups_dev = reinterpret_cast<uint32_t*>(misc + 4);
     int* queue = reinterpret_cast<int*>(misc + 5);
     int* status = reinterpret_cast<int*>(misc + 6);
+    int* n_retry = reinterpret_cast<int*>(misc + 7);
+    int* retry_list = reinterpret_cast<int*>(b + o_retry);
 
     GZ_HIP(hipEventRecord(ev[8], st));
     GZ_TRY(gz::fill_zero_u32(reinterpret_cast<uint32_t*>(misc), 16, st));
     GZ_TRY(gz::launch_group_prep(targets, n_t, tree->leaf_of, tree->parent, tree->wn, batch,
                                  group_size, p->n, gkey, flag, gid, ngroups_dev, status, b + o_tmp,
                                  st));
-    uint32_t prep[4] = {0, 0, 0, 0}; // misc + 4 .. misc + 6 as 32-bit words: ngroups, _, queue, _
-    GZ_HIP(hipMemcpyAsync(prep, ngroups_dev, sizeof prep, hipMemcpyDeviceToHost, st));
+    uint32_t ngroups0 = 0;
+    GZ_HIP(hipMemcpyAsync(&ngroups0, ngroups_dev, sizeof ngroups0, hipMemcpyDeviceToHost, st));
     int prep_status = 0;
     GZ_HIP(hipMemcpyAsync(&prep_status, status, sizeof prep_status, hipMemcpyDeviceToHost, st));
     GZ_HIP(hipStreamSynchronize(st));
-    const uint32_t ngroups = prep[0];
     if (prep_status & 2)
         return fail(GZ_ERR_INVALID, "%s: a target index is outside the particle set (n=%zu)", who,
                     p->n);
     GZ_TRY(gz::launch_group_starts(n_t, flag, gid, gstart, st));
+    uint32_t ngroups = ngroups0;
+    if (group_size > 0 && pack && ngroups0 > 1) {
+        uint32_t* ng2 = reinterpret_cast<uint32_t*>(misc + 3);
+        int* gstart2 = reinterpret_cast<int*>(b + o_gstart2);
+        GZ_TRY(gz::launch_group_pack((int)ngroups0, n_t, group_size, gstart, gkey, tree->parent,
+                                     reinterpret_cast<int*>(b + o_pk),
+                                     reinterpret_cast<uint32_t*>(b + o_mflag),
+                                     reinterpret_cast<uint32_t*>(b + o_mid), ng2, gstart2,
+                                     b + o_tmp, st));
+        GZ_HIP(hipMemcpyAsync(&ngroups, ng2, sizeof ngroups, hipMemcpyDeviceToHost, st));
+        GZ_HIP(hipStreamSynchronize(st));
+        gstart = gstart2;
+    }
 
     gz::DevMem& sg = ctx->scratch[6];
     GZ_TRY(sg.reserve((size_t)ngroups * sizeof(gz::GroupInfo) + 256));
@@ -127,17 +144,43 @@ gz_status walk(gz_ctx* ctx, const gz_tree* tree, const gz_particles* p, const ui
     a.queue = queue;
     a.status = status;
     a.counts = misc;
+    a.n_retry = n_retry;
+    a.retry_list = retry_list;
+    a.gstack = nullptr;
+    a.gstack_cap = 0;
+    a.stack_limit = stack_limit;
     a.G = G;
     a.ax = ax; a.ay = ay; a.az = az;
-    GZ_TRY(gz::launch_walk(a, precision == GZ_PREC_FP64, count && counts != nullptr, st));
+    const bool fp64 = precision == GZ_PREC_FP64, cnt = count && counts != nullptr;
+    GZ_TRY(gz::launch_walk(a, fp64, cnt, st));
     GZ_HIP(hipEventRecord(ev[10], st));
 
     long long host_misc[8] = {0};
     GZ_HIP(hipMemcpyAsync(host_misc, misc, sizeof host_misc, hipMemcpyDeviceToHost, st));
     GZ_HIP(hipStreamSynchronize(st));
-    if (host_misc[6] & 1)
-        return fail(GZ_ERR_INTERNAL, "%s: a group's node stack overflowed its LDS capacity", who);
+    const int retries = (int)(host_misc[7] & 0xFFFFFFFF);
+    if (retries > 0) {
+        // groups whose node stack outgrew LDS: walk them again with global-memory stacks. A
+        // group's stack never holds more entries than the tree has nodes.
+        const int cap = std::max(1024, std::min(tree->nnodes, 1 << 22));
+        const int per_launch = std::max(1, (1 << 28) / cap); // at most 1 GiB of stacks
+        gz::DevMem& gs = ctx->scratch[4];
+        GZ_TRY(gs.reserve((size_t)std::min(per_launch, retries) * cap * sizeof(int)));
+        for (int off = 0; off < retries; off += per_launch) {
+            gz::WalkArgs r = a;
+            r.retry_list = retry_list + off;
+            r.gstack = gs.as<int>();
+            r.gstack_cap = cap;
+            GZ_TRY(gz::launch_walk_retry(r, std::min(per_launch, retries - off), fp64, cnt, st));
+        }
+        GZ_HIP(hipMemcpyAsync(host_misc, misc, sizeof host_misc, hipMemcpyDeviceToHost, st));
+        GZ_HIP(hipStreamSynchronize(st));
+        if (host_misc[6] & 1)
+            return fail(GZ_ERR_INTERNAL, "%s: a group's node stack outgrew %d entries", who, cap);
+    }
     if (counts) {
+        counts->retries = retries;
+        counts->max_stack = host_misc[3];
         counts->nodes_visited = host_misc[0];
         counts->multipole = host_misc[1];
         counts->pairs = host_misc[2];
@@ -355,13 +398,14 @@ gz_status gz_tree_accel_grouped(gz_ctx* ctx, const gz_tree* tree, const gz_parti
                                 int batch, const double* aold, double* ax, double* ay, double* az,
                                 const gz_walk_opts* opts, gz_walk_counts* counts) {
     gz::clear_error();
-    int precision = GZ_PREC_FP64, group_size = 0, persistent = 0, count = 0;
+    int precision = GZ_PREC_FP64, group_size = 0, persistent = 0, count = 0, stack_limit = 0;
     const float* group_theta = nullptr;
     if (opts) {
         precision = opts->precision;
         group_size = opts->group_size;
         persistent = opts->persistent;
         count = opts->count;
+        stack_limit = opts->stack_limit;
         if (opts->criterion == GZ_CRIT_LEARNED)
             return fail(GZ_ERR_UNSUPPORTED,
                         "gz_tree_accel_grouped: use gz_tree_accel_learned for the learned criterion");
@@ -372,7 +416,8 @@ gz_status gz_tree_accel_grouped(gz_ctx* ctx, const gz_tree* tree, const gz_parti
     if (batch > 0 && group_size > 0)
         return fail(GZ_ERR_INVALID, "gz_tree_accel_grouped: batch > 0 and group_size > 0");
     return walk(ctx, tree, p, targets, n_t, theta, G, batch, group_size, aold, group_theta,
-                precision, persistent, 0, count, ax, ay, az, counts, "gz_tree_accel_grouped");
+                precision, persistent, 0, count, stack_limit, 1, ax, ay, az, counts,
+                "gz_tree_accel_grouped");
 }
 
 gz_status gz_tree_accel(gz_ctx* ctx, const gz_tree* tree, const gz_particles* p,
@@ -382,7 +427,7 @@ gz_status gz_tree_accel(gz_ctx* ctx, const gz_tree* tree, const gz_particles* p,
     // one group per target: the group box is the target's position, so the opening distance is
     // |COM - x| per axis exactly as shmem::accel computes it (tree.cc:552-557)
     return walk(ctx, tree, p, targets, n_t, theta, G, 1, 0, aold, nullptr, GZ_PREC_FP64, 0, 1, 0,
-                ax, ay, az, nullptr, "gz_tree_accel");
+                0, 0, ax, ay, az, nullptr, "gz_tree_accel");
 }
 
 gz_status gz_direct_sum(gz_ctx* ctx, const gz_particles* p, const uint32_t* targets, size_t n_t,

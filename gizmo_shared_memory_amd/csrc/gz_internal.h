@@ -52,8 +52,13 @@ struct WalkArgs {
     int persistent;
     int leaf_always_open;   // 1: a reached leaf is always summed pair by pair (shmem::accel)
     int* queue;             // persistent work counter (zeroed before the launch)
-    int* status;            // bit 0: a group's node stack overflowed
-    long long* counts;      // nodes visited, multipole, pairs (COUNT builds only)
+    int* status;            // bit 0: a group's global node stack overflowed too
+    long long* counts;      // nodes visited, multipole, pairs, max stack depth (COUNT builds)
+    int* n_retry;           // groups whose LDS node stack overflowed ...
+    int* retry_list;        // ... and their ids, for the global-stack launch
+    int* gstack;            // global node stacks of the retry launch, gstack_cap ints per wave
+    int gstack_cap;
+    int stack_limit;        // > 0: use at most this many LDS stack entries (exercises the retry)
     double G;
     double *ax, *ay, *az;
 };
@@ -111,10 +116,16 @@ gz_status launch_group_prep(const uint32_t* targets, size_t nt, const int* leaf_
                             uint32_t* ngroups_dev, int* status, void* scan_tmp, hipStream_t st);
 gz_status launch_group_starts(size_t nt, const uint32_t* flag, const uint32_t* gid, int* gstart,
                               hipStream_t st);
+gz_status launch_group_pack(int ngroups, size_t nt, int group_size, const int* gstart,
+                            const int* gkey, const int* parent, int* pk, uint32_t* mflag,
+                            uint32_t* mid, uint32_t* ngroups2_dev, int* gstart2, void* scan_tmp,
+                            hipStream_t st);
 gz_status launch_group_bounds(int ngroups, size_t nt, const int* gstart, const uint32_t* targets,
                               const gz_particles* p, const double* aold, const float* group_theta,
                               double theta_sq, GroupInfo* gi, hipStream_t st);
 gz_status launch_walk(const WalkArgs& a, bool fp64, bool count, hipStream_t st);
+gz_status launch_walk_retry(const WalkArgs& a, int n_retry, bool fp64, bool count,
+                            hipStream_t st);
 gz_status launch_direct(const gz_particles* p, const uint32_t* targets, size_t nt, double G,
                         double* ax, double* ay, double* az, hipStream_t st);
 

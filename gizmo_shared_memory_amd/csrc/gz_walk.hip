@@ -29,8 +29,9 @@
 
 namespace gz {
 
-constexpr int kWalkStack = 2048;  // node ids per wave
-constexpr int kListCap = 512;     // interaction-list entries per wave
+constexpr int kWalkStack = 1024;  // node ids per wave in LDS
+constexpr int kListCapF = 256;    // interaction-list entries per wave, FP32 entries (20 B)
+constexpr int kListCapD = 128;    // interaction-list entries per wave, FP64 entries (40 B)
 constexpr int kMaxChunk = 64;     // targets evaluated together (one wave)
 
 __device__ __forceinline__ double dmin_w(double a, double b) { return (b < a) ? b : a; }
@@ -118,6 +119,53 @@ __global__ __launch_bounds__(256) void k_group_starts(size_t nt, const uint32_t*
     if (flag[i]) gstart[gid[i]] = (int)i;
 }
 
+// Sibling packing (group_size > 0 only): groups whose key nodes are children of the same parent
+// are merged greedily, in octant order, while the merged group holds at most group_size targets.
+// A merged group's box lies inside the parent's cell, so it stays compact, and the wavefront that
+// walks it has more of its 64 lanes busy in the evaluation.
+__global__ __launch_bounds__(256) void k_pack_parent(int ngroups, size_t nt,
+                                                     const int* __restrict__ gstart,
+                                                     const int* __restrict__ gkey,
+                                                     const int* __restrict__ parent,
+                                                     int* __restrict__ pk) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= ngroups) return;
+    const int key = gkey[gstart[i]];
+    pk[i] = (key >= 0) ? parent[key] : -2 - i; // a target outside the tree never merges
+}
+
+__global__ __launch_bounds__(256) void k_pack_greedy(int ngroups, size_t nt, int group_size,
+                                                     const int* __restrict__ gstart,
+                                                     const int* __restrict__ pk,
+                                                     uint32_t* __restrict__ mflag) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= ngroups) return;
+    const int p = pk[i];
+    const bool seg_start = (i == 0) || (p < 0) || (pk[i - 1] != p);
+    if (!seg_start) return;
+    int acc = 0;
+    for (int j = i; j < ngroups && (j == i || (p >= 0 && pk[j] == p)); ++j) {
+        const int hi = (j + 1 < ngroups) ? gstart[j + 1] : (int)nt;
+        const int cnt = hi - gstart[j];
+        if (j == i || acc + cnt > group_size) {
+            mflag[j] = 1u;
+            acc = cnt;
+        } else {
+            mflag[j] = 0u;
+            acc += cnt;
+        }
+    }
+}
+
+__global__ __launch_bounds__(256) void k_pack_starts(int ngroups, const int* __restrict__ gstart,
+                                                     const uint32_t* __restrict__ mflag,
+                                                     const uint32_t* __restrict__ mid,
+                                                     int* __restrict__ gstart2) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= ngroups) return;
+    if (mflag[i]) gstart2[mid[i]] = gstart[i];
+}
+
 // Per-group opening bounds (tree.cc:683-702): bounding box, largest softening, smallest aold.
 __global__ __launch_bounds__(256) void k_group_bounds(
     int ngroups, size_t nt, const int* __restrict__ gstart, const uint32_t* __restrict__ targets,
@@ -195,37 +243,49 @@ __device__ __forceinline__ float spline_inner_f(float r, float h) {
                     10.666666666666667f * u * u * u - 0.06666666666666667f / (u * u * u));
 }
 
-template <bool FP64, bool COUNT>
+template <bool FP64, bool COUNT, bool GSTACK>
 __global__ __launch_bounds__(64) void k_walk(WalkArgs a) {
     using real = typename WalkTypes<FP64>::real;
     using real4 = typename WalkTypes<FP64>::real4;
-    __shared__ int s_stack[kWalkStack];
-    __shared__ real4 s_src[kListCap];
-    __shared__ real s_soft[kListCap];
+    constexpr int kList = FP64 ? kListCapD : kListCapF;
+    // The node stack lives in LDS; a group whose stack would outgrow it is handed to a second
+    // launch of this kernel (GSTACK) that keeps the stack in a per-wave slice of global memory.
+    __shared__ int s_stack[GSTACK ? 1 : kWalkStack];
+    __shared__ real4 s_src[kList];
+    __shared__ real s_soft[kList];
     __shared__ int s_lpre[65];
     __shared__ int s_lplo[64];
-    __shared__ unsigned char s_kind[COUNT ? kListCap : 1];
+    __shared__ unsigned char s_kind[COUNT ? kList : 1];
+    int* const stk = GSTACK ? a.gstack + (size_t)blockIdx.x * (size_t)a.gstack_cap : s_stack;
+    const int scap = GSTACK ? a.gstack_cap
+                            : (a.stack_limit > 0 ? min(a.stack_limit, kWalkStack) : kWalkStack);
 
     const int lane = threadIdx.x;
     long long n_visit = 0, n_multi = 0, n_pair = 0;
+    int max_sp = 0;
 
     for (int item = 0;; ++item) {
-        // Item order: block b of XCD (b % 8) takes the b/8-th item of that XCD's contiguous share,
-        // so each XCD's L2 serves one compact stretch of Morton-ordered groups.
+        long long g_visit = 0, g_multi = 0, g_pair = 0; // this group's share of the counts
         int g;
-        if (a.persistent) {
+        if (GSTACK) {
+            if (item > 0) break;
+            g = a.retry_list[blockIdx.x];
+        } else if (a.persistent) {
             int v = 0;
             if (lane == 0) v = atomicAdd(a.queue, 1);
             g = __shfl(v, 0);
+            if (g >= a.ngroups) break;
         } else {
+            // Block b runs on XCD b % 8; give each XCD one contiguous share of the Morton-ordered
+            // groups so that its L2 serves one compact region of the tree.
             if (item > 0) break;
             const int b = blockIdx.x;
             const int nb = gridDim.x;
             const int xcd = b & 7, slot = b >> 3;
             const int per = nb >> 3, rem = nb & 7;
             g = xcd * per + min(xcd, rem) + slot; // a bijection of [0, nb)
+            if (g >= a.ngroups) break;
         }
-        if (g >= a.ngroups) break;
 
         const GroupInfo gi = a.groups[g];
         const double gcx = 0.5 * (gi.lo[0] + gi.hi[0]);
@@ -254,11 +314,8 @@ __global__ __launch_bounds__(64) void k_walk(WalkArgs a) {
                 te = a.soft ? (real)a.soft[p] : (real)0;
             }
             double accx = 0, accy = 0, accz = 0;
-            int sp = 1;
-            if (lane == 0) s_stack[0] = a.root;
             int lcount = 0;
             bool overflow = false;
-            __syncthreads();
 
             // Evaluate the list against every target of the chunk.
             auto flush = [&]() {
@@ -271,7 +328,7 @@ __global__ __launch_bounds__(64) void k_walk(WalkArgs a) {
                                      dz = (double)s.z - (double)tz;
                         const double r2 = dx * dx + dy * dy + dz * dz;
                         if (r2 <= 0) continue;
-                        if (COUNT && s_kind[j] && tvalid) ++n_pair;
+                        if (COUNT && s_kind[j] && tvalid) ++g_pair;
                         const double h = dmax_w(dmax_w((double)te, es), 1e-300);
                         const double f = (double)s.w * spline_force_over_r_d(sqrt(r2), h);
                         accx += dx * f;
@@ -281,19 +338,21 @@ __global__ __launch_bounds__(64) void k_walk(WalkArgs a) {
                 } else {
                     float fx = 0.f, fy = 0.f, fz = 0.f;
                     const float tef = (float)te;
+                    const float txf = (float)tx, tyf = (float)ty, tzf = (float)tz;
+#pragma unroll 4
                     for (int j = sub; j < lcount; j += L) {
                         const real4 s = s_src[j];
                         const float es = (float)s_soft[j];
-                        const float dx = (float)s.x - (float)tx, dy = (float)s.y - (float)ty,
-                                    dz = (float)s.z - (float)tz;
+                        const float dx = (float)s.x - txf, dy = (float)s.y - tyf,
+                                    dz = (float)s.z - tzf;
                         const float r2 = __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, dz * dz));
                         const float h = fmaxf(tef, es);
                         const float inv = __builtin_amdgcn_rsqf(r2);
-                        float g = inv * inv * inv;
-                        if (r2 < h * h) g = spline_inner_f(r2 * inv, h);
-                        float f = (float)s.w * g;
+                        float g3 = inv * inv * inv;
+                        if (r2 < h * h) g3 = spline_inner_f(r2 * inv, h);
+                        float f = (float)s.w * g3;
                         f = (r2 > 0.f) ? f : 0.f;
-                        if (COUNT && s_kind[j] && tvalid && r2 > 0.f) ++n_pair;
+                        if (COUNT && s_kind[j] && tvalid && r2 > 0.f) ++g_pair;
                         fx = __builtin_fmaf(dx, f, fx);
                         fy = __builtin_fmaf(dy, f, fy);
                         fz = __builtin_fmaf(dz, f, fz);
@@ -306,17 +365,17 @@ __global__ __launch_bounds__(64) void k_walk(WalkArgs a) {
                 lcount = 0;
             };
 
-            while (sp > 0 && !overflow) {
-                const int k = min(sp, 64);
-                const int base = sp - k;
-                const int node = (lane < k) ? s_stack[base + lane] : -1;
-                sp = base;
-                __syncthreads();
+            // Node batches are loaded one iteration ahead: the loads for the next batch are in
+            // flight while this batch's leaves are copied and the list is evaluated.
+            int sp = 0;
+            int k = 1;
+            int node = (lane == 0) ? a.root : -1;
+            WNode w;
+            if (node >= 0) w = a.wn[node];
+            while (k > 0 && !overflow) {
                 bool accept = false, open_leaf = false;
                 int nch = 0, first = -1, plo = 0, phi = 0;
-                WNode w;
                 if (node >= 0) {
-                    w = a.wn[node];
                     const double dx = dmax_w(0.0, dmax_w(gi.lo[0] - w.cx, w.cx - gi.hi[0]));
                     const double dy = dmax_w(0.0, dmax_w(gi.lo[1] - w.cy, w.cy - gi.hi[1]));
                     const double dz = dmax_w(0.0, dmax_w(gi.lo[2] - w.cz, w.cz - gi.hi[2]));
@@ -335,38 +394,53 @@ __global__ __launch_bounds__(64) void k_walk(WalkArgs a) {
                         }
                     }
                 }
-                if (COUNT && c0 == 0) n_visit += (lane == 0) ? k : 0;
+                if (COUNT && c0 == 0) g_visit += (lane == 0) ? k : 0;
 
-                // children of opened interior nodes go back on the stack
+                // monopole entries (kept in registers until the list has room)
+                real4 me;
+                real msoft = 0;
+                if (accept) {
+                    if (FP64) {
+                        me.x = (real)w.cx; me.y = (real)w.cy; me.z = (real)w.cz;
+                    } else {
+                        me.x = (real)(w.cx - gcx); me.y = (real)(w.cy - gcy);
+                        me.z = (real)(w.cz - gcz);
+                    }
+                    me.w = (real)w.mass;
+                    msoft = (real)(double)w.soft;
+                }
+
+                // children of opened interior nodes go on the stack, then the next batch is
+                // popped and its loads are issued
                 int ctot;
                 const int coff = wave_excl_scan(nch, ctot);
-                if (sp + ctot > kWalkStack) {
+                if (sp + ctot > scap) {
                     overflow = true;
                     break;
                 }
-                for (int c = 0; c < nch; ++c) s_stack[sp + coff + c] = first + c;
+                for (int c = 0; c < nch; ++c) stk[sp + coff + c] = first + c;
                 sp += ctot;
+                if (COUNT) max_sp = max(max_sp, sp);
+                if (GSTACK) __threadfence_block();
+                __syncthreads();
+                k = min(sp, 64);
+                const int base = sp - k;
+                const int next_node = (lane < k) ? stk[base + lane] : -1;
+                sp = base;
+                WNode wn_next;
+                if (next_node >= 0) wn_next = a.wn[next_node];
 
-                // accepted nodes become monopole entries
                 const uint64_t mb = __ballot(accept);
                 const int mcount = __popcll(mb);
                 if (mcount > 0) {
-                    if (lcount + mcount > kListCap) flush();
+                    if (lcount + mcount > kList) flush();
                     if (accept) {
                         const int pos = lcount + __popcll(mb & lanemask_lt());
-                        real4 e;
-                        if (FP64) {
-                            e.x = (real)w.cx; e.y = (real)w.cy; e.z = (real)w.cz;
-                        } else {
-                            e.x = (real)(w.cx - gcx); e.y = (real)(w.cy - gcy);
-                            e.z = (real)(w.cz - gcz);
-                        }
-                        e.w = (real)w.mass;
-                        s_src[pos] = e;
-                        s_soft[pos] = (real)(double)w.soft;
+                        s_src[pos] = me;
+                        s_soft[pos] = msoft;
                         if (COUNT) s_kind[pos] = 0;
                     }
-                    if (COUNT && tvalid && sub == 0) n_multi += mcount;
+                    if (COUNT && tvalid && sub == 0) g_multi += mcount;
                     lcount += mcount;
                 }
 
@@ -385,8 +459,8 @@ __global__ __launch_bounds__(64) void k_walk(WalkArgs a) {
                     if (lane == 0) s_lpre[nleaf] = ptot;
                     __syncthreads();
                     for (int done = 0; done < ptot;) {
-                        if (lcount == kListCap) flush();
-                        const int ncopy = min(kListCap - lcount, ptot - done);
+                        if (lcount == kList) flush();
+                        const int ncopy = min(kList - lcount, ptot - done);
                         for (int j = lane; j < ncopy; j += 64) {
                             const int idx = done + j;
                             int lo_ = 0, hi_ = nleaf; // last li with s_lpre[li] <= idx
@@ -414,11 +488,18 @@ __global__ __launch_bounds__(64) void k_walk(WalkArgs a) {
                     }
                 }
                 __syncthreads();
+                node = next_node;
+                w = wn_next;
             }
             if (overflow) {
-                if (lane == 0) atomicOr(a.status, 1);
+                if (!GSTACK && lane == 0) {
+                    const int slot = atomicAdd(a.n_retry, 1);
+                    a.retry_list[slot] = g;
+                }
+                if (GSTACK && lane == 0) atomicOr(a.status, 1);
+                g_visit = g_multi = g_pair = 0; // the retry counts this group
                 __syncthreads();
-                continue;
+                break; // the whole group is redone by the global-stack launch
             }
             if (lcount > 0) flush();
 
@@ -434,6 +515,9 @@ __global__ __launch_bounds__(64) void k_walk(WalkArgs a) {
                 a.az[tlo + t] = a.G * accz;
             }
         }
+        n_visit += g_visit;
+        n_multi += g_multi;
+        n_pair += g_pair;
     }
     if (COUNT) {
         // wave totals, one atomic per wave
@@ -449,6 +533,8 @@ __global__ __launch_bounds__(64) void k_walk(WalkArgs a) {
                       (unsigned long long)n_multi);
             atomicAdd(reinterpret_cast<unsigned long long*>(&a.counts[2]),
                       (unsigned long long)n_pair);
+            atomicMax(reinterpret_cast<unsigned long long*>(&a.counts[3]),
+                      (unsigned long long)max_sp);
         }
     }
 }
@@ -524,11 +610,26 @@ gz_status launch_walk(const WalkArgs& a, bool fp64, bool count, hipStream_t st) 
         grid = (unsigned)a.ngroups;
     }
     if (fp64) {
-        if (count) k_walk<true, true><<<grid, 64, 0, st>>>(a);
-        else k_walk<true, false><<<grid, 64, 0, st>>>(a);
+        if (count) k_walk<true, true, false><<<grid, 64, 0, st>>>(a);
+        else k_walk<true, false, false><<<grid, 64, 0, st>>>(a);
     } else {
-        if (count) k_walk<false, true><<<grid, 64, 0, st>>>(a);
-        else k_walk<false, false><<<grid, 64, 0, st>>>(a);
+        if (count) k_walk<false, true, false><<<grid, 64, 0, st>>>(a);
+        else k_walk<false, false, false><<<grid, 64, 0, st>>>(a);
+    }
+    GZ_HIP(hipGetLastError());
+    return GZ_OK;
+}
+
+// Groups whose node stack outgrew LDS, again, with the stack in global memory (n_retry waves).
+gz_status launch_walk_retry(const WalkArgs& a, int n_retry, bool fp64, bool count,
+                            hipStream_t st) {
+    if (n_retry <= 0) return GZ_OK;
+    if (fp64) {
+        if (count) k_walk<true, true, true><<<n_retry, 64, 0, st>>>(a);
+        else k_walk<true, false, true><<<n_retry, 64, 0, st>>>(a);
+    } else {
+        if (count) k_walk<false, true, true><<<n_retry, 64, 0, st>>>(a);
+        else k_walk<false, false, true><<<n_retry, 64, 0, st>>>(a);
     }
     GZ_HIP(hipGetLastError());
     return GZ_OK;
@@ -550,6 +651,21 @@ gz_status launch_group_prep(const uint32_t* targets, size_t nt, const int* leaf_
 gz_status launch_group_starts(size_t nt, const uint32_t* flag, const uint32_t* gid, int* gstart,
                               hipStream_t st) {
     k_group_starts<<<(unsigned)((nt + 255) / 256), 256, 0, st>>>(nt, flag, gid, gstart);
+    GZ_HIP(hipGetLastError());
+    return GZ_OK;
+}
+
+gz_status launch_group_pack(int ngroups, size_t nt, int group_size, const int* gstart,
+                            const int* gkey, const int* parent, int* pk, uint32_t* mflag,
+                            uint32_t* mid, uint32_t* ngroups2_dev, int* gstart2, void* scan_tmp,
+                            hipStream_t st) {
+    if (ngroups <= 0) return GZ_OK;
+    const unsigned grid = (unsigned)((ngroups + 255) / 256);
+    k_pack_parent<<<grid, 256, 0, st>>>(ngroups, nt, gstart, gkey, parent, pk);
+    k_pack_greedy<<<grid, 256, 0, st>>>(ngroups, nt, group_size, gstart, pk, mflag);
+    GZ_HIP(hipGetLastError());
+    GZ_TRY(exclusive_scan_u32(mflag, mid, (size_t)ngroups, ngroups2_dev, scan_tmp, st));
+    k_pack_starts<<<grid, 256, 0, st>>>(ngroups, gstart, mflag, mid, gstart2);
     GZ_HIP(hipGetLastError());
     return GZ_OK;
 }

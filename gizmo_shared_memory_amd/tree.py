@@ -199,7 +199,7 @@ def _outputs(n, device):
 def accel_grouped(T: Tree, P: Particles, targets, theta: float, G: float = 1.0, batch: int = 0,
                   aold=None, *, precision: str = "fp64", group_size: int = 0,
                   persistent: bool = False, counts: bool = False, timing: bool = False,
-                  out=None):
+                  stack_limit: int = 0, out=None):
     """shmem::accel_grouped (tree.h:428-440) without tidal/jerk/lazy/sink terms.
 
     precision "fp64" is the parity walk; "fp32" evaluates interactions in FP32 on positions
@@ -215,7 +215,7 @@ def accel_grouped(T: Tree, P: Particles, targets, theta: float, G: float = 1.0, 
     ao = None if aold is None else _dev_f64(aold, P.device)
     prec = {"fp64": L.PREC_FP64, "fp32": L.PREC_FP32_REL}[precision]
     opts = L.gz_walk_opts(prec, L.CRIT_REFERENCE, int(group_size), int(persistent), int(counts),
-                          None)
+                          int(stack_limit), None)
     cnt = L.gz_walk_counts()
     want = counts or timing
     L.check(L.load().gz_tree_accel_grouped(

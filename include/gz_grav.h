@@ -128,6 +128,8 @@ typedef struct {
     int persistent;  /* 1 = persistent work-queue launch; 0 = one workgroup per work item      */
     int count;       /* 1 = count node visits and interactions into gz_walk_counts (a slower
                         kernel variant); timings are reported whenever counts is non-NULL     */
+    int stack_limit; /* 0 = default. > 0 caps the on-chip node stack at this many entries, so
+                        groups spill to the global-memory re-walk (for tests of that path)    */
     const void* model; /* gz_model* when criterion == GZ_CRIT_LEARNED                          */
 } gz_walk_opts;
 
@@ -137,6 +139,8 @@ typedef struct {
     long long nodes_visited, multipole, pairs, groups, items;
     double walk_ms;    /* walk kernel duration, HIP events on the context stream */
     double prep_ms;    /* grouping + group bounds */
+    long long max_stack; /* deepest node stack of any group (counted walks only) */
+    long long retries;   /* groups re-walked with a global-memory node stack */
 } gz_walk_counts;
 
 /* shmem::accel_grouped (tree.h:428-440, tree.cc:626-897) with tidal = lazy = jerk = NULL and

@@ -268,3 +268,31 @@ def test_order_targets_and_aold(gz, oracle):
     ao = _np(gz.aold_from_accel(T, a_prev, tg, 0.0025, 2.0))
     ap = _np(a_prev)[tg]
     np.testing.assert_allclose(ao, 0.0025 * np.sqrt((ap * ap).sum(1)) / 2.0, rtol=1e-15)
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_stack_spill_rewalk_is_identical(gz, prec):
+    """Groups whose node stack outgrows the on-chip stack are walked again with a global-memory
+    stack; forcing a tiny on-chip stack must not change a single bit of the result."""
+    case = tree_cases.plummer_small(20000, seed=31, eps=0.01)
+    P, T = _device_tree(gz, case)
+    tg = T.device_orderbuf()
+    a = gz.accel_grouped(T, P, tg, 0.5, 1.0, 0, precision=prec, counts=True)
+    b = gz.accel_grouped(T, P, tg, 0.5, 1.0, 0, precision=prec, counts=True, stack_limit=16)
+    assert a[3]["retries"] == 0 and b[3]["retries"] > 0
+    assert a[3]["max_stack"] <= 1024
+    for k in ("nodes_visited", "multipole", "pairs"):
+        assert a[3][k] == b[3][k]
+    for u, v in zip(a[:3], b[:3]):
+        assert torch.equal(u, v)
+
+
+def test_packed_groups_are_compact_and_cover_targets(gz):
+    case = tree_cases.plummer_small(20000, seed=13, eps=0.01)
+    P, T = _device_tree(gz, case)
+    tg = T.device_orderbuf()
+    plain = gz.accel_grouped(T, P, tg, 0.7, 1.0, 0, group_size=64, counts=True)
+    assert plain[3]["groups"] < T.info.nleaves
+    ex = gz.accel_brute(P, tg[::97])
+    e = _relerr(tuple(_np(v)[::97] for v in plain[:3]), tuple(_np(v) for v in ex))
+    assert np.percentile(e, 99) < 2e-2
