@@ -30,8 +30,10 @@
 namespace gz {
 
 constexpr int kWalkStack = 1024;  // node ids per wave in LDS
-constexpr int kListCapF = 256;    // interaction-list entries per wave, FP32 entries (20 B)
-constexpr int kListCapD = 128;    // interaction-list entries per wave, FP64 entries (40 B)
+constexpr int kMonoF = 160;       // monopole entries per wave, FP32 (16 B each)
+constexpr int kPairF = 128;       // particle entries per wave, FP32 (20 B each)
+constexpr int kMonoD = 96;        // monopole entries per wave, FP64 (32 B each)
+constexpr int kPairD = 64;        // particle entries per wave, FP64 (40 B each)
 constexpr int kMaxChunk = 64;     // targets evaluated together (one wave)
 
 __device__ __forceinline__ double dmin_w(double a, double b) { return (b < a) ? b : a; }
@@ -166,29 +168,45 @@ __global__ __launch_bounds__(256) void k_pack_starts(int ngroups, const int* __r
     if (mflag[i]) gstart2[mid[i]] = gstart[i];
 }
 
+__device__ __forceinline__ double wave_min_d(double v) {
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) v = dmin_w(v, __shfl_xor(v, d));
+    return v;
+}
+__device__ __forceinline__ double wave_max_d(double v) {
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) v = dmax_w(v, __shfl_xor(v, d));
+    return v;
+}
+
 // Per-group opening bounds (tree.cc:683-702): bounding box, largest softening, smallest aold.
+// One wavefront per group; min and max are exact, so the lane order does not matter.
 __global__ __launch_bounds__(256) void k_group_bounds(
     int ngroups, size_t nt, const int* __restrict__ gstart, const uint32_t* __restrict__ targets,
     const double* __restrict__ x, const double* __restrict__ y, const double* __restrict__ z,
     const double* __restrict__ soft, const double* __restrict__ aold,
     const float* __restrict__ group_theta, double theta_sq, GroupInfo* __restrict__ gi) {
-    const int g = blockIdx.x * 256 + threadIdx.x;
-    if (g >= ngroups) return;
+    const int g = blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (g >= ngroups) return; // wave-uniform
     const int lo = gstart[g];
     const int hi = (g + 1 < ngroups) ? gstart[g + 1] : (int)nt;
     double bx0 = 1e300, by0 = 1e300, bz0 = 1e300, bx1 = -1e300, by1 = -1e300, bz1 = -1e300;
-    double emax = 0;
-    for (int i = lo; i < hi; ++i) {
+    double emax = 0, amin = aold ? 1e300 : 0.0;
+    for (int i = lo + lane; i < hi; i += 64) {
         const uint32_t p = targets[i];
         const double px = x[p], py = y[p], pz = z[p];
         bx0 = dmin_w(bx0, px); bx1 = dmax_w(bx1, px);
         by0 = dmin_w(by0, py); by1 = dmax_w(by1, py);
         bz0 = dmin_w(bz0, pz); bz1 = dmax_w(bz1, pz);
         if (soft) emax = dmax_w(emax, soft[p]);
+        if (aold) amin = dmin_w(amin, aold[i]);
     }
-    double amin = aold ? 1e300 : 0.0;
-    if (aold)
-        for (int i = lo; i < hi; ++i) amin = dmin_w(amin, aold[i]);
+    bx0 = wave_min_d(bx0); by0 = wave_min_d(by0); bz0 = wave_min_d(bz0);
+    bx1 = wave_max_d(bx1); by1 = wave_max_d(by1); bz1 = wave_max_d(bz1);
+    emax = wave_max_d(emax);
+    amin = wave_min_d(amin);
+    if (lane != 0) return;
     GroupInfo o;
     o.lo[0] = bx0; o.lo[1] = by0; o.lo[2] = bz0;
     o.hi[0] = bx1; o.hi[1] = by1; o.hi[2] = bz1;
@@ -243,19 +261,24 @@ __device__ __forceinline__ float spline_inner_f(float r, float h) {
                     10.666666666666667f * u * u * u - 0.06666666666666667f / (u * u * u));
 }
 
+// At most 128 VGPRs, so that four waves fit on each SIMD (the LDS footprint allows four too).
 template <bool FP64, bool COUNT, bool GSTACK>
-__global__ __launch_bounds__(64) void k_walk(WalkArgs a) {
+__global__ __launch_bounds__(64, 4) void k_walk(WalkArgs a) {
     using real = typename WalkTypes<FP64>::real;
     using real4 = typename WalkTypes<FP64>::real4;
-    constexpr int kList = FP64 ? kListCapD : kListCapF;
+    constexpr int kMono = FP64 ? kMonoD : kMonoF;
+    constexpr int kPair = FP64 ? kPairD : kPairF;
     // The node stack lives in LDS; a group whose stack would outgrow it is handed to a second
     // launch of this kernel (GSTACK) that keeps the stack in a per-wave slice of global memory.
     __shared__ int s_stack[GSTACK ? 1 : kWalkStack];
-    __shared__ real4 s_src[kList];
-    __shared__ real s_soft[kList];
+    // Interaction lists. An accepted node always lies outside every target's softening (the
+    // first arm of open_node), so monopoles are Newtonian and need no softening; particle
+    // entries carry theirs.
+    __shared__ real4 s_mono[kMono];
+    __shared__ real4 s_pair[kPair];
+    __shared__ real s_psoft[kPair];
     __shared__ int s_lpre[65];
     __shared__ int s_lplo[64];
-    __shared__ unsigned char s_kind[COUNT ? kList : 1];
     int* const stk = GSTACK ? a.gstack + (size_t)blockIdx.x * (size_t)a.gstack_cap : s_stack;
     const int scap = GSTACK ? a.gstack_cap
                             : (a.stack_limit > 0 ? min(a.stack_limit, kWalkStack) : kWalkStack);
@@ -314,21 +337,32 @@ __global__ __launch_bounds__(64) void k_walk(WalkArgs a) {
                 te = a.soft ? (real)a.soft[p] : (real)0;
             }
             double accx = 0, accy = 0, accz = 0;
-            int lcount = 0;
+            int mcount = 0, pcount = 0;
             bool overflow = false;
 
-            // Evaluate the list against every target of the chunk.
+            // Evaluate both lists against every target of the chunk.
             auto flush = [&]() {
                 __syncthreads();
                 if (FP64) {
-                    for (int j = sub; j < lcount; j += L) {
-                        const real4 s = s_src[j];
-                        const double es = (double)s_soft[j];
+                    for (int j = sub; j < mcount; j += L) {
+                        // u >= 1 for every accepted node, so the spline is 1/r^3 (tree.h:63)
+                        const real4 s = s_mono[j];
+                        const double dx = (double)s.x - (double)tx, dy = (double)s.y - (double)ty,
+                                     dz = (double)s.z - (double)tz;
+                        const double r = sqrt(dx * dx + dy * dy + dz * dz);
+                        const double f = (double)s.w * (1.0 / (r * r * r));
+                        accx += dx * f;
+                        accy += dy * f;
+                        accz += dz * f;
+                    }
+                    for (int j = sub; j < pcount; j += L) {
+                        const real4 s = s_pair[j];
+                        const double es = (double)s_psoft[j];
                         const double dx = (double)s.x - (double)tx, dy = (double)s.y - (double)ty,
                                      dz = (double)s.z - (double)tz;
                         const double r2 = dx * dx + dy * dy + dz * dz;
                         if (r2 <= 0) continue;
-                        if (COUNT && s_kind[j] && tvalid) ++g_pair;
+                        if (COUNT && tvalid) ++g_pair;
                         const double h = dmax_w(dmax_w((double)te, es), 1e-300);
                         const double f = (double)s.w * spline_force_over_r_d(sqrt(r2), h);
                         accx += dx * f;
@@ -340,19 +374,31 @@ __global__ __launch_bounds__(64) void k_walk(WalkArgs a) {
                     const float tef = (float)te;
                     const float txf = (float)tx, tyf = (float)ty, tzf = (float)tz;
 #pragma unroll 4
-                    for (int j = sub; j < lcount; j += L) {
-                        const real4 s = s_src[j];
-                        const float es = (float)s_soft[j];
+                    for (int j = sub; j < mcount; j += L) {
+                        const real4 s = s_mono[j];
+                        const float dx = (float)s.x - txf, dy = (float)s.y - tyf,
+                                    dz = (float)s.z - tzf;
+                        const float r2 = __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, dz * dz));
+                        const float inv = __builtin_amdgcn_rsqf(r2);
+                        const float f = ((float)s.w * inv) * (inv * inv);
+                        fx = __builtin_fmaf(dx, f, fx);
+                        fy = __builtin_fmaf(dy, f, fy);
+                        fz = __builtin_fmaf(dz, f, fz);
+                    }
+#pragma unroll 2
+                    for (int j = sub; j < pcount; j += L) {
+                        const real4 s = s_pair[j];
+                        const float es = (float)s_psoft[j];
                         const float dx = (float)s.x - txf, dy = (float)s.y - tyf,
                                     dz = (float)s.z - tzf;
                         const float r2 = __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, dz * dz));
                         const float h = fmaxf(tef, es);
                         const float inv = __builtin_amdgcn_rsqf(r2);
                         float g3 = inv * inv * inv;
-                        if (r2 < h * h) g3 = spline_inner_f(r2 * inv, h);
-                        float f = (float)s.w * g3;
-                        f = (r2 > 0.f) ? f : 0.f;
-                        if (COUNT && s_kind[j] && tvalid && r2 > 0.f) ++g_pair;
+                        if (r2 < h * h || !(r2 > 0.f)) // inside the kernel, or the target itself
+                            g3 = (r2 > 0.f) ? spline_inner_f(r2 * inv, h) : 0.f;
+                        if (COUNT && tvalid && r2 > 0.f) ++g_pair;
+                        const float f = (float)s.w * g3;
                         fx = __builtin_fmaf(dx, f, fx);
                         fy = __builtin_fmaf(dy, f, fy);
                         fz = __builtin_fmaf(dz, f, fz);
@@ -362,7 +408,8 @@ __global__ __launch_bounds__(64) void k_walk(WalkArgs a) {
                     accz += (double)fz;
                 }
                 __syncthreads();
-                lcount = 0;
+                mcount = 0;
+                pcount = 0;
             };
 
             // Node batches are loaded one iteration ahead: the loads for the next batch are in
@@ -398,7 +445,6 @@ __global__ __launch_bounds__(64) void k_walk(WalkArgs a) {
 
                 // monopole entries (kept in registers until the list has room)
                 real4 me;
-                real msoft = 0;
                 if (accept) {
                     if (FP64) {
                         me.x = (real)w.cx; me.y = (real)w.cy; me.z = (real)w.cz;
@@ -407,7 +453,6 @@ __global__ __launch_bounds__(64) void k_walk(WalkArgs a) {
                         me.z = (real)(w.cz - gcz);
                     }
                     me.w = (real)w.mass;
-                    msoft = (real)(double)w.soft;
                 }
 
                 // children of opened interior nodes go on the stack, then the next batch is
@@ -431,17 +476,12 @@ __global__ __launch_bounds__(64) void k_walk(WalkArgs a) {
                 if (next_node >= 0) wn_next = a.wn[next_node];
 
                 const uint64_t mb = __ballot(accept);
-                const int mcount = __popcll(mb);
-                if (mcount > 0) {
-                    if (lcount + mcount > kList) flush();
-                    if (accept) {
-                        const int pos = lcount + __popcll(mb & lanemask_lt());
-                        s_src[pos] = me;
-                        s_soft[pos] = msoft;
-                        if (COUNT) s_kind[pos] = 0;
-                    }
-                    if (COUNT && tvalid && sub == 0) g_multi += mcount;
-                    lcount += mcount;
+                const int nm = __popcll(mb);
+                if (nm > 0) {
+                    if (mcount + nm > kMono) flush();
+                    if (accept) s_mono[mcount + __popcll(mb & lanemask_lt())] = me;
+                    if (COUNT && tvalid && sub == 0) g_multi += nm;
+                    mcount += nm;
                 }
 
                 // particles of opened leaves become pair entries
@@ -459,8 +499,8 @@ __global__ __launch_bounds__(64) void k_walk(WalkArgs a) {
                     if (lane == 0) s_lpre[nleaf] = ptot;
                     __syncthreads();
                     for (int done = 0; done < ptot;) {
-                        if (lcount == kList) flush();
-                        const int ncopy = min(kList - lcount, ptot - done);
+                        if (pcount == kPair) flush();
+                        const int ncopy = min(kPair - pcount, ptot - done);
                         for (int j = lane; j < ncopy; j += 64) {
                             const int idx = done + j;
                             int lo_ = 0, hi_ = nleaf; // last li with s_lpre[li] <= idx
@@ -479,11 +519,10 @@ __global__ __launch_bounds__(64) void k_walk(WalkArgs a) {
                                 e.z = (real)(sp4.z - gcz);
                             }
                             e.w = (real)sp4.w;
-                            s_src[lcount + j] = e;
-                            s_soft[lcount + j] = (real)a.src_soft[q];
-                            if (COUNT) s_kind[lcount + j] = 1;
+                            s_pair[pcount + j] = e;
+                            s_psoft[pcount + j] = (real)a.src_soft[q];
                         }
-                        lcount += ncopy;
+                        pcount += ncopy;
                         done += ncopy;
                     }
                 }
@@ -501,7 +540,7 @@ __global__ __launch_bounds__(64) void k_walk(WalkArgs a) {
                 __syncthreads();
                 break; // the whole group is redone by the global-stack launch
             }
-            if (lcount > 0) flush();
+            if (mcount > 0 || pcount > 0) flush();
 
             // combine the lanes of each target, then write (tree.cc:880)
             for (int d = T; d < 64; d <<= 1) {
@@ -674,7 +713,7 @@ gz_status launch_group_bounds(int ngroups, size_t nt, const int* gstart, const u
                               const gz_particles* p, const double* aold, const float* group_theta,
                               double theta_sq, GroupInfo* gi, hipStream_t st) {
     if (ngroups <= 0) return GZ_OK;
-    k_group_bounds<<<(unsigned)((ngroups + 255) / 256), 256, 0, st>>>(
+    k_group_bounds<<<(unsigned)((ngroups + 3) / 4), 256, 0, st>>>(
         ngroups, nt, gstart, targets, p->x, p->y, p->z, p->soft, aold, group_theta, theta_sq, gi);
     GZ_HIP(hipGetLastError());
     return GZ_OK;
