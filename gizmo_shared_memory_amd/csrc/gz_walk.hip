@@ -319,12 +319,13 @@ __global__ __launch_bounds__(64, 4) void k_walk(WalkArgs a) {
         for (int c0 = 0; c0 < nt_all; c0 += kMaxChunk) {
             const int tlo = gi.tlo + c0;
             const int nt = min(kMaxChunk, nt_all - c0);
-            int T = 8;
-            while (T < nt) T <<= 1; // targets per sub-slice: 8, 16, 32 or 64
-            const int L = 64 / T;   // lanes per target
-            const int t = lane & (T - 1);
-            const int sub = lane / T;
-            const bool tvalid = t < nt;
+            // Lanes are (target t, slice sub) pairs: lane = sub * nt + t. Each of the L slices
+            // takes every L-th list entry; lanes beyond L * nt sit out the evaluation.
+            const int L = min(64 / nt, 8);
+            const int t = lane % nt;
+            const int sub = lane / nt;
+            const bool tvalid = sub < L;
+            const int jstart = tvalid ? sub : (1 << 30);
             real tx = 0, ty = 0, tz = 0, te = 0;
             if (tvalid) {
                 const uint32_t p = a.targets[tlo + t];
@@ -344,7 +345,7 @@ __global__ __launch_bounds__(64, 4) void k_walk(WalkArgs a) {
             auto flush = [&]() {
                 __syncthreads();
                 if (FP64) {
-                    for (int j = sub; j < mcount; j += L) {
+                    for (int j = jstart; j < mcount; j += L) {
                         // u >= 1 for every accepted node, so the spline is 1/r^3 (tree.h:63)
                         const real4 s = s_mono[j];
                         const double dx = (double)s.x - (double)tx, dy = (double)s.y - (double)ty,
@@ -355,7 +356,7 @@ __global__ __launch_bounds__(64, 4) void k_walk(WalkArgs a) {
                         accy += dy * f;
                         accz += dz * f;
                     }
-                    for (int j = sub; j < pcount; j += L) {
+                    for (int j = jstart; j < pcount; j += L) {
                         const real4 s = s_pair[j];
                         const double es = (double)s_psoft[j];
                         const double dx = (double)s.x - (double)tx, dy = (double)s.y - (double)ty,
@@ -373,9 +374,8 @@ __global__ __launch_bounds__(64, 4) void k_walk(WalkArgs a) {
                     float fx = 0.f, fy = 0.f, fz = 0.f;
                     const float tef = (float)te;
                     const float txf = (float)tx, tyf = (float)ty, tzf = (float)tz;
-#pragma unroll 4
-                    for (int j = sub; j < mcount; j += L) {
-                        const real4 s = s_mono[j];
+                    // four entries in flight per step, so their LDS reads overlap
+                    auto mono = [&](const real4 s) {
                         const float dx = (float)s.x - txf, dy = (float)s.y - tyf,
                                     dz = (float)s.z - tzf;
                         const float r2 = __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, dz * dz));
@@ -384,9 +384,19 @@ __global__ __launch_bounds__(64, 4) void k_walk(WalkArgs a) {
                         fx = __builtin_fmaf(dx, f, fx);
                         fy = __builtin_fmaf(dy, f, fy);
                         fz = __builtin_fmaf(dz, f, fz);
+                    };
+                    int j = jstart;
+                    for (; j + 3 * L < mcount; j += 4 * L) {
+                        const real4 s0 = s_mono[j], s1 = s_mono[j + L], s2 = s_mono[j + 2 * L],
+                                    s3 = s_mono[j + 3 * L];
+                        mono(s0);
+                        mono(s1);
+                        mono(s2);
+                        mono(s3);
                     }
+                    for (; j < mcount; j += L) mono(s_mono[j]);
 #pragma unroll 2
-                    for (int j = sub; j < pcount; j += L) {
+                    for (int j = jstart; j < pcount; j += L) {
                         const real4 s = s_pair[j];
                         const float es = (float)s_psoft[j];
                         const float dx = (float)s.x - txf, dy = (float)s.y - tyf,
@@ -542,11 +552,16 @@ __global__ __launch_bounds__(64, 4) void k_walk(WalkArgs a) {
             }
             if (mcount > 0 || pcount > 0) flush();
 
-            // combine the lanes of each target, then write (tree.cc:880)
-            for (int d = T; d < 64; d <<= 1) {
-                accx += __shfl_xor(accx, d);
-                accy += __shfl_xor(accy, d);
-                accz += __shfl_xor(accz, d);
+            // combine the slices of each target, then write (tree.cc:880)
+            for (int k = 1; k < L; ++k) {
+                const int src = min(lane + k * nt, 63);
+                const double ux = __shfl(accx, src), uy = __shfl(accy, src),
+                             uz = __shfl(accz, src);
+                if (sub == 0) {
+                    accx += ux;
+                    accy += uy;
+                    accz += uz;
+                }
             }
             if (tvalid && sub == 0) {
                 a.ax[tlo + t] = a.G * accx;
@@ -554,9 +569,11 @@ __global__ __launch_bounds__(64, 4) void k_walk(WalkArgs a) {
                 a.az[tlo + t] = a.G * accz;
             }
         }
-        n_visit += g_visit;
-        n_multi += g_multi;
-        n_pair += g_pair;
+        if (COUNT) {
+            n_visit += g_visit;
+            n_multi += g_multi;
+            n_pair += g_pair;
+        }
     }
     if (COUNT) {
         // wave totals, one atomic per wave
