@@ -15,6 +15,9 @@ GZ_OK = 0
 STATUS_NAMES = {0: "GZ_OK", 1: "GZ_ERR_INVALID", 2: "GZ_ERR_HIP", 3: "GZ_ERR_NOMEM",
                 4: "GZ_ERR_UNSUPPORTED", 5: "GZ_ERR_ARENA", 6: "GZ_ERR_INTERNAL"}
 
+MLP_FEATURES = 16
+MLP_PARAMS = 5313
+
 PREC_FP64 = 0
 PREC_FP32_REL = 1
 CRIT_REFERENCE = 0
@@ -68,7 +71,7 @@ class gz_walk_counts(C.Structure):
     _fields_ = [("nodes_visited", C.c_longlong), ("multipole", C.c_longlong),
                 ("pairs", C.c_longlong), ("groups", C.c_longlong), ("items", C.c_longlong),
                 ("walk_ms", C.c_double), ("prep_ms", C.c_double), ("max_stack", C.c_longlong),
-                ("retries", C.c_longlong)]
+                ("retries", C.c_longlong), ("fallback_groups", C.c_longlong)]
 
 
 _PROTOS = {
@@ -95,6 +98,20 @@ _PROTOS = {
                                 C.c_void_p, C.c_void_p]),
     "gz_direct_sum": (C.c_int, [C.c_void_p, C.POINTER(gz_particles), C.c_void_p, C.c_size_t,
                                 C.c_double, C.c_void_p, C.c_void_p, C.c_void_p]),
+    "gz_model_create": (C.c_int, [C.c_void_p, _fp, C.POINTER(C.c_void_p)]),
+    "gz_model_free": (C.c_int, [C.c_void_p]),
+    "gz_model_set_params": (C.c_int, [C.c_void_p, _fp]),
+    "gz_model_get_params": (C.c_int, [C.c_void_p, _fp]),
+    "gz_model_set_calibration": (C.c_int, [C.c_void_p, _fp, _fp, C.c_float, C.c_float,
+                                           C.c_float, C.c_float]),
+    "gz_model_forward": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t, C.c_void_p]),
+    "gz_model_grad": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t,
+                                C.c_void_p, _fp]),
+    "gz_model_train_step": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t,
+                                      C.c_float, _fp]),
+    "gz_tree_group_features": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(gz_particles),
+                                         C.c_void_p, C.c_size_t, C.c_int, C.c_void_p, C.c_void_p,
+                                         C.c_void_p, C.POINTER(C.c_size_t)]),
     "gz_order_targets": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t, C.c_void_p,
                                    C.POINTER(C.c_size_t)]),
     "gz_aold_from_accel": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t,

@@ -46,26 +46,39 @@ struct Carver {
     }
 };
 
-// The walk shared by gz_tree_accel_grouped and gz_tree_accel.
+// What a walk call adds beyond the reference's arguments: the learned model, and the
+// features-only mode that stops after computing the groups and their features.
+struct WalkExtra {
+    gz_model* model = nullptr;
+    float* feats_out = nullptr; // features-only: n_groups x 16, device
+    int* gstart_out = nullptr;  // features-only: n_groups + 1 group starts, device
+    size_t* ngroups_out = nullptr;
+};
+
+// The walk shared by gz_tree_accel_grouped, gz_tree_accel and gz_tree_group_features.
 gz_status walk(gz_ctx* ctx, const gz_tree* tree, const gz_particles* p, const uint32_t* targets,
                size_t n_t, double theta, double G, int batch, int group_size, const double* aold,
                const float* group_theta, int precision, int persistent, int leaf_always_open,
                int count, int stack_limit, int pack, double* ax, double* ay, double* az,
-               gz_walk_counts* counts, const char* who) {
+               gz_walk_counts* counts, const char* who, const WalkExtra& ex = WalkExtra()) {
+    const bool features_only = ex.feats_out != nullptr;
     GZ_TRY(check_ctx(ctx));
     if (!tree) return fail(GZ_ERR_INVALID, "%s: null tree", who);
     GZ_TRY(check_particles(p, who));
     if (p->n != tree->n)
         return fail(GZ_ERR_INVALID, "%s: particle count %zu differs from the tree's %zu", who,
                     p->n, tree->n);
-    if (n_t > 0 && (!targets || !ax || !ay || !az))
+    if (n_t > 0 && (!targets || (!features_only && (!ax || !ay || !az))))
         return fail(GZ_ERR_INVALID, "%s: null target or output array", who);
+    if ((ex.model || features_only) && !aold)
+        return fail(GZ_ERR_INVALID, "%s: the learned criterion needs aold", who);
     if (n_t >= 0x7FFFFFFFull) return fail(GZ_ERR_INVALID, "%s: n_t=%zu too large", who, n_t);
     if (precision != GZ_PREC_FP64 && precision != GZ_PREC_FP32_REL)
         return fail(GZ_ERR_INVALID, "%s: unknown precision %d", who, precision);
     if (group_size < 0 || group_size > 4096)
         return fail(GZ_ERR_INVALID, "%s: group_size %d out of range [0, 4096]", who, group_size);
     if (counts) std::memset(counts, 0, sizeof *counts);
+    if (ex.ngroups_out) *ex.ngroups_out = 0;
     if (n_t == 0) return GZ_OK;
     hipStream_t st = ctx->stream;
     hipEvent_t* ev = ctx->ev;
@@ -128,6 +141,36 @@ gz_status walk(gz_ctx* ctx, const gz_tree* tree, const gz_particles* p, const ui
     // counters, queue and status start at zero (a fill kernel rather than a memset node, so the
     // sequence stays valid under stream capture)
     GZ_TRY(gz::fill_zero_u32(reinterpret_cast<uint32_t*>(misc), 16, st));
+    long long n_fallback = 0;
+    if (ex.model || features_only) {
+        // per-group features -> network -> per-group tolerance (or the reference's)
+        gz::DevMem& sf = ctx->scratch[8];
+        const size_t fbytes = ((size_t)ngroups * gz::kMlpIn * sizeof(float) + 255) & ~size_t(255);
+        GZ_TRY(sf.reserve(fbytes + (size_t)ngroups * sizeof(float) + 256));
+        float* feats = features_only ? ex.feats_out : sf.as<float>();
+        float* pred = reinterpret_cast<float*>(sf.as<uint8_t>() + fbytes);
+        GZ_TRY(gz::launch_group_features((int)ngroups, gi, targets, p, aold, tree, feats, st));
+        if (features_only) {
+            GZ_HIP(hipMemcpyAsync(ex.gstart_out, gstart, ngroups * sizeof(int),
+                                  hipMemcpyDeviceToDevice, st));
+            const int end = (int)n_t;
+            GZ_HIP(hipMemcpyAsync(ex.gstart_out + ngroups, &end, sizeof(int),
+                                  hipMemcpyHostToDevice, st));
+            GZ_HIP(hipStreamSynchronize(st));
+            *ex.ngroups_out = ngroups;
+            return GZ_OK;
+        }
+        gz_model* M = ex.model;
+        GZ_TRY(gz::launch_mlp_forward(feats, (int)ngroups, M->wbf, pred, st));
+        GZ_TRY(gz::launch_group_scale((int)ngroups, feats, pred, M->frange, M->lab_lo, M->lab_hi,
+                                      M->margin, M->log_alpha_ref, gi,
+                                      reinterpret_cast<int*>(misc + 2), st));
+        int nf = 0;
+        GZ_HIP(hipMemcpyAsync(&nf, misc + 2, sizeof nf, hipMemcpyDeviceToHost, st));
+        GZ_HIP(hipStreamSynchronize(st));
+        n_fallback = nf;
+        GZ_TRY(gz::fill_zero_u32(reinterpret_cast<uint32_t*>(misc), 16, st));
+    }
     GZ_HIP(hipEventRecord(ev[9], st));
 
     gz::WalkArgs a{};
@@ -179,6 +222,7 @@ gz_status walk(gz_ctx* ctx, const gz_tree* tree, const gz_particles* p, const ui
             return fail(GZ_ERR_INTERNAL, "%s: a group's node stack outgrew %d entries", who, cap);
     }
     if (counts) {
+        counts->fallback_groups = n_fallback;
         counts->retries = retries;
         counts->max_stack = host_misc[3];
         counts->nodes_visited = host_misc[0];
@@ -399,6 +443,7 @@ gz_status gz_tree_accel_grouped(gz_ctx* ctx, const gz_tree* tree, const gz_parti
                                 const gz_walk_opts* opts, gz_walk_counts* counts) {
     gz::clear_error();
     int precision = GZ_PREC_FP64, group_size = 0, persistent = 0, count = 0, stack_limit = 0;
+    WalkExtra ex;
     const float* group_theta = nullptr;
     if (opts) {
         precision = opts->precision;
@@ -406,10 +451,11 @@ gz_status gz_tree_accel_grouped(gz_ctx* ctx, const gz_tree* tree, const gz_parti
         persistent = opts->persistent;
         count = opts->count;
         stack_limit = opts->stack_limit;
-        if (opts->criterion == GZ_CRIT_LEARNED)
-            return fail(GZ_ERR_UNSUPPORTED,
-                        "gz_tree_accel_grouped: use gz_tree_accel_learned for the learned criterion");
-        if (opts->criterion != GZ_CRIT_REFERENCE)
+        if (opts->criterion == GZ_CRIT_LEARNED) {
+            if (!opts->model)
+                return fail(GZ_ERR_INVALID, "gz_tree_accel_grouped: learned criterion without a model");
+            ex.model = static_cast<gz_model*>(const_cast<void*>(opts->model));
+        } else if (opts->criterion != GZ_CRIT_REFERENCE)
             return fail(GZ_ERR_INVALID, "gz_tree_accel_grouped: unknown criterion %d",
                         opts->criterion);
     }
@@ -417,7 +463,160 @@ gz_status gz_tree_accel_grouped(gz_ctx* ctx, const gz_tree* tree, const gz_parti
         return fail(GZ_ERR_INVALID, "gz_tree_accel_grouped: batch > 0 and group_size > 0");
     return walk(ctx, tree, p, targets, n_t, theta, G, batch, group_size, aold, group_theta,
                 precision, persistent, 0, count, stack_limit, 1, ax, ay, az, counts,
-                "gz_tree_accel_grouped");
+                "gz_tree_accel_grouped", ex);
+}
+
+gz_status gz_tree_group_features(gz_ctx* ctx, const gz_tree* tree, const gz_particles* p,
+                                 const uint32_t* targets, size_t n_t, int group_size,
+                                 const double* aold, float* feats, int* group_start,
+                                 size_t* n_groups) {
+    gz::clear_error();
+    if (!feats || !group_start || !n_groups)
+        return fail(GZ_ERR_INVALID, "gz_tree_group_features: null output");
+    WalkExtra ex;
+    ex.feats_out = feats;
+    ex.gstart_out = group_start;
+    ex.ngroups_out = n_groups;
+    return walk(ctx, tree, p, targets, n_t, 0.0, 1.0, 0, group_size, aold, nullptr, GZ_PREC_FP32_REL,
+                0, 0, 0, 0, 1, nullptr, nullptr, nullptr, nullptr, "gz_tree_group_features", ex);
+}
+
+// ---- the learned model ------------------------------------------------------------------------------
+gz_status gz_model_create(gz_ctx* ctx, const float* host_params, gz_model** out) {
+    gz::clear_error();
+    if (!out) return fail(GZ_ERR_INVALID, "gz_model_create: null out");
+    *out = nullptr;
+    GZ_TRY(check_ctx(ctx));
+    gz_model* M = new (std::nothrow) gz_model();
+    if (!M) return fail(GZ_ERR_NOMEM, "gz_model_create: out of host memory");
+    M->ctx = ctx;
+    const size_t nf = (size_t)gz::kMlpParams;
+    const size_t bytes = 4 * nf * sizeof(float) + 256 + 2 * gz::kMlpIn * sizeof(float) +
+                         nf * sizeof(uint16_t) + 1024;
+    hipError_t e = hipMalloc(&M->mem, bytes);
+    if (e != hipSuccess) {
+        delete M;
+        return gz::hip_fail(e, "hipMalloc", __FILE__, __LINE__);
+    }
+    float* f = static_cast<float*>(M->mem);
+    M->theta = f;
+    M->m = f + nf;
+    M->v = f + 2 * nf;
+    M->grad = f + 3 * nf;
+    M->loss = f + 4 * nf;
+    M->frange = f + 4 * nf + 64;
+    M->wbf = reinterpret_cast<uint16_t*>(f + 4 * nf + 64 + 2 * gz::kMlpIn);
+    hipStream_t st = ctx->stream;
+    gz_status s = gz::fill_zero_u32(reinterpret_cast<uint32_t*>(f), 4 * nf + 64, st);
+    if (s == GZ_OK && host_params) s = gz_model_set_params(M, host_params);
+    if (s == GZ_OK) {
+        float fr[2 * gz::kMlpIn];
+        for (int k = 0; k < gz::kMlpIn; ++k) {
+            fr[k] = -1e30f;
+            fr[gz::kMlpIn + k] = 1e30f;
+        }
+        if (hipMemcpyAsync(M->frange, fr, sizeof fr, hipMemcpyHostToDevice, st) != hipSuccess ||
+            hipStreamSynchronize(st) != hipSuccess)
+            s = fail(GZ_ERR_HIP, "gz_model_create: upload failed");
+    }
+    if (s != GZ_OK) {
+        gz_model_free(M);
+        return s;
+    }
+    *out = M;
+    return GZ_OK;
+}
+
+gz_status gz_model_free(gz_model* M) {
+    if (!M) return GZ_OK;
+    if (M->mem) {
+        (void)hipStreamSynchronize(M->ctx->stream);
+        (void)hipFree(M->mem);
+    }
+    delete M;
+    return GZ_OK;
+}
+
+gz_status gz_model_set_params(gz_model* M, const float* host_params) {
+    if (!M || !host_params) return fail(GZ_ERR_INVALID, "gz_model_set_params: null argument");
+    hipStream_t st = M->ctx->stream;
+    const size_t nf = (size_t)gz::kMlpParams;
+    GZ_HIP(hipMemcpyAsync(M->theta, host_params, nf * sizeof(float), hipMemcpyHostToDevice, st));
+    GZ_TRY(gz::fill_zero_u32(reinterpret_cast<uint32_t*>(M->m), 2 * nf, st)); // Adam state
+    GZ_TRY(gz::launch_to_bf16(M->theta, M->wbf, st));
+    GZ_HIP(hipStreamSynchronize(st));
+    M->step = 0;
+    return GZ_OK;
+}
+
+gz_status gz_model_get_params(const gz_model* M, float* host_params) {
+    if (!M || !host_params) return fail(GZ_ERR_INVALID, "gz_model_get_params: null argument");
+    hipStream_t st = M->ctx->stream;
+    GZ_HIP(hipMemcpyAsync(host_params, M->theta, gz::kMlpParams * sizeof(float),
+                          hipMemcpyDeviceToHost, st));
+    GZ_HIP(hipStreamSynchronize(st));
+    return GZ_OK;
+}
+
+gz_status gz_model_set_calibration(gz_model* M, const float* feat_lo, const float* feat_hi,
+                                   float label_lo, float label_hi, float margin,
+                                   float log10_alpha_ref) {
+    if (!M || !feat_lo || !feat_hi)
+        return fail(GZ_ERR_INVALID, "gz_model_set_calibration: null argument");
+    float fr[2 * gz::kMlpIn];
+    for (int k = 0; k < gz::kMlpIn; ++k) {
+        fr[k] = feat_lo[k];
+        fr[gz::kMlpIn + k] = feat_hi[k];
+    }
+    hipStream_t st = M->ctx->stream;
+    GZ_HIP(hipMemcpyAsync(M->frange, fr, sizeof fr, hipMemcpyHostToDevice, st));
+    GZ_HIP(hipStreamSynchronize(st));
+    M->lab_lo = label_lo;
+    M->lab_hi = label_hi;
+    M->margin = margin;
+    M->log_alpha_ref = log10_alpha_ref;
+    return GZ_OK;
+}
+
+gz_status gz_model_forward(gz_ctx* ctx, const gz_model* M, const float* X, size_t B, float* out) {
+    gz::clear_error();
+    GZ_TRY(check_ctx(ctx));
+    if (!M || (B && (!X || !out))) return fail(GZ_ERR_INVALID, "gz_model_forward: null argument");
+    if (B >= 0x7FFFFFFFull) return fail(GZ_ERR_INVALID, "gz_model_forward: B too large");
+    return gz::launch_mlp_forward(X, (int)B, M->wbf, out, ctx->stream);
+}
+
+gz_status gz_model_grad(gz_ctx* ctx, gz_model* M, const float* X, const float* T, size_t B,
+                        float* grad_out, float* loss_out) {
+    gz::clear_error();
+    GZ_TRY(check_ctx(ctx));
+    if (!M || (B && (!X || !T))) return fail(GZ_ERR_INVALID, "gz_model_grad: null argument");
+    if (B == 0 || B >= 0x7FFFFFFFull) return fail(GZ_ERR_INVALID, "gz_model_grad: bad B");
+    hipStream_t st = ctx->stream;
+    GZ_TRY(gz::fill_zero_u32(reinterpret_cast<uint32_t*>(M->grad), gz::kMlpParams + 64, st));
+    GZ_TRY(gz::launch_mlp_grad(X, T, (int)B, M->wbf, M->grad, M->loss, st));
+    if (grad_out)
+        GZ_HIP(hipMemcpyAsync(grad_out, M->grad, gz::kMlpParams * sizeof(float),
+                              hipMemcpyDeviceToDevice, st));
+    if (loss_out) {
+        GZ_HIP(hipMemcpyAsync(loss_out, M->loss, sizeof(float), hipMemcpyDeviceToHost, st));
+        GZ_HIP(hipStreamSynchronize(st));
+    }
+    return GZ_OK;
+}
+
+gz_status gz_model_train_step(gz_ctx* ctx, gz_model* M, const float* X, const float* T, size_t B,
+                              float lr, float* loss_out) {
+    GZ_TRY(gz_model_grad(ctx, M, X, T, B, nullptr, nullptr));
+    M->step += 1;
+    GZ_TRY(gz::launch_adam(M->theta, M->m, M->v, M->grad, lr, 0.9f, 0.999f, 1e-8f, M->step,
+                           M->wbf, ctx->stream));
+    if (loss_out) {
+        GZ_HIP(hipMemcpyAsync(loss_out, M->loss, sizeof(float), hipMemcpyDeviceToHost,
+                              ctx->stream));
+        GZ_HIP(hipStreamSynchronize(ctx->stream));
+    }
+    return GZ_OK;
 }
 
 gz_status gz_tree_accel(gz_ctx* ctx, const gz_tree* tree, const gz_particles* p,

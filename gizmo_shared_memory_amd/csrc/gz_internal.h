@@ -14,6 +14,8 @@ namespace gz {
 constexpr int kLeafMax = 16;  // particles per leaf (shmem/tree.h:357)
 constexpr int kMaxLevel = 20; // deepest level; keys carry 20 populated bits per axis (tree.h:358)
 constexpr int kWave = 64;     // CDNA wavefront width
+constexpr int kMlpIn = 16;    // features per target group
+constexpr int kMlpParams = 64 * 16 + 64 + 64 * 64 + 64 + 64 + 1; // 5313
 
 // Packed walk node: everything one opening test and one multipole interaction need, in one
 // 64-byte line (the reference's WNode, tree.h:195-204, with the threaded `next` replaced by the
@@ -129,6 +131,22 @@ gz_status launch_walk_retry(const WalkArgs& a, int n_retry, bool fp64, bool coun
 gz_status launch_direct(const gz_particles* p, const uint32_t* targets, size_t nt, double G,
                         double* ax, double* ay, double* az, hipStream_t st);
 
+// ---- learned opening criterion (gz_learned.hip) ----------------------------------------------------
+gz_status launch_group_features(int ngroups, const GroupInfo* gi, const uint32_t* targets,
+                                const gz_particles* p, const double* amag, const ::gz_tree* t,
+                                float* feats, hipStream_t st);
+gz_status launch_mlp_forward(const float* X, int B, const uint16_t* wbf, float* out,
+                             hipStream_t st);
+gz_status launch_mlp_grad(const float* X, const float* T, int B, const uint16_t* wbf,
+                          float* grad, float* loss, hipStream_t st);
+gz_status launch_adam(float* theta, float* m, float* v, const float* grad, float lr, float b1,
+                      float b2, float eps, int step, uint16_t* wbf, hipStream_t st);
+gz_status launch_to_bf16(const float* theta, uint16_t* wbf, hipStream_t st);
+gz_status launch_group_scale(int ngroups, const float* feats, const float* pred,
+                             const float* frange, float lab_lo, float lab_hi, float margin,
+                             float log_alpha_ref, GroupInfo* gi, int* n_fallback,
+                             hipStream_t st);
+
 } // namespace gz
 
 // ---- handle types -----------------------------------------------------------------------------
@@ -136,8 +154,21 @@ struct gz_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
     bool own_stream = false;
-    gz::DevMem scratch[8];
+    gz::DevMem scratch[12];
     hipEvent_t ev[16] = {};
+};
+
+// The learned opening-parameter network: FP32 master weights with Adam state, their bf16 copy
+// used by every forward pass, and the calibration that decides when a group falls back to the
+// reference criterion. One device allocation holds all of it.
+struct gz_model {
+    gz_ctx* ctx = nullptr;
+    void* mem = nullptr;
+    float *theta = nullptr, *m = nullptr, *v = nullptr, *grad = nullptr, *loss = nullptr;
+    float* frange = nullptr; // feature lower bounds [16] then upper bounds [16]
+    uint16_t* wbf = nullptr;
+    int step = 0;
+    float lab_lo = -30.f, lab_hi = 30.f, margin = 0.f, log_alpha_ref = 0.f;
 };
 
 struct gz_tree {

@@ -47,17 +47,25 @@ def _dev_u32(a, device):
 
 
 class Context:
-    """One gz_ctx per device, ordered on that device's current torch stream at creation."""
+    """One gz_ctx per device.
+
+    The library orders its work on its own stream (the context's). Torch produces and consumes
+    the tensors passed in, so every call here first waits for torch's current stream (inputs are
+    complete) and, when the call returns before its kernels finish, waits for the library's
+    stream before handing results back (outputs are complete)."""
 
     _by_device: dict = {}
 
     def __init__(self, device: int = 0):
         lib = L.load()
         self.device = device
-        stream = torch.cuda.current_stream(device).cuda_stream
         h = C.c_void_p()
-        L.check(lib.gz_ctx_create(device, C.c_void_p(stream), C.byref(h)), "gz_ctx_create")
+        L.check(lib.gz_ctx_create(device, None, C.byref(h)), "gz_ctx_create")
         self.handle = h
+
+    def enter(self):
+        """Wait for torch's pending work on this device (the inputs)."""
+        torch.cuda.current_stream(self.device).synchronize()
 
     @classmethod
     def get(cls, device: int = 0) -> "Context":
@@ -177,8 +185,10 @@ class Tree:
     def update_sources(self, P: Particles | None = None):
         """Re-read source positions/masses/softenings into the tree's sorted copy."""
         P = P or self.P
+        self.ctx.enter()
         L.check(L.load().gz_tree_update_sources(self.ctx.handle, self.handle, P.cstruct()),
                 "gz_tree_update_sources")
+        self.ctx.synchronize()
 
 
 def build(P: Particles, randomize_seed: int = -1, shift=(0.0, 0.0, 0.0), max_nodes: int = 0,
@@ -187,6 +197,7 @@ def build(P: Particles, randomize_seed: int = -1, shift=(0.0, 0.0, 0.0), max_nod
     ctx = ctx or Context.get(P.device.index or 0)
     opts = L.gz_build_opts(int(randomize_seed), (C.c_double * 3)(*shift), int(max_nodes))
     h = C.c_void_p()
+    ctx.enter()
     L.check(L.load().gz_tree_build(ctx.handle, P.cstruct(), C.byref(opts), C.byref(h)),
             "gz_tree_build")
     return Tree(h, ctx, P)
@@ -196,10 +207,102 @@ def _outputs(n, device):
     return tuple(torch.empty(n, dtype=torch.float64, device=device) for _ in range(3))
 
 
+class Model:
+    """The learned opening-parameter network (gz_model): 16 -> 64 -> 64 -> 1, GELU, bf16 weights
+    and FP32 accumulation on the device, FP32 master weights trained with Adam."""
+
+    N_PARAMS = L.MLP_PARAMS
+    N_FEATURES = L.MLP_FEATURES
+
+    def __init__(self, params=None, ctx: Context | None = None):
+        self.ctx = ctx or Context.get(0)
+        h = C.c_void_p()
+        arr = None if params is None else np.ascontiguousarray(params, dtype=np.float32)
+        if arr is not None and arr.shape != (self.N_PARAMS,):
+            raise ValueError(f"expected {self.N_PARAMS} parameters, got {arr.shape}")
+        L.check(L.load().gz_model_create(self.ctx.handle,
+                                         None if arr is None else arr.ctypes.data_as(L._fp),
+                                         C.byref(h)), "gz_model_create")
+        self.handle = h
+        self.calibration = None
+
+    def __del__(self):
+        h = getattr(self, "handle", None)
+        if h is not None and L._lib is not None:
+            L._lib.gz_model_free(h)
+            self.handle = None
+
+    def params(self) -> np.ndarray:
+        out = np.zeros(self.N_PARAMS, dtype=np.float32)
+        L.check(L.load().gz_model_get_params(self.handle, out.ctypes.data_as(L._fp)),
+                "gz_model_get_params")
+        return out
+
+    def set_params(self, params):
+        arr = np.ascontiguousarray(params, dtype=np.float32)
+        L.check(L.load().gz_model_set_params(self.handle, arr.ctypes.data_as(L._fp)),
+                "gz_model_set_params")
+
+    def set_calibration(self, feat_lo, feat_hi, label_lo, label_hi, margin, log10_alpha_ref):
+        lo = np.ascontiguousarray(feat_lo, dtype=np.float32)
+        hi = np.ascontiguousarray(feat_hi, dtype=np.float32)
+        L.check(L.load().gz_model_set_calibration(self.handle, lo.ctypes.data_as(L._fp),
+                                                  hi.ctypes.data_as(L._fp), float(label_lo),
+                                                  float(label_hi), float(margin),
+                                                  float(log10_alpha_ref)),
+                "gz_model_set_calibration")
+        self.calibration = dict(feat_lo=lo, feat_hi=hi, label_lo=float(label_lo),
+                                label_hi=float(label_hi), margin=float(margin),
+                                log10_alpha_ref=float(log10_alpha_ref))
+
+    def forward(self, X: torch.Tensor) -> torch.Tensor:
+        X = X.to(torch.float32).contiguous()
+        out = torch.empty(X.shape[0], dtype=torch.float32, device=X.device)
+        self.ctx.enter()
+        L.check(L.load().gz_model_forward(self.ctx.handle, self.handle, _ptr(X), X.shape[0],
+                                          _ptr(out)), "gz_model_forward")
+        self.ctx.synchronize()
+        return out
+
+    def grad(self, X: torch.Tensor, T: torch.Tensor):
+        X = X.to(torch.float32).contiguous()
+        T = T.to(torch.float32).contiguous()
+        g = torch.empty(self.N_PARAMS, dtype=torch.float32, device=X.device)
+        loss = C.c_float(0.0)
+        self.ctx.enter()
+        L.check(L.load().gz_model_grad(self.ctx.handle, self.handle, _ptr(X), _ptr(T), X.shape[0],
+                                       _ptr(g), C.byref(loss)), "gz_model_grad")
+        return float(loss.value), g
+
+    def train_step(self, X: torch.Tensor, T: torch.Tensor, lr: float) -> float:
+        loss = C.c_float(0.0)
+        self.ctx.enter()
+        L.check(L.load().gz_model_train_step(self.ctx.handle, self.handle, _ptr(X), _ptr(T),
+                                             X.shape[0], float(lr), C.byref(loss)),
+                "gz_model_train_step")
+        return float(loss.value)
+
+
+def group_features(T: Tree, P: Particles, targets, aold, group_size: int = 0):
+    """The groups the walk forms for these targets and their 16 features (device tensors)."""
+    tg = _dev_u32(targets, P.device)
+    nt = int(tg.shape[0])
+    ao = _dev_f64(aold, P.device)
+    feats = torch.empty((max(nt, 1), L.MLP_FEATURES), dtype=torch.float32, device=P.device)
+    gstart = torch.empty(nt + 1, dtype=torch.int32, device=P.device)
+    ng = C.c_size_t(0)
+    T.ctx.enter()
+    L.check(L.load().gz_tree_group_features(T.ctx.handle, T.handle, P.cstruct(), _ptr(tg), nt,
+                                            int(group_size), _ptr(ao), _ptr(feats), _ptr(gstart),
+                                            C.byref(ng)), "gz_tree_group_features")
+    n = ng.value
+    return feats[:n], gstart[: n + 1]
+
+
 def accel_grouped(T: Tree, P: Particles, targets, theta: float, G: float = 1.0, batch: int = 0,
                   aold=None, *, precision: str = "fp64", group_size: int = 0,
                   persistent: bool = False, counts: bool = False, timing: bool = False,
-                  stack_limit: int = 0, out=None):
+                  stack_limit: int = 0, model: Model | None = None, out=None):
     """shmem::accel_grouped (tree.h:428-440) without tidal/jerk/lazy/sink terms.
 
     precision "fp64" is the parity walk; "fp32" evaluates interactions in FP32 on positions
@@ -214,10 +317,12 @@ def accel_grouped(T: Tree, P: Particles, targets, theta: float, G: float = 1.0, 
     ax, ay, az = out if out is not None else _outputs(nt, P.device)
     ao = None if aold is None else _dev_f64(aold, P.device)
     prec = {"fp64": L.PREC_FP64, "fp32": L.PREC_FP32_REL}[precision]
-    opts = L.gz_walk_opts(prec, L.CRIT_REFERENCE, int(group_size), int(persistent), int(counts),
-                          int(stack_limit), None)
+    crit = L.CRIT_REFERENCE if model is None else L.CRIT_LEARNED
+    opts = L.gz_walk_opts(prec, crit, int(group_size), int(persistent), int(counts),
+                          int(stack_limit), None if model is None else model.handle)
     cnt = L.gz_walk_counts()
     want = counts or timing
+    T.ctx.enter()
     L.check(L.load().gz_tree_accel_grouped(
         T.ctx.handle, T.handle, P.cstruct(), _ptr(tg), nt, float(theta), float(G), int(batch),
         _ptr(ao), _ptr(ax), _ptr(ay), _ptr(az), C.byref(opts),
@@ -234,6 +339,7 @@ def accel(T: Tree, P: Particles, targets, theta: float, G: float = 1.0, aold=Non
     nt = int(tg.shape[0])
     ax, ay, az = _outputs(nt, P.device)
     ao = None if aold is None else _dev_f64(aold, P.device)
+    T.ctx.enter()
     L.check(L.load().gz_tree_accel(T.ctx.handle, T.handle, P.cstruct(), _ptr(tg), nt,
                                    float(theta), float(G), _ptr(ao), _ptr(ax), _ptr(ay),
                                    _ptr(az)), "gz_tree_accel")
@@ -246,6 +352,7 @@ def accel_brute(P: Particles, targets, G: float = 1.0, ctx: Context | None = Non
     tg = _dev_u32(targets, P.device)
     nt = int(tg.shape[0])
     ax, ay, az = _outputs(nt, P.device)
+    ctx.enter()
     L.check(L.load().gz_direct_sum(ctx.handle, P.cstruct(), _ptr(tg), nt, float(G), _ptr(ax),
                                    _ptr(ay), _ptr(az)), "gz_direct_sum")
     ctx.synchronize()
@@ -256,6 +363,7 @@ def order_targets(T: Tree, active=None) -> torch.Tensor:
     """Active particles in the tree's Morton order (mfm.cc:698-718). None = all active."""
     n_out = C.c_size_t(0)
     out = torch.empty(max(T.n_live, 1), dtype=torch.int32, device=T.P.device)
+    T.ctx.enter()
     if active is None:
         L.check(L.load().gz_order_targets(T.ctx.handle, T.handle, None, T.P.n, _ptr(out),
                                           C.byref(n_out)), "gz_order_targets")
@@ -272,7 +380,9 @@ def aold_from_accel(T: Tree, a_prev: torch.Tensor, targets, err_tol_force_acc: f
     tg = _dev_u32(targets, T.P.device)
     a_prev = a_prev.to(T.P.device, torch.float64).contiguous()
     out = torch.empty(int(tg.shape[0]), dtype=torch.float64, device=T.P.device)
+    T.ctx.enter()
     L.check(L.load().gz_aold_from_accel(T.ctx.handle, _ptr(a_prev), _ptr(tg), int(tg.shape[0]),
                                         float(err_tol_force_acc), float(G), _ptr(out)),
             "gz_aold_from_accel")
+    T.ctx.synchronize()
     return out

@@ -40,6 +40,7 @@ typedef enum {
 
 typedef struct gz_ctx gz_ctx;   /* one per device + stream */
 typedef struct gz_tree gz_tree; /* device-resident tree, owned by the library */
+typedef struct gz_model gz_model; /* learned opening-parameter network (device-resident) */
 
 /* Message for the last failing call on this thread ("" if none). */
 const char* gz_last_error(void);
@@ -141,6 +142,7 @@ typedef struct {
     double prep_ms;    /* grouping + group bounds */
     long long max_stack; /* deepest node stack of any group (counted walks only) */
     long long retries;   /* groups re-walked with a global-memory node stack */
+    long long fallback_groups; /* learned criterion: groups that kept the reference tolerance */
 } gz_walk_counts;
 
 /* shmem::accel_grouped (tree.h:428-440, tree.cc:626-897) with tidal = lazy = jerk = NULL and
@@ -163,6 +165,46 @@ gz_status gz_tree_accel(gz_ctx* ctx, const gz_tree* tree, const gz_particles* p,
  * device counterpart is shmem_cuda_accel_bruteforce (grav_cuda.cu:186-192), which omits G. */
 gz_status gz_direct_sum(gz_ctx* ctx, const gz_particles* p, const uint32_t* targets, size_t n_t,
                         double G, double* ax, double* ay, double* az);
+
+/* ---- learned opening criterion --------------------------------------------------------------
+ * No counterpart in the reference, whose opening parameter is one global tolerance
+ * (ErrTolForceAcc, shmem/mfm.cc:743-751, used by open_node's relative test, tree.h:134-135).
+ * A 16 -> 64 -> 64 -> 1 GELU network (bf16 weights, FP32 accumulation) maps the 16 features of a
+ * target group to log10 of that group's tolerance alpha_g; the walk then applies the reference
+ * test with aold_min scaled by alpha_g / alpha_ref. A group whose features lie outside the
+ * calibrated ranges, or whose prediction lies outside the trained label range, keeps the
+ * reference tolerance. Select it with gz_walk_opts.criterion = GZ_CRIT_LEARNED, .model = model.
+ * Parameters: W1[64][16], b1[64], W2[64][64], b2[64], w3[64], b3 (GZ_MLP_PARAMS floats). */
+#define GZ_MLP_FEATURES 16
+#define GZ_MLP_PARAMS 5313
+gz_status gz_model_create(gz_ctx* ctx, const float* host_params /* NULL: zeros */,
+                          gz_model** out);
+gz_status gz_model_free(gz_model* model);
+/* Upload FP32 master weights (resets the optimiser state) / read them back. Host arrays. */
+gz_status gz_model_set_params(gz_model* model, const float* host_params);
+gz_status gz_model_get_params(const gz_model* model, float* host_params);
+/* Host arrays of GZ_MLP_FEATURES bounds; labels and margin in log10(alpha). The group tolerance
+ * is 10^(prediction - margin); log10_alpha_ref is the tolerance aold was made with. */
+gz_status gz_model_set_calibration(gz_model* model, const float* feat_lo, const float* feat_hi,
+                                   float label_lo, float label_hi, float margin,
+                                   float log10_alpha_ref);
+/* Device arrays: X is B x GZ_MLP_FEATURES, out/T are B. */
+gz_status gz_model_forward(gz_ctx* ctx, const gz_model* model, const float* X, size_t B,
+                           float* out);
+/* Gradient of the mean squared error sum_b (y_b - T_b)^2 / B into grad_out (device,
+ * GZ_MLP_PARAMS, may be NULL) and the loss into *loss_out (host, may be NULL). */
+gz_status gz_model_grad(gz_ctx* ctx, gz_model* model, const float* X, const float* T, size_t B,
+                        float* grad_out, float* loss_out);
+/* One Adam step (beta 0.9 / 0.999, eps 1e-8) on the batch's gradient. */
+gz_status gz_model_train_step(gz_ctx* ctx, gz_model* model, const float* X, const float* T,
+                              size_t B, float lr, float* loss_out);
+/* The groups gz_tree_accel_grouped would form for these targets (batch 0, group_size as given)
+ * and their features: feats (device, room for n_t x GZ_MLP_FEATURES), group_start (device, room
+ * for n_t + 1), *n_groups (host). aold as for the walk (required). */
+gz_status gz_tree_group_features(gz_ctx* ctx, const gz_tree* tree, const gz_particles* p,
+                                 const uint32_t* targets, size_t n_t, int group_size,
+                                 const double* aold, float* feats, int* group_start,
+                                 size_t* n_groups);
 
 /* ---- caller side: compute_gravity + kick (mfm.cc:679-905, 3699-3813) ---------------------- */
 /* Active set in the tree's Morton order (mfm.cc:700-718). active: n_active particle indices in

@@ -1,0 +1,109 @@
+"""CPU oracle for the learned opening-parameter network (test infrastructure only).
+
+A float64 numpy restatement of the device MLP (gizmo_shared_memory_amd/csrc/gz_learned.hip):
+
+    16 features -> 64 (GELU) -> 64 (GELU) -> 1
+
+with the exact GELU, x * (1 + erf(x / sqrt 2)) / 2, and the mean-squared-error loss
+L = sum_b (y_b - t_b)^2 / B. The device keeps FP32 master weights, computes in FP32 from bf16
+copies of the weights, and accumulates in FP32; this oracle takes the same bf16-rounded weights
+and works in float64, so the two agree to FP32 rounding.
+
+Parameter layout (5,313 floats, the device's): W1[64][16], b1[64], W2[64][64], b2[64], w3[64],
+b3[1].
+
+Pinning: the forward pass against an independent scalar loop (tests), the backward pass against
+central finite differences of this forward pass, and the bf16 rounding against numpy's float32
+bit pattern truncated with round-to-nearest-even.
+"""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+
+N_IN, N_H = 16, 64
+SIZES = {"W1": (N_H, N_IN), "b1": (N_H,), "W2": (N_H, N_H), "b2": (N_H,), "w3": (N_H,),
+         "b3": (1,)}
+N_PARAMS = sum(int(np.prod(s)) for s in SIZES.values())  # 5313
+
+_erf = np.vectorize(math.erf)
+
+
+def unpack(theta: np.ndarray) -> dict:
+    out, o = {}, 0
+    for k, s in SIZES.items():
+        n = int(np.prod(s))
+        out[k] = np.asarray(theta[o:o + n], dtype=np.float64).reshape(s)
+        o += n
+    return out
+
+
+def pack(p: dict) -> np.ndarray:
+    return np.concatenate([np.asarray(p[k], dtype=np.float64).ravel() for k in SIZES])
+
+
+def bf16_round(x: np.ndarray) -> np.ndarray:
+    """float32 -> bf16 -> float32 with round-to-nearest-even (NaN-free inputs)."""
+    u = np.ascontiguousarray(x, dtype=np.float32).view(np.uint32).astype(np.uint64)
+    r = ((u + 0x7FFF + ((u >> 16) & 1)) >> 16) << 16
+    return r.astype(np.uint32).view(np.float32)
+
+
+def gelu(x):
+    return 0.5 * x * (1.0 + _erf(x / math.sqrt(2.0)))
+
+
+def gelu_grad(x):
+    return 0.5 * (1.0 + _erf(x / math.sqrt(2.0))) + x * np.exp(-0.5 * x * x) / math.sqrt(2 * math.pi)
+
+
+def forward(theta, X):
+    p = unpack(theta)
+    X = np.asarray(X, dtype=np.float64)
+    z1 = X @ p["W1"].T + p["b1"]
+    h1 = gelu(z1)
+    z2 = h1 @ p["W2"].T + p["b2"]
+    h2 = gelu(z2)
+    y = h2 @ p["w3"] + p["b3"][0]
+    return y, (X, z1, h1, z2, h2)
+
+
+def loss_and_grad(theta, X, t):
+    """MSE loss and its gradient with respect to every parameter (backpropagation)."""
+    p = unpack(theta)
+    y, (X, z1, h1, z2, h2) = forward(theta, X)
+    B = X.shape[0]
+    r = y - np.asarray(t, dtype=np.float64)
+    loss = float(np.sum(r * r) / B)
+    dy = 2.0 * r / B
+    g = {"w3": h2.T @ dy, "b3": np.array([dy.sum()])}
+    dz2 = np.outer(dy, p["w3"]) * gelu_grad(z2)
+    g["W2"] = dz2.T @ h1
+    g["b2"] = dz2.sum(0)
+    dh1 = dz2 @ p["W2"]
+    dz1 = dh1 * gelu_grad(z1)
+    g["W1"] = dz1.T @ X
+    g["b1"] = dz1.sum(0)
+    return loss, pack(g)
+
+
+def adam_step(theta, m, v, grad, t, lr, beta1=0.9, beta2=0.999, eps=1e-8):
+    """One Adam update (bias-corrected), returning new (theta, m, v)."""
+    m = beta1 * m + (1 - beta1) * grad
+    v = beta2 * v + (1 - beta2) * grad * grad
+    mh = m / (1 - beta1 ** t)
+    vh = v / (1 - beta2 ** t)
+    return theta - lr * mh / (np.sqrt(vh) + eps), m, v
+
+
+def init_params(seed=0):
+    """He-style uniform initialisation; the device is seeded from these values."""
+    rng = np.random.default_rng(seed)
+    p = {"W1": rng.uniform(-1, 1, (N_H, N_IN)) * math.sqrt(3.0 / N_IN),
+         "b1": np.zeros(N_H),
+         "W2": rng.uniform(-1, 1, (N_H, N_H)) * math.sqrt(3.0 / N_H),
+         "b2": np.zeros(N_H),
+         "w3": rng.uniform(-1, 1, N_H) * math.sqrt(3.0 / N_H),
+         "b3": np.zeros(1)}
+    return pack(p)

@@ -1,0 +1,71 @@
+"""The MLP oracle (oracle/mlp_oracle.py) pinned by checks that share none of its code paths:
+central finite differences for the backward pass, an explicit scalar loop for the forward pass,
+torch's bfloat16 conversion for the rounding, and the closed form of Adam's first step."""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+import pytest
+
+from oracle import mlp_oracle as mo
+
+
+def test_param_count_matches_layout():
+    assert mo.N_PARAMS == 5313
+    th = mo.init_params(1)
+    assert th.shape == (5313,)
+    assert np.array_equal(mo.pack(mo.unpack(th)), th)
+
+
+def test_forward_matches_scalar_loop():
+    rng = np.random.default_rng(0)
+    th = mo.init_params(2)
+    X = rng.normal(size=(5, 16))
+    y, _ = mo.forward(th, X)
+    p = mo.unpack(th)
+    for b in range(5):
+        h1 = []
+        for i in range(64):
+            z = p["b1"][i] + sum(p["W1"][i, k] * X[b, k] for k in range(16))
+            h1.append(0.5 * z * (1 + math.erf(z / math.sqrt(2))))
+        out = p["b3"][0]
+        for i in range(64):
+            z = p["b2"][i] + sum(p["W2"][i, j] * h1[j] for j in range(64))
+            out += p["w3"][i] * 0.5 * z * (1 + math.erf(z / math.sqrt(2)))
+        assert y[b] == pytest.approx(out, rel=1e-12, abs=1e-12)
+
+
+def test_backward_matches_finite_differences():
+    rng = np.random.default_rng(3)
+    th = mo.init_params(4)
+    X = rng.normal(size=(7, 16))
+    t = rng.normal(size=7)
+    _, g = mo.loss_and_grad(th, X, t)
+    idx = rng.choice(mo.N_PARAMS, 60, replace=False)
+    idx = np.concatenate([idx, [0, 1023, 1024, 5312, 5311, 5248]])  # one of each block
+    h = 1e-6
+    for i in idx:
+        tp, tm = th.copy(), th.copy()
+        tp[i] += h
+        tm[i] -= h
+        fd = (mo.loss_and_grad(tp, X, t)[0] - mo.loss_and_grad(tm, X, t)[0]) / (2 * h)
+        assert g[i] == pytest.approx(fd, rel=1e-5, abs=1e-8), i
+
+
+def test_bf16_rounding_matches_torch():
+    torch = pytest.importorskip("torch")
+    rng = np.random.default_rng(5)
+    x = np.concatenate([rng.normal(size=1000).astype(np.float32) * 10.0 ** rng.uniform(-5, 5, 1000),
+                        np.array([1.0, -2.5, 3.0e-3, 65504.0], dtype=np.float32)]).astype(np.float32)
+    ours = mo.bf16_round(x)
+    ref = torch.from_numpy(x).to(torch.bfloat16).to(torch.float32).numpy()
+    assert np.array_equal(ours.view(np.uint32), ref.view(np.uint32))
+
+
+def test_adam_first_step_closed_form():
+    th = np.array([1.0, -2.0, 0.5])
+    g = np.array([0.3, -4.0, 1e-3])
+    new, m, v = mo.adam_step(th, np.zeros(3), np.zeros(3), g, 1, lr=0.01)
+    # bias-corrected first step is lr * g / (|g| + eps)
+    np.testing.assert_allclose(new, th - 0.01 * g / (np.abs(g) + 1e-8), rtol=1e-12)
