@@ -53,6 +53,8 @@ def err_stats(e):
 
 
 def time_walk(gz, T, P, targets, aold, steps, warmup, **kw):
+    """Mean wall time per step (host clock around the calls, synchronised) and the walk kernel's
+    mean HIP-event time over the same steps."""
     import torch
     out = None
     for _ in range(warmup):
@@ -136,6 +138,8 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--group-sizes", default="0,16,32,64")
+    ap.add_argument("--model", default=os.path.join(ROOT, "gizmo_shared_memory_amd",
+                                                     "learned_criterion.json"))
     ap.add_argument("--copy-mb", type=int, default=0,
                     help="after the walks, run one plain device copy of this many MiB (a known "
                          "traffic reference for hardware byte counters)")
@@ -178,32 +182,59 @@ def main():
     ex = [v.cpu().numpy() for v in ex]
     log(f"direct sum on {len(probes)} probes: {time.perf_counter() - t0:.2f} s")
 
-    results = {}
-    # plain GPU walks with the reference criterion
-    for prec in ("fp64", "fp32"):
-        sizes = [0] if prec == "fp64" else [int(s) for s in args.group_sizes.split(",")]
-        best = None
-        for gs in sizes:
-            (ax, ay, az, c), dt, kms = time_walk(gz, T, P, targets, aold, 1, 1,
-                                                 precision=prec, group_size=gs)
-            log(f"tune {prec} group_size={gs}: {dt * 1e3:.2f} ms/step, kernel {kms:.2f} ms")
-            if best is None or dt < best[1]:
-                best = (gs, dt)
-        gs = best[0]
-        (ax, ay, az, c), dt, kms = time_walk(gz, T, P, targets, aold, args.steps, args.warmup,
-                                             precision=prec, group_size=gs)
-        cnt = gz.accel_grouped(T, P, targets, 0.0, 1.0, 0, aold=aold, precision=prec,
-                               group_size=gs, counts=True)[3]
+    def measure(label, prec, gs, steps, warmup, scale=1.0, model=None):
+        ao = aold if scale == 1.0 else aold * scale
+        (ax, ay, az, c), dt, kms = time_walk(gz, T, P, targets, ao, steps, warmup, precision=prec,
+                                             group_size=gs, model=model)
+        cnt = gz.accel_grouped(T, P, targets, 0.0, 1.0, 0, aold=ao, precision=prec,
+                               group_size=gs, counts=True, model=model)[3]
         inter = cnt["multipole"] + cnt["pairs"]
         e = rel_err(tuple(v.cpu().numpy()[probes] for v in (ax, ay, az)), ex)
-        results[prec] = {
-            "group_size": gs, "ms_per_step": dt * 1e3, "kernel_ms": kms,
-            "evals_per_s": nt / dt, "interactions_per_target": inter / nt,
-            "nodes_visited_per_target": cnt["nodes_visited"] / nt, "groups": cnt["groups"],
-            "err": err_stats(e),
-            "achieved_tflops": FLOPS_PER_INTERACTION * inter / (kms * 1e-3) / 1e12,
-        }
-        log(f"{prec}: {results[prec]}")
+        r = {"group_size": gs, "ms_per_step": dt * 1e3, "kernel_ms": kms,
+             "evals_per_s": nt / dt, "interactions_per_target": inter / nt,
+             "nodes_visited_per_target": cnt["nodes_visited"] / nt, "groups": cnt["groups"],
+             "err": err_stats(e),
+             "achieved_tflops": FLOPS_PER_INTERACTION * inter / (kms * 1e-3) / 1e12}
+        if model is not None:
+            r["fallback_groups"] = cnt["fallback_groups"]
+        if scale != 1.0:
+            r["err_tol_force_acc"] = ERR_TOL_FORCE_ACC * scale
+        log(f"{label}: {r}")
+        return r
+
+    results = {}
+    # plain GPU walks with the reference criterion; group size tuned on the FP32 walk
+    results["fp64"] = measure("fp64", "fp64", 0, args.steps, args.warmup)
+    best = None
+    for gs in [int(v) for v in args.group_sizes.split(",")]:
+        (_, dt, kms) = time_walk(gz, T, P, targets, aold, 1, 1, precision="fp32", group_size=gs)
+        log(f"tune fp32 group_size={gs}: {dt * 1e3:.2f} ms/step, kernel {kms:.2f} ms")
+        if best is None or dt < best[1]:
+            best = (gs, dt)
+    gsz = best[0]
+    results["fp32"] = measure("fp32", "fp32", gsz, args.steps, args.warmup)
+
+    # learned criterion (FP32 walk)
+    model = None
+    if os.path.exists(args.model):
+        from gizmo_shared_memory_amd import learned
+        model = learned.load_model(args.model)
+        results["learned_fp32"] = measure("learned_fp32", "fp32", gsz, args.steps, args.warmup,
+                                          model=model)
+        # the method without the network: one global tolerance for every group, the largest
+        # on a grid whose p99 error stays within the learned walk's
+        goal = max(results["learned_fp32"]["err"]["p99"], 1e-12)
+        glob = None
+        for k in range(0, 11):
+            sc = 2.0 ** (k / 2.0)
+            r = measure(f"global_tol_x{sc:.2f}", "fp32", gsz, 1, 1, scale=sc)
+            if r["err"]["p99"] <= goal:
+                glob = r
+            else:
+                break
+        results["global_tolerance_fp32"] = glob
+    else:
+        log(f"no learned model at {args.model}")
 
     if args.copy_mb > 0:
         src = torch.ones(args.copy_mb << 18, dtype=torch.float32, device="cuda:0")
@@ -214,7 +245,8 @@ def main():
         log(f"copy calibration: {args.copy_mb} MiB read + {args.copy_mb} MiB written")
         del src, dst
 
-    head = results["fp32"]
+    head = results.get("learned_fp32") or results["fp32"]
+    learned_ok = "learned_fp32" in results and head["err"]["p99"] <= 1e-2
     ref_e = rel_err(tuple(v.cpu().numpy()[probes] for v in boot[:3]), ex)
     cpu = None
     if not args.no_cpu:
@@ -241,10 +273,16 @@ def main():
         "config": {
             "workload": f"halo+disk N={args.n} (Hernquist a=1 70% + exp disk Rd=0.1 zd=0.01 30%),"
                         f" h=0.005, all active",
-            "walk": f"FP32 group-relative, reference criterion (relative {ERR_TOL_FORCE_ACC}, "
-                    f"aold from theta={THETA_BOOT} bootstrap), group_size={head['group_size']}",
-            "learned_criterion": None,
+            "walk": ("FP32 group-relative, learned per-group tolerance with reference "
+                     "fallback" if "learned_fp32" in results else
+                     f"FP32 group-relative, reference criterion (relative {ERR_TOL_FORCE_ACC}, "
+                     f"aold from theta={THETA_BOOT} bootstrap)") +
+                    f", group_size={head['group_size']}",
+            "learned_criterion": (os.path.relpath(args.model, ROOT)
+                                  if "learned_fp32" in results else None),
             "n_probes": len(probes),
+            "p99_rel_err": head["err"]["p99"],
+            "p99_within_1e-2": bool(head["err"]["p99"] <= 1e-2),
         },
         "roofline": {"bound": "mfma", "achieved": head["achieved_tflops"],
                      "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
@@ -258,6 +296,11 @@ def main():
     }
     if cpu is not None:
         line["gpu_over_cpu"] = head["evals_per_s"] / cpu["value"]
+    line["speedup_vs_plain_fp32"] = head["evals_per_s"] / results["fp32"]["evals_per_s"]
+    line["speedup_vs_plain_fp64"] = head["evals_per_s"] / results["fp64"]["evals_per_s"]
+    if results.get("global_tolerance_fp32"):
+        line["speedup_vs_global_tolerance"] = (head["evals_per_s"] /
+                                               results["global_tolerance_fp32"]["evals_per_s"])
     print(json.dumps(line), flush=True)
 
 

@@ -104,6 +104,8 @@ _PROTOS = {
     "gz_model_get_params": (C.c_int, [C.c_void_p, _fp]),
     "gz_model_set_calibration": (C.c_int, [C.c_void_p, _fp, _fp, C.c_float, C.c_float,
                                            C.c_float, C.c_float]),
+    "gz_model_set_input_map": (C.c_int, [C.c_void_p, _fp, _fp]),
+    "gz_model_set_loss": (C.c_int, [C.c_void_p, C.c_float]),
     "gz_model_forward": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t, C.c_void_p]),
     "gz_model_grad": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t,
                                 C.c_void_p, _fp]),

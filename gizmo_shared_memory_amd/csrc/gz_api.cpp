@@ -161,7 +161,7 @@ gz_status walk(gz_ctx* ctx, const gz_tree* tree, const gz_particles* p, const ui
             return GZ_OK;
         }
         gz_model* M = ex.model;
-        GZ_TRY(gz::launch_mlp_forward(feats, (int)ngroups, M->wbf, pred, st));
+        GZ_TRY(gz::launch_mlp_forward(feats, (int)ngroups, M->wbf, M->theta, M->xnorm, pred, st));
         GZ_TRY(gz::launch_group_scale((int)ngroups, feats, pred, M->frange, M->lab_lo, M->lab_hi,
                                       M->margin, M->log_alpha_ref, gi,
                                       reinterpret_cast<int*>(misc + 2), st));
@@ -491,7 +491,7 @@ gz_status gz_model_create(gz_ctx* ctx, const float* host_params, gz_model** out)
     if (!M) return fail(GZ_ERR_NOMEM, "gz_model_create: out of host memory");
     M->ctx = ctx;
     const size_t nf = (size_t)gz::kMlpParams;
-    const size_t bytes = 4 * nf * sizeof(float) + 256 + 2 * gz::kMlpIn * sizeof(float) +
+    const size_t bytes = 4 * nf * sizeof(float) + 256 + 4 * gz::kMlpIn * sizeof(float) +
                          nf * sizeof(uint16_t) + 1024;
     hipError_t e = hipMalloc(&M->mem, bytes);
     if (e != hipSuccess) {
@@ -505,15 +505,18 @@ gz_status gz_model_create(gz_ctx* ctx, const float* host_params, gz_model** out)
     M->grad = f + 3 * nf;
     M->loss = f + 4 * nf;
     M->frange = f + 4 * nf + 64;
-    M->wbf = reinterpret_cast<uint16_t*>(f + 4 * nf + 64 + 2 * gz::kMlpIn);
+    M->xnorm = M->frange + 2 * gz::kMlpIn;
+    M->wbf = reinterpret_cast<uint16_t*>(M->xnorm + 2 * gz::kMlpIn);
     hipStream_t st = ctx->stream;
     gz_status s = gz::fill_zero_u32(reinterpret_cast<uint32_t*>(f), 4 * nf + 64, st);
     if (s == GZ_OK && host_params) s = gz_model_set_params(M, host_params);
     if (s == GZ_OK) {
-        float fr[2 * gz::kMlpIn];
+        float fr[4 * gz::kMlpIn]; // open feature ranges, identity input map
         for (int k = 0; k < gz::kMlpIn; ++k) {
             fr[k] = -1e30f;
             fr[gz::kMlpIn + k] = 1e30f;
+            fr[2 * gz::kMlpIn + k] = 0.f;
+            fr[3 * gz::kMlpIn + k] = 1.f;
         }
         if (hipMemcpyAsync(M->frange, fr, sizeof fr, hipMemcpyHostToDevice, st) != hipSuccess ||
             hipStreamSynchronize(st) != hipSuccess)
@@ -578,12 +581,33 @@ gz_status gz_model_set_calibration(gz_model* M, const float* feat_lo, const floa
     return GZ_OK;
 }
 
+gz_status gz_model_set_input_map(gz_model* M, const float* mean, const float* scale) {
+    if (!M || !mean || !scale) return fail(GZ_ERR_INVALID, "gz_model_set_input_map: null argument");
+    float nm[2 * gz::kMlpIn];
+    for (int k = 0; k < gz::kMlpIn; ++k) {
+        nm[k] = mean[k];
+        nm[gz::kMlpIn + k] = scale[k];
+    }
+    hipStream_t st = M->ctx->stream;
+    GZ_HIP(hipMemcpyAsync(M->xnorm, nm, sizeof nm, hipMemcpyHostToDevice, st));
+    GZ_HIP(hipStreamSynchronize(st));
+    return GZ_OK;
+}
+
+gz_status gz_model_set_loss(gz_model* M, float quantile) {
+    if (!M) return fail(GZ_ERR_INVALID, "gz_model_set_loss: null model");
+    if (!(quantile >= 0.f && quantile < 1.f))
+        return fail(GZ_ERR_INVALID, "gz_model_set_loss: quantile %g outside [0, 1)", quantile);
+    M->tau = quantile;
+    return GZ_OK;
+}
+
 gz_status gz_model_forward(gz_ctx* ctx, const gz_model* M, const float* X, size_t B, float* out) {
     gz::clear_error();
     GZ_TRY(check_ctx(ctx));
     if (!M || (B && (!X || !out))) return fail(GZ_ERR_INVALID, "gz_model_forward: null argument");
     if (B >= 0x7FFFFFFFull) return fail(GZ_ERR_INVALID, "gz_model_forward: B too large");
-    return gz::launch_mlp_forward(X, (int)B, M->wbf, out, ctx->stream);
+    return gz::launch_mlp_forward(X, (int)B, M->wbf, M->theta, M->xnorm, out, ctx->stream);
 }
 
 gz_status gz_model_grad(gz_ctx* ctx, gz_model* M, const float* X, const float* T, size_t B,
@@ -594,7 +618,8 @@ gz_status gz_model_grad(gz_ctx* ctx, gz_model* M, const float* X, const float* T
     if (B == 0 || B >= 0x7FFFFFFFull) return fail(GZ_ERR_INVALID, "gz_model_grad: bad B");
     hipStream_t st = ctx->stream;
     GZ_TRY(gz::fill_zero_u32(reinterpret_cast<uint32_t*>(M->grad), gz::kMlpParams + 64, st));
-    GZ_TRY(gz::launch_mlp_grad(X, T, (int)B, M->wbf, M->grad, M->loss, st));
+    GZ_TRY(gz::launch_mlp_grad(X, T, (int)B, M->wbf, M->theta, M->xnorm, M->tau, M->grad,
+                               M->loss, st));
     if (grad_out)
         GZ_HIP(hipMemcpyAsync(grad_out, M->grad, gz::kMlpParams * sizeof(float),
                               hipMemcpyDeviceToDevice, st));

@@ -135,10 +135,11 @@ gz_status launch_direct(const gz_particles* p, const uint32_t* targets, size_t n
 gz_status launch_group_features(int ngroups, const GroupInfo* gi, const uint32_t* targets,
                                 const gz_particles* p, const double* amag, const ::gz_tree* t,
                                 float* feats, hipStream_t st);
-gz_status launch_mlp_forward(const float* X, int B, const uint16_t* wbf, float* out,
-                             hipStream_t st);
+gz_status launch_mlp_forward(const float* X, int B, const uint16_t* wbf, const float* theta,
+                             const float* xnorm, float* out, hipStream_t st);
 gz_status launch_mlp_grad(const float* X, const float* T, int B, const uint16_t* wbf,
-                          float* grad, float* loss, hipStream_t st);
+                          const float* theta, const float* xnorm, float tau, float* grad,
+                          float* loss, hipStream_t st);
 gz_status launch_adam(float* theta, float* m, float* v, const float* grad, float lr, float b1,
                       float b2, float eps, int step, uint16_t* wbf, hipStream_t st);
 gz_status launch_to_bf16(const float* theta, uint16_t* wbf, hipStream_t st);
@@ -166,8 +167,10 @@ struct gz_model {
     void* mem = nullptr;
     float *theta = nullptr, *m = nullptr, *v = nullptr, *grad = nullptr, *loss = nullptr;
     float* frange = nullptr; // feature lower bounds [16] then upper bounds [16]
+    float* xnorm = nullptr;  // input map x' = (x - mean) * scale: mean [16] then scale [16]
     uint16_t* wbf = nullptr;
     int step = 0;
+    float tau = 0.f; // loss: 0 = mean squared error, (0, 1) = pinball loss of that quantile
     float lab_lo = -30.f, lab_hi = 30.f, margin = 0.f, log_alpha_ref = 0.f;
 };
 

@@ -49,12 +49,11 @@ __global__ __launch_bounds__(256) void k_group_features(
     if (g >= ngroups) return;
     const GroupInfo G = gi[g];
     const int nt = G.thi - G.tlo;
-    double msum = 0, m2 = 0, amin = 1e300, amax = 0, a2 = 0;
+    double msum = 0, amin = 1e300, amax = 0, a2 = 0;
     for (int i = G.tlo + lane; i < G.thi; i += 64) {
         const uint32_t p = targets[i];
         const double mm = m[p];
         msum += mm;
-        m2 += mm * mm;
         const double am = amag ? amag[i] : 0.0;
         amin = fmin(amin, am);
         amax = fmax(amax, am);
@@ -63,7 +62,6 @@ __global__ __launch_bounds__(256) void k_group_features(
 #pragma unroll
     for (int d = 32; d >= 1; d >>= 1) {
         msum += __shfl_xor(msum, d);
-        m2 += __shfl_xor(m2, d);
         amin = fmin(amin, __shfl_xor(amin, d));
         amax = fmax(amax, __shfl_xor(amax, d));
         a2 += __shfl_xor(a2, d);
@@ -83,44 +81,61 @@ __global__ __launch_bounds__(256) void k_group_features(
     int depth = 0, leaf = leaf_of[targets[G.tlo]];
     const double leaf_len = (leaf >= 0) ? (double)wn[leaf].len : side;
     for (int q = leaf; q >= 0 && parent[q] >= 0; q = parent[q]) ++depth;
+    // All features are ratios of the group's own scales (extent, leaf cell, softening, field,
+    // mass) and of its position in the system, so none depends on the units or the box.
+    const double rho_g = mg / (ext * ext * ext), rho_c = Mr / (dist * dist * dist);
+    const double amn = fmax(amin, 1e-300);
     float f[kIn];
     f[0] = flog10((double)nt);
-    f[1] = flog10(ext / side);
-    f[2] = flog10(amin * side * side / Mr);
-    f[3] = flog10(amax / fmax(amin, 1e-300));
-    f[4] = flog10(G.emax / side);
-    f[5] = flog10(G.emax / ext);
-    f[6] = flog10(mg / Mr);
-    f[7] = flog10((mg / (ext * ext * ext)) / (Mr / (side * side * side)));
-    f[8] = flog10(dist / side);
-    f[9] = flog10(amin * ext * ext / mg);
-    f[10] = (float)depth / 20.0f;
-    f[11] = flog10(emn / ext);
-    f[12] = flog10(arms / fmax(amin, 1e-300));
-    f[13] = flog10(ext / dist);
-    f[14] = (float)(fabs(cz) / dist);
-    f[15] = flog10(leaf_len / ext);
+    f[1] = flog10(ext / leaf_len);
+    f[2] = flog10(amn * ext * ext / mg);       // external field / self field
+    f[3] = flog10(amax / amn);
+    f[4] = flog10(G.emax / ext);
+    f[5] = flog10(emn / ext);                  // aspect of the group box
+    f[6] = flog10(arms / amn);
+    f[7] = (float)depth / 20.0f;
+    f[8] = flog10(ext / dist);                 // angular size seen from the centre of mass
+    f[9] = (float)(fabs(cz) / dist);
+    f[10] = flog10(Mr / (dist * dist * amn));  // bulk monopole field / actual field
+    f[11] = flog10(mg / Mr);
+    f[12] = flog10(rho_g / rho_c);             // local over enclosed-scale density
+    f[13] = flog10(leaf_len / dist);
+    f[14] = flog10(G.emax / dist);
+    f[15] = flog10(G.emax / leaf_len);
     float* o = feats + (size_t)g * kIn;
 #pragma unroll
     for (int k = 0; k < kIn; ++k) o[k] = f[k];
 }
 
 // ---- inference -------------------------------------------------------------------------------------
-__device__ __forceinline__ void stage_weights(const uint16_t* __restrict__ wbf, float* sW) {
-    for (int k = threadIdx.x; k < kMlpParams; k += blockDim.x) sW[k] = bf16_to_f32(wbf[k]);
+// Weight matrices from their bf16 copy, biases from the FP32 masters, and the input map
+// x' = (x - mean) * scale (xnorm = mean[16], scale[16]).
+__device__ __forceinline__ bool is_bias(int k) {
+    return (k >= kOffB1 && k < kOffW2) || (k >= kOffB2 && k < kOffW3) || k == kOffB3;
+}
+__device__ __forceinline__ void stage_weights(const uint16_t* __restrict__ wbf,
+                                             const float* __restrict__ theta,
+                                             const float* __restrict__ xnorm, float* sW,
+                                             float* sN) {
+    for (int k = threadIdx.x; k < kMlpParams; k += blockDim.x)
+        sW[k] = is_bias(k) ? theta[k] : bf16_to_f32(wbf[k]);
+    for (int k = threadIdx.x; k < 2 * kIn; k += blockDim.x) sN[k] = xnorm[k];
     __syncthreads();
 }
 
 __global__ __launch_bounds__(256) void k_mlp_forward(const float* __restrict__ X, int B,
                                                      const uint16_t* __restrict__ wbf,
+                                                     const float* __restrict__ theta,
+                                                     const float* __restrict__ xnorm,
                                                      float* __restrict__ out) {
     __shared__ float sW[kMlpParams];
-    stage_weights(wbf, sW);
+    __shared__ float sN[2 * kIn];
+    stage_weights(wbf, theta, xnorm, sW, sN);
     const int b = blockIdx.x * 256 + threadIdx.x;
     if (b >= B) return;
     float xin[kIn];
 #pragma unroll
-    for (int k = 0; k < kIn; ++k) xin[k] = X[(size_t)b * kIn + k];
+    for (int k = 0; k < kIn; ++k) xin[k] = (X[(size_t)b * kIn + k] - sN[k]) * sN[kIn + k];
     float h1[kH];
 #pragma unroll 4
     for (int i = 0; i < kH; ++i) {
@@ -140,20 +155,26 @@ __global__ __launch_bounds__(256) void k_mlp_forward(const float* __restrict__ X
 }
 
 // ---- backpropagation -------------------------------------------------------------------------------
-// One wavefront per block of 64 samples. Gradients of L = sum_b (y_b - t_b)^2 / B_total are summed
-// over the block in LDS and added to grad[] with one atomic per parameter per block.
+// One wavefront per block of 64 samples. Gradients of the loss are summed over the block in LDS
+// and added to grad[] with one atomic per parameter per block. The loss is the mean squared error
+// L = sum_b (y_b - t_b)^2 / B (tau <= 0), or the pinball loss of the tau-quantile,
+// L = sum_b (t_b - y_b) (tau - [t_b < y_b]) / B (0 < tau < 1), which makes y the tau-quantile of
+// the label given the features.
 constexpr int kGB = 64;
 __global__ __launch_bounds__(64) void k_mlp_grad(const float* __restrict__ X,
                                                  const float* __restrict__ T, int B, float inv_b,
                                                  const uint16_t* __restrict__ wbf,
+                                                 const float* __restrict__ theta,
+                                                 const float* __restrict__ xnorm, float tau,
                                                  float* __restrict__ grad,
                                                  float* __restrict__ loss) {
     __shared__ float sW[kMlpParams];
+    __shared__ float sN[2 * kIn];
     __shared__ float sH1[kGB][kH + 1]; // h1, later dz1
     __shared__ float sZ2[kGB][kH + 1]; // z2, later dz2
     __shared__ float sX[kGB][kIn + 1];
     __shared__ float sDy[kGB];
-    stage_weights(wbf, sW);
+    stage_weights(wbf, theta, xnorm, sW, sN);
     const int tid = threadIdx.x;
     const int b = blockIdx.x * kGB + tid;
     const bool valid = b < B;
@@ -161,7 +182,7 @@ __global__ __launch_bounds__(64) void k_mlp_grad(const float* __restrict__ X,
     float xin[kIn];
 #pragma unroll
     for (int k = 0; k < kIn; ++k) {
-        xin[k] = valid ? X[(size_t)b * kIn + k] : 0.f;
+        xin[k] = valid ? (X[(size_t)b * kIn + k] - sN[k]) * sN[kIn + k] : 0.f;
         sX[tid][k] = xin[k];
     }
     float h1[kH];
@@ -182,9 +203,15 @@ __global__ __launch_bounds__(64) void k_mlp_grad(const float* __restrict__ X,
         yy = __builtin_fmaf(sW[kOffW3 + i], gelu(zz), yy);
     }
     const float r = valid ? (yy - T[b]) : 0.f;
-    const float dy = 2.0f * r * inv_b;
+    float dy, l2;
+    if (tau > 0.f) {
+        dy = valid ? ((r > 0.f ? 1.0f : 0.0f) - tau) * inv_b : 0.f;
+        l2 = (r > 0.f) ? (1.0f - tau) * r : -tau * r;
+    } else {
+        dy = 2.0f * r * inv_b;
+        l2 = r * r;
+    }
     sDy[tid] = dy;
-    float l2 = r * r;
 #pragma unroll
     for (int d = 32; d >= 1; d >>= 1) l2 += __shfl_xor(l2, d);
     if (tid == 0) atomicAdd(loss, l2 * inv_b);
@@ -296,6 +323,7 @@ __global__ __launch_bounds__(256) void k_group_scale(int ngroups, const float* _
     bool ok = true;
 #pragma unroll
     for (int k = 0; k < kIn; ++k) ok = ok && (f[k] >= frange[k]) && (f[k] <= frange[kIn + k]);
+    // frange holds the trusted band (the caller widens the training range as it sees fit)
     const float p = pred[g];
     ok = ok && (p >= lab_lo) && (p <= lab_hi);
     const float s = ok ? exp10f(p - margin - log_alpha_ref) : 1.0f;
@@ -315,19 +343,20 @@ gz_status launch_group_features(int ngroups, const GroupInfo* gi, const uint32_t
     return GZ_OK;
 }
 
-gz_status launch_mlp_forward(const float* X, int B, const uint16_t* wbf, float* out,
-                             hipStream_t st) {
+gz_status launch_mlp_forward(const float* X, int B, const uint16_t* wbf, const float* theta,
+                             const float* xnorm, float* out, hipStream_t st) {
     if (B <= 0) return GZ_OK;
-    k_mlp_forward<<<(unsigned)((B + 255) / 256), 256, 0, st>>>(X, B, wbf, out);
+    k_mlp_forward<<<(unsigned)((B + 255) / 256), 256, 0, st>>>(X, B, wbf, theta, xnorm, out);
     GZ_HIP(hipGetLastError());
     return GZ_OK;
 }
 
 gz_status launch_mlp_grad(const float* X, const float* T, int B, const uint16_t* wbf,
-                          float* grad, float* loss, hipStream_t st) {
+                          const float* theta, const float* xnorm, float tau, float* grad,
+                          float* loss, hipStream_t st) {
     if (B <= 0) return GZ_OK;
     k_mlp_grad<<<(unsigned)((B + kGB - 1) / kGB), 64, 0, st>>>(X, T, B, 1.0f / (float)B, wbf,
-                                                                grad, loss);
+                                                                theta, xnorm, tau, grad, loss);
     GZ_HIP(hipGetLastError());
     return GZ_OK;
 }

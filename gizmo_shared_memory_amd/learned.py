@@ -1,0 +1,276 @@
+"""Training and calibration of the learned opening criterion.
+
+The reference opens node X for a target when M_X L_X^2 > r^4 * aold, aold = ErrTolForceAcc *
+|a_prev| / G (shmem/tree.h:134-135, shmem/mfm.cc:743-751), with one global ErrTolForceAcc
+(alpha_ref = 0.0025 by default). Here each walk group gets its own tolerance alpha_g, chosen by a
+small network from 16 features of the group so that the group's force error sits at a target
+(1e-2 relative by default). Everything that computes runs on the device: the walks, the FP64
+direct-sum judge, the features, and the network's forward and backward passes and Adam.
+
+Samples. For a particle set: build the tree, run the reference bootstrap walk (theta = 0.7) for
+aold, form the groups the FP32 walk uses, and pick some groups with up to K probe targets each.
+The probes' exact accelerations come from the direct sum. The FP32 walk is then run with every
+group at each tolerance of a grid (aold scaled uniformly), which gives each sampled group its error
+e_g(alpha) = max over its probes of |a_walk - a_exact| / |a_exact| at every grid point: one
+(group, tolerance, tree-vs-direct error) sample per grid point. The label of a group is
+log10(alpha*), the largest tolerance whose error stays at the target, interpolated in log-log
+between grid points and clipped to the grid.
+
+Model file: JSON with the 5,313 FP32 parameters, the input map (mean and 1/sd of the training
+features, applied on the device before the first layer), the feature ranges seen in training, the
+label range, the calibrated margin and provenance.
+"""
+from __future__ import annotations
+
+import json
+import math
+import os
+import time
+
+import numpy as np
+import torch
+
+from . import ics
+from . import tree as gz
+
+ALPHA_REF = 0.0025
+THETA_BOOT = 0.7
+ALPHA_GRID = ALPHA_REF * 2.0 ** (np.arange(-2, 17) / 2.0)  # 0.00125 .. 0.64
+DEFAULT_MODEL = os.path.join(os.path.dirname(os.path.abspath(__file__)), "learned_criterion.json")
+
+
+def log(*a):
+    print("[learned]", *a, flush=True)
+
+
+def _rel_err(a, b):
+    d = torch.linalg.norm(a - b, dim=1)
+    return d / torch.clamp(torch.linalg.norm(b, dim=1), min=1e-300)
+
+
+def reference_state(P, group_size):
+    """Tree, Morton-ordered targets and the reference bootstrap aold (no-G units, G = 1)."""
+    T = gz.build(P)
+    tg = T.device_orderbuf()
+    boot = gz.accel_grouped(T, P, tg, THETA_BOOT, 1.0, 0)
+    aold = ALPHA_REF * torch.linalg.norm(torch.stack(boot, 1), dim=1)
+    return T, tg, aold
+
+
+def group_errors(P, T, tg, aold, gstart, sel, probes_per_group, rng, group_size, alphas,
+                 G=1.0):
+    """Errors e_g(alpha) of the FP32 walk for the selected groups at each tolerance."""
+    gs = gstart.cpu().numpy().astype(np.int64)
+    slots, owner = [], []
+    for k, g in enumerate(sel):
+        lo, hi = gs[g], gs[g + 1]
+        pick = rng.choice(np.arange(lo, hi), min(probes_per_group, hi - lo), replace=False)
+        slots.append(pick)
+        owner.append(np.full(pick.shape[0], k))
+    slots = np.concatenate(slots)
+    owner = np.concatenate(owner)
+    sl = torch.as_tensor(slots, device=P.device)
+    ptg = tg.view(torch.int32)[sl].view(torch.uint32)
+    ex = torch.stack(gz.accel_brute(P, ptg, G), 1)
+    own = torch.as_tensor(owner, device=P.device)
+    errs = np.zeros((len(sel), len(alphas)))
+    inter = np.zeros(len(alphas))
+    for j, a in enumerate(alphas):
+        out = gz.accel_grouped(T, P, tg, 0.0, G, 0, aold=aold * (a / ALPHA_REF), precision="fp32",
+                               group_size=group_size, counts=True)
+        e = _rel_err(torch.stack(out[:3], 1)[sl], ex)
+        m = torch.zeros(len(sel), dtype=torch.float64, device=P.device)
+        m.scatter_reduce_(0, own, e, reduce="amax")
+        errs[:, j] = m.cpu().numpy()
+        inter[j] = (out[3]["multipole"] + out[3]["pairs"]) / tg.shape[0]
+    return errs, inter, slots
+
+
+def labels_from_errors(errs, alphas, target):
+    """log10 of the largest tolerance whose error stays at the target (log-log interpolation
+    between grid points, clipped to the grid)."""
+    la = np.log10(alphas)
+    le = np.log10(np.maximum(errs, 1e-30))
+    lt = math.log10(target)
+    out = np.empty(errs.shape[0])
+    for i in range(errs.shape[0]):
+        ok = le[i] <= lt
+        if ok.all():
+            out[i] = la[-1]
+            continue
+        if not ok[0]:
+            out[i] = la[0]
+            continue
+        j = int(np.argmin(ok)) - 1  # last ok before the first failure
+        e0, e1 = le[i, j], le[i, j + 1]
+        f = 0.0 if e1 <= e0 else (lt - e0) / (e1 - e0)
+        out[i] = la[j] + min(max(f, 0.0), 1.0) * (la[j + 1] - la[j])
+    return out
+
+
+def make_samples(P, n_groups, probes_per_group, group_size, target, seed, alphas=ALPHA_GRID):
+    rng = np.random.default_rng(seed)
+    T, tg, aold = reference_state(P, group_size)
+    feats, gstart = gz.group_features(T, P, tg, aold, group_size)
+    ng = feats.shape[0]
+    sel = np.sort(rng.choice(ng, min(n_groups, ng), replace=False))
+    errs, inter, _ = group_errors(P, T, tg, aold, gstart, sel, probes_per_group, rng, group_size,
+                                  alphas)
+    lab = labels_from_errors(errs, alphas, target)
+    return feats[torch.as_tensor(sel, device=P.device)].cpu().numpy(), lab, errs, inter
+
+
+TRAIN_SETS = (
+    # (name, generator, kwargs, groups sampled, probes per group)
+    ("plummer_64k", ics.plummer, dict(n=65536, eps=0.01, seed=101), 4000, 4),
+    ("plummer_1m", ics.plummer, dict(n=1 << 20, eps=0.005, seed=102), 30000, 4),
+    ("hernquist_4m", ics.hernquist, dict(n=1 << 22, eps=0.005, seed=103), 40000, 4),
+    ("hernquist_1m_eps02", ics.hernquist, dict(n=1 << 20, eps=0.02, seed=104), 20000, 4),
+    ("clustered_4m", ics.clustered, dict(n=1 << 22, eps=0.002, seed=105), 40000, 4),
+    ("clustered_16m", ics.clustered, dict(n=1 << 24, eps=0.001, seed=106), 40000, 4),
+)
+VALID_SETS = (
+    ("plummer_256k_val", ics.plummer, dict(n=1 << 18, eps=0.01, seed=201), 8000, 4),
+    ("hernquist_2m_val", ics.hernquist, dict(n=1 << 21, eps=0.005, seed=202), 15000, 4),
+    ("clustered_2m_val", ics.clustered, dict(n=1 << 21, eps=0.002, seed=203), 15000, 4),
+)
+
+
+def build_dataset(sets, group_size, target, device="cuda:0"):
+    X, Y, E, names = [], [], [], []
+    for name, gen, kw, ngroups, ppg in sets:
+        t0 = time.time()
+        x, y, z, m, soft = gen(**kw)
+        P = gz.Particles(x, y, z, m, soft, device=device)
+        f, lab, errs, inter = make_samples(P, ngroups, ppg, group_size, target, kw["seed"])
+        X.append(f)
+        Y.append(lab)
+        E.append(errs)
+        names += [name] * len(lab)
+        log(f"{name}: {len(lab)} groups x {len(ALPHA_GRID)} tolerances, labels "
+            f"{np.percentile(lab, [5, 50, 95]).round(3)}, interactions/target at the grid "
+            f"ends {inter[0]:.0f}..{inter[-1]:.0f} ({time.time() - t0:.1f} s)")
+        del P
+        torch.cuda.empty_cache()
+    return np.concatenate(X), np.concatenate(Y), np.concatenate(E), np.array(names)
+
+
+def _init_params(seed):
+    rng = np.random.default_rng(seed)
+    w1 = rng.uniform(-1, 1, (64, 16)) * math.sqrt(3.0 / 16)
+    w2 = rng.uniform(-1, 1, (64, 64)) * math.sqrt(3.0 / 64)
+    w3 = rng.uniform(-1, 1, 64) * math.sqrt(3.0 / 64)
+    return np.concatenate([w1.ravel(), np.zeros(64), w2.ravel(), np.zeros(64), w3, [0.0]])
+
+
+def train_network(X, Y, steps=4000, batch=4096, lr=2e-3, seed=0, quantile=0.0,
+                  device="cuda:0"):
+    """Fit the device network to (X, Y). Inputs are standardised on the device by the model's
+    input map (mean and 1/sd of the training features)."""
+    mu = X.mean(0)
+    sd = np.maximum(X.std(0), 1e-6)
+    Xd = torch.as_tensor(X.astype(np.float32), device=device)
+    Yd = torch.as_tensor(Y.astype(np.float32), device=device)
+    th0 = _init_params(seed).astype(np.float32)
+    th0[-1] = float(Y.mean())
+    M = gz.Model(th0)
+    M.set_input_map(mu, 1.0 / sd)
+    M.set_loss(quantile)
+    g = torch.Generator(device="cpu").manual_seed(seed)
+    hist = []
+    n = Xd.shape[0]
+    for s in range(steps):
+        idx = torch.randint(0, n, (min(batch, n),), generator=g).to(device)
+        lr_s = lr * 0.5 * (1.0 + math.cos(math.pi * s / steps))  # cosine decay
+        loss = M.train_step(Xd[idx].contiguous(), Yd[idx].contiguous(), lr_s)
+        if s % 200 == 0 or s == steps - 1:
+            hist.append((s, loss))
+    return M, M.params().astype(np.float64), mu, 1.0 / sd, hist
+
+
+def save_model(path, params, mean, scale, feat_lo, feat_hi, label_lo, label_hi, margin, meta):
+    d = dict(format="gz_learned_criterion/1", alpha_ref=ALPHA_REF, n_features=16,
+             params=[float(v) for v in np.asarray(params, dtype=np.float32)],
+             input_mean=[float(v) for v in mean], input_scale=[float(v) for v in scale],
+             feat_lo=[float(v) for v in feat_lo], feat_hi=[float(v) for v in feat_hi],
+             label_lo=float(label_lo), label_hi=float(label_hi), margin=float(margin), meta=meta)
+    with open(path, "w") as f:
+        json.dump(d, f)
+
+
+def load_model(path=DEFAULT_MODEL, margin=None) -> gz.Model:
+    """The device model with its calibration, from a model file."""
+    with open(path) as f:
+        d = json.load(f)
+    M = gz.Model(np.asarray(d["params"], dtype=np.float32))
+    M.set_input_map(d["input_mean"], d["input_scale"])
+    M.set_calibration(d["feat_lo"], d["feat_hi"], d["label_lo"], d["label_hi"],
+                      d["margin"] if margin is None else margin, math.log10(d["alpha_ref"]))
+    M.meta = d.get("meta", {})
+    return M
+
+
+def calibrate_margin(M, X, E, alphas, target_p99, feat_lo, feat_hi, label_lo, label_hi):
+    """Smallest margin (log10 units) whose predicted tolerances keep the p99 of the validation
+    groups' errors at or below target_p99, reading each group's error off its measured curve."""
+    la = np.log10(alphas)
+    pred = M.forward(torch.as_tensor(X.astype(np.float32), device="cuda:0")).cpu().numpy()
+    ok = np.all((X >= feat_lo) & (X <= feat_hi), axis=1) & (pred >= label_lo) & (pred <= label_hi)
+    le = np.log10(np.maximum(E, 1e-30))
+    ref_j = int(np.argmin(np.abs(alphas - ALPHA_REF)))
+    for margin in np.arange(0.0, 1.5001, 0.02):
+        a = np.where(ok, pred - margin, la[ref_j])
+        e = np.array([10 ** np.interp(a[i], la, le[i]) for i in range(len(a))])
+        if np.percentile(e, 99) <= target_p99:
+            return float(margin), float(np.percentile(e, 99)), float(ok.mean())
+    return 1.5, float("nan"), float(ok.mean())
+
+
+def main():
+    import argparse
+    ap = argparse.ArgumentParser(description="train the learned opening criterion on the GPU")
+    ap.add_argument("--out", default=DEFAULT_MODEL)
+    ap.add_argument("--group-size", type=int, default=32)
+    ap.add_argument("--target", type=float, default=1e-2)
+    ap.add_argument("--p99", type=float, default=8e-3)
+    ap.add_argument("--steps", type=int, default=4000)
+    ap.add_argument("--quantile", type=float, default=0.1,
+                    help="fit this quantile of log10(alpha*) (pinball loss); 0 = mean squared")
+    ap.add_argument("--quick", action="store_true", help="small sets (for a smoke run)")
+    args = ap.parse_args()
+    t0 = time.time()
+    sets, vsets = TRAIN_SETS, VALID_SETS
+    if args.quick:
+        sets, vsets = TRAIN_SETS[:1], VALID_SETS[:1]
+    X, Y, E, names = build_dataset(sets, args.group_size, args.target)
+    Xv, Yv, Ev, _ = build_dataset(vsets, args.group_size, args.target)
+    log(f"training samples: {len(Y)} groups x {len(ALPHA_GRID)} tolerances = "
+        f"{len(Y) * len(ALPHA_GRID)} (group, tolerance, error) samples; validation {len(Yv)}")
+    M, params, mu, scale, hist = train_network(X, Y, steps=args.steps, quantile=args.quantile)
+    pv = M.forward(torch.as_tensor(Xv.astype(np.float32), device="cuda:0")).cpu().numpy()
+    rmse_v = float(np.sqrt(np.mean((pv - Yv) ** 2)))
+    rmse_c = float(np.sqrt(np.mean((Yv - Y.mean()) ** 2)))
+    log(f"loss history {hist[:3]} ... {hist[-2:]}; validation RMSE {rmse_v:.3f} dex "
+        f"(constant predictor {rmse_c:.3f})")
+    # trusted band: the training range widened by 10% of its width plus 0.05 on each side
+    w = X.max(0) - X.min(0)
+    lo, hi = X.min(0) - 0.1 * w - 0.05, X.max(0) + 0.1 * w + 0.05
+    lab_lo, lab_hi = float(np.log10(ALPHA_GRID[0])), float(np.log10(ALPHA_GRID[-1]))
+    margin, p99v, frac = calibrate_margin(M, Xv, Ev, ALPHA_GRID, args.p99, lo, hi, lab_lo, lab_hi)
+    pv_all = M.forward(torch.as_tensor(Xv.astype(np.float32), device="cuda:0")).cpu().numpy()
+    mean_alpha = float(np.mean(np.clip(pv_all - margin, lab_lo, lab_hi)))
+    log(f"margin {margin:.2f} dex: validation p99 {p99v:.4g}, groups in range {frac:.3f}, "
+        f"mean log10(alpha) after the margin {mean_alpha:.3f} (reference {math.log10(ALPHA_REF):.3f})")
+    meta = dict(train_sets=[s[0] for s in sets], valid_sets=[s[0] for s in vsets],
+                n_train_groups=int(len(Y)), n_samples=int(len(Y) * len(ALPHA_GRID)),
+                alpha_grid=[float(a) for a in ALPHA_GRID], target=args.target,
+                p99_goal=args.p99, group_size=args.group_size, steps=args.steps,
+                quantile=args.quantile, valid_mean_log_alpha=mean_alpha,
+                valid_rmse_dex=rmse_v, constant_rmse_dex=rmse_c, valid_p99=p99v,
+                in_range_fraction=frac, loss_history=hist, seconds=time.time() - t0)
+    save_model(args.out, params, mu, scale, lo, hi, lab_lo, lab_hi, margin, meta)
+    log(f"wrote {args.out} ({time.time() - t0:.1f} s)")
+
+
+if __name__ == "__main__":
+    main()

@@ -126,8 +126,9 @@ class Tree:
 
     def __del__(self):
         h = getattr(self, "handle", None)
-        if h is not None and L._lib is not None:
-            L._lib.gz_tree_free(h)
+        lib = getattr(L, "_lib", None) if L is not None else None
+        if h is not None and lib is not None:
+            lib.gz_tree_free(h)
             self.handle = None
 
     @property
@@ -228,8 +229,9 @@ class Model:
 
     def __del__(self):
         h = getattr(self, "handle", None)
-        if h is not None and L._lib is not None:
-            L._lib.gz_model_free(h)
+        lib = getattr(L, "_lib", None) if L is not None else None
+        if h is not None and lib is not None:
+            lib.gz_model_free(h)
             self.handle = None
 
     def params(self) -> np.ndarray:
@@ -254,6 +256,18 @@ class Model:
         self.calibration = dict(feat_lo=lo, feat_hi=hi, label_lo=float(label_lo),
                                 label_hi=float(label_hi), margin=float(margin),
                                 log10_alpha_ref=float(log10_alpha_ref))
+
+    def set_input_map(self, mean, scale):
+        """Inputs enter the first layer as (x - mean) * scale."""
+        mu = np.ascontiguousarray(mean, dtype=np.float32)
+        sc = np.ascontiguousarray(scale, dtype=np.float32)
+        L.check(L.load().gz_model_set_input_map(self.handle, mu.ctypes.data_as(L._fp),
+                                                sc.ctypes.data_as(L._fp)),
+                "gz_model_set_input_map")
+
+    def set_loss(self, quantile: float = 0.0):
+        """0: mean squared error; 0 < q < 1: pinball loss of the q-quantile."""
+        L.check(L.load().gz_model_set_loss(self.handle, float(quantile)), "gz_model_set_loss")
 
     def forward(self, X: torch.Tensor) -> torch.Tensor:
         X = X.to(torch.float32).contiguous()

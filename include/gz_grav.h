@@ -169,7 +169,8 @@ gz_status gz_direct_sum(gz_ctx* ctx, const gz_particles* p, const uint32_t* targ
 /* ---- learned opening criterion --------------------------------------------------------------
  * No counterpart in the reference, whose opening parameter is one global tolerance
  * (ErrTolForceAcc, shmem/mfm.cc:743-751, used by open_node's relative test, tree.h:134-135).
- * A 16 -> 64 -> 64 -> 1 GELU network (bf16 weights, FP32 accumulation) maps the 16 features of a
+ * A 16 -> 64 -> 64 -> 1 GELU network (bf16 weight matrices, FP32 biases and accumulation) maps
+ * the 16 features of a
  * target group to log10 of that group's tolerance alpha_g; the walk then applies the reference
  * test with aold_min scaled by alpha_g / alpha_ref. A group whose features lie outside the
  * calibrated ranges, or whose prediction lies outside the trained label range, keeps the
@@ -188,11 +189,18 @@ gz_status gz_model_get_params(const gz_model* model, float* host_params);
 gz_status gz_model_set_calibration(gz_model* model, const float* feat_lo, const float* feat_hi,
                                    float label_lo, float label_hi, float margin,
                                    float log10_alpha_ref);
+/* Inputs enter the first layer as (x - mean) * scale (host arrays of GZ_MLP_FEATURES, FP32);
+ * the default is mean 0, scale 1. */
+gz_status gz_model_set_input_map(gz_model* model, const float* mean, const float* scale);
 /* Device arrays: X is B x GZ_MLP_FEATURES, out/T are B. */
 gz_status gz_model_forward(gz_ctx* ctx, const gz_model* model, const float* X, size_t B,
                            float* out);
-/* Gradient of the mean squared error sum_b (y_b - T_b)^2 / B into grad_out (device,
- * GZ_MLP_PARAMS, may be NULL) and the loss into *loss_out (host, may be NULL). */
+/* Training loss: quantile 0 = mean squared error sum_b (y_b - T_b)^2 / B (the default);
+ * 0 < quantile < 1 = pinball loss sum_b (T_b - y_b)(quantile - [T_b < y_b]) / B, which fits the
+ * conditional quantile of the label. */
+gz_status gz_model_set_loss(gz_model* model, float quantile);
+/* Gradient of the training loss into grad_out (device, GZ_MLP_PARAMS, may be NULL) and the loss
+ * into *loss_out (host, may be NULL). */
 gz_status gz_model_grad(gz_ctx* ctx, gz_model* model, const float* X, const float* T, size_t B,
                         float* grad_out, float* loss_out);
 /* One Adam step (beta 0.9 / 0.999, eps 1e-8) on the batch's gradient. */

@@ -2,12 +2,12 @@
 
 A float64 numpy restatement of the device MLP (gizmo_shared_memory_amd/csrc/gz_learned.hip):
 
-    16 features -> 64 (GELU) -> 64 (GELU) -> 1
+    16 features -> standardise -> 64 (GELU) -> 64 (GELU) -> 1
 
 with the exact GELU, x * (1 + erf(x / sqrt 2)) / 2, and the mean-squared-error loss
 L = sum_b (y_b - t_b)^2 / B. The device keeps FP32 master weights, computes in FP32 from bf16
-copies of the weights, and accumulates in FP32; this oracle takes the same bf16-rounded weights
-and works in float64, so the two agree to FP32 rounding.
+copies of the weight matrices (biases stay FP32), and accumulates in FP32; this oracle takes the
+same rounded parameters (bf16_weights) and works in float64, so the two agree to FP32 rounding.
 
 Parameter layout (5,313 floats, the device's): W1[64][16], b1[64], W2[64][64], b2[64], w3[64],
 b3[1].
@@ -58,9 +58,22 @@ def gelu_grad(x):
     return 0.5 * (1.0 + _erf(x / math.sqrt(2.0))) + x * np.exp(-0.5 * x * x) / math.sqrt(2 * math.pi)
 
 
-def forward(theta, X):
+def bf16_weights(theta):
+    """The device's working copy: W1, W2 and w3 rounded to bf16, biases kept in FP32."""
+    th = np.asarray(theta, dtype=np.float32).astype(np.float64)
+    p = unpack(th)
+    for k in ("W1", "W2", "w3"):
+        p[k] = bf16_round(p[k].astype(np.float32)).astype(np.float64)
+    return pack(p)
+
+
+def forward(theta, X, mean=None, scale=None):
+    """The network on inputs standardised as (X - mean) * scale (the device applies the same
+    FP32 affine map before the first layer)."""
     p = unpack(theta)
     X = np.asarray(X, dtype=np.float64)
+    if mean is not None:
+        X = (X - np.asarray(mean, dtype=np.float64)) * np.asarray(scale, dtype=np.float64)
     z1 = X @ p["W1"].T + p["b1"]
     h1 = gelu(z1)
     z2 = h1 @ p["W2"].T + p["b2"]
@@ -69,14 +82,19 @@ def forward(theta, X):
     return y, (X, z1, h1, z2, h2)
 
 
-def loss_and_grad(theta, X, t):
-    """MSE loss and its gradient with respect to every parameter (backpropagation)."""
+def loss_and_grad(theta, X, t, mean=None, scale=None, quantile=0.0):
+    """Loss and its gradient with respect to every parameter (backpropagation). quantile 0 is
+    the mean squared error; 0 < quantile < 1 the pinball loss of that quantile."""
     p = unpack(theta)
-    y, (X, z1, h1, z2, h2) = forward(theta, X)
+    y, (X, z1, h1, z2, h2) = forward(theta, X, mean, scale)
     B = X.shape[0]
     r = y - np.asarray(t, dtype=np.float64)
-    loss = float(np.sum(r * r) / B)
-    dy = 2.0 * r / B
+    if quantile > 0:
+        loss = float(np.sum(np.where(r > 0, (1 - quantile) * r, -quantile * r)) / B)
+        dy = (np.where(r > 0, 1.0, 0.0) - quantile) / B
+    else:
+        loss = float(np.sum(r * r) / B)
+        dy = 2.0 * r / B
     g = {"w3": h2.T @ dy, "b3": np.array([dy.sum()])}
     dz2 = np.outer(dy, p["w3"]) * gelu_grad(z2)
     g["W2"] = dz2.T @ h1

@@ -22,16 +22,24 @@ def gz():
 
 
 def _bf(theta):
-    return mo.bf16_round(np.asarray(theta, dtype=np.float32)).astype(np.float64)
+    return mo.bf16_weights(theta)
 
 
 def test_forward_matches_oracle(gz):
     rng = np.random.default_rng(1)
     th = mo.init_params(7).astype(np.float32)
+    th[1024:1088] = rng.normal(size=64)  # non-zero FP32 biases
     M = gz.Model(th)
     X = rng.normal(size=(1000, 16)).astype(np.float32)
     y = M.forward(torch.from_numpy(X).cuda()).cpu().numpy()
     yo, _ = mo.forward(_bf(th), X.astype(np.float64))
+    np.testing.assert_allclose(y, yo, rtol=1e-4, atol=1e-4)
+    # with an input map, as the trained model uses it
+    mean = rng.normal(size=16).astype(np.float32)
+    scale = rng.uniform(0.5, 3.0, 16).astype(np.float32)
+    M.set_input_map(mean, scale)
+    y = M.forward(torch.from_numpy(X).cuda()).cpu().numpy()
+    yo, _ = mo.forward(_bf(th), X.astype(np.float64), mean, scale)
     np.testing.assert_allclose(y, yo, rtol=1e-4, atol=1e-4)
 
 
@@ -40,10 +48,13 @@ def test_gradient_matches_oracle(gz, B):
     rng = np.random.default_rng(B)
     th = mo.init_params(3).astype(np.float32)
     M = gz.Model(th)
+    mean = rng.normal(size=16).astype(np.float32)
+    scale = rng.uniform(0.5, 2.0, 16).astype(np.float32)
+    M.set_input_map(mean, scale)
     X = rng.normal(size=(B, 16)).astype(np.float32)
     T = rng.normal(size=B).astype(np.float32)
     loss, g = M.grad(torch.from_numpy(X).cuda(), torch.from_numpy(T).cuda())
-    lo, go = mo.loss_and_grad(_bf(th), X.astype(np.float64), T.astype(np.float64))
+    lo, go = mo.loss_and_grad(_bf(th), X.astype(np.float64), T.astype(np.float64), mean, scale)
     g = g.cpu().numpy().astype(np.float64)
     assert loss == pytest.approx(lo, rel=1e-4)
     assert np.linalg.norm(g - go) <= 1e-4 * np.linalg.norm(go) + 1e-7
@@ -51,6 +62,20 @@ def test_gradient_matches_oracle(gz, B):
     for k, v in mo.unpack(go).items():
         gk = mo.unpack(g)[k]
         assert np.linalg.norm(gk - v) <= 2e-4 * (np.linalg.norm(v) + 1e-6), k
+
+
+def test_pinball_gradient_matches_oracle(gz):
+    rng = np.random.default_rng(11)
+    th = mo.init_params(12).astype(np.float32)
+    M = gz.Model(th)
+    M.set_loss(0.1)
+    X = rng.normal(size=(300, 16)).astype(np.float32)
+    T = (rng.normal(size=300) * 2.0).astype(np.float32)
+    loss, g = M.grad(torch.from_numpy(X).cuda(), torch.from_numpy(T).cuda())
+    lo, go = mo.loss_and_grad(_bf(th), X.astype(np.float64), T.astype(np.float64), quantile=0.1)
+    g = g.cpu().numpy().astype(np.float64)
+    assert loss == pytest.approx(lo, rel=1e-4)
+    assert np.linalg.norm(g - go) <= 1e-4 * np.linalg.norm(go) + 1e-7
 
 
 def test_adam_step_matches_oracle(gz):

@@ -69,3 +69,30 @@ def test_adam_first_step_closed_form():
     new, m, v = mo.adam_step(th, np.zeros(3), np.zeros(3), g, 1, lr=0.01)
     # bias-corrected first step is lr * g / (|g| + eps)
     np.testing.assert_allclose(new, th - 0.01 * g / (np.abs(g) + 1e-8), rtol=1e-12)
+
+
+def test_pinball_backward_matches_finite_differences():
+    rng = np.random.default_rng(8)
+    th = mo.init_params(9)
+    X = rng.normal(size=(9, 16))
+    t = rng.normal(size=9) * 3.0  # labels away from the outputs: no kink inside the stencil
+    _, g = mo.loss_and_grad(th, X, t, quantile=0.2)
+    h = 1e-6
+    for i in rng.choice(mo.N_PARAMS, 40, replace=False):
+        tp, tm = th.copy(), th.copy()
+        tp[i] += h
+        tm[i] -= h
+        fd = (mo.loss_and_grad(tp, X, t, quantile=0.2)[0] -
+              mo.loss_and_grad(tm, X, t, quantile=0.2)[0]) / (2 * h)
+        assert g[i] == pytest.approx(fd, rel=1e-5, abs=1e-9), i
+
+
+def test_pinball_fits_the_quantile():
+    """A constant model trained with the pinball loss converges to the label quantile."""
+    rng = np.random.default_rng(2)
+    t = rng.normal(size=4000)
+    y = 0.0
+    for _ in range(4000):
+        dy = np.mean(np.where(y - t > 0, 1.0, 0.0) - 0.1)
+        y -= 0.05 * dy
+    assert y == pytest.approx(np.quantile(t, 0.1), abs=0.03)
