@@ -20,6 +20,8 @@ MLP_PARAMS = 5313
 
 PREC_FP64 = 0
 PREC_FP32_REL = 1
+LAUNCH_QUEUE = 0
+LAUNCH_PER_GROUP = 1
 CRIT_REFERENCE = 0
 CRIT_LEARNED = 1
 
@@ -63,7 +65,7 @@ class gz_tree_info(C.Structure):
 
 class gz_walk_opts(C.Structure):
     _fields_ = [("precision", C.c_int), ("criterion", C.c_int), ("group_size", C.c_int),
-                ("persistent", C.c_int), ("count", C.c_int), ("stack_limit", C.c_int),
+                ("launch", C.c_int), ("count", C.c_int), ("stack_limit", C.c_int),
                 ("model", C.c_void_p)]
 
 

@@ -58,7 +58,7 @@ struct WalkExtra {
 // The walk shared by gz_tree_accel_grouped, gz_tree_accel and gz_tree_group_features.
 gz_status walk(gz_ctx* ctx, const gz_tree* tree, const gz_particles* p, const uint32_t* targets,
                size_t n_t, double theta, double G, int batch, int group_size, const double* aold,
-               const float* group_theta, int precision, int persistent, int leaf_always_open,
+               int precision, int persistent, int leaf_always_open,
                int count, int stack_limit, int pack, double* ax, double* ay, double* az,
                gz_walk_counts* counts, const char* who, const WalkExtra& ex = WalkExtra()) {
     const bool features_only = ex.feats_out != nullptr;
@@ -133,23 +133,23 @@ gz_status walk(gz_ctx* ctx, const gz_tree* tree, const gz_particles* p, const ui
         gstart = gstart2;
     }
 
-    gz::DevMem& sg = ctx->scratch[6];
-    GZ_TRY(sg.reserve((size_t)ngroups * sizeof(gz::GroupInfo) + 256));
-    gz::GroupInfo* gi = sg.as<gz::GroupInfo>();
-    GZ_TRY(gz::launch_group_bounds((int)ngroups, n_t, gstart, targets, p, aold, group_theta,
-                                   theta * theta, gi, st));
     // counters, queue and status start at zero (a fill kernel rather than a memset node, so the
     // sequence stays valid under stream capture)
     GZ_TRY(gz::fill_zero_u32(reinterpret_cast<uint32_t*>(misc), 16, st));
-    long long n_fallback = 0;
+    float* group_scale = nullptr;
+    int* nfb = nullptr;
     if (ex.model || features_only) {
         // per-group features -> network -> per-group tolerance (or the reference's)
         gz::DevMem& sf = ctx->scratch[8];
         const size_t fbytes = ((size_t)ngroups * gz::kMlpIn * sizeof(float) + 255) & ~size_t(255);
-        GZ_TRY(sf.reserve(fbytes + (size_t)ngroups * sizeof(float) + 256));
+        const size_t pbytes = ((size_t)ngroups * sizeof(float) + 255) & ~size_t(255);
+        GZ_TRY(sf.reserve(fbytes + 2 * pbytes + 256));
         float* feats = features_only ? ex.feats_out : sf.as<float>();
         float* pred = reinterpret_cast<float*>(sf.as<uint8_t>() + fbytes);
-        GZ_TRY(gz::launch_group_features((int)ngroups, gi, targets, p, aold, tree, feats, st));
+        group_scale = reinterpret_cast<float*>(sf.as<uint8_t>() + fbytes + pbytes);
+        nfb = reinterpret_cast<int*>(sf.as<uint8_t>() + fbytes + 2 * pbytes);
+        GZ_TRY(gz::launch_group_features((int)ngroups, (int)n_t, gstart, targets, p, aold, tree,
+                                         feats, st));
         if (features_only) {
             GZ_HIP(hipMemcpyAsync(ex.gstart_out, gstart, ngroups * sizeof(int),
                                   hipMemcpyDeviceToDevice, st));
@@ -161,15 +161,10 @@ gz_status walk(gz_ctx* ctx, const gz_tree* tree, const gz_particles* p, const ui
             return GZ_OK;
         }
         gz_model* M = ex.model;
+        GZ_TRY(gz::fill_zero_u32(reinterpret_cast<uint32_t*>(nfb), 1, st));
         GZ_TRY(gz::launch_mlp_forward(feats, (int)ngroups, M->wbf, M->theta, M->xnorm, pred, st));
         GZ_TRY(gz::launch_group_scale((int)ngroups, feats, pred, M->frange, M->lab_lo, M->lab_hi,
-                                      M->margin, M->log_alpha_ref, gi,
-                                      reinterpret_cast<int*>(misc + 2), st));
-        int nf = 0;
-        GZ_HIP(hipMemcpyAsync(&nf, misc + 2, sizeof nf, hipMemcpyDeviceToHost, st));
-        GZ_HIP(hipStreamSynchronize(st));
-        n_fallback = nf;
-        GZ_TRY(gz::fill_zero_u32(reinterpret_cast<uint32_t*>(misc), 16, st));
+                                      M->margin, M->log_alpha_ref, group_scale, nfb, st));
     }
     GZ_HIP(hipEventRecord(ev[9], st));
 
@@ -179,7 +174,12 @@ gz_status walk(gz_ctx* ctx, const gz_tree* tree, const gz_particles* p, const ui
     a.src_soft = tree->src_soft;
     a.targets = targets;
     a.x = p->x; a.y = p->y; a.z = p->z; a.soft = p->soft;
-    a.groups = gi;
+    a.groups = nullptr; // bounds are computed by each group's wavefront
+    a.gstart = gstart;
+    a.nt = (int)n_t;
+    a.aold = aold;
+    a.group_scale = group_scale;
+    a.theta_sq = theta * theta;
     a.ngroups = (int)ngroups;
     a.root = tree->root;
     a.persistent = persistent;
@@ -200,6 +200,8 @@ gz_status walk(gz_ctx* ctx, const gz_tree* tree, const gz_particles* p, const ui
 
     long long host_misc[8] = {0};
     GZ_HIP(hipMemcpyAsync(host_misc, misc, sizeof host_misc, hipMemcpyDeviceToHost, st));
+    int n_fallback = 0;
+    if (nfb) GZ_HIP(hipMemcpyAsync(&n_fallback, nfb, sizeof n_fallback, hipMemcpyDeviceToHost, st));
     GZ_HIP(hipStreamSynchronize(st));
     const int retries = (int)(host_misc[7] & 0xFFFFFFFF);
     if (retries > 0) {
@@ -442,13 +444,14 @@ gz_status gz_tree_accel_grouped(gz_ctx* ctx, const gz_tree* tree, const gz_parti
                                 int batch, const double* aold, double* ax, double* ay, double* az,
                                 const gz_walk_opts* opts, gz_walk_counts* counts) {
     gz::clear_error();
-    int precision = GZ_PREC_FP64, group_size = 0, persistent = 0, count = 0, stack_limit = 0;
+    int precision = GZ_PREC_FP64, group_size = 0, persistent = 1, count = 0, stack_limit = 0;
     WalkExtra ex;
-    const float* group_theta = nullptr;
     if (opts) {
         precision = opts->precision;
         group_size = opts->group_size;
-        persistent = opts->persistent;
+        if (opts->launch != GZ_LAUNCH_QUEUE && opts->launch != GZ_LAUNCH_PER_GROUP)
+            return fail(GZ_ERR_INVALID, "gz_tree_accel_grouped: unknown launch %d", opts->launch);
+        persistent = opts->launch == GZ_LAUNCH_QUEUE;
         count = opts->count;
         stack_limit = opts->stack_limit;
         if (opts->criterion == GZ_CRIT_LEARNED) {
@@ -461,8 +464,8 @@ gz_status gz_tree_accel_grouped(gz_ctx* ctx, const gz_tree* tree, const gz_parti
     }
     if (batch > 0 && group_size > 0)
         return fail(GZ_ERR_INVALID, "gz_tree_accel_grouped: batch > 0 and group_size > 0");
-    return walk(ctx, tree, p, targets, n_t, theta, G, batch, group_size, aold, group_theta,
-                precision, persistent, 0, count, stack_limit, 1, ax, ay, az, counts,
+    return walk(ctx, tree, p, targets, n_t, theta, G, batch, group_size, aold, precision,
+                persistent, 0, count, stack_limit, 1, ax, ay, az, counts,
                 "gz_tree_accel_grouped", ex);
 }
 
@@ -477,7 +480,7 @@ gz_status gz_tree_group_features(gz_ctx* ctx, const gz_tree* tree, const gz_part
     ex.feats_out = feats;
     ex.gstart_out = group_start;
     ex.ngroups_out = n_groups;
-    return walk(ctx, tree, p, targets, n_t, 0.0, 1.0, 0, group_size, aold, nullptr, GZ_PREC_FP32_REL,
+    return walk(ctx, tree, p, targets, n_t, 0.0, 1.0, 0, group_size, aold, GZ_PREC_FP32_REL,
                 0, 0, 0, 0, 1, nullptr, nullptr, nullptr, nullptr, "gz_tree_group_features", ex);
 }
 
@@ -650,7 +653,7 @@ gz_status gz_tree_accel(gz_ctx* ctx, const gz_tree* tree, const gz_particles* p,
     gz::clear_error();
     // one group per target: the group box is the target's position, so the opening distance is
     // |COM - x| per axis exactly as shmem::accel computes it (tree.cc:552-557)
-    return walk(ctx, tree, p, targets, n_t, theta, G, 1, 0, aold, nullptr, GZ_PREC_FP64, 0, 1, 0,
+    return walk(ctx, tree, p, targets, n_t, theta, G, 1, 0, aold, GZ_PREC_FP64, 1, 1, 0,
                 0, 0, ax, ay, az, nullptr, "gz_tree_accel");
 }
 

@@ -368,7 +368,7 @@ static gz_status grow_nodes(gz_tree* t, int used, int cap, hipStream_t st) {
 void tree_release(gz_tree* t) {
     void* ptrs[] = {t->skey, t->orderbuf, t->rank, t->leaf_of, t->src, t->src_soft, t->wn,
                     t->next, t->parent, t->nlo, t->nhi, t->gcx, t->gcy, t->gcz, t->size,
-                    t->delta, t->soft};
+                    t->delta, t->soft, t->level_begin_dev};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
 }
@@ -526,6 +526,14 @@ gz_status build_tree(gz_ctx* ctx, const gz_particles* p, const gz_build_opts* op
     }
     t->nnodes = nnodes;
     t->depth = (int)t->level_begin.size() - 2;
+    {
+        // the level table on the device, padded with nnodes so any level index is valid
+        int lb[kMaxLevel + 2];
+        for (int L = 0; L < kMaxLevel + 2; ++L)
+            lb[L] = (L < (int)t->level_begin.size()) ? t->level_begin[L] : nnodes;
+        GZ_TRY(dalloc(reinterpret_cast<void**>(&t->level_begin_dev), sizeof lb));
+        GZ_HIP(hipMemcpy(t->level_begin_dev, lb, sizeof lb, hipMemcpyHostToDevice));
+    }
     GZ_HIP(hipEventRecord(ev[5], st));
 
     // --- moments + pack, deepest level first

@@ -48,7 +48,12 @@ struct WalkArgs {
     const double* src_soft; // softening per sorted slot
     const uint32_t* targets;
     const double *x, *y, *z, *soft; // target data, indexed by particle
-    const GroupInfo* groups;
+    const GroupInfo* groups; // precomputed group bounds, or NULL: bounds from gstart in-kernel
+    const int* gstart;       // group g is target slots [gstart[g], gstart[g+1]) (or nt at the end)
+    int nt;
+    const double* aold;      // per target slot, or NULL
+    const float* group_scale; // per group multiplier of aold_min (learned tolerance), or NULL
+    double theta_sq;
     int ngroups;
     int root;
     int persistent;
@@ -122,9 +127,6 @@ gz_status launch_group_pack(int ngroups, size_t nt, int group_size, const int* g
                             const int* gkey, const int* parent, int* pk, uint32_t* mflag,
                             uint32_t* mid, uint32_t* ngroups2_dev, int* gstart2, void* scan_tmp,
                             hipStream_t st);
-gz_status launch_group_bounds(int ngroups, size_t nt, const int* gstart, const uint32_t* targets,
-                              const gz_particles* p, const double* aold, const float* group_theta,
-                              double theta_sq, GroupInfo* gi, hipStream_t st);
 gz_status launch_walk(const WalkArgs& a, bool fp64, bool count, hipStream_t st);
 gz_status launch_walk_retry(const WalkArgs& a, int n_retry, bool fp64, bool count,
                             hipStream_t st);
@@ -132,7 +134,7 @@ gz_status launch_direct(const gz_particles* p, const uint32_t* targets, size_t n
                         double* ax, double* ay, double* az, hipStream_t st);
 
 // ---- learned opening criterion (gz_learned.hip) ----------------------------------------------------
-gz_status launch_group_features(int ngroups, const GroupInfo* gi, const uint32_t* targets,
+gz_status launch_group_features(int ngroups, int nt, const int* gstart, const uint32_t* targets,
                                 const gz_particles* p, const double* amag, const ::gz_tree* t,
                                 float* feats, hipStream_t st);
 gz_status launch_mlp_forward(const float* X, int B, const uint16_t* wbf, const float* theta,
@@ -145,7 +147,7 @@ gz_status launch_adam(float* theta, float* m, float* v, const float* grad, float
 gz_status launch_to_bf16(const float* theta, uint16_t* wbf, hipStream_t st);
 gz_status launch_group_scale(int ngroups, const float* feats, const float* pred,
                              const float* frange, float lab_lo, float lab_hi, float margin,
-                             float log_alpha_ref, GroupInfo* gi, int* n_fallback,
+                             float log_alpha_ref, float* scale, int* n_fallback,
                              hipStream_t st);
 
 } // namespace gz
@@ -181,6 +183,7 @@ struct gz_tree {
     int cap = 0;
     double root_box[4] = {0, 0, 0, 0};
     std::vector<int> level_begin; // node ids of level L are [level_begin[L], level_begin[L+1])
+    int* level_begin_dev = nullptr; // the same table on the device (kMaxLevel + 2 entries)
     gz_build_times times = {};
 
     // particles, Morton order

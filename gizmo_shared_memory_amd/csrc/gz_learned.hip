@@ -35,38 +35,41 @@ __device__ __forceinline__ float gelu_grad(float x) {
 }
 
 // ---- group features ------------------------------------------------------------------------------
-// One wavefront per group. amag is |a_prev| per target slot (no-G units, the basis of aold).
+// amag is |a_prev| per target slot (no-G units, the basis of aold).
 __device__ __forceinline__ float flog10(double v) { return (float)log10(fmax(v, 1e-300)); }
 
 __global__ __launch_bounds__(256) void k_group_features(
-    int ngroups, const GroupInfo* __restrict__ gi, const uint32_t* __restrict__ targets,
-    const double* __restrict__ x, const double* __restrict__ y, const double* __restrict__ z,
-    const double* __restrict__ m, const double* __restrict__ amag, const int* __restrict__ leaf_of,
-    const int* __restrict__ parent, const WNode* __restrict__ wn, int root,
-    float* __restrict__ feats) {
-    const int g = blockIdx.x * 4 + (threadIdx.x >> 6);
-    const int lane = threadIdx.x & 63;
+    int ngroups, int nt_total, const int* __restrict__ gstart,
+    const uint32_t* __restrict__ targets, const double* __restrict__ x,
+    const double* __restrict__ y, const double* __restrict__ z, const double* __restrict__ m,
+    const double* __restrict__ soft, const double* __restrict__ amag,
+    const int* __restrict__ leaf_of, const int* __restrict__ level_begin,
+    const WNode* __restrict__ wn, int root, float* __restrict__ feats) {
+    // one thread per group: groups are small, and many independent groups in flight hide the
+    // latency of the target gathers better than one wavefront per group
+    const int g = blockIdx.x * 256 + threadIdx.x;
     if (g >= ngroups) return;
-    const GroupInfo G = gi[g];
+    GroupInfo G;
+    G.tlo = gstart[g];
+    G.thi = (g + 1 < ngroups) ? gstart[g + 1] : nt_total;
     const int nt = G.thi - G.tlo;
-    double msum = 0, amin = 1e300, amax = 0, a2 = 0;
-    for (int i = G.tlo + lane; i < G.thi; i += 64) {
+    double msum = 0, amin = 1e300, amax = 0, a2 = 0, emax = 0;
+    double lo0 = 1e300, lo1 = 1e300, lo2 = 1e300, hi0 = -1e300, hi1 = -1e300, hi2 = -1e300;
+    for (int i = G.tlo; i < G.thi; ++i) {
         const uint32_t p = targets[i];
-        const double mm = m[p];
-        msum += mm;
+        msum += m[p];
         const double am = amag ? amag[i] : 0.0;
         amin = fmin(amin, am);
         amax = fmax(amax, am);
         a2 += am * am;
+        if (soft) emax = fmax(emax, soft[p]);
+        lo0 = fmin(lo0, x[p]); hi0 = fmax(hi0, x[p]);
+        lo1 = fmin(lo1, y[p]); hi1 = fmax(hi1, y[p]);
+        lo2 = fmin(lo2, z[p]); hi2 = fmax(hi2, z[p]);
     }
-#pragma unroll
-    for (int d = 32; d >= 1; d >>= 1) {
-        msum += __shfl_xor(msum, d);
-        amin = fmin(amin, __shfl_xor(amin, d));
-        amax = fmax(amax, __shfl_xor(amax, d));
-        a2 += __shfl_xor(a2, d);
-    }
-    if (lane != 0) return;
+    G.lo[0] = lo0; G.lo[1] = lo1; G.lo[2] = lo2;
+    G.hi[0] = hi0; G.hi[1] = hi1; G.hi[2] = hi2;
+    G.emax = emax;
     const WNode R = wn[root];
     const double side = fmax((double)R.len, 1e-300);
     const double Mr = fmax(R.mass, 1e-300);
@@ -78,9 +81,11 @@ __global__ __launch_bounds__(256) void k_group_features(
     const double dist = fmax(sqrt(cx * cx + cy * cy + cz * cz), 1e-12 * side);
     const double arms = sqrt(a2 / fmax((double)nt, 1.0));
     const double mg = fmax(msum, 1e-300);
-    int depth = 0, leaf = leaf_of[targets[G.tlo]];
+    const int leaf = leaf_of[targets[G.tlo]];
     const double leaf_len = (leaf >= 0) ? (double)wn[leaf].len : side;
-    for (int q = leaf; q >= 0 && parent[q] >= 0; q = parent[q]) ++depth;
+    int depth = 0; // node ids are level-major: the level is where the id falls in the table
+    if (leaf >= 0)
+        while (depth < kMaxLevel && level_begin[depth + 1] <= leaf) ++depth;
     // All features are ratios of the group's own scales (extent, leaf cell, softening, field,
     // mass) and of its position in the system, so none depends on the units or the box.
     const double rho_g = mg / (ext * ext * ext), rho_c = Mr / (dist * dist * dist);
@@ -315,7 +320,7 @@ __global__ __launch_bounds__(256) void k_group_scale(int ngroups, const float* _
                                                      const float* __restrict__ frange,
                                                      float lab_lo, float lab_hi, float margin,
                                                      float log_alpha_ref,
-                                                     GroupInfo* __restrict__ gi,
+                                                     float* __restrict__ scale,
                                                      int* __restrict__ n_fallback) {
     const int g = blockIdx.x * 256 + threadIdx.x;
     if (g >= ngroups) return;
@@ -327,18 +332,18 @@ __global__ __launch_bounds__(256) void k_group_scale(int ngroups, const float* _
     const float p = pred[g];
     ok = ok && (p >= lab_lo) && (p <= lab_hi);
     const float s = ok ? exp10f(p - margin - log_alpha_ref) : 1.0f;
-    gi[g].amin *= (double)s;
+    scale[g] = s;
     if (!ok) atomicAdd(n_fallback, 1);
 }
 
 // ---- host launchers ---------------------------------------------------------------------------------
-gz_status launch_group_features(int ngroups, const GroupInfo* gi, const uint32_t* targets,
+gz_status launch_group_features(int ngroups, int nt, const int* gstart, const uint32_t* targets,
                                 const gz_particles* p, const double* amag, const gz_tree* t,
                                 float* feats, hipStream_t st) {
     if (ngroups <= 0) return GZ_OK;
-    k_group_features<<<(unsigned)((ngroups + 3) / 4), 256, 0, st>>>(
-        ngroups, gi, targets, p->x, p->y, p->z, p->m, amag, t->leaf_of, t->parent, t->wn, t->root,
-        feats);
+    k_group_features<<<(unsigned)((ngroups + 255) / 256), 256, 0, st>>>(
+        ngroups, nt, gstart, targets, p->x, p->y, p->z, p->m, p->soft, amag, t->leaf_of,
+        t->level_begin_dev, t->wn, t->root, feats);
     GZ_HIP(hipGetLastError());
     return GZ_OK;
 }
@@ -378,11 +383,11 @@ gz_status launch_to_bf16(const float* theta, uint16_t* wbf, hipStream_t st) {
 
 gz_status launch_group_scale(int ngroups, const float* feats, const float* pred,
                              const float* frange, float lab_lo, float lab_hi, float margin,
-                             float log_alpha_ref, GroupInfo* gi, int* n_fallback,
+                             float log_alpha_ref, float* scale, int* n_fallback,
                              hipStream_t st) {
     if (ngroups <= 0) return GZ_OK;
     k_group_scale<<<(unsigned)((ngroups + 255) / 256), 256, 0, st>>>(
-        ngroups, feats, pred, frange, lab_lo, lab_hi, margin, log_alpha_ref, gi, n_fallback);
+        ngroups, feats, pred, frange, lab_lo, lab_hi, margin, log_alpha_ref, scale, n_fallback);
     GZ_HIP(hipGetLastError());
     return GZ_OK;
 }

@@ -179,48 +179,33 @@ __device__ __forceinline__ double wave_max_d(double v) {
     return v;
 }
 
-// Per-group opening bounds (tree.cc:683-702): bounding box, largest softening, smallest aold.
-// One wavefront per group; min and max are exact, so the lane order does not matter.
-__global__ __launch_bounds__(256) void k_group_bounds(
-    int ngroups, size_t nt, const int* __restrict__ gstart, const uint32_t* __restrict__ targets,
-    const double* __restrict__ x, const double* __restrict__ y, const double* __restrict__ z,
-    const double* __restrict__ soft, const double* __restrict__ aold,
-    const float* __restrict__ group_theta, double theta_sq, GroupInfo* __restrict__ gi) {
-    const int g = blockIdx.x * 4 + (threadIdx.x >> 6);
-    const int lane = threadIdx.x & 63;
-    if (g >= ngroups) return; // wave-uniform
-    const int lo = gstart[g];
-    const int hi = (g + 1 < ngroups) ? gstart[g + 1] : (int)nt;
+// The opening bounds of group g (tree.cc:683-702: bounding box, largest softening, smallest aold),
+// computed by the group's own wavefront. Min and max are exact, so every lane ends with the same
+// values whatever the reduction order.
+__device__ GroupInfo group_bounds(const WalkArgs& a, int g, int lane) {
+    const int lo = a.gstart[g];
+    const int hi = (g + 1 < a.ngroups) ? a.gstart[g + 1] : a.nt;
     double bx0 = 1e300, by0 = 1e300, bz0 = 1e300, bx1 = -1e300, by1 = -1e300, bz1 = -1e300;
-    double emax = 0, amin = aold ? 1e300 : 0.0;
+    double emax = 0, amin = a.aold ? 1e300 : 0.0;
     for (int i = lo + lane; i < hi; i += 64) {
-        const uint32_t p = targets[i];
-        const double px = x[p], py = y[p], pz = z[p];
+        const uint32_t p = a.targets[i];
+        const double px = a.x[p], py = a.y[p], pz = a.z[p];
         bx0 = dmin_w(bx0, px); bx1 = dmax_w(bx1, px);
         by0 = dmin_w(by0, py); by1 = dmax_w(by1, py);
         bz0 = dmin_w(bz0, pz); bz1 = dmax_w(bz1, pz);
-        if (soft) emax = dmax_w(emax, soft[p]);
-        if (aold) amin = dmin_w(amin, aold[i]);
+        if (a.soft) emax = dmax_w(emax, a.soft[p]);
+        if (a.aold) amin = dmin_w(amin, a.aold[i]);
     }
-    bx0 = wave_min_d(bx0); by0 = wave_min_d(by0); bz0 = wave_min_d(bz0);
-    bx1 = wave_max_d(bx1); by1 = wave_max_d(by1); bz1 = wave_max_d(bz1);
-    emax = wave_max_d(emax);
-    amin = wave_min_d(amin);
-    if (lane != 0) return;
     GroupInfo o;
-    o.lo[0] = bx0; o.lo[1] = by0; o.lo[2] = bz0;
-    o.hi[0] = bx1; o.hi[1] = by1; o.hi[2] = bz1;
-    o.emax = emax;
-    o.amin = amin;
-    if (group_theta) {
-        const double th = (double)group_theta[g];
-        o.theta_sq = th * th;
-    } else {
-        o.theta_sq = theta_sq;
-    }
+    o.lo[0] = wave_min_d(bx0); o.lo[1] = wave_min_d(by0); o.lo[2] = wave_min_d(bz0);
+    o.hi[0] = wave_max_d(bx1); o.hi[1] = wave_max_d(by1); o.hi[2] = wave_max_d(bz1);
+    o.emax = wave_max_d(emax);
+    o.amin = wave_min_d(amin);
+    if (a.group_scale) o.amin *= (double)a.group_scale[g];
+    o.theta_sq = a.theta_sq;
     o.tlo = lo;
     o.thi = hi;
-    gi[g] = o;
+    return o;
 }
 
 // ---- the walk --------------------------------------------------------------------------------------
@@ -310,7 +295,7 @@ __global__ __launch_bounds__(64, 4) void k_walk(WalkArgs a) {
             if (g >= a.ngroups) break;
         }
 
-        const GroupInfo gi = a.groups[g];
+        const GroupInfo gi = a.groups ? a.groups[g] : group_bounds(a, g, lane);
         const double gcx = 0.5 * (gi.lo[0] + gi.hi[0]);
         const double gcy = 0.5 * (gi.lo[1] + gi.hi[1]);
         const double gcz = 0.5 * (gi.lo[2] + gi.hi[2]);
@@ -722,16 +707,6 @@ gz_status launch_group_pack(int ngroups, size_t nt, int group_size, const int* g
     GZ_HIP(hipGetLastError());
     GZ_TRY(exclusive_scan_u32(mflag, mid, (size_t)ngroups, ngroups2_dev, scan_tmp, st));
     k_pack_starts<<<grid, 256, 0, st>>>(ngroups, gstart, mflag, mid, gstart2);
-    GZ_HIP(hipGetLastError());
-    return GZ_OK;
-}
-
-gz_status launch_group_bounds(int ngroups, size_t nt, const int* gstart, const uint32_t* targets,
-                              const gz_particles* p, const double* aold, const float* group_theta,
-                              double theta_sq, GroupInfo* gi, hipStream_t st) {
-    if (ngroups <= 0) return GZ_OK;
-    k_group_bounds<<<(unsigned)((ngroups + 3) / 4), 256, 0, st>>>(
-        ngroups, nt, gstart, targets, p->x, p->y, p->z, p->soft, aold, group_theta, theta_sq, gi);
     GZ_HIP(hipGetLastError());
     return GZ_OK;
 }

@@ -315,13 +315,15 @@ def group_features(T: Tree, P: Particles, targets, aold, group_size: int = 0):
 
 def accel_grouped(T: Tree, P: Particles, targets, theta: float, G: float = 1.0, batch: int = 0,
                   aold=None, *, precision: str = "fp64", group_size: int = 0,
-                  persistent: bool = False, counts: bool = False, timing: bool = False,
+                  persistent: bool = True, counts: bool = False, timing: bool = False,
                   stack_limit: int = 0, model: Model | None = None, out=None):
     """shmem::accel_grouped (tree.h:428-440) without tidal/jerk/lazy/sink terms.
 
     precision "fp64" is the parity walk; "fp32" evaluates interactions in FP32 on positions
     relative to the group centre (opening decisions and the per-target sum stay FP64).
     group_size > 0 groups targets under the highest node holding at most that many particles.
+    persistent=True (the default) walks with a persistent grid fed by a work queue; False launches
+    one workgroup per group.
     Returns (ax, ay, az) device tensors, plus a dict of walk counts when counts=True (node
     visits and interactions, SHMEM_COUNT_WALK's definition) or timing=True (walk and prep ms only,
     from the kernel variant without counters).
@@ -332,7 +334,8 @@ def accel_grouped(T: Tree, P: Particles, targets, theta: float, G: float = 1.0, 
     ao = None if aold is None else _dev_f64(aold, P.device)
     prec = {"fp64": L.PREC_FP64, "fp32": L.PREC_FP32_REL}[precision]
     crit = L.CRIT_REFERENCE if model is None else L.CRIT_LEARNED
-    opts = L.gz_walk_opts(prec, crit, int(group_size), int(persistent), int(counts),
+    launch = L.LAUNCH_QUEUE if persistent else L.LAUNCH_PER_GROUP
+    opts = L.gz_walk_opts(prec, crit, int(group_size), launch, int(counts),
                           int(stack_limit), None if model is None else model.handle)
     cnt = L.gz_walk_counts()
     want = counts or timing

@@ -116,6 +116,8 @@ typedef enum {
                             node COM and final per-target sum                                  */
 } gz_precision;
 
+typedef enum { GZ_LAUNCH_QUEUE = 0, GZ_LAUNCH_PER_GROUP = 1 } gz_launch;
+
 typedef enum {
     GZ_CRIT_REFERENCE = 0, /* shmem open_node (tree.h:109-144)                                 */
     GZ_CRIT_LEARNED = 1    /* per-group theta from the learned model, reference as fallback    */
@@ -126,7 +128,9 @@ typedef struct {
     int criterion;   /* gz_criterion                                                           */
     int group_size;  /* batch <= 0 grouping: 0 = one group per leaf (the reference); G > 0 =
                         targets under the highest node holding <= G particles                  */
-    int persistent;  /* 1 = persistent work-queue launch; 0 = one workgroup per work item      */
+    int launch;      /* GZ_LAUNCH_QUEUE (0, default): a persistent grid of wavefronts takes
+                        groups from a work queue, which balances dense and sparse regions;
+                        GZ_LAUNCH_PER_GROUP (1): one workgroup per group                       */
     int count;       /* 1 = count node visits and interactions into gz_walk_counts (a slower
                         kernel variant); timings are reported whenever counts is non-NULL     */
     int stack_limit; /* 0 = default. > 0 caps the on-chip node stack at this many entries, so

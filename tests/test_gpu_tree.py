@@ -195,7 +195,7 @@ def test_persistent_matches_one_group_per_workgroup(gz):
     case = tree_cases.plummer_small(20000, seed=8, eps=0.01)
     P, T = _device_tree(gz, case)
     tg = T.device_orderbuf()
-    a = gz.accel_grouped(T, P, tg, 0.7, 1.0, 0)
+    a = gz.accel_grouped(T, P, tg, 0.7, 1.0, 0, persistent=False)
     b = gz.accel_grouped(T, P, tg, 0.7, 1.0, 0, persistent=True)
     for u, v in zip(a, b):
         assert torch.equal(u, v)
