@@ -116,6 +116,17 @@ _PROTOS = {
     "gz_tree_group_features": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(gz_particles),
                                          C.c_void_p, C.c_size_t, C.c_int, C.c_void_p, C.c_void_p,
                                          C.c_void_p, C.POINTER(C.c_size_t)]),
+    "gz_walk_plan_create": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(gz_particles),
+                                      C.c_void_p, C.c_size_t, C.POINTER(C.c_size_t), C.c_int,
+                                      C.POINTER(gz_walk_opts), C.POINTER(C.c_void_p)]),
+    "gz_walk_plan_launch": (C.c_int, [C.c_void_p, C.c_void_p, C.c_double, C.c_double,
+                                      C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
+    "gz_walk_plan_status": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(gz_walk_counts)]),
+    "gz_walk_plan_free": (C.c_int, [C.c_void_p]),
+    "gz_graph_begin": (C.c_int, [C.c_void_p]),
+    "gz_graph_end": (C.c_int, [C.c_void_p, C.POINTER(C.c_void_p)]),
+    "gz_graph_launch": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "gz_graph_free": (C.c_int, [C.c_void_p]),
     "gz_order_targets": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t, C.c_void_p,
                                    C.POINTER(C.c_size_t)]),
     "gz_aold_from_accel": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t,

@@ -51,8 +51,11 @@ struct Carver {
 struct WalkExtra {
     gz_model* model = nullptr;
     float* feats_out = nullptr; // features-only: n_groups x 16, device
-    int* gstart_out = nullptr;  // features-only: n_groups + 1 group starts, device
+    int* gstart_out = nullptr;  // features-only / groups-only: n_groups + 1 group starts, device
     size_t* ngroups_out = nullptr;
+    bool groups_only = false;   // stop after the grouping (walk plans)
+    const uint32_t* seg = nullptr; // device: nseg + 1 slot offsets of independent target lists
+    int nseg = 0;
 };
 
 // The walk shared by gz_tree_accel_grouped, gz_tree_accel and gz_tree_group_features.
@@ -62,13 +65,14 @@ gz_status walk(gz_ctx* ctx, const gz_tree* tree, const gz_particles* p, const ui
                int count, int stack_limit, int pack, double* ax, double* ay, double* az,
                gz_walk_counts* counts, const char* who, const WalkExtra& ex = WalkExtra()) {
     const bool features_only = ex.feats_out != nullptr;
+    const bool groups_only = ex.groups_only;
     GZ_TRY(check_ctx(ctx));
     if (!tree) return fail(GZ_ERR_INVALID, "%s: null tree", who);
     GZ_TRY(check_particles(p, who));
     if (p->n != tree->n)
         return fail(GZ_ERR_INVALID, "%s: particle count %zu differs from the tree's %zu", who,
                     p->n, tree->n);
-    if (n_t > 0 && (!targets || (!features_only && (!ax || !ay || !az))))
+    if (n_t > 0 && (!targets || (!features_only && !groups_only && (!ax || !ay || !az))))
         return fail(GZ_ERR_INVALID, "%s: null target or output array", who);
     if ((ex.model || features_only) && !aold)
         return fail(GZ_ERR_INVALID, "%s: the learned criterion needs aold", who);
@@ -87,9 +91,10 @@ gz_status walk(gz_ctx* ctx, const gz_tree* tree, const gz_particles* p, const ui
     Carver c;
     const size_t o_key = c.add<int>(n_t), o_flag = c.add<uint32_t>(n_t),
                  o_gid = c.add<uint32_t>(n_t), o_gstart = c.add<int>(n_t + 1),
-                 o_retry = c.add<int>(n_t), o_pk = c.add<int>(n_t), o_mflag = c.add<uint32_t>(n_t),
+                 o_retry = c.add<int>(n_t), o_pk = c.add<long long>(n_t),
+                 o_gseg = c.add<int>(ex.nseg > 1 ? n_t : 1), o_mflag = c.add<uint32_t>(n_t),
                  o_mid = c.add<uint32_t>(n_t), o_gstart2 = c.add<int>(n_t + 1),
-                 o_misc = c.add<long long>(8),
+                 o_misc = c.add<long long>(16),
                  o_tmp = c.add<uint8_t>(gz::scan_tmp_bytes(n_t));
     gz::DevMem& s = ctx->scratch[5];
     GZ_TRY(s.reserve(c.used));
@@ -106,10 +111,11 @@ gz_status walk(gz_ctx* ctx, const gz_tree* tree, const gz_particles* p, const ui
     int* retry_list = reinterpret_cast<int*>(b + o_retry);
 
     GZ_HIP(hipEventRecord(ev[8], st));
-    GZ_TRY(gz::fill_zero_u32(reinterpret_cast<uint32_t*>(misc), 16, st));
+    GZ_TRY(gz::fill_zero_u32(reinterpret_cast<uint32_t*>(misc), 32, st));
+    int* gseg = ex.nseg > 1 ? reinterpret_cast<int*>(b + o_gseg) : nullptr;
     GZ_TRY(gz::launch_group_prep(targets, n_t, tree->leaf_of, tree->parent, tree->wn, batch,
-                                 group_size, p->n, gkey, flag, gid, ngroups_dev, status, b + o_tmp,
-                                 st));
+                                 group_size, p->n, ex.seg, ex.nseg, gseg, gkey, flag, gid,
+                                 ngroups_dev, status, b + o_tmp, st));
     uint32_t ngroups0 = 0;
     GZ_HIP(hipMemcpyAsync(&ngroups0, ngroups_dev, sizeof ngroups0, hipMemcpyDeviceToHost, st));
     int prep_status = 0;
@@ -124,7 +130,7 @@ gz_status walk(gz_ctx* ctx, const gz_tree* tree, const gz_particles* p, const ui
         uint32_t* ng2 = reinterpret_cast<uint32_t*>(misc + 3);
         int* gstart2 = reinterpret_cast<int*>(b + o_gstart2);
         GZ_TRY(gz::launch_group_pack((int)ngroups0, n_t, group_size, gstart, gkey, tree->parent,
-                                     reinterpret_cast<int*>(b + o_pk),
+                                     gseg, reinterpret_cast<long long*>(b + o_pk),
                                      reinterpret_cast<uint32_t*>(b + o_mflag),
                                      reinterpret_cast<uint32_t*>(b + o_mid), ng2, gstart2,
                                      b + o_tmp, st));
@@ -135,36 +141,33 @@ gz_status walk(gz_ctx* ctx, const gz_tree* tree, const gz_particles* p, const ui
 
     // counters, queue and status start at zero (a fill kernel rather than a memset node, so the
     // sequence stays valid under stream capture)
-    GZ_TRY(gz::fill_zero_u32(reinterpret_cast<uint32_t*>(misc), 16, st));
-    float* group_scale = nullptr;
+    GZ_TRY(gz::fill_zero_u32(reinterpret_cast<uint32_t*>(misc), 32, st));
     int* nfb = nullptr;
-    if (ex.model || features_only) {
-        // per-group features -> network -> per-group tolerance (or the reference's)
-        gz::DevMem& sf = ctx->scratch[8];
-        const size_t fbytes = ((size_t)ngroups * gz::kMlpIn * sizeof(float) + 255) & ~size_t(255);
-        const size_t pbytes = ((size_t)ngroups * sizeof(float) + 255) & ~size_t(255);
-        GZ_TRY(sf.reserve(fbytes + 2 * pbytes + 256));
-        float* feats = features_only ? ex.feats_out : sf.as<float>();
-        float* pred = reinterpret_cast<float*>(sf.as<uint8_t>() + fbytes);
-        group_scale = reinterpret_cast<float*>(sf.as<uint8_t>() + fbytes + pbytes);
-        nfb = reinterpret_cast<int*>(sf.as<uint8_t>() + fbytes + 2 * pbytes);
+    if (groups_only) {
+        GZ_HIP(hipMemcpyAsync(ex.gstart_out, gstart, ngroups * sizeof(int),
+                              hipMemcpyDeviceToDevice, st));
+        const int end = (int)n_t;
+        GZ_HIP(hipMemcpyAsync(ex.gstart_out + ngroups, &end, sizeof(int), hipMemcpyHostToDevice,
+                              st));
+        GZ_HIP(hipStreamSynchronize(st));
+        *ex.ngroups_out = ngroups;
+        return GZ_OK;
+    }
+    if (features_only) {
+        // the groups' features for training (the walk computes the same ones for itself)
         GZ_TRY(gz::launch_group_features((int)ngroups, (int)n_t, gstart, targets, p, aold, tree,
-                                         feats, st));
-        if (features_only) {
-            GZ_HIP(hipMemcpyAsync(ex.gstart_out, gstart, ngroups * sizeof(int),
-                                  hipMemcpyDeviceToDevice, st));
-            const int end = (int)n_t;
-            GZ_HIP(hipMemcpyAsync(ex.gstart_out + ngroups, &end, sizeof(int),
-                                  hipMemcpyHostToDevice, st));
-            GZ_HIP(hipStreamSynchronize(st));
-            *ex.ngroups_out = ngroups;
-            return GZ_OK;
-        }
-        gz_model* M = ex.model;
-        GZ_TRY(gz::fill_zero_u32(reinterpret_cast<uint32_t*>(nfb), 1, st));
-        GZ_TRY(gz::launch_mlp_forward(feats, (int)ngroups, M->wbf, M->theta, M->xnorm, pred, st));
-        GZ_TRY(gz::launch_group_scale((int)ngroups, feats, pred, M->frange, M->lab_lo, M->lab_hi,
-                                      M->margin, M->log_alpha_ref, group_scale, nfb, st));
+                                         ex.feats_out, st));
+        GZ_HIP(hipMemcpyAsync(ex.gstart_out, gstart, ngroups * sizeof(int),
+                              hipMemcpyDeviceToDevice, st));
+        const int end = (int)n_t;
+        GZ_HIP(hipMemcpyAsync(ex.gstart_out + ngroups, &end, sizeof(int), hipMemcpyHostToDevice,
+                              st));
+        GZ_HIP(hipStreamSynchronize(st));
+        *ex.ngroups_out = ngroups;
+        return GZ_OK;
+    }
+    if (ex.model) {
+        nfb = reinterpret_cast<int*>(misc + 8);
     }
     GZ_HIP(hipEventRecord(ev[9], st));
 
@@ -178,8 +181,28 @@ gz_status walk(gz_ctx* ctx, const gz_tree* tree, const gz_particles* p, const ui
     a.gstart = gstart;
     a.nt = (int)n_t;
     a.aold = aold;
-    a.group_scale = group_scale;
+    a.group_scale = nullptr;
     a.theta_sq = theta * theta;
+    if (ex.model) {
+        const gz_model* M = ex.model;
+        a.m = p->m;
+        a.leaf_of = tree->leaf_of;
+        a.level_begin = tree->level_begin_dev;
+        a.root_cx = tree->root_com[0];
+        a.root_cy = tree->root_com[1];
+        a.root_cz = tree->root_com[2];
+        a.root_side = tree->root_box[3];
+        a.root_mass = tree->root_mass;
+        a.mlp_w = M->wbf;
+        a.mlp_theta = M->theta;
+        a.mlp_xnorm = M->xnorm;
+        a.mlp_frange = M->frange;
+        a.mlp_lab_lo = M->lab_lo;
+        a.mlp_lab_hi = M->lab_hi;
+        a.mlp_margin = M->margin;
+        a.mlp_log_alpha_ref = M->log_alpha_ref;
+        a.n_fallback = nfb;
+    }
     a.ngroups = (int)ngroups;
     a.root = tree->root;
     a.persistent = persistent;
@@ -198,11 +221,10 @@ gz_status walk(gz_ctx* ctx, const gz_tree* tree, const gz_particles* p, const ui
     GZ_TRY(gz::launch_walk(a, fp64, cnt, st));
     GZ_HIP(hipEventRecord(ev[10], st));
 
-    long long host_misc[8] = {0};
+    long long host_misc[16] = {0};
     GZ_HIP(hipMemcpyAsync(host_misc, misc, sizeof host_misc, hipMemcpyDeviceToHost, st));
-    int n_fallback = 0;
-    if (nfb) GZ_HIP(hipMemcpyAsync(&n_fallback, nfb, sizeof n_fallback, hipMemcpyDeviceToHost, st));
     GZ_HIP(hipStreamSynchronize(st));
+    const int n_fallback = (int)(host_misc[8] & 0xFFFFFFFF);
     const int retries = (int)(host_misc[7] & 0xFFFFFFFF);
     if (retries > 0) {
         // groups whose node stack outgrew LDS: walk them again with global-memory stacks. A
@@ -211,13 +233,10 @@ gz_status walk(gz_ctx* ctx, const gz_tree* tree, const gz_particles* p, const ui
         const int per_launch = std::max(1, (1 << 28) / cap); // at most 1 GiB of stacks
         gz::DevMem& gs = ctx->scratch[4];
         GZ_TRY(gs.reserve((size_t)std::min(per_launch, retries) * cap * sizeof(int)));
-        for (int off = 0; off < retries; off += per_launch) {
-            gz::WalkArgs r = a;
-            r.retry_list = retry_list + off;
-            r.gstack = gs.as<int>();
-            r.gstack_cap = cap;
-            GZ_TRY(gz::launch_walk_retry(r, std::min(per_launch, retries - off), fp64, cnt, st));
-        }
+        gz::WalkArgs r = a;
+        r.gstack = gs.as<int>();
+        r.gstack_cap = cap;
+        GZ_TRY(gz::launch_walk_retry(r, std::min(per_launch, retries), fp64, cnt, st));
         GZ_HIP(hipMemcpyAsync(host_misc, misc, sizeof host_misc, hipMemcpyDeviceToHost, st));
         GZ_HIP(hipStreamSynchronize(st));
         if (host_misc[6] & 1)
@@ -482,6 +501,234 @@ gz_status gz_tree_group_features(gz_ctx* ctx, const gz_tree* tree, const gz_part
     ex.ngroups_out = n_groups;
     return walk(ctx, tree, p, targets, n_t, 0.0, 1.0, 0, group_size, aold, GZ_PREC_FP32_REL,
                 0, 0, 0, 0, 1, nullptr, nullptr, nullptr, nullptr, "gz_tree_group_features", ex);
+}
+
+// ---- walk plans ---------------------------------------------------------------------------------
+gz_status gz_walk_plan_create(gz_ctx* ctx, const gz_tree* tree, const gz_particles* p,
+                              const uint32_t* targets, size_t n_t, const size_t* seg_start,
+                              int n_seg, const gz_walk_opts* opts, gz_walk_plan** out) {
+    gz::clear_error();
+    if (!out) return fail(GZ_ERR_INVALID, "gz_walk_plan_create: null out");
+    *out = nullptr;
+    GZ_TRY(check_ctx(ctx));
+    if (!tree || !p) return fail(GZ_ERR_INVALID, "gz_walk_plan_create: null tree or particles");
+    if (n_seg < 0 || (n_seg > 1 && !seg_start))
+        return fail(GZ_ERR_INVALID, "gz_walk_plan_create: bad segments");
+    int precision = GZ_PREC_FP64, group_size = 0, persistent = 1;
+    gz_model* model = nullptr;
+    if (opts) {
+        precision = opts->precision;
+        group_size = opts->group_size;
+        persistent = opts->launch == GZ_LAUNCH_QUEUE;
+        if (opts->count) return fail(GZ_ERR_UNSUPPORTED, "gz_walk_plan_create: counting");
+        if (opts->criterion == GZ_CRIT_LEARNED) {
+            if (!opts->model) return fail(GZ_ERR_INVALID, "gz_walk_plan_create: no model");
+            model = static_cast<gz_model*>(const_cast<void*>(opts->model));
+        }
+    }
+    if (n_seg > 1) {
+        if (seg_start[0] != 0 || seg_start[n_seg] != n_t)
+            return fail(GZ_ERR_INVALID, "gz_walk_plan_create: segments must cover [0, n_t)");
+        for (int k = 0; k < n_seg; ++k)
+            if (seg_start[k + 1] < seg_start[k])
+                return fail(GZ_ERR_INVALID, "gz_walk_plan_create: segments must be ordered");
+    }
+    gz_walk_plan* P = new (std::nothrow) gz_walk_plan();
+    if (!P) return fail(GZ_ERR_NOMEM, "gz_walk_plan_create: out of host memory");
+    P->ctx = ctx;
+    P->tree = tree;
+    P->p = *p;
+    P->targets = targets;
+    P->n_t = n_t;
+    P->precision = precision;
+    P->persistent = persistent;
+    P->model = model;
+    // device memory: group starts (n_t + 1), counters (16 x i64), retry list (n_t), segment
+    // offsets, and the global stacks of the re-walk
+    P->gstack_cap = std::max(1024, std::min(tree->nnodes, 1 << 18));
+    P->retry_waves = 64;
+    Carver c;
+    const size_t o_gs = c.add<int>(n_t + 1), o_misc = c.add<long long>(16),
+                 o_rl = c.add<int>(n_t + 1), o_seg = c.add<uint32_t>(n_seg + 2),
+                 o_stk = c.add<int>((size_t)P->retry_waves * P->gstack_cap);
+    hipError_t e = hipMalloc(&P->mem, c.used);
+    if (e != hipSuccess) {
+        delete P;
+        return gz::hip_fail(e, "hipMalloc", __FILE__, __LINE__);
+    }
+    uint8_t* b = static_cast<uint8_t*>(P->mem);
+    P->gstart = reinterpret_cast<int*>(b + o_gs);
+    P->misc = reinterpret_cast<long long*>(b + o_misc);
+    P->retry_list = reinterpret_cast<int*>(b + o_rl);
+    P->gstack = reinterpret_cast<int*>(b + o_stk);
+    uint32_t* seg = reinterpret_cast<uint32_t*>(b + o_seg);
+    gz_status s = GZ_OK;
+    if (n_seg > 1) {
+        std::vector<uint32_t> h(n_seg + 1);
+        for (int k = 0; k <= n_seg; ++k) h[k] = (uint32_t)seg_start[k];
+        if (hipMemcpy(seg, h.data(), h.size() * sizeof(uint32_t), hipMemcpyHostToDevice) !=
+            hipSuccess)
+            s = fail(GZ_ERR_HIP, "gz_walk_plan_create: upload failed");
+    }
+    if (s == GZ_OK && n_t > 0) {
+        WalkExtra ex;
+        ex.groups_only = true;
+        ex.gstart_out = P->gstart;
+        size_t ng = 0;
+        ex.ngroups_out = &ng;
+        ex.seg = seg;
+        ex.nseg = n_seg;
+        s = walk(ctx, tree, p, targets, n_t, 0.0, 1.0, 0, group_size, nullptr, precision,
+                 persistent, 0, 0, 0, 1, nullptr, nullptr, nullptr, nullptr,
+                 "gz_walk_plan_create", ex);
+        P->ngroups = (int)ng;
+    }
+    if (s != GZ_OK) {
+        gz_walk_plan_free(P);
+        return s;
+    }
+    *out = P;
+    return GZ_OK;
+}
+
+gz_status gz_walk_plan_launch(gz_ctx* ctx, gz_walk_plan* P, double theta, double G,
+                              const double* aold, double* ax, double* ay, double* az) {
+    gz::clear_error();
+    if (!ctx || !P) return fail(GZ_ERR_INVALID, "gz_walk_plan_launch: null argument");
+    if (P->n_t > 0 && (!ax || !ay || !az))
+        return fail(GZ_ERR_INVALID, "gz_walk_plan_launch: null output");
+    if (P->model && !aold)
+        return fail(GZ_ERR_INVALID, "gz_walk_plan_launch: the learned criterion needs aold");
+    if (P->n_t == 0) return GZ_OK;
+    hipStream_t st = ctx->stream;
+    const gz_tree* tree = P->tree;
+    GZ_TRY(gz::fill_zero_u32(reinterpret_cast<uint32_t*>(P->misc), 32, st));
+    gz::WalkArgs a{};
+    a.wn = tree->wn;
+    a.src = tree->src;
+    a.src_soft = tree->src_soft;
+    a.targets = P->targets;
+    a.x = P->p.x; a.y = P->p.y; a.z = P->p.z; a.soft = P->p.soft;
+    a.groups = nullptr;
+    a.gstart = P->gstart;
+    a.nt = (int)P->n_t;
+    a.aold = aold;
+    a.group_scale = nullptr;
+    a.theta_sq = theta * theta;
+    a.ngroups = P->ngroups;
+    a.root = tree->root;
+    a.persistent = P->persistent;
+    a.leaf_always_open = 0;
+    a.queue = reinterpret_cast<int*>(P->misc + 5);
+    a.status = reinterpret_cast<int*>(P->misc + 6);
+    a.counts = P->misc;
+    a.n_retry = reinterpret_cast<int*>(P->misc + 7);
+    a.retry_list = P->retry_list;
+    a.G = G;
+    a.ax = ax; a.ay = ay; a.az = az;
+    if (P->model) {
+        const gz_model* M = P->model;
+        a.m = P->p.m;
+        a.leaf_of = tree->leaf_of;
+        a.level_begin = tree->level_begin_dev;
+        a.root_cx = tree->root_com[0];
+        a.root_cy = tree->root_com[1];
+        a.root_cz = tree->root_com[2];
+        a.root_side = tree->root_box[3];
+        a.root_mass = tree->root_mass;
+        a.mlp_w = M->wbf;
+        a.mlp_theta = M->theta;
+        a.mlp_xnorm = M->xnorm;
+        a.mlp_frange = M->frange;
+        a.mlp_lab_lo = M->lab_lo;
+        a.mlp_lab_hi = M->lab_hi;
+        a.mlp_margin = M->margin;
+        a.mlp_log_alpha_ref = M->log_alpha_ref;
+        a.n_fallback = reinterpret_cast<int*>(P->misc + 8);
+    }
+    const bool fp64 = P->precision == GZ_PREC_FP64;
+    GZ_TRY(gz::launch_walk(a, fp64, false, st));
+    // the re-walk of groups whose node stack outgrew LDS; it exits at once when there are none
+    gz::WalkArgs r = a;
+    r.gstack = P->gstack;
+    r.gstack_cap = P->gstack_cap;
+    GZ_TRY(gz::launch_walk_retry(r, P->retry_waves, fp64, false, st));
+    return GZ_OK;
+}
+
+gz_status gz_walk_plan_status(gz_ctx* ctx, gz_walk_plan* P, gz_walk_counts* counts) {
+    if (!ctx || !P) return fail(GZ_ERR_INVALID, "gz_walk_plan_status: null argument");
+    long long h[16] = {0};
+    GZ_HIP(hipMemcpyAsync(h, P->misc, sizeof h, hipMemcpyDeviceToHost, ctx->stream));
+    GZ_HIP(hipStreamSynchronize(ctx->stream));
+    if (counts) {
+        std::memset(counts, 0, sizeof *counts);
+        counts->groups = P->ngroups;
+        counts->items = P->ngroups;
+        counts->retries = h[7] & 0xFFFFFFFF;
+        counts->fallback_groups = h[8] & 0xFFFFFFFF;
+    }
+    if (h[6] & 1)
+        return fail(GZ_ERR_INTERNAL, "gz_walk_plan: a group's node stack outgrew %d entries",
+                    P->gstack_cap);
+    return GZ_OK;
+}
+
+gz_status gz_walk_plan_free(gz_walk_plan* P) {
+    if (!P) return GZ_OK;
+    if (P->mem) {
+        (void)hipStreamSynchronize(P->ctx->stream);
+        (void)hipFree(P->mem);
+    }
+    delete P;
+    return GZ_OK;
+}
+
+// ---- graphs ---------------------------------------------------------------------------------------
+} // extern "C"
+struct gz_graph {
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+};
+extern "C" {
+
+gz_status gz_graph_begin(gz_ctx* ctx) {
+    gz::clear_error();
+    GZ_TRY(check_ctx(ctx));
+    GZ_HIP(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+    return GZ_OK;
+}
+
+gz_status gz_graph_end(gz_ctx* ctx, gz_graph** out) {
+    gz::clear_error();
+    if (!out) return fail(GZ_ERR_INVALID, "gz_graph_end: null out");
+    *out = nullptr;
+    GZ_TRY(check_ctx(ctx));
+    gz_graph* g = new (std::nothrow) gz_graph();
+    if (!g) return fail(GZ_ERR_NOMEM, "gz_graph_end: out of host memory");
+    hipError_t e = hipStreamEndCapture(ctx->stream, &g->graph);
+    if (e == hipSuccess) e = hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0);
+    if (e != hipSuccess) {
+        gz_graph_free(g);
+        return gz::hip_fail(e, "graph capture", __FILE__, __LINE__);
+    }
+    *out = g;
+    return GZ_OK;
+}
+
+gz_status gz_graph_launch(gz_ctx* ctx, gz_graph* g) {
+    gz::clear_error();
+    if (!ctx || !g) return fail(GZ_ERR_INVALID, "gz_graph_launch: null argument");
+    GZ_HIP(hipGraphLaunch(g->exec, ctx->stream));
+    return GZ_OK;
+}
+
+gz_status gz_graph_free(gz_graph* g) {
+    if (!g) return GZ_OK;
+    if (g->exec) (void)hipGraphExecDestroy(g->exec);
+    if (g->graph) (void)hipGraphDestroy(g->graph);
+    delete g;
+    return GZ_OK;
 }
 
 // ---- the learned model ------------------------------------------------------------------------------

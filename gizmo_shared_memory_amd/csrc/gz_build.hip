@@ -554,6 +554,14 @@ gz_status build_tree(gz_ctx* ctx, const gz_particles* p, const gz_build_opts* op
     GZ_HIP(hipStreamSynchronize(st));
     t->nleaves = nleaf;
     t->root = 0;
+    {
+        WNode r;
+        GZ_HIP(hipMemcpy(&r, t->wn, sizeof r, hipMemcpyDeviceToHost));
+        t->root_com[0] = r.cx;
+        t->root_com[1] = r.cy;
+        t->root_com[2] = r.cz;
+        t->root_mass = r.mass;
+    }
     t->times.bbox = ev_ms(ev[0], ev[1]);
     t->times.keys = ev_ms(ev[1], ev[2]);
     t->times.sort = ev_ms(ev[2], ev[3]);

@@ -1,0 +1,99 @@
+// Device functions shared by the feature kernel (training data) and the walk kernel (inference):
+// the 16 features of a target group, and one wavefront's evaluation of the opening-parameter
+// network for its group. Keeping one copy of each is what makes the features a group sees in
+// training the features it sees in the walk.
+#pragma once
+
+#include "gz_internal.h"
+
+namespace gz {
+
+// What the features need to know about a group: its box, the sums over its targets, and the tree
+// cell of its first target.
+struct GroupStats {
+    double lo[3], hi[3];
+    double emax;       // largest softening
+    double msum;       // total mass
+    double amin, amax; // min / max of |a_prev| (aold units)
+    double a2;         // sum of squares of |a_prev|
+    int nt;            // targets
+    double leaf_len;   // side of the first target's leaf
+    int depth;         // level of that leaf
+};
+
+struct RootInfo {
+    double cx, cy, cz, side, mass;
+};
+
+__device__ __forceinline__ float flog10(double v) { return (float)log10(fmax(v, 1e-300)); }
+
+// All features are ratios of the group's own scales (extent, leaf cell, softening, field, mass)
+// and of its position in the system, so none depends on the units or the box.
+__device__ __forceinline__ void group_features16(const GroupStats& S, const RootInfo& R,
+                                                 float* f) {
+    const double side = fmax(R.side, 1e-300);
+    const double Mr = fmax(R.mass, 1e-300);
+    const double ex = S.hi[0] - S.lo[0], ey = S.hi[1] - S.lo[1], ez = S.hi[2] - S.lo[2];
+    const double ext = fmax(fmax(ex, ey), fmax(ez, 1e-12 * side));
+    const double emn = fmax(fmin(ex, fmin(ey, ez)), 1e-12 * side);
+    const double cx = 0.5 * (S.lo[0] + S.hi[0]) - R.cx, cy = 0.5 * (S.lo[1] + S.hi[1]) - R.cy,
+                 cz = 0.5 * (S.lo[2] + S.hi[2]) - R.cz;
+    const double dist = fmax(sqrt(cx * cx + cy * cy + cz * cz), 1e-12 * side);
+    const double arms = sqrt(S.a2 / fmax((double)S.nt, 1.0));
+    const double mg = fmax(S.msum, 1e-300);
+    const double rho_g = mg / (ext * ext * ext), rho_c = Mr / (dist * dist * dist);
+    const double amn = fmax(S.amin, 1e-300);
+    f[0] = flog10((double)S.nt);
+    f[1] = flog10(ext / S.leaf_len);
+    f[2] = flog10(amn * ext * ext / mg);      // external field / self field
+    f[3] = flog10(S.amax / amn);
+    f[4] = flog10(S.emax / ext);
+    f[5] = flog10(emn / ext);                 // aspect of the group box
+    f[6] = flog10(arms / amn);
+    f[7] = (float)S.depth / 20.0f;
+    f[8] = flog10(ext / dist);                // angular size seen from the centre of mass
+    f[9] = (float)(fabs(cz) / dist);
+    f[10] = flog10(Mr / (dist * dist * amn)); // bulk monopole field / actual field
+    f[11] = flog10(mg / Mr);
+    f[12] = flog10(rho_g / rho_c);            // local over enclosed-scale density
+    f[13] = flog10(S.leaf_len / dist);
+    f[14] = flog10(S.emax / dist);
+    f[15] = flog10(S.emax / S.leaf_len);
+}
+
+constexpr int kFIn = 16, kFH = 64;
+constexpr int kFOffW1 = 0, kFOffB1 = kFH * kFIn, kFOffW2 = kFOffB1 + kFH,
+              kFOffB2 = kFOffW2 + kFH * kFH, kFOffW3 = kFOffB2 + kFH, kFOffB3 = kFOffW3 + kFH;
+
+__device__ __forceinline__ float bf16f(uint16_t b) { return __uint_as_float((uint32_t)b << 16); }
+__device__ __forceinline__ float gelu_f(float x) {
+    return 0.5f * x * (1.0f + erff(x * 0.70710678118654752f));
+}
+
+// The network for one group, evaluated by its wavefront: lane j owns hidden unit j of both
+// layers, the first layer's outputs are exchanged through LDS (sh, 64 floats), and the output is
+// a wave reduction. f must be the same in every lane. Weight matrices are bf16, biases and the
+// input map FP32; all arithmetic is FP32. Returns the output in every lane.
+__device__ __forceinline__ float mlp_wave(const float* f, const uint16_t* __restrict__ wbf,
+                                          const float* __restrict__ theta,
+                                          const float* __restrict__ xnorm, float* sh, int lane) {
+    float x[kFIn];
+#pragma unroll
+    for (int k = 0; k < kFIn; ++k) x[k] = (f[k] - xnorm[k]) * xnorm[kFIn + k];
+    float z = theta[kFOffB1 + lane];
+#pragma unroll
+    for (int k = 0; k < kFIn; ++k) z = __builtin_fmaf(bf16f(wbf[kFOffW1 + lane * kFIn + k]), x[k], z);
+    __syncthreads();
+    sh[lane] = gelu_f(z);
+    __syncthreads();
+    float z2 = theta[kFOffB2 + lane];
+    for (int i = 0; i < kFH; ++i)
+        z2 = __builtin_fmaf(bf16f(wbf[kFOffW2 + lane * kFH + i]), sh[i], z2);
+    float y = bf16f(wbf[kFOffW3 + lane]) * gelu_f(z2);
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) y += __shfl_xor(y, d);
+    __syncthreads();
+    return y + theta[kFOffB3];
+}
+
+} // namespace gz

@@ -54,6 +54,18 @@ struct WalkArgs {
     const double* aold;      // per target slot, or NULL
     const float* group_scale; // per group multiplier of aold_min (learned tolerance), or NULL
     double theta_sq;
+    // learned criterion evaluated in the walk (mlp_w non-NULL): the group's features, the
+    // network, and the tolerance or the fallback to the reference's
+    const double* m;
+    const int* leaf_of;
+    const int* level_begin;
+    double root_cx, root_cy, root_cz, root_side, root_mass;
+    const uint16_t* mlp_w;
+    const float* mlp_theta;
+    const float* mlp_xnorm;
+    const float* mlp_frange;
+    float mlp_lab_lo, mlp_lab_hi, mlp_margin, mlp_log_alpha_ref;
+    int* n_fallback;
     int ngroups;
     int root;
     int persistent;
@@ -119,14 +131,15 @@ void clear_error();
 // ---- walks (gz_walk.hip) ------------------------------------------------------------------------
 gz_status launch_group_prep(const uint32_t* targets, size_t nt, const int* leaf_of,
                             const int* parent, const WNode* wn, int batch, int group_size,
-                            size_t n, int* gkey, uint32_t* flag, uint32_t* gid,
-                            uint32_t* ngroups_dev, int* status, void* scan_tmp, hipStream_t st);
+                            size_t n, const uint32_t* seg, int nseg, int* gseg, int* gkey,
+                            uint32_t* flag, uint32_t* gid, uint32_t* ngroups_dev, int* status,
+                            void* scan_tmp, hipStream_t st);
 gz_status launch_group_starts(size_t nt, const uint32_t* flag, const uint32_t* gid, int* gstart,
                               hipStream_t st);
 gz_status launch_group_pack(int ngroups, size_t nt, int group_size, const int* gstart,
-                            const int* gkey, const int* parent, int* pk, uint32_t* mflag,
-                            uint32_t* mid, uint32_t* ngroups2_dev, int* gstart2, void* scan_tmp,
-                            hipStream_t st);
+                            const int* gkey, const int* parent, const int* gseg, long long* pk,
+                            uint32_t* mflag, uint32_t* mid, uint32_t* ngroups2_dev, int* gstart2,
+                            void* scan_tmp, hipStream_t st);
 gz_status launch_walk(const WalkArgs& a, bool fp64, bool count, hipStream_t st);
 gz_status launch_walk_retry(const WalkArgs& a, int n_retry, bool fp64, bool count,
                             hipStream_t st);
@@ -145,10 +158,6 @@ gz_status launch_mlp_grad(const float* X, const float* T, int B, const uint16_t*
 gz_status launch_adam(float* theta, float* m, float* v, const float* grad, float lr, float b1,
                       float b2, float eps, int step, uint16_t* wbf, hipStream_t st);
 gz_status launch_to_bf16(const float* theta, uint16_t* wbf, hipStream_t st);
-gz_status launch_group_scale(int ngroups, const float* feats, const float* pred,
-                             const float* frange, float lab_lo, float lab_hi, float margin,
-                             float log_alpha_ref, float* scale, int* n_fallback,
-                             hipStream_t st);
 
 } // namespace gz
 
@@ -159,6 +168,26 @@ struct gz_ctx {
     bool own_stream = false;
     gz::DevMem scratch[12];
     hipEvent_t ev[16] = {};
+};
+
+// A walk whose groups were formed once for a fixed target list; launching it enqueues only
+// kernels (no host synchronisation), so launches can be captured into a hipGraph.
+struct gz_walk_plan {
+    gz_ctx* ctx = nullptr;
+    const gz_tree* tree = nullptr;
+    gz_particles p = {};
+    const uint32_t* targets = nullptr;
+    size_t n_t = 0;
+    int ngroups = 0;
+    int precision = 0;
+    int persistent = 1;
+    gz_model* model = nullptr;
+    int gstack_cap = 0, retry_waves = 0;
+    void* mem = nullptr; // gstart, counters, retry list, global stacks
+    int* gstart = nullptr;
+    long long* misc = nullptr;
+    int* retry_list = nullptr;
+    int* gstack = nullptr;
 };
 
 // The learned opening-parameter network: FP32 master weights with Adam state, their bf16 copy
@@ -182,6 +211,7 @@ struct gz_tree {
     int nnodes = 0, nleaves = 0, depth = 0, arena_grows = 0, root = 0;
     int cap = 0;
     double root_box[4] = {0, 0, 0, 0};
+    double root_com[3] = {0, 0, 0}, root_mass = 0; // the root node's moments
     std::vector<int> level_begin; // node ids of level L are [level_begin[L], level_begin[L+1])
     int* level_begin_dev = nullptr; // the same table on the device (kMaxLevel + 2 entries)
     gz_build_times times = {};

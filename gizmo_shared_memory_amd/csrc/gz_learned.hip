@@ -10,7 +10,7 @@
 // tolerance falls outside the trained label range, keeps the reference tolerance.
 #include <cmath>
 
-#include "gz_internal.h"
+#include "gz_features.h"
 
 namespace gz {
 
@@ -36,7 +36,6 @@ __device__ __forceinline__ float gelu_grad(float x) {
 
 // ---- group features ------------------------------------------------------------------------------
 // amag is |a_prev| per target slot (no-G units, the basis of aold).
-__device__ __forceinline__ float flog10(double v) { return (float)log10(fmax(v, 1e-300)); }
 
 __global__ __launch_bounds__(256) void k_group_features(
     int ngroups, int nt_total, const int* __restrict__ gstart,
@@ -67,46 +66,24 @@ __global__ __launch_bounds__(256) void k_group_features(
         lo1 = fmin(lo1, y[p]); hi1 = fmax(hi1, y[p]);
         lo2 = fmin(lo2, z[p]); hi2 = fmax(hi2, z[p]);
     }
-    G.lo[0] = lo0; G.lo[1] = lo1; G.lo[2] = lo2;
-    G.hi[0] = hi0; G.hi[1] = hi1; G.hi[2] = hi2;
-    G.emax = emax;
-    const WNode R = wn[root];
-    const double side = fmax((double)R.len, 1e-300);
-    const double Mr = fmax(R.mass, 1e-300);
-    const double ex = G.hi[0] - G.lo[0], ey = G.hi[1] - G.lo[1], ez = G.hi[2] - G.lo[2];
-    const double ext = fmax(fmax(ex, ey), fmax(ez, 1e-12 * side));
-    const double emn = fmax(fmin(ex, fmin(ey, ez)), 1e-12 * side);
-    const double cx = 0.5 * (G.lo[0] + G.hi[0]) - R.cx, cy = 0.5 * (G.lo[1] + G.hi[1]) - R.cy,
-                 cz = 0.5 * (G.lo[2] + G.hi[2]) - R.cz;
-    const double dist = fmax(sqrt(cx * cx + cy * cy + cz * cz), 1e-12 * side);
-    const double arms = sqrt(a2 / fmax((double)nt, 1.0));
-    const double mg = fmax(msum, 1e-300);
+    GroupStats S;
+    S.lo[0] = lo0; S.lo[1] = lo1; S.lo[2] = lo2;
+    S.hi[0] = hi0; S.hi[1] = hi1; S.hi[2] = hi2;
+    S.emax = emax;
+    S.msum = msum;
+    S.amin = amin;
+    S.amax = amax;
+    S.a2 = a2;
+    S.nt = nt;
+    const WNode Rn = wn[root];
+    const RootInfo R{Rn.cx, Rn.cy, Rn.cz, (double)Rn.len, Rn.mass};
     const int leaf = leaf_of[targets[G.tlo]];
-    const double leaf_len = (leaf >= 0) ? (double)wn[leaf].len : side;
-    int depth = 0; // node ids are level-major: the level is where the id falls in the table
+    S.leaf_len = (leaf >= 0) ? (double)wn[leaf].len : fmax(R.side, 1e-300);
+    S.depth = 0; // node ids are level-major: the level is where the id falls in the table
     if (leaf >= 0)
-        while (depth < kMaxLevel && level_begin[depth + 1] <= leaf) ++depth;
-    // All features are ratios of the group's own scales (extent, leaf cell, softening, field,
-    // mass) and of its position in the system, so none depends on the units or the box.
-    const double rho_g = mg / (ext * ext * ext), rho_c = Mr / (dist * dist * dist);
-    const double amn = fmax(amin, 1e-300);
+        while (S.depth < kMaxLevel && level_begin[S.depth + 1] <= leaf) ++S.depth;
     float f[kIn];
-    f[0] = flog10((double)nt);
-    f[1] = flog10(ext / leaf_len);
-    f[2] = flog10(amn * ext * ext / mg);       // external field / self field
-    f[3] = flog10(amax / amn);
-    f[4] = flog10(G.emax / ext);
-    f[5] = flog10(emn / ext);                  // aspect of the group box
-    f[6] = flog10(arms / amn);
-    f[7] = (float)depth / 20.0f;
-    f[8] = flog10(ext / dist);                 // angular size seen from the centre of mass
-    f[9] = (float)(fabs(cz) / dist);
-    f[10] = flog10(Mr / (dist * dist * amn));  // bulk monopole field / actual field
-    f[11] = flog10(mg / Mr);
-    f[12] = flog10(rho_g / rho_c);             // local over enclosed-scale density
-    f[13] = flog10(leaf_len / dist);
-    f[14] = flog10(G.emax / dist);
-    f[15] = flog10(G.emax / leaf_len);
+    group_features16(S, R, f);
     float* o = feats + (size_t)g * kIn;
 #pragma unroll
     for (int k = 0; k < kIn; ++k) o[k] = f[k];
@@ -312,30 +289,6 @@ __global__ __launch_bounds__(256) void k_to_bf16(const float* __restrict__ theta
     if (i < n) wbf[i] = f32_to_bf16(theta[i]);
 }
 
-// ---- per-group opening parameter ---------------------------------------------------------------------
-// scale = alpha_g / alpha_ref from the network's log10(alpha_g), or 1 (the reference tolerance)
-// when a feature lies outside [fmin, fmax] or the prediction outside [lab_lo, lab_hi].
-__global__ __launch_bounds__(256) void k_group_scale(int ngroups, const float* __restrict__ feats,
-                                                     const float* __restrict__ pred,
-                                                     const float* __restrict__ frange,
-                                                     float lab_lo, float lab_hi, float margin,
-                                                     float log_alpha_ref,
-                                                     float* __restrict__ scale,
-                                                     int* __restrict__ n_fallback) {
-    const int g = blockIdx.x * 256 + threadIdx.x;
-    if (g >= ngroups) return;
-    const float* f = feats + (size_t)g * kIn;
-    bool ok = true;
-#pragma unroll
-    for (int k = 0; k < kIn; ++k) ok = ok && (f[k] >= frange[k]) && (f[k] <= frange[kIn + k]);
-    // frange holds the trusted band (the caller widens the training range as it sees fit)
-    const float p = pred[g];
-    ok = ok && (p >= lab_lo) && (p <= lab_hi);
-    const float s = ok ? exp10f(p - margin - log_alpha_ref) : 1.0f;
-    scale[g] = s;
-    if (!ok) atomicAdd(n_fallback, 1);
-}
-
 // ---- host launchers ---------------------------------------------------------------------------------
 gz_status launch_group_features(int ngroups, int nt, const int* gstart, const uint32_t* targets,
                                 const gz_particles* p, const double* amag, const gz_tree* t,
@@ -377,17 +330,6 @@ gz_status launch_adam(float* theta, float* m, float* v, const float* grad, float
 
 gz_status launch_to_bf16(const float* theta, uint16_t* wbf, hipStream_t st) {
     k_to_bf16<<<(kMlpParams + 255) / 256, 256, 0, st>>>(theta, kMlpParams, wbf);
-    GZ_HIP(hipGetLastError());
-    return GZ_OK;
-}
-
-gz_status launch_group_scale(int ngroups, const float* feats, const float* pred,
-                             const float* frange, float lab_lo, float lab_hi, float margin,
-                             float log_alpha_ref, float* scale, int* n_fallback,
-                             hipStream_t st) {
-    if (ngroups <= 0) return GZ_OK;
-    k_group_scale<<<(unsigned)((ngroups + 255) / 256), 256, 0, st>>>(
-        ngroups, feats, pred, frange, lab_lo, lab_hi, margin, log_alpha_ref, scale, n_fallback);
     GZ_HIP(hipGetLastError());
     return GZ_OK;
 }

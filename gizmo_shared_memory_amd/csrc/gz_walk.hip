@@ -25,7 +25,7 @@
 #include <cmath>
 #include <cstdio>
 
-#include "gz_internal.h"
+#include "gz_features.h"
 
 namespace gz {
 
@@ -74,10 +74,21 @@ __global__ __launch_bounds__(256) void k_group_flags(const uint32_t* __restrict_
                                                      const int* __restrict__ parent,
                                                      const WNode* __restrict__ wn, int batch,
                                                      int group_size, size_t n,
+                                                     const uint32_t* __restrict__ seg, int nseg,
+                                                     int* __restrict__ gseg,
                                                      int* __restrict__ gkey,
                                                      int* __restrict__ status) {
     const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
     if (i >= nt) return;
+    if (nseg > 1) { // the segment (independent target list) this slot belongs to
+        int lo = 0, hi = nseg; // last s with seg[s] <= i
+        while (hi - lo > 1) {
+            const int mid = (lo + hi) >> 1;
+            if (seg[mid] <= i) lo = mid;
+            else hi = mid;
+        }
+        gseg[i] = lo;
+    }
     if (targets[i] >= n) { // reported to the caller before anything reads through it
         atomicOr(status, 2);
         gkey[i] = -1;
@@ -101,6 +112,7 @@ __global__ __launch_bounds__(256) void k_group_flags(const uint32_t* __restrict_
 
 __global__ __launch_bounds__(256) void k_group_flags2(size_t nt, int batch,
                                                       const int* __restrict__ gkey,
+                                                      const int* __restrict__ gseg,
                                                       uint32_t* __restrict__ flag) {
     const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
     if (i >= nt) return;
@@ -109,8 +121,9 @@ __global__ __launch_bounds__(256) void k_group_flags2(size_t nt, int batch,
         return;
     }
     const int k = gkey[i], kp = gkey[i - 1];
-    if (batch > 0) flag[i] = (k != kp) ? 1u : 0u;
-    else flag[i] = (k != kp || k < 0) ? 1u : 0u;
+    bool f = (batch > 0) ? (k != kp) : (k != kp || k < 0);
+    if (gseg && gseg[i] != gseg[i - 1]) f = true; // groups never span two segments
+    flag[i] = f ? 1u : 0u;
 }
 
 __global__ __launch_bounds__(256) void k_group_starts(size_t nt, const uint32_t* __restrict__ flag,
@@ -129,20 +142,23 @@ __global__ __launch_bounds__(256) void k_pack_parent(int ngroups, size_t nt,
                                                      const int* __restrict__ gstart,
                                                      const int* __restrict__ gkey,
                                                      const int* __restrict__ parent,
-                                                     int* __restrict__ pk) {
+                                                     const int* __restrict__ gseg,
+                                                     long long* __restrict__ pk) {
     const int i = blockIdx.x * 256 + threadIdx.x;
     if (i >= ngroups) return;
     const int key = gkey[gstart[i]];
-    pk[i] = (key >= 0) ? parent[key] : -2 - i; // a target outside the tree never merges
+    // merge partner: same parent node, same segment; a target outside the tree never merges
+    const long long sg = gseg ? (long long)gseg[gstart[i]] : 0;
+    pk[i] = (key >= 0 && parent[key] >= 0) ? ((sg << 32) | (long long)parent[key]) : -2 - i;
 }
 
 __global__ __launch_bounds__(256) void k_pack_greedy(int ngroups, size_t nt, int group_size,
                                                      const int* __restrict__ gstart,
-                                                     const int* __restrict__ pk,
+                                                     const long long* __restrict__ pk,
                                                      uint32_t* __restrict__ mflag) {
     const int i = blockIdx.x * 256 + threadIdx.x;
     if (i >= ngroups) return;
-    const int p = pk[i];
+    const long long p = pk[i];
     const bool seg_start = (i == 0) || (p < 0) || (pk[i - 1] != p);
     if (!seg_start) return;
     int acc = 0;
@@ -182,11 +198,12 @@ __device__ __forceinline__ double wave_max_d(double v) {
 // The opening bounds of group g (tree.cc:683-702: bounding box, largest softening, smallest aold),
 // computed by the group's own wavefront. Min and max are exact, so every lane ends with the same
 // values whatever the reduction order.
-__device__ GroupInfo group_bounds(const WalkArgs& a, int g, int lane) {
+__device__ __noinline__ GroupInfo group_bounds(const WalkArgs& a, int g, int lane, float* sh) {
     const int lo = a.gstart[g];
     const int hi = (g + 1 < a.ngroups) ? a.gstart[g + 1] : a.nt;
+    const bool learned = a.mlp_w != nullptr;
     double bx0 = 1e300, by0 = 1e300, bz0 = 1e300, bx1 = -1e300, by1 = -1e300, bz1 = -1e300;
-    double emax = 0, amin = a.aold ? 1e300 : 0.0;
+    double emax = 0, amin = a.aold ? 1e300 : 0.0, amax = 0, a2 = 0, msum = 0;
     for (int i = lo + lane; i < hi; i += 64) {
         const uint32_t p = a.targets[i];
         const double px = a.x[p], py = a.y[p], pz = a.z[p];
@@ -194,17 +211,61 @@ __device__ GroupInfo group_bounds(const WalkArgs& a, int g, int lane) {
         by0 = dmin_w(by0, py); by1 = dmax_w(by1, py);
         bz0 = dmin_w(bz0, pz); bz1 = dmax_w(bz1, pz);
         if (a.soft) emax = dmax_w(emax, a.soft[p]);
-        if (a.aold) amin = dmin_w(amin, a.aold[i]);
+        if (a.aold) {
+            const double ao = a.aold[i];
+            amin = dmin_w(amin, ao);
+            if (learned) {
+                amax = fmax(amax, ao);
+                a2 += ao * ao;
+            }
+        }
+        if (learned) msum += a.m[p];
     }
     GroupInfo o;
     o.lo[0] = wave_min_d(bx0); o.lo[1] = wave_min_d(by0); o.lo[2] = wave_min_d(bz0);
     o.hi[0] = wave_max_d(bx1); o.hi[1] = wave_max_d(by1); o.hi[2] = wave_max_d(bz1);
     o.emax = wave_max_d(emax);
     o.amin = wave_min_d(amin);
-    if (a.group_scale) o.amin *= (double)a.group_scale[g];
     o.theta_sq = a.theta_sq;
     o.tlo = lo;
     o.thi = hi;
+    if (a.group_scale) o.amin *= (double)a.group_scale[g];
+    if (learned) {
+        // the learned tolerance of this group (the same features and network as the training
+        // path, gz_learned.hip), or the reference's when the group is outside the trusted band
+        GroupStats S;
+        for (int k = 0; k < 3; ++k) {
+            S.lo[k] = o.lo[k];
+            S.hi[k] = o.hi[k];
+        }
+        S.emax = o.emax;
+        S.amin = o.amin;
+        S.amax = wave_max_d(amax);
+        double t = a2, u = msum;
+#pragma unroll
+        for (int d = 32; d >= 1; d >>= 1) {
+            t += __shfl_xor(t, d);
+            u += __shfl_xor(u, d);
+        }
+        S.a2 = t;
+        S.msum = u;
+        S.nt = hi - lo;
+        const int leaf = a.leaf_of[a.targets[lo]];
+        S.leaf_len = (leaf >= 0) ? (double)a.wn[leaf].len : fmax(a.root_side, 1e-300);
+        S.depth = 0;
+        if (leaf >= 0)
+            while (S.depth < kMaxLevel && a.level_begin[S.depth + 1] <= leaf) ++S.depth;
+        const RootInfo R{a.root_cx, a.root_cy, a.root_cz, a.root_side, a.root_mass};
+        float f[16];
+        group_features16(S, R, f);
+        const float pred = mlp_wave(f, a.mlp_w, a.mlp_theta, a.mlp_xnorm, sh, lane);
+        bool ok = (pred >= a.mlp_lab_lo) && (pred <= a.mlp_lab_hi);
+#pragma unroll
+        for (int k = 0; k < 16; ++k)
+            ok = ok && (f[k] >= a.mlp_frange[k]) && (f[k] <= a.mlp_frange[16 + k]);
+        if (ok) o.amin *= (double)exp10f(pred - a.mlp_margin - a.mlp_log_alpha_ref);
+        else if (lane == 0) atomicAdd(a.n_fallback, 1);
+    }
     return o;
 }
 
@@ -276,8 +337,9 @@ __global__ __launch_bounds__(64, 4) void k_walk(WalkArgs a) {
         long long g_visit = 0, g_multi = 0, g_pair = 0; // this group's share of the counts
         int g;
         if (GSTACK) {
-            if (item > 0) break;
-            g = a.retry_list[blockIdx.x];
+            const int k = blockIdx.x + item * gridDim.x;
+            if (k >= *a.n_retry) break;
+            g = a.retry_list[k];
         } else if (a.persistent) {
             int v = 0;
             if (lane == 0) v = atomicAdd(a.queue, 1);
@@ -295,7 +357,8 @@ __global__ __launch_bounds__(64, 4) void k_walk(WalkArgs a) {
             if (g >= a.ngroups) break;
         }
 
-        const GroupInfo gi = a.groups ? a.groups[g] : group_bounds(a, g, lane);
+        const GroupInfo gi =
+            a.groups ? a.groups[g] : group_bounds(a, g, lane, reinterpret_cast<float*>(s_lpre));
         const double gcx = 0.5 * (gi.lo[0] + gi.hi[0]);
         const double gcy = 0.5 * (gi.lo[1] + gi.hi[1]);
         const double gcz = 0.5 * (gi.lo[2] + gi.hi[2]);
@@ -661,10 +724,13 @@ gz_status launch_walk(const WalkArgs& a, bool fp64, bool count, hipStream_t st) 
     return GZ_OK;
 }
 
-// Groups whose node stack outgrew LDS, again, with the stack in global memory (n_retry waves).
-gz_status launch_walk_retry(const WalkArgs& a, int n_retry, bool fp64, bool count,
+// Groups whose node stack outgrew LDS, again, with the stack in global memory: `waves`
+// wavefronts (each with gstack_cap entries of a.gstack) take the groups listed in a.retry_list
+// until *a.n_retry is exhausted, so the launch needs no host-side count.
+gz_status launch_walk_retry(const WalkArgs& a, int waves, bool fp64, bool count,
                             hipStream_t st) {
-    if (n_retry <= 0) return GZ_OK;
+    if (waves <= 0) return GZ_OK;
+    const int n_retry = waves;
     if (fp64) {
         if (count) k_walk<true, true, true><<<n_retry, 64, 0, st>>>(a);
         else k_walk<true, false, true><<<n_retry, 64, 0, st>>>(a);
@@ -678,12 +744,13 @@ gz_status launch_walk_retry(const WalkArgs& a, int n_retry, bool fp64, bool coun
 
 gz_status launch_group_prep(const uint32_t* targets, size_t nt, const int* leaf_of,
                             const int* parent, const WNode* wn, int batch, int group_size,
-                            size_t n, int* gkey, uint32_t* flag, uint32_t* gid,
-                            uint32_t* ngroups_dev, int* status, void* scan_tmp, hipStream_t st) {
+                            size_t n, const uint32_t* seg, int nseg, int* gseg, int* gkey,
+                            uint32_t* flag, uint32_t* gid, uint32_t* ngroups_dev, int* status,
+                            void* scan_tmp, hipStream_t st) {
     const unsigned grid = (unsigned)((nt + 255) / 256);
     k_group_flags<<<grid, 256, 0, st>>>(targets, nt, leaf_of, parent, wn, batch, group_size, n,
-                                        gkey, status);
-    k_group_flags2<<<grid, 256, 0, st>>>(nt, batch, gkey, flag);
+                                        seg, nseg, gseg, gkey, status);
+    k_group_flags2<<<grid, 256, 0, st>>>(nt, batch, gkey, nseg > 1 ? gseg : nullptr, flag);
     GZ_HIP(hipGetLastError());
     GZ_TRY(exclusive_scan_u32(flag, gid, nt, ngroups_dev, scan_tmp, st));
     return GZ_OK;
@@ -697,12 +764,12 @@ gz_status launch_group_starts(size_t nt, const uint32_t* flag, const uint32_t* g
 }
 
 gz_status launch_group_pack(int ngroups, size_t nt, int group_size, const int* gstart,
-                            const int* gkey, const int* parent, int* pk, uint32_t* mflag,
-                            uint32_t* mid, uint32_t* ngroups2_dev, int* gstart2, void* scan_tmp,
-                            hipStream_t st) {
+                            const int* gkey, const int* parent, const int* gseg, long long* pk,
+                            uint32_t* mflag, uint32_t* mid, uint32_t* ngroups2_dev, int* gstart2,
+                            void* scan_tmp, hipStream_t st) {
     if (ngroups <= 0) return GZ_OK;
     const unsigned grid = (unsigned)((ngroups + 255) / 256);
-    k_pack_parent<<<grid, 256, 0, st>>>(ngroups, nt, gstart, gkey, parent, pk);
+    k_pack_parent<<<grid, 256, 0, st>>>(ngroups, nt, gstart, gkey, parent, gseg, pk);
     k_pack_greedy<<<grid, 256, 0, st>>>(ngroups, nt, group_size, gstart, pk, mflag);
     GZ_HIP(hipGetLastError());
     GZ_TRY(exclusive_scan_u32(mflag, mid, (size_t)ngroups, ngroups2_dev, scan_tmp, st));

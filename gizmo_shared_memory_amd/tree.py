@@ -350,6 +350,85 @@ def accel_grouped(T: Tree, P: Particles, targets, theta: float, G: float = 1.0, 
     return ax, ay, az
 
 
+class WalkPlan:
+    """Groups of a fixed target list formed once (gz_walk_plan); launch() only enqueues the walk,
+    so launches can be captured into a graph. segments: slot offsets (length n_seg + 1) of
+    independent target lists walked together, e.g. the active sets of several substeps."""
+
+    def __init__(self, T: Tree, P: Particles, targets, *, segments=None, precision="fp32",
+                 group_size=32, model: Model | None = None):
+        self.T, self.P = T, P
+        self.targets = _dev_u32(targets, P.device)
+        nt = int(self.targets.shape[0])
+        prec = {"fp64": L.PREC_FP64, "fp32": L.PREC_FP32_REL}[precision]
+        crit = L.CRIT_REFERENCE if model is None else L.CRIT_LEARNED
+        opts = L.gz_walk_opts(prec, crit, int(group_size), L.LAUNCH_QUEUE, 0, 0,
+                              None if model is None else model.handle)
+        self.model = model
+        seg = None
+        nseg = 0
+        if segments is not None:
+            arr = np.ascontiguousarray(segments, dtype=np.uint64)
+            seg = (C.c_size_t * len(arr))(*[int(v) for v in arr])
+            nseg = len(arr) - 1
+        h = C.c_void_p()
+        T.ctx.enter()
+        L.check(L.load().gz_walk_plan_create(T.ctx.handle, T.handle, P.cstruct(),
+                                             _ptr(self.targets), nt, seg, nseg, C.byref(opts),
+                                             C.byref(h)), "gz_walk_plan_create")
+        self.handle = h
+        self.nt = nt
+        self.ax, self.ay, self.az = _outputs(nt, P.device)
+
+    def __del__(self):
+        h = getattr(self, "handle", None)
+        lib = getattr(L, "_lib", None) if L is not None else None
+        if h is not None and lib is not None:
+            lib.gz_walk_plan_free(h)
+            self.handle = None
+
+    def launch(self, theta, G=1.0, aold=None):
+        """Enqueue the walk on the library stream (results in self.ax/ay/az once it finishes)."""
+        self._aold = None if aold is None else _dev_f64(aold, self.P.device)
+        L.check(L.load().gz_walk_plan_launch(self.T.ctx.handle, self.handle, float(theta),
+                                             float(G), _ptr(self._aold), _ptr(self.ax),
+                                             _ptr(self.ay), _ptr(self.az)),
+                "gz_walk_plan_launch")
+
+    def status(self) -> dict:
+        cnt = L.gz_walk_counts()
+        L.check(L.load().gz_walk_plan_status(self.T.ctx.handle, self.handle, C.byref(cnt)),
+                "gz_walk_plan_status")
+        return {k: getattr(cnt, k) for k, _ in L.gz_walk_counts._fields_}
+
+
+class Graph:
+    """Work enqueued on the library stream between begin() and end(), replayable (a hipGraph)."""
+
+    def __init__(self, ctx: Context):
+        self.ctx = ctx
+        self.handle = None
+
+    def begin(self):
+        self.ctx.enter()
+        L.check(L.load().gz_graph_begin(self.ctx.handle), "gz_graph_begin")
+
+    def end(self):
+        h = C.c_void_p()
+        L.check(L.load().gz_graph_end(self.ctx.handle, C.byref(h)), "gz_graph_end")
+        self.handle = h
+
+    def launch(self):
+        L.check(L.load().gz_graph_launch(self.ctx.handle, self.handle), "gz_graph_launch")
+
+    def __del__(self):
+        h = getattr(self, "handle", None)
+        lib = getattr(L, "_lib", None) if L is not None else None
+        if h is not None and lib is not None:
+            lib.gz_graph_free(h)
+            self.handle = None
+
+
 def accel(T: Tree, P: Particles, targets, theta: float, G: float = 1.0, aold=None):
     """shmem::accel (tree.h:392-394): per-target walk, reached leaves always summed."""
     tg = _dev_u32(targets, P.device)

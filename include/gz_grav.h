@@ -41,6 +41,7 @@ typedef enum {
 typedef struct gz_ctx gz_ctx;   /* one per device + stream */
 typedef struct gz_tree gz_tree; /* device-resident tree, owned by the library */
 typedef struct gz_model gz_model; /* learned opening-parameter network (device-resident) */
+typedef struct gz_walk_plan gz_walk_plan; /* groups of a fixed target list, walked repeatedly */
 
 /* Message for the last failing call on this thread ("" if none). */
 const char* gz_last_error(void);
@@ -157,6 +158,36 @@ gz_status gz_tree_accel_grouped(gz_ctx* ctx, const gz_tree* tree, const gz_parti
                                 const uint32_t* targets, size_t n_t, double theta, double G,
                                 int batch, const double* aold, double* ax, double* ay, double* az,
                                 const gz_walk_opts* opts, gz_walk_counts* counts);
+
+/* ---- walk plans: batched and graph-captured walks ------------------------------------------
+ * No counterpart in the reference, which groups targets inside every accel_grouped call
+ * (tree.cc:656-667). A plan forms the groups of a fixed target list once; launching it enqueues
+ * only the walk kernels, with no host synchronisation, so a sequence of launches can be captured
+ * into a hipGraph and replayed. seg_start (host, n_seg + 1 slot offsets into targets, or NULL for
+ * one list) splits the targets into independent lists, e.g. the active sets of several substeps,
+ * whose groups never mix: one launch then walks all of them through one work queue. opts as for
+ * gz_tree_accel_grouped (counting is not available in plans). The plan keeps pointers to tree,
+ * p and targets, which must stay valid while it is used. */
+gz_status gz_walk_plan_create(gz_ctx* ctx, const gz_tree* tree, const gz_particles* p,
+                              const uint32_t* targets, size_t n_t, const size_t* seg_start,
+                              int n_seg, const gz_walk_opts* opts, gz_walk_plan** out);
+/* Enqueue the walk on the context's stream: as gz_tree_accel_grouped with batch = 0. */
+gz_status gz_walk_plan_launch(gz_ctx* ctx, gz_walk_plan* plan, double theta, double G,
+                              const double* aold, double* ax, double* ay, double* az);
+/* Synchronise and report the last launch: groups, retries (groups re-walked with a global
+ * node stack), fallback_groups. GZ_ERR_INTERNAL if a global node stack overflowed. */
+gz_status gz_walk_plan_status(gz_ctx* ctx, gz_walk_plan* plan, gz_walk_counts* counts);
+gz_status gz_walk_plan_free(gz_walk_plan* plan);
+
+/* Capture the work enqueued on the context's stream between begin and end into an executable
+ * hipGraph (hipStreamBeginCapture / hipGraphInstantiate), and replay it. Only plan launches and
+ * the caller-side kernels (order, aold, scatter, kick) may be enqueued while capturing: calls
+ * that synchronise fail. */
+typedef struct gz_graph gz_graph;
+gz_status gz_graph_begin(gz_ctx* ctx);
+gz_status gz_graph_end(gz_ctx* ctx, gz_graph** out);
+gz_status gz_graph_launch(gz_ctx* ctx, gz_graph* graph);
+gz_status gz_graph_free(gz_graph* graph);
 
 /* shmem::accel (tree.h:392-394, tree.cc:528-582): per-target walk, a reached leaf is always
  * direct-summed. */
