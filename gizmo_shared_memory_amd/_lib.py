@@ -48,7 +48,8 @@ class gz_particles(C.Structure):
 
 class gz_build_opts(C.Structure):
     _fields_ = [("randomize_seed", C.c_longlong), ("shift", C.c_double * 3),
-                ("max_nodes", C.c_int)]
+                ("max_nodes", C.c_int), ("cluster_start", C.POINTER(C.c_size_t)),
+                ("n_clusters", C.c_int)]
 
 
 class gz_build_times(C.Structure):

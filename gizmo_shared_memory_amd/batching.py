@@ -1,0 +1,192 @@
+"""The batching workloads: uneven active sets of a hierarchical timestep, and many independent
+small clusters, walked through one work queue and replayed from a captured graph.
+
+(a) Substeps. The 8,388,608-particle halo+disk with power-of-two timestep bins (5 levels, 16
+    substeps; level b holds 4^-b of the particles, placed where the dynamical time is short). At
+    substep s the active set is every level whose step ends at s (the reference's hierarchical
+    timebins, shmem/mfm.h:276-306); its targets are put in Morton order (gz_order_targets, the
+    reference's mfm.cc:700-718) and walked with the relative criterion (aold from the bootstrap
+    walk). Measured three ways:
+      * eager: per substep, form the groups (a plan) and walk them, timed alone (latency);
+      * graph: the 16 substeps' walks captured once into a hipGraph and replayed;
+      * batched: all 16 active sets in one segmented plan, walked by one launch.
+(b) Spheres. 512 independent Plummer spheres (N = floor(2^U(10, 16)), a = 1, h = 0.01), built as
+    one forest (one tree per sphere) and walked by one graph-captured launch; force errors are
+    judged against each sphere's own FP64 direct sum on probe particles.
+"""
+from __future__ import annotations
+
+import json
+import time
+
+import numpy as np
+import torch
+
+from . import ics
+from . import tree as gz
+
+ALPHA_REF = 0.0025
+THETA_BOOT = 0.7
+
+
+def _sync(ctx):
+    ctx.synchronize()
+    torch.cuda.synchronize()
+
+
+def substeps(n=1 << 23, n_levels=5, precision="fp32", group_size=32, model=None, reps=3):
+    x, y, z, m, soft = ics.halo_disk(n=n)
+    P = gz.Particles(x, y, z, m, soft)
+    T = gz.build(P)
+    ctx = T.ctx
+    tg = T.device_orderbuf()
+    boot = gz.accel_grouped(T, P, tg, THETA_BOOT, 1.0, 0)
+    amag = torch.empty(P.n, dtype=torch.float64, device=P.device)
+    amag[tg.view(torch.int32).long()] = torch.linalg.norm(torch.stack(boot, 1), dim=1)
+    level = ics.timestep_bins(x, y, z, n_levels=n_levels)
+    nsub = 2 ** (n_levels - 1)
+    lists, aolds = [], []
+    for s in range(nsub):
+        t = gz.order_targets(T, ics.active_for_substep(level, s, n_levels))
+        lists.append(t)
+        aolds.append(ALPHA_REF * amag[t.view(torch.int32).long()])
+    counts = [int(t.shape[0]) for t in lists]
+    kw = dict(precision=precision, group_size=group_size, model=model)
+
+    # eager: groups + walk per substep, each timed alone
+    eager = np.zeros((reps, nsub))
+    plans = []
+    for r in range(reps):
+        plans = []
+        for s in range(nsub):
+            _sync(ctx)
+            t0 = time.perf_counter()
+            p = gz.WalkPlan(T, P, lists[s], **kw)
+            p.launch(0.0, 1.0, aolds[s])
+            _sync(ctx)
+            eager[r, s] = time.perf_counter() - t0
+            plans.append(p)
+    walk_only = np.zeros((reps, nsub))
+    for r in range(reps):
+        for s in range(nsub):
+            _sync(ctx)
+            t0 = time.perf_counter()
+            plans[s].launch(0.0, 1.0, aolds[s])
+            _sync(ctx)
+            walk_only[r, s] = time.perf_counter() - t0
+
+    # graph: the 16 walks captured once, replayed
+    g = gz.Graph(ctx)
+    g.begin()
+    for s in range(nsub):
+        plans[s].launch(0.0, 1.0, aolds[s])
+    g.end()
+    g.launch()
+    _sync(ctx)
+    t_graph = []
+    for _ in range(reps):
+        _sync(ctx)
+        t0 = time.perf_counter()
+        g.launch()
+        _sync(ctx)
+        t_graph.append(time.perf_counter() - t0)
+
+    # batched: all active sets in one segmented plan and one launch
+    seg = np.concatenate([[0], np.cumsum(counts)])
+    allt = torch.cat([t.view(torch.int32) for t in lists]).view(torch.uint32)
+    alla = torch.cat(aolds)
+    _sync(ctx)
+    t0 = time.perf_counter()
+    bp = gz.WalkPlan(T, P, allt, segments=seg, **kw)
+    _sync(ctx)
+    t_bplan = time.perf_counter() - t0
+    bp.launch(0.0, 1.0, alla)
+    _sync(ctx)
+    t_batched = []
+    for _ in range(reps):
+        _sync(ctx)
+        t0 = time.perf_counter()
+        bp.launch(0.0, 1.0, alla)
+        _sync(ctx)
+        t_batched.append(time.perf_counter() - t0)
+    # the batched results are the per-substep results
+    same = all(torch.equal(bp.ax[seg[s]:seg[s + 1]], plans[s].ax) for s in range(nsub))
+    total = int(seg[-1])
+    return {
+        "workload": f"halo+disk N={n}, {nsub} substeps of {n_levels} power-of-two bins",
+        "precision": precision, "group_size": group_size, "learned": model is not None,
+        "active_per_substep": counts,
+        "eager_ms_per_substep": (eager.min(0) * 1e3).round(3).tolist(),
+        "walk_only_ms_per_substep": (walk_only.min(0) * 1e3).round(3).tolist(),
+        "eager_evals_per_s": total / eager.min(0).sum(),
+        "graph_ms_total": min(t_graph) * 1e3,
+        "graph_evals_per_s": total / min(t_graph),
+        "batched_plan_ms": t_bplan * 1e3,
+        "batched_ms": min(t_batched) * 1e3,
+        "batched_evals_per_s": total / min(t_batched),
+        "batched_equals_per_substep": bool(same),
+    }
+
+
+def spheres(n_spheres=512, precision="fp32", group_size=32, theta=0.7, n_probe_spheres=16,
+            reps=3):
+    (x, y, z, m, soft), starts = ics.spheres(n_spheres=n_spheres)
+    P = gz.Particles(x, y, z, m, soft)
+    t0 = time.perf_counter()
+    F = gz.build(P, clusters=starts)
+    t_build = time.perf_counter() - t0
+    ctx = F.ctx
+    tg = F.device_orderbuf()
+    plan = gz.WalkPlan(F, P, tg, precision=precision, group_size=group_size)
+    g = gz.Graph(ctx)
+    g.begin()
+    plan.launch(theta, 1.0, None)
+    g.end()
+    g.launch()
+    _sync(ctx)
+    ts = []
+    for _ in range(reps):
+        _sync(ctx)
+        t0 = time.perf_counter()
+        g.launch()
+        _sync(ctx)
+        ts.append(time.perf_counter() - t0)
+    # judge: each probed sphere's own direct sum
+    order = tg.view(torch.int32).cpu().numpy()
+    pos = np.empty(P.n, dtype=np.int64)
+    pos[order] = np.arange(order.shape[0])
+    rng = np.random.default_rng(7)
+    errs = []
+    for c in rng.choice(n_spheres, n_probe_spheres, replace=False):
+        lo, hi = int(starts[c]), int(starts[c + 1])
+        Pc = gz.Particles(x[lo:hi], y[lo:hi], z[lo:hi], m[lo:hi], soft[lo:hi])
+        pr = np.sort(rng.choice(hi - lo, min(512, hi - lo), replace=False)).astype(np.uint32)
+        ex = np.stack([v.cpu().numpy() for v in gz.accel_brute(Pc, pr)], 1)
+        got = np.stack([v.cpu().numpy()[pos[pr + lo]] for v in (plan.ax, plan.ay, plan.az)], 1)
+        errs.append(np.linalg.norm(got - ex, axis=1) / np.linalg.norm(ex, axis=1))
+    e = np.concatenate(errs)
+    return {
+        "workload": f"{n_spheres} Plummer spheres, N={P.n} total, forest of one tree each",
+        "precision": precision, "group_size": group_size, "theta": theta,
+        "build_s": t_build, "graph_ms": min(ts) * 1e3, "evals_per_s": P.n / min(ts),
+        "err": {"median": float(np.median(e)), "p99": float(np.percentile(e, 99)),
+                "max": float(e.max())},
+        "status": plan.status(),
+    }
+
+
+def main():
+    import argparse
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--learned", action="store_true")
+    args = ap.parse_args()
+    model = None
+    if args.learned:
+        from . import learned
+        model = learned.load_model()
+    print(json.dumps(substeps(model=model)), flush=True)
+    print(json.dumps(spheres()), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -81,6 +81,9 @@ gz_status walk(gz_ctx* ctx, const gz_tree* tree, const gz_particles* p, const ui
         return fail(GZ_ERR_INVALID, "%s: unknown precision %d", who, precision);
     if (group_size < 0 || group_size > 4096)
         return fail(GZ_ERR_INVALID, "%s: group_size %d out of range [0, 4096]", who, group_size);
+    if (tree->n_clusters > 1 && batch > 1)
+        return fail(GZ_ERR_INVALID, "%s: fixed windows (batch > 1) would mix trees of a forest",
+                    who);
     if (counts) std::memset(counts, 0, sizeof *counts);
     if (ex.ngroups_out) *ex.ngroups_out = 0;
     if (n_t == 0) return GZ_OK;
@@ -205,6 +208,8 @@ gz_status walk(gz_ctx* ctx, const gz_tree* tree, const gz_particles* p, const ui
     }
     a.ngroups = (int)ngroups;
     a.root = tree->root;
+    a.cluster_start = tree->cluster_start_dev;
+    a.nclusters = tree->n_clusters;
     a.persistent = persistent;
     a.leaf_always_open = leaf_always_open;
     a.queue = queue;
@@ -617,6 +622,8 @@ gz_status gz_walk_plan_launch(gz_ctx* ctx, gz_walk_plan* P, double theta, double
     a.theta_sq = theta * theta;
     a.ngroups = P->ngroups;
     a.root = tree->root;
+    a.cluster_start = tree->cluster_start_dev;
+    a.nclusters = tree->n_clusters;
     a.persistent = P->persistent;
     a.leaf_always_open = 0;
     a.queue = reinterpret_cast<int*>(P->misc + 5);

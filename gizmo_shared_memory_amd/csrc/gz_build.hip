@@ -106,6 +106,93 @@ __global__ void k_root_cell(const double* __restrict__ partial, int nb, long lon
     for (int k = 0; k < 3; ++k) { root[4 + k] = lo[k]; root[7 + k] = hi[k]; }
 }
 
+// ---- independent clusters (a forest) -------------------------------------------------------------
+// Cluster c is particles [cstart[c], cstart[c+1]); each gets its own root cell from its own live
+// bounding box, exactly as the single-tree root (tree.cc:361-372), and its own tree.
+__device__ __forceinline__ int cluster_of(uint32_t i, const uint32_t* __restrict__ cstart, int nc) {
+    int lo = 0, hi = nc; // last c with cstart[c] <= i
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (cstart[mid] <= i) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+
+__global__ __launch_bounds__(kThreads) void k_bbox_clusters(const double* __restrict__ x,
+                                                            const double* __restrict__ y,
+                                                            const double* __restrict__ z,
+                                                            const double* __restrict__ m,
+                                                            const uint32_t* __restrict__ cstart,
+                                                            double* __restrict__ roots) {
+    const int c = blockIdx.x;
+    double lo0 = 1e300, lo1 = 1e300, lo2 = 1e300, hi0 = -1e300, hi1 = -1e300, hi2 = -1e300;
+    for (uint32_t i = cstart[c] + threadIdx.x; i < cstart[c + 1]; i += kThreads) {
+        if (!(m[i] > 0)) continue;
+        lo0 = dmin_(lo0, x[i]); hi0 = dmax_(hi0, x[i]);
+        lo1 = dmin_(lo1, y[i]); hi1 = dmax_(hi1, y[i]);
+        lo2 = dmin_(lo2, z[i]); hi2 = dmax_(hi2, z[i]);
+    }
+    lo0 = wave_min(lo0); lo1 = wave_min(lo1); lo2 = wave_min(lo2);
+    hi0 = wave_max(hi0); hi1 = wave_max(hi1); hi2 = wave_max(hi2);
+    __shared__ double s[4][6];
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    if (lane == 0) {
+        s[w][0] = lo0; s[w][1] = lo1; s[w][2] = lo2; s[w][3] = hi0; s[w][4] = hi1; s[w][5] = hi2;
+    }
+    __syncthreads();
+    if (threadIdx.x != 0) return;
+    double lo[3], hi[3];
+    for (int k = 0; k < 3; ++k) {
+        lo[k] = s[0][k];
+        hi[k] = s[0][3 + k];
+        for (int j = 1; j < 4; ++j) {
+            lo[k] = dmin_(lo[k], s[j][k]);
+            hi[k] = dmax_(hi[k], s[j][3 + k]);
+        }
+    }
+    double side = dmax_(hi[0] - lo[0], dmax_(hi[1] - lo[1], hi[2] - lo[2])) * 1.0000001;
+    if (side <= 0) side = 1.0;
+    double* r = roots + (size_t)c * 4;
+    r[0] = 0.5 * (lo[0] + hi[0]);
+    r[1] = 0.5 * (lo[1] + hi[1]);
+    r[2] = 0.5 * (lo[2] + hi[2]);
+    r[3] = side;
+}
+
+__global__ __launch_bounds__(kThreads) void k_cluster_keys(const uint32_t* __restrict__ v,
+                                                           size_t nl,
+                                                           const uint32_t* __restrict__ cstart,
+                                                           int nc, uint64_t* __restrict__ ck) {
+    const size_t r = (size_t)blockIdx.x * kThreads + threadIdx.x;
+    if (r < nl) ck[r] = (uint64_t)cluster_of(v[r], cstart, nc);
+}
+
+__global__ __launch_bounds__(kThreads) void k_cluster_counts(const uint32_t* __restrict__ v,
+                                                             size_t nl,
+                                                             const uint32_t* __restrict__ cstart,
+                                                             int nc, int* __restrict__ cnt) {
+    const size_t r = (size_t)blockIdx.x * kThreads + threadIdx.x;
+    if (r < nl) atomicAdd(&cnt[cluster_of(v[r], cstart, nc)], 1);
+}
+
+__global__ __launch_bounds__(kThreads) void k_init_roots(int nc, const double* __restrict__ roots,
+                                                         const int* __restrict__ rstart, int* nlo,
+                                                         int* nhi, double* gcx, double* gcy,
+                                                         double* gcz, double* size, int* parent,
+                                                         int* next) {
+    const int c = blockIdx.x * kThreads + threadIdx.x;
+    if (c >= nc) return;
+    nlo[c] = rstart[c];
+    nhi[c] = rstart[c + 1];
+    gcx[c] = roots[4 * c];
+    gcy[c] = roots[4 * c + 1];
+    gcz[c] = roots[4 * c + 2];
+    size[c] = roots[4 * c + 3];
+    parent[c] = -1;
+    next[c] = -1;
+}
+
 // ---- live filter + keys (tree.cc:355-359, 409-418) ----------------------------------------------
 __global__ __launch_bounds__(kThreads) void k_live_flags(const double* __restrict__ m, size_t n,
                                                          uint32_t* __restrict__ flag) {
@@ -134,10 +221,12 @@ __global__ __launch_bounds__(kThreads) void k_keys(const uint32_t* __restrict__ 
                                                    const double* __restrict__ x,
                                                    const double* __restrict__ y,
                                                    const double* __restrict__ z,
-                                                   const double* __restrict__ root,
+                                                   const double* __restrict__ roots,
+                                                   const uint32_t* __restrict__ cstart, int nc,
                                                    uint64_t* __restrict__ keys) {
     const size_t r = (size_t)blockIdx.x * kThreads + threadIdx.x;
     if (r >= nl) return;
+    const double* root = roots + (nc > 1 ? (size_t)cluster_of(order[r], cstart, nc) * 4 : 0);
     const double side = root[3];
     const double scale = (double)((1u << kMaxLevel) - 1) / side;
     const double bx = root[0] - 0.5 * side, by = root[1] - 0.5 * side, bz = root[2] - 0.5 * side;
@@ -368,7 +457,7 @@ static gz_status grow_nodes(gz_tree* t, int used, int cap, hipStream_t st) {
 void tree_release(gz_tree* t) {
     void* ptrs[] = {t->skey, t->orderbuf, t->rank, t->leaf_of, t->src, t->src_soft, t->wn,
                     t->next, t->parent, t->nlo, t->nhi, t->gcx, t->gcy, t->gcz, t->size,
-                    t->delta, t->soft, t->level_begin_dev};
+                    t->delta, t->soft, t->level_begin_dev, t->cluster_start_dev};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
 }
@@ -391,18 +480,47 @@ gz_status build_tree(gz_ctx* ctx, const gz_particles* p, const gz_build_opts* op
     const double fx = opts ? opts->shift[0] : 0.0, fy = opts ? opts->shift[1] : 0.0,
                  fz = opts ? opts->shift[2] : 0.0;
     const int max_nodes = opts ? opts->max_nodes : 0;
+    const int nc = (opts && opts->n_clusters > 1) ? opts->n_clusters : 1;
+    if (nc > 1) {
+        if (!opts->cluster_start)
+            return fail(GZ_ERR_INVALID, "gz_tree_build: n_clusters > 1 without cluster_start");
+        if (seed >= 0 || fx != 0.0 || fy != 0.0 || fz != 0.0)
+            return fail(GZ_ERR_UNSUPPORTED, "gz_tree_build: root randomisation with clusters");
+        if (opts->cluster_start[0] != 0 || opts->cluster_start[nc] != n)
+            return fail(GZ_ERR_INVALID, "gz_tree_build: clusters must cover [0, n)");
+        for (int c = 0; c < nc; ++c)
+            if (opts->cluster_start[c + 1] < opts->cluster_start[c])
+                return fail(GZ_ERR_INVALID, "gz_tree_build: cluster offsets must not decrease");
+    }
     t->n = n;
+    t->n_clusters = nc;
     hipEvent_t* ev = ctx->ev;
     GZ_HIP(hipEventRecord(ev[0], st));
 
-    // --- bbox + root cell
+    // --- bbox + root cell (one per cluster for a forest)
     DevMem& s0 = ctx->scratch[0];
     const unsigned nbb = (unsigned)std::max<size_t>(1, std::min<size_t>(grid_for(n), 1024));
-    GZ_TRY(s0.reserve(nbb * 6 * sizeof(double) + 16 * sizeof(double)));
+    GZ_TRY(s0.reserve(nbb * 6 * sizeof(double) + 16 * sizeof(double) +
+                      (nc > 1 ? (size_t)nc * (4 * sizeof(double) + 2 * sizeof(int)) + 512 : 0)));
     double* partial = s0.as<double>();
     double* droot = partial + (size_t)nbb * 6;
-    k_bbox_partial<<<nbb, kThreads, 0, st>>>(p->x, p->y, p->z, p->m, n, partial);
-    k_root_cell<<<1, 64, 0, st>>>(partial, (int)nbb, seed, fx, fy, fz, droot);
+    double* croots = droot + 16;                                     // nc x 4 (forest)
+    int* ccount = reinterpret_cast<int*>(croots + (size_t)4 * nc);   // nc + 1 (forest)
+    uint32_t* cstart_dev = nullptr;
+    if (nc > 1) {
+        GZ_TRY(dalloc(reinterpret_cast<void**>(&t->cluster_start_dev), (nc + 1) * sizeof(uint32_t)));
+        std::vector<uint32_t> hc(nc + 1);
+        for (int c = 0; c <= nc; ++c) hc[c] = (uint32_t)opts->cluster_start[c];
+        GZ_HIP(hipMemcpy(t->cluster_start_dev, hc.data(), hc.size() * sizeof(uint32_t),
+                         hipMemcpyHostToDevice));
+        cstart_dev = t->cluster_start_dev;
+        k_bbox_clusters<<<nc, kThreads, 0, st>>>(p->x, p->y, p->z, p->m, cstart_dev, croots);
+        // droot keeps the first cluster's cell for the tree's info
+        GZ_HIP(hipMemcpyAsync(droot, croots, 4 * sizeof(double), hipMemcpyDeviceToDevice, st));
+    } else {
+        k_bbox_partial<<<nbb, kThreads, 0, st>>>(p->x, p->y, p->z, p->m, n, partial);
+        k_root_cell<<<1, 64, 0, st>>>(partial, (int)nbb, seed, fx, fy, fz, droot);
+    }
     GZ_HIP(hipGetLastError());
     double hroot[10];
     GZ_HIP(hipMemcpyAsync(hroot, droot, sizeof hroot, hipMemcpyDeviceToHost, st));
@@ -437,20 +555,39 @@ gz_status build_tree(gz_ctx* ctx, const gz_particles* p, const gz_build_opts* op
     for (int k = 0; k < 4; ++k) t->root_box[k] = hroot[k];
     const size_t nl = n_live32;
     t->n_live = nl;
+    const double* key_roots = nc > 1 ? croots : droot;
     if (nl > 0) {
-        k_keys<<<grid_for(nl), kThreads, 0, st>>>(v0, nl, p->x, p->y, p->z, droot, k0);
+        k_keys<<<grid_for(nl), kThreads, 0, st>>>(v0, nl, p->x, p->y, p->z, key_roots, cstart_dev,
+                                                  nc, k0);
         GZ_HIP(hipGetLastError());
     }
     GZ_HIP(hipEventRecord(ev[2], st));
 
-    // --- sort
+    // --- sort (a forest: by key, then stably by cluster, so each cluster is one sorted block)
     GZ_TRY(radix_sort_pairs(k0, v0, k1, v1, nl, 3 * kMaxLevel, ctx->scratch[3], st));
     GZ_TRY(dalloc(reinterpret_cast<void**>(&t->skey), nl * sizeof(uint64_t)));
     GZ_TRY(dalloc(reinterpret_cast<void**>(&t->orderbuf), nl * sizeof(uint32_t)));
-    if (nl) {
+    std::vector<int> rstart_h(nc + 1, 0);
+    if (nc > 1 && nl > 0) {
+        k_cluster_keys<<<grid_for(nl), kThreads, 0, st>>>(v0, nl, cstart_dev, nc, k1);
+        GZ_HIP(hipGetLastError());
+        int bits = 1;
+        while ((1 << bits) < nc) ++bits;
+        GZ_TRY(radix_sort_pairs(k1, v0, k0, v1, nl, bits, ctx->scratch[3], st));
+        k_keys<<<grid_for(nl), kThreads, 0, st>>>(v0, nl, p->x, p->y, p->z, key_roots,
+                                                  cstart_dev, nc, t->skey);
+        GZ_TRY(fill_zero_u32(reinterpret_cast<uint32_t*>(ccount), nc, st));
+        k_cluster_counts<<<grid_for(nl), kThreads, 0, st>>>(v0, nl, cstart_dev, nc, ccount);
+        GZ_HIP(hipGetLastError());
+        std::vector<int> cnt(nc);
+        GZ_HIP(hipMemcpyAsync(cnt.data(), ccount, nc * sizeof(int), hipMemcpyDeviceToHost, st));
+        GZ_HIP(hipStreamSynchronize(st));
+        for (int c = 0; c < nc; ++c) rstart_h[c + 1] = rstart_h[c] + cnt[c];
+    } else if (nl) {
         GZ_HIP(hipMemcpyAsync(t->skey, k0, nl * sizeof(uint64_t), hipMemcpyDeviceToDevice, st));
-        GZ_HIP(hipMemcpyAsync(t->orderbuf, v0, nl * sizeof(uint32_t), hipMemcpyDeviceToDevice, st));
     }
+    if (nl)
+        GZ_HIP(hipMemcpyAsync(t->orderbuf, v0, nl * sizeof(uint32_t), hipMemcpyDeviceToDevice, st));
     GZ_HIP(hipEventRecord(ev[3], st));
 
     // --- sorted source copy, rank, leaf_of reset
@@ -479,11 +616,26 @@ gz_status build_tree(gz_ctx* ctx, const gz_particles* p, const gz_build_opts* op
     {
         // root: range [0, nl), the root cell, no parent, no successor. An empty particle set
         // still gets a root (a leaf of mass 0), as the reference's build_node does for [0, 0).
-        k_init_root<<<1, 1, 0, st>>>(droot, (int)nl, t->nlo, t->nhi, t->gcx, t->gcy, t->gcz,
-                                     t->size, t->parent, t->next);
+        if (nc > 1) {
+            // one root per cluster at level 0, each over its block of the sorted order
+            int* rstart_d = ccount; // reuse: nc + 1 sorted offsets
+            GZ_HIP(hipMemcpyAsync(rstart_d, rstart_h.data(), (nc + 1) * sizeof(int),
+                                  hipMemcpyHostToDevice, st));
+            if (nc > cap) {
+                GZ_TRY(grow_nodes(t, 0, nc + 1024, st));
+                cap = nc + 1024;
+            }
+            k_init_roots<<<grid_for(nc), kThreads, 0, st>>>(nc, croots, rstart_d, t->nlo, t->nhi,
+                                                            t->gcx, t->gcy, t->gcz, t->size,
+                                                            t->parent, t->next);
+            GZ_HIP(hipStreamSynchronize(st)); // rstart_h is a host vector read by the copy
+        } else {
+            k_init_root<<<1, 1, 0, st>>>(droot, (int)nl, t->nlo, t->nhi, t->gcx, t->gcy, t->gcz,
+                                         t->size, t->parent, t->next);
+        }
         GZ_HIP(hipGetLastError());
-        nnodes = 1;
-        t->level_begin.push_back(1);
+        nnodes = nc;
+        t->level_begin.push_back(nc);
         DevMem& s4 = ctx->scratch[4];
         for (int level = 0;; ++level) {
             const int lb = t->level_begin[level], le = t->level_begin[level + 1];

@@ -25,6 +25,19 @@ struct RootInfo {
     double cx, cy, cz, side, mass;
 };
 
+// The tree a target belongs to: in a forest, cluster c's root is node c.
+__device__ __forceinline__ int root_of(uint32_t particle, const uint32_t* __restrict__ cstart,
+                                       int nc, int root) {
+    if (nc <= 1 || !cstart) return root;
+    int lo = 0, hi = nc;
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (cstart[mid] <= particle) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+
 __device__ __forceinline__ float flog10(double v) { return (float)log10(fmax(v, 1e-300)); }
 
 // All features are ratios of the group's own scales (extent, leaf cell, softening, field, mass)

@@ -68,6 +68,8 @@ struct WalkArgs {
     int* n_fallback;
     int ngroups;
     int root;
+    const uint32_t* cluster_start; // forest: particle offsets of the clusters; NULL: one tree
+    int nclusters;
     int persistent;
     int leaf_always_open;   // 1: a reached leaf is always summed pair by pair (shmem::accel)
     int* queue;             // persistent work counter (zeroed before the launch)
@@ -214,6 +216,8 @@ struct gz_tree {
     double root_com[3] = {0, 0, 0}, root_mass = 0; // the root node's moments
     std::vector<int> level_begin; // node ids of level L are [level_begin[L], level_begin[L+1])
     int* level_begin_dev = nullptr; // the same table on the device (kMaxLevel + 2 entries)
+    int n_clusters = 1;             // independent trees (a forest); their roots are nodes 0..nc-1
+    uint32_t* cluster_start_dev = nullptr; // forest: particle offsets of the clusters (nc + 1)
     gz_build_times times = {};
 
     // particles, Morton order

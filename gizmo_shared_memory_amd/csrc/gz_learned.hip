@@ -43,7 +43,8 @@ __global__ __launch_bounds__(256) void k_group_features(
     const double* __restrict__ y, const double* __restrict__ z, const double* __restrict__ m,
     const double* __restrict__ soft, const double* __restrict__ amag,
     const int* __restrict__ leaf_of, const int* __restrict__ level_begin,
-    const WNode* __restrict__ wn, int root, float* __restrict__ feats) {
+    const WNode* __restrict__ wn, int root, const uint32_t* __restrict__ cstart, int nc,
+    float* __restrict__ feats) {
     // one thread per group: groups are small, and many independent groups in flight hide the
     // latency of the target gathers better than one wavefront per group
     const int g = blockIdx.x * 256 + threadIdx.x;
@@ -75,7 +76,7 @@ __global__ __launch_bounds__(256) void k_group_features(
     S.amax = amax;
     S.a2 = a2;
     S.nt = nt;
-    const WNode Rn = wn[root];
+    const WNode Rn = wn[root_of(targets[G.tlo], cstart, nc, root)];
     const RootInfo R{Rn.cx, Rn.cy, Rn.cz, (double)Rn.len, Rn.mass};
     const int leaf = leaf_of[targets[G.tlo]];
     S.leaf_len = (leaf >= 0) ? (double)wn[leaf].len : fmax(R.side, 1e-300);
@@ -296,7 +297,7 @@ gz_status launch_group_features(int ngroups, int nt, const int* gstart, const ui
     if (ngroups <= 0) return GZ_OK;
     k_group_features<<<(unsigned)((ngroups + 255) / 256), 256, 0, st>>>(
         ngroups, nt, gstart, targets, p->x, p->y, p->z, p->m, p->soft, amag, t->leaf_of,
-        t->level_begin_dev, t->wn, t->root, feats);
+        t->level_begin_dev, t->wn, t->root, t->cluster_start_dev, t->n_clusters, feats);
     GZ_HIP(hipGetLastError());
     return GZ_OK;
 }

@@ -255,7 +255,11 @@ __device__ __noinline__ GroupInfo group_bounds(const WalkArgs& a, int g, int lan
         S.depth = 0;
         if (leaf >= 0)
             while (S.depth < kMaxLevel && a.level_begin[S.depth + 1] <= leaf) ++S.depth;
-        const RootInfo R{a.root_cx, a.root_cy, a.root_cz, a.root_side, a.root_mass};
+        RootInfo R{a.root_cx, a.root_cy, a.root_cz, a.root_side, a.root_mass};
+        if (a.nclusters > 1) { // the group's own tree
+            const WNode Rn = a.wn[root_of(a.targets[lo], a.cluster_start, a.nclusters, a.root)];
+            R = RootInfo{Rn.cx, Rn.cy, Rn.cz, (double)Rn.len, Rn.mass};
+        }
         float f[16];
         group_features16(S, R, f);
         const float pred = mlp_wave(f, a.mlp_w, a.mlp_theta, a.mlp_xnorm, sh, lane);
@@ -474,7 +478,9 @@ __global__ __launch_bounds__(64, 4) void k_walk(WalkArgs a) {
             // flight while this batch's leaves are copied and the list is evaluated.
             int sp = 0;
             int k = 1;
-            int node = (lane == 0) ? a.root : -1;
+            int node = (lane == 0)
+                           ? root_of(a.targets[gi.tlo], a.cluster_start, a.nclusters, a.root)
+                           : -1;
             WNode w;
             if (node >= 0) w = a.wn[node];
             while (k > 0 && !overflow) {

@@ -193,10 +193,16 @@ class Tree:
 
 
 def build(P: Particles, randomize_seed: int = -1, shift=(0.0, 0.0, 0.0), max_nodes: int = 0,
-          ctx: Context | None = None) -> Tree:
-    """shmem::build (tree.h:384-385, tree.cc:350-526)."""
+          clusters=None, ctx: Context | None = None) -> Tree:
+    """shmem::build (tree.h:384-385, tree.cc:350-526). clusters: particle offsets (length
+    n_clusters + 1) of independent clusters, each given its own tree (a forest)."""
     ctx = ctx or Context.get(P.device.index or 0)
-    opts = L.gz_build_opts(int(randomize_seed), (C.c_double * 3)(*shift), int(max_nodes))
+    cs, nc = None, 0
+    if clusters is not None:
+        cl = [int(v) for v in clusters]
+        cs = (C.c_size_t * len(cl))(*cl)
+        nc = len(cl) - 1
+    opts = L.gz_build_opts(int(randomize_seed), (C.c_double * 3)(*shift), int(max_nodes), cs, nc)
     h = C.c_void_p()
     ctx.enter()
     L.check(L.load().gz_tree_build(ctx.handle, P.cstruct(), C.byref(opts), C.byref(h)),

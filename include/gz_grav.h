@@ -69,6 +69,13 @@ typedef struct {
     double shift[3];          /* SHMEM_TREE_SHIFT fractions (tree.cc:400-407); 0 = off         */
     int max_nodes;            /* 0: grow the node arena as needed (the reference's retry,
                                  tree.cc:438-462); > 0: fail with GZ_ERR_ARENA beyond this     */
+    /* A forest of independent trees, one per cluster (no counterpart in the reference, which
+     * builds one tree): cluster c is particles [cluster_start[c], cluster_start[c+1]) (host
+     * array of n_clusters + 1 offsets). Each cluster gets its own root cell (the reference's
+     * rule applied to its own particles), nodes 0..n_clusters-1 are the cluster roots, and a
+     * walk of a target sees only its own cluster. n_clusters <= 1: one tree. */
+    const size_t* cluster_start;
+    int n_clusters;
 } gz_build_opts;
 
 /* Stage timings in ms, the reference's BuildTimes (tree.h:371) measured with HIP events. */

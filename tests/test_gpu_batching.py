@@ -106,3 +106,32 @@ def test_plan_with_learned_model_matches_walk(gz, setup):
     assert st["fallback_groups"] == ref[3]["fallback_groups"]
     for u, v in zip((plan.ax, plan.ay, plan.az), ref[:3]):
         assert torch.equal(u, v)
+
+
+@pytest.mark.parametrize("prec,gs", [("fp64", 0), ("fp32", 32)])
+def test_forest_equals_separate_trees(gz, prec, gs):
+    """A forest built over several independent clusters walks each cluster exactly as a tree
+    built over that cluster alone."""
+    (x, y, z, m, soft), starts = ics.spheres(n_spheres=5, lo_log2=9.0, hi_log2=12.0, eps=0.01,
+                                             spacing=100.0, seed=4)
+    P = gz.Particles(x, y, z, m, soft)
+    F = gz.build(P, clusters=starts)
+    assert F.info.nnodes >= 5
+    tg = F.device_orderbuf()
+    fa = gz.accel_grouped(F, P, tg, 0.6, 1.0, 0, precision=prec, group_size=gs, counts=True)
+    order = tg.view(torch.int32).cpu().numpy()
+    pos = np.empty(P.n, dtype=np.int64)
+    pos[order] = np.arange(order.shape[0])
+    tot = 0
+    for c in range(5):
+        lo, hi = int(starts[c]), int(starts[c + 1])
+        Pc = gz.Particles(x[lo:hi], y[lo:hi], z[lo:hi], m[lo:hi], soft[lo:hi])
+        Tc = gz.build(Pc)
+        tc = Tc.device_orderbuf()
+        ca = gz.accel_grouped(Tc, Pc, tc, 0.6, 1.0, 0, precision=prec, group_size=gs,
+                              counts=True)
+        oc = tc.view(torch.int32).cpu().numpy() + lo
+        for u, v in zip(fa[:3], ca[:3]):
+            assert np.array_equal(u.cpu().numpy()[pos[oc]], v.cpu().numpy()), c
+        tot += ca[3]["multipole"] + ca[3]["pairs"]
+    assert fa[3]["multipole"] + fa[3]["pairs"] == tot
