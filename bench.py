@@ -138,6 +138,9 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--group-sizes", default="0,16,32,64")
+    ap.add_argument("--variants", default="fp64,fp32,learned,global",
+                    help="walks to measure (the headline is the learned walk when measured, else "
+                         "the FP32 walk); a profiler run can keep to one walk kernel")
     ap.add_argument("--model", default=os.path.join(ROOT, "gizmo_shared_memory_amd",
                                                      "learned_criterion.json"))
     ap.add_argument("--copy-mb", type=int, default=0,
@@ -203,24 +206,36 @@ def main():
         return r
 
     results = {}
+    variants = set(args.variants.split(","))
     # plain GPU walks with the reference criterion; group size tuned on the FP32 walk
-    results["fp64"] = measure("fp64", "fp64", 0, args.steps, args.warmup)
-    best = None
-    for gs in [int(v) for v in args.group_sizes.split(",")]:
-        (_, dt, kms) = time_walk(gz, T, P, targets, aold, 1, 1, precision="fp32", group_size=gs)
-        log(f"tune fp32 group_size={gs}: {dt * 1e3:.2f} ms/step, kernel {kms:.2f} ms")
-        if best is None or dt < best[1]:
-            best = (gs, dt)
+    if "fp64" in variants:
+        results["fp64"] = measure("fp64", "fp64", 0, args.steps, args.warmup)
+    sizes = [int(v) for v in args.group_sizes.split(",")]
+    best = (sizes[0], 0.0)
+    if len(sizes) > 1:
+        best = None
+        for gs in sizes:
+            (_, dt, kms) = time_walk(gz, T, P, targets, aold, 1, 1, precision="fp32",
+                                     group_size=gs)
+            log(f"tune fp32 group_size={gs}: {dt * 1e3:.2f} ms/step, kernel {kms:.2f} ms")
+            if best is None or dt < best[1]:
+                best = (gs, dt)
     gsz = best[0]
-    results["fp32"] = measure("fp32", "fp32", gsz, args.steps, args.warmup)
+    if "fp32" in variants:
+        results["fp32"] = measure("fp32", "fp32", gsz, args.steps, args.warmup)
 
     # learned criterion (FP32 walk)
     model = None
-    if os.path.exists(args.model):
+    if "learned" not in variants:
+        pass
+    elif os.path.exists(args.model):
         from gizmo_shared_memory_amd import learned
         model = learned.load_model(args.model)
         results["learned_fp32"] = measure("learned_fp32", "fp32", gsz, args.steps, args.warmup,
                                           model=model)
+    else:
+        log(f"no learned model at {args.model}")
+    if "global" in variants and "learned_fp32" in results:
         # the method without the network: one global tolerance for every group, the largest
         # on a grid whose p99 error stays within the learned walk's
         goal = max(results["learned_fp32"]["err"]["p99"], 1e-12)
@@ -233,8 +248,6 @@ def main():
             else:
                 break
         results["global_tolerance_fp32"] = glob
-    else:
-        log(f"no learned model at {args.model}")
 
     if args.copy_mb > 0:
         src = torch.ones(args.copy_mb << 18, dtype=torch.float32, device="cuda:0")
@@ -245,7 +258,7 @@ def main():
         log(f"copy calibration: {args.copy_mb} MiB read + {args.copy_mb} MiB written")
         del src, dst
 
-    head = results.get("learned_fp32") or results["fp32"]
+    head = results.get("learned_fp32") or results.get("fp32") or results["fp64"]
     learned_ok = "learned_fp32" in results and head["err"]["p99"] <= 1e-2
     ref_e = rel_err(tuple(v.cpu().numpy()[probes] for v in boot[:3]), ex)
     cpu = None
@@ -296,8 +309,9 @@ def main():
     }
     if cpu is not None:
         line["gpu_over_cpu"] = head["evals_per_s"] / cpu["value"]
-    line["speedup_vs_plain_fp32"] = head["evals_per_s"] / results["fp32"]["evals_per_s"]
-    line["speedup_vs_plain_fp64"] = head["evals_per_s"] / results["fp64"]["evals_per_s"]
+    for k in ("fp32", "fp64"):
+        if k in results:
+            line[f"speedup_vs_plain_{k}"] = head["evals_per_s"] / results[k]["evals_per_s"]
     if results.get("global_tolerance_fp32"):
         line["speedup_vs_global_tolerance"] = (head["evals_per_s"] /
                                                results["global_tolerance_fp32"]["evals_per_s"])

@@ -9,6 +9,7 @@
 #include <new>
 #include <vector>
 
+#include "gz_features.h"
 #include "gz_internal.h"
 
 using gz::fail;
@@ -52,6 +53,7 @@ struct WalkExtra {
     gz_model* model = nullptr;
     float* feats_out = nullptr; // features-only: n_groups x 16, device
     int* gstart_out = nullptr;  // features-only / groups-only: n_groups + 1 group starts, device
+    int* tslot_out = nullptr;   // groups-only: the targets' sorted slots (n_t), device
     size_t* ngroups_out = nullptr;
     bool groups_only = false;   // stop after the grouping (walk plans)
     const uint32_t* seg = nullptr; // device: nseg + 1 slot offsets of independent target lists
@@ -97,6 +99,7 @@ gz_status walk(gz_ctx* ctx, const gz_tree* tree, const gz_particles* p, const ui
                  o_retry = c.add<int>(n_t), o_pk = c.add<long long>(n_t),
                  o_gseg = c.add<int>(ex.nseg > 1 ? n_t : 1), o_mflag = c.add<uint32_t>(n_t),
                  o_mid = c.add<uint32_t>(n_t), o_gstart2 = c.add<int>(n_t + 1),
+                 o_scale = c.add<float>(ex.model ? n_t : 1), o_tslot = c.add<int>(n_t),
                  o_misc = c.add<long long>(16),
                  o_tmp = c.add<uint8_t>(gz::scan_tmp_bytes(n_t));
     gz::DevMem& s = ctx->scratch[5];
@@ -116,7 +119,9 @@ gz_status walk(gz_ctx* ctx, const gz_tree* tree, const gz_particles* p, const ui
     GZ_HIP(hipEventRecord(ev[8], st));
     GZ_TRY(gz::fill_zero_u32(reinterpret_cast<uint32_t*>(misc), 32, st));
     int* gseg = ex.nseg > 1 ? reinterpret_cast<int*>(b + o_gseg) : nullptr;
-    GZ_TRY(gz::launch_group_prep(targets, n_t, tree->leaf_of, tree->parent, tree->wn, batch,
+    int* tslot = reinterpret_cast<int*>(b + o_tslot);
+    GZ_TRY(gz::launch_group_prep(targets, n_t, tree->rank, tree->slot_leaf, tslot, tree->parent,
+                                 tree->wn, batch,
                                  group_size, p->n, ex.seg, ex.nseg, gseg, gkey, flag, gid,
                                  ngroups_dev, status, b + o_tmp, st));
     uint32_t ngroups0 = 0;
@@ -149,6 +154,8 @@ gz_status walk(gz_ctx* ctx, const gz_tree* tree, const gz_particles* p, const ui
     if (groups_only) {
         GZ_HIP(hipMemcpyAsync(ex.gstart_out, gstart, ngroups * sizeof(int),
                               hipMemcpyDeviceToDevice, st));
+        GZ_HIP(hipMemcpyAsync(ex.tslot_out, tslot, n_t * sizeof(int), hipMemcpyDeviceToDevice,
+                              st));
         const int end = (int)n_t;
         GZ_HIP(hipMemcpyAsync(ex.gstart_out + ngroups, &end, sizeof(int), hipMemcpyHostToDevice,
                               st));
@@ -158,8 +165,10 @@ gz_status walk(gz_ctx* ctx, const gz_tree* tree, const gz_particles* p, const ui
     }
     if (features_only) {
         // the groups' features for training (the walk computes the same ones for itself)
-        GZ_TRY(gz::launch_group_features((int)ngroups, (int)n_t, gstart, targets, p, aold, tree,
-                                         ex.feats_out, st));
+        gz::DevMem& fm = ctx->scratch[6];
+        GZ_TRY(fm.reserve((size_t)ngroups * sizeof(gz::GroupStats)));
+        GZ_TRY(gz::launch_group_features((int)ngroups, (int)n_t, gstart, targets, tslot, p, aold,
+                                         tree, fm.as<gz::GroupStats>(), ex.feats_out, st));
         GZ_HIP(hipMemcpyAsync(ex.gstart_out, gstart, ngroups * sizeof(int),
                               hipMemcpyDeviceToDevice, st));
         const int end = (int)n_t;
@@ -169,8 +178,16 @@ gz_status walk(gz_ctx* ctx, const gz_tree* tree, const gz_particles* p, const ui
         *ex.ngroups_out = ngroups;
         return GZ_OK;
     }
+    float* scale = nullptr;
     if (ex.model) {
+        // the learned tolerance of every group (features + network), before the walk
         nfb = reinterpret_cast<int*>(misc + 8);
+        scale = reinterpret_cast<float*>(b + o_scale);
+        gz::DevMem& fm = ctx->scratch[6];
+        GZ_TRY(fm.reserve((size_t)ngroups * sizeof(gz::GroupStats)));
+        GZ_TRY(gz::launch_group_tolerance((int)ngroups, (int)n_t, gstart, targets, tslot, p, aold,
+                                          tree, ex.model, fm.as<gz::GroupStats>(), scale, nfb,
+                                          st));
     }
     GZ_HIP(hipEventRecord(ev[9], st));
 
@@ -179,33 +196,14 @@ gz_status walk(gz_ctx* ctx, const gz_tree* tree, const gz_particles* p, const ui
     a.src = tree->src;
     a.src_soft = tree->src_soft;
     a.targets = targets;
+    a.tslot = tslot;
     a.x = p->x; a.y = p->y; a.z = p->z; a.soft = p->soft;
     a.groups = nullptr; // bounds are computed by each group's wavefront
     a.gstart = gstart;
     a.nt = (int)n_t;
     a.aold = aold;
-    a.group_scale = nullptr;
+    a.group_scale = scale;
     a.theta_sq = theta * theta;
-    if (ex.model) {
-        const gz_model* M = ex.model;
-        a.m = p->m;
-        a.leaf_of = tree->leaf_of;
-        a.level_begin = tree->level_begin_dev;
-        a.root_cx = tree->root_com[0];
-        a.root_cy = tree->root_com[1];
-        a.root_cz = tree->root_com[2];
-        a.root_side = tree->root_box[3];
-        a.root_mass = tree->root_mass;
-        a.mlp_w = M->wbf;
-        a.mlp_theta = M->theta;
-        a.mlp_xnorm = M->xnorm;
-        a.mlp_frange = M->frange;
-        a.mlp_lab_lo = M->lab_lo;
-        a.mlp_lab_hi = M->lab_hi;
-        a.mlp_margin = M->margin;
-        a.mlp_log_alpha_ref = M->log_alpha_ref;
-        a.n_fallback = nfb;
-    }
     a.ngroups = (int)ngroups;
     a.root = tree->root;
     a.cluster_start = tree->cluster_start_dev;
@@ -220,6 +218,7 @@ gz_status walk(gz_ctx* ctx, const gz_tree* tree, const gz_particles* p, const ui
     a.gstack = nullptr;
     a.gstack_cap = 0;
     a.stack_limit = stack_limit;
+    if (persistent) GZ_TRY(gz::ctx_spill(ctx, &a.spill, &a.spill_cap));
     a.G = G;
     a.ax = ax; a.ay = ay; a.az = az;
     const bool fp64 = precision == GZ_PREC_FP64, cnt = count && counts != nullptr;
@@ -319,6 +318,7 @@ gz_status gz_ctx_destroy(gz_ctx* ctx) {
     (void)hipSetDevice(ctx->device);
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     for (auto& m : ctx->scratch) m.release();
+    if (ctx->spill) (void)hipFree(ctx->spill);
     for (auto& e : ctx->ev)
         if (e) (void)hipEventDestroy(e);
     if (ctx->own_stream && ctx->stream) (void)hipStreamDestroy(ctx->stream);
@@ -555,6 +555,7 @@ gz_status gz_walk_plan_create(gz_ctx* ctx, const gz_tree* tree, const gz_particl
     Carver c;
     const size_t o_gs = c.add<int>(n_t + 1), o_misc = c.add<long long>(16),
                  o_rl = c.add<int>(n_t + 1), o_seg = c.add<uint32_t>(n_seg + 2),
+                 o_sc = c.add<float>(model ? n_t : 1), o_ts = c.add<int>(n_t),
                  o_stk = c.add<int>((size_t)P->retry_waves * P->gstack_cap);
     hipError_t e = hipMalloc(&P->mem, c.used);
     if (e != hipSuccess) {
@@ -566,6 +567,8 @@ gz_status gz_walk_plan_create(gz_ctx* ctx, const gz_tree* tree, const gz_particl
     P->misc = reinterpret_cast<long long*>(b + o_misc);
     P->retry_list = reinterpret_cast<int*>(b + o_rl);
     P->gstack = reinterpret_cast<int*>(b + o_stk);
+    P->scale = reinterpret_cast<float*>(b + o_sc);
+    P->tslot = reinterpret_cast<int*>(b + o_ts);
     uint32_t* seg = reinterpret_cast<uint32_t*>(b + o_seg);
     gz_status s = GZ_OK;
     if (n_seg > 1) {
@@ -575,10 +578,17 @@ gz_status gz_walk_plan_create(gz_ctx* ctx, const gz_tree* tree, const gz_particl
             hipSuccess)
             s = fail(GZ_ERR_HIP, "gz_walk_plan_create: upload failed");
     }
+    // the spill slices are allocated here, never during a launch (which may be captured)
+    if (s == GZ_OK && persistent) {
+        int* sp = nullptr;
+        int cap = 0;
+        s = gz::ctx_spill(ctx, &sp, &cap);
+    }
     if (s == GZ_OK && n_t > 0) {
         WalkExtra ex;
         ex.groups_only = true;
         ex.gstart_out = P->gstart;
+        ex.tslot_out = P->tslot;
         size_t ng = 0;
         ex.ngroups_out = &ng;
         ex.seg = seg;
@@ -587,6 +597,12 @@ gz_status gz_walk_plan_create(gz_ctx* ctx, const gz_tree* tree, const gz_particl
                  persistent, 0, 0, 0, 1, nullptr, nullptr, nullptr, nullptr,
                  "gz_walk_plan_create", ex);
         P->ngroups = (int)ng;
+    }
+    if (s == GZ_OK && model && P->ngroups > 0) {
+        void* f = nullptr;
+        if (hipMalloc(&f, (size_t)P->ngroups * sizeof(gz::GroupStats)) != hipSuccess)
+            s = fail(GZ_ERR_NOMEM, "gz_walk_plan_create: no device memory for group features");
+        P->stats = static_cast<gz::GroupStats*>(f);
     }
     if (s != GZ_OK) {
         gz_walk_plan_free(P);
@@ -613,6 +629,7 @@ gz_status gz_walk_plan_launch(gz_ctx* ctx, gz_walk_plan* P, double theta, double
     a.src = tree->src;
     a.src_soft = tree->src_soft;
     a.targets = P->targets;
+    a.tslot = P->tslot;
     a.x = P->p.x; a.y = P->p.y; a.z = P->p.z; a.soft = P->p.soft;
     a.groups = nullptr;
     a.gstart = P->gstart;
@@ -631,27 +648,14 @@ gz_status gz_walk_plan_launch(gz_ctx* ctx, gz_walk_plan* P, double theta, double
     a.counts = P->misc;
     a.n_retry = reinterpret_cast<int*>(P->misc + 7);
     a.retry_list = P->retry_list;
+    if (P->persistent) GZ_TRY(gz::ctx_spill(ctx, &a.spill, &a.spill_cap));
     a.G = G;
     a.ax = ax; a.ay = ay; a.az = az;
     if (P->model) {
-        const gz_model* M = P->model;
-        a.m = P->p.m;
-        a.leaf_of = tree->leaf_of;
-        a.level_begin = tree->level_begin_dev;
-        a.root_cx = tree->root_com[0];
-        a.root_cy = tree->root_com[1];
-        a.root_cz = tree->root_com[2];
-        a.root_side = tree->root_box[3];
-        a.root_mass = tree->root_mass;
-        a.mlp_w = M->wbf;
-        a.mlp_theta = M->theta;
-        a.mlp_xnorm = M->xnorm;
-        a.mlp_frange = M->frange;
-        a.mlp_lab_lo = M->lab_lo;
-        a.mlp_lab_hi = M->lab_hi;
-        a.mlp_margin = M->margin;
-        a.mlp_log_alpha_ref = M->log_alpha_ref;
-        a.n_fallback = reinterpret_cast<int*>(P->misc + 8);
+        a.group_scale = P->scale;
+        GZ_TRY(gz::launch_group_tolerance(P->ngroups, (int)P->n_t, P->gstart, P->targets, P->tslot,
+                                          &P->p, aold, tree, P->model, P->stats, P->scale,
+                                          reinterpret_cast<int*>(P->misc + 8), st));
     }
     const bool fp64 = P->precision == GZ_PREC_FP64;
     GZ_TRY(gz::launch_walk(a, fp64, false, st));
@@ -683,10 +687,9 @@ gz_status gz_walk_plan_status(gz_ctx* ctx, gz_walk_plan* P, gz_walk_counts* coun
 
 gz_status gz_walk_plan_free(gz_walk_plan* P) {
     if (!P) return GZ_OK;
-    if (P->mem) {
-        (void)hipStreamSynchronize(P->ctx->stream);
-        (void)hipFree(P->mem);
-    }
+    if (P->mem || P->stats) (void)hipStreamSynchronize(P->ctx->stream);
+    if (P->mem) (void)hipFree(P->mem);
+    if (P->stats) (void)hipFree(P->stats);
     delete P;
     return GZ_OK;
 }

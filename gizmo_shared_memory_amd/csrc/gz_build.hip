@@ -360,7 +360,7 @@ __global__ __launch_bounds__(kThreads) void k_moments(
     const double* __restrict__ gcz, const double* __restrict__ size,
     const double4* __restrict__ src, const double* __restrict__ src_soft,
     const uint32_t* __restrict__ orderbuf, int* __restrict__ leaf_of,
-    double* __restrict__ delta_out, double* __restrict__ soft_out, int* __restrict__ nleaf) {
+    int* __restrict__ slot_leaf, double* __restrict__ delta_out, double* __restrict__ soft_out, int* __restrict__ nleaf) {
     const int i = blockIdx.x * kThreads + threadIdx.x;
     const bool live = i < count;
     const int node = lb + (live ? i : 0);
@@ -382,6 +382,7 @@ __global__ __launch_bounds__(kThreads) void k_moments(
             const double s = src_soft[k];
             if (s > smax) smax = s;
             leaf_of[orderbuf[k]] = node;
+            slot_leaf[k] = node;
         }
     } else {
         for (int c = first; c < first + nk; ++c) {
@@ -455,8 +456,8 @@ static gz_status grow_nodes(gz_tree* t, int used, int cap, hipStream_t st) {
 }
 
 void tree_release(gz_tree* t) {
-    void* ptrs[] = {t->skey, t->orderbuf, t->rank, t->leaf_of, t->src, t->src_soft, t->wn,
-                    t->next, t->parent, t->nlo, t->nhi, t->gcx, t->gcy, t->gcz, t->size,
+    void* ptrs[] = {t->skey, t->orderbuf, t->rank, t->leaf_of, t->slot_leaf, t->src, t->src_soft,
+                    t->wn, t->next, t->parent, t->nlo, t->nhi, t->gcx, t->gcy, t->gcz, t->size,
                     t->delta, t->soft, t->level_begin_dev, t->cluster_start_dev};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
@@ -529,6 +530,7 @@ gz_status build_tree(gz_ctx* ctx, const gz_particles* p, const gz_build_opts* op
     // --- live filter + keys
     GZ_TRY(dalloc(reinterpret_cast<void**>(&t->rank), n * sizeof(uint32_t)));
     GZ_TRY(dalloc(reinterpret_cast<void**>(&t->leaf_of), n * sizeof(int)));
+    GZ_TRY(dalloc(reinterpret_cast<void**>(&t->slot_leaf), n * sizeof(int)));
     DevMem& s1 = ctx->scratch[1];
     const size_t nalloc = std::max<size_t>(n, 1);
     const size_t flag_bytes = (nalloc + 64) * sizeof(uint32_t);
@@ -697,6 +699,7 @@ gz_status build_tree(gz_ctx* ctx, const gz_particles* p, const gz_build_opts* op
         k_moments<<<grid_for(le - lb), kThreads, 0, st>>>(lb, le - lb, t->wn, t->nlo, t->nhi,
                                                           t->gcx, t->gcy, t->gcz, t->size, t->src,
                                                           t->src_soft, t->orderbuf, t->leaf_of,
+                                                          t->slot_leaf,
                                                           t->delta, t->soft, dleaf);
         GZ_HIP(hipGetLastError());
     }

@@ -1,7 +1,6 @@
-// Device functions shared by the feature kernel (training data) and the walk kernel (inference):
-// the 16 features of a target group, and one wavefront's evaluation of the opening-parameter
-// network for its group. Keeping one copy of each is what makes the features a group sees in
-// training the features it sees in the walk.
+// Device functions shared by the feature kernel (training data), the tolerance kernel (inference)
+// and the walk: the 16 features of a target group, and the tree a target belongs to. One copy of
+// the features is what makes the features a group sees in training the ones it sees in the walk.
 #pragma once
 
 #include "gz_internal.h"
@@ -17,9 +16,29 @@ struct GroupStats {
     double amin, amax; // min / max of |a_prev| (aold units)
     double a2;         // sum of squares of |a_prev|
     int nt;            // targets
-    double leaf_len;   // side of the first target's leaf
+    double leaf_len;   // side of the first target's leaf (< 0: the target is not in the tree)
     int depth;         // level of that leaf
+    int root;          // the root node of the group's tree
 };
+
+// Minimum and maximum over the wavefront; exact, so every lane gets the same value whatever the
+// order of the exchange.
+__device__ __forceinline__ double wave_min_d(double v) {
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) {
+        const double o = __shfl_xor(v, d);
+        v = (o < v) ? o : v;
+    }
+    return v;
+}
+__device__ __forceinline__ double wave_max_d(double v) {
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) {
+        const double o = __shfl_xor(v, d);
+        v = (v < o) ? o : v;
+    }
+    return v;
+}
 
 struct RootInfo {
     double cx, cy, cz, side, mass;
@@ -72,41 +91,6 @@ __device__ __forceinline__ void group_features16(const GroupStats& S, const Root
     f[13] = flog10(S.leaf_len / dist);
     f[14] = flog10(S.emax / dist);
     f[15] = flog10(S.emax / S.leaf_len);
-}
-
-constexpr int kFIn = 16, kFH = 64;
-constexpr int kFOffW1 = 0, kFOffB1 = kFH * kFIn, kFOffW2 = kFOffB1 + kFH,
-              kFOffB2 = kFOffW2 + kFH * kFH, kFOffW3 = kFOffB2 + kFH, kFOffB3 = kFOffW3 + kFH;
-
-__device__ __forceinline__ float bf16f(uint16_t b) { return __uint_as_float((uint32_t)b << 16); }
-__device__ __forceinline__ float gelu_f(float x) {
-    return 0.5f * x * (1.0f + erff(x * 0.70710678118654752f));
-}
-
-// The network for one group, evaluated by its wavefront: lane j owns hidden unit j of both
-// layers, the first layer's outputs are exchanged through LDS (sh, 64 floats), and the output is
-// a wave reduction. f must be the same in every lane. Weight matrices are bf16, biases and the
-// input map FP32; all arithmetic is FP32. Returns the output in every lane.
-__device__ __forceinline__ float mlp_wave(const float* f, const uint16_t* __restrict__ wbf,
-                                          const float* __restrict__ theta,
-                                          const float* __restrict__ xnorm, float* sh, int lane) {
-    float x[kFIn];
-#pragma unroll
-    for (int k = 0; k < kFIn; ++k) x[k] = (f[k] - xnorm[k]) * xnorm[kFIn + k];
-    float z = theta[kFOffB1 + lane];
-#pragma unroll
-    for (int k = 0; k < kFIn; ++k) z = __builtin_fmaf(bf16f(wbf[kFOffW1 + lane * kFIn + k]), x[k], z);
-    __syncthreads();
-    sh[lane] = gelu_f(z);
-    __syncthreads();
-    float z2 = theta[kFOffB2 + lane];
-    for (int i = 0; i < kFH; ++i)
-        z2 = __builtin_fmaf(bf16f(wbf[kFOffW2 + lane * kFH + i]), sh[i], z2);
-    float y = bf16f(wbf[kFOffW3 + lane]) * gelu_f(z2);
-#pragma unroll
-    for (int d = 32; d >= 1; d >>= 1) y += __shfl_xor(y, d);
-    __syncthreads();
-    return y + theta[kFOffB3];
 }
 
 } // namespace gz

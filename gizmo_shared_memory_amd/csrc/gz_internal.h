@@ -47,25 +47,14 @@ struct WalkArgs {
     const double4* src;     // (x, y, z, m) per sorted slot
     const double* src_soft; // softening per sorted slot
     const uint32_t* targets;
-    const double *x, *y, *z, *soft; // target data, indexed by particle
+    const int* tslot;        // sorted slot of each target (its data in src), -1 if not in the tree
+    const double *x, *y, *z, *soft; // target data by particle, for targets outside the tree
     const GroupInfo* groups; // precomputed group bounds, or NULL: bounds from gstart in-kernel
     const int* gstart;       // group g is target slots [gstart[g], gstart[g+1]) (or nt at the end)
     int nt;
     const double* aold;      // per target slot, or NULL
     const float* group_scale; // per group multiplier of aold_min (learned tolerance), or NULL
     double theta_sq;
-    // learned criterion evaluated in the walk (mlp_w non-NULL): the group's features, the
-    // network, and the tolerance or the fallback to the reference's
-    const double* m;
-    const int* leaf_of;
-    const int* level_begin;
-    double root_cx, root_cy, root_cz, root_side, root_mass;
-    const uint16_t* mlp_w;
-    const float* mlp_theta;
-    const float* mlp_xnorm;
-    const float* mlp_frange;
-    float mlp_lab_lo, mlp_lab_hi, mlp_margin, mlp_log_alpha_ref;
-    int* n_fallback;
     int ngroups;
     int root;
     const uint32_t* cluster_start; // forest: particle offsets of the clusters; NULL: one tree
@@ -79,7 +68,10 @@ struct WalkArgs {
     int* retry_list;        // ... and their ids, for the global-stack launch
     int* gstack;            // global node stacks of the retry launch, gstack_cap ints per wave
     int gstack_cap;
-    int stack_limit;        // > 0: use at most this many LDS stack entries (exercises the retry)
+    int* spill;             // persistent launches: spill_cap stack entries per wave, or NULL
+    int spill_cap;
+    int stack_limit;        // > 0: use at most this many LDS stack entries (exercises the spill
+                            // tier, and the retry when below the 512 entries one push can need)
     double G;
     double *ax, *ay, *az;
 };
@@ -131,7 +123,8 @@ const char* last_error_cstr();
 void clear_error();
 
 // ---- walks (gz_walk.hip) ------------------------------------------------------------------------
-gz_status launch_group_prep(const uint32_t* targets, size_t nt, const int* leaf_of,
+gz_status launch_group_prep(const uint32_t* targets, size_t nt, const uint32_t* rank,
+                            const int* slot_leaf, int* tslot,
                             const int* parent, const WNode* wn, int batch, int group_size,
                             size_t n, const uint32_t* seg, int nseg, int* gseg, int* gkey,
                             uint32_t* flag, uint32_t* gid, uint32_t* ngroups_dev, int* status,
@@ -149,9 +142,19 @@ gz_status launch_direct(const gz_particles* p, const uint32_t* targets, size_t n
                         double* ax, double* ay, double* az, hipStream_t st);
 
 // ---- learned opening criterion (gz_learned.hip) ----------------------------------------------------
+struct GroupStats;
+// The per-group sums behind the learned criterion's features (stats, one per group), and the
+// features themselves (feats, ngroups x 16, when non-NULL).
 gz_status launch_group_features(int ngroups, int nt, const int* gstart, const uint32_t* targets,
-                                const gz_particles* p, const double* amag, const ::gz_tree* t,
-                                float* feats, hipStream_t st);
+                                const int* tslot, const gz_particles* p, const double* amag,
+                                const gz_tree* t, GroupStats* stats, float* feats,
+                                hipStream_t st);
+// scale[g] = the learned tolerance of group g over the reference's (1 on fallback); n_fallback
+// counts the groups that fell back; stats is working space for ngroups GroupStats
+gz_status launch_group_tolerance(int ngroups, int nt, const int* gstart, const uint32_t* targets,
+                                 const int* tslot, const gz_particles* p, const double* aold,
+                                 const gz_tree* t, const gz_model* M, GroupStats* stats,
+                                 float* scale, int* n_fallback, hipStream_t st);
 gz_status launch_mlp_forward(const float* X, int B, const uint16_t* wbf, const float* theta,
                              const float* xnorm, float* out, hipStream_t st);
 gz_status launch_mlp_grad(const float* X, const float* T, int B, const uint16_t* wbf,
@@ -170,7 +173,16 @@ struct gz_ctx {
     bool own_stream = false;
     gz::DevMem scratch[12];
     hipEvent_t ev[16] = {};
+    // the walk's node-stack spill slices: allocated once, at a fixed size, on the first walk, so
+    // that captured graphs can keep the pointer (walks on one context never overlap)
+    int* spill = nullptr;
+    int spill_cap = 0;
 };
+
+namespace gz {
+// The spill slices of ctx (allocating them on first use) and their size per wave.
+gz_status ctx_spill(gz_ctx* ctx, int** spill, int* cap);
+} // namespace gz
 
 // A walk whose groups were formed once for a fixed target list; launching it enqueues only
 // kernels (no host synchronisation), so launches can be captured into a hipGraph.
@@ -185,8 +197,11 @@ struct gz_walk_plan {
     int persistent = 1;
     gz_model* model = nullptr;
     int gstack_cap = 0, retry_waves = 0;
-    void* mem = nullptr; // gstart, counters, retry list, global stacks
+    void* mem = nullptr; // gstart, counters, retry list, global stacks, learned group scales
     int* gstart = nullptr;
+    int* tslot = nullptr; // sorted slot of each target
+    float* scale = nullptr;
+    gz::GroupStats* stats = nullptr; // learned: the groups' sums (working space of a launch)
     long long* misc = nullptr;
     int* retry_list = nullptr;
     int* gstack = nullptr;
@@ -225,6 +240,7 @@ struct gz_tree {
     uint32_t* orderbuf = nullptr; // n_live particle index per sorted slot
     uint32_t* rank = nullptr;     // n: sorted slot per particle, UINT32_MAX if not in the tree
     int* leaf_of = nullptr;       // n: leaf node per particle, -1 if not in the tree
+    int* slot_leaf = nullptr;     // n_live: leaf node per sorted slot
     double4* src = nullptr;       // n_live (x, y, z, m) in sorted order
     double* src_soft = nullptr;   // n_live softening in sorted order
 

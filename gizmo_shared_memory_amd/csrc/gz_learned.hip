@@ -35,56 +35,98 @@ __device__ __forceinline__ float gelu_grad(float x) {
 }
 
 // ---- group features ------------------------------------------------------------------------------
-// amag is |a_prev| per target slot (no-G units, the basis of aold).
+// amag is |a_prev| per target slot (no-G units, the basis of aold). The sums over a group's
+// targets are taken by one wavefront per group (lanes read consecutive targets, so the reads
+// coalesce); the features and the network, arithmetic on a handful of scalars, run one thread
+// per group.
 
-__global__ __launch_bounds__(256) void k_group_features(
-    int ngroups, int nt_total, const int* __restrict__ gstart,
-    const uint32_t* __restrict__ targets, const double* __restrict__ x,
-    const double* __restrict__ y, const double* __restrict__ z, const double* __restrict__ m,
-    const double* __restrict__ soft, const double* __restrict__ amag,
-    const int* __restrict__ leaf_of, const int* __restrict__ level_begin,
-    const WNode* __restrict__ wn, int root, const uint32_t* __restrict__ cstart, int nc,
-    float* __restrict__ feats) {
-    // one thread per group: groups are small, and many independent groups in flight hide the
-    // latency of the target gathers better than one wavefront per group
-    const int g = blockIdx.x * 256 + threadIdx.x;
-    if (g >= ngroups) return;
-    GroupInfo G;
-    G.tlo = gstart[g];
-    G.thi = (g + 1 < ngroups) ? gstart[g + 1] : nt_total;
-    const int nt = G.thi - G.tlo;
+struct FeatureInputs {
+    int ngroups, nt_total;
+    const int* gstart;
+    const uint32_t* targets;
+    const int* tslot;        // sorted slot per target (data in src), -1 if not in the tree
+    const double4* src;
+    const double* src_soft;
+    const double *x, *y, *z, *m, *soft, *amag;
+    const int* slot_leaf;
+    const int* level_begin;
+    const WNode* wn;
+    int root;
+    const uint32_t* cstart;
+    int nc;
+};
+
+__device__ __forceinline__ double wave_sum_d(double v) {
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) v += __shfl_xor(v, d);
+    return v;
+}
+
+__global__ __launch_bounds__(256) void k_group_stats(FeatureInputs in,
+                                                     GroupStats* __restrict__ out) {
+    const int lane = threadIdx.x & 63;
+    const int g = (int)(((size_t)blockIdx.x * 256 + threadIdx.x) >> 6);
+    if (g >= in.ngroups) return;
+    const int tlo = in.gstart[g];
+    const int thi = (g + 1 < in.ngroups) ? in.gstart[g + 1] : in.nt_total;
     double msum = 0, amin = 1e300, amax = 0, a2 = 0, emax = 0;
     double lo0 = 1e300, lo1 = 1e300, lo2 = 1e300, hi0 = -1e300, hi1 = -1e300, hi2 = -1e300;
-    for (int i = G.tlo; i < G.thi; ++i) {
-        const uint32_t p = targets[i];
-        msum += m[p];
-        const double am = amag ? amag[i] : 0.0;
+    for (int i = tlo + lane; i < thi; i += 64) {
+        const int sl = in.tslot[i];
+        double px, py, pz, pm, pe;
+        if (sl >= 0) { // the tree's sorted copy: consecutive targets, nearby slots
+            const double4 q = in.src[sl];
+            px = q.x; py = q.y; pz = q.z; pm = q.w;
+            pe = in.src_soft[sl];
+        } else {
+            const uint32_t p = in.targets[i];
+            px = in.x[p]; py = in.y[p]; pz = in.z[p]; pm = in.m[p];
+            pe = in.soft ? in.soft[p] : 0.0;
+        }
+        msum += pm;
+        const double am = in.amag ? in.amag[i] : 0.0;
         amin = fmin(amin, am);
         amax = fmax(amax, am);
         a2 += am * am;
-        if (soft) emax = fmax(emax, soft[p]);
-        lo0 = fmin(lo0, x[p]); hi0 = fmax(hi0, x[p]);
-        lo1 = fmin(lo1, y[p]); hi1 = fmax(hi1, y[p]);
-        lo2 = fmin(lo2, z[p]); hi2 = fmax(hi2, z[p]);
+        emax = fmax(emax, pe);
+        lo0 = fmin(lo0, px); hi0 = fmax(hi0, px);
+        lo1 = fmin(lo1, py); hi1 = fmax(hi1, py);
+        lo2 = fmin(lo2, pz); hi2 = fmax(hi2, pz);
     }
     GroupStats S;
-    S.lo[0] = lo0; S.lo[1] = lo1; S.lo[2] = lo2;
-    S.hi[0] = hi0; S.hi[1] = hi1; S.hi[2] = hi2;
-    S.emax = emax;
-    S.msum = msum;
-    S.amin = amin;
-    S.amax = amax;
-    S.a2 = a2;
-    S.nt = nt;
-    const WNode Rn = wn[root_of(targets[G.tlo], cstart, nc, root)];
+    S.lo[0] = wave_min_d(lo0); S.lo[1] = wave_min_d(lo1); S.lo[2] = wave_min_d(lo2);
+    S.hi[0] = wave_max_d(hi0); S.hi[1] = wave_max_d(hi1); S.hi[2] = wave_max_d(hi2);
+    S.emax = wave_max_d(emax);
+    S.msum = wave_sum_d(msum);
+    S.amin = wave_min_d(amin);
+    S.amax = wave_max_d(amax);
+    S.a2 = wave_sum_d(a2);
+    S.nt = thi - tlo;
+    S.root = root_of(in.targets[tlo], in.cstart, in.nc, in.root);
+    const int leaf = (in.tslot[tlo] >= 0) ? in.slot_leaf[in.tslot[tlo]] : -1;
+    S.leaf_len = (leaf >= 0) ? (double)in.wn[leaf].len : -1.0;
+    // node ids are level-major: the leaf's level is the number of level starts at or below it
+    const bool below = leaf >= 0 && lane < kMaxLevel && in.level_begin[lane + 1] <= leaf;
+    S.depth = (int)__popcll(__ballot(below));
+    if (lane == 0) out[g] = S;
+}
+
+__device__ __forceinline__ void features_of(const GroupStats& S0, const WNode* __restrict__ wn,
+                                            float* f) {
+    GroupStats S = S0;
+    const WNode Rn = wn[S.root];
     const RootInfo R{Rn.cx, Rn.cy, Rn.cz, (double)Rn.len, Rn.mass};
-    const int leaf = leaf_of[targets[G.tlo]];
-    S.leaf_len = (leaf >= 0) ? (double)wn[leaf].len : fmax(R.side, 1e-300);
-    S.depth = 0; // node ids are level-major: the level is where the id falls in the table
-    if (leaf >= 0)
-        while (S.depth < kMaxLevel && level_begin[S.depth + 1] <= leaf) ++S.depth;
-    float f[kIn];
+    if (S.leaf_len < 0) S.leaf_len = fmax(R.side, 1e-300); // a target outside the tree
     group_features16(S, R, f);
+}
+
+__global__ __launch_bounds__(256) void k_group_features(const GroupStats* __restrict__ stats,
+                                                        int ngroups, const WNode* __restrict__ wn,
+                                                        float* __restrict__ feats) {
+    const int g = blockIdx.x * 256 + threadIdx.x;
+    if (g >= ngroups) return;
+    float f[kIn];
+    features_of(stats[g], wn, f);
     float* o = feats + (size_t)g * kIn;
 #pragma unroll
     for (int k = 0; k < kIn; ++k) o[k] = f[k];
@@ -106,21 +148,15 @@ __device__ __forceinline__ void stage_weights(const uint16_t* __restrict__ wbf,
     __syncthreads();
 }
 
-__global__ __launch_bounds__(256) void k_mlp_forward(const float* __restrict__ X, int B,
-                                                     const uint16_t* __restrict__ wbf,
-                                                     const float* __restrict__ theta,
-                                                     const float* __restrict__ xnorm,
-                                                     float* __restrict__ out) {
-    __shared__ float sW[kMlpParams];
-    __shared__ float sN[2 * kIn];
-    stage_weights(wbf, theta, xnorm, sW, sN);
-    const int b = blockIdx.x * 256 + threadIdx.x;
-    if (b >= B) return;
+// The network on one sample, weights read from LDS (the same address in every lane: broadcasts).
+__device__ __forceinline__ float mlp_forward_one(const float* raw, const float* sW,
+                                                const float* sN) {
     float xin[kIn];
 #pragma unroll
-    for (int k = 0; k < kIn; ++k) xin[k] = (X[(size_t)b * kIn + k] - sN[k]) * sN[kIn + k];
+    for (int k = 0; k < kIn; ++k) xin[k] = (raw[k] - sN[k]) * sN[kIn + k];
+    // h1 stays in registers: every index into it is a compile-time constant
     float h1[kH];
-#pragma unroll 4
+#pragma unroll
     for (int i = 0; i < kH; ++i) {
         float zz = sW[kOffB1 + i];
 #pragma unroll
@@ -134,7 +170,66 @@ __global__ __launch_bounds__(256) void k_mlp_forward(const float* __restrict__ X
         for (int j = 0; j < kH; ++j) zz = __builtin_fmaf(sW[kOffW2 + i * kH + j], h1[j], zz);
         yy = __builtin_fmaf(sW[kOffW3 + i], gelu(zz), yy);
     }
-    out[b] = yy;
+    return yy;
+}
+
+__global__ __launch_bounds__(256) void k_mlp_forward(const float* __restrict__ X, int B,
+                                                     const uint16_t* __restrict__ wbf,
+                                                     const float* __restrict__ theta,
+                                                     const float* __restrict__ xnorm,
+                                                     float* __restrict__ out) {
+    __shared__ float sW[kMlpParams];
+    __shared__ float sN[2 * kIn];
+    stage_weights(wbf, theta, xnorm, sW, sN);
+    const int b = blockIdx.x * 256 + threadIdx.x;
+    if (b >= B) return;
+    float raw[kIn];
+#pragma unroll
+    for (int k = 0; k < kIn; ++k) raw[k] = X[(size_t)b * kIn + k];
+    out[b] = mlp_forward_one(raw, sW, sN);
+}
+
+// ---- the per-group tolerance -------------------------------------------------------------------
+// scale[g] = alpha_g / alpha_ref = 10^(pred - margin - log10 alpha_ref), the factor the walk applies
+// to the group's aold minimum, or 1 (the reference tolerance) when a feature lies outside the range
+// seen in training or the prediction outside the trained label range. Fallbacks are counted.
+struct ToleranceModel {
+    const uint16_t* wbf;
+    const float* theta;
+    const float* xnorm;
+    const float* frange; // lo[16], hi[16]
+    float lab_lo, lab_hi, margin, log_alpha_ref;
+};
+
+__global__ __launch_bounds__(256) void k_group_scale(const GroupStats* __restrict__ stats,
+                                                     int ngroups, const WNode* __restrict__ wn,
+                                                     ToleranceModel md, float* __restrict__ scale,
+                                                     int* __restrict__ n_fallback) {
+    __shared__ float sW[kMlpParams];
+    __shared__ float sN[2 * kIn];
+    __shared__ float sR[2 * kIn];
+    stage_weights(md.wbf, md.theta, md.xnorm, sW, sN);
+    for (int k = threadIdx.x; k < 2 * kIn; k += blockDim.x) sR[k] = md.frange[k];
+    __syncthreads();
+    const int g = blockIdx.x * 256 + threadIdx.x;
+    bool fell_back = false;
+    if (g < ngroups) {
+        float f[kIn];
+        features_of(stats[g], wn, f);
+        bool ok = true;
+#pragma unroll
+        for (int k = 0; k < kIn; ++k) ok = ok && (f[k] >= sR[k]) && (f[k] <= sR[kIn + k]);
+        float sc = 1.0f;
+        if (ok) {
+            const float pred = mlp_forward_one(f, sW, sN);
+            ok = (pred >= md.lab_lo) && (pred <= md.lab_hi);
+            if (ok) sc = exp10f(pred - md.margin - md.log_alpha_ref);
+        }
+        scale[g] = sc;
+        fell_back = !ok;
+    }
+    const uint64_t b = __ballot(fell_back);
+    if (b && (threadIdx.x & 63) == 0) atomicAdd(n_fallback, (int)__popcll(b));
 }
 
 // ---- backpropagation -------------------------------------------------------------------------------
@@ -291,13 +386,55 @@ __global__ __launch_bounds__(256) void k_to_bf16(const float* __restrict__ theta
 }
 
 // ---- host launchers ---------------------------------------------------------------------------------
+static FeatureInputs feature_inputs(int ngroups, int nt, const int* gstart,
+                                    const uint32_t* targets, const int* tslot,
+                                    const gz_particles* p, const double* amag, const gz_tree* t) {
+    FeatureInputs in;
+    in.ngroups = ngroups;
+    in.nt_total = nt;
+    in.gstart = gstart;
+    in.targets = targets;
+    in.tslot = tslot;
+    in.src = t->src;
+    in.src_soft = t->src_soft;
+    in.x = p->x; in.y = p->y; in.z = p->z; in.m = p->m; in.soft = p->soft;
+    in.amag = amag;
+    in.slot_leaf = t->slot_leaf;
+    in.level_begin = t->level_begin_dev;
+    in.wn = t->wn;
+    in.root = t->root;
+    in.cstart = t->cluster_start_dev;
+    in.nc = t->n_clusters;
+    return in;
+}
+
 gz_status launch_group_features(int ngroups, int nt, const int* gstart, const uint32_t* targets,
-                                const gz_particles* p, const double* amag, const gz_tree* t,
-                                float* feats, hipStream_t st) {
+                                const int* tslot, const gz_particles* p, const double* amag,
+                                const gz_tree* t, GroupStats* stats, float* feats,
+                                hipStream_t st) {
     if (ngroups <= 0) return GZ_OK;
-    k_group_features<<<(unsigned)((ngroups + 255) / 256), 256, 0, st>>>(
-        ngroups, nt, gstart, targets, p->x, p->y, p->z, p->m, p->soft, amag, t->leaf_of,
-        t->level_begin_dev, t->wn, t->root, t->cluster_start_dev, t->n_clusters, feats);
+    k_group_stats<<<(unsigned)(((size_t)ngroups + 3) / 4), 256, 0, st>>>(
+        feature_inputs(ngroups, nt, gstart, targets, tslot, p, amag, t), stats);
+    GZ_HIP(hipGetLastError());
+    if (feats) {
+        k_group_features<<<(unsigned)((ngroups + 255) / 256), 256, 0, st>>>(stats, ngroups, t->wn,
+                                                                            feats);
+        GZ_HIP(hipGetLastError());
+    }
+    return GZ_OK;
+}
+
+gz_status launch_group_tolerance(int ngroups, int nt, const int* gstart, const uint32_t* targets,
+                                 const int* tslot, const gz_particles* p, const double* aold,
+                                 const gz_tree* t, const gz_model* M, GroupStats* stats,
+                                 float* scale, int* n_fallback, hipStream_t st) {
+    if (ngroups <= 0) return GZ_OK;
+    GZ_TRY(launch_group_features(ngroups, nt, gstart, targets, tslot, p, aold, t, stats, nullptr,
+                                 st));
+    ToleranceModel md{M->wbf, M->theta, M->xnorm, M->frange, M->lab_lo, M->lab_hi, M->margin,
+                      M->log_alpha_ref};
+    k_group_scale<<<(unsigned)((ngroups + 255) / 256), 256, 0, st>>>(stats, ngroups, t->wn, md,
+                                                                     scale, n_fallback);
     GZ_HIP(hipGetLastError());
     return GZ_OK;
 }

@@ -30,6 +30,7 @@
 namespace gz {
 
 constexpr int kWalkStack = 1024;  // node ids per wave in LDS
+constexpr int kSpillKeep = 256;   // LDS entries kept when the bottom of the stack is spilled
 constexpr int kMonoF = 160;       // monopole entries per wave, FP32 (16 B each)
 constexpr int kPairF = 128;       // particle entries per wave, FP32 (20 B each)
 constexpr int kMonoD = 96;        // monopole entries per wave, FP64 (32 B each)
@@ -70,7 +71,9 @@ __device__ __forceinline__ bool open_node_d(double r_sq, double len, double l_op
 // leaf holding at most group_size particles (group_size > 0), or the window index (batch > 0).
 // A target outside the tree (leaf_of < 0) always starts a group of its own (tree.cc:663-664).
 __global__ __launch_bounds__(256) void k_group_flags(const uint32_t* __restrict__ targets,
-                                                     size_t nt, const int* __restrict__ leaf_of,
+                                                     size_t nt, const uint32_t* __restrict__ rank,
+                                                     const int* __restrict__ slot_leaf,
+                                                     int* __restrict__ tslot,
                                                      const int* __restrict__ parent,
                                                      const WNode* __restrict__ wn, int batch,
                                                      int group_size, size_t n,
@@ -92,13 +95,19 @@ __global__ __launch_bounds__(256) void k_group_flags(const uint32_t* __restrict_
     if (targets[i] >= n) { // reported to the caller before anything reads through it
         atomicOr(status, 2);
         gkey[i] = -1;
+        tslot[i] = -1;
         return;
     }
+    // the target's sorted slot: its data is read from the tree's sorted copy, which for a
+    // Morton-ordered target list is a near-sequential read (the particle arrays are gathers)
+    const uint32_t r = rank[targets[i]];
+    const int slot = (r == 0xFFFFFFFFu) ? -1 : (int)r;
+    tslot[i] = slot;
     int key;
     if (batch > 0) {
         key = (int)(i / (size_t)batch);
     } else {
-        key = leaf_of[targets[i]];
+        key = (slot >= 0) ? slot_leaf[slot] : -1;
         if (key >= 0 && group_size > 0) {
             int p = parent[key];
             while (p >= 0 && wn[p].phi - wn[p].plo <= group_size) {
@@ -184,42 +193,39 @@ __global__ __launch_bounds__(256) void k_pack_starts(int ngroups, const int* __r
     if (mflag[i]) gstart2[mid[i]] = gstart[i];
 }
 
-__device__ __forceinline__ double wave_min_d(double v) {
-#pragma unroll
-    for (int d = 32; d >= 1; d >>= 1) v = dmin_w(v, __shfl_xor(v, d));
-    return v;
-}
-__device__ __forceinline__ double wave_max_d(double v) {
-#pragma unroll
-    for (int d = 32; d >= 1; d >>= 1) v = dmax_w(v, __shfl_xor(v, d));
-    return v;
+
+// Position and softening of target slot i: from the tree's sorted copy when the target is in the
+// tree, else from the particle arrays.
+__device__ __forceinline__ void target_data(const WalkArgs& a, int i, double& px, double& py,
+                                            double& pz, double& pe) {
+    const int s = a.tslot[i];
+    if (s >= 0) {
+        const double4 q = a.src[s];
+        px = q.x; py = q.y; pz = q.z;
+        pe = a.src_soft[s];
+    } else {
+        const uint32_t p = a.targets[i];
+        px = a.x[p]; py = a.y[p]; pz = a.z[p];
+        pe = a.soft ? a.soft[p] : 0.0;
+    }
 }
 
 // The opening bounds of group g (tree.cc:683-702: bounding box, largest softening, smallest aold),
 // computed by the group's own wavefront. Min and max are exact, so every lane ends with the same
-// values whatever the reduction order.
-__device__ __noinline__ GroupInfo group_bounds(const WalkArgs& a, int g, int lane, float* sh) {
+// values whatever the reduction order. A learned tolerance enters as a factor on aold_min.
+__device__ __forceinline__ GroupInfo group_bounds(const WalkArgs& a, int g, int lane) {
     const int lo = a.gstart[g];
     const int hi = (g + 1 < a.ngroups) ? a.gstart[g + 1] : a.nt;
-    const bool learned = a.mlp_w != nullptr;
     double bx0 = 1e300, by0 = 1e300, bz0 = 1e300, bx1 = -1e300, by1 = -1e300, bz1 = -1e300;
-    double emax = 0, amin = a.aold ? 1e300 : 0.0, amax = 0, a2 = 0, msum = 0;
+    double emax = 0, amin = a.aold ? 1e300 : 0.0;
     for (int i = lo + lane; i < hi; i += 64) {
-        const uint32_t p = a.targets[i];
-        const double px = a.x[p], py = a.y[p], pz = a.z[p];
+        double px, py, pz, pe;
+        target_data(a, i, px, py, pz, pe);
         bx0 = dmin_w(bx0, px); bx1 = dmax_w(bx1, px);
         by0 = dmin_w(by0, py); by1 = dmax_w(by1, py);
         bz0 = dmin_w(bz0, pz); bz1 = dmax_w(bz1, pz);
-        if (a.soft) emax = dmax_w(emax, a.soft[p]);
-        if (a.aold) {
-            const double ao = a.aold[i];
-            amin = dmin_w(amin, ao);
-            if (learned) {
-                amax = fmax(amax, ao);
-                a2 += ao * ao;
-            }
-        }
-        if (learned) msum += a.m[p];
+        emax = dmax_w(emax, pe);
+        if (a.aold) amin = dmin_w(amin, a.aold[i]);
     }
     GroupInfo o;
     o.lo[0] = wave_min_d(bx0); o.lo[1] = wave_min_d(by0); o.lo[2] = wave_min_d(bz0);
@@ -230,46 +236,6 @@ __device__ __noinline__ GroupInfo group_bounds(const WalkArgs& a, int g, int lan
     o.tlo = lo;
     o.thi = hi;
     if (a.group_scale) o.amin *= (double)a.group_scale[g];
-    if (learned) {
-        // the learned tolerance of this group (the same features and network as the training
-        // path, gz_learned.hip), or the reference's when the group is outside the trusted band
-        GroupStats S;
-        for (int k = 0; k < 3; ++k) {
-            S.lo[k] = o.lo[k];
-            S.hi[k] = o.hi[k];
-        }
-        S.emax = o.emax;
-        S.amin = o.amin;
-        S.amax = wave_max_d(amax);
-        double t = a2, u = msum;
-#pragma unroll
-        for (int d = 32; d >= 1; d >>= 1) {
-            t += __shfl_xor(t, d);
-            u += __shfl_xor(u, d);
-        }
-        S.a2 = t;
-        S.msum = u;
-        S.nt = hi - lo;
-        const int leaf = a.leaf_of[a.targets[lo]];
-        S.leaf_len = (leaf >= 0) ? (double)a.wn[leaf].len : fmax(a.root_side, 1e-300);
-        S.depth = 0;
-        if (leaf >= 0)
-            while (S.depth < kMaxLevel && a.level_begin[S.depth + 1] <= leaf) ++S.depth;
-        RootInfo R{a.root_cx, a.root_cy, a.root_cz, a.root_side, a.root_mass};
-        if (a.nclusters > 1) { // the group's own tree
-            const WNode Rn = a.wn[root_of(a.targets[lo], a.cluster_start, a.nclusters, a.root)];
-            R = RootInfo{Rn.cx, Rn.cy, Rn.cz, (double)Rn.len, Rn.mass};
-        }
-        float f[16];
-        group_features16(S, R, f);
-        const float pred = mlp_wave(f, a.mlp_w, a.mlp_theta, a.mlp_xnorm, sh, lane);
-        bool ok = (pred >= a.mlp_lab_lo) && (pred <= a.mlp_lab_hi);
-#pragma unroll
-        for (int k = 0; k < 16; ++k)
-            ok = ok && (f[k] >= a.mlp_frange[k]) && (f[k] <= a.mlp_frange[16 + k]);
-        if (ok) o.amin *= (double)exp10f(pred - a.mlp_margin - a.mlp_log_alpha_ref);
-        else if (lane == 0) atomicAdd(a.n_fallback, 1);
-    }
     return o;
 }
 
@@ -332,6 +298,12 @@ __global__ __launch_bounds__(64, 4) void k_walk(WalkArgs a) {
     int* const stk = GSTACK ? a.gstack + (size_t)blockIdx.x * (size_t)a.gstack_cap : s_stack;
     const int scap = GSTACK ? a.gstack_cap
                             : (a.stack_limit > 0 ? min(a.stack_limit, kWalkStack) : kWalkStack);
+    // Persistent waves extend the LDS stack downwards into a slice of global memory: when a push
+    // would overflow LDS, the bottom of the LDS stack moves to the slice, and it comes back when
+    // LDS runs empty. The logical stack, and so the traversal order, is that of an unbounded stack.
+    int* const spill = (!GSTACK && a.persistent && a.spill)
+                           ? a.spill + (size_t)blockIdx.x * (size_t)a.spill_cap
+                           : nullptr;
 
     const int lane = threadIdx.x;
     long long n_visit = 0, n_multi = 0, n_pair = 0;
@@ -362,7 +334,7 @@ __global__ __launch_bounds__(64, 4) void k_walk(WalkArgs a) {
         }
 
         const GroupInfo gi =
-            a.groups ? a.groups[g] : group_bounds(a, g, lane, reinterpret_cast<float*>(s_lpre));
+            a.groups ? a.groups[g] : group_bounds(a, g, lane);
         const double gcx = 0.5 * (gi.lo[0] + gi.hi[0]);
         const double gcy = 0.5 * (gi.lo[1] + gi.hi[1]);
         const double gcz = 0.5 * (gi.lo[2] + gi.hi[2]);
@@ -380,14 +352,14 @@ __global__ __launch_bounds__(64, 4) void k_walk(WalkArgs a) {
             const int jstart = tvalid ? sub : (1 << 30);
             real tx = 0, ty = 0, tz = 0, te = 0;
             if (tvalid) {
-                const uint32_t p = a.targets[tlo + t];
-                const double px = a.x[p], py = a.y[p], pz = a.z[p];
+                double px, py, pz, pe;
+                target_data(a, tlo + t, px, py, pz, pe);
                 if (FP64) {
                     tx = (real)px; ty = (real)py; tz = (real)pz;
                 } else {
                     tx = (real)(px - gcx); ty = (real)(py - gcy); tz = (real)(pz - gcz);
                 }
-                te = a.soft ? (real)a.soft[p] : (real)0;
+                te = (real)pe;
             }
             double accx = 0, accy = 0, accz = 0;
             int mcount = 0, pcount = 0;
@@ -477,6 +449,7 @@ __global__ __launch_bounds__(64, 4) void k_walk(WalkArgs a) {
             // Node batches are loaded one iteration ahead: the loads for the next batch are in
             // flight while this batch's leaves are copied and the list is evaluated.
             int sp = 0;
+            int gsp = 0; // entries below the LDS stack, in the spill slice
             int k = 1;
             int node = (lane == 0)
                            ? root_of(a.targets[gi.tlo], a.cluster_start, a.nclusters, a.root)
@@ -524,14 +497,48 @@ __global__ __launch_bounds__(64, 4) void k_walk(WalkArgs a) {
                 int ctot;
                 const int coff = wave_excl_scan(nch, ctot);
                 if (sp + ctot > scap) {
-                    overflow = true;
-                    break;
+                    // keep the top `keep` entries in LDS, move the rest to the spill slice
+                    const int keep = min(min(sp, kSpillKeep), scap - ctot);
+                    if (!spill || keep < 0 || gsp + (sp - keep) > a.spill_cap) {
+                        overflow = true;
+                        break;
+                    }
+                    const int ms = sp - keep;
+                    for (int i = lane; i < ms; i += 64) spill[gsp + i] = stk[i];
+                    int hold[kSpillKeep / 64];
+#pragma unroll
+                    for (int r = 0; r < kSpillKeep / 64; ++r) {
+                        const int i = lane + 64 * r;
+                        hold[r] = (i < keep) ? stk[ms + i] : 0;
+                    }
+                    __syncthreads();
+#pragma unroll
+                    for (int r = 0; r < kSpillKeep / 64; ++r) {
+                        const int i = lane + 64 * r;
+                        if (i < keep) stk[i] = hold[r];
+                    }
+                    gsp += ms;
+                    sp = keep;
                 }
                 for (int c = 0; c < nch; ++c) stk[sp + coff + c] = first + c;
                 sp += ctot;
-                if (COUNT) max_sp = max(max_sp, sp);
+                if (COUNT) max_sp = max(max_sp, gsp + sp);
                 if (GSTACK) __threadfence_block();
                 __syncthreads();
+                if (sp < 64 && gsp > 0) {
+                    // fewer than a batch left in LDS: slide them up and bring back the top of
+                    // the spilled part beneath them, so the batch popped next is the same top
+                    // 64 entries an unbounded stack would give
+                    const int r = min(gsp, min(scap / 2, scap - sp));
+                    const int held = (lane < sp) ? stk[lane] : 0;
+                    __threadfence_block();
+                    __syncthreads();
+                    if (lane < sp) stk[r + lane] = held;
+                    for (int i = lane; i < r; i += 64) stk[i] = spill[gsp - r + i];
+                    gsp -= r;
+                    sp += r;
+                    __syncthreads();
+                }
                 k = min(sp, 64);
                 const int base = sp - k;
                 const int next_node = (lane < k) ? stk[base + lane] : -1;
@@ -708,14 +715,41 @@ __global__ __launch_bounds__(kDsThreads) void k_direct_fp64(
 }
 
 // ---- host launchers ----------------------------------------------------------------------------
+// Wavefronts of a persistent walk launch: 16 per CU (four per SIMD, the occupancy the walk's
+// registers and LDS allow).
+static gz_status persistent_waves(int* waves) {
+    int dev = 0, ncu = 0;
+    GZ_HIP(hipGetDevice(&dev));
+    GZ_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+    *waves = std::max(1, ncu) * 16;
+    return GZ_OK;
+}
+
+// A group's logical stack never exceeds 448 entries per tree level plus one batch of children
+// (each batch pops 64 nodes and pushes at most 512), under 10k for kMaxLevel + 1 levels.
+constexpr int kSpillCap = 16384;
+
+gz_status ctx_spill(gz_ctx* ctx, int** spill, int* cap) {
+    if (!ctx->spill) {
+        int waves = 0;
+        GZ_TRY(persistent_waves(&waves));
+        void* p = nullptr;
+        GZ_HIP(hipMalloc(&p, (size_t)waves * kSpillCap * sizeof(int)));
+        ctx->spill = static_cast<int*>(p);
+        ctx->spill_cap = kSpillCap;
+    }
+    *spill = ctx->spill;
+    *cap = ctx->spill_cap;
+    return GZ_OK;
+}
+
 gz_status launch_walk(const WalkArgs& a, bool fp64, bool count, hipStream_t st) {
     if (a.ngroups <= 0) return GZ_OK;
     unsigned grid;
     if (a.persistent) {
-        int dev = 0, ncu = 0;
-        GZ_HIP(hipGetDevice(&dev));
-        GZ_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
-        grid = (unsigned)std::min<long long>((long long)ncu * 16, a.ngroups);
+        int waves = 0;
+        GZ_TRY(persistent_waves(&waves));
+        grid = (unsigned)std::min<long long>(waves, a.ngroups);
     } else {
         grid = (unsigned)a.ngroups;
     }
@@ -748,13 +782,14 @@ gz_status launch_walk_retry(const WalkArgs& a, int waves, bool fp64, bool count,
     return GZ_OK;
 }
 
-gz_status launch_group_prep(const uint32_t* targets, size_t nt, const int* leaf_of,
-                            const int* parent, const WNode* wn, int batch, int group_size,
+gz_status launch_group_prep(const uint32_t* targets, size_t nt, const uint32_t* rank,
+                            const int* slot_leaf, int* tslot, const int* parent, const WNode* wn, int batch, int group_size,
                             size_t n, const uint32_t* seg, int nseg, int* gseg, int* gkey,
                             uint32_t* flag, uint32_t* gid, uint32_t* ngroups_dev, int* status,
                             void* scan_tmp, hipStream_t st) {
     const unsigned grid = (unsigned)((nt + 255) / 256);
-    k_group_flags<<<grid, 256, 0, st>>>(targets, nt, leaf_of, parent, wn, batch, group_size, n,
+    k_group_flags<<<grid, 256, 0, st>>>(targets, nt, rank, slot_leaf, tslot, parent, wn, batch,
+                                        group_size, n,
                                         seg, nseg, gseg, gkey, status);
     k_group_flags2<<<grid, 256, 0, st>>>(nt, batch, gkey, nseg > 1 ? gseg : nullptr, flag);
     GZ_HIP(hipGetLastError());

@@ -230,7 +230,8 @@ def main():
     import argparse
     ap = argparse.ArgumentParser(description="train the learned opening criterion on the GPU")
     ap.add_argument("--out", default=DEFAULT_MODEL)
-    ap.add_argument("--group-size", type=int, default=32)
+    ap.add_argument("--group-sizes", default="32,64",
+                    help="walk group sizes whose groups are sampled (the features cover them all)")
     ap.add_argument("--target", type=float, default=1e-2)
     ap.add_argument("--p99", type=float, default=8e-3)
     ap.add_argument("--steps", type=int, default=4000)
@@ -242,8 +243,11 @@ def main():
     sets, vsets = TRAIN_SETS, VALID_SETS
     if args.quick:
         sets, vsets = TRAIN_SETS[:1], VALID_SETS[:1]
-    X, Y, E, names = build_dataset(sets, args.group_size, args.target)
-    Xv, Yv, Ev, _ = build_dataset(vsets, args.group_size, args.target)
+    sizes = [int(v) for v in args.group_sizes.split(",")]
+    parts = [build_dataset(sets, gs, args.target) for gs in sizes]
+    X, Y, E = (np.concatenate([p[k] for p in parts]) for k in range(3))
+    vparts = [build_dataset(vsets, gs, args.target) for gs in sizes]
+    Xv, Yv, Ev = (np.concatenate([p[k] for p in vparts]) for k in range(3))
     log(f"training samples: {len(Y)} groups x {len(ALPHA_GRID)} tolerances = "
         f"{len(Y) * len(ALPHA_GRID)} (group, tolerance, error) samples; validation {len(Yv)}")
     M, params, mu, scale, hist = train_network(X, Y, steps=args.steps, quantile=args.quantile)
@@ -264,7 +268,7 @@ def main():
     meta = dict(train_sets=[s[0] for s in sets], valid_sets=[s[0] for s in vsets],
                 n_train_groups=int(len(Y)), n_samples=int(len(Y) * len(ALPHA_GRID)),
                 alpha_grid=[float(a) for a in ALPHA_GRID], target=args.target,
-                p99_goal=args.p99, group_size=args.group_size, steps=args.steps,
+                p99_goal=args.p99, group_sizes=sizes, steps=args.steps,
                 quantile=args.quantile, valid_mean_log_alpha=mean_alpha,
                 valid_rmse_dex=rmse_v, constant_rmse_dex=rmse_c, valid_p99=p99v,
                 in_range_fraction=frac, loss_history=hist, seconds=time.time() - t0)

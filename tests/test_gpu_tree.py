@@ -287,6 +287,27 @@ def test_stack_spill_rewalk_is_identical(gz, prec):
         assert torch.equal(u, v)
 
 
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_stack_spill_tier_is_identical(gz, prec):
+    """A persistent walk moves the bottom of an overfull on-chip stack to global memory and back;
+    the logical stack is unchanged, so the result is bit for bit the unlimited one, with no
+    group walked twice. The per-group launch has no spill tier and walks such groups again."""
+    case = tree_cases.plummer_small(20000, seed=31, eps=0.01)
+    P, T = _device_tree(gz, case)
+    tg = T.device_orderbuf()
+    a = gz.accel_grouped(T, P, tg, 0.2, 1.0, 0, precision=prec, counts=True)
+    b = gz.accel_grouped(T, P, tg, 0.2, 1.0, 0, precision=prec, counts=True, stack_limit=512)
+    c = gz.accel_grouped(T, P, tg, 0.2, 1.0, 0, precision=prec, counts=True, stack_limit=512,
+                         persistent=False)
+    assert a[3]["max_stack"] > 512  # the limited walks had to spill
+    assert b[3]["retries"] == 0 and c[3]["retries"] > 0
+    for r in (b, c):
+        for k in ("nodes_visited", "multipole", "pairs", "max_stack"):
+            assert a[3][k] == r[3][k], k
+        for u, v in zip(a[:3], r[:3]):
+            assert torch.equal(u, v)
+
+
 def test_packed_groups_are_compact_and_cover_targets(gz):
     case = tree_cases.plummer_small(20000, seed=13, eps=0.01)
     P, T = _device_tree(gz, case)
