@@ -52,6 +52,28 @@ def err_stats(e):
             "max": float(np.max(e))}
 
 
+PROFILE = os.path.join(ROOT, "profiles", "k_walk_fp32_learned_halo_disk_8m.md")
+
+
+def profiled_traffic(path, kernel_ms):
+    """HBM bytes per walk launch from the committed rocprofv3 counter summary (FETCH_SIZE and
+    WRITE_SIZE scaled by a plain copy's readings), when that summary's kernel time matches this
+    run's within 15% (otherwise the profile describes another build and None is returned)."""
+    if not os.path.exists(path):
+        return None
+    vals = {}
+    with open(path) as f:
+        for line in f:
+            if line.startswith("traffic_bytes_per_launch:") or line.startswith("kernel_avg_ms:"):
+                k, v = line.split(":", 1)
+                vals[k.strip()] = float(v.strip())
+    if "traffic_bytes_per_launch" not in vals or "kernel_avg_ms" not in vals:
+        return None
+    if abs(vals["kernel_avg_ms"] - kernel_ms) > 0.15 * kernel_ms:
+        return None
+    return vals["traffic_bytes_per_launch"]
+
+
 def time_walk(gz, T, P, targets, aold, steps, warmup, **kw):
     """Mean wall time per step (host clock around the calls, synchronised) and the walk kernel's
     mean HIP-event time over the same steps."""
@@ -195,6 +217,7 @@ def main():
         e = rel_err(tuple(v.cpu().numpy()[probes] for v in (ax, ay, az)), ex)
         r = {"group_size": gs, "ms_per_step": dt * 1e3, "kernel_ms": kms,
              "evals_per_s": nt / dt, "interactions_per_target": inter / nt,
+             "multipole_per_target": cnt["multipole"] / nt, "pairs_per_target": cnt["pairs"] / nt,
              "nodes_visited_per_target": cnt["nodes_visited"] / nt, "groups": cnt["groups"],
              "err": err_stats(e),
              "achieved_tflops": FLOPS_PER_INTERACTION * inter / (kms * 1e-3) / 1e12}
@@ -299,7 +322,9 @@ def main():
         },
         "roofline": {"bound": "mfma", "achieved": head["achieved_tflops"],
                      "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
-                     "frac": head["achieved_tflops"] / PEAK_FP32_TFLOPS, "traffic": None},
+                     "frac": head["achieved_tflops"] / PEAK_FP32_TFLOPS,
+                     "traffic": (profiled_traffic(PROFILE, head["kernel_ms"])
+                                 if "learned_fp32" in results else None)},
         "cpu_baseline": None if cpu is None else {k: cpu[k] for k in
                                                   ("value", "unit", "cores", "kind", "sample")},
         "walks": results,

@@ -111,7 +111,7 @@ gz_status walk(gz_ctx* ctx, const gz_tree* tree, const gz_particles* p, const ui
     int* gstart = reinterpret_cast<int*>(b + o_gstart);
     long long* misc = reinterpret_cast<long long*>(b + o_misc);
     uint32_t* ngroups_dev = reinterpret_cast<uint32_t*>(misc + 4);
-    int* queue = reinterpret_cast<int*>(misc + 5);
+    int* queue = reinterpret_cast<int*>(misc + 9); // eight ints: misc[9..12]
     int* status = reinterpret_cast<int*>(misc + 6);
     int* n_retry = reinterpret_cast<int*>(misc + 7);
     int* retry_list = reinterpret_cast<int*>(b + o_retry);
@@ -643,7 +643,7 @@ gz_status gz_walk_plan_launch(gz_ctx* ctx, gz_walk_plan* P, double theta, double
     a.nclusters = tree->n_clusters;
     a.persistent = P->persistent;
     a.leaf_always_open = 0;
-    a.queue = reinterpret_cast<int*>(P->misc + 5);
+    a.queue = reinterpret_cast<int*>(P->misc + 9); // eight ints: misc[9..12]
     a.status = reinterpret_cast<int*>(P->misc + 6);
     a.counts = P->misc;
     a.n_retry = reinterpret_cast<int*>(P->misc + 7);

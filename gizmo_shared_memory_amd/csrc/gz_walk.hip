@@ -31,6 +31,7 @@ namespace gz {
 
 constexpr int kWalkStack = 1024;  // node ids per wave in LDS
 constexpr int kSpillKeep = 256;   // LDS entries kept when the bottom of the stack is spilled
+constexpr int kXcdChunk = 64;     // groups per chunk of the persistent launch's XCD interleave
 constexpr int kMonoF = 160;       // monopole entries per wave, FP32 (16 B each)
 constexpr int kPairF = 128;       // particle entries per wave, FP32 (20 B each)
 constexpr int kMonoD = 96;        // monopole entries per wave, FP64 (32 B each)
@@ -317,10 +318,31 @@ __global__ __launch_bounds__(64, 4) void k_walk(WalkArgs a) {
             if (k >= *a.n_retry) break;
             g = a.retry_list[k];
         } else if (a.persistent) {
-            int v = 0;
-            if (lane == 0) v = atomicAdd(a.queue, 1);
+            // Eight queues, one per XCD (block b runs on XCD b % 8). The Morton-ordered groups
+            // are cut into chunks of kXcdChunk, dealt to the XCDs in turn, so that the waves of
+            // one XCD walk compact runs of neighbouring groups (their L2 then holds the nodes
+            // those groups share) while all XCDs sweep the Morton order at the same pace. A
+            // wave whose queue is empty helps the others. Every group is handed out exactly
+            // once; a wave stops when all eight queues are empty.
+            int v = -1;
+            if (lane == 0) {
+                const int x0 = blockIdx.x & 7;
+                constexpr int C = kXcdChunk;
+                const int nch = (a.ngroups + C - 1) / C; // chunks; chunk k belongs to XCD k % 8
+                for (int k = 0; k < 8 && v < 0; ++k) {
+                    const int x = (x0 + k) & 7;
+                    const int myc = (nch > x) ? (nch - x + 7) / 8 : 0; // chunks of XCD x
+                    const int n = myc * C;
+                    const int c = atomicAdd(&a.queue[x], 1);
+                    if (c < n) {
+                        const int gg = ((c / C) * 8 + x) * C + (c % C);
+                        if (gg < a.ngroups) v = gg;
+                        else k = -1; // the tail of the last chunk: claim again (the loop ends)
+                    }
+                }
+            }
             g = __shfl(v, 0);
-            if (g >= a.ngroups) break;
+            if (g < 0) break;
         } else {
             // Block b runs on XCD b % 8; give each XCD one contiguous share of the Morton-ordered
             // groups so that its L2 serves one compact region of the tree.
