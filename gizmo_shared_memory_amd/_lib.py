@@ -49,7 +49,8 @@ class gz_particles(C.Structure):
 class gz_build_opts(C.Structure):
     _fields_ = [("randomize_seed", C.c_longlong), ("shift", C.c_double * 3),
                 ("max_nodes", C.c_int), ("cluster_start", C.POINTER(C.c_size_t)),
-                ("n_clusters", C.c_int)]
+                ("n_clusters", C.c_int), ("vx", C.c_void_p), ("vy", C.c_void_p),
+                ("vz", C.c_void_p)]
 
 
 class gz_build_times(C.Structure):
@@ -92,6 +93,14 @@ _PROTOS = {
     "gz_tree_device_arrays": (C.c_int, [C.c_void_p, C.POINTER(C.c_void_p),
                                         C.POINTER(C.c_void_p), C.POINTER(C.c_void_p)]),
     "gz_tree_update_sources": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(gz_particles)]),
+    "gz_tree_kick_nodes": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t, C.c_void_p,
+                                     C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                     C.c_void_p, C.c_double]),
+    "gz_tree_refresh_walk_nodes": (C.c_int, [C.c_void_p, C.c_void_p, C.c_double]),
+    "gz_tree_raise_soft": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t,
+                                     C.c_void_p]),
+    "gz_tree_download_reuse": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                         C.c_void_p]),
     "gz_tree_accel_grouped": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(gz_particles),
                                         C.c_void_p, C.c_size_t, C.c_double, C.c_double, C.c_int,
                                         C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,

@@ -425,6 +425,89 @@ gz_status gz_tree_download_nodes(const gz_tree* tree, int* first, int* next, int
     return GZ_OK;
 }
 
+gz_status gz_tree_kick_nodes(gz_ctx* ctx, gz_tree* tree, const uint32_t* active, size_t n_active,
+                             const double* vx, const double* vy, const double* vz,
+                             const double* m, double* vlast_x, double* vlast_y, double* vlast_z,
+                             double tau) {
+    gz::clear_error();
+    GZ_TRY(check_ctx(ctx));
+    if (!tree) return fail(GZ_ERR_INVALID, "gz_tree_kick_nodes: null tree");
+    if (n_active == 0) return GZ_OK;
+    if (!active || !vx || !vy || !vz)
+        return fail(GZ_ERR_INVALID, "gz_tree_kick_nodes: null index or velocity array");
+    const bool vl = vlast_x || vlast_y || vlast_z;
+    if (vl && !(vlast_x && vlast_y && vlast_z))
+        return fail(GZ_ERR_INVALID, "gz_tree_kick_nodes: v_last needs all three components");
+    if (vl && !m) return fail(GZ_ERR_INVALID, "gz_tree_kick_nodes: the momentum feed needs m");
+    hipStream_t st = ctx->stream;
+    gz::DevMem& sm = ctx->scratch[8];
+    GZ_TRY(sm.reserve(256));
+    int* status = sm.as<int>();
+    GZ_TRY(gz::fill_zero_u32(reinterpret_cast<uint32_t*>(status), 1, st));
+    GZ_TRY(gz::launch_kick_nodes(tree, active, n_active, vx, vy, vz, m, vlast_x, vlast_y, vlast_z,
+                                 tau, status, st));
+    int h = 0;
+    GZ_HIP(hipMemcpyAsync(&h, status, sizeof h, hipMemcpyDeviceToHost, st));
+    GZ_HIP(hipStreamSynchronize(st));
+    if (h & 1)
+        return fail(GZ_ERR_INVALID, "gz_tree_kick_nodes: an index is outside the particle set");
+    return GZ_OK;
+}
+
+gz_status gz_tree_refresh_walk_nodes(gz_ctx* ctx, gz_tree* tree, double tau) {
+    gz::clear_error();
+    GZ_TRY(check_ctx(ctx));
+    if (!tree) return fail(GZ_ERR_INVALID, "gz_tree_refresh_walk_nodes: null tree");
+    return gz::launch_refresh(tree, tau, ctx->stream);
+}
+
+gz_status gz_tree_raise_soft(gz_ctx* ctx, gz_tree* tree, const uint32_t* idx, size_t n,
+                             const double* soft) {
+    gz::clear_error();
+    GZ_TRY(check_ctx(ctx));
+    if (!tree) return fail(GZ_ERR_INVALID, "gz_tree_raise_soft: null tree");
+    if (n == 0) return GZ_OK;
+    if (!idx || !soft) return fail(GZ_ERR_INVALID, "gz_tree_raise_soft: null array");
+    hipStream_t st = ctx->stream;
+    gz::DevMem& sm = ctx->scratch[8];
+    GZ_TRY(sm.reserve(256));
+    int* status = sm.as<int>();
+    GZ_TRY(gz::fill_zero_u32(reinterpret_cast<uint32_t*>(status), 1, st));
+    GZ_TRY(gz::launch_raise_soft(tree, idx, n, soft, status, st));
+    int h = 0;
+    GZ_HIP(hipMemcpyAsync(&h, status, sizeof h, hipMemcpyDeviceToHost, st));
+    GZ_HIP(hipStreamSynchronize(st));
+    if (h & 1) return fail(GZ_ERR_INVALID, "gz_tree_raise_soft: an index is outside the particles");
+    return GZ_OK;
+}
+
+gz_status gz_tree_download_reuse(const gz_tree* tree, float* vmax, double* vcom, double* dp,
+                                 double* dq) {
+    if (!tree) return fail(GZ_ERR_INVALID, "gz_tree_download_reuse: null tree");
+    GZ_TRY(check_ctx(tree->ctx));
+    hipStream_t st = tree->ctx->stream;
+    const size_t nn = (size_t)tree->nnodes;
+    if (nn == 0) return GZ_OK;
+    if (vmax) GZ_HIP(hipMemcpyAsync(vmax, tree->vmax, nn * sizeof(float), hipMemcpyDeviceToHost, st));
+    std::vector<double> h(3 * nn);
+    const double* srcs[3] = {tree->vcom, tree->dp, tree->dq};
+    double* dsts[3] = {vcom, dp, dq};
+    for (int k = 0; k < 3; ++k) {
+        if (!dsts[k]) continue;
+        if (!srcs[k]) {
+            std::memset(dsts[k], 0, 3 * nn * sizeof(double));
+            continue;
+        }
+        GZ_HIP(hipMemcpyAsync(h.data(), srcs[k], 3 * nn * sizeof(double), hipMemcpyDeviceToHost,
+                              st));
+        GZ_HIP(hipStreamSynchronize(st));
+        for (size_t i = 0; i < nn; ++i)
+            for (int d = 0; d < 3; ++d) dsts[k][3 * i + d] = h[d * nn + i];
+    }
+    GZ_HIP(hipStreamSynchronize(st));
+    return GZ_OK;
+}
+
 gz_status gz_tree_download_particles(const gz_tree* tree, uint32_t* orderbuf, uint32_t* rank,
                                      int* leaf_of, uint64_t* skey) {
     if (!tree) return fail(GZ_ERR_INVALID, "gz_tree_download_particles: null tree");

@@ -360,7 +360,8 @@ __global__ __launch_bounds__(kThreads) void k_moments(
     const double* __restrict__ gcz, const double* __restrict__ size,
     const double4* __restrict__ src, const double* __restrict__ src_soft,
     const uint32_t* __restrict__ orderbuf, int* __restrict__ leaf_of,
-    int* __restrict__ slot_leaf, double* __restrict__ delta_out, double* __restrict__ soft_out, int* __restrict__ nleaf) {
+    int* __restrict__ slot_leaf, NodeVel nv, double* __restrict__ delta_out,
+    double* __restrict__ soft_out, int* __restrict__ nleaf) {
     const int i = blockIdx.x * kThreads + threadIdx.x;
     const bool live = i < count;
     const int node = lb + (live ? i : 0);
@@ -371,6 +372,7 @@ __global__ __launch_bounds__(kThreads) void k_moments(
     const int first = wn[node].first;
     const int lo = nlo[node], hi = nhi[node];
     double M = 0, sx = 0, sy = 0, sz = 0, smax = 0;
+    double vmax = 0, pvx = 0, pvy = 0, pvz = 0; // with velocities: max |v| and momentum
     if (nk == 0) {
         for (int k = lo; k < hi; ++k) {
             const double4 p = src[k];
@@ -381,6 +383,15 @@ __global__ __launch_bounds__(kThreads) void k_moments(
             sz += mm * p.z;
             const double s = src_soft[k];
             if (s > smax) smax = s;
+            if (nv.vx) { // tree.cc:161-166
+                const uint32_t q = orderbuf[k];
+                const double ux = nv.vx[q], uy = nv.vy[q], uz = nv.vz[q];
+                const double v = sqrt(ux * ux + uy * uy + uz * uz);
+                if (v > vmax) vmax = v;
+                pvx += mm * ux;
+                pvy += mm * uy;
+                pvz += mm * uz;
+            }
             leaf_of[orderbuf[k]] = node;
             slot_leaf[k] = node;
         }
@@ -393,6 +404,12 @@ __global__ __launch_bounds__(kThreads) void k_moments(
             sz += mm * wn[c].cz;
             const double s = soft_out[c];
             if (s > smax) smax = s;
+            if (nv.vx) { // tree.cc:177-181
+                if (nv.vmax[c] > vmax) vmax = nv.vmax[c];
+                pvx += mm * nv.vcom[c];
+                pvy += mm * nv.vcom[nv.nn + c];
+                pvz += mm * nv.vcom[2 * (size_t)nv.nn + c];
+            }
         }
     }
     if (M > 0) {
@@ -418,6 +435,25 @@ __global__ __launch_bounds__(kThreads) void k_moments(
     wn[node] = w;
     delta_out[node] = delta;
     soft_out[node] = smax;
+    nv.vmax[node] = (float)vmax; // 0 without velocities
+    if (nv.vx) {
+        // centre-of-mass velocity (tree.cc:188-190), the build moments that refreshes start
+        // from, and zeroed incremental moments
+        const size_t nn = (size_t)nv.nn;
+        nv.vcom[node] = (M > 0) ? pvx / M : 0.0;
+        nv.vcom[nn + node] = (M > 0) ? pvy / M : 0.0;
+        nv.vcom[2 * nn + node] = (M > 0) ? pvz / M : 0.0;
+        nv.m0[node] = M;
+        nv.m0[nn + node] = sx;
+        nv.m0[2 * nn + node] = sy;
+        nv.m0[3 * nn + node] = sz;
+        for (int d = 0; d < 3; ++d) {
+            nv.dp[d * nn + node] = 0.0;
+            nv.dq[d * nn + node] = 0.0;
+        }
+        nv.dmass[node] = 0.0;
+    }
+    nv.xpad[node] = 0.0f;
 }
 
 // ---- host driver ---------------------------------------------------------------------------------
@@ -455,10 +491,38 @@ static gz_status grow_nodes(gz_tree* t, int used, int cap, hipStream_t st) {
     return GZ_OK;
 }
 
+// The reuse arrays of a tree with nn nodes, in one allocation: vmax and xpad (floats), and with
+// velocities vcom (3 nn), the build moments (4 nn: mass, then COM), dp and dq (3 nn each) and
+// dmass (nn).
+static gz_status alloc_reuse(gz_tree* t, int nn, bool vel) {
+    if (t->reuse_mem) {
+        (void)hipFree(t->reuse_mem);
+        t->reuse_mem = nullptr;
+    }
+    const size_t n = (size_t)std::max(nn, 1);
+    const size_t fl = (2 * n * sizeof(float) + 255) & ~size_t(255);
+    const size_t db = vel ? 14 * n * sizeof(double) : 0;
+    GZ_TRY(dalloc(&t->reuse_mem, fl + db));
+    uint8_t* b = static_cast<uint8_t*>(t->reuse_mem);
+    t->vmax = reinterpret_cast<float*>(b);
+    t->xpad = t->vmax + n;
+    t->has_vel = vel;
+    t->vcom = t->m0 = t->dp = t->dq = t->dmass = nullptr;
+    if (vel) {
+        double* d = reinterpret_cast<double*>(b + fl);
+        t->vcom = d;
+        t->m0 = d + 3 * n;
+        t->dp = d + 7 * n;
+        t->dq = d + 10 * n;
+        t->dmass = d + 13 * n;
+    }
+    return GZ_OK;
+}
+
 void tree_release(gz_tree* t) {
     void* ptrs[] = {t->skey, t->orderbuf, t->rank, t->leaf_of, t->slot_leaf, t->src, t->src_soft,
                     t->wn, t->next, t->parent, t->nlo, t->nhi, t->gcx, t->gcy, t->gcz, t->size,
-                    t->delta, t->soft, t->level_begin_dev, t->cluster_start_dev};
+                    t->delta, t->soft, t->level_begin_dev, t->cluster_start_dev, t->reuse_mem};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
 }
@@ -690,6 +754,23 @@ gz_status build_tree(gz_ctx* ctx, const gz_particles* p, const gz_build_opts* op
     }
     GZ_HIP(hipEventRecord(ev[5], st));
 
+    // --- reuse state (tree.h:250-317): vmax and xpad always, the velocity moments with velocities
+    NodeVel nv{};
+    GZ_TRY(alloc_reuse(t, nnodes, opts && opts->vx && opts->vy && opts->vz));
+    nv.nn = nnodes;
+    nv.vmax = t->vmax;
+    nv.xpad = t->xpad;
+    if (t->has_vel) {
+        nv.vx = opts->vx;
+        nv.vy = opts->vy;
+        nv.vz = opts->vz;
+        nv.vcom = t->vcom;
+        nv.m0 = t->m0;
+        nv.dp = t->dp;
+        nv.dq = t->dq;
+        nv.dmass = t->dmass;
+    }
+
     // --- moments + pack, deepest level first
     int* dleaf = reinterpret_cast<int*>(droot + 12);
     k_fill_u32<<<1, kThreads, 0, st>>>(reinterpret_cast<uint32_t*>(dleaf), 1, 0u);
@@ -699,8 +780,8 @@ gz_status build_tree(gz_ctx* ctx, const gz_particles* p, const gz_build_opts* op
         k_moments<<<grid_for(le - lb), kThreads, 0, st>>>(lb, le - lb, t->wn, t->nlo, t->nhi,
                                                           t->gcx, t->gcy, t->gcz, t->size, t->src,
                                                           t->src_soft, t->orderbuf, t->leaf_of,
-                                                          t->slot_leaf,
-                                                          t->delta, t->soft, dleaf);
+                                                          t->slot_leaf, nv, t->delta, t->soft,
+                                                          dleaf);
         GZ_HIP(hipGetLastError());
     }
     GZ_HIP(hipEventRecord(ev[6], st));

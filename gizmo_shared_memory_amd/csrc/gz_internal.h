@@ -42,6 +42,15 @@ struct GroupInfo {
 };
 
 // Everything the walk kernel reads. Passed by value.
+// Per-node reuse state written by the build (tree.h:250-317); vectors are SoA with stride nn.
+struct NodeVel {
+    int nn;
+    const double *vx, *vy, *vz; // particle velocities, or NULL (vmax = 0, no moments)
+    float* vmax;
+    float* xpad;
+    double *vcom, *m0, *dp, *dq, *dmass;
+};
+
 struct WalkArgs {
     const WNode* wn;
     const double4* src;     // (x, y, z, m) per sorted slot
@@ -180,6 +189,14 @@ struct gz_ctx {
 };
 
 namespace gz {
+// Tree reuse (gz_reuse.hip). status bit 0: an index outside the particle set.
+gz_status launch_kick_nodes(gz_tree* t, const uint32_t* active, size_t na, const double* vx,
+                            const double* vy, const double* vz, const double* m, double* vlx,
+                            double* vly, double* vlz, double tau, int* status, hipStream_t st);
+gz_status launch_refresh(gz_tree* t, double tau, hipStream_t st);
+gz_status launch_raise_soft(gz_tree* t, const uint32_t* idx, size_t n_idx, const double* soft,
+                            int* status, hipStream_t st);
+
 // The spill slices of ctx (allocating them on first use) and their size per wave.
 gz_status ctx_spill(gz_ctx* ctx, int** spill, int* cap);
 } // namespace gz
@@ -256,6 +273,15 @@ struct gz_tree {
     double* size = nullptr;
     double* delta = nullptr;
     double* soft = nullptr; // FP64 max softening
+
+    // reuse between rebuilds (tree.h:250-317), one allocation: vmax and xpad always; with
+    // build-time velocities the centre-of-mass velocity, the build moments (mass, COM) that
+    // refreshes start from, and the incremental moments. Vectors are SoA with stride nnodes.
+    void* reuse_mem = nullptr;
+    float* vmax = nullptr;
+    float* xpad = nullptr;
+    bool has_vel = false;
+    double *vcom = nullptr, *m0 = nullptr, *dp = nullptr, *dq = nullptr, *dmass = nullptr;
 };
 
 namespace gz {

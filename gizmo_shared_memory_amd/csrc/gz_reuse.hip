@@ -1,0 +1,158 @@
+// Reusing a tree across steps instead of rebuilding it (shmem/tree.cc:211-319, tree.h:250-331):
+// the kick feed that raises each node's speed bound and accumulates its momentum change
+// (mfm.cc:3798-3811: Tree::raise_vmax and Tree::kick_node), the refresh of the walk nodes from the
+// incremental moments (Tree::refresh_walk_nodes), and raising a node's softening bound
+// (Tree::raise_soft).
+//
+// The reference climbs the parent chain of each active particle's leaf with atomics on the CPU;
+// here one thread per particle does the same climb with device atomics. Maxima (vmax, softening)
+// come out the same in any order; the momentum sums are FP64 atomic additions, whose rounding
+// depends on the order, as the reference's OpenMP atomics do.
+#include <cmath>
+
+#include "gz_internal.h"
+
+namespace gz {
+
+__global__ __launch_bounds__(256) void k_kick_nodes(
+    const uint32_t* __restrict__ active, size_t na, size_t n, const int* __restrict__ leaf_of,
+    const int* __restrict__ parent, const double* __restrict__ vx, const double* __restrict__ vy,
+    const double* __restrict__ vz, const double* __restrict__ m, double* __restrict__ vlx,
+    double* __restrict__ vly, double* __restrict__ vlz, double tau, uint32_t* __restrict__ vmax,
+    double* __restrict__ dp, double* __restrict__ dq, size_t nn, int* __restrict__ status) {
+    const size_t k = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (k >= na) return;
+    const uint32_t i = active[k];
+    if (i >= n) {
+        atomicOr(status, 1);
+        return;
+    }
+    const int leaf = leaf_of[i];
+    if (leaf < 0) return;
+    const double ux = vx[i], uy = vy[i], uz = vz[i];
+    // raise_vmax (tree.cc:276-285): a non-negative float's bits order as its value, and a node's
+    // bound covers its children's, so the climb stops at the first ancestor already covering it
+    const float speed = (float)sqrt(ux * ux + uy * uy + uz * uz);
+    const uint32_t want = __float_as_uint(speed);
+    for (int no = leaf; no >= 0; no = parent[no])
+        if (atomicMax(&vmax[no], want) >= want) break;
+    if (vlx) {
+        if (dp) { // kick_node (tree.cc:211-227)
+            const double px = (ux - vlx[i]) * m[i], py = (uy - vly[i]) * m[i],
+                         pz = (uz - vlz[i]) * m[i];
+            const double qx = px * tau, qy = py * tau, qz = pz * tau;
+            for (int no = leaf; no >= 0; no = parent[no]) {
+                atomicAdd(&dp[no], px);
+                atomicAdd(&dp[nn + no], py);
+                atomicAdd(&dp[2 * nn + no], pz);
+                atomicAdd(&dq[no], qx);
+                atomicAdd(&dq[nn + no], qy);
+                atomicAdd(&dq[2 * nn + no], qz);
+            }
+        }
+        vlx[i] = ux;
+        vly[i] = uy;
+        vlz[i] = uz;
+    }
+}
+
+// refresh_walk_nodes (tree.cc:260-274) with node_mass and node_com (tree.h:318-320,
+// tree.cc:230-237), expression for expression.
+__global__ __launch_bounds__(256) void k_refresh(int nn, WNode* __restrict__ wn,
+                                                 const double* __restrict__ size,
+                                                 const double* __restrict__ delta,
+                                                 const float* __restrict__ vmax,
+                                                 const float* __restrict__ xpad,
+                                                 const double* __restrict__ vcom,
+                                                 const double* __restrict__ m0,
+                                                 const double* __restrict__ dp,
+                                                 const double* __restrict__ dq,
+                                                 const double* __restrict__ dmass, double tau) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= nn) return;
+    const size_t n = (size_t)nn;
+    WNode w = wn[i];
+    if (m0) {
+        const double mass = m0[i];
+        const double M = mass + dmass[i];
+        const double cx = m0[n + i], cy = m0[2 * n + i], cz = m0[3 * n + i];
+        if (M > 0) {
+            w.cx = (mass * cx + (mass * vcom[i] + dp[i]) * tau - dq[i]) / M;
+            w.cy = (mass * cy + (mass * vcom[n + i] + dp[n + i]) * tau - dq[n + i]) / M;
+            w.cz = (mass * cz + (mass * vcom[2 * n + i] + dp[2 * n + i]) * tau - dq[2 * n + i]) / M;
+        } else {
+            w.cx = cx;
+            w.cy = cy;
+            w.cz = cz;
+        }
+        w.mass = M;
+    }
+    // every particle may have moved vmax*tau, and the COM with them
+    const double grow = 2.0 * ((double)vmax[i] * tau + (double)xpad[i]);
+    w.s = size[i] + delta[i] + grow;
+    w.len = (float)(size[i] + grow);
+    wn[i] = w;
+}
+
+// The softening bound rounded up to float, so the node's bound stays one-sided (tree.cc:289-292).
+__device__ __forceinline__ float soft_as_bound(double h) {
+    const float f = (float)h;
+    return ((double)f < h) ? nextafterf(f, 1e30f) : f;
+}
+
+// raise_soft (tree.cc:311-319): the float bound of the walk node, and the FP64 node softening.
+__global__ __launch_bounds__(256) void k_raise_soft(const uint32_t* __restrict__ idx, size_t n_idx,
+                                                    size_t n, const int* __restrict__ leaf_of,
+                                                    const int* __restrict__ parent,
+                                                    const double* __restrict__ soft,
+                                                    WNode* __restrict__ wn,
+                                                    double* __restrict__ soft64,
+                                                    int* __restrict__ status) {
+    const size_t k = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (k >= n_idx) return;
+    const uint32_t i = idx[k];
+    if (i >= n) {
+        atomicOr(status, 1);
+        return;
+    }
+    const int leaf = leaf_of[i];
+    if (leaf < 0) return;
+    const double h = soft[i];
+    const uint32_t want = __float_as_uint(soft_as_bound(h));
+    const unsigned long long want64 = (unsigned long long)__double_as_longlong(h);
+    for (int no = leaf; no >= 0; no = parent[no]) {
+        if (atomicMax(reinterpret_cast<uint32_t*>(&wn[no].soft), want) >= want) break;
+        atomicMax(reinterpret_cast<unsigned long long*>(&soft64[no]), want64);
+    }
+}
+
+gz_status launch_kick_nodes(gz_tree* t, const uint32_t* active, size_t na, const double* vx,
+                            const double* vy, const double* vz, const double* m, double* vlx,
+                            double* vly, double* vlz, double tau, int* status, hipStream_t st) {
+    if (na == 0 || t->nnodes == 0) return GZ_OK;
+    k_kick_nodes<<<(unsigned)((na + 255) / 256), 256, 0, st>>>(
+        active, na, t->n, t->leaf_of, t->parent, vx, vy, vz, m, vlx, vly, vlz, tau,
+        reinterpret_cast<uint32_t*>(t->vmax), t->dp, t->dq, (size_t)t->nnodes, status);
+    GZ_HIP(hipGetLastError());
+    return GZ_OK;
+}
+
+gz_status launch_refresh(gz_tree* t, double tau, hipStream_t st) {
+    if (t->nnodes == 0) return GZ_OK;
+    k_refresh<<<(unsigned)((t->nnodes + 255) / 256), 256, 0, st>>>(
+        t->nnodes, t->wn, t->size, t->delta, t->vmax, t->xpad, t->vcom, t->m0, t->dp, t->dq,
+        t->dmass, tau);
+    GZ_HIP(hipGetLastError());
+    return GZ_OK;
+}
+
+gz_status launch_raise_soft(gz_tree* t, const uint32_t* idx, size_t n_idx, const double* soft,
+                            int* status, hipStream_t st) {
+    if (n_idx == 0 || t->nnodes == 0) return GZ_OK;
+    k_raise_soft<<<(unsigned)((n_idx + 255) / 256), 256, 0, st>>>(
+        idx, n_idx, t->n, t->leaf_of, t->parent, soft, t->wn, t->soft, status);
+    GZ_HIP(hipGetLastError());
+    return GZ_OK;
+}
+
+} // namespace gz

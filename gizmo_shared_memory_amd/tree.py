@@ -183,6 +183,52 @@ class Tree:
         out.update(flts)
         return out
 
+    def kick_nodes(self, active, v, m=None, vlast=None, tau: float = 0.0):
+        """The kick feed (mfm.cc:3798-3811): raise_vmax for every active particle, and with
+        vlast (three float64 device tensors, updated in place) kick_node with m (v - vlast)."""
+        act = _dev_u32(active, self.P.device)
+        vt = tuple(_dev_f64(a, self.P.device) for a in v)
+        mt = self.P.m if m is None else _dev_f64(m, self.P.device)
+        vl = (None, None, None)
+        if vlast is not None:
+            for a in vlast:
+                assert a.dtype == torch.float64 and a.is_contiguous() and a.device == self.P.device
+            vl = tuple(a.data_ptr() for a in vlast)
+        self.ctx.enter()
+        L.check(L.load().gz_tree_kick_nodes(self.ctx.handle, self.handle, _ptr(act),
+                                            int(act.shape[0]), *(a.data_ptr() for a in vt),
+                                            mt.data_ptr(), *vl, float(tau)),
+                "gz_tree_kick_nodes")
+        self.ctx.synchronize()
+
+    def refresh_walk_nodes(self, tau: float):
+        """Tree::refresh_walk_nodes (tree.cc:260-274)."""
+        self.ctx.enter()
+        L.check(L.load().gz_tree_refresh_walk_nodes(self.ctx.handle, self.handle, float(tau)),
+                "gz_tree_refresh_walk_nodes")
+        self.ctx.synchronize()
+
+    def raise_soft(self, idx, soft):
+        """Tree::raise_soft (tree.cc:311-319) for the particles idx; soft indexed by particle."""
+        ix = _dev_u32(idx, self.P.device)
+        st = _dev_f64(soft, self.P.device)
+        self.ctx.enter()
+        L.check(L.load().gz_tree_raise_soft(self.ctx.handle, self.handle, _ptr(ix),
+                                            int(ix.shape[0]), st.data_ptr()),
+                "gz_tree_raise_soft")
+        self.ctx.synchronize()
+
+    def download_reuse(self) -> dict:
+        """Host copies of vmax (nnodes) and vcom, dp, dq (nnodes x 3)."""
+        nn = self.nnodes
+        out = dict(vmax=np.zeros(nn, np.float32), vcom=np.zeros((nn, 3)), dp=np.zeros((nn, 3)),
+                   dq=np.zeros((nn, 3)))
+        L.check(L.load().gz_tree_download_reuse(self.handle,
+                                                *(out[k].ctypes.data for k in
+                                                  ("vmax", "vcom", "dp", "dq"))),
+                "gz_tree_download_reuse")
+        return out
+
     def update_sources(self, P: Particles | None = None):
         """Re-read source positions/masses/softenings into the tree's sorted copy."""
         P = P or self.P
@@ -193,21 +239,27 @@ class Tree:
 
 
 def build(P: Particles, randomize_seed: int = -1, shift=(0.0, 0.0, 0.0), max_nodes: int = 0,
-          clusters=None, ctx: Context | None = None) -> Tree:
+          clusters=None, vel=None, ctx: Context | None = None) -> Tree:
     """shmem::build (tree.h:384-385, tree.cc:350-526). clusters: particle offsets (length
-    n_clusters + 1) of independent clusters, each given its own tree (a forest)."""
+    n_clusters + 1) of independent clusters, each given its own tree (a forest). vel: particle
+    velocities (vx, vy, vz), kept as the reference does with want_vcom for reusing the tree."""
     ctx = ctx or Context.get(P.device.index or 0)
     cs, nc = None, 0
     if clusters is not None:
         cl = [int(v) for v in clusters]
         cs = (C.c_size_t * len(cl))(*cl)
         nc = len(cl) - 1
-    opts = L.gz_build_opts(int(randomize_seed), (C.c_double * 3)(*shift), int(max_nodes), cs, nc)
+    vt = (None, None, None) if vel is None else tuple(_dev_f64(v, P.device) for v in vel)
+    opts = L.gz_build_opts(int(randomize_seed), (C.c_double * 3)(*shift), int(max_nodes), cs, nc,
+                           *(None if v is None else v.data_ptr() for v in vt))
     h = C.c_void_p()
     ctx.enter()
     L.check(L.load().gz_tree_build(ctx.handle, P.cstruct(), C.byref(opts), C.byref(h)),
             "gz_tree_build")
-    return Tree(h, ctx, P)
+    T = Tree(h, ctx, P)
+    T._vel = vt  # the build reads them on the stream; keep them alive until it has
+    ctx.synchronize()
+    return T
 
 
 def _outputs(n, device):

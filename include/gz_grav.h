@@ -76,6 +76,10 @@ typedef struct {
      * walk of a target sees only its own cluster. n_clusters <= 1: one tree. */
     const size_t* cluster_start;
     int n_clusters;
+    /* Particle velocities (device, length n), or NULL: the build then also keeps what reusing
+     * the tree across steps needs (tree.h:384-385 with want_vcom): per-node max |v| and
+     * centre-of-mass velocity, and zeroed incremental moments (rebuild_tree, mfm.cc:1408-1413). */
+    const double *vx, *vy, *vz;
 } gz_build_opts;
 
 /* Stage timings in ms, the reference's BuildTimes (tree.h:371) measured with HIP events. */
@@ -114,8 +118,38 @@ gz_status gz_tree_device_arrays(const gz_tree* tree, const uint32_t** orderbuf,
 /* Re-read source positions, masses and softenings from p into the tree's Morton-ordered copy,
  * for callers that move particles between a build and a walk. The reference's walk reads source
  * particles from the live arrays (shmem/tree.cc:787-788); this library's walk reads the copy made
- * at build time, so a caller that drifts particles calls this before walking. */
+ * at build time, for sources and targets alike, so a caller that drifts particles calls this
+ * before walking. */
 gz_status gz_tree_update_sources(gz_ctx* ctx, gz_tree* tree, const gz_particles* p);
+
+/* ---- reusing a tree across steps (tree.h:250-331) -------------------------------------------
+ * Between rebuilds the reference keeps the tree and feeds it each step's velocity changes; before
+ * a walk it rewrites the walk nodes from the incremental moments. A caller that drifts particles
+ * also calls gz_tree_update_sources before walking (the walk reads the tree's sorted copy). */
+
+/* The kick feed of mfm.cc:3798-3811 for the particles in active (device, n_active): raise every
+ * ancestor's speed bound to |v| (Tree::raise_vmax, tree.cc:276-285) and, when v_last is given,
+ * add m (v - v_last) and its product with tau to every ancestor's incremental momentum and first
+ * moment (Tree::kick_node, tree.cc:211-227), then set v_last = v. The momentum feed needs a tree
+ * built with velocities; without them only the speed bound is raised. v_last may be NULL. */
+gz_status gz_tree_kick_nodes(gz_ctx* ctx, gz_tree* tree, const uint32_t* active, size_t n_active,
+                             const double* vx, const double* vy, const double* vz,
+                             const double* m, double* vlast_x, double* vlast_y, double* vlast_z,
+                             double tau);
+/* Tree::refresh_walk_nodes (tree.cc:260-274): every walk node's mass and centre of mass at tau
+ * (time since the build) from the incremental moments, and its opening radius and side widened
+ * by 2 (vmax tau + xpad). The identity on mass and centre of mass for a tree built without
+ * velocities. */
+gz_status gz_tree_refresh_walk_nodes(gz_ctx* ctx, gz_tree* tree, double tau);
+/* Tree::raise_soft (tree.cc:311-319) for the particles idx (device, n): raise every ancestor's
+ * softening bound to soft[i] (soft: device, indexed by particle). */
+gz_status gz_tree_raise_soft(gz_ctx* ctx, gz_tree* tree, const uint32_t* idx, size_t n,
+                             const double* soft);
+/* The reuse state to HOST buffers (any may be NULL): vmax (nnodes floats), and vcom, dp, dq
+ * (3 x nnodes doubles each, x y z of node i at 3i..3i+2; zero for a tree built without
+ * velocities). The refreshed walk-node fields come from gz_tree_download_nodes. */
+gz_status gz_tree_download_reuse(const gz_tree* tree, float* vmax, double* vcom, double* dp,
+                                 double* dq);
 
 /* ---- walks -------------------------------------------------------------------------------- */
 typedef enum {

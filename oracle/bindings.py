@@ -89,6 +89,35 @@ class TreeArrays:
     leaf_of: np.ndarray
 
 
+@dataclass
+class ReuseState:
+    """Per-node state of a tree reused between rebuilds (tree.h:266-317) and its walk view."""
+    vmax: np.ndarray
+    vcom: np.ndarray
+    dp: np.ndarray
+    dq: np.ndarray
+    wcom: np.ndarray
+    wmass: np.ndarray
+    wn_s: np.ndarray
+    wn_soft: np.ndarray
+    wn_len: np.ndarray
+
+
+def _reuse_arrays(nn):
+    return ReuseState(np.zeros(nn, np.float32), np.zeros((nn, 3)), np.zeros((nn, 3)),
+                      np.zeros((nn, 3)), np.zeros((nn, 3)), np.zeros(nn), np.zeros(nn),
+                      np.zeros(nn, np.float32), np.zeros(nn, np.float32))
+
+
+def _opt3(v):
+    """Three float64 arrays the callee writes in place (or None)."""
+    if v is None:
+        return (None, None, None)
+    for a in v:
+        assert a.dtype == np.float64 and a.flags["C_CONTIGUOUS"], "v_last must be float64 arrays"
+    return tuple(_ptr(a, _dp) for a in v)
+
+
 class _Lib:
     prefix = ""
 
@@ -161,6 +190,13 @@ class Oracle(_Lib):
                                 C.c_double, C.c_double, _dp, _dp, _dp, _dp]
         L.gzo_accel_brute.argtypes = [C.POINTER(_GzoParticles), _up, C.c_size_t, C.c_double,
                                       _dp, _dp, _dp]
+        L.gzo_build_vel.restype = C.c_void_p
+        L.gzo_build_vel.argtypes = [C.POINTER(_GzoParticles), C.c_longlong, _dp, _dp, _dp]
+        L.gzo_kick_nodes.argtypes = [C.c_void_p, _up, C.c_size_t, _dp, _dp, _dp, _dp, _dp, _dp,
+                                     _dp, C.c_double]
+        L.gzo_refresh_walk_nodes.argtypes = [C.c_void_p, C.c_double]
+        L.gzo_raise_soft.argtypes = [C.c_void_p, _up, C.c_size_t, _dp]
+        L.gzo_tree_reuse.argtypes = [C.c_void_p, _fp, _dp, _dp, _dp, _dp, _dp]
 
     @staticmethod
     def _cp(P: HostParticles):
@@ -210,6 +246,34 @@ class Oracle(_Lib):
             return ax, ay, az, counts
         return ax, ay, az
 
+    def build_vel(self, P: HostParticles, v, seed: int = -1):
+        cp = self._cp(P)
+        v = tuple(_f64(a) for a in v)
+        t = self.lib.gzo_build_vel(C.byref(cp), seed, *(_ptr(a, _dp) for a in v))
+        return t
+
+    def kick_nodes(self, t, active, v, m, vlast, tau):
+        act = np.ascontiguousarray(active, dtype=np.uint32)
+        v = tuple(_f64(a) for a in v)
+        self.lib.gzo_kick_nodes(t, _ptr(act, _up), act.shape[0], *(_ptr(a, _dp) for a in v),
+                                _ptr(_f64(m), _dp), *_opt3(vlast), tau)
+
+    def refresh_walk_nodes(self, t, tau):
+        self.lib.gzo_refresh_walk_nodes(t, tau)
+
+    def raise_soft(self, t, idx, soft):
+        ix = np.ascontiguousarray(idx, dtype=np.uint32)
+        self.lib.gzo_raise_soft(t, _ptr(ix, _up), ix.shape[0], _ptr(_f64(soft), _dp))
+
+    def reuse(self, t) -> ReuseState:
+        nn = self._tree_arrays(t).nnodes
+        r = _reuse_arrays(nn)
+        self.lib.gzo_tree_reuse(t, _ptr(r.vmax, _fp), _ptr(r.vcom, _dp), _ptr(r.dp, _dp),
+                                _ptr(r.dq, _dp), _ptr(r.wcom, _dp), _ptr(r.wmass, _dp))
+        self.lib.gzo_tree_nodes(t, *([None] * 15), _ptr(r.wn_s, _dp), _ptr(r.wn_soft, _fp),
+                                _ptr(r.wn_len, _fp))
+        return r
+
     def accel(self, t, P: HostParticles, targets, theta, G=1.0, aold=None):
         tg = np.ascontiguousarray(targets, dtype=np.uint32)
         nt = tg.shape[0]
@@ -253,6 +317,13 @@ class Reference(_Lib):
         L.gzref_accel_brute.argtypes = [C.c_void_p, _up, C.c_size_t, C.c_double, _dp, _dp, _dp]
         L.gzref_audit.restype = C.c_longlong
         L.gzref_audit.argtypes = [C.c_void_p, C.c_void_p, C.c_char_p, C.c_size_t]
+        L.gzref_build_vel.restype = C.c_void_p
+        L.gzref_build_vel.argtypes = [C.c_void_p, C.c_longlong, _dp, _dp, _dp]
+        L.gzref_kick_nodes.argtypes = [C.c_void_p, _up, C.c_size_t, _dp, _dp, _dp, _dp, _dp, _dp,
+                                       _dp, C.c_double]
+        L.gzref_refresh_walk_nodes.argtypes = [C.c_void_p, C.c_double]
+        L.gzref_raise_soft.argtypes = [C.c_void_p, _up, C.c_size_t, _dp]
+        L.gzref_tree_reuse.argtypes = [C.c_void_p, _fp, _dp, _dp, _dp, _dp, _dp, _dp, _fp, _fp]
         L.gzref_max_threads.restype = C.c_int
         L.gzref_set_threads.argtypes = [C.c_int]
         self._handles = {}
@@ -269,6 +340,31 @@ class Reference(_Lib):
 
     def free(self, t):
         self.lib.gzref_tree_free(t)
+
+    def build_vel(self, rp, v, seed: int = -1):
+        v = tuple(_f64(a) for a in v)
+        return self.lib.gzref_build_vel(rp, seed, *(_ptr(a, _dp) for a in v))
+
+    def kick_nodes(self, t, active, v, m, vlast, tau):
+        act = np.ascontiguousarray(active, dtype=np.uint32)
+        v = tuple(_f64(a) for a in v)
+        self.lib.gzref_kick_nodes(t, _ptr(act, _up), act.shape[0], *(_ptr(a, _dp) for a in v),
+                                  _ptr(_f64(m), _dp), *_opt3(vlast), tau)
+
+    def refresh_walk_nodes(self, t, tau):
+        self.lib.gzref_refresh_walk_nodes(t, tau)
+
+    def raise_soft(self, t, idx, soft):
+        ix = np.ascontiguousarray(idx, dtype=np.uint32)
+        self.lib.gzref_raise_soft(t, _ptr(ix, _up), ix.shape[0], _ptr(_f64(soft), _dp))
+
+    def reuse(self, t) -> ReuseState:
+        nn = self._tree_arrays(t).nnodes
+        r = _reuse_arrays(nn)
+        self.lib.gzref_tree_reuse(t, _ptr(r.vmax, _fp), _ptr(r.vcom, _dp), _ptr(r.dp, _dp),
+                                  _ptr(r.dq, _dp), _ptr(r.wcom, _dp), _ptr(r.wmass, _dp),
+                                  _ptr(r.wn_s, _dp), _ptr(r.wn_soft, _fp), _ptr(r.wn_len, _fp))
+        return r
 
     def times(self, t):
         out = np.zeros(7)

@@ -131,9 +131,17 @@ typedef struct {
     int *first, *next, *plo, *phi, *parent;
     double *size, *mass, *cx, *cy, *cz, *soft, *delta, *gcx, *gcy, *gcz, *s;
     float *wsoft, *wlen;
+    /* the walk's view of each node (WNode, tree.h:195-204): COM and mass, rewritten by
+     * gzo_refresh_walk_nodes; the build moments above stay as built */
+    double *wcx, *wcy, *wcz, *wmass;
+    float* vmax; /* max |v| below the node since the build (0 without velocities) */
+    float* xpad;
+    /* with velocities: centre-of-mass velocity and the incremental moments (tree.h:300-317) */
+    double *vcom_x, *vcom_y, *vcom_z, *dp_x, *dp_y, *dp_z, *dq_x, *dq_y, *dq_z, *dmass;
     uint32_t *orderbuf, *rank;
     int* leaf_of;
     uint64_t* skey;
+    const double *vx, *vy, *vz; /* build-time velocities (only during the build) */
 } otree;
 
 static void grow(otree* T) {
@@ -142,7 +150,10 @@ static void grow(otree* T) {
     RE(first, int); RE(next, int); RE(plo, int); RE(phi, int); RE(parent, int);
     RE(size, double); RE(mass, double); RE(cx, double); RE(cy, double); RE(cz, double);
     RE(soft, double); RE(delta, double); RE(gcx, double); RE(gcy, double); RE(gcz, double);
-    RE(s, double); RE(wsoft, float); RE(wlen, float);
+    RE(s, double); RE(wsoft, float); RE(wlen, float); RE(vmax, float);
+    if (T->vx) {
+        RE(vcom_x, double); RE(vcom_y, double); RE(vcom_z, double);
+    }
 #undef RE
     T->cap = cap;
 }
@@ -192,12 +203,19 @@ static int make_node(otree* T, const gzo_particles* P, int lo, int hi, int level
     /* Moments: a leaf sums its particles in sorted order; an interior node combines its
      * children's final moments in octant order (tree.cc:153-190). */
     double M = 0, sx = 0, sy = 0, sz = 0, smax = 0;
+    double vmax = 0, pvx = 0, pvy = 0, pvz = 0; /* with velocities (tree.cc:161-166, 177-181) */
     if (nk == 0) {
         for (int i = lo; i < hi; ++i) {
             const uint32_t p = T->orderbuf[i];
             const double m = P->m[p];
             M += m; sx += m * P->x[p]; sy += m * P->y[p]; sz += m * P->z[p];
             if (P->soft && P->soft[p] > smax) smax = P->soft[p];
+            if (T->vx) {
+                const double v = sqrt(T->vx[p] * T->vx[p] + T->vy[p] * T->vy[p] +
+                                      T->vz[p] * T->vz[p]);
+                if (v > vmax) vmax = v;
+                pvx += m * T->vx[p]; pvy += m * T->vy[p]; pvz += m * T->vz[p];
+            }
             T->leaf_of[p] = me;
         }
     } else {
@@ -206,10 +224,17 @@ static int make_node(otree* T, const gzo_particles* P, int lo, int hi, int level
             const double m = T->mass[k];
             M += m; sx += m * T->cx[k]; sy += m * T->cy[k]; sz += m * T->cz[k];
             if (T->soft[k] > smax) smax = T->soft[k];
+            if (T->vmax[k] > vmax) vmax = T->vmax[k];
+            if (T->vx) { pvx += m * T->vcom_x[k]; pvy += m * T->vcom_y[k]; pvz += m * T->vcom_z[k]; }
         }
     }
     if (M > 0) { sx /= M; sy /= M; sz /= M; }
     T->mass[me] = M; T->cx[me] = sx; T->cy[me] = sy; T->cz[me] = sz; T->soft[me] = smax;
+    T->vmax[me] = (float)vmax;
+    if (T->vx) {
+        T->vcom_x[me] = 0; T->vcom_y[me] = 0; T->vcom_z[me] = 0;
+        if (M > 0) { T->vcom_x[me] = pvx / M; T->vcom_y[me] = pvy / M; T->vcom_z[me] = pvz / M; }
+    }
     const double dx = sx - gx, dy = sy - gy, dz = sz - gz;
     T->delta[me] = sqrt(dx * dx + dy * dy + dz * dz);
     return me;
@@ -227,7 +252,13 @@ static void thread_links(otree* T, int node, int successor) {
 }
 
 void* gzo_build(const gzo_particles* P, long long seed) {
+    return gzo_build_vel(P, seed, NULL, NULL, NULL);
+}
+
+void* gzo_build_vel(const gzo_particles* P, long long seed, const double* vx, const double* vy,
+                    const double* vz) {
     otree* T = (otree*)calloc(1, sizeof(otree));
+    if (vx && vy && vz) { T->vx = vx; T->vy = vy; T->vz = vz; }
     const size_t n = P->n;
     T->n = n;
     double root[4];
@@ -257,11 +288,25 @@ void* gzo_build(const gzo_particles* P, long long seed) {
     for (size_t r = 0; r < nl; ++r) T->rank[T->orderbuf[r]] = (uint32_t)r;
 
     /* packed walk fields (tree.cc:507-523) */
+    const size_t nn = (size_t)(T->nnodes ? T->nnodes : 1);
+    T->wcx = (double*)malloc(nn * sizeof(double)); T->wcy = (double*)malloc(nn * sizeof(double));
+    T->wcz = (double*)malloc(nn * sizeof(double));
+    T->wmass = (double*)malloc(nn * sizeof(double));
+    T->xpad = (float*)calloc(nn, sizeof(float));
     for (int i = 0; i < T->nnodes; ++i) {
         T->s[i] = T->size[i] + T->delta[i];
         T->wsoft[i] = (float)T->soft[i];
         T->wlen[i] = (float)T->size[i];
+        T->wcx[i] = T->cx[i]; T->wcy[i] = T->cy[i]; T->wcz[i] = T->cz[i];
+        T->wmass[i] = T->mass[i];
     }
+    if (T->vx) { /* incremental moments start at zero (tree.cc:449-453) */
+        T->dp_x = (double*)calloc(nn, sizeof(double)); T->dp_y = (double*)calloc(nn, sizeof(double));
+        T->dp_z = (double*)calloc(nn, sizeof(double)); T->dq_x = (double*)calloc(nn, sizeof(double));
+        T->dq_y = (double*)calloc(nn, sizeof(double)); T->dq_z = (double*)calloc(nn, sizeof(double));
+        T->dmass = (double*)calloc(nn, sizeof(double));
+    }
+    T->vx = T->vy = T->vz = NULL;
     return T;
 }
 
@@ -272,6 +317,9 @@ void gzo_tree_free(void* h) {
     free(T->size); free(T->mass); free(T->cx); free(T->cy); free(T->cz); free(T->soft);
     free(T->delta); free(T->gcx); free(T->gcy); free(T->gcz); free(T->s); free(T->wsoft);
     free(T->wlen); free(T->orderbuf); free(T->rank); free(T->leaf_of); free(T->skey);
+    free(T->wcx); free(T->wcy); free(T->wcz); free(T->wmass); free(T->vmax); free(T->xpad);
+    free(T->vcom_x); free(T->vcom_y); free(T->vcom_z); free(T->dp_x); free(T->dp_y);
+    free(T->dp_z); free(T->dq_x); free(T->dq_y); free(T->dq_z); free(T->dmass);
     free(T);
 }
 
@@ -305,6 +353,113 @@ void gzo_tree_particles(void* h, uint32_t* orderbuf, uint32_t* rank, int* leaf_o
 void gzo_tree_skeys(void* h, uint64_t* skey) {
     otree* T = (otree*)h;
     memcpy(skey, T->skey, T->n_live * sizeof(uint64_t));
+}
+
+/* ---------------------------------------------------------------------------------------------
+ * Tree reuse between rebuilds (tree.cc:211-274, 276-285, 311-319; the kick feed mfm.cc:3798-3811)
+ * ------------------------------------------------------------------------------------------- */
+
+/* The feed of one step's velocity changes, in the order of `active`: raise every ancestor's vmax
+ * to |v| (stopping at the first that already covers it, tree.cc:276-285), add the momentum change
+ * m (v - v_last) and its first moment times tau up the parents (tree.cc:211-227), then
+ * v_last = v. Without velocities at build time only vmax is raised; with vlast NULL no momentum is
+ * fed. Sequential: the result is that of the reference run on one thread. */
+void gzo_kick_nodes(void* h, const uint32_t* active, size_t na, const double* vx, const double* vy,
+                    const double* vz, const double* m, double* vlx, double* vly, double* vlz,
+                    double tau) {
+    otree* T = (otree*)h;
+    for (size_t k = 0; k < na; ++k) {
+        const uint32_t i = active[k];
+        const int leaf = T->leaf_of[i];
+        if (leaf < 0) continue;
+        const double vn = sqrt(vx[i] * vx[i] + vy[i] * vy[i] + vz[i] * vz[i]);
+        const float speed = (float)vn;
+        for (int no = leaf; no >= 0; no = T->parent[no]) {
+            if (T->vmax[no] >= speed) break;
+            T->vmax[no] = speed;
+        }
+        if (T->dp_x && vlx) {
+            const double px = (vx[i] - vlx[i]) * m[i], py = (vy[i] - vly[i]) * m[i],
+                         pz = (vz[i] - vlz[i]) * m[i];
+            for (int no = leaf; no >= 0; no = T->parent[no]) {
+                T->dp_x[no] += px; T->dp_y[no] += py; T->dp_z[no] += pz;
+                T->dq_x[no] += px * tau; T->dq_y[no] += py * tau; T->dq_z[no] += pz * tau;
+            }
+        }
+        if (vlx) { vlx[i] = vx[i]; vly[i] = vy[i]; vlz[i] = vz[i]; }
+    }
+}
+
+/* refresh_walk_nodes (tree.cc:260-274) with node_mass / node_com (tree.h:318-320, tree.cc:230-237):
+ * M = mass + dmass, COM = (m0 C + (m0 vcom + dp) tau - dq) / M, opening radius and side widened
+ * by 2 (vmax tau + xpad). */
+void gzo_refresh_walk_nodes(void* h, double tau) {
+    otree* T = (otree*)h;
+    for (int i = 0; i < T->nnodes; ++i) {
+        const double M = T->mass[i] + (T->dmass ? T->dmass[i] : 0.0);
+        double c0 = T->cx[i], c1 = T->cy[i], c2 = T->cz[i];
+        if (T->dp_x && M > 0) {
+            const double m0 = T->mass[i];
+            c0 = (m0 * T->cx[i] + (m0 * T->vcom_x[i] + T->dp_x[i]) * tau - T->dq_x[i]) / M;
+            c1 = (m0 * T->cy[i] + (m0 * T->vcom_y[i] + T->dp_y[i]) * tau - T->dq_y[i]) / M;
+            c2 = (m0 * T->cz[i] + (m0 * T->vcom_z[i] + T->dp_z[i]) * tau - T->dq_z[i]) / M;
+        }
+        T->wcx[i] = c0; T->wcy[i] = c1; T->wcz[i] = c2;
+        T->wmass[i] = M;
+        const double grow = 2.0 * ((double)T->vmax[i] * tau + (double)T->xpad[i]);
+        T->s[i] = T->size[i] + T->delta[i] + grow;
+        T->wlen[i] = (float)(T->size[i] + grow);
+    }
+}
+
+/* soft rounded up to float, so the node bound stays one-sided (tree.cc:289-292). */
+static float soft_as_bound(double h) {
+    float f = (float)h;
+    return ((double)f < h) ? nextafterf(f, 1e30f) : f;
+}
+
+/* raise_soft (tree.cc:311-319) for each listed particle, in order. */
+void gzo_raise_soft(void* h, const uint32_t* idx, size_t n, const double* soft) {
+    otree* T = (otree*)h;
+    for (size_t k = 0; k < n; ++k) {
+        const uint32_t i = idx[k];
+        const int leaf = T->leaf_of[i];
+        if (leaf < 0) continue;
+        const float sf = soft_as_bound(soft[i]);
+        for (int no = leaf; no >= 0; no = T->parent[no]) {
+            if (T->wsoft[no] >= sf) break;
+            T->wsoft[no] = sf;
+            if (T->soft[no] < soft[i]) T->soft[no] = soft[i];
+        }
+    }
+}
+
+/* The reuse state per node; any pointer may be NULL (vcom, dp and dq are zero without
+ * velocities at build time). */
+void gzo_tree_reuse(void* h, float* vmax, double* vcom, double* dp, double* dq, double* wcom,
+                    double* wmass) {
+    otree* T = (otree*)h;
+    const int nn = T->nnodes;
+    for (int i = 0; i < nn; ++i) {
+        if (vmax) vmax[i] = T->vmax[i];
+        if (vcom) {
+            vcom[3 * i] = T->vcom_x ? T->vcom_x[i] : 0.0;
+            vcom[3 * i + 1] = T->vcom_x ? T->vcom_y[i] : 0.0;
+            vcom[3 * i + 2] = T->vcom_x ? T->vcom_z[i] : 0.0;
+        }
+        if (dp) {
+            dp[3 * i] = T->dp_x ? T->dp_x[i] : 0.0;
+            dp[3 * i + 1] = T->dp_x ? T->dp_y[i] : 0.0;
+            dp[3 * i + 2] = T->dp_x ? T->dp_z[i] : 0.0;
+        }
+        if (dq) {
+            dq[3 * i] = T->dq_x ? T->dq_x[i] : 0.0;
+            dq[3 * i + 1] = T->dq_x ? T->dq_y[i] : 0.0;
+            dq[3 * i + 2] = T->dq_x ? T->dq_z[i] : 0.0;
+        }
+        if (wcom) { wcom[3 * i] = T->wcx[i]; wcom[3 * i + 1] = T->wcy[i]; wcom[3 * i + 2] = T->wcz[i]; }
+        if (wmass) wmass[i] = T->wmass[i];
+    }
 }
 
 /* ---------------------------------------------------------------------------------------------
@@ -363,13 +518,13 @@ void gzo_accel_grouped(void* h, const gzo_particles* P, const uint32_t* targets,
         int node = T->root;
         while (node >= 0) {
             ++visited;
-            const double wcx = T->cx[node], wcy = T->cy[node], wcz = T->cz[node];
+            const double wcx = T->wcx[node], wcy = T->wcy[node], wcz = T->wcz[node];
             const double len = (double)T->wlen[node], wsoft = (double)T->wsoft[node];
             const double dx = dmax(0.0, dmax(bx0 - wcx, wcx - bx1));
             const double dy = dmax(0.0, dmax(by0 - wcy, wcy - by1));
             const double dz = dmax(0.0, dmax(bz0 - wcz, wcz - bz1));
             const double rmin2 = dx * dx + dy * dy + dz * dz;
-            const int accept = !gzo_open_node(rmin2, len, T->s[node], T->mass[node], wsoft, emax,
+            const int accept = !gzo_open_node(rmin2, len, T->s[node], T->wmass[node], wsoft, emax,
                                               amin, theta2, dx, dy, dz);
             if (accept || T->first[node] < 0) {
                 if (!accept) {
@@ -396,7 +551,7 @@ void gzo_accel_grouped(void* h, const gzo_particles* P, const uint32_t* targets,
                         const double ox = wcx - tx[i], oy = wcy - ty[i], oz = wcz - tz[i];
                         const double r2 = ox * ox + oy * oy + oz * oz;
                         if (r2 <= 0) continue;
-                        const double f = T->mass[node] *
+                        const double f = T->wmass[node] *
                                          gzo_spline_force_over_r(sqrt(r2), dmax(e, 1e-300));
                         oax[i] += ox * f; oay[i] += oy * f; oaz[i] += oz * f;
                     }
@@ -429,10 +584,10 @@ void gzo_accel(void* h, const gzo_particles* P, const uint32_t* targets, size_t 
         double sx = 0, sy = 0, sz = 0;
         int node = T->root;
         while (node >= 0) {
-            const double ox = T->cx[node] - px, oy = T->cy[node] - py, oz = T->cz[node] - pz;
+            const double ox = T->wcx[node] - px, oy = T->wcy[node] - py, oz = T->wcz[node] - pz;
             const double r2 = ox * ox + oy * oy + oz * oz;
             if (T->first[node] < 0 ||
-                !gzo_open_node(r2, (double)T->wlen[node], T->s[node], T->mass[node],
+                !gzo_open_node(r2, (double)T->wlen[node], T->s[node], T->wmass[node],
                                (double)T->wsoft[node], ep, at, theta2, fabs(ox), fabs(oy),
                                fabs(oz))) {
                 if (T->first[node] < 0) {
@@ -450,7 +605,7 @@ void gzo_accel(void* h, const gzo_particles* P, const uint32_t* targets, size_t 
                     const double e = dmax(ep, (double)T->wsoft[node]);
                     const double r = sqrt(r2);
                     if (r > 0) {
-                        const double f = T->mass[node] * gzo_spline_force_over_r(r, dmax(e, 1e-300));
+                        const double f = T->wmass[node] * gzo_spline_force_over_r(r, dmax(e, 1e-300));
                         sx += ox * f; sy += oy * f; sz += oz * f;
                     }
                 }

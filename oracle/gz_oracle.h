@@ -16,6 +16,11 @@
  *   gzo_accel             bit for bit against the reference accel
  *   gzo_accel_brute       bit for bit against the reference accel_brute, Newton's third law, and
  *                         a two-body closed form
+ *   gzo_build_vel, gzo_kick_nodes, gzo_refresh_walk_nodes, gzo_raise_soft
+ *                         bit for bit against the reference (one thread) through vmax, vcom, the
+ *                         incremental moments and the refreshed walk nodes; the refreshed COM
+ *                         against the centre of mass of the drifted particles (ballistic motion
+ *                         between kicks makes it exact); vmax against max |v| below each node
  */
 #ifndef GZ_ORACLE_H
 #define GZ_ORACLE_H
@@ -55,6 +60,19 @@ void gzo_tree_nodes(void* t, int* first, int* next, int* plo, int* phi, int* par
                     double* soft, double* delta, double* gcx, double* gcy, double* gcz,
                     double* wn_s, float* wn_soft, float* wn_len);
 void gzo_tree_particles(void* t, uint32_t* orderbuf, uint32_t* rank, int* leaf_of);
+
+/* Build with particle velocities (tree.h:384-385, want_vcom = true): also per-node vmax and
+ * centre-of-mass velocity, and zeroed incremental moments. */
+void* gzo_build_vel(const gzo_particles* P, long long randomize_seed, const double* vx,
+                    const double* vy, const double* vz);
+/* Tree reuse (tree.cc:211-319; the kick feed of mfm.cc:3798-3811). */
+void gzo_kick_nodes(void* t, const uint32_t* active, size_t na, const double* vx, const double* vy,
+                    const double* vz, const double* m, double* vlx, double* vly, double* vlz,
+                    double tau);
+void gzo_refresh_walk_nodes(void* t, double tau);
+void gzo_raise_soft(void* t, const uint32_t* idx, size_t n, const double* soft);
+void gzo_tree_reuse(void* t, float* vmax, double* vcom, double* dp, double* dq, double* wcom,
+                    double* wmass);
 /* sorted keys (n_live) */
 void gzo_tree_skeys(void* t, uint64_t* skey);
 

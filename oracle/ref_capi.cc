@@ -180,6 +180,78 @@ long long gzref_audit(void* t, void* particles, char* msg, size_t msglen) {
                       msglen);
 }
 
+// shmem::build with velocities and want_vcom (tree.h:384-385): vmax, vcom and the incremental
+// moments, as rebuild_tree does (mfm.cc:1408-1413).
+void* gzref_build_vel(void* particles, long long randomize_seed, const double* vx,
+                      const double* vy, const double* vz) {
+    auto* rp = static_cast<RefParticles*>(particles);
+    auto* rt = new RefTree();
+    const double* vel[3] = {vx, vy, vz};
+    rt->T = build(rp->P, &rt->bt, vel, true, randomize_seed);
+    return rt;
+}
+
+// The kick feed of mfm.cc:3798-3811, run serially in the order of `active`: Tree::raise_vmax,
+// then Tree::kick_node with m (v - v_last) when v_last is given, then v_last = v.
+void gzref_kick_nodes(void* t, const uint32_t* active, size_t na, const double* vx,
+                      const double* vy, const double* vz, const double* m, double* vlx,
+                      double* vly, double* vlz, double tau) {
+    Tree& T = static_cast<RefTree*>(t)->T;
+    for (size_t k = 0; k < na; ++k) {
+        const uint32_t i = active[k];
+        const int leaf = T.leaf_of[i];
+        if (leaf < 0) continue;
+        const Vec3d v_now{vx[i], vy[i], vz[i]};
+        T.raise_vmax(leaf, (float)v_now.norm());
+        if (!T.dp_x.empty() && vlx)
+            T.kick_node(leaf, (v_now - Vec3d{vlx[i], vly[i], vlz[i]}) * m[i], tau);
+        if (vlx) { vlx[i] = vx[i]; vly[i] = vy[i]; vlz[i] = vz[i]; }
+    }
+}
+
+// Tree::refresh_walk_nodes (tree.cc:260-274).
+void gzref_refresh_walk_nodes(void* t, double tau) {
+    static_cast<RefTree*>(t)->T.refresh_walk_nodes(tau);
+}
+
+// Tree::raise_soft (tree.cc:311-319) for each listed particle, in order.
+void gzref_raise_soft(void* t, const uint32_t* idx, size_t n, const double* soft) {
+    Tree& T = static_cast<RefTree*>(t)->T;
+    for (size_t k = 0; k < n; ++k) T.raise_soft(T.leaf_of[idx[k]], soft[idx[k]]);
+}
+
+// Reuse state per node (tree.h:266-317) and the walk node view (wn): any pointer may be NULL;
+// vectors hold 3 values per node.
+void gzref_tree_reuse(void* t, float* vmax, double* vcom, double* dp, double* dq, double* wcom,
+                      double* wmass, double* wn_s, float* wn_soft, float* wn_len) {
+    const Tree& T = static_cast<RefTree*>(t)->T;
+    const size_t nn = T.nnodes();
+    const bool inc = !T.dp_x.empty();
+    for (size_t i = 0; i < nn; ++i) {
+        if (vmax) vmax[i] = T.vmax[i];
+        if (vcom) {
+            vcom[3 * i] = inc ? T.vcom_x[i] : 0.0;
+            vcom[3 * i + 1] = inc ? T.vcom_y[i] : 0.0;
+            vcom[3 * i + 2] = inc ? T.vcom_z[i] : 0.0;
+        }
+        if (dp) {
+            dp[3 * i] = inc ? T.dp_x[i] : 0.0;
+            dp[3 * i + 1] = inc ? T.dp_y[i] : 0.0;
+            dp[3 * i + 2] = inc ? T.dp_z[i] : 0.0;
+        }
+        if (dq) {
+            dq[3 * i] = inc ? T.dq_x[i] : 0.0;
+            dq[3 * i + 1] = inc ? T.dq_y[i] : 0.0;
+            dq[3 * i + 2] = inc ? T.dq_z[i] : 0.0;
+        }
+        if (wcom) { wcom[3 * i] = T.wn[i].cx; wcom[3 * i + 1] = T.wn[i].cy; wcom[3 * i + 2] = T.wn[i].cz; }
+        if (wmass) wmass[i] = T.wn[i].mass;
+        if (wn_s) wn_s[i] = T.wn[i].s;
+        if (wn_soft) wn_soft[i] = T.wn[i].soft;
+        if (wn_len) wn_len[i] = T.wn[i].len;
+    }
+}
+
 int gzref_max_threads(void) { return omp_get_max_threads(); }
 void gzref_set_threads(int n) { omp_set_num_threads(n); }
 
