@@ -128,6 +128,73 @@ def substeps(n=1 << 23, n_levels=5, precision="fp32", group_size=32, model=None,
     }
 
 
+def substeps_reuse(n=1 << 23, n_levels=5, precision="fp32", group_size=32, model=None, dt=1e-4,
+                   sigma_v=0.3, seed=11):
+    """The 16 substeps with one tree for all of them, as the reference runs between rebuilds
+    (mfm.cc:679-905, 3788-3811): at substep s every particle drifts by v dt, the tree's sorted
+    copy is re-read (update_sources), the walk nodes are refreshed from the incremental moments
+    (refresh_walk_nodes), the active set is walked (a plan formed once per active list: the leaves
+    do not change while the tree is reused), and its velocity changes are fed to the tree
+    (kick_nodes). Velocities are Gaussian with sigma_v per axis; the kick adds sigma_v / 10."""
+    x, y, z, m, soft = ics.halo_disk(n=n)
+    rng = np.random.default_rng(seed)
+    v = [torch.as_tensor(rng.normal(scale=sigma_v, size=n), device="cuda:0") for _ in range(3)]
+    P = gz.Particles(x, y, z, m, soft)
+    T = gz.build(P, vel=v)
+    ctx = T.ctx
+    tg = T.device_orderbuf()
+    boot = gz.accel_grouped(T, P, tg, THETA_BOOT, 1.0, 0)
+    amag = torch.empty(P.n, dtype=torch.float64, device=P.device)
+    amag[tg.view(torch.int32).long()] = torch.linalg.norm(torch.stack(boot, 1), dim=1)
+    level = ics.timestep_bins(x, y, z, n_levels=n_levels)
+    nsub = 2 ** (n_levels - 1)
+    lists = [gz.order_targets(T, ics.active_for_substep(level, s, n_levels)) for s in range(nsub)]
+    aolds = [ALPHA_REF * amag[t.view(torch.int32).long()] for t in lists]
+    plans = [gz.WalkPlan(T, P, t, precision=precision, group_size=group_size, model=model)
+             for t in lists]
+    vlast = [a.clone() for a in v]
+    g = torch.Generator(device="cuda:0").manual_seed(seed)
+    for a, va in zip((P.x, P.y, P.z), v):  # load the drift's kernels before timing
+        a.add_(va, alpha=0.0)
+    parts = np.zeros((nsub, 5))
+    tau = 0.0
+    for s in range(nsub):
+        _sync(ctx)
+        t0 = time.perf_counter()
+        for a, va in zip((P.x, P.y, P.z), v):
+            a.add_(va, alpha=dt)
+        tau += dt
+        _sync(ctx)
+        t1 = time.perf_counter()
+        T.update_sources(P)
+        t2 = time.perf_counter()
+        T.refresh_walk_nodes(tau)
+        t3 = time.perf_counter()
+        plans[s].launch(0.0, 1.0, aolds[s])
+        _sync(ctx)
+        t4 = time.perf_counter()
+        act = lists[s].view(torch.int32).long()
+        for va in v:
+            va[act] += sigma_v / 10 * torch.randn(act.shape[0], generator=g, device="cuda:0",
+                                                  dtype=torch.float64)
+        T.kick_nodes(lists[s], v, None, vlast, tau)
+        _sync(ctx)
+        t5 = time.perf_counter()
+        parts[s] = (t1 - t0, t2 - t1, t3 - t2, t4 - t3, t5 - t4)
+    total = sum(int(t.shape[0]) for t in lists)
+    return {
+        "workload": f"halo+disk N={n}, {nsub} substeps on one reused tree (drift dt={dt}, "
+                    f"sigma_v={sigma_v})",
+        "precision": precision, "group_size": group_size, "learned": model is not None,
+        "ms_per_substep": (parts.sum(1) * 1e3).round(3).tolist(),
+        "walk_ms_per_substep": (parts[:, 3] * 1e3).round(3).tolist(),
+        "mean_ms": {k: float(parts[:, i].mean() * 1e3) for i, k in
+                    enumerate(("drift", "update_sources", "refresh", "walk", "kick_feed"))},
+        "evals_per_s_incl_reuse": total / parts.sum(),
+        "plan_status": plans[0].status(),
+    }
+
+
 def spheres(n_spheres=512, precision="fp32", group_size=32, theta=0.7, n_probe_spheres=16,
             reps=3):
     (x, y, z, m, soft), starts = ics.spheres(n_spheres=n_spheres)
@@ -185,6 +252,7 @@ def main():
         from . import learned
         model = learned.load_model()
     print(json.dumps(substeps(model=model)), flush=True)
+    print(json.dumps(substeps_reuse(model=model)), flush=True)
     print(json.dumps(spheres()), flush=True)
 
 

@@ -14,42 +14,91 @@
 
 namespace gz {
 
+// Sum of v over the lanes of this lane's run (runs: maximal stretches of consecutive lanes with
+// the same run id), left in the run's first lane; exact to the order of the doubling steps.
+__device__ __forceinline__ double run_sum(double v, int run, int lane) {
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const double o = __shfl_down(v, d);
+        const int r = __shfl_down(run, d);
+        if (lane + d < 64 && r == run) v += o;
+    }
+    return v;
+}
+
+// One thread per active particle. The speed bound climbs each particle's own parent chain with
+// atomicMax, stopping at the first ancestor that already covers it (tree.cc:276-285). The momentum
+// feed (tree.cc:211-227) climbs level by level, all lanes of the wave together: at level L the
+// lanes whose chain has reached level L hold the node there; lanes holding the same node in
+// consecutive lanes (in a Morton-ordered active list, all lanes under one node are consecutive)
+// add their contributions in registers first, and one lane per run adds the run's total to the
+// node atomically. Any order of `active` is correct; Morton order makes the runs long.
 __global__ __launch_bounds__(256) void k_kick_nodes(
     const uint32_t* __restrict__ active, size_t na, size_t n, const int* __restrict__ leaf_of,
-    const int* __restrict__ parent, const double* __restrict__ vx, const double* __restrict__ vy,
-    const double* __restrict__ vz, const double* __restrict__ m, double* __restrict__ vlx,
-    double* __restrict__ vly, double* __restrict__ vlz, double tau, uint32_t* __restrict__ vmax,
-    double* __restrict__ dp, double* __restrict__ dq, size_t nn, int* __restrict__ status) {
+    const int* __restrict__ parent, const int* __restrict__ level_begin,
+    const double* __restrict__ vx, const double* __restrict__ vy, const double* __restrict__ vz,
+    const double* __restrict__ m, double* __restrict__ vlx, double* __restrict__ vly,
+    double* __restrict__ vlz, double tau, uint32_t* __restrict__ vmax, double* __restrict__ dp,
+    double* __restrict__ dq, size_t nn, int* __restrict__ status) {
     const size_t k = (size_t)blockIdx.x * 256 + threadIdx.x;
-    if (k >= na) return;
-    const uint32_t i = active[k];
-    if (i >= n) {
-        atomicOr(status, 1);
-        return;
+    const int lane = threadIdx.x & 63;
+    int leaf = -1;
+    uint32_t i = 0;
+    if (k < na) {
+        i = active[k];
+        if (i >= n) atomicOr(status, 1);
+        else leaf = leaf_of[i];
     }
-    const int leaf = leaf_of[i];
-    if (leaf < 0) return;
-    const double ux = vx[i], uy = vy[i], uz = vz[i];
-    // raise_vmax (tree.cc:276-285): a non-negative float's bits order as its value, and a node's
-    // bound covers its children's, so the climb stops at the first ancestor already covering it
-    const float speed = (float)sqrt(ux * ux + uy * uy + uz * uz);
-    const uint32_t want = __float_as_uint(speed);
-    for (int no = leaf; no >= 0; no = parent[no])
-        if (atomicMax(&vmax[no], want) >= want) break;
-    if (vlx) {
-        if (dp) { // kick_node (tree.cc:211-227)
-            const double px = (ux - vlx[i]) * m[i], py = (uy - vly[i]) * m[i],
-                         pz = (uz - vlz[i]) * m[i];
-            const double qx = px * tau, qy = py * tau, qz = pz * tau;
-            for (int no = leaf; no >= 0; no = parent[no]) {
-                atomicAdd(&dp[no], px);
-                atomicAdd(&dp[nn + no], py);
-                atomicAdd(&dp[2 * nn + no], pz);
-                atomicAdd(&dq[no], qx);
-                atomicAdd(&dq[nn + no], qy);
-                atomicAdd(&dq[2 * nn + no], qz);
+    double ux = 0, uy = 0, uz = 0;
+    if (leaf >= 0) {
+        ux = vx[i]; uy = vy[i]; uz = vz[i];
+        const float speed = (float)sqrt(ux * ux + uy * uy + uz * uz);
+        const uint32_t want = __float_as_uint(speed);
+        for (int no = leaf; no >= 0; no = parent[no])
+            if (atomicMax(&vmax[no], want) >= want) break;
+    }
+    if (!vlx) return;
+    if (dp) {
+        double c[6] = {0, 0, 0, 0, 0, 0};
+        int lvl = -1, node = leaf;
+        if (leaf >= 0) {
+            const double mi = m[i];
+            c[0] = (ux - vlx[i]) * mi;
+            c[1] = (uy - vly[i]) * mi;
+            c[2] = (uz - vlz[i]) * mi;
+            c[3] = c[0] * tau;
+            c[4] = c[1] * tau;
+            c[5] = c[2] * tau;
+            lvl = 0; // node ids are level-major: the leaf's level is where its id falls
+            while (lvl < kMaxLevel && level_begin[lvl + 1] <= leaf) ++lvl;
+        }
+        for (int L = kMaxLevel; L >= 0; --L) {
+            const bool here = (lvl == L);
+            const uint64_t on = __ballot(here);
+            if (!on) continue;
+            // runs: consecutive participating lanes at the same node
+            const int prev = __shfl_up(here ? node : -1, 1);
+            const bool head = here && (lane == 0 || prev != node);
+            const uint64_t heads = __ballot(head);
+            const uint64_t below = (lane == 63) ? heads : (heads & ((2ull << lane) - 1));
+            const int run = here ? (int)__popcll(below) : -1 - lane;
+            double t[6];
+#pragma unroll
+            for (int q = 0; q < 6; ++q) t[q] = run_sum(here ? c[q] : 0.0, run, lane);
+            if (head) {
+#pragma unroll
+                for (int q = 0; q < 3; ++q) {
+                    atomicAdd(&dp[q * nn + node], t[q]);
+                    atomicAdd(&dq[q * nn + node], t[3 + q]);
+                }
+            }
+            if (here) {
+                node = parent[node];
+                lvl = (node >= 0) ? L - 1 : -1;
             }
         }
+    }
+    if (leaf >= 0) {
         vlx[i] = ux;
         vly[i] = uy;
         vlz[i] = uz;
@@ -131,7 +180,8 @@ gz_status launch_kick_nodes(gz_tree* t, const uint32_t* active, size_t na, const
                             double* vly, double* vlz, double tau, int* status, hipStream_t st) {
     if (na == 0 || t->nnodes == 0) return GZ_OK;
     k_kick_nodes<<<(unsigned)((na + 255) / 256), 256, 0, st>>>(
-        active, na, t->n, t->leaf_of, t->parent, vx, vy, vz, m, vlx, vly, vlz, tau,
+        active, na, t->n, t->leaf_of, t->parent, t->level_begin_dev, vx, vy, vz, m, vlx, vly,
+        vlz, tau,
         reinterpret_cast<uint32_t*>(t->vmax), t->dp, t->dq, (size_t)t->nnodes, status);
     GZ_HIP(hipGetLastError());
     return GZ_OK;
