@@ -58,6 +58,7 @@ struct WalkExtra {
     bool groups_only = false;   // stop after the grouping (walk plans)
     const uint32_t* seg = nullptr; // device: nseg + 1 slot offsets of independent target lists
     int nseg = 0;
+    bool potential = false;        // sum the potential into ax (ay, az unused)
 };
 
 // The walk shared by gz_tree_accel_grouped, gz_tree_accel and gz_tree_group_features.
@@ -74,7 +75,9 @@ gz_status walk(gz_ctx* ctx, const gz_tree* tree, const gz_particles* p, const ui
     if (p->n != tree->n)
         return fail(GZ_ERR_INVALID, "%s: particle count %zu differs from the tree's %zu", who,
                     p->n, tree->n);
-    if (n_t > 0 && (!targets || (!features_only && !groups_only && (!ax || !ay || !az))))
+    const bool need_xyz = !features_only && !groups_only && !ex.potential;
+    if (n_t > 0 && (!targets || (need_xyz && (!ay || !az)) ||
+                    (!features_only && !groups_only && !ax)))
         return fail(GZ_ERR_INVALID, "%s: null target or output array", who);
     if ((ex.model || features_only) && !aold)
         return fail(GZ_ERR_INVALID, "%s: the learned criterion needs aold", who);
@@ -210,6 +213,7 @@ gz_status walk(gz_ctx* ctx, const gz_tree* tree, const gz_particles* p, const ui
     a.nclusters = tree->n_clusters;
     a.persistent = persistent;
     a.leaf_always_open = leaf_always_open;
+    a.potential = ex.potential ? 1 : 0;
     a.queue = queue;
     a.status = status;
     a.counts = misc;
@@ -995,6 +999,19 @@ gz_status gz_tree_accel(gz_ctx* ctx, const gz_tree* tree, const gz_particles* p,
     // |COM - x| per axis exactly as shmem::accel computes it (tree.cc:552-557)
     return walk(ctx, tree, p, targets, n_t, theta, G, 1, 0, aold, GZ_PREC_FP64, 1, 1, 0,
                 0, 0, ax, ay, az, nullptr, "gz_tree_accel");
+}
+
+gz_status gz_tree_potential(gz_ctx* ctx, const gz_tree* tree, const gz_particles* p,
+                            const uint32_t* targets, size_t n_t, double theta, double G,
+                            const double* aold, double* phi) {
+    gz::clear_error();
+    if (n_t > 0 && !phi) return fail(GZ_ERR_INVALID, "gz_tree_potential: null output");
+    // shmem::potential walks each target alone, as shmem::accel does: one group per target,
+    // every reached leaf summed particle by particle
+    WalkExtra ex;
+    ex.potential = true;
+    return walk(ctx, tree, p, targets, n_t, theta, G, 1, 0, aold, GZ_PREC_FP64, 1, 1, 0, 0, 0,
+                phi, nullptr, nullptr, nullptr, "gz_tree_potential", ex);
 }
 
 gz_status gz_direct_sum(gz_ctx* ctx, const gz_particles* p, const uint32_t* targets, size_t n_t,

@@ -70,6 +70,7 @@ struct WalkArgs {
     int nclusters;
     int persistent;
     int leaf_always_open;   // 1: a reached leaf is always summed pair by pair (shmem::accel)
+    int potential;          // 1: sum the spline potential into ax instead (shmem::potential, FP64)
     int* queue;             // persistent launch: eight work counters, one per XCD (zeroed)
     int* status;            // bit 0: a group's global node stack overflowed too
     long long* counts;      // nodes visited, multipole, pairs, max stack depth (COUNT builds)

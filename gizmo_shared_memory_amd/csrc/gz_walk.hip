@@ -52,6 +52,16 @@ __device__ __forceinline__ double spline_force_over_r_d(double r, double h) {
                      0.06666666666666667 / (u * u * u));
 }
 
+// spline_potential (tree.h:72-80), operand for operand: exactly -1/r beyond h.
+__device__ __forceinline__ double spline_potential_d(double r, double h) {
+    const double h_inv = 1.0 / h;
+    const double u = r * h_inv;
+    if (u >= 1.0) return -1.0 / r;
+    if (u < 0.5) return h_inv * (-2.8 + u * u * (5.333333333333333 + u * u * (6.4 * u - 9.6)));
+    return h_inv * (-3.2 + 0.06666666666666667 / u +
+                    u * u * (10.666666666666667 + u * (-16.0 + u * (9.6 - 2.1333333333333333 * u))));
+}
+
 // open_node (tree.h:109-144), operand for operand.
 __device__ __forceinline__ bool open_node_d(double r_sq, double len, double l_open, double mass,
                                             double node_soft, double soft_t, double aold,
@@ -390,7 +400,28 @@ __global__ __launch_bounds__(64, 4) void k_walk(WalkArgs a) {
             // Evaluate both lists against every target of the chunk.
             auto flush = [&]() {
                 __syncthreads();
-                if (FP64) {
+                if (FP64 && a.potential) {
+                    // shmem::potential (tree.cc:899-944): an accepted node lies outside both
+                    // softenings (open_node's first test), where the spline is exactly -1/r
+                    for (int j = jstart; j < mcount; j += L) {
+                        const real4 s = s_mono[j];
+                        const double dx = (double)s.x - (double)tx, dy = (double)s.y - (double)ty,
+                                     dz = (double)s.z - (double)tz;
+                        const double r = sqrt(dx * dx + dy * dy + dz * dz);
+                        if (r > 0) accx += (double)s.w * (-1.0 / r);
+                    }
+                    for (int j = jstart; j < pcount; j += L) {
+                        const real4 s = s_pair[j];
+                        const double es = (double)s_psoft[j];
+                        const double dx = (double)s.x - (double)tx, dy = (double)s.y - (double)ty,
+                                     dz = (double)s.z - (double)tz;
+                        const double r = sqrt(dx * dx + dy * dy + dz * dz);
+                        if (r <= 0) continue;
+                        if (COUNT && tvalid) ++g_pair;
+                        accx += (double)s.w *
+                                spline_potential_d(r, dmax_w(dmax_w((double)te, es), 1e-300));
+                    }
+                } else if (FP64) {
                     for (int j = jstart; j < mcount; j += L) {
                         // u >= 1 for every accepted node, so the spline is 1/r^3 (tree.h:63)
                         const real4 s = s_mono[j];
@@ -647,9 +678,11 @@ __global__ __launch_bounds__(64, 4) void k_walk(WalkArgs a) {
                 }
             }
             if (tvalid && sub == 0) {
-                a.ax[tlo + t] = a.G * accx;
-                a.ay[tlo + t] = a.G * accy;
-                a.az[tlo + t] = a.G * accz;
+                a.ax[tlo + t] = a.G * accx; // the potential, in potential walks
+                if (!a.potential) {
+                    a.ay[tlo + t] = a.G * accy;
+                    a.az[tlo + t] = a.G * accz;
+                }
             }
         }
         if (COUNT) {

@@ -487,6 +487,20 @@ class Graph:
             self.handle = None
 
 
+def potential(T: Tree, P: Particles, targets, theta: float, G: float = 1.0, aold=None):
+    """shmem::potential (tree.h:446-447): spline-softened potential per target (FP64)."""
+    tg = _dev_u32(targets, P.device)
+    nt = int(tg.shape[0])
+    phi = torch.empty(nt, dtype=torch.float64, device=P.device)
+    ao = None if aold is None else _dev_f64(aold, P.device)
+    T.ctx.enter()
+    L.check(L.load().gz_tree_potential(T.ctx.handle, T.handle, P.cstruct(), _ptr(tg), nt,
+                                       float(theta), float(G), _ptr(ao), _ptr(phi)),
+            "gz_tree_potential")
+    T.ctx.synchronize()
+    return phi
+
+
 def accel(T: Tree, P: Particles, targets, theta: float, G: float = 1.0, aold=None):
     """shmem::accel (tree.h:392-394): per-target walk, reached leaves always summed."""
     tg = _dev_u32(targets, P.device)

@@ -236,6 +236,12 @@ gz_status gz_tree_accel(gz_ctx* ctx, const gz_tree* tree, const gz_particles* p,
                         const uint32_t* targets, size_t n_t, double theta, double G,
                         const double* aold, double* ax, double* ay, double* az);
 
+/* shmem::potential (tree.h:446-447, tree.cc:899-944): the spline-softened potential at each
+ * target, G included, walked per target like gz_tree_accel (leaves always summed), FP64. */
+gz_status gz_tree_potential(gz_ctx* ctx, const gz_tree* tree, const gz_particles* p,
+                            const uint32_t* targets, size_t n_t, double theta, double G,
+                            const double* aold, double* phi);
+
 /* shmem::accel_brute (tree.h:450-451, tree.cc:946-971): exact O(n_t * N) sum in FP64 with the
  * walk's pair rule, in source index order; includes G. This is the error judge. The reference's
  * device counterpart is shmem_cuda_accel_bruteforce (grav_cuda.cu:186-192), which omits G. */

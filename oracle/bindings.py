@@ -190,6 +190,8 @@ class Oracle(_Lib):
                                 C.c_double, C.c_double, _dp, _dp, _dp, _dp]
         L.gzo_accel_brute.argtypes = [C.POINTER(_GzoParticles), _up, C.c_size_t, C.c_double,
                                       _dp, _dp, _dp]
+        L.gzo_potential.argtypes = [C.c_void_p, C.POINTER(_GzoParticles), _up, C.c_size_t,
+                                    C.c_double, C.c_double, _dp, _dp]
         L.gzo_build_vel.restype = C.c_void_p
         L.gzo_build_vel.argtypes = [C.POINTER(_GzoParticles), C.c_longlong, _dp, _dp, _dp]
         L.gzo_kick_nodes.argtypes = [C.c_void_p, _up, C.c_size_t, _dp, _dp, _dp, _dp, _dp, _dp,
@@ -245,6 +247,15 @@ class Oracle(_Lib):
         if want_counts:
             return ax, ay, az, counts
         return ax, ay, az
+
+    def potential(self, t, P: HostParticles, targets, theta, G=1.0, aold=None):
+        tg = np.ascontiguousarray(targets, dtype=np.uint32)
+        out = np.zeros(tg.shape[0])
+        ao = None if aold is None else _f64(aold)
+        cp = self._cp(P)
+        self.lib.gzo_potential(t, C.byref(cp), _ptr(tg, _up), tg.shape[0], theta, G,
+                               _ptr(ao, _dp), _ptr(out, _dp))
+        return out
 
     def build_vel(self, P: HostParticles, v, seed: int = -1):
         cp = self._cp(P)

@@ -618,6 +618,52 @@ void gzo_accel(void* h, const gzo_particles* P, const uint32_t* targets, size_t 
     }
 }
 
+/* The potential walk (tree.cc:899-944): per target, the accel walk's traversal summing the spline
+ * potential (tree.h:72-80) of accepted nodes (softening max(target, node)) and of every particle
+ * of a reached leaf (softening max of the pair). */
+void gzo_potential(void* h, const gzo_particles* P, const uint32_t* targets, size_t nt,
+                   double theta, double G, const double* aold, double* phi) {
+    const otree* T = (const otree*)h;
+    const double theta2 = theta * theta;
+#pragma omp parallel for schedule(dynamic, 16)
+    for (long long t = 0; t < (long long)nt; ++t) {
+        const uint32_t p = targets[t];
+        const double px = P->x[p], py = P->y[p], pz = P->z[p];
+        const double ep = soft_of(P, p);
+        const double at = aold ? aold[t] : 0.0;
+        double sum = 0;
+        int node = T->root;
+        while (node >= 0) {
+            const double ox = T->wcx[node] - px, oy = T->wcy[node] - py, oz = T->wcz[node] - pz;
+            const double r2 = ox * ox + oy * oy + oz * oz;
+            if (T->first[node] < 0 ||
+                !gzo_open_node(r2, (double)T->wlen[node], T->s[node], T->wmass[node],
+                               (double)T->wsoft[node], ep, at, theta2, fabs(ox), fabs(oy),
+                               fabs(oz))) {
+                if (T->first[node] < 0) {
+                    for (int k = T->plo[node]; k < T->phi[node]; ++k) {
+                        const uint32_t q = T->orderbuf[k];
+                        if (q == p) continue;
+                        const double dx = P->x[q] - px, dy = P->y[q] - py, dz = P->z[q] - pz;
+                        const double r = sqrt(dx * dx + dy * dy + dz * dz);
+                        if (r <= 0) continue;
+                        sum += P->m[q] *
+                               gzo_spline_potential(r, dmax(dmax(ep, soft_of(P, q)), 1e-300));
+                    }
+                } else {
+                    const double e = dmax(ep, (double)T->wsoft[node]);
+                    const double r = sqrt(r2);
+                    if (r > 0) sum += T->wmass[node] * gzo_spline_potential(r, dmax(e, 1e-300));
+                }
+                node = T->next[node];
+            } else {
+                node = T->first[node];
+            }
+        }
+        phi[t] = G * sum;
+    }
+}
+
 /* Exact FP64 sum over every particle with the walk's pair rule (tree.cc:946-971). */
 void gzo_accel_brute(const gzo_particles* P, const uint32_t* targets, size_t nt, double G,
                      double* ax, double* ay, double* az) {

@@ -14,6 +14,8 @@
  *   gzo_accel_grouped     bit for bit against the reference accel_grouped (both regimes), and
  *                         theta -> 0 against gzo_accel_brute
  *   gzo_accel             bit for bit against the reference accel
+ *   gzo_potential         bit for bit against the reference potential, and theta -> 0 against
+ *                         the direct sum of the spline potential
  *   gzo_accel_brute       bit for bit against the reference accel_brute, Newton's third law, and
  *                         a two-body closed form
  *   gzo_build_vel, gzo_kick_nodes, gzo_refresh_walk_nodes, gzo_raise_soft
@@ -83,6 +85,8 @@ void gzo_accel_grouped(void* t, const gzo_particles* P, const uint32_t* targets,
                        double* ay, double* az, long long* counts);
 void gzo_accel(void* t, const gzo_particles* P, const uint32_t* targets, size_t nt, double theta,
                double G, const double* aold, double* ax, double* ay, double* az);
+void gzo_potential(void* t, const gzo_particles* P, const uint32_t* targets, size_t nt,
+                   double theta, double G, const double* aold, double* phi);
 void gzo_accel_brute(const gzo_particles* P, const uint32_t* targets, size_t nt, double G,
                      double* ax, double* ay, double* az);
 

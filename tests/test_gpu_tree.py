@@ -317,3 +317,22 @@ def test_packed_groups_are_compact_and_cover_targets(gz):
     ex = gz.accel_brute(P, tg[::97])
     e = _relerr(tuple(_np(v)[::97] for v in plain[:3]), tuple(_np(v) for v in ex))
     assert np.percentile(e, 99) < 2e-2
+
+
+@pytest.mark.parametrize("name", ["plummer20k", "tombstones", "close_pair", "n17"])
+def test_potential_matches_oracle(gz, oracle, name):
+    """shmem::potential (tree.cc:899-944) in FP64, per target, both opening regimes."""
+    case = tree_cases.CASES[name]()
+    print(f"[potential] case={name}", flush=True)
+    P, T = _device_tree(gz, case)
+    hp = _host(case)
+    to = oracle.build(hp)
+    try:
+        tg = oracle.tree(to, hp).orderbuf
+        for theta, aold in ((0.7, None), (0.0, np.full(tg.shape[0], 1e-3))):
+            g = _np(gz.potential(T, P, tg, theta, 2.0, aold))
+            o = oracle.potential(to, hp, tg, theta, 2.0, aold)
+            if tg.size:
+                assert np.all(np.abs(g - o) <= 1e-12 * np.abs(o)), np.max(np.abs(g - o) / np.abs(o))
+    finally:
+        oracle.free(to)

@@ -291,3 +291,29 @@ def test_walk_error_decreases_with_theta(oracle):
     oracle.free(to)
     assert med == sorted(med, reverse=True)
     assert med[-1] < 1e-3
+
+
+def test_potential_bitwise_and_direct_limit(oracle, reference):
+    """shmem::potential (tree.cc:899-944): bit for bit against the reference in both regimes, and
+    theta -> 0 (every node opened) equal to the direct sum of the spline potential."""
+    P = _hp(tree_cases.plummer_small(3000, seed=19, eps=0.02))
+    targets = np.arange(0, P.n, 7, dtype=np.uint32)
+    to = oracle.build(P)
+    rp = reference.particles(P)
+    tr = reference.build(rp)
+    try:
+        for theta, aold in ((0.7, None), (0.0, np.full(targets.shape[0], 2e-3))):
+            a = oracle.potential(to, P, targets, theta, 1.5, aold)
+            b = reference.potential(tr, rp, targets, theta, 1.5, aold)
+            assert np.array_equal(a, b)
+        phi = oracle.potential(to, P, targets, 1e-9, 1.0, None)
+    finally:
+        oracle.free(to); reference.free(tr); reference.free_particles(rp)
+    ref = np.zeros(targets.shape[0])
+    for k, i in enumerate(targets):
+        d = np.sqrt((P.x - P.x[i]) ** 2 + (P.y - P.y[i]) ** 2 + (P.z - P.z[i]) ** 2)
+        h = np.maximum(np.maximum(P.soft, P.soft[i]), 1e-300)
+        ok = d > 0
+        ref[k] = np.sum(P.m[ok] * np.array([oracle.spline_potential(r, hh)
+                                              for r, hh in zip(d[ok], h[ok])]))
+    np.testing.assert_allclose(phi, ref, rtol=1e-12)
