@@ -93,6 +93,11 @@ _PROTOS = {
     "gz_tree_device_arrays": (C.c_int, [C.c_void_p, C.POINTER(C.c_void_p),
                                         C.POINTER(C.c_void_p), C.POINTER(C.c_void_p)]),
     "gz_tree_update_sources": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(gz_particles)]),
+    "gz_tree_accel_grouped_tidal": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(gz_particles),
+                                              C.c_void_p, C.c_size_t, C.c_double, C.c_double,
+                                              C.c_int, C.c_void_p, C.c_void_p, C.c_void_p,
+                                              C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                              C.c_void_p, C.c_void_p]),
     "gz_tree_potential": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(gz_particles), C.c_void_p,
                                     C.c_size_t, C.c_double, C.c_double, C.c_void_p, C.c_void_p]),
     "gz_tree_kick_nodes": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t, C.c_void_p,

@@ -59,6 +59,9 @@ struct WalkExtra {
     const uint32_t* seg = nullptr; // device: nseg + 1 slot offsets of independent target lists
     int nseg = 0;
     bool potential = false;        // sum the potential into ax (ay, az unused)
+    double* tidal = nullptr;       // tidal tensor out (6 x n_t), FP64 walks
+    double* jerk = nullptr;        // jerk out (3 x n_t), with vx/vy/vz
+    const double *vx = nullptr, *vy = nullptr, *vz = nullptr;
 };
 
 // The walk shared by gz_tree_accel_grouped, gz_tree_accel and gz_tree_group_features.
@@ -214,6 +217,26 @@ gz_status walk(gz_ctx* ctx, const gz_tree* tree, const gz_particles* p, const ui
     a.persistent = persistent;
     a.leaf_always_open = leaf_always_open;
     a.potential = ex.potential ? 1 : 0;
+    if (ex.tidal) {
+        if (precision != GZ_PREC_FP64)
+            return fail(GZ_ERR_UNSUPPORTED, "%s: the tidal tensor is an FP64 walk", who);
+        a.tidal = ex.tidal;
+        a.orderbuf = tree->orderbuf;
+        a.nnodes = tree->nnodes;
+        const bool jerk = ex.jerk && ex.vx && ex.vy && ex.vz && tree->has_vel;
+        if (ex.jerk && !jerk) // the reference leaves the jerk zero without velocities
+            GZ_TRY(gz::fill_zero_u32(reinterpret_cast<uint32_t*>(ex.jerk), 6 * n_t, st));
+        if (jerk) {
+            gz::DevMem& nv = ctx->scratch[9];
+            GZ_TRY(nv.reserve(3 * (size_t)std::max(tree->nnodes, 1) * sizeof(double)));
+            GZ_TRY(gz::launch_node_vel(tree, nv.as<double>(), st));
+            a.jerk = ex.jerk;
+            a.nvel = nv.as<double>();
+            a.pvx = ex.vx;
+            a.pvy = ex.vy;
+            a.pvz = ex.vz;
+        }
+    }
     a.queue = queue;
     a.status = status;
     a.counts = misc;
@@ -225,7 +248,7 @@ gz_status walk(gz_ctx* ctx, const gz_tree* tree, const gz_particles* p, const ui
     if (persistent) GZ_TRY(gz::ctx_spill(ctx, &a.spill, &a.spill_cap));
     a.G = G;
     a.ax = ax; a.ay = ay; a.az = az;
-    const bool fp64 = precision == GZ_PREC_FP64, cnt = count && counts != nullptr;
+    const bool fp64 = precision == GZ_PREC_FP64, cnt = count && counts != nullptr && !ex.tidal;
     GZ_TRY(gz::launch_walk(a, fp64, cnt, st));
     GZ_HIP(hipEventRecord(ev[10], st));
 
@@ -999,6 +1022,23 @@ gz_status gz_tree_accel(gz_ctx* ctx, const gz_tree* tree, const gz_particles* p,
     // |COM - x| per axis exactly as shmem::accel computes it (tree.cc:552-557)
     return walk(ctx, tree, p, targets, n_t, theta, G, 1, 0, aold, GZ_PREC_FP64, 1, 1, 0,
                 0, 0, ax, ay, az, nullptr, "gz_tree_accel");
+}
+
+gz_status gz_tree_accel_grouped_tidal(gz_ctx* ctx, const gz_tree* tree, const gz_particles* p,
+                                      const uint32_t* targets, size_t n_t, double theta, double G,
+                                      int batch, const double* aold, const double* vx,
+                                      const double* vy, const double* vz, double* ax, double* ay,
+                                      double* az, double* tidal, double* jerk) {
+    gz::clear_error();
+    if (n_t > 0 && !tidal) return fail(GZ_ERR_INVALID, "gz_tree_accel_grouped_tidal: null tidal");
+    WalkExtra ex;
+    ex.tidal = tidal;
+    ex.jerk = jerk;
+    ex.vx = vx;
+    ex.vy = vy;
+    ex.vz = vz;
+    return walk(ctx, tree, p, targets, n_t, theta, G, batch, 0, aold, GZ_PREC_FP64, 1, 0, 0, 0, 0,
+                ax, ay, az, nullptr, "gz_tree_accel_grouped_tidal", ex);
 }
 
 gz_status gz_tree_potential(gz_ctx* ctx, const gz_tree* tree, const gz_particles* p,

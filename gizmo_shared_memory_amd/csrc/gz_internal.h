@@ -71,6 +71,14 @@ struct WalkArgs {
     int persistent;
     int leaf_always_open;   // 1: a reached leaf is always summed pair by pair (shmem::accel)
     int potential;          // 1: sum the spline potential into ax instead (shmem::potential, FP64)
+    // tidal tensor (6 x nt, no G) and jerk (3 x nt), FP64 only; NULL when not wanted. The jerk
+    // needs particle velocities (pv*, by particle) and node velocities (nvel, 3 x nnodes).
+    double* tidal;
+    double* jerk;
+    const double *pvx, *pvy, *pvz;
+    const double* nvel;
+    const uint32_t* orderbuf;
+    int nnodes;
     int* queue;             // persistent launch: eight work counters, one per XCD (zeroed)
     int* status;            // bit 0: a group's global node stack overflowed too
     long long* counts;      // nodes visited, multipole, pairs, max stack depth (COUNT builds)
@@ -195,6 +203,7 @@ gz_status launch_kick_nodes(gz_tree* t, const uint32_t* active, size_t na, const
                             const double* vy, const double* vz, const double* m, double* vlx,
                             double* vly, double* vlz, double tau, int* status, hipStream_t st);
 gz_status launch_refresh(gz_tree* t, double tau, hipStream_t st);
+gz_status launch_node_vel(const gz_tree* t, double* out, hipStream_t st);
 gz_status launch_raise_soft(gz_tree* t, const uint32_t* idx, size_t n_idx, const double* soft,
                             int* status, hipStream_t st);
 

@@ -175,6 +175,30 @@ __global__ __launch_bounds__(256) void k_raise_soft(const uint32_t* __restrict__
     }
 }
 
+// Tree::node_vel (tree.h:323-331): each node's centre-of-mass velocity corrected for the kicks
+// since the build, (m0 vcom + dp) / M, for the jerk's multipole terms. Out: 3 x nn SoA.
+__global__ __launch_bounds__(256) void k_node_vel(int nn, const double* __restrict__ vcom,
+                                                  const double* __restrict__ m0,
+                                                  const double* __restrict__ dp,
+                                                  const double* __restrict__ dmass,
+                                                  double* __restrict__ out) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= nn) return;
+    const size_t n = (size_t)nn;
+    const double mass = m0[i];
+    const double M = mass + dmass[i];
+    for (int d = 0; d < 3; ++d)
+        out[d * n + i] = (M > 0) ? (mass * vcom[d * n + i] + dp[d * n + i]) / M : vcom[d * n + i];
+}
+
+gz_status launch_node_vel(const gz_tree* t, double* out, hipStream_t st) {
+    if (t->nnodes == 0 || !t->has_vel) return GZ_OK;
+    k_node_vel<<<(unsigned)((t->nnodes + 255) / 256), 256, 0, st>>>(t->nnodes, t->vcom, t->m0,
+                                                                    t->dp, t->dmass, out);
+    GZ_HIP(hipGetLastError());
+    return GZ_OK;
+}
+
 gz_status launch_kick_nodes(gz_tree* t, const uint32_t* active, size_t na, const double* vx,
                             const double* vy, const double* vz, const double* m, double* vlx,
                             double* vly, double* vlz, double tau, int* status, hipStream_t st) {

@@ -62,6 +62,19 @@ __device__ __forceinline__ double spline_potential_d(double r, double h) {
                     u * u * (10.666666666666667 + u * (-16.0 + u * (9.6 - 2.1333333333333333 * u))));
 }
 
+// grav_tidal_factor (tree.h:86-96), operand for operand: the second kernel derivative, 3/r^5
+// beyond h.
+__device__ __forceinline__ double tidal_factor_d(double r, double h) {
+    const double h_inv = 1.0 / h;
+    const double u = r * h_inv;
+    if (u >= 1.0) return 3.0 / (r * r * r * r * r);
+    const double h_inv5 = h_inv * h_inv * h_inv * h_inv * h_inv;
+    double wk;
+    if (u < 0.5) wk = 76.8 - 96.0 * u;
+    else wk = -0.2 / (u * u * u * u * u) + 48.0 / u - 76.8 + 32.0 * u;
+    return wk * h_inv5;
+}
+
 // open_node (tree.h:109-144), operand for operand.
 __device__ __forceinline__ bool open_node_d(double r_sq, double len, double l_open, double mass,
                                             double node_soft, double soft_t, double aold,
@@ -289,8 +302,11 @@ __device__ __forceinline__ float spline_inner_f(float r, float h) {
 }
 
 // At most 128 VGPRs, so that four waves fit on each SIMD (the LDS footprint allows four too).
-template <bool FP64, bool COUNT, bool GSTACK>
+// TJ (FP64 only): the tidal tensor and, when velocities are given, the jerk, accumulated in the
+// same walk (tree.cc:811-871).
+template <bool FP64, bool COUNT, bool GSTACK, bool TJ = false>
 __global__ __launch_bounds__(64, 4) void k_walk(WalkArgs a) {
+    static_assert(!TJ || FP64, "the tidal tensor and jerk are FP64 walks");
     using real = typename WalkTypes<FP64>::real;
     using real4 = typename WalkTypes<FP64>::real4;
     constexpr int kMono = FP64 ? kMonoD : kMonoF;
@@ -304,6 +320,9 @@ __global__ __launch_bounds__(64, 4) void k_walk(WalkArgs a) {
     __shared__ real4 s_mono[kMono];
     __shared__ real4 s_pair[kPair];
     __shared__ real s_psoft[kPair];
+    // TJ: node velocity and softening per monopole, source velocity per particle entry
+    __shared__ double4 s_mext[TJ ? kMono : 1];
+    __shared__ double4 s_pext[TJ ? kPair : 1];
     __shared__ int s_lpre[65];
     __shared__ int s_lplo[64];
     int* const stk = GSTACK ? a.gstack + (size_t)blockIdx.x * (size_t)a.gstack_cap : s_stack;
@@ -394,13 +413,66 @@ __global__ __launch_bounds__(64, 4) void k_walk(WalkArgs a) {
                 te = (real)pe;
             }
             double accx = 0, accy = 0, accz = 0;
+            double tt[TJ ? 6 : 1] = {0}, jj[TJ ? 3 : 1] = {0}; // tidal (xx yy zz xy yz xz), jerk
+            double tvx = 0, tvy = 0, tvz = 0;
+            if (TJ && tvalid && a.jerk) {
+                const uint32_t p = a.targets[tlo + t];
+                tvx = a.pvx[p]; tvy = a.pvy[p]; tvz = a.pvz[p];
+            }
             int mcount = 0, pcount = 0;
             bool overflow = false;
 
             // Evaluate both lists against every target of the chunk.
             auto flush = [&]() {
                 __syncthreads();
-                if (FP64 && a.potential) {
+                if (TJ) {
+                    // tree.cc:811-871: g1 and g2 are the first and second kernel-derivative
+                    // factors; the tidal tensor is -g1 delta + g2 d d (no G), the jerk
+                    // g1 dv - (dv . d) g2 d
+                    auto tj = [&](double dx, double dy, double dz, double g1, double g2,
+                                  const double4& v) {
+                        accx += dx * g1;
+                        accy += dy * g1;
+                        accz += dz * g1;
+                        tt[0] += -g1 + dx * dx * g2;
+                        tt[1] += -g1 + dy * dy * g2;
+                        tt[2] += -g1 + dz * dz * g2;
+                        tt[3] += dx * dy * g2;
+                        tt[4] += dy * dz * g2;
+                        tt[5] += dx * dz * g2;
+                        if (a.jerk) {
+                            const double dvx = v.x - tvx, dvy = v.y - tvy, dvz = v.z - tvz;
+                            const double vdotr = dvx * dx + dvy * dy + dvz * dz;
+                            jj[0] += g1 * dvx - vdotr * g2 * dx;
+                            jj[1] += g1 * dvy - vdotr * g2 * dy;
+                            jj[2] += g1 * dvz - vdotr * g2 * dz;
+                        }
+                    };
+                    for (int j = jstart; j < mcount; j += L) {
+                        const real4 s = s_mono[j];
+                        const double4 v = s_mext[j];
+                        const double dx = (double)s.x - (double)tx, dy = (double)s.y - (double)ty,
+                                     dz = (double)s.z - (double)tz;
+                        const double r2 = dx * dx + dy * dy + dz * dz;
+                        if (r2 <= 0) continue;
+                        const double r = sqrt(r2);
+                        const double es = dmax_w(dmax_w((double)te, v.w), 1e-300);
+                        tj(dx, dy, dz, (double)s.w * spline_force_over_r_d(r, es),
+                           (double)s.w * tidal_factor_d(r, es), v);
+                    }
+                    for (int j = jstart; j < pcount; j += L) {
+                        const real4 s = s_pair[j];
+                        const double es = (double)s_psoft[j];
+                        const double dx = (double)s.x - (double)tx, dy = (double)s.y - (double)ty,
+                                     dz = (double)s.z - (double)tz;
+                        const double r2 = dx * dx + dy * dy + dz * dz;
+                        if (r2 <= 0) continue;
+                        const double r = sqrt(r2);
+                        const double h = dmax_w(dmax_w((double)te, es), 1e-300);
+                        tj(dx, dy, dz, (double)s.w * spline_force_over_r_d(r, h),
+                           (double)s.w * tidal_factor_d(r, h), s_pext[j]);
+                    }
+                } else if (FP64 && a.potential) {
                     // shmem::potential (tree.cc:899-944): an accepted node lies outside both
                     // softenings (open_node's first test), where the spline is exactly -1/r
                     for (int j = jstart; j < mcount; j += L) {
@@ -603,7 +675,20 @@ __global__ __launch_bounds__(64, 4) void k_walk(WalkArgs a) {
                 const int nm = __popcll(mb);
                 if (nm > 0) {
                     if (mcount + nm > kMono) flush();
-                    if (accept) s_mono[mcount + __popcll(mb & lanemask_lt())] = me;
+                    if (accept) {
+                        const int slot_m = mcount + __popcll(mb & lanemask_lt());
+                        s_mono[slot_m] = me;
+                        if (TJ) {
+                            double4 ext = make_double4(0.0, 0.0, 0.0, (double)w.soft);
+                            if (a.jerk) {
+                                const size_t nn = (size_t)a.nnodes;
+                                ext.x = a.nvel[node];
+                                ext.y = a.nvel[nn + node];
+                                ext.z = a.nvel[2 * nn + node];
+                            }
+                            s_mext[slot_m] = ext;
+                        }
+                    }
                     if (COUNT && tvalid && sub == 0) g_multi += nm;
                     mcount += nm;
                 }
@@ -645,6 +730,14 @@ __global__ __launch_bounds__(64, 4) void k_walk(WalkArgs a) {
                             e.w = (real)sp4.w;
                             s_pair[pcount + j] = e;
                             s_psoft[pcount + j] = (real)a.src_soft[q];
+                            if (TJ) {
+                                double4 v = make_double4(0.0, 0.0, 0.0, 0.0);
+                                if (a.jerk) {
+                                    const uint32_t pq = a.orderbuf[q];
+                                    v.x = a.pvx[pq]; v.y = a.pvy[pq]; v.z = a.pvz[pq];
+                                }
+                                s_pext[pcount + j] = v;
+                            }
                         }
                         pcount += ncopy;
                         done += ncopy;
@@ -675,6 +768,27 @@ __global__ __launch_bounds__(64, 4) void k_walk(WalkArgs a) {
                     accx += ux;
                     accy += uy;
                     accz += uz;
+                }
+                if (TJ) {
+#pragma unroll
+                    for (int c = 0; c < 6; ++c) {
+                        const double u = __shfl(tt[c], src);
+                        if (sub == 0) tt[c] += u;
+                    }
+#pragma unroll
+                    for (int c = 0; c < 3; ++c) {
+                        const double u = __shfl(jj[c], src);
+                        if (sub == 0) jj[c] += u;
+                    }
+                }
+            }
+            if (TJ && tvalid && sub == 0) {
+                const size_t ntt = (size_t)a.nt, slot = (size_t)(tlo + t);
+#pragma unroll
+                for (int c = 0; c < 6; ++c) a.tidal[c * ntt + slot] = tt[c];
+                if (a.jerk) {
+#pragma unroll
+                    for (int c = 0; c < 3; ++c) a.jerk[c * ntt + slot] = a.G * jj[c];
                 }
             }
             if (tvalid && sub == 0) {
@@ -800,6 +914,7 @@ gz_status ctx_spill(gz_ctx* ctx, int** spill, int* cap) {
 
 gz_status launch_walk(const WalkArgs& a, bool fp64, bool count, hipStream_t st) {
     if (a.ngroups <= 0) return GZ_OK;
+    const bool tj = a.tidal != nullptr;
     unsigned grid;
     if (a.persistent) {
         int waves = 0;
@@ -808,7 +923,9 @@ gz_status launch_walk(const WalkArgs& a, bool fp64, bool count, hipStream_t st) 
     } else {
         grid = (unsigned)a.ngroups;
     }
-    if (fp64) {
+    if (tj) {
+        k_walk<true, false, false, true><<<grid, 64, 0, st>>>(a);
+    } else if (fp64) {
         if (count) k_walk<true, true, false><<<grid, 64, 0, st>>>(a);
         else k_walk<true, false, false><<<grid, 64, 0, st>>>(a);
     } else {
@@ -826,7 +943,9 @@ gz_status launch_walk_retry(const WalkArgs& a, int waves, bool fp64, bool count,
                             hipStream_t st) {
     if (waves <= 0) return GZ_OK;
     const int n_retry = waves;
-    if (fp64) {
+    if (a.tidal) {
+        k_walk<true, false, true, true><<<n_retry, 64, 0, st>>>(a);
+    } else if (fp64) {
         if (count) k_walk<true, true, true><<<n_retry, 64, 0, st>>>(a);
         else k_walk<true, false, true><<<n_retry, 64, 0, st>>>(a);
     } else {

@@ -487,6 +487,27 @@ class Graph:
             self.handle = None
 
 
+def accel_grouped_tidal(T: Tree, P: Particles, targets, theta: float, G: float = 1.0,
+                        batch: int = 0, aold=None, vel=None, jerk: bool = False):
+    """shmem::accel_grouped with the tidal tensor (and the jerk when vel is given and the tree
+    was built with velocities), FP64 (tree.h:414-440). Returns (ax, ay, az, tidal (6, nt),
+    jerk (3, nt) or None); the tidal tensor excludes G, as the reference's does."""
+    tg = _dev_u32(targets, P.device)
+    nt = int(tg.shape[0])
+    ax, ay, az = _outputs(nt, P.device)
+    tid = torch.empty((6, nt), dtype=torch.float64, device=P.device)
+    jk = torch.empty((3, nt), dtype=torch.float64, device=P.device) if jerk else None
+    ao = None if aold is None else _dev_f64(aold, P.device)
+    v = (None, None, None) if vel is None else tuple(_dev_f64(a, P.device) for a in vel)
+    T.ctx.enter()
+    L.check(L.load().gz_tree_accel_grouped_tidal(
+        T.ctx.handle, T.handle, P.cstruct(), _ptr(tg), nt, float(theta), float(G), int(batch),
+        _ptr(ao), *(_ptr(a) for a in v), _ptr(ax), _ptr(ay), _ptr(az), _ptr(tid), _ptr(jk)),
+        "gz_tree_accel_grouped_tidal")
+    T.ctx.synchronize()
+    return ax, ay, az, tid, jk
+
+
 def potential(T: Tree, P: Particles, targets, theta: float, G: float = 1.0, aold=None):
     """shmem::potential (tree.h:446-447): spline-softened potential per target (FP64)."""
     tg = _dev_u32(targets, P.device)

@@ -236,6 +236,17 @@ gz_status gz_tree_accel(gz_ctx* ctx, const gz_tree* tree, const gz_particles* p,
                         const uint32_t* targets, size_t n_t, double theta, double G,
                         const double* aold, double* ax, double* ay, double* az);
 
+/* shmem::accel_grouped with the tidal tensor and the jerk (tree.h:414-440, tree.cc:811-871),
+ * FP64 with the reference criterion and grouping: tidal (6 x n_t doubles, xx yy zz xy yz xz,
+ * component-major, WITHOUT G as the reference leaves it) and, when jerk and vx/vy/vz (particle
+ * velocities, device) are given and the tree was built with velocities, the jerk (3 x n_t, G
+ * included); otherwise the jerk is zero. */
+gz_status gz_tree_accel_grouped_tidal(gz_ctx* ctx, const gz_tree* tree, const gz_particles* p,
+                                      const uint32_t* targets, size_t n_t, double theta, double G,
+                                      int batch, const double* aold, const double* vx,
+                                      const double* vy, const double* vz, double* ax, double* ay,
+                                      double* az, double* tidal, double* jerk);
+
 /* shmem::potential (tree.h:446-447, tree.cc:899-944): the spline-softened potential at each
  * target, G included, walked per target like gz_tree_accel (leaves always summed), FP64. */
 gz_status gz_tree_potential(gz_ctx* ctx, const gz_tree* tree, const gz_particles* p,

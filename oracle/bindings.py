@@ -190,6 +190,9 @@ class Oracle(_Lib):
                                 C.c_double, C.c_double, _dp, _dp, _dp, _dp]
         L.gzo_accel_brute.argtypes = [C.POINTER(_GzoParticles), _up, C.c_size_t, C.c_double,
                                       _dp, _dp, _dp]
+        L.gzo_accel_grouped_tj.argtypes = [C.c_void_p, C.POINTER(_GzoParticles), _up,
+                                           C.c_size_t, C.c_double, C.c_double, C.c_int, _dp,
+                                           _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _llp]
         L.gzo_potential.argtypes = [C.c_void_p, C.POINTER(_GzoParticles), _up, C.c_size_t,
                                     C.c_double, C.c_double, _dp, _dp]
         L.gzo_build_vel.restype = C.c_void_p
@@ -247,6 +250,23 @@ class Oracle(_Lib):
         if want_counts:
             return ax, ay, az, counts
         return ax, ay, az
+
+    def accel_grouped_tj(self, t, P: HostParticles, targets, theta, G=1.0, batch=0, aold=None,
+                         vel=None, jerk=False):
+        """(ax, ay, az, tidal (6, nt), jerk (3, nt) or None)."""
+        tg = np.ascontiguousarray(targets, dtype=np.uint32)
+        nt = tg.shape[0]
+        ax, ay, az = (np.zeros(nt) for _ in range(3))
+        tid = np.zeros((6, nt))
+        jk = np.zeros((3, nt)) if jerk else None
+        ao = None if aold is None else _f64(aold)
+        v = (None, None, None) if vel is None else tuple(_f64(a) for a in vel)
+        cp = self._cp(P)
+        self.lib.gzo_accel_grouped_tj(t, C.byref(cp), _ptr(tg, _up), nt, theta, G, batch,
+                                      _ptr(ao, _dp), *(_ptr(a, _dp) for a in v), _ptr(ax, _dp),
+                                      _ptr(ay, _dp), _ptr(az, _dp), _ptr(tid, _dp),
+                                      _ptr(jk, _dp), None)
+        return ax, ay, az, tid, jk
 
     def potential(self, t, P: HostParticles, targets, theta, G=1.0, aold=None):
         tg = np.ascontiguousarray(targets, dtype=np.uint32)
@@ -328,6 +348,9 @@ class Reference(_Lib):
         L.gzref_accel_brute.argtypes = [C.c_void_p, _up, C.c_size_t, C.c_double, _dp, _dp, _dp]
         L.gzref_audit.restype = C.c_longlong
         L.gzref_audit.argtypes = [C.c_void_p, C.c_void_p, C.c_char_p, C.c_size_t]
+        L.gzref_accel_grouped_tj.argtypes = [C.c_void_p, C.c_void_p, _up, C.c_size_t,
+                                             C.c_double, C.c_double, C.c_int, _dp, _dp, _dp, _dp,
+                                             _dp, _dp, _dp, _dp, _dp]
         L.gzref_build_vel.restype = C.c_void_p
         L.gzref_build_vel.argtypes = [C.c_void_p, C.c_longlong, _dp, _dp, _dp]
         L.gzref_kick_nodes.argtypes = [C.c_void_p, _up, C.c_size_t, _dp, _dp, _dp, _dp, _dp, _dp,
@@ -399,6 +422,20 @@ class Reference(_Lib):
         self.lib.gzref_accel_grouped(t, rp, _ptr(tg, _up), nt, theta, G, batch, _ptr(ao, _dp),
                                      _ptr(ax, _dp), _ptr(ay, _dp), _ptr(az, _dp))
         return ax, ay, az
+
+    def accel_grouped_tj(self, t, rp, targets, theta, G=1.0, batch=0, aold=None, vel=None,
+                         jerk=False):
+        tg = np.ascontiguousarray(targets, dtype=np.uint32)
+        nt = tg.shape[0]
+        ax, ay, az = (np.zeros(nt) for _ in range(3))
+        tid = np.zeros((6, nt))
+        jk = np.zeros((3, nt)) if jerk else None
+        ao = None if aold is None else _f64(aold)
+        v = (None, None, None) if vel is None else tuple(_f64(a) for a in vel)
+        self.lib.gzref_accel_grouped_tj(t, rp, _ptr(tg, _up), nt, theta, G, batch, _ptr(ao, _dp),
+                                        *(_ptr(a, _dp) for a in v), _ptr(ax, _dp), _ptr(ay, _dp),
+                                        _ptr(az, _dp), _ptr(tid, _dp), _ptr(jk, _dp))
+        return ax, ay, az, tid, jk
 
     def accel(self, t, rp, targets, theta, G=1.0, aold=None):
         tg = np.ascontiguousarray(targets, dtype=np.uint32)

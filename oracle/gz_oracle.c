@@ -471,10 +471,45 @@ static inline double soft_of(const gzo_particles* P, size_t i) { return P->soft 
 /* Grouped walk (tree.cc:626-897) without tidal/jerk/lazy/sink terms. Groups are runs of targets
  * sharing a leaf when batch <= 0, fixed windows of `batch` otherwise. Opening is decided once per
  * group against its bounding box, largest softening and smallest aold. */
+/* grav_tidal_factor (tree.h:86-96): the second kernel derivative, 3/r^5 beyond h. */
+static double tidal_factor(double r, double h) {
+    const double h_inv = 1.0 / h;
+    const double u = r * h_inv;
+    if (u >= 1.0) return 3.0 / (r * r * r * r * r);
+    const double h_inv5 = h_inv * h_inv * h_inv * h_inv * h_inv;
+    double wk;
+    if (u < 0.5) wk = 76.8 - 96.0 * u;
+    else wk = -0.2 / (u * u * u * u * u) + 48.0 / u - 76.8 + 32.0 * u;
+    return wk * h_inv5;
+}
+
+/* Tree::node_vel (tree.h:323-331): centre-of-mass velocity corrected for kicks. */
+static void node_vel(const otree* T, int no, double* v) {
+    const double M = T->mass[no] + (T->dmass ? T->dmass[no] : 0.0);
+    if (!T->dp_x || M <= 0) { v[0] = T->vcom_x[no]; v[1] = T->vcom_y[no]; v[2] = T->vcom_z[no]; return; }
+    const double m0 = T->mass[no];
+    v[0] = (m0 * T->vcom_x[no] + T->dp_x[no]) / M;
+    v[1] = (m0 * T->vcom_y[no] + T->dp_y[no]) / M;
+    v[2] = (m0 * T->vcom_z[no] + T->dp_z[no]) / M;
+}
+
 void gzo_accel_grouped(void* h, const gzo_particles* P, const uint32_t* targets, size_t nt,
                        double theta, double G, int batch, const double* aold, double* ax,
                        double* ay, double* az, long long* counts) {
+    gzo_accel_grouped_tj(h, P, targets, nt, theta, G, batch, aold, NULL, NULL, NULL, ax, ay, az,
+                         NULL, NULL, counts);
+}
+
+/* ... with the tidal tensor (6 x nt, xx yy zz xy yz xz, no G) and the jerk (3 x nt, G included;
+ * needs velocities and a tree built with them), accumulated in the same walk (tree.cc:811-871). */
+void gzo_accel_grouped_tj(void* h, const gzo_particles* P, const uint32_t* targets, size_t nt,
+                          double theta, double G, int batch, const double* aold, const double* vx,
+                          const double* vy, const double* vz, double* ax, double* ay,
+                          double* az, double* tidal, double* jerk, long long* counts) {
     const otree* T = (const otree*)h;
+    const int want_tidal = tidal != NULL;
+    const int want_jerk = jerk != NULL && vx != NULL && T->vcom_x != NULL;
+    if (jerk && !want_jerk) memset(jerk, 0, 3 * nt * sizeof(double));
     const double theta2 = theta * theta;
     size_t ngroups = 0;
     size_t* gstart = (size_t*)malloc((nt + 2) * sizeof(size_t));
@@ -507,13 +542,17 @@ void gzo_accel_grouped(void* h, const gzo_particles* P, const uint32_t* targets,
         }
         double amin = aold ? 1e300 : 0.0;
         if (aold) for (size_t i = lo; i < hi; ++i) amin = dmin(amin, aold[i]);
-        double* buf = (double*)malloc((size_t)nb * 7 * sizeof(double));
+        double* buf = (double*)calloc((size_t)nb * 19, sizeof(double));
         double *tx = buf, *ty = buf + nb, *tz = buf + 2 * nb, *te = buf + 3 * nb;
         double *oax = buf + 4 * nb, *oay = buf + 5 * nb, *oaz = buf + 6 * nb;
+        double* ott = buf + 7 * nb;                 /* 6 x nb */
+        double *tvx = buf + 13 * nb, *tvy = buf + 14 * nb, *tvz = buf + 15 * nb;
+        double *ojx = buf + 16 * nb, *ojy = buf + 17 * nb, *ojz = buf + 18 * nb;
         for (int i = 0; i < nb; ++i) {
             const uint32_t p = targets[lo + i];
             tx[i] = P->x[p]; ty[i] = P->y[p]; tz[i] = P->z[p]; te[i] = soft_of(P, p);
             oax[i] = 0; oay[i] = 0; oaz[i] = 0;
+            if (want_jerk) { tvx[i] = vx[p]; tvy[i] = vy[p]; tvz[i] = vz[p]; }
         }
         int node = T->root;
         while (node >= 0) {
@@ -532,6 +571,8 @@ void gzo_accel_grouped(void* h, const gzo_particles* P, const uint32_t* targets,
                         const uint32_t q = T->orderbuf[k];
                         const double qx = P->x[q], qy = P->y[q], qz = P->z[q], qm = P->m[q];
                         const double qs = soft_of(P, q);
+                        double qvx = 0, qvy = 0, qvz = 0;
+                        if (want_jerk) { qvx = vx[q]; qvy = vy[q]; qvz = vz[q]; }
                         for (int i = 0; i < nb; ++i) {
                             if (targets[lo + i] == q) continue;
                             const double ddx = qx - tx[i], ddy = qy - ty[i], ddz = qz - tz[i];
@@ -542,10 +583,32 @@ void gzo_accel_grouped(void* h, const gzo_particles* P, const uint32_t* targets,
                             const double fac =
                                 qm * gzo_spline_force_over_r(r, dmax(dmax(te[i], qs), 1e-300));
                             oax[i] += ddx * fac; oay[i] += ddy * fac; oaz[i] += ddz * fac;
+                            if (want_tidal || want_jerk) { /* tree.cc:811-837 */
+                                const double g1 = fac;
+                                const double g2 = qm * tidal_factor(r, dmax(dmax(te[i], qs), 1e-300));
+                                if (want_tidal) {
+                                    ott[i] += -g1 + ddx * ddx * g2;
+                                    ott[nb + i] += -g1 + ddy * ddy * g2;
+                                    ott[2 * nb + i] += -g1 + ddz * ddz * g2;
+                                    ott[3 * nb + i] += ddx * ddy * g2;
+                                    ott[4 * nb + i] += ddy * ddz * g2;
+                                    ott[5 * nb + i] += ddx * ddz * g2;
+                                }
+                                if (want_jerk) {
+                                    const double dvx = qvx - tvx[i], dvy = qvy - tvy[i],
+                                                 dvz = qvz - tvz[i];
+                                    const double vdotr = dvx * ddx + dvy * ddy + dvz * ddz;
+                                    ojx[i] += g1 * dvx - vdotr * g2 * ddx;
+                                    ojy[i] += g1 * dvy - vdotr * g2 * ddy;
+                                    ojz[i] += g1 * dvz - vdotr * g2 * ddz;
+                                }
+                            }
                         }
                     }
                 } else {
                     multipole += nb;
+                    double nv[3] = {0, 0, 0};
+                    if (want_jerk) node_vel(T, node, nv);
                     for (int i = 0; i < nb; ++i) {
                         const double e = dmax(te[i], wsoft);
                         const double ox = wcx - tx[i], oy = wcy - ty[i], oz = wcz - tz[i];
@@ -554,6 +617,27 @@ void gzo_accel_grouped(void* h, const gzo_particles* P, const uint32_t* targets,
                         const double f = T->wmass[node] *
                                          gzo_spline_force_over_r(sqrt(r2), dmax(e, 1e-300));
                         oax[i] += ox * f; oay[i] += oy * f; oaz[i] += oz * f;
+                        if (want_tidal || want_jerk) { /* tree.cc:845-871 */
+                            const double r = sqrt(r2), es = dmax(e, 1e-300);
+                            const double g1 = T->wmass[node] * gzo_spline_force_over_r(r, es);
+                            const double g2 = T->wmass[node] * tidal_factor(r, es);
+                            if (want_tidal) {
+                                ott[i] += -g1 + ox * ox * g2;
+                                ott[nb + i] += -g1 + oy * oy * g2;
+                                ott[2 * nb + i] += -g1 + oz * oz * g2;
+                                ott[3 * nb + i] += ox * oy * g2;
+                                ott[4 * nb + i] += oy * oz * g2;
+                                ott[5 * nb + i] += ox * oz * g2;
+                            }
+                            if (want_jerk) {
+                                const double dvx = nv[0] - tvx[i], dvy = nv[1] - tvy[i],
+                                             dvz = nv[2] - tvz[i];
+                                const double vdotr = dvx * ox + dvy * oy + dvz * oz;
+                                ojx[i] += g1 * dvx - vdotr * g2 * ox;
+                                ojy[i] += g1 * dvy - vdotr * g2 * oy;
+                                ojz[i] += g1 * dvz - vdotr * g2 * oz;
+                            }
+                        }
                     }
                 }
                 node = T->next[node];
@@ -563,6 +647,12 @@ void gzo_accel_grouped(void* h, const gzo_particles* P, const uint32_t* targets,
         }
         for (int i = 0; i < nb; ++i) {
             ax[lo + i] = G * oax[i]; ay[lo + i] = G * oay[i]; az[lo + i] = G * oaz[i];
+            if (want_tidal)
+                for (int c = 0; c < 6; ++c) tidal[c * nt + lo + i] = ott[c * nb + i];
+            if (want_jerk) {
+                jerk[lo + i] = G * ojx[i]; jerk[nt + lo + i] = G * ojy[i];
+                jerk[2 * nt + lo + i] = G * ojz[i];
+            }
         }
         free(buf);
     }

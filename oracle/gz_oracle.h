@@ -14,6 +14,9 @@
  *   gzo_accel_grouped     bit for bit against the reference accel_grouped (both regimes), and
  *                         theta -> 0 against gzo_accel_brute
  *   gzo_accel             bit for bit against the reference accel
+ *   gzo_accel_grouped_tj  tidal tensor and jerk bit for bit against the reference, and against
+ *                         finite differences of the direct-sum acceleration (in space and
+ *                         along ballistic motion)
  *   gzo_potential         bit for bit against the reference potential, and theta -> 0 against
  *                         the direct sum of the spline potential
  *   gzo_accel_brute       bit for bit against the reference accel_brute, Newton's third law, and
@@ -83,6 +86,12 @@ void gzo_tree_skeys(void* t, uint64_t* skey);
 void gzo_accel_grouped(void* t, const gzo_particles* P, const uint32_t* targets, size_t nt,
                        double theta, double G, int batch, const double* aold, double* ax,
                        double* ay, double* az, long long* counts);
+/* With the tidal tensor (tidal: 6 x nt, xx yy zz xy yz xz, no G) and the jerk (jerk: 3 x nt, G
+ * included; needs vx/vy/vz and a tree built with velocities) (tree.cc:811-871). */
+void gzo_accel_grouped_tj(void* t, const gzo_particles* P, const uint32_t* targets, size_t nt,
+                          double theta, double G, int batch, const double* aold, const double* vx,
+                          const double* vy, const double* vz, double* ax, double* ay,
+                          double* az, double* tidal, double* jerk, long long* counts);
 void gzo_accel(void* t, const gzo_particles* P, const uint32_t* targets, size_t nt, double theta,
                double G, const double* aold, double* ax, double* ay, double* az);
 void gzo_potential(void* t, const gzo_particles* P, const uint32_t* targets, size_t nt,

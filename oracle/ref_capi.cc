@@ -136,6 +136,38 @@ void gzref_accel_grouped(void* t, void* particles, const uint32_t* targets, size
     std::memcpy(az, vz.data(), nt * sizeof(double));
 }
 
+// shmem::accel_grouped with the tidal tensor and, when vx is given, the jerk (tree.h:414-440).
+// tidal: 6 x nt (xx yy zz xy yz xz, component-major); jerk: 3 x nt; either may be NULL.
+void gzref_accel_grouped_tj(void* t, void* particles, const uint32_t* targets, size_t nt,
+                            double theta, double G, int batch, const double* aold,
+                            const double* vx, const double* vy, const double* vz, double* ax,
+                            double* ay, double* az, double* tidal, double* jerk) {
+    auto* rt = static_cast<RefTree*>(t);
+    auto* rp = static_cast<RefParticles*>(particles);
+    std::vector<uint32_t> tg(targets, targets + nt);
+    std::vector<double> ox, oy, oz;
+    std::vector<SymTensor3d> tt;
+    std::vector<Vec3d> jj;
+    const double* vel[3] = {vx, vy, vz};
+    accel_grouped(rt->T, rp->P, tg, theta, G, batch, ox, oy, oz, tidal ? &tt : nullptr, aold,
+                  nullptr, jerk ? &jj : nullptr, vx ? vel : nullptr);
+    std::memcpy(ax, ox.data(), nt * sizeof(double));
+    std::memcpy(ay, oy.data(), nt * sizeof(double));
+    std::memcpy(az, oz.data(), nt * sizeof(double));
+    for (size_t i = 0; i < nt; ++i) {
+        if (tidal) {
+            const SymTensor3d& q = tt[i];
+            tidal[i] = q[0][0]; tidal[nt + i] = q[1][1]; tidal[2 * nt + i] = q[2][2];
+            tidal[3 * nt + i] = q[0][1]; tidal[4 * nt + i] = q[1][2]; tidal[5 * nt + i] = q[0][2];
+        }
+        if (jerk) {
+            jerk[i] = jj.empty() ? 0.0 : jj[i][0];
+            jerk[nt + i] = jj.empty() ? 0.0 : jj[i][1];
+            jerk[2 * nt + i] = jj.empty() ? 0.0 : jj[i][2];
+        }
+    }
+}
+
 // shmem::accel (tree.h:392-394, tree.cc:528-582), the per-target walk.
 void gzref_accel(void* t, void* particles, const uint32_t* targets, size_t nt, double theta,
                  double G, const double* aold, double* ax, double* ay, double* az) {

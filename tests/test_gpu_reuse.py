@@ -134,3 +134,41 @@ def test_kick_nodes_rejects_bad_index(gz):
     v = tuple(np.zeros(1000) for _ in range(3))
     with pytest.raises(Exception, match="outside"):
         T.kick_nodes(np.array([5, 1000], dtype=np.uint32), v)
+
+
+@pytest.mark.parametrize("batch", [0, 8])
+def test_tidal_and_jerk_match_oracle(gz, oracle, batch):
+    """The tidal tensor and jerk of the FP64 grouped walk (tree.cc:811-871) against the oracle
+    (pinned to the reference and to finite differences of the direct sum), on a tree built with
+    velocities and fed one kick."""
+    from oracle.bindings import HostParticles
+    pos, v0, kicks, _ = _scenario(n=5000, seed=43)
+    hp = HostParticles(*pos)
+    P = gz.Particles(*pos)
+    T = gz.build(P, vel=v0)
+    to = oracle.build_vel(hp, v0)
+    try:
+        tk, act, vk = kicks[0]
+        T.kick_nodes(act, vk, None, tuple(_dev(a.copy()) for a in v0), tk)
+        oracle.kick_nodes(to, act, vk, pos[3], tuple(a.copy() for a in v0), tk)
+        tg = oracle.tree(to, hp).orderbuf
+        for theta, aold in ((0.7, None), (0.0, np.full(tg.shape[0], 1e-3))):
+            g = gz.accel_grouped_tidal(T, P, tg, theta, 1.3, batch, aold, vel=vk, jerk=True)
+            o = oracle.accel_grouped_tj(to, hp, tg, theta, 1.3, batch, aold, vk, True)
+            acc_g = np.stack([_np(v) for v in g[:3]], 1)
+            acc_o = np.stack(o[:3], 1)
+            e = np.linalg.norm(acc_g - acc_o, axis=1) / np.linalg.norm(acc_o, axis=1)
+            assert e.max() < 1e-12
+            tid_g, tid_o = _np(g[3]), o[3]
+            scale = np.abs(tid_o).max(0)
+            assert np.all(np.abs(tid_g - tid_o) <= 1e-11 * scale), \
+                np.max(np.abs(tid_g - tid_o) / scale)
+            jk_g, jk_o = _np(g[4]), o[4]
+            jscale = np.linalg.norm(jk_o, axis=0)
+            assert np.all(np.linalg.norm(jk_g - jk_o, axis=0) <= 1e-10 * jscale)
+    finally:
+        oracle.free(to)
+
+
+def _np(t):
+    return t.detach().cpu().numpy()
