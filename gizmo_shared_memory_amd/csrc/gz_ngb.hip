@@ -1,0 +1,292 @@
+// Neighbour search over the tree (shmem/hydro.cc:37-108, the SNode of tree.h:206-226): every
+// particle within radius_q of centre q, for many centres at once.
+//
+// One wavefront per group of 64 consecutive queries (a Morton-ordered query list makes the groups
+// compact). A node is skipped when it lies farther from the group's bounding box than the group's
+// largest radius plus the node's half side (widened, as the reference's SNode is, for float
+// centres) plus vmax * tau + xpad — the reference's prune taken against the box instead of one
+// centre, so it skips nothing any query of the group needs. Each lane then tests its own query
+// against the particles of every reached leaf with the reference's FP64 expression, so the sets
+// are exact. Two passes: counts, then (with the offsets) the sorted slots of each query's
+// neighbours, which map to particle indices in Morton order, the reference's traversal order.
+#include <cfloat>
+
+#include "gz_features.h"
+#include "gz_internal.h"
+
+namespace gz {
+
+struct NgbArgs {
+    const WNode* wn;
+    const double* gcx;
+    const double* gcy;
+    const double* gcz;
+    const double* size;
+    const float* vmax;
+    const float* xpad;
+    const double4* src;
+    const uint32_t* orderbuf;
+    const double *cx, *cy, *cz, *radius;
+    int nq, ngroups, root, nclusters;
+    double snode_eps, tau;
+    int* queue;
+    int* stack; // stack_cap ints per wave
+    int stack_cap;
+    int* status; // bit 0: a stack outgrew stack_cap
+    uint32_t* count;     // pass 1: neighbours per query
+    const size_t* start; // pass 2: CSR offsets
+    uint32_t* slots;     // pass 2: the neighbours' sorted slots
+};
+
+// The search node's half side (tree.cc:496-506): float-safe, never smaller than the true one.
+__device__ __forceinline__ float snode_half(const NgbArgs& a, int i) {
+    const double cmax = fmax(fabs(a.gcx[i]), fmax(fabs(a.gcy[i]), fabs(a.gcz[i])));
+    return (float)(0.5 * a.size[i] + a.snode_eps + 4.0 * (double)FLT_EPSILON * cmax);
+}
+
+template <bool FILL>
+__global__ __launch_bounds__(64) void k_ngb(NgbArgs a) {
+    const int lane = threadIdx.x;
+    int* const stk = a.stack + (size_t)blockIdx.x * (size_t)a.stack_cap;
+    for (;;) {
+        int v = 0;
+        if (lane == 0) v = atomicAdd(a.queue, 1);
+        const int g = __shfl(v, 0);
+        if (g >= a.ngroups) break;
+        const int q = g * 64 + lane;
+        const bool live = q < a.nq;
+        double qx = 0, qy = 0, qz = 0, r = 0;
+        if (live) {
+            qx = a.cx[q]; qy = a.cy[q]; qz = a.cz[q];
+            r = a.radius[q];
+        }
+        const double lo0 = wave_min_d(live ? qx : 1e300), hi0 = wave_max_d(live ? qx : -1e300);
+        const double lo1 = wave_min_d(live ? qy : 1e300), hi1 = wave_max_d(live ? qy : -1e300);
+        const double lo2 = wave_min_d(live ? qz : 1e300), hi2 = wave_max_d(live ? qz : -1e300);
+        const double rmax = wave_max_d(live ? r : 0.0);
+        const double r2 = r * r;
+        uint32_t found = 0;
+        size_t out = live && FILL ? a.start[q] : 0;
+
+        // the roots: one per tree of a forest
+        int sp = 0;
+        for (int c = lane; c < a.nclusters; c += 64) stk[c] = (a.nclusters > 1) ? c : a.root;
+        sp = a.nclusters;
+        __threadfence_block();
+        __syncthreads();
+        bool overflow = false;
+        while (sp > 0) {
+            const int k = min(sp, 64);
+            const int base = sp - k;
+            const int node = (lane < k) ? stk[base + lane] : -1;
+            sp = base;
+            bool keep = false, leaf = false;
+            int first = -1, nch = 0, plo = 0, phi = 0;
+            if (node >= 0) {
+                const WNode w = a.wn[node];
+                const float half = snode_half(a, node);
+                double dist = rmax + (double)half + (double)a.xpad[node];
+                if (a.tau > 0) dist += (double)a.vmax[node] * a.tau;
+                const double c0 = (double)(float)a.gcx[node], c1 = (double)(float)a.gcy[node],
+                             c2 = (double)(float)a.gcz[node];
+                const double d0 = fmax(0.0, fmax(lo0 - c0, c0 - hi0));
+                const double d1 = fmax(0.0, fmax(lo1 - c1, c1 - hi1));
+                const double d2 = fmax(0.0, fmax(lo2 - c2, c2 - hi2));
+                if (d0 <= dist && d1 <= dist && d2 <= dist) {
+                    const double ds = dist + 0.7320508075688772 * (double)half;
+                    keep = d0 * d0 + d1 * d1 + d2 * d2 <= ds * ds;
+                }
+                if (keep) {
+                    first = w.first;
+                    if (first < 0) {
+                        leaf = true;
+                        plo = w.plo;
+                        phi = w.phi;
+                    } else {
+                        nch = w.nchild;
+                    }
+                }
+            }
+            // children of kept interior nodes go on the stack
+            int ctot = 0, coff;
+            {
+                int x = nch;
+#pragma unroll
+                for (int d = 1; d < 64; d <<= 1) {
+                    const int y = __shfl_up(x, d);
+                    if (lane >= d) x += y;
+                }
+                ctot = __shfl(x, 63);
+                coff = x - nch;
+            }
+            if (sp + ctot > a.stack_cap) {
+                overflow = true;
+                break;
+            }
+            for (int c = 0; c < nch; ++c) stk[sp + coff + c] = first + c;
+            sp += ctot;
+            // particles of kept leaves, tested by every lane against its own query, leaf by leaf
+            uint64_t lb = __ballot(leaf);
+            while (lb) {
+                const int src_lane = __ffsll((unsigned long long)lb) - 1;
+                lb &= lb - 1;
+                const int lo = __shfl(plo, src_lane), hi = __shfl(phi, src_lane);
+                for (int s = lo; s < hi; ++s) {
+                    const double4 p = a.src[s];
+                    const double ex = p.x - qx, ey = p.y - qy, ez = p.z - qz;
+                    if (live && ex * ex + ey * ey + ez * ez < r2) {
+                        if (FILL) a.slots[out++] = (uint32_t)s;
+                        ++found;
+                    }
+                }
+            }
+            __threadfence_block();
+            __syncthreads();
+        }
+        if (overflow) {
+            if (lane == 0) atomicOr(a.status, 1);
+            continue;
+        }
+        if (!FILL && live) a.count[q] = found;
+    }
+}
+
+// Each query's neighbours in slot (Morton) order, then as particle indices. Lists are short (a
+// kernel's worth of neighbours); a Shell sort keeps the rare long one tolerable.
+__global__ __launch_bounds__(256) void k_ngb_finish(int nq, const size_t* __restrict__ start,
+                                                    uint32_t* __restrict__ slots,
+                                                    const uint32_t* __restrict__ orderbuf,
+                                                    uint32_t* __restrict__ found) {
+    const int q = blockIdx.x * 256 + threadIdx.x;
+    if (q >= nq) return;
+    uint32_t* s = slots + start[q];
+    const long long n = (long long)(start[q + 1] - start[q]);
+    for (long long gap = n / 2; gap > 0; gap = (gap == 2) ? 1 : gap * 5 / 11) {
+        for (long long i = gap; i < n; ++i) {
+            const uint32_t v = s[i];
+            long long j = i;
+            while (j >= gap && s[j - gap] > v) {
+                s[j] = s[j - gap];
+                j -= gap;
+            }
+            s[j] = v;
+        }
+    }
+    uint32_t* f = found + start[q];
+    for (long long i = 0; i < n; ++i) f[i] = orderbuf[s[i]];
+}
+
+__global__ void k_u32_to_size(int n, const uint32_t* __restrict__ in, size_t* __restrict__ out) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i < n) out[i] = in[i];
+}
+
+} // namespace gz
+
+using namespace gz;
+
+namespace {
+
+gz_status ngb_pass(NgbArgs& a, bool fill, int waves, hipStream_t st) {
+    GZ_TRY(fill_zero_u32(reinterpret_cast<uint32_t*>(a.queue), 2, st));
+    const int grid = std::max(1, std::min(waves, a.ngroups));
+    if (fill) k_ngb<true><<<grid, 64, 0, st>>>(a);
+    else k_ngb<false><<<grid, 64, 0, st>>>(a);
+    GZ_HIP(hipGetLastError());
+    return GZ_OK;
+}
+} // namespace
+
+extern "C" gz_status gz_tree_ngb_search(gz_ctx* ctx, const gz_tree* tree, const double* cx,
+                                        const double* cy, const double* cz, const double* radius,
+                                        size_t n_q, double tau, size_t* start, uint32_t* found,
+                                        size_t cap, size_t* total) {
+    clear_error();
+    if (!ctx || !tree || !total) return fail(GZ_ERR_INVALID, "gz_tree_ngb_search: null argument");
+    GZ_HIP(hipSetDevice(ctx->device));
+    *total = 0;
+    if (n_q == 0) return GZ_OK;
+    if (!cx || !cy || !cz || !radius || !start)
+        return fail(GZ_ERR_INVALID, "gz_tree_ngb_search: null centre, radius or start array");
+    if (n_q >= 0x7FFFFFFFull) return fail(GZ_ERR_INVALID, "gz_tree_ngb_search: n_q too large");
+    hipStream_t st = ctx->stream;
+    if (tree->n_live == 0 || tree->nnodes == 0) {
+        GZ_TRY(fill_zero_u32(reinterpret_cast<uint32_t*>(start), 2 * (n_q + 1), st));
+        GZ_HIP(hipStreamSynchronize(st));
+        return GZ_OK;
+    }
+    int* stack = nullptr;
+    int stack_cap = 0;
+    GZ_TRY(ctx_spill(ctx, &stack, &stack_cap));
+    int waves = 0;
+    {
+        int dev = 0, ncu = 0;
+        GZ_HIP(hipGetDevice(&dev));
+        GZ_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+        waves = std::max(1, ncu) * 16; // the spill arena holds this many slices
+    }
+    DevMem& s = ctx->scratch[10];
+    const size_t cnt_bytes = ((n_q + 1) * sizeof(uint32_t) + 255) & ~size_t(255);
+    GZ_TRY(s.reserve(cnt_bytes + 256 + scan_tmp_bytes(n_q + 1)));
+    uint8_t* b = s.as<uint8_t>();
+    uint32_t* count = reinterpret_cast<uint32_t*>(b);
+    int* misc = reinterpret_cast<int*>(b + cnt_bytes);
+    NgbArgs a{};
+    a.wn = tree->wn;
+    a.gcx = tree->gcx; a.gcy = tree->gcy; a.gcz = tree->gcz;
+    a.size = tree->size;
+    a.vmax = tree->vmax;
+    a.xpad = tree->xpad;
+    a.src = tree->src;
+    a.orderbuf = tree->orderbuf;
+    a.cx = cx; a.cy = cy; a.cz = cz; a.radius = radius;
+    a.nq = (int)n_q;
+    a.ngroups = (int)((n_q + 63) / 64);
+    a.root = tree->root;
+    a.nclusters = tree->n_clusters;
+    a.snode_eps = 1e-6 * tree->root_box[3];
+    a.tau = tau;
+    a.queue = misc;
+    a.status = misc + 1;
+    a.stack = stack;
+    a.stack_cap = stack_cap;
+    a.count = count;
+    GZ_TRY(fill_zero_u32(count, n_q + 1, st));
+    GZ_TRY(ngb_pass(a, false, waves, st));
+    // offsets: exclusive scan of the counts (count[n_q] = 0 gives the total at start[n_q])
+    uint32_t tot32 = 0;
+    {
+        DevMem& o = ctx->scratch[11];
+        GZ_TRY(o.reserve((n_q + 1) * sizeof(uint32_t)));
+        GZ_TRY(exclusive_scan_u32(count, o.as<uint32_t>(), n_q + 1, nullptr,
+                                  b + cnt_bytes + 256, st));
+        k_u32_to_size<<<(unsigned)((n_q + 1 + 255) / 256), 256, 0, st>>>((int)(n_q + 1),
+                                                                          o.as<uint32_t>(), start);
+        GZ_HIP(hipGetLastError());
+        GZ_HIP(hipMemcpyAsync(&tot32, o.as<uint32_t>() + n_q, sizeof tot32,
+                              hipMemcpyDeviceToHost, st));
+    }
+    int h_status = 0;
+    GZ_HIP(hipMemcpyAsync(&h_status, a.status, sizeof h_status, hipMemcpyDeviceToHost, st));
+    GZ_HIP(hipStreamSynchronize(st));
+    if (h_status & 1)
+        return fail(GZ_ERR_INTERNAL, "gz_tree_ngb_search: a search stack outgrew %d entries",
+                    stack_cap);
+    *total = tot32;
+    if (!found || cap < (size_t)tot32 || tot32 == 0) return GZ_OK;
+    // pass 2: the neighbours' slots, then sorted and mapped to particle indices
+    DevMem& sl = ctx->scratch[11];
+    GZ_TRY(sl.reserve((size_t)tot32 * sizeof(uint32_t)));
+    a.start = start;
+    a.slots = sl.as<uint32_t>();
+    GZ_TRY(ngb_pass(a, true, waves, st));
+    k_ngb_finish<<<(unsigned)((n_q + 255) / 256), 256, 0, st>>>((int)n_q, start, a.slots,
+                                                                 tree->orderbuf, found);
+    GZ_HIP(hipGetLastError());
+    GZ_HIP(hipMemcpyAsync(&h_status, a.status, sizeof h_status, hipMemcpyDeviceToHost, st));
+    GZ_HIP(hipStreamSynchronize(st));
+    if (h_status & 1)
+        return fail(GZ_ERR_INTERNAL, "gz_tree_ngb_search: a search stack outgrew %d entries",
+                    stack_cap);
+    return GZ_OK;
+}

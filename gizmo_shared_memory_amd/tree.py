@@ -508,6 +508,32 @@ def accel_grouped_tidal(T: Tree, P: Particles, targets, theta: float, G: float =
     return ax, ay, az, tid, jk
 
 
+def ngb_search(T: Tree, centres, radius, tau: float = 0.0):
+    """shmem::ngb_search (hydro.h:73-75) for many centres: (start, found) device tensors, CSR —
+    the particles within radius[q] of centres[q] (n_q x 3), each list in Morton order."""
+    dev = T.P.device
+    c = centres if isinstance(centres, torch.Tensor) else torch.as_tensor(np.asarray(centres))
+    c = c.to(device=dev, dtype=torch.float64)
+    cs = [c[:, d].contiguous() for d in range(3)]
+    r = _dev_f64(radius, dev)
+    nq = int(r.shape[0])
+    start = torch.zeros(nq + 1, dtype=torch.int64, device=dev)
+    total = C.c_size_t(0)
+    lib = L.load()
+    T.ctx.enter()
+    L.check(lib.gz_tree_ngb_search(T.ctx.handle, T.handle, *(a.data_ptr() for a in cs),
+                                   r.data_ptr(), nq, float(tau), start.data_ptr(), None, 0,
+                                   C.byref(total)), "gz_tree_ngb_search")
+    found = torch.empty(max(int(total.value), 1), dtype=torch.int32, device=dev)
+    if total.value:
+        L.check(lib.gz_tree_ngb_search(T.ctx.handle, T.handle, *(a.data_ptr() for a in cs),
+                                       r.data_ptr(), nq, float(tau), start.data_ptr(),
+                                       found.data_ptr(), int(total.value), C.byref(total)),
+                "gz_tree_ngb_search")
+    T.ctx.synchronize()
+    return start, found[:int(total.value)].view(torch.uint32)
+
+
 def potential(T: Tree, P: Particles, targets, theta: float, G: float = 1.0, aold=None):
     """shmem::potential (tree.h:446-447): spline-softened potential per target (FP64)."""
     tg = _dev_u32(targets, P.device)

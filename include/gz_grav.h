@@ -247,6 +247,17 @@ gz_status gz_tree_accel_grouped_tidal(gz_ctx* ctx, const gz_tree* tree, const gz
                                       const double* vy, const double* vz, double* ax, double* ay,
                                       double* az, double* tidal, double* jerk);
 
+/* shmem::ngb_search (hydro.h:73-75, hydro.cc:37-108) for n_q centres at once (non-periodic): the
+ * particles within radius[q] of (cx, cy, cz)[q] (device arrays), read from the tree's sorted copy,
+ * with node boxes widened by vmax * tau for a tree aged tau. start (device, n_q + 1) receives the
+ * CSR offsets and *total (host) the number found; when found (device) holds at least *total
+ * entries it receives, per query, the particle indices in Morton order (the reference's traversal
+ * order). Call with found = NULL to size it. Queries in Morton order search fastest. */
+gz_status gz_tree_ngb_search(gz_ctx* ctx, const gz_tree* tree, const double* cx,
+                             const double* cy, const double* cz, const double* radius,
+                             size_t n_q, double tau, size_t* start, uint32_t* found, size_t cap,
+                             size_t* total);
+
 /* shmem::potential (tree.h:446-447, tree.cc:899-944): the spline-softened potential at each
  * target, G included, walked per target like gz_tree_accel (leaves always summed), FP64. */
 gz_status gz_tree_potential(gz_ctx* ctx, const gz_tree* tree, const gz_particles* p,

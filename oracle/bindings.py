@@ -193,6 +193,10 @@ class Oracle(_Lib):
         L.gzo_accel_grouped_tj.argtypes = [C.c_void_p, C.POINTER(_GzoParticles), _up,
                                            C.c_size_t, C.c_double, C.c_double, C.c_int, _dp,
                                            _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _llp]
+        L.gzo_ngb_search.restype = C.c_size_t
+        L.gzo_ngb_search.argtypes = [C.c_void_p, C.POINTER(_GzoParticles), _dp, _dp, _dp, _dp,
+                                     C.c_size_t, C.c_double, C.POINTER(C.c_size_t), _up,
+                                     C.c_size_t]
         L.gzo_potential.argtypes = [C.c_void_p, C.POINTER(_GzoParticles), _up, C.c_size_t,
                                     C.c_double, C.c_double, _dp, _dp]
         L.gzo_build_vel.restype = C.c_void_p
@@ -267,6 +271,22 @@ class Oracle(_Lib):
                                       _ptr(ay, _dp), _ptr(az, _dp), _ptr(tid, _dp),
                                       _ptr(jk, _dp), None)
         return ax, ay, az, tid, jk
+
+    def ngb_search(self, t, P: HostParticles, centres, radius, tau=0.0):
+        """CSR (start, found) of the particles within radius[q] of centres[q] (n_q x 3)."""
+        c = np.ascontiguousarray(centres, dtype=np.float64)
+        cs = [np.ascontiguousarray(c[:, d]) for d in range(3)]
+        r = _f64(radius)
+        nq = r.shape[0]
+        start = np.zeros(nq + 1, dtype=np.uint64)
+        sp = start.ctypes.data_as(C.POINTER(C.c_size_t))
+        cp = self._cp(P)
+        total = self.lib.gzo_ngb_search(t, C.byref(cp), *(_ptr(a, _dp) for a in cs),
+                                        _ptr(r, _dp), nq, tau, sp, None, 0)
+        found = np.zeros(max(int(total), 1), dtype=np.uint32)
+        self.lib.gzo_ngb_search(t, C.byref(cp), *(_ptr(a, _dp) for a in cs), _ptr(r, _dp), nq,
+                                tau, sp, _ptr(found, _up), found.shape[0])
+        return start.astype(np.int64), found[:int(total)]
 
     def potential(self, t, P: HostParticles, targets, theta, G=1.0, aold=None):
         tg = np.ascontiguousarray(targets, dtype=np.uint32)
@@ -351,6 +371,9 @@ class Reference(_Lib):
         L.gzref_accel_grouped_tj.argtypes = [C.c_void_p, C.c_void_p, _up, C.c_size_t,
                                              C.c_double, C.c_double, C.c_int, _dp, _dp, _dp, _dp,
                                              _dp, _dp, _dp, _dp, _dp]
+        L.gzref_ngb_search.restype = C.c_size_t
+        L.gzref_ngb_search.argtypes = [C.c_void_p, C.c_void_p, _dp, _dp, _dp, _dp, C.c_size_t,
+                                       C.c_double, C.POINTER(C.c_size_t), _up, C.c_size_t]
         L.gzref_build_vel.restype = C.c_void_p
         L.gzref_build_vel.argtypes = [C.c_void_p, C.c_longlong, _dp, _dp, _dp]
         L.gzref_kick_nodes.argtypes = [C.c_void_p, _up, C.c_size_t, _dp, _dp, _dp, _dp, _dp, _dp,
@@ -422,6 +445,20 @@ class Reference(_Lib):
         self.lib.gzref_accel_grouped(t, rp, _ptr(tg, _up), nt, theta, G, batch, _ptr(ao, _dp),
                                      _ptr(ax, _dp), _ptr(ay, _dp), _ptr(az, _dp))
         return ax, ay, az
+
+    def ngb_search(self, t, rp, centres, radius, tau=0.0):
+        c = np.ascontiguousarray(centres, dtype=np.float64)
+        cs = [np.ascontiguousarray(c[:, d]) for d in range(3)]
+        r = _f64(radius)
+        nq = r.shape[0]
+        start = np.zeros(nq + 1, dtype=np.uint64)
+        sp = start.ctypes.data_as(C.POINTER(C.c_size_t))
+        total = self.lib.gzref_ngb_search(t, rp, *(_ptr(a, _dp) for a in cs), _ptr(r, _dp), nq,
+                                          tau, sp, None, 0)
+        found = np.zeros(max(int(total), 1), dtype=np.uint32)
+        self.lib.gzref_ngb_search(t, rp, *(_ptr(a, _dp) for a in cs), _ptr(r, _dp), nq, tau, sp,
+                                  _ptr(found, _up), found.shape[0])
+        return start.astype(np.int64), found[:int(total)]
 
     def accel_grouped_tj(self, t, rp, targets, theta, G=1.0, batch=0, aold=None, vel=None,
                          jerk=False):

@@ -8,6 +8,7 @@
  * equality (spline kernel, opening test, moment sums) it is kept as the reference writes it. */
 #include "gz_oracle.h"
 
+#include <float.h>
 #include <math.h>
 #include <stdlib.h>
 #include <string.h>
@@ -460,6 +461,63 @@ void gzo_tree_reuse(void* h, float* vmax, double* vcom, double* dp, double* dq, 
         if (wcom) { wcom[3 * i] = T->wcx[i]; wcom[3 * i + 1] = T->wcy[i]; wcom[3 * i + 2] = T->wcz[i]; }
         if (wmass) wmass[i] = T->wmass[i];
     }
+}
+
+/* ---------------------------------------------------------------------------------------------
+ * Neighbour search (hydro.cc:37-108) over the search node (SNode, tree.h:206-226, packed at
+ * tree.cc:496-506): float geometric centre and a conservatively widened half side
+ * ------------------------------------------------------------------------------------------- */
+
+/* All particles within radius[q] of centre q (non-periodic), in traversal order, for a tree aged
+ * tau. CSR: start[q] .. start[q+1]; found written while it fits in cap. Returns the total. */
+size_t gzo_ngb_search(void* h, const gzo_particles* P, const double* cx, const double* cy,
+                      const double* cz, const double* radius, size_t nq, double tau, size_t* start,
+                      uint32_t* found, size_t cap) {
+    const otree* T = (const otree*)h;
+    const double snode_eps = 1e-6 * (T->nnodes ? T->size[T->root] : 1.0);
+    const int nn = T->nnodes;
+    float* sc = (float*)malloc((size_t)(nn ? nn : 1) * 4 * sizeof(float));
+    for (int i = 0; i < nn; ++i) {
+        sc[4 * i] = (float)T->gcx[i]; sc[4 * i + 1] = (float)T->gcy[i];
+        sc[4 * i + 2] = (float)T->gcz[i];
+        const double cmax = fmax(fabs(T->gcx[i]), fmax(fabs(T->gcy[i]), fabs(T->gcz[i])));
+        sc[4 * i + 3] = (float)(0.5 * T->size[i] + snode_eps + 4.0 * (double)FLT_EPSILON * cmax);
+    }
+    size_t total = 0;
+    start[0] = 0;
+    for (size_t q = 0; q < nq; ++q) {
+        const double r2max = radius[q] * radius[q];
+        int node = nn ? T->root : -1;
+        while (node >= 0) {
+            const float* sn = sc + 4 * node;
+            double dist = radius[q] + (double)sn[3] + (double)T->xpad[node];
+            if (tau > 0) dist += (double)T->vmax[node] * tau;
+            const double dx = (double)sn[0] - cx[q];
+            if (dx > dist || -dx > dist) { node = T->next[node]; continue; }
+            const double dy = (double)sn[1] - cy[q];
+            if (dy > dist || -dy > dist) { node = T->next[node]; continue; }
+            const double dz = (double)sn[2] - cz[q];
+            if (dz > dist || -dz > dist) { node = T->next[node]; continue; }
+            dist += 0.7320508075688772 * (double)sn[3];
+            if (dx * dx + dy * dy + dz * dz > dist * dist) { node = T->next[node]; continue; }
+            if (T->first[node] < 0) {
+                for (int k = T->plo[node]; k < T->phi[node]; ++k) {
+                    const uint32_t j = T->orderbuf[k];
+                    const double ex = P->x[j] - cx[q], ey = P->y[j] - cy[q], ez = P->z[j] - cz[q];
+                    if (ex * ex + ey * ey + ez * ez < r2max) {
+                        if (found && total < cap) found[total] = j;
+                        ++total;
+                    }
+                }
+                node = T->next[node];
+            } else {
+                node = T->first[node];
+            }
+        }
+        start[q + 1] = total;
+    }
+    free(sc);
+    return total;
 }
 
 /* ---------------------------------------------------------------------------------------------

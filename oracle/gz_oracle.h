@@ -17,6 +17,8 @@
  *   gzo_accel_grouped_tj  tidal tensor and jerk bit for bit against the reference, and against
  *                         finite differences of the direct-sum acceleration (in space and
  *                         along ballistic motion)
+ *   gzo_ngb_search        lists equal to the reference's ngb_search (same order) and, as sets,
+ *                         to a brute-force distance test
  *   gzo_potential         bit for bit against the reference potential, and theta -> 0 against
  *                         the direct sum of the spline potential
  *   gzo_accel_brute       bit for bit against the reference accel_brute, Newton's third law, and
@@ -94,6 +96,11 @@ void gzo_accel_grouped_tj(void* t, const gzo_particles* P, const uint32_t* targe
                           double* az, double* tidal, double* jerk, long long* counts);
 void gzo_accel(void* t, const gzo_particles* P, const uint32_t* targets, size_t nt, double theta,
                double G, const double* aold, double* ax, double* ay, double* az);
+/* Neighbour search (hydro.cc:37-108): CSR of all particles within radius[q] of centre q, in the
+ * reference's traversal order (= Morton order); found written while it fits in cap. */
+size_t gzo_ngb_search(void* t, const gzo_particles* P, const double* cx, const double* cy,
+                      const double* cz, const double* radius, size_t nq, double tau, size_t* start,
+                      uint32_t* found, size_t cap);
 void gzo_potential(void* t, const gzo_particles* P, const uint32_t* targets, size_t nt,
                    double theta, double G, const double* aold, double* phi);
 void gzo_accel_brute(const gzo_particles* P, const uint32_t* targets, size_t nt, double G,

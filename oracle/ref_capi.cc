@@ -8,6 +8,7 @@
 // reference function named in its comment, unchanged.
 
 #include "tree.h"
+#include "hydro.h"
 
 #include <cstdint>
 #include <cstring>
@@ -282,6 +283,27 @@ void gzref_tree_reuse(void* t, float* vmax, double* vcom, double* dp, double* dq
         if (wn_soft) wn_soft[i] = T.wn[i].soft;
         if (wn_len) wn_len[i] = T.wn[i].len;
     }
+}
+
+// shmem::ngb_search (hydro.h:73-75, hydro.cc:37-108) for n_q centres, non-periodic, with the tree
+// aged tau (t_since_build). CSR output: start[q] .. start[q+1] in found; returns the total, and
+// writes found only while it fits in cap.
+size_t gzref_ngb_search(void* t, void* particles, const double* cx, const double* cy,
+                        const double* cz, const double* radius, size_t n_q, double tau,
+                        size_t* start, uint32_t* found, size_t cap) {
+    auto* rt = static_cast<RefTree*>(t);
+    auto* rp = static_cast<RefParticles*>(particles);
+    rt->T.t_since_build = tau;
+    std::vector<uint32_t> all, one;
+    start[0] = 0;
+    for (size_t q = 0; q < n_q; ++q) {
+        one.clear();
+        ngb_search(rt->T, rp->P, Vec3d{cx[q], cy[q], cz[q]}, radius[q], one);
+        all.insert(all.end(), one.begin(), one.end());
+        start[q + 1] = all.size();
+    }
+    if (found && all.size() <= cap) std::memcpy(found, all.data(), all.size() * sizeof(uint32_t));
+    return all.size();
 }
 
 int gzref_max_threads(void) { return omp_get_max_threads(); }

@@ -1,0 +1,77 @@
+"""The device neighbour search against the oracle (pinned to the reference's ngb_search and to a
+brute-force distance test): identical CSR lists, Morton order within each query."""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from tests import tree_cases
+from tests.test_oracle_ngb import _queries
+from tests.test_oracle_reuse import _scenario
+
+pytestmark = pytest.mark.gpu
+torch = pytest.importorskip("torch")
+
+
+@pytest.fixture(scope="module")
+def gz():
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    from gizmo_shared_memory_amd import tree
+    return tree
+
+
+@pytest.mark.parametrize("name", ["plummer20k", "collocated", "tombstones", "n1", "n17"])
+def test_ngb_search_matches_oracle(gz, oracle, name):
+    from oracle.bindings import HostParticles
+    x, y, z, m, soft = tree_cases.CASES[name]()
+    print(f"[ngb] case={name}", flush=True)
+    hp = HostParticles(x, y, z, m, soft)
+    P = gz.Particles(x, y, z, m, soft)
+    T = gz.build(P)
+    nq = min(300, x.shape[0])
+    c, r = _queries(x, y, z, nq, 3, 0.05)
+    # Morton-ordered queries, as a hydro step would pass them, and also scattered ones
+    for order in (np.argsort(np.asarray(T.download()["rank"])[:nq]), np.arange(nq)):
+        cq, rq = c[order], r[order]
+        s_d, f_d = gz.ngb_search(T, cq, rq)
+        to = oracle.build(hp)
+        try:
+            s_o, f_o = oracle.ngb_search(to, hp, cq, rq)
+        finally:
+            oracle.free(to)
+        assert np.array_equal(s_d.cpu().numpy(), s_o)
+        assert np.array_equal(f_d.cpu().numpy(), f_o)
+
+
+def test_ngb_search_on_an_aged_tree(gz, oracle):
+    from oracle.bindings import HostParticles
+    pos, v0, _, _ = _scenario(n=5000, seed=29)
+    tau = 0.05
+    xt = [pos[d] + v0[d] * tau for d in range(3)]
+    P0 = gz.Particles(*pos)
+    T = gz.build(P0, vel=v0)
+    Pt = gz.Particles(xt[0], xt[1], xt[2], pos[3], pos[4])
+    T.update_sources(Pt)
+    c, r = _queries(xt[0], xt[1], xt[2], 400, 8, 0.1)
+    s_d, f_d = gz.ngb_search(T, c, r, tau)
+    hp0 = HostParticles(*pos)
+    hpt = HostParticles(xt[0], xt[1], xt[2], pos[3], pos[4])
+    to = oracle.build_vel(hp0, v0)
+    try:
+        s_o, f_o = oracle.ngb_search(to, hpt, c, r, tau)
+    finally:
+        oracle.free(to)
+    assert np.array_equal(s_d.cpu().numpy(), s_o)
+    assert np.array_equal(f_d.cpu().numpy(), f_o)
+
+
+def test_ngb_search_empty_and_large(gz):
+    x, y, z, m, soft = tree_cases.plummer_small(20000, seed=2)
+    P = gz.Particles(x, y, z, m, soft)
+    T = gz.build(P)
+    s, f = gz.ngb_search(T, np.zeros((0, 3)), np.zeros(0))
+    assert s.shape[0] == 1 and f.shape[0] == 0
+    # one query covering everything: every live particle, in Morton order
+    s, f = gz.ngb_search(T, np.zeros((1, 3)), np.array([1e6]))
+    assert np.array_equal(f.cpu().numpy(), T.download()["orderbuf"])

@@ -1,0 +1,74 @@
+"""Neighbour search (hydro.cc:37-108) in the oracle: the same lists, in the same order, as the
+reference's ngb_search, and as sets the brute-force distance test."""
+from __future__ import annotations
+
+import numpy as np
+
+from oracle.bindings import HostParticles
+from tests import tree_cases
+from tests.test_oracle_reuse import _scenario
+
+
+def _queries(x, y, z, n_q, seed, scale):
+    rng = np.random.default_rng(seed)
+    idx = rng.choice(x.shape[0], n_q, replace=False)
+    c = np.stack([x[idx], y[idx], z[idx]], 1) + rng.normal(scale=scale * 0.1, size=(n_q, 3))
+    r = scale * rng.uniform(0.3, 1.5, n_q)
+    return c, r
+
+
+def test_ngb_search_matches_reference_and_brute_force(oracle, reference):
+    for name in ("plummer20k", "collocated", "tombstones"):
+        x, y, z, m, soft = tree_cases.CASES[name]()
+        P = HostParticles(x, y, z, m, soft)
+        c, r = _queries(x, y, z, 200, 3, 0.05)
+        to = oracle.build(P)
+        rp = reference.particles(P)
+        tr = reference.build(rp)
+        try:
+            so, fo = oracle.ngb_search(to, P, c, r)
+            sr, fr = reference.ngb_search(tr, rp, c, r)
+        finally:
+            oracle.free(to); reference.free(tr); reference.free_particles(rp)
+        assert np.array_equal(so, sr), name
+        keys, _ = oracle.keys(P)
+        ties = np.unique(keys[m > 0]).shape[0] < int((m > 0).sum())
+        if not ties:
+            assert np.array_equal(fo, fr), name
+        for q in range(c.shape[0]):
+            # traversal order is Morton order (ties inside a run of equal keys are unspecified
+            # in the reference's sort)
+            a, b = fo[so[q]:so[q + 1]], fr[sr[q]:sr[q + 1]]
+            assert np.all(np.diff(keys[a].astype(np.int64)) >= 0) and \
+                np.all(np.diff(keys[b].astype(np.int64)) >= 0)
+            assert np.array_equal(np.sort(a), np.sort(b))
+        live = m > 0
+        for q in range(c.shape[0]):
+            d2 = (x - c[q, 0]) ** 2 + (y - c[q, 1]) ** 2 + (z - c[q, 2]) ** 2
+            want = np.nonzero((d2 < r[q] ** 2) & live)[0]
+            assert np.array_equal(np.sort(fo[so[q]:so[q + 1]]), want), (name, q)
+
+
+def test_ngb_search_on_an_aged_tree(oracle, reference):
+    """After a drift the search still finds every neighbour at the current positions: the node
+    boxes are widened by vmax * tau (the tree built with velocities, the reference likewise)."""
+    pos, v0, _, _ = _scenario(n=4000, seed=17)
+    P0 = HostParticles(*pos)
+    tau = 0.05
+    xt = [pos[d] + v0[d] * tau for d in range(3)]
+    Pt = HostParticles(xt[0], xt[1], xt[2], pos[3], pos[4])
+    c, r = _queries(xt[0], xt[1], xt[2], 150, 5, 0.1)
+    to = oracle.build_vel(P0, v0)
+    rp0 = reference.particles(P0)
+    tr = reference.build_vel(rp0, v0)
+    rpt = reference.particles(Pt)
+    try:
+        so, fo = oracle.ngb_search(to, Pt, c, r, tau)
+        sr, fr = reference.ngb_search(tr, rpt, c, r, tau)
+    finally:
+        oracle.free(to); reference.free(tr)
+        reference.free_particles(rp0); reference.free_particles(rpt)
+    assert np.array_equal(so, sr) and np.array_equal(fo, fr)
+    for q in range(c.shape[0]):
+        d2 = (xt[0] - c[q, 0]) ** 2 + (xt[1] - c[q, 1]) ** 2 + (xt[2] - c[q, 2]) ** 2
+        assert np.array_equal(np.sort(fo[so[q]:so[q + 1]]), np.nonzero(d2 < r[q] ** 2)[0])
