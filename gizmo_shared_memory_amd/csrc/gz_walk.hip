@@ -29,11 +29,11 @@
 
 namespace gz {
 
-constexpr int kWalkStack = 1024;  // node ids per wave in LDS
+constexpr int kWalkStack = 768;   // node ids per wave in LDS
 constexpr int kSpillKeep = 256;   // LDS entries kept when the bottom of the stack is spilled
 constexpr int kXcdChunk = 64;     // groups per chunk of the persistent launch's XCD interleave
-constexpr int kMonoF = 160;       // monopole entries per wave, FP32 (16 B each)
-constexpr int kPairF = 128;       // particle entries per wave, FP32 (20 B each)
+constexpr int kMonoF = 128;       // monopole entries per wave, FP32 (16 B each)
+constexpr int kPairF = 96;        // particle entries per wave, FP32 (20 B each)
 constexpr int kMonoD = 96;        // monopole entries per wave, FP64 (32 B each)
 constexpr int kPairD = 64;        // particle entries per wave, FP64 (40 B each)
 constexpr int kMaxChunk = 64;     // targets evaluated together (one wave)
@@ -218,6 +218,14 @@ __global__ __launch_bounds__(256) void k_pack_starts(int ngroups, const int* __r
 }
 
 
+// A value known to be the same in every lane, moved to scalar registers.
+__device__ __forceinline__ double uniform_d(double v) {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readfirstlane((int)(b & 0xFFFFFFFFll));
+    const int hi = __builtin_amdgcn_readfirstlane((int)(b >> 32));
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
 // Position and softening of target slot i: from the tree's sorted copy when the target is in the
 // tree, else from the particle arrays.
 __device__ __forceinline__ void target_data(const WalkArgs& a, int i, double& px, double& py,
@@ -301,11 +309,12 @@ __device__ __forceinline__ float spline_inner_f(float r, float h) {
                     10.666666666666667f * u * u * u - 0.06666666666666667f / (u * u * u));
 }
 
-// At most 128 VGPRs, so that four waves fit on each SIMD (the LDS footprint allows four too).
+// Five waves per SIMD: at most 96 VGPRs (the wave-uniform group data lives in scalar registers)
+// and, in the FP32 walk, under 8 KB of LDS per wave (the node stack, both lists).
 // TJ (FP64 only): the tidal tensor and, when velocities are given, the jerk, accumulated in the
 // same walk (tree.cc:811-871).
 template <bool FP64, bool COUNT, bool GSTACK, bool TJ = false>
-__global__ __launch_bounds__(64, 4) void k_walk(WalkArgs a) {
+__global__ __launch_bounds__(64, 5) void k_walk(WalkArgs a) {
     static_assert(!TJ || FP64, "the tidal tensor and jerk are FP64 walks");
     using real = typename WalkTypes<FP64>::real;
     using real4 = typename WalkTypes<FP64>::real4;
@@ -384,8 +393,17 @@ __global__ __launch_bounds__(64, 4) void k_walk(WalkArgs a) {
             if (g >= a.ngroups) break;
         }
 
-        const GroupInfo gi =
-            a.groups ? a.groups[g] : group_bounds(a, g, lane);
+        GroupInfo gi = a.groups ? a.groups[g] : group_bounds(a, g, lane);
+        // every field is the same in all lanes: keep them in scalar registers
+        for (int k = 0; k < 3; ++k) {
+            gi.lo[k] = uniform_d(gi.lo[k]);
+            gi.hi[k] = uniform_d(gi.hi[k]);
+        }
+        gi.emax = uniform_d(gi.emax);
+        gi.amin = uniform_d(gi.amin);
+        gi.theta_sq = uniform_d(gi.theta_sq);
+        gi.tlo = __builtin_amdgcn_readfirstlane(gi.tlo);
+        gi.thi = __builtin_amdgcn_readfirstlane(gi.thi);
         const double gcx = 0.5 * (gi.lo[0] + gi.hi[0]);
         const double gcy = 0.5 * (gi.lo[1] + gi.hi[1]);
         const double gcz = 0.5 * (gi.lo[2] + gi.hi[2]);
@@ -523,7 +541,7 @@ __global__ __launch_bounds__(64, 4) void k_walk(WalkArgs a) {
                     float fx = 0.f, fy = 0.f, fz = 0.f;
                     const float tef = (float)te;
                     const float txf = (float)tx, tyf = (float)ty, tzf = (float)tz;
-                    // four entries in flight per step, so their LDS reads overlap
+                    // two entries in flight per step, so their LDS reads overlap
                     auto mono = [&](const real4 s) {
                         const float dx = (float)s.x - txf, dy = (float)s.y - tyf,
                                     dz = (float)s.z - tzf;
@@ -535,13 +553,10 @@ __global__ __launch_bounds__(64, 4) void k_walk(WalkArgs a) {
                         fz = __builtin_fmaf(dz, f, fz);
                     };
                     int j = jstart;
-                    for (; j + 3 * L < mcount; j += 4 * L) {
-                        const real4 s0 = s_mono[j], s1 = s_mono[j + L], s2 = s_mono[j + 2 * L],
-                                    s3 = s_mono[j + 3 * L];
+                    for (; j + L < mcount; j += 2 * L) {
+                        const real4 s0 = s_mono[j], s1 = s_mono[j + L];
                         mono(s0);
                         mono(s1);
-                        mono(s2);
-                        mono(s3);
                     }
                     for (; j < mcount; j += L) mono(s_mono[j]);
 #pragma unroll 2
@@ -884,13 +899,13 @@ __global__ __launch_bounds__(kDsThreads) void k_direct_fp64(
 }
 
 // ---- host launchers ----------------------------------------------------------------------------
-// Wavefronts of a persistent walk launch: 16 per CU (four per SIMD, the occupancy the walk's
+// Wavefronts of a persistent walk launch: 20 per CU (five per SIMD, the occupancy the walk's
 // registers and LDS allow).
 static gz_status persistent_waves(int* waves) {
     int dev = 0, ncu = 0;
     GZ_HIP(hipGetDevice(&dev));
     GZ_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
-    *waves = std::max(1, ncu) * 16;
+    *waves = std::max(1, ncu) * 20;
     return GZ_OK;
 }
 
