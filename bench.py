@@ -230,31 +230,39 @@ def main():
 
     results = {}
     variants = set(args.variants.split(","))
-    # plain GPU walks with the reference criterion; group size tuned on the FP32 walk
-    if "fp64" in variants:
-        results["fp64"] = measure("fp64", "fp64", 0, args.steps, args.warmup)
     sizes = [int(v) for v in args.group_sizes.split(",")]
-    best = (sizes[0], 0.0)
-    if len(sizes) > 1:
+
+    def tune(label, model=None):
+        """The group size with the lowest time per step for this walk (two timed steps each)."""
+        if len(sizes) == 1:
+            return sizes[0]
         best = None
         for gs in sizes:
-            (_, dt, kms) = time_walk(gz, T, P, targets, aold, 1, 1, precision="fp32",
-                                     group_size=gs)
-            log(f"tune fp32 group_size={gs}: {dt * 1e3:.2f} ms/step, kernel {kms:.2f} ms")
+            (_, dt, kms) = time_walk(gz, T, P, targets, aold, 2, 1, precision="fp32",
+                                     group_size=gs, model=model)
+            log(f"tune {label} group_size={gs}: {dt * 1e3:.2f} ms/step, kernel {kms:.2f} ms")
             if best is None or dt < best[1]:
                 best = (gs, dt)
-    gsz = best[0]
+        return best[0]
+
+    # plain GPU walks with the reference criterion
+    if "fp64" in variants:
+        results["fp64"] = measure("fp64", "fp64", 0, args.steps, args.warmup)
+    gsz = sizes[0]
+    if "fp32" in variants or "global" in variants:
+        gsz = tune("fp32")
     if "fp32" in variants:
         results["fp32"] = measure("fp32", "fp32", gsz, args.steps, args.warmup)
 
-    # learned criterion (FP32 walk)
+    # learned criterion (FP32 walk), its group size tuned on its own
     model = None
     if "learned" not in variants:
         pass
     elif os.path.exists(args.model):
         from gizmo_shared_memory_amd import learned
         model = learned.load_model(args.model)
-        results["learned_fp32"] = measure("learned_fp32", "fp32", gsz, args.steps, args.warmup,
+        gsl = tune("learned", model)
+        results["learned_fp32"] = measure("learned_fp32", "fp32", gsl, args.steps, args.warmup,
                                           model=model)
     else:
         log(f"no learned model at {args.model}")
@@ -262,14 +270,18 @@ def main():
         # the method without the network: one global tolerance for every group, the largest
         # on a grid whose p99 error stays within the learned walk's
         goal = max(results["learned_fp32"]["err"]["p99"], 1e-12)
+        gsl = results["learned_fp32"]["group_size"]
         glob = None
         for k in range(0, 11):
             sc = 2.0 ** (k / 2.0)
-            r = measure(f"global_tol_x{sc:.2f}", "fp32", gsz, 1, 1, scale=sc)
+            r = measure(f"global_tol_x{sc:.2f}", "fp32", gsl, 1, 1, scale=sc)
             if r["err"]["p99"] <= goal:
-                glob = r
+                glob = sc
             else:
                 break
+        if glob is not None:  # the chosen tolerance, timed like the other walks
+            glob = measure(f"global_tol_x{glob:.2f}", "fp32", gsl, args.steps, args.warmup,
+                           scale=glob)
         results["global_tolerance_fp32"] = glob
 
     if args.copy_mb > 0:

@@ -36,9 +36,10 @@ __device__ __forceinline__ float gelu_grad(float x) {
 
 // ---- group features ------------------------------------------------------------------------------
 // amag is |a_prev| per target slot (no-G units, the basis of aold). The sums over a group's
-// targets are taken by one wavefront per group (lanes read consecutive targets, so the reads
-// coalesce); the features and the network, arithmetic on a handful of scalars, run one thread
-// per group.
+// targets are taken by LPG consecutive lanes per group (LPG = 16, 32 or 64: small groups share a
+// wavefront, so the lanes are not idle and the per-group reductions are shared); lanes read
+// consecutive targets, so the reads coalesce. The features and the network, arithmetic on a
+// handful of scalars, run one thread per group.
 
 struct FeatureInputs {
     int ngroups, nt_total;
@@ -56,22 +57,48 @@ struct FeatureInputs {
     int nc;
 };
 
-__device__ __forceinline__ double wave_sum_d(double v) {
+// Reductions over aligned runs of W lanes (W a power of two up to 64): min and max are exact, the
+// sum is the same doubling tree in every run.
+template <int W> __device__ __forceinline__ double sub_sum_d(double v) {
 #pragma unroll
-    for (int d = 32; d >= 1; d >>= 1) v += __shfl_xor(v, d);
+    for (int d = W / 2; d >= 1; d >>= 1) v += __shfl_xor(v, d);
+    return v;
+}
+template <int W> __device__ __forceinline__ double sub_min_d(double v) {
+#pragma unroll
+    for (int d = W / 2; d >= 1; d >>= 1) {
+        const double o = __shfl_xor(v, d);
+        v = (o < v) ? o : v;
+    }
+    return v;
+}
+template <int W> __device__ __forceinline__ double sub_max_d(double v) {
+#pragma unroll
+    for (int d = W / 2; d >= 1; d >>= 1) {
+        const double o = __shfl_xor(v, d);
+        v = (v < o) ? o : v;
+    }
+    return v;
+}
+template <int W> __device__ __forceinline__ int sub_sum_i(int v) {
+#pragma unroll
+    for (int d = W / 2; d >= 1; d >>= 1) v += __shfl_xor(v, d);
     return v;
 }
 
+template <int LPG>
 __global__ __launch_bounds__(256) void k_group_stats(FeatureInputs in,
                                                      GroupStats* __restrict__ out) {
-    const int lane = threadIdx.x & 63;
-    const int g = (int)(((size_t)blockIdx.x * 256 + threadIdx.x) >> 6);
+    static_assert(LPG == 16 || LPG == 32 || LPG == 64, "lanes per group");
+    const int sl_lane = threadIdx.x & (LPG - 1);
+    const int g = (int)(((size_t)blockIdx.x * 256 + threadIdx.x) / LPG);
+    // a whole run of LPG lanes shares g, so a run is either all in range or all out
     if (g >= in.ngroups) return;
     const int tlo = in.gstart[g];
     const int thi = (g + 1 < in.ngroups) ? in.gstart[g + 1] : in.nt_total;
     double msum = 0, amin = 1e300, amax = 0, a2 = 0, emax = 0;
     double lo0 = 1e300, lo1 = 1e300, lo2 = 1e300, hi0 = -1e300, hi1 = -1e300, hi2 = -1e300;
-    for (int i = tlo + lane; i < thi; i += 64) {
+    for (int i = tlo + sl_lane; i < thi; i += LPG) {
         const int sl = in.tslot[i];
         double px, py, pz, pm, pe;
         if (sl >= 0) { // the tree's sorted copy: consecutive targets, nearby slots
@@ -94,21 +121,23 @@ __global__ __launch_bounds__(256) void k_group_stats(FeatureInputs in,
         lo2 = fmin(lo2, pz); hi2 = fmax(hi2, pz);
     }
     GroupStats S;
-    S.lo[0] = wave_min_d(lo0); S.lo[1] = wave_min_d(lo1); S.lo[2] = wave_min_d(lo2);
-    S.hi[0] = wave_max_d(hi0); S.hi[1] = wave_max_d(hi1); S.hi[2] = wave_max_d(hi2);
-    S.emax = wave_max_d(emax);
-    S.msum = wave_sum_d(msum);
-    S.amin = wave_min_d(amin);
-    S.amax = wave_max_d(amax);
-    S.a2 = wave_sum_d(a2);
+    S.lo[0] = sub_min_d<LPG>(lo0); S.lo[1] = sub_min_d<LPG>(lo1); S.lo[2] = sub_min_d<LPG>(lo2);
+    S.hi[0] = sub_max_d<LPG>(hi0); S.hi[1] = sub_max_d<LPG>(hi1); S.hi[2] = sub_max_d<LPG>(hi2);
+    S.emax = sub_max_d<LPG>(emax);
+    S.msum = sub_sum_d<LPG>(msum);
+    S.amin = sub_min_d<LPG>(amin);
+    S.amax = sub_max_d<LPG>(amax);
+    S.a2 = sub_sum_d<LPG>(a2);
     S.nt = thi - tlo;
     S.root = root_of(in.targets[tlo], in.cstart, in.nc, in.root);
     const int leaf = (in.tslot[tlo] >= 0) ? in.slot_leaf[in.tslot[tlo]] : -1;
     S.leaf_len = (leaf >= 0) ? (double)in.wn[leaf].len : -1.0;
     // node ids are level-major: the leaf's level is the number of level starts at or below it
-    const bool below = leaf >= 0 && lane < kMaxLevel && in.level_begin[lane + 1] <= leaf;
-    S.depth = (int)__popcll(__ballot(below));
-    if (lane == 0) out[g] = S;
+    int below = 0;
+    if (leaf >= 0)
+        for (int l = sl_lane; l < kMaxLevel; l += LPG) below += (in.level_begin[l + 1] <= leaf);
+    S.depth = sub_sum_i<LPG>(below);
+    if (sl_lane == 0) out[g] = S;
 }
 
 __device__ __forceinline__ void features_of(const GroupStats& S0, const WNode* __restrict__ wn,
@@ -413,8 +442,15 @@ gz_status launch_group_features(int ngroups, int nt, const int* gstart, const ui
                                 const gz_tree* t, GroupStats* stats, float* feats,
                                 hipStream_t st) {
     if (ngroups <= 0) return GZ_OK;
-    k_group_stats<<<(unsigned)(((size_t)ngroups + 3) / 4), 256, 0, st>>>(
-        feature_inputs(ngroups, nt, gstart, targets, tslot, p, amag, t), stats);
+    // lanes per group from the mean group size: one pass over most groups' targets
+    const FeatureInputs in = feature_inputs(ngroups, nt, gstart, targets, tslot, p, amag, t);
+    const double mean = (double)nt / (double)ngroups;
+    if (mean <= 12.0)
+        k_group_stats<16><<<(unsigned)(((size_t)ngroups + 15) / 16), 256, 0, st>>>(in, stats);
+    else if (mean <= 24.0)
+        k_group_stats<32><<<(unsigned)(((size_t)ngroups + 7) / 8), 256, 0, st>>>(in, stats);
+    else
+        k_group_stats<64><<<(unsigned)(((size_t)ngroups + 3) / 4), 256, 0, st>>>(in, stats);
     GZ_HIP(hipGetLastError());
     if (feats) {
         k_group_features<<<(unsigned)((ngroups + 255) / 256), 256, 0, st>>>(stats, ngroups, t->wn,

@@ -32,11 +32,12 @@ namespace gz {
 constexpr int kWalkStack = 768;   // node ids per wave in LDS
 constexpr int kSpillKeep = 256;   // LDS entries kept when the bottom of the stack is spilled
 constexpr int kXcdChunk = 64;     // groups per chunk of the persistent launch's XCD interleave
-constexpr int kMonoF = 128;       // monopole entries per wave, FP32 (16 B each)
-constexpr int kPairF = 96;        // particle entries per wave, FP32 (20 B each)
+constexpr int kMonoF = 120;       // monopole entries per wave, FP32 (16 B each)
+constexpr int kPairF = 88;        // particle entries per wave, FP32 (20 B each)
 constexpr int kMonoD = 96;        // monopole entries per wave, FP64 (32 B each)
 constexpr int kPairD = 64;        // particle entries per wave, FP64 (40 B each)
 constexpr int kMaxChunk = 64;     // targets evaluated together (one wave)
+constexpr int kPadF = 16;         // FP32 lists: zero-mass entries after the last (two slice steps)
 
 __device__ __forceinline__ double dmin_w(double a, double b) { return (b < a) ? b : a; }
 __device__ __forceinline__ double dmax_w(double a, double b) { return (a < b) ? b : a; }
@@ -299,14 +300,21 @@ __device__ __forceinline__ int wave_excl_scan(int v, int& total) {
     return x - v;
 }
 
-// FP32 cubic-spline force/r for u < 1 (tree.h:59-67 in single precision), r = |d|, h > r.
+// FP32 cubic-spline force/r for 0 < u < 1 (tree.h:59-67 in single precision), r = |d|, h > r.
+// Both branches are evaluated and one selected, with the hardware reciprocal (1 ulp) in place
+// of the two divisions: no divergence inside the wave, and a few instructions instead of two
+// IEEE division sequences.
 __device__ __forceinline__ float spline_inner_f(float r, float h) {
-    const float hinv = 1.0f / h;
+    const float hinv = __builtin_amdgcn_rcpf(h);
     const float hinv3 = hinv * hinv * hinv;
     const float u = r * hinv;
-    if (u < 0.5f) return hinv3 * (10.666666666666667f + u * u * (32.0f * u - 38.4f));
-    return hinv3 * (21.333333333333332f - 48.0f * u + 38.4f * u * u -
-                    10.666666666666667f * u * u * u - 0.06666666666666667f / (u * u * u));
+    const float u2 = u * u, u3 = u2 * u;
+    // 32/3 + u^2 (32 u - 38.4) and 64/3 - 48 u + 38.4 u^2 - 32/3 u^3 - 1/(15 u^3), in Horner form
+    const float lo = __builtin_fmaf(u2, __builtin_fmaf(32.0f, u, -38.4f), 10.666666666666667f);
+    const float hi = __builtin_fmaf(
+        u, __builtin_fmaf(u, __builtin_fmaf(u, -10.666666666666667f, 38.4f), -48.0f),
+        __builtin_fmaf(-0.06666666666666667f, __builtin_amdgcn_rcpf(u3), 21.333333333333332f));
+    return hinv3 * ((u < 0.5f) ? lo : hi);
 }
 
 // Five waves per SIMD: at most 96 VGPRs (the wave-uniform group data lives in scalar registers)
@@ -326,9 +334,12 @@ __global__ __launch_bounds__(64, 5) void k_walk(WalkArgs a) {
     // Interaction lists. An accepted node always lies outside every target's softening (the
     // first arm of open_node), so monopoles are Newtonian and need no softening; particle
     // entries carry theirs.
-    __shared__ real4 s_mono[kMono];
-    __shared__ real4 s_pair[kPair];
-    __shared__ real s_psoft[kPair];
+    // The FP32 lists end in up to kPadF zero-mass entries far away, so that every lane steps
+    // through whole slice steps with no per-lane bounds test (the loop stays wave-uniform).
+    constexpr int kPad = FP64 ? 0 : kPadF;
+    __shared__ real4 s_mono[kMono + kPad];
+    __shared__ real4 s_pair[kPair + kPad];
+    __shared__ real s_psoft[kPair + kPad];
     // TJ: node velocity and softening per monopole, source velocity per particle entry
     __shared__ double4 s_mext[TJ ? kMono : 1];
     __shared__ double4 s_pext[TJ ? kPair : 1];
@@ -431,7 +442,7 @@ __global__ __launch_bounds__(64, 5) void k_walk(WalkArgs a) {
                 te = (real)pe;
             }
             double accx = 0, accy = 0, accz = 0;
-            double tt[TJ ? 6 : 1] = {0}, jj[TJ ? 3 : 1] = {0}; // tidal (xx yy zz xy yz xz), jerk
+            double tt[6] = {0}, jj[3] = {0}; // tidal (xx yy zz xy yz xz), jerk (TJ only)
             double tvx = 0, tvy = 0, tvz = 0;
             if (TJ && tvalid && a.jerk) {
                 const uint32_t p = a.targets[tlo + t];
@@ -442,6 +453,12 @@ __global__ __launch_bounds__(64, 5) void k_walk(WalkArgs a) {
 
             // Evaluate both lists against every target of the chunk.
             auto flush = [&]() {
+                if (!FP64 && lane < 2 * L) {
+                    const real4 far = {(real)1e18, (real)1e18, (real)1e18, (real)0};
+                    s_mono[mcount + lane] = far;
+                    s_pair[pcount + lane] = far;
+                    s_psoft[pcount + lane] = (real)0;
+                }
                 __syncthreads();
                 if (TJ) {
                     // tree.cc:811-871: g1 and g2 are the first and second kernel-derivative
@@ -541,7 +558,8 @@ __global__ __launch_bounds__(64, 5) void k_walk(WalkArgs a) {
                     float fx = 0.f, fy = 0.f, fz = 0.f;
                     const float tef = (float)te;
                     const float txf = (float)tx, tyf = (float)ty, tzf = (float)tz;
-                    // two entries in flight per step, so their LDS reads overlap
+                    // lanes outside the slices repeat slice 0 (their sums are never used)
+                    const int js = tvalid ? sub : 0;
                     auto mono = [&](const real4 s) {
                         const float dx = (float)s.x - txf, dy = (float)s.y - tyf,
                                     dz = (float)s.z - tzf;
@@ -552,30 +570,34 @@ __global__ __launch_bounds__(64, 5) void k_walk(WalkArgs a) {
                         fy = __builtin_fmaf(dy, f, fy);
                         fz = __builtin_fmaf(dz, f, fz);
                     };
-                    int j = jstart;
-                    for (; j + L < mcount; j += 2 * L) {
-                        const real4 s0 = s_mono[j], s1 = s_mono[j + L];
-                        mono(s0);
-                        mono(s1);
-                    }
-                    for (; j < mcount; j += L) mono(s_mono[j]);
-#pragma unroll 2
-                    for (int j = jstart; j < pcount; j += L) {
+                    auto pair = [&](int j) {
                         const real4 s = s_pair[j];
                         const float es = (float)s_psoft[j];
                         const float dx = (float)s.x - txf, dy = (float)s.y - tyf,
                                     dz = (float)s.z - tzf;
                         const float r2 = __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, dz * dz));
-                        const float h = fmaxf(tef, es);
+                        const float h = (tef < es) ? es : tef;
                         const float inv = __builtin_amdgcn_rsqf(r2);
                         float g3 = inv * inv * inv;
-                        if (r2 < h * h || !(r2 > 0.f)) // inside the kernel, or the target itself
-                            g3 = (r2 > 0.f) ? spline_inner_f(r2 * inv, h) : 0.f;
-                        if (COUNT && tvalid && r2 > 0.f) ++g_pair;
+                        // inside the kernel (u <= 1, where the spline meets 1/r^3), or the target
+                        // itself (r2 = 0, skipped as tree.cc:799-802 does)
+                        if (r2 <= h * h) g3 = (r2 > 0.f) ? spline_inner_f(r2 * inv, h) : 0.f;
+                        if (COUNT && tvalid && j < pcount && r2 > 0.f) ++g_pair;
                         const float f = (float)s.w * g3;
                         fx = __builtin_fmaf(dx, f, fx);
                         fy = __builtin_fmaf(dy, f, fy);
                         fz = __builtin_fmaf(dz, f, fz);
+                    };
+                    // two slice steps per iteration, so the LDS reads of two entries overlap;
+                    // the padding covers the second step past the end
+                    for (int j0 = 0; j0 < mcount; j0 += 2 * L) {
+                        const real4 s0 = s_mono[j0 + js], s1 = s_mono[j0 + L + js];
+                        mono(s0);
+                        mono(s1);
+                    }
+                    for (int j0 = 0; j0 < pcount; j0 += 2 * L) {
+                        pair(j0 + js);
+                        pair(j0 + L + js);
                     }
                     accx += (double)fx;
                     accy += (double)fy;

@@ -15,6 +15,10 @@ b3[1].
 Pinning: the forward pass against an independent scalar loop (tests), the backward pass against
 central finite differences of this forward pass, and the bf16 rounding against numpy's float32
 bit pattern truncated with round-to-nearest-even.
+
+group_features restates the 16 features of a target group (gz_features.h: group_features16 over
+the statistics k_group_stats gathers). They are pinned by a group whose features are worked out
+by hand, and by their invariance under a change of units and a translation of the system.
 """
 from __future__ import annotations
 
@@ -125,3 +129,47 @@ def init_params(seed=0):
          "w3": rng.uniform(-1, 1, N_H) * math.sqrt(3.0 / N_H),
          "b3": np.zeros(1)}
     return pack(p)
+
+
+def group_features(gstart, x, y, z, m, soft, amag, leaf_len, depth, root):
+    """The 16 features of every group (float32, [ngroups, 16]).
+
+    gstart: group starts in the target list, ngroups + 1 entries; x, y, z, m, soft, amag: per
+    target in list order; leaf_len, depth: side and level of the leaf of each group's first
+    target; root: (cx, cy, cz, side, mass) of the tree's root (centre of mass, side, mass)."""
+    gstart = np.asarray(gstart, dtype=np.int64)
+    ng = gstart.shape[0] - 1
+    rcx, rcy, rcz, rside, rmass = (float(v) for v in root)
+    side = max(rside, 1e-300)
+    Mr = max(rmass, 1e-300)
+    out = np.zeros((ng, N_IN), dtype=np.float32)
+
+    def lg(v):
+        return math.log10(max(v, 1e-300))
+
+    for g in range(ng):
+        s = slice(gstart[g], gstart[g + 1])
+        nt = gstart[g + 1] - gstart[g]
+        lo = [float(np.min(a[s])) for a in (x, y, z)]
+        hi = [float(np.max(a[s])) for a in (x, y, z)]
+        emax = max(float(np.max(soft[s])), 0.0)
+        msum = float(np.sum(m[s]))
+        amin, amax = float(np.min(amag[s])), max(float(np.max(amag[s])), 0.0)
+        a2 = float(np.sum(amag[s] ** 2))
+        ex, ey, ez = hi[0] - lo[0], hi[1] - lo[1], hi[2] - lo[2]
+        ext = max(ex, ey, ez, 1e-12 * side)
+        emn = max(min(ex, ey, ez), 1e-12 * side)
+        cx = 0.5 * (lo[0] + hi[0]) - rcx
+        cy = 0.5 * (lo[1] + hi[1]) - rcy
+        cz = 0.5 * (lo[2] + hi[2]) - rcz
+        dist = max(math.sqrt(cx * cx + cy * cy + cz * cz), 1e-12 * side)
+        arms = math.sqrt(a2 / max(nt, 1))
+        mg = max(msum, 1e-300)
+        rho_g, rho_c = mg / ext ** 3, Mr / dist ** 3
+        amn = max(amin, 1e-300)
+        ll = float(leaf_len[g])
+        out[g] = [lg(nt), lg(ext / ll), lg(amn * ext * ext / mg), lg(amax / amn), lg(emax / ext),
+                  lg(emn / ext), lg(arms / amn), float(depth[g]) / 20.0, lg(ext / dist),
+                  abs(cz) / dist, lg(Mr / (dist * dist * amn)), lg(mg / Mr), lg(rho_g / rho_c),
+                  lg(ll / dist), lg(emax / dist), lg(emax / ll)]
+    return out

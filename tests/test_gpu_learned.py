@@ -125,6 +125,25 @@ def test_group_features_are_finite_and_match_walk_groups(gz):
     assert gs[0] == 0 and gs[-1] == tg.shape[0] and np.all(np.diff(gs) > 0)
 
 
+@pytest.mark.parametrize("group_size", [0, 16, 32, 64])
+def test_group_features_match_oracle(gz, group_size):
+    """The device features of every walk group against the oracle's restatement, from the same
+    groups, targets and tree (features are logarithms in FP32: 1e-5 absolute)."""
+    P, T, tg, aold = _walk_setup(gz)
+    feats, gstart = gz.group_features(T, P, tg, aold, group_size=group_size)
+    feats, gstart = feats.cpu().numpy(), gstart.cpu().numpy()
+    d = T.download()
+    x, y, z, m, soft = tree_cases.plummer_small(20000, seed=17, eps=0.01)
+    t = tg.cpu().numpy().astype(np.int64)
+    leaf = d["leaf_of"][t[gstart[:-1]]]
+    r = d["root"]
+    root = (d["cx"][r], d["cy"][r], d["cz"][r], float(d["wn_len"][r]), d["mass"][r])
+    want = mo.group_features(gstart, x[t], y[t], z[t], m[t], soft[t], aold.cpu().numpy(),
+                             d["wn_len"][leaf].astype(np.float64), d["level"][leaf], root)
+    assert feats.shape == want.shape
+    np.testing.assert_allclose(feats, want, rtol=0, atol=1e-5)
+
+
 def test_learned_walk_falls_back_to_reference(gz):
     """With an empty trusted label range every group keeps the reference tolerance, and the
     learned walk is the reference walk bit for bit."""
