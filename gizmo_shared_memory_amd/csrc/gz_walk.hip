@@ -332,8 +332,9 @@ __global__ __launch_bounds__(64, 5) void k_walk(WalkArgs a) {
     // launch of this kernel (GSTACK) that keeps the stack in a per-wave slice of global memory.
     __shared__ int s_stack[GSTACK ? 1 : kWalkStack];
     // Interaction lists. An accepted node always lies outside every target's softening (the
-    // first arm of open_node), so monopoles are Newtonian and need no softening; particle
-    // entries carry theirs.
+    // first arm of open_node), so monopoles are Newtonian and need no softening (the FP32 walk
+    // also lists there the particles of opened leaves that lie outside every target's kernel);
+    // particle entries carry theirs.
     // The FP32 lists end in up to kPadF zero-mass entries far away, so that every lane steps
     // through whole slice steps with no per-lane bounds test (the loop stays wave-uniform).
     constexpr int kPad = FP64 ? 0 : kPadF;
@@ -419,6 +420,16 @@ __global__ __launch_bounds__(64, 5) void k_walk(WalkArgs a) {
         const double gcy = 0.5 * (gi.lo[1] + gi.hi[1]);
         const double gcz = 0.5 * (gi.lo[2] + gi.hi[2]);
         const int nt_all = gi.thi - gi.tlo;
+        // FP32: the group box's half-sides about the centre, widened past the rounding of the
+        // group-relative coordinates, and the largest target softening. A leaf particle farther
+        // from the box than max(emax, its own softening) lies outside the kernel of every target
+        // (u > 1), where the pair force is exactly the monopole form, so it joins the monopole
+        // list instead of the particle list.
+        const double gslack = 1e-12 * (fabs(gcx) + fabs(gcy) + fabs(gcz));
+        const float bhx = (float)(0.5 * (gi.hi[0] - gi.lo[0]) * (1.0 + 1e-5) + gslack);
+        const float bhy = (float)(0.5 * (gi.hi[1] - gi.lo[1]) * (1.0 + 1e-5) + gslack);
+        const float bhz = (float)(0.5 * (gi.hi[2] - gi.lo[2]) * (1.0 + 1e-5) + gslack);
+        const float emaxf = (float)gi.emax;
 
         for (int c0 = 0; c0 < nt_all; c0 += kMaxChunk) {
             const int tlo = gi.tlo + c0;
@@ -744,40 +755,84 @@ __global__ __launch_bounds__(64, 5) void k_walk(WalkArgs a) {
                     }
                     if (lane == 0) s_lpre[nleaf] = ptot;
                     __syncthreads();
-                    for (int done = 0; done < ptot;) {
-                        if (pcount == kPair) flush();
-                        const int ncopy = min(kPair - pcount, ptot - done);
-                        for (int j = lane; j < ncopy; j += 64) {
-                            const int idx = done + j;
-                            int lo_ = 0, hi_ = nleaf; // last li with s_lpre[li] <= idx
-                            while (hi_ - lo_ > 1) {
-                                const int mid = (lo_ + hi_) >> 1;
-                                if (s_lpre[mid] <= idx) lo_ = mid;
-                                else hi_ = mid;
-                            }
-                            const int q = s_lplo[lo_] + (idx - s_lpre[lo_]);
-                            const double4 sp4 = a.src[q];
-                            real4 e;
-                            if (FP64) {
-                                e.x = (real)sp4.x; e.y = (real)sp4.y; e.z = (real)sp4.z;
-                            } else {
-                                e.x = (real)(sp4.x - gcx); e.y = (real)(sp4.y - gcy);
-                                e.z = (real)(sp4.z - gcz);
-                            }
-                            e.w = (real)sp4.w;
-                            s_pair[pcount + j] = e;
-                            s_psoft[pcount + j] = (real)a.src_soft[q];
-                            if (TJ) {
-                                double4 v = make_double4(0.0, 0.0, 0.0, 0.0);
-                                if (a.jerk) {
-                                    const uint32_t pq = a.orderbuf[q];
-                                    v.x = a.pvx[pq]; v.y = a.pvy[pq]; v.z = a.pvz[pq];
+                    if (!FP64) {
+                        // one entry per lane: near particles to the particle list, far ones to
+                        // the monopole list (both counted as pair interactions)
+                        for (int done = 0; done < ptot; done += 64) {
+                            const int idx = done + lane;
+                            const bool have = idx < ptot;
+                            real4 e = {(real)0, (real)0, (real)0, (real)0};
+                            float es = 0.f;
+                            bool far = false;
+                            if (have) {
+                                int lo_ = 0, hi_ = nleaf; // last li with s_lpre[li] <= idx
+                                while (hi_ - lo_ > 1) {
+                                    const int mid = (lo_ + hi_) >> 1;
+                                    if (s_lpre[mid] <= idx) lo_ = mid;
+                                    else hi_ = mid;
                                 }
-                                s_pext[pcount + j] = v;
+                                const int q = s_lplo[lo_] + (idx - s_lpre[lo_]);
+                                const double4 sp4 = a.src[q];
+                                e.x = (real)(sp4.x - gcx); e.y = (real)(sp4.y - gcy);
+                                e.z = (real)(sp4.z - gcz); e.w = (real)sp4.w;
+                                es = (float)a.src_soft[q];
+                                const float ox = fabsf((float)e.x) - bhx;
+                                const float oy = fabsf((float)e.y) - bhy;
+                                const float oz = fabsf((float)e.z) - bhz;
+                                const float qx = (ox > 0.f) ? ox : 0.f, qy = (oy > 0.f) ? oy : 0.f,
+                                            qz = (oz > 0.f) ? oz : 0.f;
+                                const float H = (emaxf < es) ? es : emaxf;
+                                far = qx * qx + qy * qy + qz * qz > H * H;
                             }
+                            const uint64_t fb = __ballot(have && far);
+                            const uint64_t nb = __ballot(have && !far);
+                            const int nf = __popcll(fb), nn = __popcll(nb);
+                            if (mcount + nf > kMono || pcount + nn > kPair) flush();
+                            if (have) {
+                                const uint64_t below = lanemask_lt();
+                                if (far) {
+                                    s_mono[mcount + __popcll(fb & below)] = e;
+                                } else {
+                                    const int k = pcount + __popcll(nb & below);
+                                    s_pair[k] = e;
+                                    s_psoft[k] = (real)es;
+                                }
+                            }
+                            if (COUNT && tvalid && sub == 0) g_pair += nf;
+                            mcount += nf;
+                            pcount += nn;
                         }
-                        pcount += ncopy;
-                        done += ncopy;
+                    } else {
+                        for (int done = 0; done < ptot;) {
+                            if (pcount == kPair) flush();
+                            const int ncopy = min(kPair - pcount, ptot - done);
+                            for (int j = lane; j < ncopy; j += 64) {
+                                const int idx = done + j;
+                                int lo_ = 0, hi_ = nleaf; // last li with s_lpre[li] <= idx
+                                while (hi_ - lo_ > 1) {
+                                    const int mid = (lo_ + hi_) >> 1;
+                                    if (s_lpre[mid] <= idx) lo_ = mid;
+                                    else hi_ = mid;
+                                }
+                                const int q = s_lplo[lo_] + (idx - s_lpre[lo_]);
+                                const double4 sp4 = a.src[q];
+                                real4 e; // FP64 entries keep global coordinates
+                                e.x = (real)sp4.x; e.y = (real)sp4.y; e.z = (real)sp4.z;
+                                e.w = (real)sp4.w;
+                                s_pair[pcount + j] = e;
+                                s_psoft[pcount + j] = (real)a.src_soft[q];
+                                if (TJ) {
+                                    double4 v = make_double4(0.0, 0.0, 0.0, 0.0);
+                                    if (a.jerk) {
+                                        const uint32_t pq = a.orderbuf[q];
+                                        v.x = a.pvx[pq]; v.y = a.pvy[pq]; v.z = a.pvz[pq];
+                                    }
+                                    s_pext[pcount + j] = v;
+                                }
+                            }
+                            pcount += ncopy;
+                            done += ncopy;
+                        }
                     }
                 }
                 __syncthreads();
