@@ -76,19 +76,19 @@ __device__ __forceinline__ double tidal_factor_d(double r, double h) {
     return wk * h_inv5;
 }
 
-// open_node (tree.h:109-144), operand for operand.
+// open_node (tree.h:109-144), operand for operand. The reference returns at the first arm that
+// holds; every arm is a side-effect-free predicate, so their OR is the same decision, and
+// evaluating all of them keeps the wave on one path (no divergent branches per node).
 __device__ __forceinline__ bool open_node_d(double r_sq, double len, double l_open, double mass,
                                             double node_soft, double soft_t, double aold,
                                             double theta_sq, double adx, double ady, double adz) {
     const double grow_t = soft_t + 0.6 * len, grow_n = node_soft + 0.6 * len;
-    if (r_sq < grow_t * grow_t || r_sq < grow_n * grow_n) return true;
-    if (theta_sq > 0.0 && l_open * l_open >= theta_sq * r_sq) return true;
-    if (aold > 0) {
-        if (mass * len * len > r_sq * r_sq * aold) return true;
-    } else if (theta_sq <= 0.0) {
-        return true;
-    }
-    return adx < 0.6 * len && ady < 0.6 * len && adz < 0.6 * len;
+    const bool soft_arm = (r_sq < grow_t * grow_t) | (r_sq < grow_n * grow_n);
+    const bool theta_arm = (theta_sq > 0.0) & (l_open * l_open >= theta_sq * r_sq);
+    const bool rel_arm = (aold > 0) ? (mass * len * len > r_sq * r_sq * aold) : (theta_sq <= 0.0);
+    const double box = 0.6 * len;
+    const bool inside = (adx < box) & (ady < box) & (adz < box);
+    return soft_arm | theta_arm | rel_arm | inside;
 }
 
 // ---- grouping ------------------------------------------------------------------------------------
@@ -283,12 +283,13 @@ template <> struct WalkTypes<false> {
     using real4 = float4;
 };
 
-__device__ __forceinline__ uint64_t lanemask_lt() {
-    const int lane = __lane_id();
-    return (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+// Set bits of m below this lane (mbcnt: two VALU instructions, no cross-lane traffic).
+__device__ __forceinline__ int lanes_below(uint64_t m) {
+    return (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32),
+                                          __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
 }
 
-// Exclusive prefix sum over the wave and its total.
+// Exclusive prefix sum over the wave and its total, by shuffles (any non-negative values).
 __device__ __forceinline__ int wave_excl_scan(int v, int& total) {
     int x = v;
 #pragma unroll
@@ -296,8 +297,24 @@ __device__ __forceinline__ int wave_excl_scan(int v, int& total) {
         const int y = __shfl_up(x, d);
         if (__lane_id() >= d) x += y;
     }
-    total = __shfl(x, 63);
+    total = __builtin_amdgcn_readlane(x, 63);
     return x - v;
+}
+
+// The same for small values: B bit planes, each counted with a ballot, so the prefix needs no
+// cross-lane data movement and the total is a scalar (what keeps the walk's stack pointer and
+// loop control in scalar registers). Values of 2^B or more take the shuffle scan.
+template <int B> __device__ __forceinline__ int wave_excl_scan_small(int v, int& total) {
+    if (__ballot(v >= (1 << B)) != 0ull) return wave_excl_scan(v, total);
+    int pre = 0, tot = 0;
+#pragma unroll
+    for (int b = 0; b < B; ++b) {
+        const uint64_t m = __ballot((v >> b) & 1);
+        pre += lanes_below(m) << b;
+        tot += __popcll(m) << b;
+    }
+    total = tot;
+    return pre;
 }
 
 // FP32 cubic-spline force/r for 0 < u < 1 (tree.h:59-67 in single precision), r = |d|, h > r.
@@ -668,7 +685,7 @@ __global__ __launch_bounds__(64, 5) void k_walk(WalkArgs a) {
                 // children of opened interior nodes go on the stack, then the next batch is
                 // popped and its loads are issued
                 int ctot;
-                const int coff = wave_excl_scan(nch, ctot);
+                const int coff = wave_excl_scan_small<4>(nch, ctot); // nch <= 8
                 if (sp + ctot > scap) {
                     // keep the top `keep` entries in LDS, move the rest to the spill slice
                     const int keep = min(min(sp, kSpillKeep), scap - ctot);
@@ -724,7 +741,7 @@ __global__ __launch_bounds__(64, 5) void k_walk(WalkArgs a) {
                 if (nm > 0) {
                     if (mcount + nm > kMono) flush();
                     if (accept) {
-                        const int slot_m = mcount + __popcll(mb & lanemask_lt());
+                        const int slot_m = mcount + lanes_below(mb);
                         s_mono[slot_m] = me;
                         if (TJ) {
                             double4 ext = make_double4(0.0, 0.0, 0.0, (double)w.soft);
@@ -747,9 +764,9 @@ __global__ __launch_bounds__(64, 5) void k_walk(WalkArgs a) {
                     const int nleaf = __popcll(lb);
                     int ptot;
                     const int sz = open_leaf ? (phi - plo) : 0;
-                    const int poff = wave_excl_scan(sz, ptot);
+                    const int poff = wave_excl_scan_small<5>(sz, ptot);
                     if (open_leaf) {
-                        const int li = __popcll(lb & lanemask_lt());
+                        const int li = lanes_below(lb);
                         s_lpre[li] = poff;
                         s_lplo[li] = plo;
                     }
@@ -789,11 +806,10 @@ __global__ __launch_bounds__(64, 5) void k_walk(WalkArgs a) {
                             const int nf = __popcll(fb), nn = __popcll(nb);
                             if (mcount + nf > kMono || pcount + nn > kPair) flush();
                             if (have) {
-                                const uint64_t below = lanemask_lt();
                                 if (far) {
-                                    s_mono[mcount + __popcll(fb & below)] = e;
+                                    s_mono[mcount + lanes_below(fb)] = e;
                                 } else {
-                                    const int k = pcount + __popcll(nb & below);
+                                    const int k = pcount + lanes_below(nb);
                                     s_pair[k] = e;
                                     s_psoft[k] = (real)es;
                                 }
