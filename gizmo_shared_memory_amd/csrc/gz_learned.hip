@@ -447,7 +447,7 @@ gz_status launch_group_features(int ngroups, int nt, const int* gstart, const ui
     const double mean = (double)nt / (double)ngroups;
     if (mean <= 12.0)
         k_group_stats<16><<<(unsigned)(((size_t)ngroups + 15) / 16), 256, 0, st>>>(in, stats);
-    else if (mean <= 24.0)
+    else if (mean <= 48.0)
         k_group_stats<32><<<(unsigned)(((size_t)ngroups + 7) / 8), 256, 0, st>>>(in, stats);
     else
         k_group_stats<64><<<(unsigned)(((size_t)ngroups + 3) / 4), 256, 0, st>>>(in, stats);

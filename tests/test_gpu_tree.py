@@ -171,6 +171,27 @@ def test_fp32_walk_same_interactions_and_close(gz, oracle):
     assert np.median(rel) < 1e-6 and rel.max() < 1e-4, (np.median(rel), rel.max())
 
 
+@pytest.mark.parametrize("group_size", [0, 32, 64])
+def test_fp32_walk_matches_fp64_with_mixed_softening(gz, group_size):
+    """Softenings spread over two decades, relative criterion: the FP32 walk moves a leaf
+    particle to its monopole list only when the particle is outside every target's kernel, so it
+    makes exactly the FP64 walk's interactions and stays within FP32 rounding of its forces."""
+    rng = np.random.default_rng(12)
+    x, y, z, m, _ = tree_cases.plummer_small(20000, seed=12, eps=0.01)
+    soft = 10.0 ** rng.uniform(-3.0, -1.0, x.shape[0])
+    P, T = _device_tree(gz, (x, y, z, m, soft))
+    tg = T.device_orderbuf()
+    boot = gz.accel_grouped(T, P, tg, 0.7, 1.0, 0)
+    aold = 0.0025 * torch.linalg.norm(torch.stack(boot, 1), dim=1)
+    d64 = gz.accel_grouped(T, P, tg, 0.0, 1.0, 0, aold=aold, group_size=group_size, counts=True)
+    d32 = gz.accel_grouped(T, P, tg, 0.0, 1.0, 0, aold=aold, group_size=group_size,
+                           precision="fp32", counts=True)
+    for k in ("nodes_visited", "multipole", "pairs"):
+        assert d64[3][k] == d32[3][k], k
+    rel = _relerr(tuple(_np(v) for v in d32[:3]), tuple(_np(v) for v in d64[:3]))
+    assert np.median(rel) < 1e-6 and rel.max() < 1e-4, (np.median(rel), rel.max())
+
+
 def test_fp32_walk_precision_far_from_origin(gz):
     """Group-relative FP32 keeps its accuracy when the system sits far from the origin."""
     x, y, z, m, soft = tree_cases.plummer_small(8000, seed=4, eps=0.01)
