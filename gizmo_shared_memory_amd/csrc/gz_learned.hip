@@ -36,9 +36,9 @@ __device__ __forceinline__ float gelu_grad(float x) {
 
 // ---- group features ------------------------------------------------------------------------------
 // amag is |a_prev| per target slot (no-G units, the basis of aold). The sums over a group's
-// targets are taken by LPG consecutive lanes per group (LPG = 16, 32 or 64: small groups share a
-// wavefront, so the lanes are not idle and the per-group reductions are shared); lanes read
-// consecutive targets, so the reads coalesce. The features and the network, arithmetic on a
+// targets are taken by LPG consecutive lanes per group (LPG = 16, 32 or 64: groups share a
+// wavefront, so its lanes are busy and its load latencies overlap); lanes read consecutive
+// targets, so the reads coalesce. The features and the network, arithmetic on a
 // handful of scalars, run one thread per group.
 
 struct FeatureInputs {
@@ -442,12 +442,13 @@ gz_status launch_group_features(int ngroups, int nt, const int* gstart, const ui
                                 const gz_tree* t, GroupStats* stats, float* feats,
                                 hipStream_t st) {
     if (ngroups <= 0) return GZ_OK;
-    // lanes per group from the mean group size: one pass over most groups' targets
+    // lanes per group from the mean group size: the kernel waits on chains of dependent loads,
+    // so several groups per wavefront (a few passes each) beat one pass of a whole wavefront
     const FeatureInputs in = feature_inputs(ngroups, nt, gstart, targets, tslot, p, amag, t);
     const double mean = (double)nt / (double)ngroups;
-    if (mean <= 12.0)
+    if (mean <= 64.0)
         k_group_stats<16><<<(unsigned)(((size_t)ngroups + 15) / 16), 256, 0, st>>>(in, stats);
-    else if (mean <= 48.0)
+    else if (mean <= 128.0)
         k_group_stats<32><<<(unsigned)(((size_t)ngroups + 7) / 8), 256, 0, st>>>(in, stats);
     else
         k_group_stats<64><<<(unsigned)(((size_t)ngroups + 3) / 4), 256, 0, st>>>(in, stats);
