@@ -317,21 +317,22 @@ template <int B> __device__ __forceinline__ int wave_excl_scan_small(int v, int&
     return pre;
 }
 
-// FP32 cubic-spline force/r for 0 < u < 1 (tree.h:59-67 in single precision), r = |d|, h > r.
-// Both branches are evaluated and one selected, with the hardware reciprocal (1 ulp) in place
-// of the two divisions: no divergence inside the wave, and a few instructions instead of two
-// IEEE division sequences.
-__device__ __forceinline__ float spline_inner_f(float r, float h) {
+// FP32 cubic-spline force/r inside the kernel, 0 < r <= h (tree.h:59-67 in single precision),
+// given r^2, inv = 1/r and g3 = 1/r^3 from the Newtonian path. Both branches are evaluated and one
+// selected, with the hardware reciprocal (1 ulp) of h in place of the division; the outer
+// branch's -1/(15 u^3) term times h^-3 is -1/(15 r^3) = -g3/15, so it needs no second reciprocal.
+__device__ __forceinline__ float spline_inside_f(float r2, float inv, float g3, float h) {
     const float hinv = __builtin_amdgcn_rcpf(h);
     const float hinv3 = hinv * hinv * hinv;
-    const float u = r * hinv;
-    const float u2 = u * u, u3 = u2 * u;
-    // 32/3 + u^2 (32 u - 38.4) and 64/3 - 48 u + 38.4 u^2 - 32/3 u^3 - 1/(15 u^3), in Horner form
+    const float u = (r2 * inv) * hinv;
+    const float u2 = u * u;
+    // 32/3 + u^2 (32 u - 38.4) and 64/3 - 48 u + 38.4 u^2 - 32/3 u^3, in Horner form
     const float lo = __builtin_fmaf(u2, __builtin_fmaf(32.0f, u, -38.4f), 10.666666666666667f);
     const float hi = __builtin_fmaf(
         u, __builtin_fmaf(u, __builtin_fmaf(u, -10.666666666666667f, 38.4f), -48.0f),
-        __builtin_fmaf(-0.06666666666666667f, __builtin_amdgcn_rcpf(u3), 21.333333333333332f));
-    return hinv3 * ((u < 0.5f) ? lo : hi);
+        21.333333333333332f);
+    const bool inner = u < 0.5f;
+    return __builtin_fmaf(-0.06666666666666667f, inner ? 0.0f : g3, hinv3 * (inner ? lo : hi));
 }
 
 // Five waves per SIMD: at most 96 VGPRs (the wave-uniform group data lives in scalar registers)
@@ -609,7 +610,10 @@ __global__ __launch_bounds__(64, 5) void k_walk(WalkArgs a) {
                         float g3 = inv * inv * inv;
                         // inside the kernel (u <= 1, where the spline meets 1/r^3), or the target
                         // itself (r2 = 0, skipped as tree.cc:799-802 does)
-                        if (r2 <= h * h) g3 = (r2 > 0.f) ? spline_inner_f(r2 * inv, h) : 0.f;
+                        if (r2 <= h * h) {
+                            const float sp = spline_inside_f(r2, inv, g3, h);
+                            g3 = (r2 > 0.f) ? sp : 0.f; // a select: sp is NaN at r2 = 0
+                        }
                         if (COUNT && tvalid && j < pcount && r2 > 0.f) ++g_pair;
                         const float f = (float)s.w * g3;
                         fx = __builtin_fmaf(dx, f, fx);
