@@ -318,11 +318,11 @@ template <int B> __device__ __forceinline__ int wave_excl_scan_small(int v, int&
 }
 
 // FP32 cubic-spline force/r inside the kernel, 0 < r <= h (tree.h:59-67 in single precision),
-// given r^2, inv = 1/r and g3 = 1/r^3 from the Newtonian path. Both branches are evaluated and one
-// selected, with the hardware reciprocal (1 ulp) of h in place of the division; the outer
+// given r^2, inv = 1/r and g3 = 1/r^3 from the Newtonian path, and h^2. Both branches are
+// evaluated and one selected, with the hardware 1/sqrt(h^2) in place of the division; the outer
 // branch's -1/(15 u^3) term times h^-3 is -1/(15 r^3) = -g3/15, so it needs no second reciprocal.
-__device__ __forceinline__ float spline_inside_f(float r2, float inv, float g3, float h) {
-    const float hinv = __builtin_amdgcn_rcpf(h);
+__device__ __forceinline__ float spline_inside_f(float r2, float inv, float g3, float h2) {
+    const float hinv = __builtin_amdgcn_rsqf(h2);
     const float hinv3 = hinv * hinv * hinv;
     const float u = (r2 * inv) * hinv;
     const float u2 = u * u;
@@ -352,7 +352,7 @@ __global__ __launch_bounds__(64, 5) void k_walk(WalkArgs a) {
     // Interaction lists. An accepted node always lies outside every target's softening (the
     // first arm of open_node), so monopoles are Newtonian and need no softening (the FP32 walk
     // also lists there the particles of opened leaves that lie outside every target's kernel);
-    // particle entries carry theirs.
+    // particle entries carry theirs (squared in the FP32 walk).
     // The FP32 lists end in up to kPadF zero-mass entries far away, so that every lane steps
     // through whole slice steps with no per-lane bounds test (the loop stays wave-uniform).
     constexpr int kPad = FP64 ? 0 : kPadF;
@@ -585,7 +585,7 @@ __global__ __launch_bounds__(64, 5) void k_walk(WalkArgs a) {
                     }
                 } else {
                     float fx = 0.f, fy = 0.f, fz = 0.f;
-                    const float tef = (float)te;
+                    const float tef = (float)te, te2 = tef * tef;
                     const float txf = (float)tx, tyf = (float)ty, tzf = (float)tz;
                     // lanes outside the slices repeat slice 0 (their sums are never used)
                     const int js = tvalid ? sub : 0;
@@ -601,17 +601,17 @@ __global__ __launch_bounds__(64, 5) void k_walk(WalkArgs a) {
                     };
                     auto pair = [&](int j) {
                         const real4 s = s_pair[j];
-                        const float es = (float)s_psoft[j];
+                        const float es2 = (float)s_psoft[j]; // FP32 lists hold squares
                         const float dx = (float)s.x - txf, dy = (float)s.y - tyf,
                                     dz = (float)s.z - tzf;
                         const float r2 = __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, dz * dz));
-                        const float h = (tef < es) ? es : tef;
+                        const float h2 = (te2 < es2) ? es2 : te2;
                         const float inv = __builtin_amdgcn_rsqf(r2);
                         float g3 = inv * inv * inv;
                         // inside the kernel (u <= 1, where the spline meets 1/r^3), or the target
                         // itself (r2 = 0, skipped as tree.cc:799-802 does)
-                        if (r2 <= h * h) {
-                            const float sp = spline_inside_f(r2, inv, g3, h);
+                        if (r2 <= h2) {
+                            const float sp = spline_inside_f(r2, inv, g3, h2);
                             g3 = (r2 > 0.f) ? sp : 0.f; // a select: sp is NaN at r2 = 0
                         }
                         if (COUNT && tvalid && j < pcount && r2 > 0.f) ++g_pair;
@@ -815,7 +815,7 @@ __global__ __launch_bounds__(64, 5) void k_walk(WalkArgs a) {
                                 } else {
                                     const int k = pcount + lanes_below(nb);
                                     s_pair[k] = e;
-                                    s_psoft[k] = (real)es;
+                                    s_psoft[k] = (real)(es * es);
                                 }
                             }
                             if (COUNT && tvalid && sub == 0) g_pair += nf;
