@@ -1,0 +1,449 @@
+// gz_shmem.hpp -- the shmem/tree.h build and force calls in C++, over the C ABI of gz_grav.h.
+//
+// A program written against shmem::build / accel / accel_grouped / potential / accel_brute
+// (shmem/tree.h:36-50, 371-451) switches to the device tree by including this header and writing
+// gz_shmem:: for shmem::. The calls take and return host std::vectors as the reference's do:
+//   * Particles is the reference's SoA; build() copies it to the device and keeps the copy;
+//   * a walk reads the particles as they are at the call (the reference's walks read the live
+//     arrays, tree.cc:787-788): the copy is refreshed from P and the tree's sorted source copy
+//     re-read (gz_tree_update_sources) before every walk, so a caller may drift P between walks;
+//   * outputs are resized to targets.size() and overwritten, accelerations include G, the tidal
+//     tensor does not (tree.h:414-440), aold is per target in no-G units;
+//   * with no options set, accel_grouped is the parity walk (FP64, the reference criterion, one
+//     group per leaf run at batch 0); set_walk_options() selects the FP32 or learned walks;
+//   * errors throw gz_shmem::Error with the library's message (the reference exits or asserts);
+//     what the library does not implement (zeta corrections, lazy drift, sink-direct summation)
+//     throws instead of silently differing.
+//
+// Header-only. Compile with the HIP runtime headers (-D__HIP_PLATFORM_AMD__ -I/opt/rocm/include)
+// and link libgzgrav.so and libamdhip64. All calls share one library context on the HIP device
+// current at the first call; they are not meant to be called from several threads at once.
+#ifndef GZ_SHMEM_HPP
+#define GZ_SHMEM_HPP
+
+#include <hip/hip_runtime_api.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "gz_grav.h"
+
+namespace gz_shmem {
+
+struct Error : std::runtime_error {
+    using std::runtime_error::runtime_error;
+};
+
+// shmem::Particles (tree.h:36-50). zeta must stay empty (the corrections are not implemented);
+// an empty type means all gas.
+struct Particles {
+    std::vector<double> x, y, z, m, soft;
+    std::vector<double> zeta;
+    std::vector<uint8_t> type;
+    size_t size() const { return m.size(); }
+};
+
+// shmem::BuildTimes (tree.h:371), in milliseconds of device time.
+struct BuildTimes {
+    double bbox = 0, keys = 0, sort = 0, recurse = 0, links = 0, pack = 0, total = 0;
+};
+
+// Storage of the reference's SymmetricTensor2<double> (xx yy zz xy yz xz) and Vec3<double>.
+struct SymTensor3d {
+    double data[6];
+};
+struct Vec3d {
+    double data[3];
+};
+
+// The reference's lazy-drift callback type (tree.h:183-188): accepted only as nullptr.
+struct LazyDrift;
+
+namespace detail {
+
+inline void check(gz_status s, const char* what) {
+    if (s != GZ_OK) throw Error(std::string(what) + ": " + gz_last_error());
+}
+
+inline void check_hip(hipError_t e, const char* what) {
+    if (e != hipSuccess) throw Error(std::string(what) + ": " + hipGetErrorString(e));
+}
+
+struct DevFree {
+    void operator()(void* p) const {
+        if (p) (void)hipFree(p);
+    }
+};
+
+// A device array that grows to the largest size asked of it.
+template <class T> class DevArray {
+  public:
+    T* get() const { return static_cast<T*>(mem_.get()); }
+    T* reserve(size_t count) {
+        if (count > cap_) {
+            void* p = nullptr;
+            check_hip(hipMalloc(&p, count * sizeof(T)), "hipMalloc");
+            mem_.reset(p);
+            cap_ = count;
+        }
+        return get();
+    }
+    T* upload(const T* host, size_t count) {
+        reserve(count);
+        if (count)
+            check_hip(hipMemcpy(get(), host, count * sizeof(T), hipMemcpyHostToDevice),
+                      "copy to device");
+        return get();
+    }
+    T* upload(const std::vector<T>& v) { return upload(v.data(), v.size()); }
+    void download(T* host, size_t count) const {
+        if (count)
+            check_hip(hipMemcpy(host, get(), count * sizeof(T), hipMemcpyDeviceToHost),
+                      "copy to host");
+    }
+
+  private:
+    std::unique_ptr<void, DevFree> mem_;
+    size_t cap_ = 0;
+};
+
+struct CtxFree {
+    void operator()(gz_ctx* c) const {
+        if (c) (void)gz_ctx_destroy(c);
+    }
+};
+
+// The one library context (its own stream) on the device current at the first call.
+inline gz_ctx* ctx() {
+    static std::unique_ptr<gz_ctx, CtxFree> c([] {
+        int dev = 0;
+        check_hip(hipGetDevice(&dev), "hipGetDevice");
+        gz_ctx* x = nullptr;
+        check(gz_ctx_create(dev, nullptr, &x), "gz_ctx_create");
+        return x;
+    }());
+    return c.get();
+}
+
+inline void sync() { check(gz_ctx_synchronize(ctx()), "gz_ctx_synchronize"); }
+
+// Device copy of a Particles.
+class DevParticles {
+  public:
+    gz_particles upload(const Particles& P) {
+        const size_t n = P.size();
+        if (P.x.size() != n || P.y.size() != n || P.z.size() != n ||
+            (!P.soft.empty() && P.soft.size() != n) || (!P.type.empty() && P.type.size() != n))
+            throw Error("Particles: x, y, z, m (and soft, type when given) differ in length");
+        if (!P.zeta.empty()) throw Error("Particles: zeta corrections are not supported");
+        gz_particles p{};
+        p.n = n;
+        p.x = x_.upload(P.x);
+        p.y = y_.upload(P.y);
+        p.z = z_.upload(P.z);
+        p.m = m_.upload(P.m);
+        p.soft = P.soft.empty() ? nullptr : soft_.upload(P.soft);
+        p.zeta = nullptr;
+        p.type = P.type.empty() ? nullptr : type_.upload(P.type);
+        return p;
+    }
+
+  private:
+    DevArray<double> x_, y_, z_, m_, soft_;
+    DevArray<uint8_t> type_;
+};
+
+// Per-call device buffers (targets, aold, outputs), reused between calls.
+struct Workspace {
+    DevArray<uint32_t> targets, idx;
+    DevArray<double> aold, a, extra, v, vlast;
+    DevParticles particles;
+};
+inline Workspace& ws() {
+    static Workspace w;
+    return w;
+}
+
+struct TreeFree {
+    void operator()(gz_tree* t) const {
+        if (t) (void)gz_tree_free(t);
+    }
+};
+
+} // namespace detail
+
+class Tree;
+Tree build(const Particles& P, BuildTimes* bt = nullptr, const double* const* vel = nullptr,
+           bool want_vcom = false, long long randomize_seed = -1);
+
+// shmem::Tree, device-resident. Host copies of its arrays on request; the reuse calls of
+// tree.h:250-331 in batched form (the reference's engine calls them once per particle).
+class Tree {
+  public:
+    Tree() = default;
+    gz_tree* handle() const { return t_.get(); }
+    size_t n() const { return info().n; }
+    int nnodes() const { return (int)info().nnodes; }
+    int root() const { return (int)info().root; }
+    double t_since_build = 0.0; // engine time since the build (tree.h:300)
+
+    gz_tree_info info() const {
+        gz_tree_info i{};
+        detail::check(gz_tree_get_info(t_.get(), &i), "gz_tree_get_info");
+        return i;
+    }
+    // orderbuf (Morton order of the live particles), rank (UINT32_MAX for tombstones) and
+    // leaf_of (-1 for tombstones), as the reference keeps them (tree.h:239-267)
+    std::vector<uint32_t> orderbuf() const {
+        std::vector<uint32_t> o((size_t)info().n_live);
+        detail::check(gz_tree_download_particles(t_.get(), o.data(), nullptr, nullptr, nullptr),
+                      "gz_tree_download_particles");
+        return o;
+    }
+    std::vector<uint32_t> rank() const {
+        std::vector<uint32_t> r(n());
+        detail::check(gz_tree_download_particles(t_.get(), nullptr, r.data(), nullptr, nullptr),
+                      "gz_tree_download_particles");
+        return r;
+    }
+    std::vector<int> leaf_of() const {
+        std::vector<int> l(n());
+        detail::check(gz_tree_download_particles(t_.get(), nullptr, nullptr, l.data(), nullptr),
+                      "gz_tree_download_particles");
+        return l;
+    }
+
+    // Tree::refresh_walk_nodes (tree.cc:260-274)
+    void refresh_walk_nodes(double tau) {
+        detail::check(gz_tree_refresh_walk_nodes(detail::ctx(), t_.get(), tau),
+                      "gz_tree_refresh_walk_nodes");
+        detail::sync();
+    }
+    // Tree::raise_soft (tree.cc:311-319) for the leaf of each listed particle, to soft[i]
+    void raise_soft(const std::vector<uint32_t>& particles, const std::vector<double>& soft) {
+        if (soft.size() != n()) throw Error("raise_soft: soft must hold one value per particle");
+        auto& w = detail::ws();
+        const uint32_t* d_idx = w.idx.upload(particles);
+        const double* d_soft = w.aold.upload(soft);
+        detail::check(gz_tree_raise_soft(detail::ctx(), t_.get(), d_idx, particles.size(), d_soft),
+                      "gz_tree_raise_soft");
+        detail::sync();
+    }
+    // The kick feed of mfm.cc:3798-3811 (Tree::raise_vmax and Tree::kick_node) for the active
+    // particles: vel = {vx, vy, vz} after the kick, vlast = {vx, vy, vz} at the previous feed
+    // (updated to vel), m the masses, tau = t_since_build. vlast may be nullptr (speed bound
+    // only, as for a tree built without velocities).
+    void kick_nodes(const std::vector<uint32_t>& active, const double* const* vel,
+                    const std::vector<double>& m, double* const* vlast, double tau) {
+        const size_t N = n(), na = active.size();
+        if (m.size() != N) throw Error("kick_nodes: m must hold one value per particle");
+        auto& w = detail::ws();
+        const uint32_t* d_act = w.idx.upload(active);
+        double* v = w.v.reserve(3 * N);
+        double* vl = vlast ? w.vlast.reserve(3 * N) : nullptr;
+        for (int d = 0; d < 3; ++d) {
+            detail::check_hip(hipMemcpy(v + d * N, vel[d], N * sizeof(double),
+                                        hipMemcpyHostToDevice), "copy to device");
+            if (vl)
+                detail::check_hip(hipMemcpy(vl + d * N, vlast[d], N * sizeof(double),
+                                            hipMemcpyHostToDevice), "copy to device");
+        }
+        const double* d_m = w.aold.upload(m);
+        detail::check(gz_tree_kick_nodes(detail::ctx(), t_.get(), d_act, na, v, v + N, v + 2 * N,
+                                         d_m, vl, vl ? vl + N : nullptr,
+                                         vl ? vl + 2 * N : nullptr, tau),
+                      "gz_tree_kick_nodes");
+        detail::sync();
+        if (vl)
+            for (int d = 0; d < 3; ++d)
+                detail::check_hip(hipMemcpy(vlast[d], vl + d * N, N * sizeof(double),
+                                            hipMemcpyDeviceToHost), "copy to host");
+    }
+
+  private:
+    friend Tree build(const Particles&, BuildTimes*, const double* const*, bool, long long);
+    friend struct WalkAccess;
+    std::shared_ptr<gz_tree> t_;
+};
+
+// Walk options for the calls below; nullptr (the default) is the parity walk.
+inline const gz_walk_opts*& walk_options() {
+    static const gz_walk_opts* o = nullptr;
+    return o;
+}
+inline void set_walk_options(const gz_walk_opts* o) { walk_options() = o; }
+
+// shmem::build (tree.h:384-385, tree.cc:350-526). want_vcom is implied by vel: a tree built with
+// velocities keeps the speed bound and the centre-of-mass velocities both.
+inline Tree build(const Particles& P, BuildTimes* bt, const double* const* vel, bool want_vcom,
+                  long long randomize_seed) {
+    (void)want_vcom;
+    auto& w = detail::ws();
+    const gz_particles p = w.particles.upload(P);
+    gz_build_opts o{};
+    o.randomize_seed = randomize_seed;
+    const size_t N = P.size();
+    if (vel) {
+        double* v = w.v.reserve(3 * N);
+        for (int d = 0; d < 3; ++d)
+            detail::check_hip(hipMemcpy(v + d * N, vel[d], N * sizeof(double),
+                                        hipMemcpyHostToDevice), "copy to device");
+        o.vx = v;
+        o.vy = v + N;
+        o.vz = v + 2 * N;
+    }
+    gz_tree* t = nullptr;
+    detail::check(gz_tree_build(detail::ctx(), &p, &o, &t), "gz_tree_build");
+    Tree T;
+    T.t_ = std::shared_ptr<gz_tree>(t, detail::TreeFree());
+    if (bt) {
+        const gz_build_times& s = T.info().times;
+        *bt = BuildTimes{s.bbox, s.keys, s.sort, s.recurse, s.links, s.pack, s.total};
+    }
+    return T;
+}
+
+struct WalkAccess {
+    // Particles to the device and into the tree's sorted source copy; targets and aold staged.
+    static gz_particles stage(const Tree& T, const Particles& P, const char* who) {
+        if (!T.t_) throw Error(std::string(who) + ": empty tree");
+        if (P.size() != T.n())
+            throw Error(std::string(who) + ": the particles differ in number from the tree's");
+        const gz_particles p = detail::ws().particles.upload(P);
+        detail::check(gz_tree_update_sources(detail::ctx(), T.t_.get(), &p),
+                      "gz_tree_update_sources");
+        return p;
+    }
+    static gz_tree* tree(const Tree& T) { return T.t_.get(); }
+};
+
+namespace detail {
+inline const uint32_t* stage_targets(const std::vector<uint32_t>& targets) {
+    return ws().targets.upload(targets);
+}
+inline const double* stage_aold(const double* aold, size_t nt) {
+    return aold ? ws().aold.upload(aold, nt) : nullptr;
+}
+inline void fetch3(double* a, size_t nt, std::vector<double>& ax, std::vector<double>& ay,
+                   std::vector<double>& az) {
+    ax.resize(nt);
+    ay.resize(nt);
+    az.resize(nt);
+    sync();
+    check_hip(hipMemcpy(ax.data(), a, nt * sizeof(double), hipMemcpyDeviceToHost), "copy");
+    check_hip(hipMemcpy(ay.data(), a + nt, nt * sizeof(double), hipMemcpyDeviceToHost), "copy");
+    check_hip(hipMemcpy(az.data(), a + 2 * nt, nt * sizeof(double), hipMemcpyDeviceToHost),
+              "copy");
+}
+} // namespace detail
+
+// shmem::accel (tree.h:392-394): the per-target walk, leaves always summed pair by pair.
+inline void accel(const Tree& T, const Particles& P, const std::vector<uint32_t>& targets,
+                  double theta, double G, std::vector<double>& ax, std::vector<double>& ay,
+                  std::vector<double>& az, const double* aold = nullptr) {
+    const size_t nt = targets.size();
+    const gz_particles p = WalkAccess::stage(T, P, "accel");
+    const uint32_t* d_t = detail::stage_targets(targets);
+    const double* d_ao = detail::stage_aold(aold, nt);
+    double* a = detail::ws().a.reserve(3 * nt + 1);
+    detail::check(gz_tree_accel(detail::ctx(), WalkAccess::tree(T), &p, d_t, nt, theta, G, d_ao,
+                                a, a + nt, a + 2 * nt),
+                  "gz_tree_accel");
+    detail::fetch3(a, nt, ax, ay, az);
+}
+
+// shmem::accel_grouped (tree.h:428-440). tidal and jerk come from the FP64 walk with the
+// reference criterion; jerk needs vel and a tree built with velocities.
+inline void accel_grouped(const Tree& T, const Particles& P, const std::vector<uint32_t>& targets,
+                          double theta, double G, int batch, std::vector<double>& ax,
+                          std::vector<double>& ay, std::vector<double>& az,
+                          std::vector<SymTensor3d>* tidal = nullptr, const double* aold = nullptr,
+                          const LazyDrift* lazy = nullptr, std::vector<Vec3d>* jerk = nullptr,
+                          const double* const* vel = nullptr, double sink_direct_radius = 0.0) {
+    if (lazy) throw Error("accel_grouped: lazy drift is not supported");
+    if (sink_direct_radius != 0.0)
+        throw Error("accel_grouped: sink-direct summation is not supported");
+    if (jerk && !vel) throw Error("accel_grouped: the jerk needs the velocities (vel)");
+    const size_t nt = targets.size();
+    const gz_particles p = WalkAccess::stage(T, P, "accel_grouped");
+    const uint32_t* d_t = detail::stage_targets(targets);
+    const double* d_ao = detail::stage_aold(aold, nt);
+    auto& w = detail::ws();
+    double* a = w.a.reserve(3 * nt + 1);
+    if (tidal || jerk) {
+        double* ex = w.extra.reserve(9 * nt + 1);
+        const double *vx = nullptr, *vy = nullptr, *vz = nullptr;
+        if (vel) {
+            const size_t N = P.size();
+            double* v = w.v.reserve(3 * N);
+            for (int d = 0; d < 3; ++d)
+                detail::check_hip(hipMemcpy(v + d * N, vel[d], N * sizeof(double),
+                                            hipMemcpyHostToDevice), "copy to device");
+            vx = v;
+            vy = v + N;
+            vz = v + 2 * N;
+        }
+        detail::check(gz_tree_accel_grouped_tidal(detail::ctx(), WalkAccess::tree(T), &p, d_t,
+                                                  nt, theta, G, batch, d_ao, vx, vy, vz, a,
+                                                  a + nt, a + 2 * nt, ex,
+                                                  jerk ? ex + 6 * nt : nullptr),
+                      "gz_tree_accel_grouped_tidal");
+        detail::fetch3(a, nt, ax, ay, az);
+        std::vector<double> h(9 * nt);
+        w.extra.download(h.data(), (jerk ? 9 : 6) * nt);
+        if (tidal) {
+            tidal->resize(nt);
+            for (size_t k = 0; k < nt; ++k)
+                for (int c = 0; c < 6; ++c) (*tidal)[k].data[c] = h[c * nt + k];
+        }
+        if (jerk) {
+            jerk->resize(nt);
+            for (size_t k = 0; k < nt; ++k)
+                for (int c = 0; c < 3; ++c) (*jerk)[k].data[c] = h[(6 + c) * nt + k];
+        }
+        return;
+    }
+    detail::check(gz_tree_accel_grouped(detail::ctx(), WalkAccess::tree(T), &p, d_t, nt, theta,
+                                        G, batch, d_ao, a, a + nt, a + 2 * nt, walk_options(),
+                                        nullptr),
+                  "gz_tree_accel_grouped");
+    detail::fetch3(a, nt, ax, ay, az);
+}
+
+// shmem::potential (tree.h:446-447), G included.
+inline void potential(const Tree& T, const Particles& P, const std::vector<uint32_t>& targets,
+                      double theta, double G, std::vector<double>& phi,
+                      const double* aold = nullptr) {
+    const size_t nt = targets.size();
+    const gz_particles p = WalkAccess::stage(T, P, "potential");
+    const uint32_t* d_t = detail::stage_targets(targets);
+    const double* d_ao = detail::stage_aold(aold, nt);
+    double* d_phi = detail::ws().a.reserve(nt + 1);
+    detail::check(gz_tree_potential(detail::ctx(), WalkAccess::tree(T), &p, d_t, nt, theta, G,
+                                    d_ao, d_phi),
+                  "gz_tree_potential");
+    phi.resize(nt);
+    detail::sync();
+    detail::ws().a.download(phi.data(), nt);
+}
+
+// shmem::accel_brute (tree.h:450-451): the exact FP64 sum over all particles, G included.
+inline void accel_brute(const Particles& P, const std::vector<uint32_t>& targets, double G,
+                        std::vector<double>& ax, std::vector<double>& ay,
+                        std::vector<double>& az) {
+    const size_t nt = targets.size();
+    const gz_particles p = detail::ws().particles.upload(P);
+    const uint32_t* d_t = detail::stage_targets(targets);
+    double* a = detail::ws().a.reserve(3 * nt + 1);
+    detail::check(gz_direct_sum(detail::ctx(), &p, d_t, nt, G, a, a + nt, a + 2 * nt),
+                  "gz_direct_sum");
+    detail::fetch3(a, nt, ax, ay, az);
+}
+
+} // namespace gz_shmem
+
+#endif // GZ_SHMEM_HPP

@@ -1,0 +1,214 @@
+// The shmem/tree.h calls of include/gz_shmem.hpp on the device, checked against the oracle's C
+// restatement (oracle/gz_oracle.h, itself pinned bit for bit to the reference's shmem/tree.cc) on
+// the same Plummer sphere, in the manner of the reference's own shmem/test_tree.cc.
+//
+// Tolerances: orderbuf and node count exact; the FP64 walks 1e-12 relative (the device sums
+// interactions in another order); the direct sum 1e-13; the tidal tensor 1e-11 of its largest
+// component; the FP32 walk 1e-4 of the FP64 walk. Exit status 0 when every check holds.
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <random>
+#include <string>
+#include <vector>
+
+#include "gz_oracle.h"
+#include "gz_shmem.hpp"
+
+namespace {
+
+int g_failures = 0;
+
+void expect(bool ok, const std::string& what) {
+    std::printf("%s %s\n", ok ? "ok  " : "FAIL", what.c_str());
+    if (!ok) ++g_failures;
+}
+
+// Plummer sphere of scale radius a: mass fraction u in [0, 0.999), r = a (u^-2/3 - 1)^-1/2,
+// isotropic directions; equal masses summing to 1.
+gz_shmem::Particles plummer(size_t n, double a, double eps, unsigned seed) {
+    gz_shmem::Particles P;
+    P.x.resize(n);
+    P.y.resize(n);
+    P.z.resize(n);
+    P.m.assign(n, 1.0 / (double)n);
+    P.soft.assign(n, eps);
+    std::mt19937_64 rng(seed);
+    std::uniform_real_distribution<double> U(0.0, 1.0);
+    for (size_t i = 0; i < n; ++i) {
+        const double u = U(rng) * 0.999;
+        const double r = a / std::sqrt(std::pow(u, -2.0 / 3.0) - 1.0);
+        const double ct = 2.0 * U(rng) - 1.0, st = std::sqrt(1.0 - ct * ct);
+        const double ph = 2.0 * M_PI * U(rng);
+        P.x[i] = r * st * std::cos(ph);
+        P.y[i] = r * st * std::sin(ph);
+        P.z[i] = r * ct;
+    }
+    return P;
+}
+
+gzo_particles view(const gz_shmem::Particles& P) {
+    return gzo_particles{P.size(), P.x.data(), P.y.data(), P.z.data(), P.m.data(),
+                         P.soft.data()};
+}
+
+// Largest |a - b| / |b| over the targets.
+double max_rel(const std::vector<double>& ax, const std::vector<double>& ay,
+               const std::vector<double>& az, const std::vector<double>& bx,
+               const std::vector<double>& by, const std::vector<double>& bz) {
+    double worst = 0.0;
+    for (size_t k = 0; k < bx.size(); ++k) {
+        const double dx = ax[k] - bx[k], dy = ay[k] - by[k], dz = az[k] - bz[k];
+        const double nb = std::sqrt(bx[k] * bx[k] + by[k] * by[k] + bz[k] * bz[k]);
+        worst = std::fmax(worst, std::sqrt(dx * dx + dy * dy + dz * dz) / std::fmax(nb, 1e-300));
+    }
+    return worst;
+}
+
+template <class F> bool throws(F&& f) {
+    try {
+        f();
+    } catch (const gz_shmem::Error&) {
+        return true;
+    }
+    return false;
+}
+
+std::string fmt(const char* what, double v) {
+    char b[160];
+    std::snprintf(b, sizeof b, "%s (%.2e)", what, v);
+    return b;
+}
+
+} // namespace
+
+int main() {
+    const double G = 1.0;
+    const size_t n = 20000;
+    gz_shmem::Particles P = plummer(n, 1.0, 0.01, 12345u);
+    const gzo_particles hp = view(P);
+
+    gz_shmem::BuildTimes bt;
+    gz_shmem::Tree T = gz_shmem::build(P, &bt);
+    void* to = gzo_build(&hp, -1);
+    long long sizes[3];
+    gzo_tree_sizes(to, sizes);
+    expect(T.nnodes() == (int)sizes[0], "node count equals the oracle's");
+    expect(bt.total > 0.0, "build times reported");
+
+    // the target list of an all-active step: the tree's Morton order
+    std::vector<uint32_t> tg = T.orderbuf();
+    std::vector<uint32_t> ob((size_t)sizes[2]);
+    gzo_tree_particles(to, ob.data(), nullptr, nullptr);
+    expect(tg == ob, "orderbuf equals the oracle's");
+    const size_t nt = tg.size();
+
+    // accel_grouped, batch 0: the Barnes-Hut bootstrap, then the relative criterion
+    std::vector<double> ax, ay, az, ox(nt), oy(nt), oz(nt);
+    gz_shmem::accel_grouped(T, P, tg, 0.7, G, 0, ax, ay, az);
+    gzo_accel_grouped(to, &hp, tg.data(), nt, 0.7, G, 0, nullptr, ox.data(), oy.data(),
+                      oz.data(), nullptr);
+    expect(ax.size() == nt, "outputs resized to the target count");
+    double e = max_rel(ax, ay, az, ox, oy, oz);
+    expect(e < 1e-12, fmt("accel_grouped theta=0.7 equals the oracle", e));
+    std::vector<double> aold(nt);
+    for (size_t k = 0; k < nt; ++k)
+        aold[k] = 0.0025 * std::sqrt(ox[k] * ox[k] + oy[k] * oy[k] + oz[k] * oz[k]) / G;
+    gz_shmem::accel_grouped(T, P, tg, 0.0, G, 0, ax, ay, az, nullptr, aold.data());
+    gzo_accel_grouped(to, &hp, tg.data(), nt, 0.0, G, 0, aold.data(), ox.data(), oy.data(),
+                      oz.data(), nullptr);
+    e = max_rel(ax, ay, az, ox, oy, oz);
+    expect(e < 1e-12, fmt("accel_grouped relative criterion equals the oracle", e));
+
+    // a random subset of targets in another order, batches of 8
+    std::vector<uint32_t> sub;
+    for (size_t i = 0; i < n; i += 7) sub.push_back((uint32_t)((i * 7919u) % n));
+    std::vector<double> sx(sub.size()), sy(sub.size()), sz(sub.size());
+    gz_shmem::accel_grouped(T, P, sub, 0.5, G, 8, ax, ay, az);
+    gzo_accel_grouped(to, &hp, sub.data(), sub.size(), 0.5, G, 8, nullptr, sx.data(), sy.data(),
+                      sz.data(), nullptr);
+    e = max_rel(ax, ay, az, sx, sy, sz);
+    expect(e < 1e-12, fmt("accel_grouped batch 8 on a shuffled subset equals the oracle", e));
+
+    // per-target walk and potential
+    gz_shmem::accel(T, P, sub, 0.6, G, ax, ay, az);
+    gzo_accel(to, &hp, sub.data(), sub.size(), 0.6, G, nullptr, sx.data(), sy.data(), sz.data());
+    e = max_rel(ax, ay, az, sx, sy, sz);
+    expect(e < 1e-12, fmt("accel equals the oracle", e));
+    std::vector<double> phi, ophi(sub.size());
+    gz_shmem::potential(T, P, sub, 0.6, G, phi);
+    gzo_potential(to, &hp, sub.data(), sub.size(), 0.6, G, nullptr, ophi.data());
+    double ep = 0.0;
+    for (size_t k = 0; k < sub.size(); ++k)
+        ep = std::fmax(ep, std::fabs(phi[k] - ophi[k]) / std::fabs(ophi[k]));
+    expect(ep < 1e-12, fmt("potential equals the oracle", ep));
+
+    // the judge
+    std::vector<uint32_t> probe(sub.begin(), sub.begin() + 512);
+    std::vector<double> bx(512), by(512), bz(512);
+    gz_shmem::accel_brute(P, probe, G, ax, ay, az);
+    gzo_accel_brute(&hp, probe.data(), probe.size(), G, bx.data(), by.data(), bz.data());
+    e = max_rel(ax, ay, az, bx, by, bz);
+    expect(e < 1e-13, fmt("accel_brute equals the oracle", e));
+
+    // tidal tensor from the same walk
+    std::vector<gz_shmem::SymTensor3d> tid;
+    std::vector<double> otid(6 * sub.size());
+    gz_shmem::accel_grouped(T, P, sub, 0.7, G, 0, ax, ay, az, &tid);
+    gzo_accel_grouped_tj(to, &hp, sub.data(), sub.size(), 0.7, G, 0, nullptr, nullptr, nullptr,
+                         nullptr, sx.data(), sy.data(), sz.data(), otid.data(), nullptr, nullptr);
+    double tmax = 0.0, terr = 0.0;
+    for (size_t k = 0; k < sub.size(); ++k)
+        for (int c = 0; c < 6; ++c) {
+            tmax = std::fmax(tmax, std::fabs(otid[c * sub.size() + k]));
+            terr = std::fmax(terr, std::fabs(tid[k].data[c] - otid[c * sub.size() + k]));
+        }
+    expect(tid.size() == sub.size() && terr <= 1e-11 * tmax,
+           fmt("tidal tensor equals the oracle", terr / tmax));
+    e = max_rel(ax, ay, az, sx, sy, sz);
+    expect(e < 1e-12, fmt("accelerations of the tidal walk equal the oracle", e));
+
+    // the particles drift after the build: the walk reads them as they are now
+    for (size_t i = 0; i < n; ++i) {
+        P.x[i] += 1e-4 * std::sin(3.0 * (double)i);
+        P.y[i] += 1e-4 * std::cos(5.0 * (double)i);
+    }
+    const gzo_particles hp2 = view(P);
+    gz_shmem::accel_grouped(T, P, tg, 0.7, G, 0, ax, ay, az);
+    gzo_accel_grouped(to, &hp2, tg.data(), nt, 0.7, G, 0, nullptr, ox.data(), oy.data(),
+                      oz.data(), nullptr);
+    e = max_rel(ax, ay, az, ox, oy, oz);
+    expect(e < 1e-12, fmt("walk on drifted particles equals the oracle's", e));
+
+    // the FP32 group-relative walk through the same call
+    gz_walk_opts wo{GZ_PREC_FP32_REL, GZ_CRIT_REFERENCE, 32, GZ_LAUNCH_QUEUE, 0, 0, nullptr};
+    gz_shmem::set_walk_options(&wo);
+    gz_shmem::accel_grouped(T, P, tg, 0.0, G, 0, ax, ay, az, nullptr, aold.data());
+    wo.precision = GZ_PREC_FP64;
+    std::vector<double> fx, fy, fz;
+    gz_shmem::accel_grouped(T, P, tg, 0.0, G, 0, fx, fy, fz, nullptr, aold.data());
+    gz_shmem::set_walk_options(nullptr);
+    e = max_rel(ax, ay, az, fx, fy, fz);
+    expect(e < 1e-4, fmt("FP32 walk within 1e-4 of the FP64 walk (32-target groups)", e));
+
+    // errors are exceptions, never a stray read
+    gz_shmem::Particles Z = P;
+    Z.zeta.assign(n, 0.0);
+    expect(throws([&] { gz_shmem::accel_grouped(T, Z, tg, 0.7, G, 0, ax, ay, az); }),
+           "zeta corrections are refused");
+    std::vector<uint32_t> bad = {0u, (uint32_t)n};
+    expect(throws([&] { gz_shmem::accel_grouped(T, P, bad, 0.7, G, 0, ax, ay, az); }),
+           "a target index outside the particles is refused");
+    expect(throws([&] {
+               gz_shmem::accel_grouped(T, P, tg, 0.7, G, 0, ax, ay, az, nullptr, nullptr,
+                                       nullptr, nullptr, nullptr, 1.0);
+           }),
+           "sink-direct summation is refused");
+    gz_shmem::Particles small = plummer(100, 1.0, 0.01, 1u);
+    expect(throws([&] { gz_shmem::accel_grouped(T, small, tg, 0.7, G, 0, ax, ay, az); }),
+           "particles of another size are refused");
+
+    gzo_tree_free(to);
+    std::printf("%s: %d failure(s)\n", g_failures ? "shim FAILED" : "shim ok", g_failures);
+    return g_failures ? 1 : 0;
+}

@@ -60,3 +60,20 @@ def test_library_exports_every_declared_symbol():
     assert set(_lib.exported_names()) == set(_declared())
     bound = _lib.load()
     assert bound.gz_status_string(0) == b"GZ_OK"
+
+
+def test_cpp_mirror_of_the_shmem_calls_compiles_and_links():
+    """include/gz_shmem.hpp (shmem/tree.h's build and walk calls in C++ over the C ABI) builds
+    with g++ against the library; its behaviour is tested on the GPU (tests/test_gpu_shim.py)."""
+    import shutil
+    import subprocess
+    if not shutil.which("g++") or not os.path.isdir("/opt/rocm/include"):
+        pytest.skip("g++ or the ROCm headers are missing")
+    src = os.path.join(ROOT, "tests", "cpp", "test_shmem_shim.cc")
+    r = subprocess.run(["g++", "-std=c++17", "-fsyntax-only", "-Wall", "-Wextra", "-Werror",
+                        "-D__HIP_PLATFORM_AMD__", "-I", os.path.join(ROOT, "include"), "-I",
+                        os.path.join(ROOT, "oracle"), "-I", "/opt/rocm/include", src],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    assert os.path.exists(os.path.join(ROOT, "tests", "cpp", "test_shmem_shim")), \
+        "tests/cpp/test_shmem_shim is not built (run __graft_entry__.build())"
