@@ -335,12 +335,14 @@ __device__ __forceinline__ float spline_inside_f(float r2, float inv, float g3, 
     return __builtin_fmaf(-0.06666666666666667f, inner ? 0.0f : g3, hinv3 * (inner ? lo : hi));
 }
 
-// Five waves per SIMD: at most 96 VGPRs (the wave-uniform group data lives in scalar registers)
-// and, in the FP32 walk, under 8 KB of LDS per wave (the node stack, both lists).
+// The FP32 walk runs five waves per SIMD: at most 96 VGPRs (the wave-uniform group data lives in
+// scalar registers) and under 8 KB of LDS per wave (the node stack, both lists). The FP64 lists
+// take 8.6 KB, so LDS holds four FP64 waves per SIMD, and the FP64 walk is given the registers of
+// four; the tidal walk's extra accumulators leave room for three.
 // TJ (FP64 only): the tidal tensor and, when velocities are given, the jerk, accumulated in the
 // same walk (tree.cc:811-871).
 template <bool FP64, bool COUNT, bool GSTACK, bool TJ = false>
-__global__ __launch_bounds__(64, 5) void k_walk(WalkArgs a) {
+__global__ __launch_bounds__(64, TJ ? 3 : (FP64 ? 4 : 5)) void k_walk(WalkArgs a) {
     static_assert(!TJ || FP64, "the tidal tensor and jerk are FP64 walks");
     using real = typename WalkTypes<FP64>::real;
     using real4 = typename WalkTypes<FP64>::real4;
