@@ -149,6 +149,56 @@ def cpu_baseline(x, y, z, m, soft, orderbuf, leaf_of, aold_all, seconds, threads
             "build_s": build_s}
 
 
+def other_configs(gz, ics, args):
+    """The other named workloads, timed like the headline walk: the FP64 parity walk (per-leaf
+    groups) and the FP32 walk (group size tuned), both with the relative criterion and aold from
+    the theta=0.7 bootstrap, and their errors against the FP64 direct sum on up to 65,536 probe
+    particles; for the Plummer case (the reference's own CPU-runnable size) also the reference
+    CPU walk on every particle."""
+    import torch
+    for name, make in (("plummer N=65536 eps=0.01", ics.plummer),
+                       ("hernquist N=4194304 eps=0.005", ics.hernquist),
+                       ("clustered N=16777216 eps=0.001", ics.clustered)):
+        x, y, z, m, soft = make()
+        n = x.shape[0]
+        P = gz.Particles(x, y, z, m, soft, device="cuda:0")
+        T = gz.build(P)
+        T = gz.build(P)
+        tg = T.device_orderbuf()
+        boot = gz.accel_grouped(T, P, tg, THETA_BOOT, 1.0, 0)
+        aold = ERR_TOL_FORCE_ACC * torch.linalg.norm(torch.stack(boot, 1), dim=1)
+        del boot
+        probes = ics.probes(n, min(args.probes, n))
+        pt = tg.view(torch.int32)[torch.as_tensor(probes.astype(np.int64), device="cuda:0")]
+        ex = [v.cpu().numpy() for v in gz.accel_brute(P, pt.view(torch.uint32), 1.0)]
+        out = {"workload": name, "build_ms": T.times()["total"]}
+        # the FP32 walk's group size: the fastest of 16, 32 and 64 on two timed steps (small
+        # systems have too few 64-target groups to fill the GPU)
+        best = None
+        for gs in (16, 32, 64):
+            dt = time_walk(gz, T, P, tg, aold, 2, 1, precision="fp32", group_size=gs)[1]
+            if best is None or dt < best[1]:
+                best = (gs, dt)
+        for label, prec, gs in (("fp64", "fp64", 0), ("fp32", "fp32", best[0])):
+            (ax, ay, az, _), dt, kms = time_walk(gz, T, P, tg, aold, args.steps, args.warmup,
+                                                 precision=prec, group_size=gs)
+            e = rel_err(tuple(v.cpu().numpy()[probes] for v in (ax, ay, az)), ex)
+            out[label] = {"group_size": gs, "ms_per_step": dt * 1e3, "kernel_ms": kms,
+                          "evals_per_s": n / dt, "err": err_stats(e)}
+            del ax, ay, az
+        if n <= 65536 and not args.no_cpu:
+            td = T.download()
+            cpu = cpu_baseline(x, y, z, m, soft, td["orderbuf"], td["leaf_of"],
+                               aold.cpu().numpy(), args.cpu_seconds,
+                               args.cpu_threads or min(16, os.cpu_count() or 1))
+            out["cpu_baseline"] = {k: cpu[k] for k in ("value", "unit", "cores", "kind",
+                                                       "sample")}
+            out["fp64_over_cpu"] = out["fp64"]["evals_per_s"] / cpu["value"]
+        log(f"config: {json.dumps(out)}")
+        del T, P, tg, aold
+        torch.cuda.empty_cache()
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -165,6 +215,11 @@ def main():
                          "the FP32 walk); a profiler run can keep to one walk kernel")
     ap.add_argument("--model", default=os.path.join(ROOT, "gizmo_shared_memory_amd",
                                                      "learned_criterion.json"))
+    ap.add_argument("--configs", action="store_true",
+                    help="also time the other named workloads (Plummer 64k, Hernquist 4M, "
+                         "clustered 16M: build, FP64 and FP32 walks, errors against the direct "
+                         "sum; the reference CPU walk beside the Plummer case) and log one JSON "
+                         "object per workload to stderr; the stdout line is unchanged")
     ap.add_argument("--copy-mb", type=int, default=0,
                     help="after the walks, run one plain device copy of this many MiB (a known "
                          "traffic reference for hardware byte counters)")
@@ -283,6 +338,9 @@ def main():
             glob = measure(f"global_tol_x{glob:.2f}", "fp32", gsl, args.steps, args.warmup,
                            scale=glob)
         results["global_tolerance_fp32"] = glob
+
+    if args.configs:
+        other_configs(gz, ics, args)
 
     if args.copy_mb > 0:
         src = torch.ones(args.copy_mb << 18, dtype=torch.float32, device="cuda:0")
