@@ -431,6 +431,58 @@ inline void potential(const Tree& T, const Particles& P, const std::vector<uint3
     detail::ws().a.download(phi.data(), nt);
 }
 
+// shmem::ngb_search (hydro.h:73-75, hydro.cc:37-108): APPENDS to found the particles within
+// radius of centre, in the tree's Morton order, with node boxes widened by vmax * t_since_build.
+// Periodic boxes (box != 0) and lazy drift are not supported. One launch per call: callers with
+// many centres use ngb_search_many.
+inline void ngb_search_many(const Tree& T, const Particles& P, const std::vector<Vec3d>& centres,
+                            const std::vector<double>& radius,
+                            std::vector<std::vector<uint32_t>>& found) {
+    const size_t nq = centres.size();
+    if (radius.size() != nq) throw Error("ngb_search: one radius per centre");
+    found.resize(nq);
+    if (nq == 0) return;
+    const gz_particles p = WalkAccess::stage(T, P, "ngb_search");
+    (void)p;
+    std::vector<double> c(3 * nq);
+    for (size_t q = 0; q < nq; ++q)
+        for (int d = 0; d < 3; ++d) c[d * nq + q] = centres[q].data[d];
+    auto& w = detail::ws();
+    double* d_c = w.v.upload(c);
+    const double* d_r = w.aold.upload(radius);
+    detail::DevArray<size_t> start;
+    start.reserve(nq + 1);
+    size_t total = 0;
+    detail::check(gz_tree_ngb_search(detail::ctx(), WalkAccess::tree(T), d_c, d_c + nq,
+                                     d_c + 2 * nq, d_r, nq, T.t_since_build, start.get(), nullptr,
+                                     0, &total),
+                  "gz_tree_ngb_search");
+    detail::DevArray<uint32_t> hits;
+    hits.reserve(total + 1);
+    detail::check(gz_tree_ngb_search(detail::ctx(), WalkAccess::tree(T), d_c, d_c + nq,
+                                     d_c + 2 * nq, d_r, nq, T.t_since_build, start.get(),
+                                     hits.get(), total, &total),
+                  "gz_tree_ngb_search");
+    detail::sync();
+    std::vector<size_t> hs(nq + 1);
+    std::vector<uint32_t> hh(total);
+    start.download(hs.data(), nq + 1);
+    hits.download(hh.data(), total);
+    for (size_t q = 0; q < nq; ++q)
+        found[q].insert(found[q].end(), hh.begin() + (std::ptrdiff_t)hs[q],
+                        hh.begin() + (std::ptrdiff_t)hs[q + 1]);
+}
+
+inline void ngb_search(const Tree& T, const Particles& P, const Vec3d& centre, double radius,
+                       std::vector<uint32_t>& found, double box = 0.0,
+                       const LazyDrift* lazy = nullptr) {
+    if (box != 0.0) throw Error("ngb_search: periodic boxes are not supported");
+    if (lazy) throw Error("ngb_search: lazy drift is not supported");
+    std::vector<std::vector<uint32_t>> f(1);
+    ngb_search_many(T, P, std::vector<Vec3d>{centre}, std::vector<double>{radius}, f);
+    found.insert(found.end(), f[0].begin(), f[0].end());
+}
+
 // shmem::accel_brute (tree.h:450-451): the exact FP64 sum over all particles, G included.
 inline void accel_brute(const Particles& P, const std::vector<uint32_t>& targets, double G,
                         std::vector<double>& ax, std::vector<double>& ay,

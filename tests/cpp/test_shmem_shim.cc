@@ -5,6 +5,7 @@
 // Tolerances: orderbuf and node count exact; the FP64 walks 1e-12 relative (the device sums
 // interactions in another order); the direct sum 1e-13; the tidal tensor 1e-11 of its largest
 // component; the FP32 walk 1e-4 of the FP64 walk. Exit status 0 when every check holds.
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -209,6 +210,109 @@ int main() {
            "particles of another size are refused");
 
     gzo_tree_free(to);
+
+    // ---- reuse between rebuilds: build with velocities, one kick feed, refresh, drift, walk;
+    // raise_soft; the neighbour search (momentum sums are FP64 atomics on the device: 1e-10)
+    {
+        const size_t m2 = 10000;
+        gz_shmem::Particles Q = plummer(m2, 1.0, 0.01, 777u);
+        std::vector<double> vx(m2), vy(m2), vz(m2);
+        std::mt19937_64 rng(99u);
+        std::normal_distribution<double> N(0.0, 0.3);
+        for (size_t i = 0; i < m2; ++i) {
+            vx[i] = N(rng);
+            vy[i] = N(rng);
+            vz[i] = N(rng);
+        }
+        const double* vel[3] = {vx.data(), vy.data(), vz.data()};
+        gz_shmem::Tree R = gz_shmem::build(Q, nullptr, vel, true);
+        const gzo_particles hq = view(Q);
+        void* tq = gzo_build_vel(&hq, -1, vx.data(), vy.data(), vz.data());
+        // a third of the particles are kicked at tau = 0.05
+        std::vector<uint32_t> act;
+        for (size_t i = 0; i < m2; i += 3) act.push_back((uint32_t)i);
+        std::vector<double> lx = vx, ly = vy, lz = vz, ox2 = vx, oy2 = vy, oz2 = vz;
+        std::vector<double> kx = vx, ky = vy, kz = vz;
+        for (uint32_t i : act) {
+            kx[i] += 0.05 * N(rng);
+            ky[i] += 0.05 * N(rng);
+            kz[i] += 0.05 * N(rng);
+        }
+        const double* kv[3] = {kx.data(), ky.data(), kz.data()};
+        double* last[3] = {lx.data(), ly.data(), lz.data()};
+        R.kick_nodes(act, kv, Q.m, last, 0.05);
+        gzo_kick_nodes(tq, act.data(), act.size(), kx.data(), ky.data(), kz.data(), Q.m.data(),
+                       ox2.data(), oy2.data(), oz2.data(), 0.05);
+        expect(lx == ox2 && ly == oy2 && lz == oz2, "kick feed updates v_last as the oracle does");
+        R.refresh_walk_nodes(0.1);
+        gzo_refresh_walk_nodes(tq, 0.1);
+        R.t_since_build = 0.1;
+        // particles at tau = 0.1: v before the kick until 0.05, the kicked v after
+        for (size_t i = 0; i < m2; ++i) {
+            Q.x[i] += 0.05 * vx[i] + 0.05 * kx[i];
+            Q.y[i] += 0.05 * vy[i] + 0.05 * ky[i];
+            Q.z[i] += 0.05 * vz[i] + 0.05 * kz[i];
+        }
+        const gzo_particles hq2 = view(Q);
+        std::vector<uint32_t> tr = R.orderbuf();
+        std::vector<double> rx(tr.size()), ry(tr.size()), rz(tr.size());
+        gz_shmem::accel_grouped(R, Q, tr, 0.6, G, 0, ax, ay, az);
+        gzo_accel_grouped(tq, &hq2, tr.data(), tr.size(), 0.6, G, 0, nullptr, rx.data(),
+                          ry.data(), rz.data(), nullptr);
+        e = max_rel(ax, ay, az, rx, ry, rz);
+        expect(e < 1e-10, fmt("walk over refreshed nodes on drifted particles equals the oracle",
+                              e));
+        // raise_soft on every 50th particle
+        std::vector<uint32_t> grow;
+        for (size_t i = 0; i < m2; i += 50) grow.push_back((uint32_t)i);
+        for (uint32_t i : grow) Q.soft[i] *= 3.0;
+        R.raise_soft(grow, Q.soft);
+        gzo_raise_soft(tq, grow.data(), grow.size(), Q.soft.data());
+        const gzo_particles hq3 = view(Q);
+        gz_shmem::accel_grouped(R, Q, tr, 0.6, G, 0, ax, ay, az);
+        gzo_accel_grouped(tq, &hq3, tr.data(), tr.size(), 0.6, G, 0, nullptr, rx.data(),
+                          ry.data(), rz.data(), nullptr);
+        e = max_rel(ax, ay, az, rx, ry, rz);
+        expect(e < 1e-10, fmt("walk after raise_soft equals the oracle", e));
+        // neighbour search around a few particles, radius 0.2, on the aged tree
+        std::vector<gz_shmem::Vec3d> cen;
+        std::vector<double> rad, hcx, hcy, hcz;
+        for (size_t i = 0; i < m2; i += 997) {
+            cen.push_back(gz_shmem::Vec3d{{Q.x[i], Q.y[i], Q.z[i]}});
+            rad.push_back(0.2);
+            hcx.push_back(Q.x[i]);
+            hcy.push_back(Q.y[i]);
+            hcz.push_back(Q.z[i]);
+        }
+        std::vector<std::vector<uint32_t>> found;
+        gz_shmem::ngb_search_many(R, Q, cen, rad, found);
+        std::vector<size_t> ostart(cen.size() + 1);
+        const size_t tot = gzo_ngb_search(tq, &hq3, hcx.data(), hcy.data(), hcz.data(),
+                                          rad.data(), cen.size(), 0.1, ostart.data(), nullptr, 0);
+        std::vector<uint32_t> ofound(tot + 1);
+        gzo_ngb_search(tq, &hq3, hcx.data(), hcy.data(), hcz.data(), rad.data(), cen.size(), 0.1,
+                       ostart.data(), ofound.data(), tot);
+        bool same = found.size() == cen.size();
+        size_t nfound = 0;
+        for (size_t q = 0; same && q < cen.size(); ++q) {
+            std::vector<uint32_t> a(found[q]), b(ofound.begin() + (std::ptrdiff_t)ostart[q],
+                                                 ofound.begin() + (std::ptrdiff_t)ostart[q + 1]);
+            std::sort(a.begin(), a.end());
+            std::sort(b.begin(), b.end());
+            same = a == b;
+            nfound += a.size();
+        }
+        expect(same && nfound > 0, "neighbour lists equal the oracle's (" +
+                                       std::to_string(nfound) + " neighbours)");
+        std::vector<uint32_t> one;
+        gz_shmem::ngb_search(R, Q, cen[0], rad[0], one);
+        std::sort(one.begin(), one.end());
+        std::vector<uint32_t> first(found[0]);
+        std::sort(first.begin(), first.end());
+        expect(one == first, "single-centre ngb_search appends the same list");
+        gzo_tree_free(tq);
+    }
+
     std::printf("%s: %d failure(s)\n", g_failures ? "shim FAILED" : "shim ok", g_failures);
     return g_failures ? 1 : 0;
 }
