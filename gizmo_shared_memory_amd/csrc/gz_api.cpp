@@ -126,7 +126,8 @@ gz_status walk(gz_ctx* ctx, const gz_tree* tree, const gz_particles* p, const ui
     GZ_TRY(gz::fill_zero_u32(reinterpret_cast<uint32_t*>(misc), 32, st));
     int* gseg = ex.nseg > 1 ? reinterpret_cast<int*>(b + o_gseg) : nullptr;
     int* tslot = reinterpret_cast<int*>(b + o_tslot);
-    GZ_TRY(gz::launch_group_prep(targets, n_t, tree->rank, tree->slot_leaf, tslot, tree->parent,
+    GZ_TRY(gz::launch_group_prep(targets, n_t, tree->rank, tree->orderbuf, tree->n_live,
+                                 tree->slot_leaf, tslot, tree->parent,
                                  tree->wn, batch,
                                  group_size, p->n, ex.seg, ex.nseg, gseg, gkey, flag, gid,
                                  ngroups_dev, status, b + o_tmp, st));

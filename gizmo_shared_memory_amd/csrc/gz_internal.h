@@ -142,6 +142,7 @@ void clear_error();
 
 // ---- walks (gz_walk.hip) ------------------------------------------------------------------------
 gz_status launch_group_prep(const uint32_t* targets, size_t nt, const uint32_t* rank,
+                            const uint32_t* orderbuf, size_t n_live,
                             const int* slot_leaf, int* tslot,
                             const int* parent, const WNode* wn, int batch, int group_size,
                             size_t n, const uint32_t* seg, int nseg, int* gseg, int* gkey,

@@ -97,6 +97,8 @@ __device__ __forceinline__ bool open_node_d(double r_sq, double len, double l_op
 // A target outside the tree (leaf_of < 0) always starts a group of its own (tree.cc:663-664).
 __global__ __launch_bounds__(256) void k_group_flags(const uint32_t* __restrict__ targets,
                                                      size_t nt, const uint32_t* __restrict__ rank,
+                                                     const uint32_t* __restrict__ orderbuf,
+                                                     size_t n_live,
                                                      const int* __restrict__ slot_leaf,
                                                      int* __restrict__ tslot,
                                                      const int* __restrict__ parent,
@@ -124,8 +126,10 @@ __global__ __launch_bounds__(256) void k_group_flags(const uint32_t* __restrict_
         return;
     }
     // the target's sorted slot: its data is read from the tree's sorted copy, which for a
-    // Morton-ordered target list is a near-sequential read (the particle arrays are gathers)
-    const uint32_t r = rank[targets[i]];
+    // Morton-ordered target list is a near-sequential read (the particle arrays are gathers).
+    // A target that is orderbuf[i] sits in slot i (an all-active list is orderbuf itself), which
+    // spares the random read of rank.
+    const uint32_t r = (i < n_live && orderbuf[i] == targets[i]) ? (uint32_t)i : rank[targets[i]];
     const int slot = (r == 0xFFFFFFFFu) ? -1 : (int)r;
     tslot[i] = slot;
     int key;
@@ -1071,12 +1075,14 @@ gz_status launch_walk_retry(const WalkArgs& a, int waves, bool fp64, bool count,
 }
 
 gz_status launch_group_prep(const uint32_t* targets, size_t nt, const uint32_t* rank,
+                            const uint32_t* orderbuf, size_t n_live,
                             const int* slot_leaf, int* tslot, const int* parent, const WNode* wn, int batch, int group_size,
                             size_t n, const uint32_t* seg, int nseg, int* gseg, int* gkey,
                             uint32_t* flag, uint32_t* gid, uint32_t* ngroups_dev, int* status,
                             void* scan_tmp, hipStream_t st) {
     const unsigned grid = (unsigned)((nt + 255) / 256);
-    k_group_flags<<<grid, 256, 0, st>>>(targets, nt, rank, slot_leaf, tslot, parent, wn, batch,
+    k_group_flags<<<grid, 256, 0, st>>>(targets, nt, rank, orderbuf, n_live, slot_leaf, tslot,
+                                        parent, wn, batch,
                                         group_size, n,
                                         seg, nseg, gseg, gkey, status);
     k_group_flags2<<<grid, 256, 0, st>>>(nt, batch, gkey, nseg > 1 ? gseg : nullptr, flag);
