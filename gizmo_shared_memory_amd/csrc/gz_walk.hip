@@ -53,6 +53,15 @@ __device__ __forceinline__ double spline_force_over_r_d(double r, double h) {
                      0.06666666666666667 / (u * u * u));
 }
 
+// 1/sqrt(x) for x > 0 in FP64 to a few ulp: the hardware estimate (about 2^-23 relative) and two
+// Newton steps y <- y (3 - x y^2) / 2, each squaring the relative error.
+__device__ __forceinline__ double rsqrt_d(double x) {
+    double y = __builtin_amdgcn_rsq(x);
+#pragma unroll
+    for (int k = 0; k < 2; ++k) y = y * __builtin_fma(-0.5, x * (y * y), 1.5);
+    return y;
+}
+
 // spline_potential (tree.h:72-80), operand for operand: exactly -1/r beyond h.
 __device__ __forceinline__ double spline_potential_d(double r, double h) {
     const double h_inv = 1.0 / h;
@@ -565,12 +574,14 @@ __global__ __launch_bounds__(64, TJ ? 3 : (FP64 ? 4 : 5)) void k_walk(WalkArgs a
                     }
                 } else if (FP64) {
                     for (int j = jstart; j < mcount; j += L) {
-                        // u >= 1 for every accepted node, so the spline is 1/r^3 (tree.h:63)
+                        // u >= 1 for every accepted node, so the spline is 1/r^3 (tree.h:63),
+                        // here from 1/sqrt(r^2) to a few ulp instead of a square root and a
+                        // division (r^2 > 0: an accepted node lies outside every target's box)
                         const real4 s = s_mono[j];
                         const double dx = (double)s.x - (double)tx, dy = (double)s.y - (double)ty,
                                      dz = (double)s.z - (double)tz;
-                        const double r = sqrt(dx * dx + dy * dy + dz * dz);
-                        const double f = (double)s.w * (1.0 / (r * r * r));
+                        const double inv = rsqrt_d(dx * dx + dy * dy + dz * dz);
+                        const double f = (double)s.w * (inv * inv * inv);
                         accx += dx * f;
                         accy += dy * f;
                         accz += dz * f;
