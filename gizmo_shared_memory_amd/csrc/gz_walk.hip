@@ -62,6 +62,22 @@ __device__ __forceinline__ double rsqrt_d(double x) {
     return y;
 }
 
+// spline_force_over_r from r^2, with r and 1/r^3 from rsqrt_d (a few ulp from the reference's
+// sqrt and division) and the spline's 1/u^3 as (h/r)^3; the branches and polynomials are the
+// reference's (tree.h:59-67).
+__device__ __forceinline__ double spline_force_over_r_r2(double r2, double h) {
+    const double inv = rsqrt_d(r2);
+    const double r = r2 * inv;
+    const double h_inv = 1.0 / h;
+    const double u = r * h_inv;
+    if (u >= 1.0) return inv * inv * inv;
+    const double h_inv3 = h_inv * h_inv * h_inv;
+    if (u < 0.5) return h_inv3 * (10.666666666666667 + u * u * (32.0 * u - 38.4));
+    const double hu = h * inv; // 1/u
+    return h_inv3 * (21.333333333333332 - 48.0 * u + 38.4 * u * u - 10.666666666666667 * u * u * u -
+                     0.06666666666666667 * (hu * hu * hu));
+}
+
 // spline_potential (tree.h:72-80), operand for operand: exactly -1/r beyond h.
 __device__ __forceinline__ double spline_potential_d(double r, double h) {
     const double h_inv = 1.0 / h;
@@ -595,7 +611,7 @@ __global__ __launch_bounds__(64, TJ ? 3 : (FP64 ? 4 : 5)) void k_walk(WalkArgs a
                         if (r2 <= 0) continue;
                         if (COUNT && tvalid) ++g_pair;
                         const double h = dmax_w(dmax_w((double)te, es), 1e-300);
-                        const double f = (double)s.w * spline_force_over_r_d(sqrt(r2), h);
+                        const double f = (double)s.w * spline_force_over_r_r2(r2, h);
                         accx += dx * f;
                         accy += dy * f;
                         accz += dz * f;
