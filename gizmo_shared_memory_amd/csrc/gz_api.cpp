@@ -131,11 +131,14 @@ gz_status walk(gz_ctx* ctx, const gz_tree* tree, const gz_particles* p, const ui
                                  tree->wn, batch,
                                  group_size, p->n, ex.seg, ex.nseg, gseg, gkey, flag, gid,
                                  ngroups_dev, status, b + o_tmp, st));
-    uint32_t ngroups0 = 0;
-    GZ_HIP(hipMemcpyAsync(&ngroups0, ngroups_dev, sizeof ngroups0, hipMemcpyDeviceToHost, st));
-    int prep_status = 0;
-    GZ_HIP(hipMemcpyAsync(&prep_status, status, sizeof prep_status, hipMemcpyDeviceToHost, st));
+    // the group count (misc[4]) and the status word (misc[6]) in one read-back
+    long long head[3] = {0, 0, 0};
+    GZ_HIP(hipMemcpyAsync(head, misc + 4, sizeof head, hipMemcpyDeviceToHost, st));
     GZ_HIP(hipStreamSynchronize(st));
+    uint32_t ngroups0 = 0;
+    int prep_status = 0;
+    std::memcpy(&ngroups0, &head[0], sizeof ngroups0);
+    std::memcpy(&prep_status, &head[2], sizeof prep_status);
     if (prep_status & 2)
         return fail(GZ_ERR_INVALID, "%s: a target index is outside the particle set (n=%zu)", who,
                     p->n);
