@@ -1,8 +1,9 @@
 // Tree walks on the device: target grouping, per-group opening bounds, the group-coherent walk
 // with an LDS interaction list (FP64 and FP32 group-relative), and the exact direct sum.
 //
-// Semantics follow shmem::accel_grouped (shmem/tree.cc:626-897) with tidal = jerk = lazy = NULL
-// and no sink-direct radius:
+// Semantics follow shmem::accel_grouped (shmem/tree.cc:626-897) without lazy drift or a
+// sink-direct radius; the tidal tensor and jerk (tree.cc:811-871, the TJ instantiation) and
+// shmem::potential (tree.cc:899-944, per-target groups) are the same walk with other sums:
 //   * groups are runs of Morton-ordered targets that share a leaf (tree.cc:656-667), or fixed
 //     windows of `batch` targets, or (group_size > 0) runs under the highest node that holds at most
 //     group_size particles;
