@@ -40,8 +40,10 @@ constexpr int kPairD = 64;        // particle entries per wave, FP64 (40 B each)
 constexpr int kMaxChunk = 64;     // targets evaluated together (one wave)
 constexpr int kPadF = 16;         // FP32 lists: zero-mass entries after the last (two slice steps)
 
-__device__ __forceinline__ double dmin_w(double a, double b) { return (b < a) ? b : a; }
-__device__ __forceinline__ double dmax_w(double a, double b) { return (a < b) ? b : a; }
+// std::min / std::max of two numbers that are never NaN: one v_min_f64 / v_max_f64 (a compare
+// and two selects otherwise); the two forms differ only in the sign of a zero result.
+__device__ __forceinline__ double dmin_w(double a, double b) { return __builtin_fmin(a, b); }
+__device__ __forceinline__ double dmax_w(double a, double b) { return __builtin_fmax(a, b); }
 
 // ---- kernels of the pair force (tree.h:59-67), FP64 as the reference writes them --------------
 __device__ __forceinline__ double spline_force_over_r_d(double r, double h) {
@@ -633,13 +635,12 @@ __global__ __launch_bounds__(64, TJ ? 3 : (FP64 ? 4 : 5)) void k_walk(WalkArgs a
                         fy = __builtin_fmaf(dy, f, fy);
                         fz = __builtin_fmaf(dz, f, fz);
                     };
-                    auto pair = [&](int j) {
-                        const real4 s = s_pair[j];
-                        const float es2 = (float)s_psoft[j]; // FP32 lists hold squares
+                    // es2: the entry's squared softening (FP32 lists hold squares)
+                    auto pair = [&](const real4 s, const float es2, int j) {
                         const float dx = (float)s.x - txf, dy = (float)s.y - tyf,
                                     dz = (float)s.z - tzf;
                         const float r2 = __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, dz * dz));
-                        const float h2 = (te2 < es2) ? es2 : te2;
+                        const float h2 = __builtin_fmaxf(te2, es2);
                         const float inv = __builtin_amdgcn_rsqf(r2);
                         float g3 = inv * inv * inv;
                         // inside the kernel (u <= 1, where the spline meets 1/r^3), or the target
@@ -656,14 +657,31 @@ __global__ __launch_bounds__(64, TJ ? 3 : (FP64 ? 4 : 5)) void k_walk(WalkArgs a
                     };
                     // two slice steps per iteration, so the LDS reads of two entries overlap;
                     // the padding covers the second step past the end
-                    for (int j0 = 0; j0 < mcount; j0 += 2 * L) {
-                        const real4 s0 = s_mono[j0 + js], s1 = s_mono[j0 + L + js];
-                        mono(s0);
-                        mono(s1);
-                    }
-                    for (int j0 = 0; j0 < pcount; j0 += 2 * L) {
-                        pair(j0 + js);
-                        pair(j0 + L + js);
+                    if (L == 1) {
+                        // one slice (more than 32 targets): every lane reads the same entries,
+                        // the second at a fixed offset from the first
+                        for (int j0 = 0; j0 < mcount; j0 += 2) {
+                            const real4 s0 = s_mono[j0], s1 = s_mono[j0 + 1];
+                            mono(s0);
+                            mono(s1);
+                        }
+                        for (int j0 = 0; j0 < pcount; j0 += 2) {
+                            const real4 s0 = s_pair[j0], s1 = s_pair[j0 + 1];
+                            const float e0 = s_psoft[j0], e1 = s_psoft[j0 + 1];
+                            pair(s0, e0, j0);
+                            pair(s1, e1, j0 + 1);
+                        }
+                    } else {
+                        for (int j0 = 0; j0 < mcount; j0 += 2 * L) {
+                            const real4 s0 = s_mono[j0 + js], s1 = s_mono[j0 + L + js];
+                            mono(s0);
+                            mono(s1);
+                        }
+                        for (int j0 = 0; j0 < pcount; j0 += 2 * L) {
+                            const int j1 = j0 + L;
+                            pair(s_pair[j0 + js], s_psoft[j0 + js], j0 + js);
+                            pair(s_pair[j1 + js], s_psoft[j1 + js], j1 + js);
+                        }
                     }
                     accx += (double)fx;
                     accy += (double)fy;
