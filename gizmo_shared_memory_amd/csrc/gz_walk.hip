@@ -321,32 +321,22 @@ __device__ __forceinline__ int lanes_below(uint64_t m) {
                                           __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
 }
 
-// Exclusive prefix sum over the wave and its total, by shuffles (any non-negative values).
+// Exclusive prefix sum over the wave and its total. The inclusive sum is built with DPP: shifts
+// by 1, 2, 4 and 8 lanes within each row of 16 (lanes shifted in from outside the row read 0),
+// then lane 15 of rows 0 and 2 added into rows 1 and 3, and lane 31 into rows 2 and 3 (a row
+// outside the row mask keeps its value). Six VALU adds and no LDS traffic; the total is a
+// scalar (what keeps the walk's stack pointer and loop control in scalar registers). Every lane
+// of the wave must be active.
 __device__ __forceinline__ int wave_excl_scan(int v, int& total) {
     int x = v;
-#pragma unroll
-    for (int d = 1; d < 64; d <<= 1) {
-        const int y = __shfl_up(x, d);
-        if (__lane_id() >= d) x += y;
-    }
+    x += __builtin_amdgcn_update_dpp(0, x, 0x111, 0xf, 0xf, true);  // row_shr:1
+    x += __builtin_amdgcn_update_dpp(0, x, 0x112, 0xf, 0xf, true);  // row_shr:2
+    x += __builtin_amdgcn_update_dpp(0, x, 0x114, 0xf, 0xf, true);  // row_shr:4
+    x += __builtin_amdgcn_update_dpp(0, x, 0x118, 0xf, 0xf, true);  // row_shr:8
+    x += __builtin_amdgcn_update_dpp(0, x, 0x142, 0xa, 0xf, false); // row_bcast:15
+    x += __builtin_amdgcn_update_dpp(0, x, 0x143, 0xc, 0xf, false); // row_bcast:31
     total = __builtin_amdgcn_readlane(x, 63);
     return x - v;
-}
-
-// The same for small values: B bit planes, each counted with a ballot, so the prefix needs no
-// cross-lane data movement and the total is a scalar (what keeps the walk's stack pointer and
-// loop control in scalar registers). Values of 2^B or more take the shuffle scan.
-template <int B> __device__ __forceinline__ int wave_excl_scan_small(int v, int& total) {
-    if (__ballot(v >= (1 << B)) != 0ull) return wave_excl_scan(v, total);
-    int pre = 0, tot = 0;
-#pragma unroll
-    for (int b = 0; b < B; ++b) {
-        const uint64_t m = __ballot((v >> b) & 1);
-        pre += lanes_below(m) << b;
-        tot += __popcll(m) << b;
-    }
-    total = tot;
-    return pre;
 }
 
 // FP32 cubic-spline force/r inside the kernel, 0 < r <= h (tree.h:59-67 in single precision),
@@ -741,7 +731,7 @@ __global__ __launch_bounds__(64, TJ ? 3 : (FP64 ? 4 : 5)) void k_walk(WalkArgs a
                 // children of opened interior nodes go on the stack, then the next batch is
                 // popped and its loads are issued
                 int ctot;
-                const int coff = wave_excl_scan_small<4>(nch, ctot); // nch <= 8
+                const int coff = wave_excl_scan(nch, ctot);
                 if (sp + ctot > scap) {
                     // keep the top `keep` entries in LDS, move the rest to the spill slice
                     const int keep = min(min(sp, kSpillKeep), scap - ctot);
@@ -820,7 +810,7 @@ __global__ __launch_bounds__(64, TJ ? 3 : (FP64 ? 4 : 5)) void k_walk(WalkArgs a
                     const int nleaf = __popcll(lb);
                     int ptot;
                     const int sz = open_leaf ? (phi - plo) : 0;
-                    const int poff = wave_excl_scan_small<5>(sz, ptot);
+                    const int poff = wave_excl_scan(sz, ptot);
                     if (open_leaf) {
                         const int li = lanes_below(lb);
                         s_lpre[li] = poff;
