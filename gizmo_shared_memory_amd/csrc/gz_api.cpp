@@ -107,6 +107,7 @@ gz_status walk(gz_ctx* ctx, const gz_tree* tree, const gz_particles* p, const ui
                  o_mid = c.add<uint32_t>(n_t), o_gstart2 = c.add<int>(n_t + 1),
                  o_scale = c.add<float>(ex.model ? n_t : 1), o_tslot = c.add<int>(n_t),
                  o_misc = c.add<long long>(16),
+                 o_lgroup = c.add<int>(group_size > 0 ? (size_t)std::max(tree->nnodes, 1) : 1),
                  o_tmp = c.add<uint8_t>(gz::scan_tmp_bytes(n_t));
     gz::DevMem& s = ctx->scratch[5];
     GZ_TRY(s.reserve(c.used));
@@ -128,7 +129,8 @@ gz_status walk(gz_ctx* ctx, const gz_tree* tree, const gz_particles* p, const ui
     int* tslot = reinterpret_cast<int*>(b + o_tslot);
     GZ_TRY(gz::launch_group_prep(targets, n_t, tree->rank, tree->orderbuf, tree->n_live,
                                  tree->slot_leaf, tslot, tree->parent,
-                                 tree->wn, batch,
+                                 tree->wn, tree->nnodes, reinterpret_cast<int*>(b + o_lgroup),
+                                 batch,
                                  group_size, p->n, ex.seg, ex.nseg, gseg, gkey, flag, gid,
                                  ngroups_dev, status, b + o_tmp, st));
     // the group count (misc[4]) and the status word (misc[6]) in one read-back

@@ -144,7 +144,8 @@ void clear_error();
 gz_status launch_group_prep(const uint32_t* targets, size_t nt, const uint32_t* rank,
                             const uint32_t* orderbuf, size_t n_live,
                             const int* slot_leaf, int* tslot,
-                            const int* parent, const WNode* wn, int batch, int group_size,
+                            const int* parent, const WNode* wn, int nnodes, int* leaf_group,
+                            int batch, int group_size,
                             size_t n, const uint32_t* seg, int nseg, int* gseg, int* gkey,
                             uint32_t* flag, uint32_t* gid, uint32_t* ngroups_dev, int* status,
                             void* scan_tmp, hipStream_t st);

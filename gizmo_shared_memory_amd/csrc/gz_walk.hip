@@ -123,6 +123,22 @@ __device__ __forceinline__ bool open_node_d(double r_sq, double len, double l_op
 // Group key of each target slot: its leaf (batch <= 0, group_size == 0), the highest ancestor of the
 // leaf holding at most group_size particles (group_size > 0), or the window index (batch > 0).
 // A target outside the tree (leaf_of < 0) always starts a group of its own (tree.cc:663-664).
+// The group node of every leaf for group_size > 0: its highest ancestor holding at most
+// group_size particles. One thread per node (interior nodes are skipped), so each leaf climbs
+// once however many targets it holds.
+__global__ __launch_bounds__(256) void k_leaf_group(int nnodes, const WNode* __restrict__ wn,
+                                                    const int* __restrict__ parent,
+                                                    int group_size, int* __restrict__ leaf_group) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= nnodes || wn[i].first >= 0) return;
+    int key = i, p = parent[i];
+    while (p >= 0 && wn[p].phi - wn[p].plo <= group_size) {
+        key = p;
+        p = parent[p];
+    }
+    leaf_group[i] = key;
+}
+
 __global__ __launch_bounds__(256) void k_group_flags(const uint32_t* __restrict__ targets,
                                                      size_t nt, const uint32_t* __restrict__ rank,
                                                      const uint32_t* __restrict__ orderbuf,
@@ -130,8 +146,9 @@ __global__ __launch_bounds__(256) void k_group_flags(const uint32_t* __restrict_
                                                      const int* __restrict__ slot_leaf,
                                                      int* __restrict__ tslot,
                                                      const int* __restrict__ parent,
-                                                     const WNode* __restrict__ wn, int batch,
-                                                     int group_size, size_t n,
+                                                     const WNode* __restrict__ wn,
+                                                     const int* __restrict__ leaf_group,
+                                                     int batch, int group_size, size_t n,
                                                      const uint32_t* __restrict__ seg, int nseg,
                                                      int* __restrict__ gseg,
                                                      int* __restrict__ gkey,
@@ -166,10 +183,14 @@ __global__ __launch_bounds__(256) void k_group_flags(const uint32_t* __restrict_
     } else {
         key = (slot >= 0) ? slot_leaf[slot] : -1;
         if (key >= 0 && group_size > 0) {
-            int p = parent[key];
-            while (p >= 0 && wn[p].phi - wn[p].plo <= group_size) {
-                key = p;
-                p = parent[p];
+            if (leaf_group) {
+                key = leaf_group[key];
+            } else {
+                int p = parent[key];
+                while (p >= 0 && wn[p].phi - wn[p].plo <= group_size) {
+                    key = p;
+                    p = parent[p];
+                }
             }
         }
     }
@@ -1112,13 +1133,20 @@ gz_status launch_walk_retry(const WalkArgs& a, int waves, bool fp64, bool count,
 
 gz_status launch_group_prep(const uint32_t* targets, size_t nt, const uint32_t* rank,
                             const uint32_t* orderbuf, size_t n_live,
-                            const int* slot_leaf, int* tslot, const int* parent, const WNode* wn, int batch, int group_size,
+                            const int* slot_leaf, int* tslot, const int* parent, const WNode* wn,
+                            int nnodes, int* leaf_group, int batch, int group_size,
                             size_t n, const uint32_t* seg, int nseg, int* gseg, int* gkey,
                             uint32_t* flag, uint32_t* gid, uint32_t* ngroups_dev, int* status,
                             void* scan_tmp, hipStream_t st) {
     const unsigned grid = (unsigned)((nt + 255) / 256);
+    // every leaf's group node once, when targets outnumber nodes (else each target climbs)
+    const bool per_leaf = batch <= 0 && group_size > 0 && leaf_group && nnodes > 0 &&
+                          (size_t)nnodes < nt;
+    if (per_leaf)
+        k_leaf_group<<<(unsigned)((nnodes + 255) / 256), 256, 0, st>>>(nnodes, wn, parent,
+                                                                       group_size, leaf_group);
     k_group_flags<<<grid, 256, 0, st>>>(targets, nt, rank, orderbuf, n_live, slot_leaf, tslot,
-                                        parent, wn, batch,
+                                        parent, wn, per_leaf ? leaf_group : nullptr, batch,
                                         group_size, n,
                                         seg, nseg, gseg, gkey, status);
     k_group_flags2<<<grid, 256, 0, st>>>(nt, batch, gkey, nseg > 1 ? gseg : nullptr, flag);
