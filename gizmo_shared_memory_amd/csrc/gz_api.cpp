@@ -132,7 +132,7 @@ gz_status walk(gz_ctx* ctx, const gz_tree* tree, const gz_particles* p, const ui
                                  tree->wn, tree->nnodes, reinterpret_cast<int*>(b + o_lgroup),
                                  batch,
                                  group_size, p->n, ex.seg, ex.nseg, gseg, gkey, flag, gid,
-                                 ngroups_dev, status, b + o_tmp, st));
+                                 gstart, ngroups_dev, status, b + o_tmp, st));
     // the group count (misc[4]) and the status word (misc[6]) in one read-back
     long long head[3] = {0, 0, 0};
     GZ_HIP(hipMemcpyAsync(head, misc + 4, sizeof head, hipMemcpyDeviceToHost, st));
@@ -144,7 +144,6 @@ gz_status walk(gz_ctx* ctx, const gz_tree* tree, const gz_particles* p, const ui
     if (prep_status & 2)
         return fail(GZ_ERR_INVALID, "%s: a target index is outside the particle set (n=%zu)", who,
                     p->n);
-    GZ_TRY(gz::launch_group_starts(n_t, flag, gid, gstart, st));
     uint32_t ngroups = ngroups0;
     if (group_size > 0 && pack && ngroups0 > 1) {
         uint32_t* ng2 = reinterpret_cast<uint32_t*>(misc + 3);

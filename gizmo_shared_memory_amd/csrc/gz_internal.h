@@ -124,8 +124,9 @@ struct DevMem {
 // Exclusive scan of n uint32 values (in may equal out). total_dev, when non-null, receives the
 // sum (device). tmp must hold scan_tmp_bytes(n).
 size_t scan_tmp_bytes(size_t n);
+// starts (optional, for 0/1 flags): starts[out[i]] = i wherever in[i] is set.
 gz_status exclusive_scan_u32(const uint32_t* in, uint32_t* out, size_t n, uint32_t* total_dev,
-                             void* tmp, hipStream_t st);
+                             void* tmp, hipStream_t st, int* starts = nullptr);
 
 // Stable LSD radix sort of (key, value) pairs by the low `bits` bits of the key, 8 bits per pass,
 // skipping passes whose digit is the same for every key. The *_alt buffers are scratch of the
@@ -147,10 +148,8 @@ gz_status launch_group_prep(const uint32_t* targets, size_t nt, const uint32_t* 
                             const int* parent, const WNode* wn, int nnodes, int* leaf_group,
                             int batch, int group_size,
                             size_t n, const uint32_t* seg, int nseg, int* gseg, int* gkey,
-                            uint32_t* flag, uint32_t* gid, uint32_t* ngroups_dev, int* status,
-                            void* scan_tmp, hipStream_t st);
-gz_status launch_group_starts(size_t nt, const uint32_t* flag, const uint32_t* gid, int* gstart,
-                              hipStream_t st);
+                            uint32_t* flag, uint32_t* gid, int* gstart, uint32_t* ngroups_dev,
+                            int* status, void* scan_tmp, hipStream_t st);
 gz_status launch_group_pack(int ngroups, size_t nt, int group_size, const int* gstart,
                             const int* gkey, const int* parent, const int* gseg, long long* pk,
                             uint32_t* mflag, uint32_t* mid, uint32_t* ngroups2_dev, int* gstart2,

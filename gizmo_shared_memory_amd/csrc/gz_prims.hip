@@ -141,25 +141,46 @@ __global__ __launch_bounds__(1024) void k_scan_partials_scan(uint32_t* __restric
     if (threadIdx.x == 0 && total_out) *total_out = s_carry;
 }
 
+// starts (optional): for a scan of 0/1 flags, starts[out[i]] = i wherever in[i] is set (the
+// first index of each run that the flags open).
 __global__ __launch_bounds__(kScanThreads) void k_scan_apply(const uint32_t* in, uint32_t* out,
                                                              size_t n,
-                                                             const uint32_t* __restrict__ partial) {
+                                                             const uint32_t* __restrict__ partial,
+                                                             int* __restrict__ starts, int vec) {
     __shared__ uint32_t s4[4];
-    // blocked arrangement: thread t owns elements base + t*kScanIpt .. + kScanIpt - 1
+    // blocked arrangement: thread t owns elements base + t*kScanIpt .. + kScanIpt - 1, read and
+    // written as two 16-byte vectors when the tile is whole and the arrays are aligned
     const size_t base = (size_t)blockIdx.x * kScanTile + (size_t)threadIdx.x * kScanIpt;
+    const bool whole = vec && base + kScanIpt <= n;
     uint32_t v[kScanIpt];
+    if (whole) {
+        const uint4 a = reinterpret_cast<const uint4*>(in + base)[0];
+        const uint4 b = reinterpret_cast<const uint4*>(in + base)[1];
+        v[0] = a.x; v[1] = a.y; v[2] = a.z; v[3] = a.w;
+        v[4] = b.x; v[5] = b.y; v[6] = b.z; v[7] = b.w;
+    } else {
+#pragma unroll
+        for (int j = 0; j < kScanIpt; ++j) v[j] = (base + j < n) ? in[base + j] : 0u;
+    }
     uint32_t sum = 0;
 #pragma unroll
-    for (int j = 0; j < kScanIpt; ++j) {
-        v[j] = (base + j < n) ? in[base + j] : 0u;
-        sum += v[j];
-    }
+    for (int j = 0; j < kScanIpt; ++j) sum += v[j];
     uint32_t total;
     uint32_t run = block256_exclusive_scan(sum, s4, total) + partial[blockIdx.x];
+    uint32_t o[kScanIpt];
 #pragma unroll
     for (int j = 0; j < kScanIpt; ++j) {
-        if (base + j < n) out[base + j] = run;
+        o[j] = run;
+        if (starts && v[j] && base + j < n) starts[run] = (int)(base + j);
         run += v[j];
+    }
+    if (whole) {
+        reinterpret_cast<uint4*>(out + base)[0] = make_uint4(o[0], o[1], o[2], o[3]);
+        reinterpret_cast<uint4*>(out + base)[1] = make_uint4(o[4], o[5], o[6], o[7]);
+    } else {
+#pragma unroll
+        for (int j = 0; j < kScanIpt; ++j)
+            if (base + j < n) out[base + j] = o[j];
     }
 }
 
@@ -169,7 +190,7 @@ size_t scan_tmp_bytes(size_t n) {
 }
 
 gz_status exclusive_scan_u32(const uint32_t* in, uint32_t* out, size_t n, uint32_t* total_dev,
-                             void* tmp, hipStream_t st) {
+                             void* tmp, hipStream_t st, int* starts) {
     if (n == 0) {
         if (total_dev) GZ_HIP(hipMemsetAsync(total_dev, 0, sizeof(uint32_t), st));
         return GZ_OK;
@@ -178,7 +199,8 @@ gz_status exclusive_scan_u32(const uint32_t* in, uint32_t* out, size_t n, uint32
     uint32_t* partial = static_cast<uint32_t*>(tmp);
     k_scan_partials<<<(unsigned)nb, kScanThreads, 0, st>>>(in, n, partial);
     k_scan_partials_scan<<<1, 1024, 0, st>>>(partial, nb, total_dev);
-    k_scan_apply<<<(unsigned)nb, kScanThreads, 0, st>>>(in, out, n, partial);
+    const int vec = ((reinterpret_cast<uintptr_t>(in) | reinterpret_cast<uintptr_t>(out)) & 15) == 0;
+    k_scan_apply<<<(unsigned)nb, kScanThreads, 0, st>>>(in, out, n, partial, starts, vec);
     GZ_HIP(hipGetLastError());
     return GZ_OK;
 }

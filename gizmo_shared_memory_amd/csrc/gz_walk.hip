@@ -213,14 +213,6 @@ __global__ __launch_bounds__(256) void k_group_flags2(size_t nt, int batch,
     flag[i] = f ? 1u : 0u;
 }
 
-__global__ __launch_bounds__(256) void k_group_starts(size_t nt, const uint32_t* __restrict__ flag,
-                                                      const uint32_t* __restrict__ gid,
-                                                      int* __restrict__ gstart) {
-    const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
-    if (i >= nt) return;
-    if (flag[i]) gstart[gid[i]] = (int)i;
-}
-
 // Sibling packing (group_size > 0 only): groups whose key nodes are children of the same parent
 // are merged greedily, in octant order, while the merged group holds at most group_size targets.
 // A merged group's box lies inside the parent's cell, so it stays compact, and the wavefront that
@@ -1136,8 +1128,8 @@ gz_status launch_group_prep(const uint32_t* targets, size_t nt, const uint32_t* 
                             const int* slot_leaf, int* tslot, const int* parent, const WNode* wn,
                             int nnodes, int* leaf_group, int batch, int group_size,
                             size_t n, const uint32_t* seg, int nseg, int* gseg, int* gkey,
-                            uint32_t* flag, uint32_t* gid, uint32_t* ngroups_dev, int* status,
-                            void* scan_tmp, hipStream_t st) {
+                            uint32_t* flag, uint32_t* gid, int* gstart, uint32_t* ngroups_dev,
+                            int* status, void* scan_tmp, hipStream_t st) {
     const unsigned grid = (unsigned)((nt + 255) / 256);
     // every leaf's group node once, when targets outnumber nodes (else each target climbs)
     const bool per_leaf = batch <= 0 && group_size > 0 && leaf_group && nnodes > 0 &&
@@ -1151,14 +1143,8 @@ gz_status launch_group_prep(const uint32_t* targets, size_t nt, const uint32_t* 
                                         seg, nseg, gseg, gkey, status);
     k_group_flags2<<<grid, 256, 0, st>>>(nt, batch, gkey, nseg > 1 ? gseg : nullptr, flag);
     GZ_HIP(hipGetLastError());
-    GZ_TRY(exclusive_scan_u32(flag, gid, nt, ngroups_dev, scan_tmp, st));
-    return GZ_OK;
-}
-
-gz_status launch_group_starts(size_t nt, const uint32_t* flag, const uint32_t* gid, int* gstart,
-                              hipStream_t st) {
-    k_group_starts<<<(unsigned)((nt + 255) / 256), 256, 0, st>>>(nt, flag, gid, gstart);
-    GZ_HIP(hipGetLastError());
+    // group ids, and each group's first target slot (gstart[gid[i]] = i where a group starts)
+    GZ_TRY(exclusive_scan_u32(flag, gid, nt, ngroups_dev, scan_tmp, st, gstart));
     return GZ_OK;
 }
 
