@@ -230,35 +230,126 @@ struct ToleranceModel {
     float lab_lo, lab_hi, margin, log_alpha_ref;
 };
 
-__global__ __launch_bounds__(256) void k_group_scale(const GroupStats* __restrict__ stats,
-                                                     int ngroups, const WNode* __restrict__ wn,
-                                                     ToleranceModel md, float* __restrict__ scale,
-                                                     int* __restrict__ n_fallback) {
-    __shared__ float sW[kMlpParams];
-    __shared__ float sN[2 * kIn];
-    __shared__ float sR[2 * kIn];
-    stage_weights(md.wbf, md.theta, md.xnorm, sW, sN);
-    for (int k = threadIdx.x; k < 2 * kIn; k += blockDim.x) sR[k] = md.frange[k];
-    __syncthreads();
-    const int g = blockIdx.x * 256 + threadIdx.x;
-    bool fell_back = false;
-    if (g < ngroups) {
-        float f[kIn];
-        features_of(stats[g], wn, f);
-        bool ok = true;
+// One wavefront per batch of 32 groups; the two hidden layers are matrix products on the matrix
+// cores (v_mfma_f32_32x32x2f32: FP32 operands, the bf16-rounded weights exactly representable).
+// Operand layouts of that instruction, for lane l: A[l % 32][l / 32] (32 x 2), B[l / 32][l % 32]
+// (2 x 32), and result register v holds D[8 (v / 4) + 4 (l / 32) + v % 4][l % 32] (32 x 32).
+// Layer 1 is X[32 x 16] W1^T in 8 steps of K = 2 per block of 32 hidden units; layer 2 is
+// H1[32 x 64] W2^T in 32 steps, its A operand read back from LDS (the result layout is not the
+// operand layout); biases start the accumulators. The output unit is a dot product per group,
+// summed in hidden-unit order by the group's lane. Each wave keeps its B operands (80 floats per
+// lane) in registers and walks batches with a grid stride.
+typedef float v16f __attribute__((ext_vector_type(16)));
+constexpr int kScaleBatch = 32;
+
+__global__ __launch_bounds__(64) void k_group_scale(const GroupStats* __restrict__ stats,
+                                                    int ngroups, const WNode* __restrict__ wn,
+                                                    ToleranceModel md, float* __restrict__ scale,
+                                                    int* __restrict__ n_fallback) {
+    static_assert(kIn == 16 && kH == 64, "the tiling assumes 16 inputs and 64 hidden units");
+    __shared__ float sH[kScaleBatch][kH + 1];
+    __shared__ float sW3[kH];
+    const int l = threadIdx.x;
+    const int col = l & 31, half = l >> 5;
+    auto wmat = [&](int k) { return bf16_to_f32(md.wbf[k]); };
+    // B operands: B[kk][jj] = W[32 jb + jj][2 ks + kk]. A lane reads its weight rows whole, in
+    // 16-byte pieces (rows start on 16-byte boundaries), and keeps every other bf16: the low
+    // halves of the 32-bit words in the first half-wave, the high halves in the second.
+    auto pick = [&](uint32_t w) { return __uint_as_float(half ? (w & 0xFFFF0000u) : (w << 16)); };
+    float w1b[2][8], w2b[2][32];
 #pragma unroll
-        for (int k = 0; k < kIn; ++k) ok = ok && (f[k] >= sR[k]) && (f[k] <= sR[kIn + k]);
-        float sc = 1.0f;
-        if (ok) {
-            const float pred = mlp_forward_one(f, sW, sN);
-            ok = (pred >= md.lab_lo) && (pred <= md.lab_hi);
-            if (ok) sc = exp10f(pred - md.margin - md.log_alpha_ref);
+    for (int jb = 0; jb < 2; ++jb) {
+        const uint4* r1 = reinterpret_cast<const uint4*>(md.wbf + kOffW1 + (32 * jb + col) * kIn);
+#pragma unroll
+        for (int q = 0; q < 2; ++q) {
+            const uint4 w = r1[q];
+            w1b[jb][4 * q] = pick(w.x); w1b[jb][4 * q + 1] = pick(w.y);
+            w1b[jb][4 * q + 2] = pick(w.z); w1b[jb][4 * q + 3] = pick(w.w);
         }
-        scale[g] = sc;
-        fell_back = !ok;
+        const uint4* r2 = reinterpret_cast<const uint4*>(md.wbf + kOffW2 + (32 * jb + col) * kH);
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            const uint4 w = r2[q];
+            w2b[jb][4 * q] = pick(w.x); w2b[jb][4 * q + 1] = pick(w.y);
+            w2b[jb][4 * q + 2] = pick(w.z); w2b[jb][4 * q + 3] = pick(w.w);
+        }
     }
-    const uint64_t b = __ballot(fell_back);
-    if (b && (threadIdx.x & 63) == 0) atomicAdd(n_fallback, (int)__popcll(b));
+    const float b1[2] = {md.theta[kOffB1 + col], md.theta[kOffB1 + 32 + col]};
+    const float b2[2] = {md.theta[kOffB2 + col], md.theta[kOffB2 + 32 + col]};
+    const float b3 = md.theta[kOffB3];
+    sW3[l] = wmat(kOffW3 + l); // the output unit's weights, read by every group's lane
+
+    const int nbatch = (ngroups + kScaleBatch - 1) / kScaleBatch;
+    for (int bt = blockIdx.x; bt < nbatch; bt += gridDim.x) {
+        // features of group col of the batch (both halves of the wave hold them)
+        const int g = bt * kScaleBatch + col;
+        const bool valid = g < ngroups;
+        float xin[kIn];
+        bool ok = valid;
+        if (valid) {
+            float f[kIn];
+            features_of(stats[g], wn, f);
+#pragma unroll
+            for (int k = 0; k < kIn; ++k) {
+                ok = ok && (f[k] >= md.frange[k]) && (f[k] <= md.frange[kIn + k]);
+                xin[k] = (f[k] - md.xnorm[k]) * md.xnorm[kIn + k];
+            }
+        } else {
+#pragma unroll
+            for (int k = 0; k < kIn; ++k) xin[k] = 0.f;
+        }
+        // layer 1
+        v16f acc[2];
+#pragma unroll
+        for (int jb = 0; jb < 2; ++jb) {
+#pragma unroll
+            for (int v = 0; v < 16; ++v) acc[jb][v] = b1[jb];
+#pragma unroll
+            for (int ks = 0; ks < 8; ++ks) {
+                const float a = half ? xin[2 * ks + 1] : xin[2 * ks];
+                acc[jb] = __builtin_amdgcn_mfma_f32_32x32x2f32(a, w1b[jb][ks], acc[jb], 0, 0, 0);
+            }
+        }
+        // GELU, then to LDS in row (group) order for the next product's A operand
+#pragma unroll
+        for (int jb = 0; jb < 2; ++jb)
+#pragma unroll
+            for (int v = 0; v < 16; ++v)
+                sH[8 * (v >> 2) + 4 * half + (v & 3)][32 * jb + col] = gelu(acc[jb][v]);
+        __syncthreads();
+        // layer 2
+#pragma unroll
+        for (int jb = 0; jb < 2; ++jb)
+#pragma unroll
+            for (int v = 0; v < 16; ++v) acc[jb][v] = b2[jb];
+#pragma unroll
+        for (int ks = 0; ks < 32; ++ks) {
+            const float a = sH[col][2 * ks + half];
+            acc[0] = __builtin_amdgcn_mfma_f32_32x32x2f32(a, w2b[0][ks], acc[0], 0, 0, 0);
+            acc[1] = __builtin_amdgcn_mfma_f32_32x32x2f32(a, w2b[1][ks], acc[1], 0, 0, 0);
+        }
+        __syncthreads();
+#pragma unroll
+        for (int jb = 0; jb < 2; ++jb)
+#pragma unroll
+            for (int v = 0; v < 16; ++v)
+                sH[8 * (v >> 2) + 4 * half + (v & 3)][32 * jb + col] = gelu(acc[jb][v]);
+        __syncthreads();
+        // output unit: group col's dot product, in hidden-unit order
+        bool fell_back = false;
+        if (half == 0 && valid) {
+            float yy = b3;
+            for (int j = 0; j < kH; ++j) yy = __builtin_fmaf(sW3[j], sH[col][j], yy);
+            float sc = 1.0f;
+            ok = ok && (yy >= md.lab_lo) && (yy <= md.lab_hi);
+            if (ok) sc = exp10f(yy - md.margin - md.log_alpha_ref);
+            scale[g] = sc;
+            fell_back = !ok;
+        }
+        const uint64_t fb = __ballot(fell_back);
+        if (fb && l == 0) atomicAdd(n_fallback, (int)__popcll(fb));
+        __syncthreads(); // sH is rewritten by the next batch
+    }
 }
 
 // ---- backpropagation -------------------------------------------------------------------------------
@@ -470,8 +561,14 @@ gz_status launch_group_tolerance(int ngroups, int nt, const int* gstart, const u
                                  st));
     ToleranceModel md{M->wbf, M->theta, M->xnorm, M->frange, M->lab_lo, M->lab_hi, M->margin,
                       M->log_alpha_ref};
-    k_group_scale<<<(unsigned)((ngroups + 255) / 256), 256, 0, st>>>(stats, ngroups, t->wn, md,
-                                                                     scale, n_fallback);
+    // the weight rows are read in 16-byte pieces
+    static_assert((kOffW1 * 2) % 16 == 0 && (kOffW2 * 2) % 16 == 0 && (kIn * 2) % 16 == 0 &&
+                      (kH * 2) % 16 == 0, "bf16 weight rows on 16-byte boundaries");
+    if (reinterpret_cast<uintptr_t>(M->wbf) % 16 != 0)
+        return fail(GZ_ERR_INTERNAL, "group tolerance: bf16 weights not 16-byte aligned");
+    const int nbatch = (ngroups + kScaleBatch - 1) / kScaleBatch;
+    k_group_scale<<<(unsigned)std::min(nbatch, 1024), 64, 0, st>>>(stats, ngroups, t->wn, md,
+                                                                  scale, n_fallback);
     GZ_HIP(hipGetLastError());
     return GZ_OK;
 }
