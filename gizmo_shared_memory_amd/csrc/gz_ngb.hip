@@ -13,6 +13,7 @@
 
 #include "gz_features.h"
 #include "gz_internal.h"
+#include "gz_wave.h"
 
 namespace gz {
 
@@ -108,17 +109,8 @@ __global__ __launch_bounds__(64) void k_ngb(NgbArgs a) {
                 }
             }
             // children of kept interior nodes go on the stack
-            int ctot = 0, coff;
-            {
-                int x = nch;
-#pragma unroll
-                for (int d = 1; d < 64; d <<= 1) {
-                    const int y = __shfl_up(x, d);
-                    if (lane >= d) x += y;
-                }
-                ctot = __shfl(x, 63);
-                coff = x - nch;
-            }
+            int ctot = 0;
+            const int coff = wave_excl_scan(nch, ctot);
             if (sp + ctot > a.stack_cap) {
                 overflow = true;
                 break;

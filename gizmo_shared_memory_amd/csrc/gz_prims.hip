@@ -7,6 +7,7 @@
 #include <cstring>
 
 #include "gz_internal.h"
+#include "gz_wave.h"
 
 namespace gz {
 
@@ -63,13 +64,7 @@ void DevMem::release() {
 
 // ---- wave / block scans ----------------------------------------------------------------------
 __device__ __forceinline__ uint32_t wave_inclusive_scan(uint32_t x) {
-    const int lane = __lane_id();
-#pragma unroll
-    for (int d = 1; d < kWave; d <<= 1) {
-        const uint32_t y = __shfl_up(x, d);
-        if (lane >= d) x += y;
-    }
-    return x;
+    return wave_incl_scan_dpp(x);
 }
 
 // Exclusive scan across a 256-thread block (4 waves). s4 is a 4-entry LDS scratch.

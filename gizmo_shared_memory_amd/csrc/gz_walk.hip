@@ -27,6 +27,7 @@
 #include <cstdio>
 
 #include "gz_features.h"
+#include "gz_wave.h"
 
 namespace gz {
 
@@ -332,24 +333,6 @@ template <> struct WalkTypes<false> {
 __device__ __forceinline__ int lanes_below(uint64_t m) {
     return (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32),
                                           __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
-}
-
-// Exclusive prefix sum over the wave and its total. The inclusive sum is built with DPP: shifts
-// by 1, 2, 4 and 8 lanes within each row of 16 (lanes shifted in from outside the row read 0),
-// then lane 15 of rows 0 and 2 added into rows 1 and 3, and lane 31 into rows 2 and 3 (a row
-// outside the row mask keeps its value). Six VALU adds and no LDS traffic; the total is a
-// scalar (what keeps the walk's stack pointer and loop control in scalar registers). Every lane
-// of the wave must be active.
-__device__ __forceinline__ int wave_excl_scan(int v, int& total) {
-    int x = v;
-    x += __builtin_amdgcn_update_dpp(0, x, 0x111, 0xf, 0xf, true);  // row_shr:1
-    x += __builtin_amdgcn_update_dpp(0, x, 0x112, 0xf, 0xf, true);  // row_shr:2
-    x += __builtin_amdgcn_update_dpp(0, x, 0x114, 0xf, 0xf, true);  // row_shr:4
-    x += __builtin_amdgcn_update_dpp(0, x, 0x118, 0xf, 0xf, true);  // row_shr:8
-    x += __builtin_amdgcn_update_dpp(0, x, 0x142, 0xa, 0xf, false); // row_bcast:15
-    x += __builtin_amdgcn_update_dpp(0, x, 0x143, 0xc, 0xf, false); // row_bcast:31
-    total = __builtin_amdgcn_readlane(x, 63);
-    return x - v;
 }
 
 // FP32 cubic-spline force/r inside the kernel, 0 < r <= h (tree.h:59-67 in single precision),
