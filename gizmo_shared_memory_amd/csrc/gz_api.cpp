@@ -145,22 +145,29 @@ gz_status walk(gz_ctx* ctx, const gz_tree* tree, const gz_particles* p, const ui
         return fail(GZ_ERR_INVALID, "%s: a target index is outside the particle set (n=%zu)", who,
                     p->n);
     uint32_t ngroups = ngroups0;
+    // After packing, a persistent walk reads the group count from the device (misc[14]); the host
+    // keeps ngroups0 as an upper bound and skips a read-back that would idle the GPU.
+    const uint32_t* packed_dev = nullptr;
     if (group_size > 0 && pack && ngroups0 > 1) {
-        uint32_t* ng2 = reinterpret_cast<uint32_t*>(misc + 3);
+        uint32_t* ng2 = reinterpret_cast<uint32_t*>(misc + 14);
         int* gstart2 = reinterpret_cast<int*>(b + o_gstart2);
         GZ_TRY(gz::launch_group_pack((int)ngroups0, n_t, group_size, gstart, gkey, tree->parent,
                                      gseg, reinterpret_cast<long long*>(b + o_pk),
                                      reinterpret_cast<uint32_t*>(b + o_mflag),
                                      reinterpret_cast<uint32_t*>(b + o_mid), ng2, gstart2,
                                      b + o_tmp, st));
-        GZ_HIP(hipMemcpyAsync(&ngroups, ng2, sizeof ngroups, hipMemcpyDeviceToHost, st));
-        GZ_HIP(hipStreamSynchronize(st));
+        if (groups_only || features_only || !persistent) {
+            GZ_HIP(hipMemcpyAsync(&ngroups, ng2, sizeof ngroups, hipMemcpyDeviceToHost, st));
+            GZ_HIP(hipStreamSynchronize(st));
+        } else {
+            packed_dev = ng2;
+        }
         gstart = gstart2;
     }
 
     // counters, queue and status start at zero (a fill kernel rather than a memset node, so the
-    // sequence stays valid under stream capture)
-    GZ_TRY(gz::fill_zero_u32(reinterpret_cast<uint32_t*>(misc), 32, st));
+    // sequence stays valid under stream capture); misc[14], the packed group count, is kept
+    GZ_TRY(gz::fill_zero_u32(reinterpret_cast<uint32_t*>(misc), 28, st));
     int* nfb = nullptr;
     if (groups_only) {
         GZ_HIP(hipMemcpyAsync(ex.gstart_out, gstart, ngroups * sizeof(int),
@@ -198,7 +205,7 @@ gz_status walk(gz_ctx* ctx, const gz_tree* tree, const gz_particles* p, const ui
         GZ_TRY(fm.reserve((size_t)ngroups * sizeof(gz::GroupStats)));
         GZ_TRY(gz::launch_group_tolerance((int)ngroups, (int)n_t, gstart, targets, tslot, p, aold,
                                           tree, ex.model, fm.as<gz::GroupStats>(), scale, nfb,
-                                          st));
+                                          st, packed_dev));
     }
     GZ_HIP(hipEventRecord(ev[9], st));
 
@@ -216,6 +223,7 @@ gz_status walk(gz_ctx* ctx, const gz_tree* tree, const gz_particles* p, const ui
     a.group_scale = scale;
     a.theta_sq = theta * theta;
     a.ngroups = (int)ngroups;
+    a.ngroups_dev = packed_dev;
     a.root = tree->root;
     a.cluster_start = tree->cluster_start_dev;
     a.nclusters = tree->n_clusters;
@@ -285,8 +293,9 @@ gz_status walk(gz_ctx* ctx, const gz_tree* tree, const gz_particles* p, const ui
         counts->nodes_visited = host_misc[0];
         counts->multipole = host_misc[1];
         counts->pairs = host_misc[2];
-        counts->groups = ngroups;
-        counts->items = ngroups;
+        const uint32_t ng = packed_dev ? (uint32_t)(host_misc[14] & 0xFFFFFFFF) : ngroups;
+        counts->groups = ng;
+        counts->items = ng;
         counts->prep_ms = ev_ms(ev[8], ev[9]);
         counts->walk_ms = ev_ms(ev[9], ev[10]);
     }

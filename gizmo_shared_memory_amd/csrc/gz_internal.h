@@ -65,6 +65,9 @@ struct WalkArgs {
     const float* group_scale; // per group multiplier of aold_min (learned tolerance), or NULL
     double theta_sq;
     int ngroups;
+    // the group count on the device, set when the host has only an upper bound (ngroups) for it:
+    // the walk then reads the count itself instead of waiting for it to be read back
+    const uint32_t* ngroups_dev;
     int root;
     const uint32_t* cluster_start; // forest: particle offsets of the clusters; NULL: one tree
     int nclusters;
@@ -163,17 +166,19 @@ gz_status launch_direct(const gz_particles* p, const uint32_t* targets, size_t n
 // ---- learned opening criterion (gz_learned.hip) ----------------------------------------------------
 struct GroupStats;
 // The per-group sums behind the learned criterion's features (stats, one per group), and the
-// features themselves (feats, ngroups x 16, when non-NULL).
+// features themselves (feats, ngroups x 16, when non-NULL). With ngroups_dev set, ngroups is an
+// upper bound and the kernels read the count from the device (not with feats).
 gz_status launch_group_features(int ngroups, int nt, const int* gstart, const uint32_t* targets,
                                 const int* tslot, const gz_particles* p, const double* amag,
                                 const gz_tree* t, GroupStats* stats, float* feats,
-                                hipStream_t st);
+                                hipStream_t st, const uint32_t* ngroups_dev = nullptr);
 // scale[g] = the learned tolerance of group g over the reference's (1 on fallback); n_fallback
 // counts the groups that fell back; stats is working space for ngroups GroupStats
 gz_status launch_group_tolerance(int ngroups, int nt, const int* gstart, const uint32_t* targets,
                                  const int* tslot, const gz_particles* p, const double* aold,
                                  const gz_tree* t, const gz_model* M, GroupStats* stats,
-                                 float* scale, int* n_fallback, hipStream_t st);
+                                 float* scale, int* n_fallback, hipStream_t st,
+                                 const uint32_t* ngroups_dev = nullptr);
 gz_status launch_mlp_forward(const float* X, int B, const uint16_t* wbf, const float* theta,
                              const float* xnorm, float* out, hipStream_t st);
 gz_status launch_mlp_grad(const float* X, const float* T, int B, const uint16_t* wbf,

@@ -43,6 +43,7 @@ __device__ __forceinline__ float gelu_grad(float x) {
 
 struct FeatureInputs {
     int ngroups, nt_total;
+    const uint32_t* ngroups_dev; // the exact count when ngroups is an upper bound, or NULL
     const int* gstart;
     const uint32_t* targets;
     const int* tslot;        // sorted slot per target (data in src), -1 if not in the tree
@@ -92,10 +93,11 @@ __global__ __launch_bounds__(256) void k_group_stats(FeatureInputs in,
     static_assert(LPG == 16 || LPG == 32 || LPG == 64, "lanes per group");
     const int sl_lane = threadIdx.x & (LPG - 1);
     const int g = (int)(((size_t)blockIdx.x * 256 + threadIdx.x) / LPG);
+    const int ngroups = in.ngroups_dev ? (int)*in.ngroups_dev : in.ngroups;
     // a whole run of LPG lanes shares g, so a run is either all in range or all out
-    if (g >= in.ngroups) return;
+    if (g >= ngroups) return;
     const int tlo = in.gstart[g];
-    const int thi = (g + 1 < in.ngroups) ? in.gstart[g + 1] : in.nt_total;
+    const int thi = (g + 1 < ngroups) ? in.gstart[g + 1] : in.nt_total;
     double msum = 0, amin = 1e300, amax = 0, a2 = 0, emax = 0;
     double lo0 = 1e300, lo1 = 1e300, lo2 = 1e300, hi0 = -1e300, hi1 = -1e300, hi2 = -1e300;
     for (int i = tlo + sl_lane; i < thi; i += LPG) {
@@ -243,9 +245,12 @@ typedef float v16f __attribute__((ext_vector_type(16)));
 constexpr int kScaleBatch = 32;
 
 __global__ __launch_bounds__(64) void k_group_scale(const GroupStats* __restrict__ stats,
-                                                    int ngroups, const WNode* __restrict__ wn,
+                                                    int ngroups_ub,
+                                                    const uint32_t* __restrict__ ngroups_dev,
+                                                    const WNode* __restrict__ wn,
                                                     ToleranceModel md, float* __restrict__ scale,
                                                     int* __restrict__ n_fallback) {
+    const int ngroups = ngroups_dev ? (int)*ngroups_dev : ngroups_ub;
     static_assert(kIn == 16 && kH == 64, "the tiling assumes 16 inputs and 64 hidden units");
     __shared__ float sH[kScaleBatch][kH + 1];
     __shared__ float sW3[kH];
@@ -508,9 +513,11 @@ __global__ __launch_bounds__(256) void k_to_bf16(const float* __restrict__ theta
 // ---- host launchers ---------------------------------------------------------------------------------
 static FeatureInputs feature_inputs(int ngroups, int nt, const int* gstart,
                                     const uint32_t* targets, const int* tslot,
-                                    const gz_particles* p, const double* amag, const gz_tree* t) {
+                                    const gz_particles* p, const double* amag, const gz_tree* t,
+                                    const uint32_t* ngroups_dev) {
     FeatureInputs in;
     in.ngroups = ngroups;
+    in.ngroups_dev = ngroups_dev;
     in.nt_total = nt;
     in.gstart = gstart;
     in.targets = targets;
@@ -531,11 +538,14 @@ static FeatureInputs feature_inputs(int ngroups, int nt, const int* gstart,
 gz_status launch_group_features(int ngroups, int nt, const int* gstart, const uint32_t* targets,
                                 const int* tslot, const gz_particles* p, const double* amag,
                                 const gz_tree* t, GroupStats* stats, float* feats,
-                                hipStream_t st) {
+                                hipStream_t st, const uint32_t* ngroups_dev) {
     if (ngroups <= 0) return GZ_OK;
+    if (feats && ngroups_dev)
+        return fail(GZ_ERR_INTERNAL, "group features need the exact group count");
     // lanes per group from the mean group size: the kernel waits on chains of dependent loads,
     // so several groups per wavefront (a few passes each) beat one pass of a whole wavefront
-    const FeatureInputs in = feature_inputs(ngroups, nt, gstart, targets, tslot, p, amag, t);
+    const FeatureInputs in =
+        feature_inputs(ngroups, nt, gstart, targets, tslot, p, amag, t, ngroups_dev);
     const double mean = (double)nt / (double)ngroups;
     if (mean <= 64.0)
         k_group_stats<16><<<(unsigned)(((size_t)ngroups + 15) / 16), 256, 0, st>>>(in, stats);
@@ -555,10 +565,11 @@ gz_status launch_group_features(int ngroups, int nt, const int* gstart, const ui
 gz_status launch_group_tolerance(int ngroups, int nt, const int* gstart, const uint32_t* targets,
                                  const int* tslot, const gz_particles* p, const double* aold,
                                  const gz_tree* t, const gz_model* M, GroupStats* stats,
-                                 float* scale, int* n_fallback, hipStream_t st) {
+                                 float* scale, int* n_fallback, hipStream_t st,
+                                 const uint32_t* ngroups_dev) {
     if (ngroups <= 0) return GZ_OK;
     GZ_TRY(launch_group_features(ngroups, nt, gstart, targets, tslot, p, aold, t, stats, nullptr,
-                                 st));
+                                 st, ngroups_dev));
     ToleranceModel md{M->wbf, M->theta, M->xnorm, M->frange, M->lab_lo, M->lab_hi, M->margin,
                       M->log_alpha_ref};
     // the weight rows are read in 16-byte pieces
@@ -567,8 +578,8 @@ gz_status launch_group_tolerance(int ngroups, int nt, const int* gstart, const u
     if (reinterpret_cast<uintptr_t>(M->wbf) % 16 != 0)
         return fail(GZ_ERR_INTERNAL, "group tolerance: bf16 weights not 16-byte aligned");
     const int nbatch = (ngroups + kScaleBatch - 1) / kScaleBatch;
-    k_group_scale<<<(unsigned)std::min(nbatch, 1024), 64, 0, st>>>(stats, ngroups, t->wn, md,
-                                                                  scale, n_fallback);
+    k_group_scale<<<(unsigned)std::min(nbatch, 1024), 64, 0, st>>>(stats, ngroups, ngroups_dev,
+                                                                  t->wn, md, scale, n_fallback);
     GZ_HIP(hipGetLastError());
     return GZ_OK;
 }

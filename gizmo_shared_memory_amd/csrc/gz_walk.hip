@@ -292,9 +292,10 @@ __device__ __forceinline__ void target_data(const WalkArgs& a, int i, double& px
 // The opening bounds of group g (tree.cc:683-702: bounding box, largest softening, smallest aold),
 // computed by the group's own wavefront. Min and max are exact, so every lane ends with the same
 // values whatever the reduction order. A learned tolerance enters as a factor on aold_min.
-__device__ __forceinline__ GroupInfo group_bounds(const WalkArgs& a, int g, int lane) {
+__device__ __forceinline__ GroupInfo group_bounds(const WalkArgs& a, int g, int ngroups,
+                                                  int lane) {
     const int lo = a.gstart[g];
-    const int hi = (g + 1 < a.ngroups) ? a.gstart[g + 1] : a.nt;
+    const int hi = (g + 1 < ngroups) ? a.gstart[g + 1] : a.nt;
     double bx0 = 1e300, by0 = 1e300, bz0 = 1e300, bx1 = -1e300, by1 = -1e300, bz1 = -1e300;
     double emax = 0, amin = a.aold ? 1e300 : 0.0;
     for (int i = lo + lane; i < hi; i += 64) {
@@ -394,6 +395,8 @@ __global__ __launch_bounds__(64, TJ ? 3 : (FP64 ? 4 : 5)) void k_walk(WalkArgs a
                            ? a.spill + (size_t)blockIdx.x * (size_t)a.spill_cap
                            : nullptr;
 
+    // the group count: read from the device when the host holds only an upper bound
+    const int ngroups = a.ngroups_dev ? (int)*a.ngroups_dev : a.ngroups;
     const int lane = threadIdx.x;
     long long n_visit = 0, n_multi = 0, n_pair = 0;
     int max_sp = 0;
@@ -416,7 +419,7 @@ __global__ __launch_bounds__(64, TJ ? 3 : (FP64 ? 4 : 5)) void k_walk(WalkArgs a
             if (lane == 0) {
                 const int x0 = blockIdx.x & 7;
                 constexpr int C = kXcdChunk;
-                const int nch = (a.ngroups + C - 1) / C; // chunks; chunk k belongs to XCD k % 8
+                const int nch = (ngroups + C - 1) / C; // chunks; chunk k belongs to XCD k % 8
                 for (int k = 0; k < 8 && v < 0; ++k) {
                     const int x = (x0 + k) & 7;
                     const int myc = (nch > x) ? (nch - x + 7) / 8 : 0; // chunks of XCD x
@@ -424,7 +427,7 @@ __global__ __launch_bounds__(64, TJ ? 3 : (FP64 ? 4 : 5)) void k_walk(WalkArgs a
                     const int c = atomicAdd(&a.queue[x], 1);
                     if (c < n) {
                         const int gg = ((c / C) * 8 + x) * C + (c % C);
-                        if (gg < a.ngroups) v = gg;
+                        if (gg < ngroups) v = gg;
                         else k = -1; // the tail of the last chunk: claim again (the loop ends)
                     }
                 }
@@ -440,10 +443,10 @@ __global__ __launch_bounds__(64, TJ ? 3 : (FP64 ? 4 : 5)) void k_walk(WalkArgs a
             const int xcd = b & 7, slot = b >> 3;
             const int per = nb >> 3, rem = nb & 7;
             g = xcd * per + min(xcd, rem) + slot; // a bijection of [0, nb)
-            if (g >= a.ngroups) break;
+            if (g >= ngroups) break;
         }
 
-        GroupInfo gi = a.groups ? a.groups[g] : group_bounds(a, g, lane);
+        GroupInfo gi = a.groups ? a.groups[g] : group_bounds(a, g, ngroups, lane);
         // every field is the same in all lanes: keep them in scalar registers
         for (int k = 0; k < 3; ++k) {
             gi.lo[k] = uniform_d(gi.lo[k]);
