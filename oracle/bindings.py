@@ -197,6 +197,12 @@ class Oracle(_Lib):
         L.gzo_ngb_search.argtypes = [C.c_void_p, C.POINTER(_GzoParticles), _dp, _dp, _dp, _dp,
                                      C.c_size_t, C.c_double, C.POINTER(C.c_size_t), _up,
                                      C.c_size_t]
+        L.gzo_init_hmax.argtypes = [C.c_void_p, _dp, C.c_size_t, C.c_size_t,
+                                    C.POINTER(C.c_float)]
+        L.gzo_ngb_search_reverse.restype = C.c_size_t
+        L.gzo_ngb_search_reverse.argtypes = [C.c_void_p, C.POINTER(_GzoParticles), _dp,
+                                             C.c_size_t, _dp, _dp, _dp, _dp, C.c_size_t,
+                                             C.c_double, C.POINTER(C.c_size_t), _up, C.c_size_t]
         L.gzo_potential.argtypes = [C.c_void_p, C.POINTER(_GzoParticles), _up, C.c_size_t,
                                     C.c_double, C.c_double, _dp, _dp]
         L.gzo_build_vel.restype = C.c_void_p
@@ -288,6 +294,32 @@ class Oracle(_Lib):
                                 tau, sp, _ptr(found, _up), found.shape[0])
         return start.astype(np.int64), found[:int(total)]
 
+    def init_hmax(self, t, h, n_gas):
+        """Node hmax (float32 per node) from the gas kernel radii; kept for the reverse search."""
+        hh = _f64(h)
+        sz = np.zeros(3, dtype=np.int64)
+        self.lib.gzo_tree_sizes(t, _ptr(sz, _llp))
+        out = np.zeros(int(sz[0]), dtype=np.float32)
+        self.lib.gzo_init_hmax(t, _ptr(hh, _dp), hh.shape[0], int(n_gas),
+                               out.ctypes.data_as(C.POINTER(C.c_float)))
+        return out
+
+    def ngb_search_reverse(self, t, P: HostParticles, h, n_gas, centres, hq, tau=0.0):
+        """CSR (start, found) of the gas particles j with hq[q] <= r < h[j] (after init_hmax)."""
+        c = np.ascontiguousarray(centres, dtype=np.float64)
+        cs = [np.ascontiguousarray(c[:, d]) for d in range(3)]
+        hh, r = _f64(h), _f64(hq)
+        nq = r.shape[0]
+        start = np.zeros(nq + 1, dtype=np.uint64)
+        sp = start.ctypes.data_as(C.POINTER(C.c_size_t))
+        cp = self._cp(P)
+        args = (t, C.byref(cp), _ptr(hh, _dp), int(n_gas), *(_ptr(a, _dp) for a in cs),
+                _ptr(r, _dp), nq, tau, sp)
+        total = self.lib.gzo_ngb_search_reverse(*args, None, 0)
+        found = np.zeros(max(int(total), 1), dtype=np.uint32)
+        self.lib.gzo_ngb_search_reverse(*args, _ptr(found, _up), found.shape[0])
+        return start.astype(np.int64), found[:int(total)]
+
     def potential(self, t, P: HostParticles, targets, theta, G=1.0, aold=None):
         tg = np.ascontiguousarray(targets, dtype=np.uint32)
         out = np.zeros(tg.shape[0])
@@ -374,6 +406,12 @@ class Reference(_Lib):
         L.gzref_ngb_search.restype = C.c_size_t
         L.gzref_ngb_search.argtypes = [C.c_void_p, C.c_void_p, _dp, _dp, _dp, _dp, C.c_size_t,
                                        C.c_double, C.POINTER(C.c_size_t), _up, C.c_size_t]
+        L.gzref_init_hmax.argtypes = [C.c_void_p, _dp, C.c_size_t, C.c_size_t,
+                                      C.POINTER(C.c_float)]
+        L.gzref_ngb_search_reverse.restype = C.c_size_t
+        L.gzref_ngb_search_reverse.argtypes = [C.c_void_p, C.c_void_p, _dp, C.c_size_t, _dp, _dp,
+                                               _dp, _dp, C.c_size_t, C.c_double,
+                                               C.POINTER(C.c_size_t), _up, C.c_size_t]
         L.gzref_build_vel.restype = C.c_void_p
         L.gzref_build_vel.argtypes = [C.c_void_p, C.c_longlong, _dp, _dp, _dp]
         L.gzref_kick_nodes.argtypes = [C.c_void_p, _up, C.c_size_t, _dp, _dp, _dp, _dp, _dp, _dp,
@@ -458,6 +496,29 @@ class Reference(_Lib):
         found = np.zeros(max(int(total), 1), dtype=np.uint32)
         self.lib.gzref_ngb_search(t, rp, *(_ptr(a, _dp) for a in cs), _ptr(r, _dp), nq, tau, sp,
                                   _ptr(found, _up), found.shape[0])
+        return start.astype(np.int64), found[:int(total)]
+
+    def init_hmax(self, t, h, n_gas):
+        hh = _f64(h)
+        sz = np.zeros(3, dtype=np.int64)
+        self.lib.gzref_tree_sizes(t, _ptr(sz, _llp))
+        out = np.zeros(int(sz[0]), dtype=np.float32)
+        self.lib.gzref_init_hmax(t, _ptr(hh, _dp), hh.shape[0], int(n_gas),
+                                 out.ctypes.data_as(C.POINTER(C.c_float)))
+        return out
+
+    def ngb_search_reverse(self, t, rp, h, n_gas, centres, hq, tau=0.0):
+        c = np.ascontiguousarray(centres, dtype=np.float64)
+        cs = [np.ascontiguousarray(c[:, d]) for d in range(3)]
+        hh, r = _f64(h), _f64(hq)
+        nq = r.shape[0]
+        start = np.zeros(nq + 1, dtype=np.uint64)
+        sp = start.ctypes.data_as(C.POINTER(C.c_size_t))
+        args = (t, rp, _ptr(hh, _dp), int(n_gas), *(_ptr(a, _dp) for a in cs), _ptr(r, _dp), nq,
+                tau, sp)
+        total = self.lib.gzref_ngb_search_reverse(*args, None, 0)
+        found = np.zeros(max(int(total), 1), dtype=np.uint32)
+        self.lib.gzref_ngb_search_reverse(*args, _ptr(found, _up), found.shape[0])
         return start.astype(np.int64), found[:int(total)]
 
     def accel_grouped_tj(self, t, rp, targets, theta, G=1.0, batch=0, aold=None, vel=None,

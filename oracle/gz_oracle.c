@@ -137,6 +137,7 @@ typedef struct {
     double *wcx, *wcy, *wcz, *wmass;
     float* vmax; /* max |v| below the node since the build (0 without velocities) */
     float* xpad;
+    float* hmax; /* largest gas kernel radius below the node (gzo_init_hmax), or NULL */
     /* with velocities: centre-of-mass velocity and the incremental moments (tree.h:300-317) */
     double *vcom_x, *vcom_y, *vcom_z, *dp_x, *dp_y, *dp_z, *dq_x, *dq_y, *dq_z, *dmass;
     uint32_t *orderbuf, *rank;
@@ -319,6 +320,7 @@ void gzo_tree_free(void* h) {
     free(T->delta); free(T->gcx); free(T->gcy); free(T->gcz); free(T->s); free(T->wsoft);
     free(T->wlen); free(T->orderbuf); free(T->rank); free(T->leaf_of); free(T->skey);
     free(T->wcx); free(T->wcy); free(T->wcz); free(T->wmass); free(T->vmax); free(T->xpad);
+    free(T->hmax);
     free(T->vcom_x); free(T->vcom_y); free(T->vcom_z); free(T->dp_x); free(T->dp_y);
     free(T->dp_z); free(T->dq_x); free(T->dq_y); free(T->dq_z); free(T->dmass);
     free(T);
@@ -505,6 +507,96 @@ size_t gzo_ngb_search(void* h, const gzo_particles* P, const double* cx, const d
                     const uint32_t j = T->orderbuf[k];
                     const double ex = P->x[j] - cx[q], ey = P->y[j] - cy[q], ez = P->z[j] - cz[q];
                     if (ex * ex + ey * ey + ez * ez < r2max) {
+                        if (found && total < cap) found[total] = j;
+                        ++total;
+                    }
+                }
+                node = T->next[node];
+            } else {
+                node = T->first[node];
+            }
+        }
+        start[q + 1] = total;
+    }
+    free(sc);
+    return total;
+}
+
+/* h rounded up to float, so that a float bound stays one-sided (tree.cc:286-290). */
+static float h_as_bound(double h) {
+    float f = (float)h;
+    return ((double)f < h) ? nextafterf(f, 1e30f) : f;
+}
+
+/* init_hmax (tree.cc:332-338, raise_hmax tree.cc:293-301): every node's hmax is the largest
+ * rounded-up h of the gas particles (index < n_gas, h > 0) below it. The reference raises each
+ * particle's leaf and ancestors with an early exit; the result is the same subtree maximum. */
+void gzo_init_hmax(void* t, const double* h, size_t nh, size_t n_gas, float* hmax_out) {
+    otree* T = (otree*)t;
+    const int nn = T->nnodes;
+    free(T->hmax);
+    T->hmax = (float*)calloc((size_t)(nn ? nn : 1), sizeof(float));
+    size_t n = n_gas < nh ? n_gas : nh;
+    if (n > T->n) n = T->n;
+    for (size_t i = 0; i < n; ++i) {
+        if (!(h[i] > 0) || T->leaf_of[i] < 0) continue;
+        const float hf = h_as_bound(h[i]);
+        for (int no = T->leaf_of[i]; no >= 0; no = T->parent[no]) {
+            if (T->hmax[no] >= hf) break;
+            T->hmax[no] = hf;
+        }
+    }
+    if (hmax_out) memcpy(hmax_out, T->hmax, (size_t)nn * sizeof(float));
+}
+
+/* ngb_search_reverse (hydro.cc:110-148), non-periodic, tree aged tau: for each centre q with
+ * kernel radius hq[q], every gas particle j (index < n_gas) with hq[q] <= r < h[j], in traversal
+ * (Morton) order. A node is skipped when no particle below it has h > hq[q] (hmax <= hq[q]) or
+ * when it lies beyond hmax + half + xpad (+ vmax tau) of the centre. Needs gzo_init_hmax. */
+size_t gzo_ngb_search_reverse(void* t, const gzo_particles* P, const double* h, size_t n_gas,
+                              const double* cx, const double* cy, const double* cz,
+                              const double* hq, size_t nq, double tau, size_t* start,
+                              uint32_t* found, size_t cap) {
+    const otree* T = (const otree*)t;
+    const int nn = T->nnodes;
+    start[0] = 0;
+    if (!T->hmax) {
+        for (size_t q = 0; q < nq; ++q) start[q + 1] = 0;
+        return 0;
+    }
+    const double snode_eps = 1e-6 * (nn ? T->size[T->root] : 1.0);
+    float* sc = (float*)malloc((size_t)(nn ? nn : 1) * 4 * sizeof(float));
+    for (int i = 0; i < nn; ++i) {
+        sc[4 * i] = (float)T->gcx[i]; sc[4 * i + 1] = (float)T->gcy[i];
+        sc[4 * i + 2] = (float)T->gcz[i];
+        const double cmax = fmax(fabs(T->gcx[i]), fmax(fabs(T->gcy[i]), fabs(T->gcz[i])));
+        sc[4 * i + 3] = (float)(0.5 * T->size[i] + snode_eps + 4.0 * (double)FLT_EPSILON * cmax);
+    }
+    size_t total = 0;
+    for (size_t q = 0; q < nq; ++q) {
+        const double hi = hq[q], hi2 = hi * hi;
+        int node = nn ? T->root : -1;
+        while (node >= 0) {
+            const float* sn = sc + 4 * node;
+            const double hm = (double)T->hmax[node];
+            if (hm <= hi) { node = T->next[node]; continue; }
+            double dist = hm + (double)sn[3] + (double)T->xpad[node];
+            if (tau > 0) dist += (double)T->vmax[node] * tau;
+            const double dx = (double)sn[0] - cx[q];
+            if (dx > dist || -dx > dist) { node = T->next[node]; continue; }
+            const double dy = (double)sn[1] - cy[q];
+            if (dy > dist || -dy > dist) { node = T->next[node]; continue; }
+            const double dz = (double)sn[2] - cz[q];
+            if (dz > dist || -dz > dist) { node = T->next[node]; continue; }
+            dist += 0.7320508075688772 * (double)sn[3];
+            if (dx * dx + dy * dy + dz * dz > dist * dist) { node = T->next[node]; continue; }
+            if (T->first[node] < 0) {
+                for (int k = T->plo[node]; k < T->phi[node]; ++k) {
+                    const uint32_t j = T->orderbuf[k];
+                    if (j >= n_gas) continue;
+                    const double ex = P->x[j] - cx[q], ey = P->y[j] - cy[q], ez = P->z[j] - cz[q];
+                    const double r2 = ex * ex + ey * ey + ez * ez;
+                    if (r2 >= hi2 && r2 < h[j] * h[j]) {
                         if (found && total < cap) found[total] = j;
                         ++total;
                     }

@@ -19,6 +19,11 @@
  *                         along ballistic motion)
  *   gzo_ngb_search        lists equal to the reference's ngb_search (same order) and, as sets,
  *                         to a brute-force distance test
+ *   gzo_init_hmax, gzo_ngb_search_reverse
+ *                         hmax bit for bit against the reference's init_hmax and against the
+ *                         maximum of the rounded-up h below each node; lists equal to the
+ *                         reference's ngb_search_reverse (same order) and, as sets, to a
+ *                         brute-force test of h_q <= r < h_j
  *   gzo_potential         bit for bit against the reference potential, and theta -> 0 against
  *                         the direct sum of the spline potential
  *   gzo_accel_brute       bit for bit against the reference accel_brute, Newton's third law, and
@@ -101,6 +106,15 @@ void gzo_accel(void* t, const gzo_particles* P, const uint32_t* targets, size_t 
 size_t gzo_ngb_search(void* t, const gzo_particles* P, const double* cx, const double* cy,
                       const double* cz, const double* radius, size_t nq, double tau, size_t* start,
                       uint32_t* found, size_t cap);
+/* init_hmax (tree.cc:332-338): node hmax from the gas kernel radii h[0..min(nh, n_gas)), kept in
+ * the tree for the reverse search; copied to hmax_out (nnodes floats) when non-NULL. */
+void gzo_init_hmax(void* t, const double* h, size_t nh, size_t n_gas, float* hmax_out);
+/* ngb_search_reverse (hydro.cc:110-148): CSR of the gas particles j with hq[q] <= r < h[j] for
+ * each centre q, in traversal (Morton) order; found written while it fits in cap. */
+size_t gzo_ngb_search_reverse(void* t, const gzo_particles* P, const double* h, size_t n_gas,
+                              const double* cx, const double* cy, const double* cz,
+                              const double* hq, size_t nq, double tau, size_t* start,
+                              uint32_t* found, size_t cap);
 void gzo_potential(void* t, const gzo_particles* P, const uint32_t* targets, size_t nt,
                    double theta, double G, const double* aold, double* phi);
 void gzo_accel_brute(const gzo_particles* P, const uint32_t* targets, size_t nt, double G,

@@ -306,6 +306,36 @@ size_t gzref_ngb_search(void* t, void* particles, const double* cx, const double
     return all.size();
 }
 
+// Tree::init_hmax (tree.cc:332-338) from h[0..nh) with the first n_gas particles gas; the node
+// values are copied to hmax_out (nnodes floats) when non-NULL.
+void gzref_init_hmax(void* t, const double* h, size_t nh, size_t n_gas, float* hmax_out) {
+    auto* rt = static_cast<RefTree*>(t);
+    rt->T.init_hmax(std::vector<double>(h, h + nh), n_gas);
+    if (hmax_out)
+        std::memcpy(hmax_out, rt->T.hmax.data(), rt->T.hmax.size() * sizeof(float));
+}
+
+// shmem::ngb_search_reverse (hydro.h:81-83, hydro.cc:110-148) for n_q centres with kernel radii
+// hq, non-periodic, the tree aged tau; CSR output as gzref_ngb_search.
+size_t gzref_ngb_search_reverse(void* t, void* particles, const double* h, size_t n_gas,
+                                const double* cx, const double* cy, const double* cz,
+                                const double* hq, size_t n_q, double tau, size_t* start,
+                                uint32_t* found, size_t cap) {
+    auto* rt = static_cast<RefTree*>(t);
+    auto* rp = static_cast<RefParticles*>(particles);
+    rt->T.t_since_build = tau;
+    std::vector<uint32_t> all, one;
+    start[0] = 0;
+    for (size_t q = 0; q < n_q; ++q) {
+        one.clear();
+        ngb_search_reverse(rt->T, rp->P, h, n_gas, Vec3d{cx[q], cy[q], cz[q]}, hq[q], one);
+        all.insert(all.end(), one.begin(), one.end());
+        start[q + 1] = all.size();
+    }
+    if (found && all.size() <= cap) std::memcpy(found, all.data(), all.size() * sizeof(uint32_t));
+    return all.size();
+}
+
 int gzref_max_threads(void) { return omp_get_max_threads(); }
 void gzref_set_threads(int n) { omp_set_num_threads(n); }
 

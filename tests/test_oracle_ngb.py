@@ -72,3 +72,73 @@ def test_ngb_search_on_an_aged_tree(oracle, reference):
     for q in range(c.shape[0]):
         d2 = (xt[0] - c[q, 0]) ** 2 + (xt[1] - c[q, 1]) ** 2 + (xt[2] - c[q, 2]) ** 2
         assert np.array_equal(np.sort(fo[so[q]:so[q + 1]]), np.nonzero(d2 < r[q] ** 2)[0])
+
+
+def _h_as_bound(h):
+    """h rounded up to float32 (tree.cc:286-290)."""
+    f = h.astype(np.float32)
+    up = f.astype(np.float64) < h
+    f[up] = np.nextafter(f[up], np.float32(1e30))
+    return f
+
+
+def _hydro_case(name, seed=9):
+    x, y, z, m, soft = tree_cases.CASES[name]()
+    rng = np.random.default_rng(seed)
+    n = x.shape[0]
+    h = 0.04 * rng.uniform(0.3, 2.0, n)
+    h[rng.random(n) < 0.05] = 0.0  # gas with no kernel yet never raises hmax
+    n_gas = int(0.7 * n)
+    return HostParticles(x, y, z, m, soft), h, n_gas
+
+
+def test_hmax_matches_reference_and_subtree_maximum(oracle, reference):
+    for name in ("plummer20k", "collocated", "tombstones"):
+        P, h, n_gas = _hydro_case(name)
+        to = oracle.build(P)
+        rp = reference.particles(P)
+        tr = reference.build(rp)
+        try:
+            ho = oracle.init_hmax(to, h, n_gas)
+            hr = reference.init_hmax(tr, h, n_gas)
+            T = oracle.tree(to, P)
+        finally:
+            oracle.free(to); reference.free(tr); reference.free_particles(rp)
+        assert np.array_equal(ho.view(np.uint32), hr.view(np.uint32)), name
+        # closed form: the largest rounded-up h of the gas particles below each node
+        hb = _h_as_bound(h)
+        gas = (np.arange(P.n) < n_gas) & (h > 0)
+        for i in range(T.nnodes):
+            js = T.orderbuf[T.plo[i]:T.phi[i]]
+            js = js[gas[js]]
+            want = hb[js].max() if js.size else np.float32(0)
+            assert ho[i] == want, (name, i)
+
+
+def test_ngb_search_reverse_matches_reference_and_brute_force(oracle, reference):
+    for name in ("plummer20k", "collocated", "tombstones"):
+        P, h, n_gas = _hydro_case(name)
+        x, y, z, m = P.x, P.y, P.z, P.m
+        rng = np.random.default_rng(4)
+        idx = rng.choice(P.n, 200, replace=False)
+        c = np.stack([x[idx], y[idx], z[idx]], 1)
+        hq = np.where(h[idx] > 0, h[idx], 0.02)
+        to = oracle.build(P)
+        rp = reference.particles(P)
+        tr = reference.build(rp)
+        try:
+            oracle.init_hmax(to, h, n_gas)
+            reference.init_hmax(tr, h, n_gas)
+            so, fo = oracle.ngb_search_reverse(to, P, h, n_gas, c, hq)
+            sr, fr = reference.ngb_search_reverse(tr, rp, h, n_gas, c, hq)
+        finally:
+            oracle.free(to); reference.free(tr); reference.free_particles(rp)
+        assert np.array_equal(so, sr) and np.array_equal(fo, fr), name
+        if name == "plummer20k":  # collocated particles sit inside every kernel: no pairs
+            assert so[-1] > 0, name
+        live = m > 0
+        gas = np.arange(P.n) < n_gas
+        for q in range(c.shape[0]):
+            d2 = (x - c[q, 0]) ** 2 + (y - c[q, 1]) ** 2 + (z - c[q, 2]) ** 2
+            want = np.nonzero(live & gas & (d2 >= hq[q] ** 2) & (d2 < h ** 2))[0]
+            assert np.array_equal(np.sort(fo[so[q]:so[q + 1]]), want), (name, q)
