@@ -522,7 +522,8 @@ static gz_status alloc_reuse(gz_tree* t, int nn, bool vel) {
 void tree_release(gz_tree* t) {
     void* ptrs[] = {t->skey, t->orderbuf, t->rank, t->leaf_of, t->slot_leaf, t->src, t->src_soft,
                     t->wn, t->next, t->parent, t->nlo, t->nhi, t->gcx, t->gcy, t->gcz, t->size,
-                    t->delta, t->soft, t->level_begin_dev, t->cluster_start_dev, t->reuse_mem};
+                    t->delta, t->soft, t->level_begin_dev, t->cluster_start_dev, t->reuse_mem,
+                    t->hmax};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
 }

@@ -298,6 +298,8 @@ struct gz_tree {
     float* xpad = nullptr;
     bool has_vel = false;
     double *vcom = nullptr, *m0 = nullptr, *dp = nullptr, *dq = nullptr, *dmass = nullptr;
+    // largest gas kernel radius below each node (tree.h:269-272), set by gz_tree_init_hmax
+    float* hmax = nullptr;
 };
 
 namespace gz {

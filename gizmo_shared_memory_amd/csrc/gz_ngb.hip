@@ -37,6 +37,10 @@ struct NgbArgs {
     uint32_t* count;     // pass 1: neighbours per query
     const size_t* start; // pass 2: CSR offsets
     uint32_t* slots;     // pass 2: the neighbours' sorted slots
+    // reverse search: node hmax, the gas kernel radii by particle, the gas count
+    const float* hmax;
+    const double* h;
+    size_t n_gas;
 };
 
 // The search node's half side (tree.cc:496-506): float-safe, never smaller than the true one.
@@ -45,7 +49,11 @@ __device__ __forceinline__ float snode_half(const NgbArgs& a, int i) {
     return (float)(0.5 * a.size[i] + a.snode_eps + 4.0 * (double)FLT_EPSILON * cmax);
 }
 
-template <bool FILL>
+// REV: the reverse search (hydro.cc:110-148). radius[q] is then the query's own kernel radius h_q,
+// a node is a candidate only when its hmax exceeds the smallest h_q of the group, its reach is
+// hmax instead of the query radius, and a leaf particle j counts when it is gas and
+// h_q <= r < h[j].
+template <bool FILL, bool REV>
 __global__ __launch_bounds__(64) void k_ngb(NgbArgs a) {
     const int lane = threadIdx.x;
     int* const stk = a.stack + (size_t)blockIdx.x * (size_t)a.stack_cap;
@@ -65,6 +73,7 @@ __global__ __launch_bounds__(64) void k_ngb(NgbArgs a) {
         const double lo1 = wave_min_d(live ? qy : 1e300), hi1 = wave_max_d(live ? qy : -1e300);
         const double lo2 = wave_min_d(live ? qz : 1e300), hi2 = wave_max_d(live ? qz : -1e300);
         const double rmax = wave_max_d(live ? r : 0.0);
+        const double rmin = REV ? wave_min_d(live ? r : 1e300) : 0.0;
         const double r2 = r * r;
         uint32_t found = 0;
         size_t out = live && FILL ? a.start[q] : 0;
@@ -86,14 +95,15 @@ __global__ __launch_bounds__(64) void k_ngb(NgbArgs a) {
             if (node >= 0) {
                 const WNode w = a.wn[node];
                 const float half = snode_half(a, node);
-                double dist = rmax + (double)half + (double)a.xpad[node];
+                const double hm = REV ? (double)a.hmax[node] : 0.0;
+                double dist = (REV ? hm : rmax) + (double)half + (double)a.xpad[node];
                 if (a.tau > 0) dist += (double)a.vmax[node] * a.tau;
                 const double c0 = (double)(float)a.gcx[node], c1 = (double)(float)a.gcy[node],
                              c2 = (double)(float)a.gcz[node];
                 const double d0 = fmax(0.0, fmax(lo0 - c0, c0 - hi0));
                 const double d1 = fmax(0.0, fmax(lo1 - c1, c1 - hi1));
                 const double d2 = fmax(0.0, fmax(lo2 - c2, c2 - hi2));
-                if (d0 <= dist && d1 <= dist && d2 <= dist) {
+                if ((!REV || hm > rmin) && d0 <= dist && d1 <= dist && d2 <= dist) {
                     const double ds = dist + 0.7320508075688772 * (double)half;
                     keep = d0 * d0 + d1 * d1 + d2 * d2 <= ds * ds;
                 }
@@ -126,7 +136,15 @@ __global__ __launch_bounds__(64) void k_ngb(NgbArgs a) {
                 for (int s = lo; s < hi; ++s) {
                     const double4 p = a.src[s];
                     const double ex = p.x - qx, ey = p.y - qy, ez = p.z - qz;
-                    if (live && ex * ex + ey * ey + ez * ez < r2) {
+                    const double d2 = ex * ex + ey * ey + ez * ez;
+                    bool in;
+                    if (REV) {
+                        const uint32_t j = a.orderbuf[s];
+                        in = j < a.n_gas && d2 >= r2 && d2 < a.h[j] * a.h[j];
+                    } else {
+                        in = d2 < r2;
+                    }
+                    if (live && in) {
                         if (FILL) a.slots[out++] = (uint32_t)s;
                         ++found;
                     }
@@ -173,34 +191,81 @@ __global__ void k_u32_to_size(int n, const uint32_t* __restrict__ in, size_t* __
     if (i < n) out[i] = in[i];
 }
 
+// h rounded up to float, so that the float bound stays one-sided (tree.cc:286-290).
+__device__ __forceinline__ float h_as_bound(double h) {
+    const float f = (float)h;
+    return ((double)f < h) ? nextafterf(f, 1e30f) : f;
+}
+
+// init_hmax (tree.cc:332-338): each leaf takes the largest rounded-up h of its gas particles (an
+// atomic max on the float's bits, which order like the values for h >= 0), then each level from
+// the bottom takes the largest of its children's. The reference's per-particle climb with an
+// early exit ends with the same subtree maxima.
+__global__ __launch_bounds__(256) void k_hmax_leaf(size_t n_live, const uint32_t* __restrict__ orderbuf,
+                                                   const int* __restrict__ slot_leaf,
+                                                   const double* __restrict__ h, size_t n_hgas,
+                                                   unsigned* __restrict__ hbits) {
+    const size_t s = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (s >= n_live) return;
+    const uint32_t j = orderbuf[s];
+    if (j >= n_hgas) return;
+    const double hj = h[j];
+    const int leaf = slot_leaf[s];
+    if (!(hj > 0) || leaf < 0) return;
+    atomicMax(&hbits[leaf], __float_as_uint(h_as_bound(hj)));
+}
+
+__global__ __launch_bounds__(256) void k_hmax_up(int lo, int hi, const WNode* __restrict__ wn,
+                                                 float* __restrict__ hmax) {
+    const int n = lo + blockIdx.x * 256 + threadIdx.x;
+    if (n >= hi) return;
+    const int f = wn[n].first;
+    if (f < 0) return;
+    float m = 0.f;
+    for (int c = f; c < f + wn[n].nchild; ++c) m = fmaxf(m, hmax[c]);
+    hmax[n] = m;
+}
+
 } // namespace gz
 
 using namespace gz;
 
 namespace {
 
-gz_status ngb_pass(NgbArgs& a, bool fill, int waves, hipStream_t st) {
+gz_status ngb_pass(NgbArgs& a, bool fill, bool rev, int waves, hipStream_t st) {
     GZ_TRY(fill_zero_u32(reinterpret_cast<uint32_t*>(a.queue), 2, st));
     const int grid = std::max(1, std::min(waves, a.ngroups));
-    if (fill) k_ngb<true><<<grid, 64, 0, st>>>(a);
-    else k_ngb<false><<<grid, 64, 0, st>>>(a);
+    if (rev) {
+        if (fill) k_ngb<true, true><<<grid, 64, 0, st>>>(a);
+        else k_ngb<false, true><<<grid, 64, 0, st>>>(a);
+    } else {
+        if (fill) k_ngb<true, false><<<grid, 64, 0, st>>>(a);
+        else k_ngb<false, false><<<grid, 64, 0, st>>>(a);
+    }
     GZ_HIP(hipGetLastError());
     return GZ_OK;
 }
 } // namespace
 
-extern "C" gz_status gz_tree_ngb_search(gz_ctx* ctx, const gz_tree* tree, const double* cx,
-                                        const double* cy, const double* cz, const double* radius,
-                                        size_t n_q, double tau, size_t* start, uint32_t* found,
-                                        size_t cap, size_t* total) {
+static gz_status ngb_run(gz_ctx* ctx, const gz_tree* tree, const double* cx, const double* cy,
+                         const double* cz, const double* radius, size_t n_q, double tau,
+                         size_t* start, uint32_t* found, size_t cap, size_t* total,
+                         const double* h, size_t n_gas, bool rev) {
+    const char* who = rev ? "gz_tree_ngb_search_reverse" : "gz_tree_ngb_search";
     clear_error();
-    if (!ctx || !tree || !total) return fail(GZ_ERR_INVALID, "gz_tree_ngb_search: null argument");
+    if (!ctx || !tree || !total) return fail(GZ_ERR_INVALID, "%s: null argument", who);
     GZ_HIP(hipSetDevice(ctx->device));
     *total = 0;
     if (n_q == 0) return GZ_OK;
     if (!cx || !cy || !cz || !radius || !start)
-        return fail(GZ_ERR_INVALID, "gz_tree_ngb_search: null centre, radius or start array");
-    if (n_q >= 0x7FFFFFFFull) return fail(GZ_ERR_INVALID, "gz_tree_ngb_search: n_q too large");
+        return fail(GZ_ERR_INVALID, "%s: null centre, radius or start array", who);
+    if (n_q >= 0x7FFFFFFFull) return fail(GZ_ERR_INVALID, "%s: n_q too large", who);
+    if (rev && n_gas > 0 && !h) return fail(GZ_ERR_INVALID, "%s: null h", who);
+    if (rev && n_gas > tree->n)
+        return fail(GZ_ERR_INVALID, "%s: n_gas=%zu exceeds the particle count %zu", who, n_gas,
+                    tree->n);
+    if (rev && tree->nnodes > 0 && !tree->hmax)
+        return fail(GZ_ERR_INVALID, "%s: no node hmax (call gz_tree_init_hmax first)", who);
     hipStream_t st = ctx->stream;
     if (tree->n_live == 0 || tree->nnodes == 0) {
         GZ_TRY(fill_zero_u32(reinterpret_cast<uint32_t*>(start), 2 * (n_q + 1), st));
@@ -243,8 +308,11 @@ extern "C" gz_status gz_tree_ngb_search(gz_ctx* ctx, const gz_tree* tree, const 
     a.stack = stack;
     a.stack_cap = stack_cap;
     a.count = count;
+    a.hmax = tree->hmax;
+    a.h = h;
+    a.n_gas = rev ? n_gas : 0;
     GZ_TRY(fill_zero_u32(count, n_q + 1, st));
-    GZ_TRY(ngb_pass(a, false, waves, st));
+    GZ_TRY(ngb_pass(a, false, rev, waves, st));
     // offsets: exclusive scan of the counts (count[n_q] = 0 gives the total at start[n_q])
     uint32_t tot32 = 0;
     {
@@ -262,7 +330,7 @@ extern "C" gz_status gz_tree_ngb_search(gz_ctx* ctx, const gz_tree* tree, const 
     GZ_HIP(hipMemcpyAsync(&h_status, a.status, sizeof h_status, hipMemcpyDeviceToHost, st));
     GZ_HIP(hipStreamSynchronize(st));
     if (h_status & 1)
-        return fail(GZ_ERR_INTERNAL, "gz_tree_ngb_search: a search stack outgrew %d entries",
+        return fail(GZ_ERR_INTERNAL, "%s: a search stack outgrew %d entries", who,
                     stack_cap);
     *total = tot32;
     if (!found || cap < (size_t)tot32 || tot32 == 0) return GZ_OK;
@@ -271,14 +339,67 @@ extern "C" gz_status gz_tree_ngb_search(gz_ctx* ctx, const gz_tree* tree, const 
     GZ_TRY(sl.reserve((size_t)tot32 * sizeof(uint32_t)));
     a.start = start;
     a.slots = sl.as<uint32_t>();
-    GZ_TRY(ngb_pass(a, true, waves, st));
+    GZ_TRY(ngb_pass(a, true, rev, waves, st));
     k_ngb_finish<<<(unsigned)((n_q + 255) / 256), 256, 0, st>>>((int)n_q, start, a.slots,
                                                                  tree->orderbuf, found);
     GZ_HIP(hipGetLastError());
     GZ_HIP(hipMemcpyAsync(&h_status, a.status, sizeof h_status, hipMemcpyDeviceToHost, st));
     GZ_HIP(hipStreamSynchronize(st));
     if (h_status & 1)
-        return fail(GZ_ERR_INTERNAL, "gz_tree_ngb_search: a search stack outgrew %d entries",
+        return fail(GZ_ERR_INTERNAL, "%s: a search stack outgrew %d entries", who,
                     stack_cap);
+    return GZ_OK;
+}
+
+extern "C" gz_status gz_tree_ngb_search(gz_ctx* ctx, const gz_tree* tree, const double* cx,
+                                        const double* cy, const double* cz, const double* radius,
+                                        size_t n_q, double tau, size_t* start, uint32_t* found,
+                                        size_t cap, size_t* total) {
+    return ngb_run(ctx, tree, cx, cy, cz, radius, n_q, tau, start, found, cap, total, nullptr, 0,
+                   false);
+}
+
+extern "C" gz_status gz_tree_ngb_search_reverse(gz_ctx* ctx, const gz_tree* tree,
+                                                const double* h, size_t n_gas, const double* cx,
+                                                const double* cy, const double* cz,
+                                                const double* h_q, size_t n_q, double tau,
+                                                size_t* start, uint32_t* found, size_t cap,
+                                                size_t* total) {
+    return ngb_run(ctx, tree, cx, cy, cz, h_q, n_q, tau, start, found, cap, total, h, n_gas,
+                   true);
+}
+
+extern "C" gz_status gz_tree_init_hmax(gz_ctx* ctx, gz_tree* tree, const double* h, size_t n_h,
+                                       size_t n_gas) {
+    clear_error();
+    if (!ctx || !tree) return fail(GZ_ERR_INVALID, "gz_tree_init_hmax: null argument");
+    const size_t n_hgas = std::min(n_h, n_gas);
+    if (n_hgas > 0 && !h) return fail(GZ_ERR_INVALID, "gz_tree_init_hmax: null h");
+    GZ_HIP(hipSetDevice(ctx->device));
+    hipStream_t st = ctx->stream;
+    const int nn = tree->nnodes;
+    if (nn <= 0) return GZ_OK;
+    if (!tree->hmax) {
+        void* p = nullptr;
+        GZ_HIP(hipMalloc(&p, (size_t)nn * sizeof(float)));
+        tree->hmax = static_cast<float*>(p);
+    }
+    GZ_TRY(fill_zero_u32(reinterpret_cast<uint32_t*>(tree->hmax), (size_t)nn, st));
+    if (n_hgas > 0 && tree->n_live > 0) {
+        k_hmax_leaf<<<(unsigned)((tree->n_live + 255) / 256), 256, 0, st>>>(
+            tree->n_live, tree->orderbuf, tree->slot_leaf, h, n_hgas,
+            reinterpret_cast<unsigned*>(tree->hmax));
+        GZ_HIP(hipGetLastError());
+        // levels bottom-up: children of level L are in level L + 1
+        const std::vector<int>& lb = tree->level_begin;
+        for (int L = (int)lb.size() - 2; L >= 0; --L) {
+            const int lo = lb[L], hi = lb[L + 1];
+            if (hi <= lo) continue;
+            k_hmax_up<<<(unsigned)((hi - lo + 255) / 256), 256, 0, st>>>(lo, hi, tree->wn,
+                                                                         tree->hmax);
+            GZ_HIP(hipGetLastError());
+        }
+    }
+    GZ_HIP(hipStreamSynchronize(st));
     return GZ_OK;
 }

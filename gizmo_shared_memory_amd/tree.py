@@ -208,6 +208,15 @@ class Tree:
                 "gz_tree_refresh_walk_nodes")
         self.ctx.synchronize()
 
+    def init_hmax(self, h, n_gas: int):
+        """Tree::init_hmax (tree.cc:332-338): node hmax from the gas kernel radii h (by particle;
+        the first n_gas particles are gas), for ngb_search_reverse."""
+        hh = _dev_f64(h, self.P.device)
+        self.ctx.enter()
+        L.check(L.load().gz_tree_init_hmax(self.ctx.handle, self.handle, _ptr(hh),
+                                           int(hh.shape[0]), int(n_gas)), "gz_tree_init_hmax")
+        self.ctx.synchronize()
+
     def raise_soft(self, idx, soft):
         """Tree::raise_soft (tree.cc:311-319) for the particles idx; soft indexed by particle."""
         ix = _dev_u32(idx, self.P.device)
@@ -530,6 +539,33 @@ def ngb_search(T: Tree, centres, radius, tau: float = 0.0):
                                        r.data_ptr(), nq, float(tau), start.data_ptr(),
                                        found.data_ptr(), int(total.value), C.byref(total)),
                 "gz_tree_ngb_search")
+    T.ctx.synchronize()
+    return start, found[:int(total.value)].view(torch.uint32)
+
+
+def ngb_search_reverse(T: Tree, h, n_gas: int, centres, h_q, tau: float = 0.0):
+    """shmem::ngb_search_reverse (hydro.h:78-83) for many centres: (start, found) device tensors,
+    CSR — the gas particles j with h_q[q] <= r < h[j] (their own kernel reaches centre q), each
+    list in Morton order. Needs Tree.init_hmax."""
+    dev = T.P.device
+    c = centres if isinstance(centres, torch.Tensor) else torch.as_tensor(np.asarray(centres))
+    c = c.to(device=dev, dtype=torch.float64)
+    cs = [c[:, d].contiguous() for d in range(3)]
+    hh = _dev_f64(h, dev)
+    r = _dev_f64(h_q, dev)
+    nq = int(r.shape[0])
+    start = torch.zeros(nq + 1, dtype=torch.int64, device=dev)
+    total = C.c_size_t(0)
+    lib = L.load()
+    T.ctx.enter()
+    args = (T.ctx.handle, T.handle, hh.data_ptr(), int(n_gas), *(a.data_ptr() for a in cs),
+            r.data_ptr(), nq, float(tau), start.data_ptr())
+    L.check(lib.gz_tree_ngb_search_reverse(*args, None, 0, C.byref(total)),
+            "gz_tree_ngb_search_reverse")
+    found = torch.empty(max(int(total.value), 1), dtype=torch.int32, device=dev)
+    if total.value:
+        L.check(lib.gz_tree_ngb_search_reverse(*args, found.data_ptr(), int(total.value),
+                                               C.byref(total)), "gz_tree_ngb_search_reverse")
     T.ctx.synchronize()
     return start, found[:int(total.value)].view(torch.uint32)
 

@@ -258,6 +258,23 @@ gz_status gz_tree_ngb_search(gz_ctx* ctx, const gz_tree* tree, const double* cx,
                              size_t n_q, double tau, size_t* start, uint32_t* found, size_t cap,
                              size_t* total);
 
+/* Tree::init_hmax (tree.h:310-312, tree.cc:332-338): each node's hmax becomes the largest gas
+ * kernel radius below it, h rounded up to float (particles 0 .. min(n_h, n_gas) - 1 are gas; h is
+ * a device array by particle index; h <= 0 raises nothing). Kept in the tree for
+ * gz_tree_ngb_search_reverse; call again when h grows. */
+gz_status gz_tree_init_hmax(gz_ctx* ctx, gz_tree* tree, const double* h, size_t n_h,
+                            size_t n_gas);
+
+/* shmem::ngb_search_reverse (hydro.h:78-83, hydro.cc:110-148) for n_q centres at once
+ * (non-periodic): the gas particles j (j < n_gas) with h_q[q] <= r < h[j], i.e. those whose own
+ * kernel reaches centre q although q's does not; needs gz_tree_init_hmax. h (by particle), the
+ * centres and h_q are device arrays; the output is CSR as gz_tree_ngb_search's, each list in
+ * Morton order (the reference's traversal order). */
+gz_status gz_tree_ngb_search_reverse(gz_ctx* ctx, const gz_tree* tree, const double* h,
+                                     size_t n_gas, const double* cx, const double* cy,
+                                     const double* cz, const double* h_q, size_t n_q, double tau,
+                                     size_t* start, uint32_t* found, size_t cap, size_t* total);
+
 /* shmem::potential (tree.h:446-447, tree.cc:899-944): the spline-softened potential at each
  * target, G included, walked per target like gz_tree_accel (leaves always summed), FP64. */
 gz_status gz_tree_potential(gz_ctx* ctx, const gz_tree* tree, const gz_particles* p,

@@ -75,3 +75,74 @@ def test_ngb_search_empty_and_large(gz):
     # one query covering everything: every live particle, in Morton order
     s, f = gz.ngb_search(T, np.zeros((1, 3)), np.array([1e6]))
     assert np.array_equal(f.cpu().numpy(), T.download()["orderbuf"])
+
+
+@pytest.mark.parametrize("name", ["plummer20k", "collocated", "tombstones", "n17"])
+def test_ngb_search_reverse_matches_oracle(gz, oracle, name):
+    """The reverse search (hydro.cc:110-148): device lists equal to the oracle's, which are the
+    reference's and the brute-force h_q <= r < h_j sets."""
+    from oracle.bindings import HostParticles
+    x, y, z, m, soft = tree_cases.CASES[name]()
+    print(f"[ngb-rev] case={name}", flush=True)
+    n = x.shape[0]
+    rng = np.random.default_rng(11)
+    h = 0.04 * rng.uniform(0.3, 2.0, n)
+    h[rng.random(n) < 0.05] = 0.0
+    n_gas = max(1, int(0.7 * n))
+    hp = HostParticles(x, y, z, m, soft)
+    P = gz.Particles(x, y, z, m, soft)
+    T = gz.build(P)
+    T.init_hmax(h, n_gas)
+    nq = min(300, n)
+    idx = rng.choice(n, nq, replace=False)
+    c = np.stack([x[idx], y[idx], z[idx]], 1)
+    hq = np.where(h[idx] > 0, h[idx], 0.02)
+    to = oracle.build(hp)
+    try:
+        oracle.init_hmax(to, h, n_gas)
+        s_o, f_o = oracle.ngb_search_reverse(to, hp, h, n_gas, c, hq)
+    finally:
+        oracle.free(to)
+    s_d, f_d = gz.ngb_search_reverse(T, h, n_gas, c, hq)
+    assert np.array_equal(s_d.cpu().numpy(), s_o)
+    assert np.array_equal(f_d.cpu().numpy(), f_o)
+
+
+def test_ngb_search_reverse_on_an_aged_tree(gz, oracle):
+    from oracle.bindings import HostParticles
+    pos, v0, _, _ = _scenario(n=5000, seed=31)
+    tau = 0.05
+    xt = [pos[d] + v0[d] * tau for d in range(3)]
+    n = pos[0].shape[0]
+    rng = np.random.default_rng(5)
+    h = 0.08 * rng.uniform(0.5, 1.5, n)
+    n_gas = n // 2
+    P0 = gz.Particles(*pos)
+    T = gz.build(P0, vel=v0)
+    T.init_hmax(h, n_gas)
+    Pt = gz.Particles(xt[0], xt[1], xt[2], pos[3], pos[4])
+    T.update_sources(Pt)
+    idx = rng.choice(n, 400, replace=False)
+    c = np.stack([xt[0][idx], xt[1][idx], xt[2][idx]], 1)
+    hq = h[idx]
+    s_d, f_d = gz.ngb_search_reverse(T, h, n_gas, c, hq, tau)
+    hp0 = HostParticles(*pos)
+    hpt = HostParticles(xt[0], xt[1], xt[2], pos[3], pos[4])
+    to = oracle.build_vel(hp0, v0)
+    try:
+        oracle.init_hmax(to, h, n_gas)
+        s_o, f_o = oracle.ngb_search_reverse(to, hpt, h, n_gas, c, hq, tau)
+    finally:
+        oracle.free(to)
+    assert s_o[-1] > 0
+    assert np.array_equal(s_d.cpu().numpy(), s_o)
+    assert np.array_equal(f_d.cpu().numpy(), f_o)
+
+
+def test_ngb_search_reverse_needs_hmax(gz):
+    x, y, z, m, soft = tree_cases.CASES["n17"]()
+    P = gz.Particles(x, y, z, m, soft)
+    T = gz.build(P)
+    with pytest.raises(Exception, match="init_hmax"):
+        gz.ngb_search_reverse(T, np.full(x.shape[0], 0.1), x.shape[0],
+                              np.zeros((2, 3)), np.full(2, 0.05))
