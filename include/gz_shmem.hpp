@@ -223,6 +223,14 @@ class Tree {
                       "gz_tree_refresh_walk_nodes");
         detail::sync();
     }
+    // Tree::init_hmax (tree.h:310-312, tree.cc:332-338): node hmax from the gas kernel radii h
+    // (one per particle; the first n_gas are gas), for ngb_search_reverse
+    void init_hmax(const std::vector<double>& h, size_t n_gas) {
+        auto& w = detail::ws();
+        const double* d_h = w.extra.upload(h);
+        detail::check(gz_tree_init_hmax(detail::ctx(), t_.get(), d_h, h.size(), n_gas),
+                      "gz_tree_init_hmax");
+    }
     // Tree::raise_soft (tree.cc:311-319) for the leaf of each listed particle, to soft[i]
     void raise_soft(const std::vector<uint32_t>& particles, const std::vector<double>& soft) {
         if (soft.size() != n()) throw Error("raise_soft: soft must hold one value per particle");
@@ -480,6 +488,62 @@ inline void ngb_search(const Tree& T, const Particles& P, const Vec3d& centre, d
     if (lazy) throw Error("ngb_search: lazy drift is not supported");
     std::vector<std::vector<uint32_t>> f(1);
     ngb_search_many(T, P, std::vector<Vec3d>{centre}, std::vector<double>{radius}, f);
+    found.insert(found.end(), f[0].begin(), f[0].end());
+}
+
+// shmem::ngb_search_reverse (hydro.h:78-83, hydro.cc:110-148): APPENDS to found[q] the gas
+// particles j (j < n_gas) with h_q[q] <= r < h[j], whose own kernel reaches centre q, in Morton
+// order; h holds at least n_gas radii by particle; needs Tree::init_hmax. Node boxes are
+// widened by vmax * t_since_build. One launch for all centres.
+inline void ngb_search_reverse_many(const Tree& T, const Particles& P, const double* h,
+                                    size_t n_gas, const std::vector<Vec3d>& centres,
+                                    const std::vector<double>& h_q,
+                                    std::vector<std::vector<uint32_t>>& found) {
+    const size_t nq = centres.size();
+    if (h_q.size() != nq) throw Error("ngb_search_reverse: one h_q per centre");
+    found.resize(nq);
+    if (nq == 0) return;
+    const gz_particles p = WalkAccess::stage(T, P, "ngb_search_reverse");
+    (void)p;
+    std::vector<double> c(3 * nq);
+    for (size_t q = 0; q < nq; ++q)
+        for (int d = 0; d < 3; ++d) c[d * nq + q] = centres[q].data[d];
+    auto& w = detail::ws();
+    double* d_c = w.v.upload(c);
+    const double* d_r = w.aold.upload(h_q);
+    const double* d_h = w.extra.upload(std::vector<double>(h, h + n_gas));
+    detail::DevArray<size_t> start;
+    start.reserve(nq + 1);
+    size_t total = 0;
+    detail::check(gz_tree_ngb_search_reverse(detail::ctx(), WalkAccess::tree(T), d_h, n_gas,
+                                             d_c, d_c + nq, d_c + 2 * nq, d_r, nq,
+                                             T.t_since_build, start.get(), nullptr, 0, &total),
+                  "gz_tree_ngb_search_reverse");
+    detail::DevArray<uint32_t> hits;
+    hits.reserve(total + 1);
+    detail::check(gz_tree_ngb_search_reverse(detail::ctx(), WalkAccess::tree(T), d_h, n_gas,
+                                             d_c, d_c + nq, d_c + 2 * nq, d_r, nq,
+                                             T.t_since_build, start.get(), hits.get(), total,
+                                             &total),
+                  "gz_tree_ngb_search_reverse");
+    detail::sync();
+    std::vector<size_t> hs(nq + 1);
+    std::vector<uint32_t> hh(total);
+    start.download(hs.data(), nq + 1);
+    hits.download(hh.data(), total);
+    for (size_t q = 0; q < nq; ++q)
+        found[q].insert(found[q].end(), hh.begin() + (std::ptrdiff_t)hs[q],
+                        hh.begin() + (std::ptrdiff_t)hs[q + 1]);
+}
+
+inline void ngb_search_reverse(const Tree& T, const Particles& P, const double* h, size_t n_gas,
+                               const Vec3d& centre, double h_i, std::vector<uint32_t>& found,
+                               double box = 0.0, const LazyDrift* lazy = nullptr) {
+    if (box != 0.0) throw Error("ngb_search_reverse: periodic boxes are not supported");
+    if (lazy) throw Error("ngb_search_reverse: lazy drift is not supported");
+    std::vector<std::vector<uint32_t>> f(1);
+    ngb_search_reverse_many(T, P, h, n_gas, std::vector<Vec3d>{centre}, std::vector<double>{h_i},
+                            f);
     found.insert(found.end(), f[0].begin(), f[0].end());
 }
 

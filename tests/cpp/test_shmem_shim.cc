@@ -310,6 +310,31 @@ int main() {
         std::vector<uint32_t> first(found[0]);
         std::sort(first.begin(), first.end());
         expect(one == first, "single-centre ngb_search appends the same list");
+        // reverse search: kernel radii 0.05..0.25, the first 70% of the particles gas
+        std::vector<double> hk(m2);
+        for (size_t i = 0; i < m2; ++i) hk[i] = 0.05 + 0.2 * (double)((i * 7919) % 1000) / 1000.0;
+        const size_t n_gas = (7 * m2) / 10;
+        std::vector<double> hqr(cen.size(), 0.06);
+        R.init_hmax(hk, n_gas);
+        gzo_init_hmax(tq, hk.data(), hk.size(), n_gas, nullptr);
+        std::vector<std::vector<uint32_t>> rfound;
+        gz_shmem::ngb_search_reverse_many(R, Q, hk.data(), n_gas, cen, hqr, rfound);
+        const size_t rtot = gzo_ngb_search_reverse(tq, &hq3, hk.data(), n_gas, hcx.data(),
+                                                   hcy.data(), hcz.data(), hqr.data(), cen.size(),
+                                                   0.1, ostart.data(), nullptr, 0);
+        std::vector<uint32_t> rofound(rtot + 1);
+        gzo_ngb_search_reverse(tq, &hq3, hk.data(), n_gas, hcx.data(), hcy.data(), hcz.data(),
+                               hqr.data(), cen.size(), 0.1, ostart.data(), rofound.data(), rtot);
+        bool rsame = rfound.size() == cen.size();
+        size_t rn = 0;
+        for (size_t q = 0; rsame && q < cen.size(); ++q) {
+            const std::vector<uint32_t> b(rofound.begin() + (std::ptrdiff_t)ostart[q],
+                                          rofound.begin() + (std::ptrdiff_t)ostart[q + 1]);
+            rsame = rfound[q] == b;
+            rn += b.size();
+        }
+        expect(rsame && rn > 0, "reverse neighbour lists equal the oracle's, in order (" +
+                                    std::to_string(rn) + " pairs)");
         gzo_tree_free(tq);
     }
 
