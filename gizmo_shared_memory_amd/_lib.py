@@ -103,6 +103,8 @@ _PROTOS = {
                                      C.c_size_t, C.POINTER(C.c_size_t)]),
     "gz_tree_init_hmax": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t,
                                     C.c_size_t]),
+    "gz_tree_raise_hmax": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t,
+                                     C.c_void_p]),
     "gz_tree_ngb_search_reverse": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t,
                                              C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
                                              C.c_size_t, C.c_double, C.c_void_p, C.c_void_p,

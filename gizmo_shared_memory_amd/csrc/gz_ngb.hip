@@ -215,6 +215,29 @@ __global__ __launch_bounds__(256) void k_hmax_leaf(size_t n_live, const uint32_t
     atomicMax(&hbits[leaf], __float_as_uint(h_as_bound(hj)));
 }
 
+// raise_hmax (tree.cc:293-301) for the leaf of each listed particle: up the ancestors with an
+// atomic max on the float bits, stopping where a node already holds as much.
+__global__ __launch_bounds__(256) void k_raise_hmax(const uint32_t* __restrict__ idx, size_t n_idx,
+                                                    size_t n, const int* __restrict__ leaf_of,
+                                                    const int* __restrict__ parent,
+                                                    const double* __restrict__ h,
+                                                    unsigned* __restrict__ hbits,
+                                                    int* __restrict__ status) {
+    const size_t k = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (k >= n_idx) return;
+    const uint32_t i = idx[k];
+    if (i >= n) {
+        atomicOr(status, 1);
+        return;
+    }
+    const int leaf = leaf_of[i];
+    const double hi = h[i];
+    if (leaf < 0 || !(hi > 0)) return;
+    const unsigned want = __float_as_uint(h_as_bound(hi));
+    for (int no = leaf; no >= 0; no = parent[no])
+        if (atomicMax(&hbits[no], want) >= want) break;
+}
+
 __global__ __launch_bounds__(256) void k_hmax_up(int lo, int hi, const WNode* __restrict__ wn,
                                                  float* __restrict__ hmax) {
     const int n = lo + blockIdx.x * 256 + threadIdx.x;
@@ -401,5 +424,31 @@ extern "C" gz_status gz_tree_init_hmax(gz_ctx* ctx, gz_tree* tree, const double*
         }
     }
     GZ_HIP(hipStreamSynchronize(st));
+    return GZ_OK;
+}
+
+extern "C" gz_status gz_tree_raise_hmax(gz_ctx* ctx, gz_tree* tree, const uint32_t* idx, size_t n,
+                                        const double* h) {
+    clear_error();
+    if (!ctx || !tree) return fail(GZ_ERR_INVALID, "gz_tree_raise_hmax: null argument");
+    if (n == 0) return GZ_OK;
+    if (!idx || !h) return fail(GZ_ERR_INVALID, "gz_tree_raise_hmax: null array");
+    if (tree->nnodes > 0 && !tree->hmax)
+        return fail(GZ_ERR_INVALID, "gz_tree_raise_hmax: no node hmax (call gz_tree_init_hmax first)");
+    if (tree->nnodes <= 0) return GZ_OK;
+    GZ_HIP(hipSetDevice(ctx->device));
+    hipStream_t st = ctx->stream;
+    DevMem& sm = ctx->scratch[8];
+    GZ_TRY(sm.reserve(256));
+    int* status = sm.as<int>();
+    GZ_TRY(fill_zero_u32(reinterpret_cast<uint32_t*>(status), 1, st));
+    k_raise_hmax<<<(unsigned)((n + 255) / 256), 256, 0, st>>>(
+        idx, n, tree->n, tree->leaf_of, tree->parent, h, reinterpret_cast<unsigned*>(tree->hmax),
+        status);
+    GZ_HIP(hipGetLastError());
+    int hs = 0;
+    GZ_HIP(hipMemcpyAsync(&hs, status, sizeof hs, hipMemcpyDeviceToHost, st));
+    GZ_HIP(hipStreamSynchronize(st));
+    if (hs & 1) return fail(GZ_ERR_INVALID, "gz_tree_raise_hmax: an index is outside the particles");
     return GZ_OK;
 }

@@ -217,6 +217,15 @@ class Tree:
                                            int(hh.shape[0]), int(n_gas)), "gz_tree_init_hmax")
         self.ctx.synchronize()
 
+    def raise_hmax(self, idx, h):
+        """Tree::raise_hmax (tree.cc:293-301) for the particles idx; h indexed by particle."""
+        ix = _dev_u32(idx, self.P.device)
+        hh = _dev_f64(h, self.P.device)
+        self.ctx.enter()
+        L.check(L.load().gz_tree_raise_hmax(self.ctx.handle, self.handle, _ptr(ix),
+                                            int(ix.shape[0]), _ptr(hh)), "gz_tree_raise_hmax")
+        self.ctx.synchronize()
+
     def raise_soft(self, idx, soft):
         """Tree::raise_soft (tree.cc:311-319) for the particles idx; soft indexed by particle."""
         ix = _dev_u32(idx, self.P.device)

@@ -265,6 +265,12 @@ gz_status gz_tree_ngb_search(gz_ctx* ctx, const gz_tree* tree, const double* cx,
 gz_status gz_tree_init_hmax(gz_ctx* ctx, gz_tree* tree, const double* h, size_t n_h,
                             size_t n_gas);
 
+/* Tree::raise_hmax (tree.h:310-311, tree.cc:293-301) for the leaf of each listed particle idx[k]
+ * (device arrays), to h[idx[k]] (h by particle; h <= 0 raises nothing): hmax only grows between
+ * rebuilds, as the reference's does. Needs gz_tree_init_hmax. */
+gz_status gz_tree_raise_hmax(gz_ctx* ctx, gz_tree* tree, const uint32_t* idx, size_t n,
+                             const double* h);
+
 /* shmem::ngb_search_reverse (hydro.h:78-83, hydro.cc:110-148) for n_q centres at once
  * (non-periodic): the gas particles j (j < n_gas) with h_q[q] <= r < h[j], i.e. those whose own
  * kernel reaches centre q although q's does not; needs gz_tree_init_hmax. h (by particle), the

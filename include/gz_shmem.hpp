@@ -231,6 +231,15 @@ class Tree {
         detail::check(gz_tree_init_hmax(detail::ctx(), t_.get(), d_h, h.size(), n_gas),
                       "gz_tree_init_hmax");
     }
+    // Tree::raise_hmax (tree.cc:293-301) for the leaf of each listed particle, to h[i]
+    void raise_hmax(const std::vector<uint32_t>& particles, const std::vector<double>& h) {
+        if (h.size() != n()) throw Error("raise_hmax: h must hold one value per particle");
+        auto& w = detail::ws();
+        const uint32_t* d_idx = w.idx.upload(particles);
+        const double* d_h = w.extra.upload(h);
+        detail::check(gz_tree_raise_hmax(detail::ctx(), t_.get(), d_idx, particles.size(), d_h),
+                      "gz_tree_raise_hmax");
+    }
     // Tree::raise_soft (tree.cc:311-319) for the leaf of each listed particle, to soft[i]
     void raise_soft(const std::vector<uint32_t>& particles, const std::vector<double>& soft) {
         if (soft.size() != n()) throw Error("raise_soft: soft must hold one value per particle");

@@ -199,6 +199,8 @@ class Oracle(_Lib):
                                      C.c_size_t]
         L.gzo_init_hmax.argtypes = [C.c_void_p, _dp, C.c_size_t, C.c_size_t,
                                     C.POINTER(C.c_float)]
+        L.gzo_raise_hmax.argtypes = [C.c_void_p, _up, C.c_size_t, _dp]
+        L.gzo_tree_hmax.argtypes = [C.c_void_p, C.POINTER(C.c_float)]
         L.gzo_ngb_search_reverse.restype = C.c_size_t
         L.gzo_ngb_search_reverse.argtypes = [C.c_void_p, C.POINTER(_GzoParticles), _dp,
                                              C.c_size_t, _dp, _dp, _dp, _dp, C.c_size_t,
@@ -302,6 +304,19 @@ class Oracle(_Lib):
         out = np.zeros(int(sz[0]), dtype=np.float32)
         self.lib.gzo_init_hmax(t, _ptr(hh, _dp), hh.shape[0], int(n_gas),
                                out.ctypes.data_as(C.POINTER(C.c_float)))
+        return out
+
+    def raise_hmax(self, t, idx, h):
+        ix = np.ascontiguousarray(idx, dtype=np.uint32)
+        hh = _f64(h)
+        self.lib.gzo_raise_hmax(t, _ptr(ix, _up), ix.shape[0], _ptr(hh, _dp))
+
+    def hmax(self, t):
+        """The node hmax as the oracle tree holds it (init_hmax, then raises)."""
+        sz = np.zeros(3, dtype=np.int64)
+        self.lib.gzo_tree_sizes(t, _ptr(sz, _llp))
+        out = np.zeros(int(sz[0]), dtype=np.float32)
+        self.lib.gzo_tree_hmax(t, out.ctypes.data_as(C.POINTER(C.c_float)))
         return out
 
     def ngb_search_reverse(self, t, P: HostParticles, h, n_gas, centres, hq, tau=0.0):
@@ -408,6 +423,7 @@ class Reference(_Lib):
                                        C.c_double, C.POINTER(C.c_size_t), _up, C.c_size_t]
         L.gzref_init_hmax.argtypes = [C.c_void_p, _dp, C.c_size_t, C.c_size_t,
                                       C.POINTER(C.c_float)]
+        L.gzref_raise_hmax.argtypes = [C.c_void_p, _up, C.c_size_t, _dp, C.POINTER(C.c_float)]
         L.gzref_ngb_search_reverse.restype = C.c_size_t
         L.gzref_ngb_search_reverse.argtypes = [C.c_void_p, C.c_void_p, _dp, C.c_size_t, _dp, _dp,
                                                _dp, _dp, C.c_size_t, C.c_double,
@@ -505,6 +521,16 @@ class Reference(_Lib):
         out = np.zeros(int(sz[0]), dtype=np.float32)
         self.lib.gzref_init_hmax(t, _ptr(hh, _dp), hh.shape[0], int(n_gas),
                                  out.ctypes.data_as(C.POINTER(C.c_float)))
+        return out
+
+    def raise_hmax(self, t, idx, h):
+        ix = np.ascontiguousarray(idx, dtype=np.uint32)
+        hh = _f64(h)
+        sz = np.zeros(3, dtype=np.int64)
+        self.lib.gzref_tree_sizes(t, _ptr(sz, _llp))
+        out = np.zeros(int(sz[0]), dtype=np.float32)
+        self.lib.gzref_raise_hmax(t, _ptr(ix, _up), ix.shape[0], _ptr(hh, _dp),
+                                  out.ctypes.data_as(C.POINTER(C.c_float)))
         return out
 
     def ngb_search_reverse(self, t, rp, h, n_gas, centres, hq, tau=0.0):

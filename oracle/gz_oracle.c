@@ -549,6 +549,28 @@ void gzo_init_hmax(void* t, const double* h, size_t nh, size_t n_gas, float* hma
     if (hmax_out) memcpy(hmax_out, T->hmax, (size_t)nn * sizeof(float));
 }
 
+/* The node hmax (nnodes floats; zeros before gzo_init_hmax). */
+void gzo_tree_hmax(void* t, float* out) {
+    const otree* T = (const otree*)t;
+    for (int i = 0; i < T->nnodes; ++i) out[i] = T->hmax ? T->hmax[i] : 0.0f;
+}
+
+/* raise_hmax (tree.cc:293-301) for the leaf of each listed particle i, to h[i] (by particle);
+ * h[i] <= 0 raises nothing. Needs gzo_init_hmax. */
+void gzo_raise_hmax(void* t, const uint32_t* idx, size_t n, const double* h) {
+    otree* T = (otree*)t;
+    if (!T->hmax) return;
+    for (size_t k = 0; k < n; ++k) {
+        const uint32_t i = idx[k];
+        if (i >= T->n || T->leaf_of[i] < 0 || !(h[i] > 0)) continue;
+        const float hf = h_as_bound(h[i]);
+        for (int no = T->leaf_of[i]; no >= 0; no = T->parent[no]) {
+            if (T->hmax[no] >= hf) break;
+            T->hmax[no] = hf;
+        }
+    }
+}
+
 /* ngb_search_reverse (hydro.cc:110-148), non-periodic, tree aged tau: for each centre q with
  * kernel radius hq[q], every gas particle j (index < n_gas) with hq[q] <= r < h[j], in traversal
  * (Morton) order. A node is skipped when no particle below it has h > hq[q] (hmax <= hq[q]) or

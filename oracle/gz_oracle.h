@@ -19,7 +19,7 @@
  *                         along ballistic motion)
  *   gzo_ngb_search        lists equal to the reference's ngb_search (same order) and, as sets,
  *                         to a brute-force distance test
- *   gzo_init_hmax, gzo_ngb_search_reverse
+ *   gzo_init_hmax, gzo_raise_hmax, gzo_ngb_search_reverse
  *                         hmax bit for bit against the reference's init_hmax and against the
  *                         maximum of the rounded-up h below each node; lists equal to the
  *                         reference's ngb_search_reverse (same order) and, as sets, to a
@@ -109,6 +109,10 @@ size_t gzo_ngb_search(void* t, const gzo_particles* P, const double* cx, const d
 /* init_hmax (tree.cc:332-338): node hmax from the gas kernel radii h[0..min(nh, n_gas)), kept in
  * the tree for the reverse search; copied to hmax_out (nnodes floats) when non-NULL. */
 void gzo_init_hmax(void* t, const double* h, size_t nh, size_t n_gas, float* hmax_out);
+/* The node hmax (nnodes floats; zeros before gzo_init_hmax). */
+void gzo_tree_hmax(void* t, float* out);
+/* raise_hmax (tree.cc:293-301) for the leaf of each listed particle i, to h[i] (by particle). */
+void gzo_raise_hmax(void* t, const uint32_t* idx, size_t n, const double* h);
 /* ngb_search_reverse (hydro.cc:110-148): CSR of the gas particles j with hq[q] <= r < h[j] for
  * each centre q, in traversal (Morton) order; found written while it fits in cap. */
 size_t gzo_ngb_search_reverse(void* t, const gzo_particles* P, const double* h, size_t n_gas,

@@ -315,6 +315,18 @@ void gzref_init_hmax(void* t, const double* h, size_t nh, size_t n_gas, float* h
         std::memcpy(hmax_out, rt->T.hmax.data(), rt->T.hmax.size() * sizeof(float));
 }
 
+// Tree::raise_hmax (tree.cc:293-301) for the leaf of each listed particle, to h[i]; the node
+// values are copied to hmax_out when non-NULL.
+void gzref_raise_hmax(void* t, const uint32_t* idx, size_t n, const double* h, float* hmax_out) {
+    auto* rt = static_cast<RefTree*>(t);
+    for (size_t k = 0; k < n; ++k) {
+        const uint32_t i = idx[k];
+        if (i < rt->T.leaf_of.size() && h[i] > 0) rt->T.raise_hmax(rt->T.leaf_of[i], h[i]);
+    }
+    if (hmax_out)
+        std::memcpy(hmax_out, rt->T.hmax.data(), rt->T.hmax.size() * sizeof(float));
+}
+
 // shmem::ngb_search_reverse (hydro.h:81-83, hydro.cc:110-148) for n_q centres with kernel radii
 // hq, non-periodic, the tree aged tau; CSR output as gzref_ngb_search.
 size_t gzref_ngb_search_reverse(void* t, void* particles, const double* h, size_t n_gas,

@@ -335,6 +335,27 @@ int main() {
         }
         expect(rsame && rn > 0, "reverse neighbour lists equal the oracle's, in order (" +
                                     std::to_string(rn) + " pairs)");
+        // kernels that grow between rebuilds: raise_hmax, then the wider reverse reach
+        std::vector<uint32_t> hgrow;
+        for (size_t i = 0; i < n_gas; i += 7) hgrow.push_back((uint32_t)i);
+        for (uint32_t i : hgrow) hk[i] *= 2.0;
+        R.raise_hmax(hgrow, hk);
+        gzo_raise_hmax(tq, hgrow.data(), hgrow.size(), hk.data());
+        std::vector<std::vector<uint32_t>> gfound;
+        gz_shmem::ngb_search_reverse_many(R, Q, hk.data(), n_gas, cen, hqr, gfound);
+        const size_t gtot = gzo_ngb_search_reverse(tq, &hq3, hk.data(), n_gas, hcx.data(),
+                                                   hcy.data(), hcz.data(), hqr.data(), cen.size(),
+                                                   0.1, ostart.data(), nullptr, 0);
+        std::vector<uint32_t> gofound(gtot + 1);
+        gzo_ngb_search_reverse(tq, &hq3, hk.data(), n_gas, hcx.data(), hcy.data(), hcz.data(),
+                               hqr.data(), cen.size(), 0.1, ostart.data(), gofound.data(), gtot);
+        bool gsame = gfound.size() == cen.size();
+        for (size_t q = 0; gsame && q < cen.size(); ++q)
+            gsame = gfound[q] == std::vector<uint32_t>(
+                                     gofound.begin() + (std::ptrdiff_t)ostart[q],
+                                     gofound.begin() + (std::ptrdiff_t)ostart[q + 1]);
+        expect(gsame && gtot > rn, "after raise_hmax the reverse lists equal the oracle's (" +
+                                       std::to_string(gtot) + " pairs)");
         gzo_tree_free(tq);
     }
 

@@ -146,3 +146,36 @@ def test_ngb_search_reverse_needs_hmax(gz):
     with pytest.raises(Exception, match="init_hmax"):
         gz.ngb_search_reverse(T, np.full(x.shape[0], 0.1), x.shape[0],
                               np.zeros((2, 3)), np.full(2, 0.05))
+
+
+def test_ngb_search_reverse_after_raise_hmax(gz, oracle):
+    """Kernel radii that grow between rebuilds (mfm.cc:297): raise_hmax on the device and in the
+    oracle, then the reverse search finds the new, wider reaches."""
+    from oracle.bindings import HostParticles
+    x, y, z, m, soft = tree_cases.CASES["plummer20k"]()
+    n = x.shape[0]
+    rng = np.random.default_rng(23)
+    h = 0.03 * rng.uniform(0.5, 1.5, n)
+    n_gas = int(0.8 * n)
+    grow = rng.choice(n_gas, 800, replace=False).astype(np.uint32)
+    h2 = h.copy()
+    h2[grow] *= 2.5
+    P = gz.Particles(x, y, z, m, soft)
+    T = gz.build(P)
+    T.init_hmax(h, n_gas)
+    T.raise_hmax(grow, h2)
+    idx = rng.choice(n, 300, replace=False)
+    c = np.stack([x[idx], y[idx], z[idx]], 1)
+    hq = h2[idx]
+    s_d, f_d = gz.ngb_search_reverse(T, h2, n_gas, c, hq)
+    hp = HostParticles(x, y, z, m, soft)
+    to = oracle.build(hp)
+    try:
+        oracle.init_hmax(to, h, n_gas)
+        oracle.raise_hmax(to, grow, h2)
+        s_o, f_o = oracle.ngb_search_reverse(to, hp, h2, n_gas, c, hq)
+    finally:
+        oracle.free(to)
+    assert s_o[-1] > 0
+    assert np.array_equal(s_d.cpu().numpy(), s_o)
+    assert np.array_equal(f_d.cpu().numpy(), f_o)

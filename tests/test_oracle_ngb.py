@@ -115,6 +115,31 @@ def test_hmax_matches_reference_and_subtree_maximum(oracle, reference):
             assert ho[i] == want, (name, i)
 
 
+def test_raise_hmax_matches_reference(oracle, reference):
+    """raise_hmax (tree.cc:293-301) after init_hmax: bit for bit the reference's, and the subtree
+    maximum of the raised radii (h only grows here, so the raises reach what init would)."""
+    P, h, n_gas = _hydro_case("plummer20k")
+    rng = np.random.default_rng(21)
+    grow = rng.choice(n_gas, 500, replace=False).astype(np.uint32)
+    h2 = h.copy()
+    h2[grow] = h2[grow] * 1.7 + 0.01
+    to = oracle.build(P)
+    rp = reference.particles(P)
+    tr = reference.build(rp)
+    to2 = oracle.build(P)
+    try:
+        oracle.init_hmax(to, h, n_gas)
+        reference.init_hmax(tr, h, n_gas)
+        oracle.raise_hmax(to, grow, h2)
+        hr = reference.raise_hmax(tr, grow, h2)
+        ho = oracle.hmax(to)
+        fresh = oracle.init_hmax(to2, h2, n_gas)
+    finally:
+        oracle.free(to); oracle.free(to2); reference.free(tr); reference.free_particles(rp)
+    assert np.array_equal(ho.view(np.uint32), hr.view(np.uint32))
+    assert np.array_equal(ho.view(np.uint32), fresh.view(np.uint32))
+
+
 def test_ngb_search_reverse_matches_reference_and_brute_force(oracle, reference):
     for name in ("plummer20k", "collocated", "tombstones"):
         P, h, n_gas = _hydro_case(name)
