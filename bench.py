@@ -149,6 +149,84 @@ def cpu_baseline(x, y, z, m, soft, orderbuf, leaf_of, aold_all, seconds, threads
             "build_s": build_s}
 
 
+def widened_rows(gz, ics, args):
+    """The path's other calls on the Hernquist 4M (a=1, eps=0.005), each beside the reference's
+    own CPU code (oracle/_ref) on a sample: the potential and the tidal-tensor walks (FP64,
+    theta=0.7, every particle a target; the reference on every 512th particle, all host
+    threads), and the forward and reverse neighbour searches (every 4th particle in Morton order
+    as a query, radius and kernel radius = the side of the particle's leaf; the reference's
+    per-centre calls, one thread, on every 64th query)."""
+    import torch
+    from oracle.bindings import HostParticles, Reference, have_reference
+    x, y, z, m, soft = ics.hernquist()
+    n = x.shape[0]
+    P = gz.Particles(x, y, z, m, soft, device="cuda:0")
+    T = gz.build(P)
+    tg = T.device_orderbuf()
+    td = T.download()
+    out = {"workload": "hernquist N=4194304 eps=0.005"}
+
+    def timed(fn):
+        r = fn()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            r = fn()
+        torch.cuda.synchronize()
+        return r, (time.perf_counter() - t0) / args.steps
+
+    _, dt = timed(lambda: gz.potential(T, P, tg, THETA_BOOT))
+    out["potential_fp64"] = {"ms_per_call": dt * 1e3, "evals_per_s": n / dt}
+    _, dt = timed(lambda: gz.accel_grouped_tidal(T, P, tg, THETA_BOOT))
+    out["tidal_fp64"] = {"ms_per_call": dt * 1e3, "evals_per_s": n / dt}
+    order = td["orderbuf"].astype(np.int64)
+    side = td["size"][td["leaf_of"]]  # per particle: its leaf's side
+    q = order[::4]
+    c = np.stack([x[q], y[q], z[q]], 1)
+    r = side[q]
+    (s, _), dt = timed(lambda: gz.ngb_search(T, c, r))
+    nq = q.shape[0]
+    out["ngb_search"] = {"queries": nq, "per_query": int(s[-1]) / nq, "ms_per_call": dt * 1e3,
+                         "queries_per_s": nq / dt}
+    T.init_hmax(side, n)
+    (s, _), dt = timed(lambda: gz.ngb_search_reverse(T, side, n, c, r))
+    out["ngb_search_reverse"] = {"queries": nq, "per_query": int(s[-1]) / nq,
+                                 "ms_per_call": dt * 1e3, "queries_per_s": nq / dt}
+    if not args.no_cpu and have_reference():
+        ref = Reference()
+        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+        ref.set_threads(threads)
+        rp = ref.particles(HostParticles(x, y, z, m, soft))
+        tr = ref.build(rp)
+        try:
+            smp = order[::512].astype(np.uint32)
+            t0 = time.perf_counter()
+            ref.potential(tr, rp, smp, THETA_BOOT)
+            out["potential_fp64"]["cpu_reference_evals_per_s"] = smp.shape[0] / (
+                time.perf_counter() - t0)
+            t0 = time.perf_counter()
+            ref.accel_grouped_tj(tr, rp, smp, THETA_BOOT)
+            out["tidal_fp64"]["cpu_reference_evals_per_s"] = smp.shape[0] / (
+                time.perf_counter() - t0)
+            cs, rs = c[::64], r[::64]
+            t0 = time.perf_counter()
+            ref.ngb_search(tr, rp, cs, rs)
+            out["ngb_search"]["cpu_reference_queries_per_s_1thread"] = cs.shape[0] / (
+                time.perf_counter() - t0)
+            ref.init_hmax(tr, side, n)
+            t0 = time.perf_counter()
+            ref.ngb_search_reverse(tr, rp, side, n, cs, rs)
+            out["ngb_search_reverse"]["cpu_reference_queries_per_s_1thread"] = cs.shape[0] / (
+                time.perf_counter() - t0)
+            out["cpu_threads_walks"] = threads
+        finally:
+            ref.free(tr)
+            ref.free_particles(rp)
+    log(f"rows: {json.dumps(out)}")
+    del T, P, tg
+    torch.cuda.empty_cache()
+
+
 def other_configs(gz, ics, args):
     """The other named workloads, timed like the headline walk: the FP64 parity walk (per-leaf
     groups) and the FP32 walk (group size tuned), both with the relative criterion and aold from
@@ -215,6 +293,10 @@ def main():
                          "the FP32 walk); a profiler run can keep to one walk kernel")
     ap.add_argument("--model", default=os.path.join(ROOT, "gizmo_shared_memory_amd",
                                                      "learned_criterion.json"))
+    ap.add_argument("--rows", action="store_true",
+                    help="also time the path's other calls (potential, tidal tensor, forward and "
+                         "reverse neighbour search) on the Hernquist 4M beside the reference's "
+                         "CPU code, logged to stderr; the stdout line is unchanged")
     ap.add_argument("--configs", action="store_true",
                     help="also time the other named workloads (Plummer 64k, Hernquist 4M, "
                          "clustered 16M: build, FP64 and FP32 walks, errors against the direct "
@@ -341,6 +423,8 @@ def main():
 
     if args.configs:
         other_configs(gz, ics, args)
+    if args.rows:
+        widened_rows(gz, ics, args)
 
     if args.copy_mb > 0:
         src = torch.ones(args.copy_mb << 18, dtype=torch.float32, device="cuda:0")

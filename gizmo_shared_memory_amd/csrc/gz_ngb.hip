@@ -55,6 +55,9 @@ __device__ __forceinline__ float snode_half(const NgbArgs& a, int i) {
 // h_q <= r < h[j].
 template <bool FILL, bool REV>
 __global__ __launch_bounds__(64) void k_ngb(NgbArgs a) {
+    __shared__ int s_lpre[64], s_lplo[64]; // kept leaves: first staged index, first slot
+    __shared__ double s_px[64], s_py[64], s_pz[64], s_h2[64];
+    __shared__ int s_slot[64];
     const int lane = threadIdx.x;
     int* const stk = a.stack + (size_t)blockIdx.x * (size_t)a.stack_cap;
     for (;;) {
@@ -91,6 +94,8 @@ __global__ __launch_bounds__(64) void k_ngb(NgbArgs a) {
             const int node = (lane < k) ? stk[base + lane] : -1;
             sp = base;
             bool keep = false, leaf = false;
+            // the node as the per-query test below needs it (kept leaves only)
+            double nc0 = 0, nc1 = 0, nc2 = 0, nhalf = 0, nxpad = 0, nvmax = 0, nhm = 0;
             int first = -1, nch = 0, plo = 0, phi = 0;
             if (node >= 0) {
                 const WNode w = a.wn[node];
@@ -113,6 +118,11 @@ __global__ __launch_bounds__(64) void k_ngb(NgbArgs a) {
                         leaf = true;
                         plo = w.plo;
                         phi = w.phi;
+                        nc0 = c0; nc1 = c1; nc2 = c2;
+                        nhalf = (double)half;
+                        nxpad = (double)a.xpad[node];
+                        nvmax = (double)a.vmax[node];
+                        nhm = hm;
                     } else {
                         nch = w.nchild;
                     }
@@ -127,27 +137,79 @@ __global__ __launch_bounds__(64) void k_ngb(NgbArgs a) {
             }
             for (int c = 0; c < nch; ++c) stk[sp + coff + c] = first + c;
             sp += ctot;
-            // particles of kept leaves, tested by every lane against its own query, leaf by leaf
-            uint64_t lb = __ballot(leaf);
-            while (lb) {
-                const int src_lane = __ffsll((unsigned long long)lb) - 1;
-                lb &= lb - 1;
-                const int lo = __shfl(plo, src_lane), hi = __shfl(phi, src_lane);
-                for (int s = lo; s < hi; ++s) {
-                    const double4 p = a.src[s];
-                    const double ex = p.x - qx, ey = p.y - qy, ez = p.z - qz;
-                    const double d2 = ex * ex + ey * ey + ez * ez;
-                    bool in;
-                    if (REV) {
-                        const uint32_t j = a.orderbuf[s];
-                        in = j < a.n_gas && d2 >= r2 && d2 < a.h[j] * a.h[j];
-                    } else {
-                        in = d2 < r2;
+            // A kept leaf is dropped when the reference's node test (hydro.cc:55-72, reverse
+            // :120-133) rejects it for every query of the group: the group's box and largest
+            // radius admit it, but no single query needs it.
+            {
+                uint64_t rem = __ballot(leaf), need = 0;
+                while (rem) {
+                    const int sl = __ffsll((unsigned long long)rem) - 1;
+                    rem &= rem - 1;
+                    const double b0 = __shfl(nc0, sl), b1 = __shfl(nc1, sl), b2 = __shfl(nc2, sl);
+                    const double bh = __shfl(nhalf, sl), bx = __shfl(nxpad, sl);
+                    const double bv = __shfl(nvmax, sl), bm = REV ? __shfl(nhm, sl) : 0.0;
+                    bool hit = false;
+                    if (live && (!REV || bm > r)) {
+                        double dist = (REV ? bm : r) + bh + bx;
+                        if (a.tau > 0) dist += bv * a.tau;
+                        const double dx = b0 - qx, dy = b1 - qy, dz = b2 - qz;
+                        if (!(dx > dist || -dx > dist) && !(dy > dist || -dy > dist) &&
+                            !(dz > dist || -dz > dist)) {
+                            dist += 0.7320508075688772 * bh;
+                            hit = !(dx * dx + dy * dy + dz * dz > dist * dist);
+                        }
                     }
-                    if (live && in) {
-                        if (FILL) a.slots[out++] = (uint32_t)s;
-                        ++found;
+                    if (__ballot(hit)) need |= 1ull << sl;
+                }
+                leaf = leaf && ((need >> lane) & 1ull);
+            }
+            // Particles of the kept leaves: the wave loads up to 64 at a time, one per lane, into
+            // LDS, and every lane then tests its own query against each staged particle.
+            const uint64_t lb = __ballot(leaf);
+            if (lb) {
+                const int nleaf = __popcll(lb);
+                int ptot = 0;
+                const int poff = wave_excl_scan(leaf ? phi - plo : 0, ptot);
+                if (leaf) {
+                    const int li = (int)__builtin_amdgcn_mbcnt_hi(
+                        (uint32_t)(lb >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)lb, 0u));
+                    s_lpre[li] = poff;
+                    s_lplo[li] = plo;
+                }
+                __syncthreads();
+                for (int base = 0; base < ptot; base += 64) {
+                    const int idx = base + lane;
+                    if (idx < ptot) {
+                        int lo_ = 0, hi_ = nleaf; // the last leaf li with s_lpre[li] <= idx
+                        while (hi_ - lo_ > 1) {
+                            const int mid = (lo_ + hi_) >> 1;
+                            if (s_lpre[mid] <= idx) lo_ = mid;
+                            else hi_ = mid;
+                        }
+                        const int s = s_lplo[lo_] + (idx - s_lpre[lo_]);
+                        const double4 p = a.src[s];
+                        s_px[lane] = p.x;
+                        s_py[lane] = p.y;
+                        s_pz[lane] = p.z;
+                        s_slot[lane] = s;
+                        if (REV) {
+                            const uint32_t j = a.orderbuf[s];
+                            const double hj = (j < a.n_gas) ? a.h[j] : 0.0;
+                            s_h2[lane] = (j < a.n_gas) ? hj * hj : -1.0; // -1: not gas
+                        }
                     }
+                    __syncthreads();
+                    const int cnt = min(64, ptot - base);
+                    for (int e = 0; e < cnt; ++e) {
+                        const double ex = s_px[e] - qx, ey = s_py[e] - qy, ez = s_pz[e] - qz;
+                        const double d2 = ex * ex + ey * ey + ez * ez;
+                        const bool in = REV ? (d2 >= r2 && d2 < s_h2[e]) : (d2 < r2);
+                        if (live && in) {
+                            if (FILL) a.slots[out++] = (uint32_t)s_slot[e];
+                            ++found;
+                        }
+                    }
+                    __syncthreads();
                 }
             }
             __threadfence_block();
