@@ -477,7 +477,8 @@ __global__ __launch_bounds__(64, TJ ? 3 : (FP64 ? 4 : 5)) void k_walk(WalkArgs a
             const int nt = min(kMaxChunk, nt_all - c0);
             // Lanes are (target t, slice sub) pairs: lane = sub * nt + t. Each of the L slices
             // takes every L-th list entry; lanes beyond L * nt sit out the evaluation.
-            const int L = min(64 / nt, 8);
+            // (the FP32 lists' padding covers two steps of up to 8 slices; FP64 loops test bounds)
+            const int L = min(64 / nt, FP64 ? 64 : 8);
             const int t = lane % nt;
             const int sub = lane / nt;
             const bool tvalid = sub < L;
@@ -912,12 +913,13 @@ __global__ __launch_bounds__(64, TJ ? 3 : (FP64 ? 4 : 5)) void k_walk(WalkArgs a
             }
             if (mcount > 0 || pcount > 0) flush();
 
-            // combine the slices of each target, then write (tree.cc:880)
-            for (int k = 1; k < L; ++k) {
-                const int src = min(lane + k * nt, 63);
+            // combine the slices of each target pairwise (log2 L steps), then write (tree.cc:880)
+            for (int step = 1; step < L; step <<= 1) {
+                const int src = min(lane + step * nt, 63);
+                const bool take = (sub % (2 * step) == 0) && (sub + step < L);
                 const double ux = __shfl(accx, src), uy = __shfl(accy, src),
                              uz = __shfl(accz, src);
-                if (sub == 0) {
+                if (take) {
                     accx += ux;
                     accy += uy;
                     accz += uz;
@@ -926,12 +928,12 @@ __global__ __launch_bounds__(64, TJ ? 3 : (FP64 ? 4 : 5)) void k_walk(WalkArgs a
 #pragma unroll
                     for (int c = 0; c < 6; ++c) {
                         const double u = __shfl(tt[c], src);
-                        if (sub == 0) tt[c] += u;
+                        if (take) tt[c] += u;
                     }
 #pragma unroll
                     for (int c = 0; c < 3; ++c) {
                         const double u = __shfl(jj[c], src);
-                        if (sub == 0) jj[c] += u;
+                        if (take) jj[c] += u;
                     }
                 }
             }
