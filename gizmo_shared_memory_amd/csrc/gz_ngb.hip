@@ -113,35 +113,28 @@ __global__ __launch_bounds__(64) void k_ngb(NgbArgs a) {
                     keep = d0 * d0 + d1 * d1 + d2 * d2 <= ds * ds;
                 }
                 if (keep) {
+                    nc0 = c0; nc1 = c1; nc2 = c2;
+                    nhalf = (double)half;
+                    nxpad = (double)a.xpad[node];
+                    nvmax = (double)a.vmax[node];
+                    nhm = hm;
                     first = w.first;
                     if (first < 0) {
                         leaf = true;
                         plo = w.plo;
                         phi = w.phi;
-                        nc0 = c0; nc1 = c1; nc2 = c2;
-                        nhalf = (double)half;
-                        nxpad = (double)a.xpad[node];
-                        nvmax = (double)a.vmax[node];
-                        nhm = hm;
                     } else {
                         nch = w.nchild;
                     }
                 }
             }
-            // children of kept interior nodes go on the stack
-            int ctot = 0;
-            const int coff = wave_excl_scan(nch, ctot);
-            if (sp + ctot > a.stack_cap) {
-                overflow = true;
-                break;
-            }
-            for (int c = 0; c < nch; ++c) stk[sp + coff + c] = first + c;
-            sp += ctot;
-            // A kept leaf is dropped when the reference's node test (hydro.cc:55-72, reverse
+            // A kept node is dropped when the reference's node test (hydro.cc:55-72, reverse
             // :120-133) rejects it for every query of the group: the group's box and largest
-            // radius admit it, but no single query needs it.
+            // radius admit it, but no single query needs it (a group that straddles a jump of
+            // the Morton order has a wide box). The wave then walks the union of its queries'
+            // own traversals.
             {
-                uint64_t rem = __ballot(leaf), need = 0;
+                uint64_t rem = __ballot(keep), need = 0;
                 while (rem) {
                     const int sl = __ffsll((unsigned long long)rem) - 1;
                     rem &= rem - 1;
@@ -161,8 +154,21 @@ __global__ __launch_bounds__(64) void k_ngb(NgbArgs a) {
                     }
                     if (__ballot(hit)) need |= 1ull << sl;
                 }
-                leaf = leaf && ((need >> lane) & 1ull);
+                if (!((need >> lane) & 1ull)) {
+                    keep = false;
+                    leaf = false;
+                    nch = 0;
+                }
             }
+            // children of kept interior nodes go on the stack
+            int ctot = 0;
+            const int coff = wave_excl_scan(nch, ctot);
+            if (sp + ctot > a.stack_cap) {
+                overflow = true;
+                break;
+            }
+            for (int c = 0; c < nch; ++c) stk[sp + coff + c] = first + c;
+            sp += ctot;
             // Particles of the kept leaves: the wave loads up to 64 at a time, one per lane, into
             // LDS, and every lane then tests its own query against each staged particle.
             const uint64_t lb = __ballot(leaf);
