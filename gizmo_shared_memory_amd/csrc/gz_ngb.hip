@@ -1,14 +1,19 @@
-// Neighbour search over the tree (shmem/hydro.cc:37-108, the SNode of tree.h:206-226): every
-// particle within radius_q of centre q, for many centres at once.
+// Neighbour searches over the tree (shmem/hydro.cc:37-148, the SNode of tree.h:206-226), for many
+// centres at once: every particle within radius_q of centre q (ngb_search), and every gas particle
+// whose own kernel reaches centre q although q's does not (ngb_search_reverse, with the node
+// hmax of Tree::init_hmax / raise_hmax, tree.cc:286-338).
 //
-// One wavefront per group of 64 consecutive queries (a Morton-ordered query list makes the groups
-// compact). A node is skipped when it lies farther from the group's bounding box than the group's
-// largest radius plus the node's half side (widened, as the reference's SNode is, for float
-// centres) plus vmax * tau + xpad — the reference's prune taken against the box instead of one
-// centre, so it skips nothing any query of the group needs. Each lane then tests its own query
-// against the particles of every reached leaf with the reference's FP64 expression, so the sets
-// are exact. Two passes: counts, then (with the offsets) the sorted slots of each query's
-// neighbours, which map to particle indices in Morton order, the reference's traversal order.
+// One wavefront per group of 64 consecutive queries. Up to 64 stacked nodes are tested at once,
+// one per lane: first against the group's bounding box with the group's largest radius (for the
+// reverse search: the node's hmax, and only if it exceeds the group's smallest kernel radius),
+// then with the reference's own node test for each query of the group in turn, a node staying
+// only if some query admits it — so the wave walks the union of its queries' traversals even when
+// the group straddles a jump of the Morton order. Node boxes are the reference's SNode boxes
+// (half side widened for float centres) grown by vmax * tau + xpad. The particles of kept leaves
+// are staged through LDS 64 at a time and each lane tests its own query against every staged
+// particle with the reference's FP64 expression, so the sets are exact. Two passes: counts, then
+// (with the offsets) the sorted slots of each query's neighbours, which map to particle indices
+// in Morton order, the reference's traversal order.
 #include <cfloat>
 
 #include "gz_features.h"
