@@ -99,7 +99,7 @@ __global__ __launch_bounds__(64) void k_ngb(NgbArgs a) {
             const int node = (lane < k) ? stk[base + lane] : -1;
             sp = base;
             bool keep = false, leaf = false;
-            // the node as the per-query test below needs it (kept leaves only)
+            // the node as the per-query test below needs it (kept nodes only)
             double nc0 = 0, nc1 = 0, nc2 = 0, nhalf = 0, nxpad = 0, nvmax = 0, nhm = 0;
             int first = -1, nch = 0, plo = 0, phi = 0;
             if (node >= 0) {
