@@ -273,9 +273,9 @@ gz_status gz_tree_raise_hmax(gz_ctx* ctx, gz_tree* tree, const uint32_t* idx, si
 
 /* shmem::ngb_search_reverse (hydro.h:78-83, hydro.cc:110-148) for n_q centres at once
  * (non-periodic): the gas particles j (j < n_gas) with h_q[q] <= r < h[j], i.e. those whose own
- * kernel reaches centre q although q's does not; needs gz_tree_init_hmax. h (by particle), the
- * centres and h_q are device arrays; the output is CSR as gz_tree_ngb_search's, each list in
- * Morton order (the reference's traversal order). */
+ * kernel reaches centre q although q's does not; needs gz_tree_init_hmax. h (by particle, at
+ * least n_gas values), the centres and h_q are device arrays; the output is CSR as
+ * gz_tree_ngb_search's, each list in Morton order (the reference's traversal order). */
 gz_status gz_tree_ngb_search_reverse(gz_ctx* ctx, const gz_tree* tree, const double* h,
                                      size_t n_gas, const double* cx, const double* cy,
                                      const double* cz, const double* h_q, size_t n_q, double tau,
