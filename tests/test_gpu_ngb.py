@@ -179,3 +179,34 @@ def test_ngb_search_reverse_after_raise_hmax(gz, oracle):
     assert s_o[-1] > 0
     assert np.array_equal(s_d.cpu().numpy(), s_o)
     assert np.array_equal(f_d.cpu().numpy(), f_o)
+
+
+def test_searches_on_a_forest_match_brute_force(gz):
+    """Both searches over a forest (one tree per cluster, spheres close enough that kernels of
+    one reach into the next) give exactly the brute-force sets."""
+    from gizmo_shared_memory_amd import ics
+    (x, y, z, m, soft), starts = ics.spheres(n_spheres=4, lo_log2=9.0, hi_log2=11.0, eps=0.01,
+                                             spacing=1.5, seed=6)
+    print("[ngb-forest] 4 clusters", flush=True)
+    n = x.shape[0]
+    P = gz.Particles(x, y, z, m, soft)
+    F = gz.build(P, clusters=starts)
+    rng = np.random.default_rng(8)
+    h = rng.uniform(0.05, 0.4, n)
+    n_gas = int(0.75 * n)
+    F.init_hmax(h, n_gas)
+    idx = rng.choice(n, 256, replace=False)
+    c = np.stack([x[idx], y[idx], z[idx]], 1) + rng.normal(scale=0.05, size=(256, 3))
+    r = rng.uniform(0.05, 0.3, 256)
+    s_f, f_f = gz.ngb_search(F, c, r)
+    s_r, f_r = gz.ngb_search_reverse(F, h, n_gas, c, r)
+    s_f, f_f, s_r, f_r = (t.cpu().numpy() for t in (s_f, f_f, s_r, f_r))
+    live = m > 0
+    gas = np.arange(n) < n_gas
+    for q in range(c.shape[0]):
+        d2 = (x - c[q, 0]) ** 2 + (y - c[q, 1]) ** 2 + (z - c[q, 2]) ** 2
+        want_f = np.nonzero(live & (d2 < r[q] ** 2))[0]
+        want_r = np.nonzero(live & gas & (d2 >= r[q] ** 2) & (d2 < h ** 2))[0]
+        assert np.array_equal(np.sort(f_f[s_f[q]:s_f[q + 1]]), want_f), q
+        assert np.array_equal(np.sort(f_r[s_r[q]:s_r[q + 1]]), want_r), q
+    assert s_f[-1] > 0 and s_r[-1] > 0
