@@ -24,7 +24,13 @@ import os
 import sys
 import time
 
-import numpy as np
+# The reference's run recipe (shmem/README.md:49-57) binds OpenMP threads one per core, spread
+# over the sockets. OpenMP reads these once, when its runtime is first loaded, so they are set
+# before anything that links it is imported; a value already in the environment is kept.
+os.environ.setdefault("OMP_PROC_BIND", "spread")
+os.environ.setdefault("OMP_PLACES", "cores")
+
+import numpy as np  # noqa: E402
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 if ROOT not in sys.path:
@@ -92,9 +98,37 @@ def time_walk(gz, T, P, targets, aold, steps, warmup, **kw):
     return out, dt, float(np.mean(walk_ms))
 
 
-def cpu_baseline(x, y, z, m, soft, orderbuf, leaf_of, aold_all, seconds, threads):
+def host_cores():
+    """Threads for the CPU baseline and a description of the host. The count is the cores this
+    process may run on (its affinity mask), capped by OMP_NUM_THREADS when that is set: a GPU
+    box grants each GPU a share of its cores and says how many there."""
+    avail = len(os.sched_getaffinity(0))
+    env = os.environ.get("OMP_NUM_THREADS", "")
+    threads = min(avail, int(env)) if env.isdigit() and int(env) > 0 else avail
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    info = {"threads": threads, "affinity_cpus": avail, "nproc_online": os.cpu_count(),
+            "omp_num_threads_env": env or None, "cpu_model": model,
+            "omp_proc_bind": os.environ.get("OMP_PROC_BIND"),
+            "omp_places": os.environ.get("OMP_PLACES")}
+    bi = os.path.join(ROOT, "oracle", "_ref", "build_info.txt")
+    if os.path.exists(bi):
+        with open(bi) as f:
+            info["reference_build"] = f.read().strip()
+    return threads, info
+
+
+def cpu_baseline(x, y, z, m, soft, orderbuf, leaf_of, aold_all, seconds, threads, reps=3):
     """The reference walk (accel_grouped, batch 0, relative criterion) on whole leaves sampled
-    uniformly along the Morton order, timed on `threads` host cores."""
+    uniformly along the Morton order, timed on `threads` host cores: one warm-up run, then the
+    minimum of `reps` timed runs (shmem/bench_layout.cc:7-11)."""
     from oracle.bindings import HostParticles, Oracle, Reference, have_reference
     hp = HostParticles(x, y, z, m, soft)
     leaf_seq = leaf_of[orderbuf]
@@ -137,16 +171,21 @@ def cpu_baseline(x, y, z, m, soft, orderbuf, leaf_of, aold_all, seconds, threads
         want = seconds / max(per, 1e-12)
         stride = int(max(1, min(n_runs, round(len(orderbuf) / max(want, 1.0)))))
         tg, ao = sample(stride)
-        t0 = time.perf_counter()
-        walk(tg, ao)
-        dt = time.perf_counter() - t0
+        walk(tg, ao)  # warm-up
+        times = []
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            walk(tg, ao)
+            times.append(time.perf_counter() - t0)
+        dt = min(times)
     finally:
         cleanup()
     return {"value": len(tg) / dt, "unit": "force evals/s", "cores": threads, "kind": kind,
             "sample": f"{len(tg)} targets = every {stride}th leaf of the {n_runs} leaves in Morton "
                       f"order, relative criterion ErrTolForceAcc={ERR_TOL_FORCE_ACC}, aold from the "
-                      f"theta={THETA_BOOT} bootstrap; {dt:.2f} s walk; tree build {build_s:.2f} s",
-            "build_s": build_s}
+                      f"theta={THETA_BOOT} bootstrap; min of {reps} timed walks after a warm-up: "
+                      f"{', '.join(f'{t:.2f}' for t in times)} s; tree build {build_s:.2f} s",
+            "build_s": build_s, "walk_s": times}
 
 
 def widened_rows(gz, ics, args):
@@ -194,7 +233,7 @@ def widened_rows(gz, ics, args):
                                  "ms_per_call": dt * 1e3, "queries_per_s": nq / dt}
     if not args.no_cpu and have_reference():
         ref = Reference()
-        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+        threads = args.cpu_threads or host_cores()[0]
         ref.set_threads(threads)
         rp = ref.particles(HostParticles(x, y, z, m, soft))
         tr = ref.build(rp)
@@ -268,7 +307,7 @@ def other_configs(gz, ics, args):
             td = T.download()
             cpu = cpu_baseline(x, y, z, m, soft, td["orderbuf"], td["leaf_of"],
                                aold.cpu().numpy(), args.cpu_seconds,
-                               args.cpu_threads or min(16, os.cpu_count() or 1))
+                               args.cpu_threads or host_cores()[0])
             out["cpu_baseline"] = {k: cpu[k] for k in ("value", "unit", "cores", "kind",
                                                        "sample")}
             out["fp64_over_cpu"] = out["fp64"]["evals_per_s"] / cpu["value"]
@@ -284,7 +323,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--n", type=int, default=8388608)
     ap.add_argument("--probes", type=int, default=65536)
-    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--cpu-seconds", type=float, default=6.0,
+                    help="CPU work per timed baseline walk (one warm-up and three timed walks)")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--group-sizes", default="0,16,32,64")
@@ -439,8 +479,10 @@ def main():
     learned_ok = "learned_fp32" in results and head["err"]["p99"] <= 1e-2
     ref_e = rel_err(tuple(v.cpu().numpy()[probes] for v in boot[:3]), ex)
     cpu = None
+    host = None
     if not args.no_cpu:
-        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+        threads, host = host_cores()
+        threads = args.cpu_threads or threads
         td = T.download()
         # targets are orderbuf, so aold is already indexed by Morton slot
         cpu = cpu_baseline(x, y, z, m, soft, td["orderbuf"], td["leaf_of"], aold.cpu().numpy(),
@@ -474,13 +516,14 @@ def main():
             "p99_rel_err": head["err"]["p99"],
             "p99_within_1e-2": bool(head["err"]["p99"] <= 1e-2),
         },
-        "roofline": {"bound": "mfma", "achieved": head["achieved_tflops"],
+        "roofline": {"bound": "valu", "achieved": head["achieved_tflops"],
                      "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
                      "frac": head["achieved_tflops"] / PEAK_FP32_TFLOPS,
                      "traffic": (profiled_traffic(PROFILE, head["kernel_ms"])
                                  if "learned_fp32" in results else None)},
-        "cpu_baseline": None if cpu is None else {k: cpu[k] for k in
-                                                  ("value", "unit", "cores", "kind", "sample")},
+        "cpu_baseline": None if cpu is None else {**{k: cpu[k] for k in
+                                                     ("value", "unit", "cores", "kind", "sample")},
+                                                  "host": host},
         "walks": results,
         "bootstrap_fp64": {"err": err_stats(ref_e), **{k: boot[3][k] for k in
                                                       ("nodes_visited", "multipole", "pairs")}},

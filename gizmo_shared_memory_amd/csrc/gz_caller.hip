@@ -54,7 +54,8 @@ __global__ __launch_bounds__(kT) void k_scatter(const uint32_t* __restrict__ tar
     a_grav[3 * i + 2] = az[k];
 }
 
-// Close the previous step's half kick, open the next one (mfm.cc:3716-3730, 3753-3766).
+// Close the previous step's half kick, open the next one (mfm.cc:3716-3730, 3753-3766). Every
+// array is indexed by particle, dt included (the reference's dt_of[i], mfm.cc:3753, :3764).
 __global__ __launch_bounds__(kT) void k_kick(const uint32_t* __restrict__ active, size_t na,
                                              const double* __restrict__ a_grav,
                                              const double* __restrict__ a_extra,
@@ -71,7 +72,7 @@ __global__ __launch_bounds__(kT) void k_kick(const uint32_t* __restrict__ active
         a2 += a_extra[3 * i + 2];
     }
     const double close_dt = pending[i];
-    const double open_dt = 0.5 * dt[k];
+    const double open_dt = 0.5 * dt[i];
     double v0 = vx[i], v1 = vy[i], v2 = vz[i];
     v0 += a0 * close_dt; v1 += a1 * close_dt; v2 += a2 * close_dt;
     v0 += a0 * open_dt;  v1 += a1 * open_dt;  v2 += a2 * open_dt;

@@ -86,14 +86,23 @@ __global__ __launch_bounds__(64) void k_ngb(NgbArgs a) {
         uint32_t found = 0;
         size_t out = live && FILL ? a.start[q] : 0;
 
-        // the roots: one per tree of a forest
+        // The roots, one per tree of a forest, form a virtual range that is fed onto the stack
+        // whenever it holds less than a batch: a forest of any size then needs no more stack
+        // than one tree plus a batch (the lists are sorted afterwards, so the order in which
+        // the trees are searched does not matter).
+        const int nroots = (a.nclusters > 1) ? a.nclusters : 1;
+        int rnext = 0;
         int sp = 0;
-        for (int c = lane; c < a.nclusters; c += 64) stk[c] = (a.nclusters > 1) ? c : a.root;
-        sp = a.nclusters;
-        __threadfence_block();
-        __syncthreads();
         bool overflow = false;
-        while (sp > 0) {
+        while (sp > 0 || rnext < nroots) {
+            if (sp < 64 && rnext < nroots) {
+                const int add = min(64 - sp, nroots - rnext);
+                if (lane < add) stk[sp + lane] = (a.nclusters > 1) ? rnext + lane : a.root;
+                sp += add;
+                rnext += add;
+                __threadfence_block();
+                __syncthreads();
+            }
             const int k = min(sp, 64);
             const int base = sp - k;
             const int node = (lane < k) ? stk[base + lane] : -1;
@@ -264,6 +273,16 @@ __global__ void k_u32_to_size(int n, const uint32_t* __restrict__ in, size_t* __
     if (i < n) out[i] = in[i];
 }
 
+// The sum of the per-query counts in 64 bits (the 32-bit scan that makes the offsets would wrap
+// past 2^32 - 1 pairs): a grid-stride sum per thread, a wave sum, one atomic per wave.
+__global__ __launch_bounds__(256) void k_sum_counts(int n, const uint32_t* __restrict__ in,
+                                                    unsigned long long* __restrict__ out) {
+    long long s = 0;
+    for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) s += in[i];
+    for (int d = 32; d >= 1; d >>= 1) s += __shfl_xor(s, d);
+    if ((threadIdx.x & 63) == 0 && s) atomicAdd(out, (unsigned long long)s);
+}
+
 // h rounded up to float, so that the float bound stays one-sided (tree.cc:286-290).
 __device__ __forceinline__ float h_as_bound(double h) {
     const float f = (float)h;
@@ -408,7 +427,12 @@ static gz_status ngb_run(gz_ctx* ctx, const gz_tree* tree, const double* cx, con
     a.h = h;
     a.n_gas = rev ? n_gas : 0;
     GZ_TRY(fill_zero_u32(count, n_q + 1, st));
+    unsigned long long* total64 = reinterpret_cast<unsigned long long*>(misc + 4);
+    GZ_TRY(fill_zero_u32(reinterpret_cast<uint32_t*>(total64), 2, st));
     GZ_TRY(ngb_pass(a, false, rev, waves, st));
+    k_sum_counts<<<(unsigned)std::min<size_t>(1024, (n_q + 255) / 256), 256, 0, st>>>(
+        (int)n_q, count, total64);
+    GZ_HIP(hipGetLastError());
     // offsets: exclusive scan of the counts (count[n_q] = 0 gives the total at start[n_q])
     uint32_t tot32 = 0;
     {
@@ -423,11 +447,17 @@ static gz_status ngb_run(gz_ctx* ctx, const gz_tree* tree, const double* cx, con
                               hipMemcpyDeviceToHost, st));
     }
     int h_status = 0;
+    unsigned long long h_total = 0;
     GZ_HIP(hipMemcpyAsync(&h_status, a.status, sizeof h_status, hipMemcpyDeviceToHost, st));
+    GZ_HIP(hipMemcpyAsync(&h_total, total64, sizeof h_total, hipMemcpyDeviceToHost, st));
     GZ_HIP(hipStreamSynchronize(st));
     if (h_status & 1)
         return fail(GZ_ERR_INTERNAL, "%s: a search stack outgrew %d entries", who,
                     stack_cap);
+    if (h_total > 0xFFFFFFFFull)
+        return fail(GZ_ERR_UNSUPPORTED,
+                    "%s: %llu neighbour pairs exceed the 2^32 - 1 one call can return; split the "
+                    "queries", who, h_total);
     *total = tot32;
     if (!found || cap < (size_t)tot32 || tot32 == 0) return GZ_OK;
     // pass 2: the neighbours' slots, then sorted and mapped to particle indices

@@ -221,6 +221,9 @@ class Tree:
         """Tree::raise_hmax (tree.cc:293-301) for the particles idx; h indexed by particle."""
         ix = _dev_u32(idx, self.P.device)
         hh = _dev_f64(h, self.P.device)
+        if int(hh.shape[0]) < self.P.n:
+            raise ValueError(f"raise_hmax: h holds {int(hh.shape[0])} values, the kernel reads "
+                             f"it by particle index (n={self.P.n})")
         self.ctx.enter()
         L.check(L.load().gz_tree_raise_hmax(self.ctx.handle, self.handle, _ptr(ix),
                                             int(ix.shape[0]), _ptr(hh)), "gz_tree_raise_hmax")
@@ -561,6 +564,9 @@ def ngb_search_reverse(T: Tree, h, n_gas: int, centres, h_q, tau: float = 0.0):
     c = c.to(device=dev, dtype=torch.float64)
     cs = [c[:, d].contiguous() for d in range(3)]
     hh = _dev_f64(h, dev)
+    if int(hh.shape[0]) < int(n_gas):
+        raise ValueError(f"ngb_search_reverse: h holds {int(hh.shape[0])} values for "
+                         f"n_gas={int(n_gas)} gas particles")
     r = _dev_f64(h_q, dev)
     nq = int(r.shape[0])
     start = torch.zeros(nq + 1, dtype=torch.int64, device=dev)
@@ -646,3 +652,38 @@ def aold_from_accel(T: Tree, a_prev: torch.Tensor, targets, err_tol_force_acc: f
             "gz_aold_from_accel")
     T.ctx.synchronize()
     return out
+
+
+def scatter_accel(targets, ax, ay, az, a_grav: torch.Tensor, ctx: Context | None = None):
+    """a_grav[targets[k]] = (ax[k], ay[k], az[k]) (mfm.cc:828-831); a_grav is an (N, 3) float64
+    device tensor, written in place."""
+    ctx = ctx or Context.get(a_grav.device.index or 0)
+    tg = _dev_u32(targets, a_grav.device)
+    if a_grav.dtype != torch.float64 or not a_grav.is_contiguous() or a_grav.dim() != 2:
+        raise ValueError("scatter_accel: a_grav must be a contiguous (N, 3) float64 tensor")
+    a = [_dev_f64(v, a_grav.device) for v in (ax, ay, az)]
+    ctx.enter()
+    L.check(L.load().gz_scatter_accel(ctx.handle, _ptr(tg), int(tg.shape[0]), *(_ptr(v) for v in a),
+                                      _ptr(a_grav)), "gz_scatter_accel")
+    ctx.synchronize()
+
+
+def kick(active, a_grav: torch.Tensor, dt: torch.Tensor, pending: torch.Tensor, v,
+         a_extra: torch.Tensor | None = None, ctx: Context | None = None):
+    """The gravity part of the KDK kick (mfm.cc:3749-3767) for the particles in active: with
+    i = active[k], v[i] += a (pending[i] + dt[i] / 2) and pending[i] = dt[i] / 2, a = a_grav[i]
+    (+ a_extra[i]); a_grav and a_extra are (N, 3), dt, pending and the three velocity tensors are
+    by particle. pending and v (float64 device tensors) are updated in place."""
+    ctx = ctx or Context.get(a_grav.device.index or 0)
+    act = _dev_u32(active, a_grav.device)
+    for t in (a_grav, dt, pending, *v) + (() if a_extra is None else (a_extra,)):
+        if t.dtype != torch.float64 or not t.is_contiguous() or t.device != a_grav.device:
+            raise ValueError("kick: every array must be a contiguous float64 device tensor")
+    n = int(a_grav.shape[0])
+    if any(int(t.shape[0]) != n for t in (dt, pending, *v)):
+        raise ValueError("kick: dt, pending and v hold one value per particle, as a_grav does")
+    ctx.enter()
+    L.check(L.load().gz_kick(ctx.handle, _ptr(act), int(act.shape[0]), _ptr(a_grav),
+                             _ptr(a_extra), _ptr(dt), _ptr(pending), *(_ptr(t) for t in v)),
+            "gz_kick")
+    ctx.synchronize()

@@ -353,9 +353,11 @@ gz_status gz_aold_from_accel(gz_ctx* ctx, const double* a_prev, const uint32_t* 
 /* a_grav[targets[k]] = (ax[k], ay[k], az[k]) (mfm.cc:828-831); a_grav N x 3 interleaved. */
 gz_status gz_scatter_accel(gz_ctx* ctx, const uint32_t* targets, size_t n_t, const double* ax,
                            const double* ay, const double* az, double* a_grav);
-/* KDK kick of the active set (mfm.cc:3749-3767, gravity part): v += a * (close_dt + open_dt),
- * with a = a_grav[i] (+ a_extra[i] when non-NULL), both N x 3 interleaved; dt per active slot;
- * pending_half_kick[i] is consumed as close_dt and replaced by 0.5 * dt[k]. */
+/* KDK kick of the active set (mfm.cc:3749-3767, gravity part): for each i = active[k],
+ * v[i] += a * (close_dt + open_dt) with a = a_grav[i] (+ a_extra[i] when non-NULL), both N x 3
+ * interleaved; close_dt = pending_half_kick[i], open_dt = 0.5 * dt[i], and pending_half_kick[i]
+ * becomes open_dt. dt, pending_half_kick and v are indexed by particle, as the reference's dt_of,
+ * pending_half_kick and vx/vy/vz are (mfm.cc:3753-3764). */
 gz_status gz_kick(gz_ctx* ctx, const uint32_t* active, size_t n_active, const double* a_grav,
                   const double* a_extra, const double* dt, double* pending_half_kick, double* vx,
                   double* vy, double* vz);

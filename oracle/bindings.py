@@ -15,6 +15,26 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 ORACLE_SO = os.path.join(HERE, "liboracle.so")
 REF_SO = os.path.join(HERE, "_ref", "libgzref.so")
+REF_SO_V4 = os.path.join(HERE, "_ref", "libgzref_v4.so")
+
+
+def _cpu_has_avx512() -> bool:
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("flags"):
+                    fl = set(line.split(":", 1)[1].split())
+                    return {"avx512f", "avx512bw", "avx512dq", "avx512vl"} <= fl
+    except OSError:
+        pass
+    return False
+
+
+def reference_path() -> str:
+    """The reference build for this CPU: the AVX-512 one where the CPU has it, else AVX2."""
+    if os.path.exists(REF_SO_V4) and _cpu_has_avx512():
+        return REF_SO_V4
+    return REF_SO
 
 _dp = C.POINTER(C.c_double)
 _fp = C.POINTER(C.c_float)
@@ -183,6 +203,9 @@ class Oracle(_Lib):
         L.gzo_keys.argtypes = [C.POINTER(_GzoParticles), C.c_longlong, _u64p, _dp]
         L.gzo_build.restype = C.c_void_p
         L.gzo_build.argtypes = [C.POINTER(_GzoParticles), C.c_longlong]
+        L.gzo_build_shift.restype = C.c_void_p
+        L.gzo_build_shift.argtypes = [C.POINTER(_GzoParticles), C.c_longlong, C.c_double,
+                                      C.c_double, C.c_double]
         L.gzo_tree_skeys.argtypes = [C.c_void_p, _u64p]
         L.gzo_accel_grouped.argtypes = [C.c_void_p, C.POINTER(_GzoParticles), _up, C.c_size_t,
                                         C.c_double, C.c_double, C.c_int, _dp, _dp, _dp, _dp, _llp]
@@ -227,8 +250,11 @@ class Oracle(_Lib):
         self.lib.gzo_keys(C.byref(cp), seed, _ptr(keys, _u64p), _ptr(root, _dp))
         return keys, root
 
-    def build(self, P: HostParticles, seed: int = -1):
+    def build(self, P: HostParticles, seed: int = -1, shift=None):
+        """shift: SHMEM_TREE_SHIFT's fractions (tree.cc:400-407), or None."""
         cp = self._cp(P)
+        if shift is not None:
+            return self.lib.gzo_build_shift(C.byref(cp), seed, *(float(v) for v in shift))
         return self.lib.gzo_build(C.byref(cp), seed)
 
     def free(self, t):
@@ -397,7 +423,9 @@ class Reference(_Lib):
 
     prefix = "gzref_"
 
-    def __init__(self, path: str = REF_SO):
+    def __init__(self, path: str | None = None):
+        path = path or reference_path()
+        self.path = path
         super().__init__(path)
         L = self.lib
         L.gzref_particles_new.restype = C.c_void_p

@@ -86,7 +86,8 @@ static double splitmix_unit(splitmix* g) {
     return (double)((z ^ (z >> 31)) >> 11) * (1.0 / 9007199254740992.0);
 }
 
-static void root_box(const gzo_particles* P, long long seed, double* root) {
+/* shift: SHMEM_TREE_SHIFT's three fractions (tree.cc:400-407), or NULL */
+static void root_box(const gzo_particles* P, long long seed, const double* shift, double* root) {
     double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
     for (size_t i = 0; i < P->n; ++i) {
         if (!(P->m[i] > 0)) continue;
@@ -104,6 +105,10 @@ static void root_box(const gzo_particles* P, long long seed, double* root) {
         cz += side * (splitmix_unit(&g) - 0.5);
         side *= 2.0;
     }
+    if (shift) {
+        cx += shift[0] * side; cy += shift[1] * side; cz += shift[2] * side;
+        side *= 1.0 + 2.0 * (fabs(shift[0]) + fabs(shift[1]) + fabs(shift[2]));
+    }
     root[0] = cx; root[1] = cy; root[2] = cz; root[3] = side;
 }
 
@@ -116,8 +121,10 @@ static inline uint64_t key_of(double x, double y, double z, const double* root) 
     return gzo_morton(a, b, c);
 }
 
+static const double* g_shift = NULL; /* the shift of the build in progress (gzo_build_shift) */
+
 void gzo_keys(const gzo_particles* P, long long seed, uint64_t* keys, double* root) {
-    root_box(P, seed, root);
+    root_box(P, seed, g_shift, root);
     for (size_t i = 0; i < P->n; ++i)
         keys[i] = (P->m[i] > 0) ? key_of(P->x[i], P->y[i], P->z[i], root) : 0ull;
 }
@@ -255,6 +262,14 @@ static void thread_links(otree* T, int node, int successor) {
 
 void* gzo_build(const gzo_particles* P, long long seed) {
     return gzo_build_vel(P, seed, NULL, NULL, NULL);
+}
+
+void* gzo_build_shift(const gzo_particles* P, long long seed, double fx, double fy, double fz) {
+    const double sh[3] = {fx, fy, fz};
+    g_shift = sh;
+    void* t = gzo_build_vel(P, seed, NULL, NULL, NULL);
+    g_shift = NULL;
+    return t;
 }
 
 void* gzo_build_vel(const gzo_particles* P, long long seed, const double* vx, const double* vy,

@@ -64,6 +64,10 @@ void gzo_keys(const gzo_particles* P, long long randomize_seed, uint64_t* keys, 
 /* Full build (tree.cc:350-526). Node ids are DFS preorder in octant order, which is the
  * reference's own numbering whenever its build runs serially. Returns an opaque handle. */
 void* gzo_build(const gzo_particles* P, long long randomize_seed);
+/* The same with the root box displaced by SHMEM_TREE_SHIFT's fractions (tree.cc:400-407), which
+ * the reference reads from the environment. Not thread-safe (one build at a time). */
+void* gzo_build_shift(const gzo_particles* P, long long randomize_seed, double fx, double fy,
+                      double fz);
 void gzo_tree_free(void* t);
 /* out[0]=nnodes, out[1]=root, out[2]=n_live */
 void gzo_tree_sizes(void* t, long long* out);

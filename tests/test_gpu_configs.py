@@ -1,10 +1,12 @@
 """The named workloads at their full sizes.
 
 * Plummer, N = 65,536 (the reference's own test_tree case): the whole device path against the
-  oracle on every particle, both opening regimes;
-* Hernquist, N = 4,194,304: the FP64 grouped walk against the oracle on the full leaves of 4,096
-  probe particles (a leaf's targets form one group in the all-active walk, so walking just those
-  leaves reproduces the all-active groups), within 1e-6 relative;
+  oracle on every particle, both opening regimes, and the tree itself bit for bit;
+* Hernquist, N = 4,194,304: sorted keys, orderbuf, rank, the canonical topology and every node
+  moment of the device tree against the oracle's tree bit for bit; the FP64 grouped walk against
+  the oracle on the full leaves of 4,096 probe particles (a leaf's targets form one group in the
+  all-active walk, so walking just those leaves reproduces the all-active groups), within 1e-6
+  relative; and the walk's error against the FP64 direct sum on 65,536 probes;
 * clustered, N = 16,777,216: the FP32 group-relative walk against the FP64 walk, both judged by
   the FP64 direct sum on 65,536 probes.
 """
@@ -13,7 +15,10 @@ from __future__ import annotations
 import numpy as np
 import pytest
 
+import types
+
 from gizmo_shared_memory_amd import ics
+from tests.treecmp import BIT_FIELDS, assert_same_tree
 
 pytestmark = pytest.mark.gpu
 torch = pytest.importorskip("torch")
@@ -42,6 +47,15 @@ def _host(x, y, z, m, soft):
     return HostParticles(x, y, z, m, soft)
 
 
+def _assert_tree_equal(d, o, oracle_skeys):
+    """Device tree (downloaded) = oracle tree: keys, order, rank, topology, every moment."""
+    assert d["n_live"] == o.n_live and d["nnodes"] == o.nnodes
+    assert np.array_equal(d["skey"], oracle_skeys), "sorted keys differ"
+    assert np.array_equal(d["orderbuf"], o.orderbuf), "orderbuf differs"
+    assert np.array_equal(d["rank"], o.rank), "rank differs"
+    assert_same_tree(types.SimpleNamespace(**d), o, fields=BIT_FIELDS, what=("device", "oracle"))
+
+
 def test_plummer_64k_all_active(gz, oracle):
     x, y, z, m, soft = ics.plummer()
     print(f"[config] plummer n={x.shape[0]}", flush=True)
@@ -52,7 +66,7 @@ def test_plummer_64k_all_active(gz, oracle):
     try:
         o = oracle.tree(to, hp)
         d = T.download()
-        assert np.array_equal(d["orderbuf"], o.orderbuf)
+        _assert_tree_equal(d, o, oracle.skeys(to, o.n_live))
         tg = o.orderbuf
         gx, gy, gz_, c = gz.accel_grouped(T, P, tg, 0.7, 1.0, 0, counts=True)
         ox, oy, oz, oc = oracle.accel_grouped(to, hp, tg, 0.7, 1.0, 0, None, want_counts=True)
@@ -93,6 +107,10 @@ def test_hernquist_4m_probe_leaves(gz, oracle):
     hp = _host(x, y, z, m, soft)
     to = oracle.build(hp)
     try:
+        o = oracle.tree(to, hp)
+        _assert_tree_equal(d, o, oracle.skeys(to, o.n_live))
+        print(f"[config] tree bit-identical: {d['nnodes']} nodes, {o.n_live} live", flush=True)
+        del o
         ao = _np(aold)[keep]
         ob = oracle.accel_grouped(to, hp, targets, 0.7, 1.0, 0, None)
         orl = oracle.accel_grouped(to, hp, targets, 0.0, 1.0, 0, ao)
@@ -102,6 +120,14 @@ def test_hernquist_4m_probe_leaves(gz, oracle):
     er = _relerr(tuple(_np(v)[keep] for v in rel), orl)
     print(f"[config] max rel diff: bootstrap {eb.max():.3g}, relative {er.max():.3g}", flush=True)
     assert eb.max() < 1e-6 and er.max() < 1e-6
+    # the walk's own error: 65,536 random probes against the FP64 direct sum
+    pr = ics.probes(order.shape[0], 65536)
+    pt = tg.view(torch.int32)[torch.as_tensor(pr.astype(np.int64), device="cuda:0")]
+    ex = tuple(_np(v) for v in gz.accel_brute(P, pt.view(torch.uint32), 1.0))
+    e = _relerr(tuple(_np(v)[pr] for v in rel), ex)
+    print(f"[config] relative-criterion walk vs direct sum: median {np.median(e):.3g} "
+          f"p99 {np.percentile(e, 99):.3g} max {e.max():.3g}", flush=True)
+    assert np.percentile(e, 99) < 1e-2
 
 
 def test_clustered_16m_fp32_vs_fp64(gz):

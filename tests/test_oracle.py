@@ -157,6 +157,38 @@ def test_build_with_randomized_root_matches_reference(oracle, reference):
             oracle.free(to); reference.free(tr); reference.free_particles(rp)
 
 
+SHIFTS = ((0.13, -0.07, 0.21), (-0.3, 0.0, 0.05))
+
+
+@pytest.mark.ref
+def test_build_with_shifted_root_matches_reference(oracle, reference, monkeypatch):
+    """SHMEM_TREE_SHIFT (tree.cc:400-407), which the reference reads from the environment at
+    each build: the oracle's shift argument gives the same tree, alone and with a seeded root."""
+    P = _hp(tree_cases.plummer_small(5000, seed=6))
+    rp = reference.particles(P)
+    try:
+        for shift in SHIFTS:
+            for seed in (-1, 42):
+                monkeypatch.setenv("SHMEM_TREE_SHIFT", ",".join(repr(v) for v in shift))
+                tr = reference.build(rp, seed)
+                monkeypatch.delenv("SHMEM_TREE_SHIFT")
+                to = oracle.build(P, seed, shift=shift)
+                try:
+                    assert_same_tree(oracle.tree(to, P), reference.tree(tr, P.n))
+                    # the shift moves every cell wall: the unshifted tree differs
+                    t0 = oracle.build(P, seed)
+                    try:
+                        assert not np.array_equal(oracle.tree(t0, P).gcx[:64],
+                                                  oracle.tree(to, P).gcx[:64])
+                    finally:
+                        oracle.free(t0)
+                finally:
+                    oracle.free(to)
+                    reference.free(tr)
+    finally:
+        reference.free_particles(rp)
+
+
 def _walk_pair(oracle, reference, P, targets, theta, batch, aold):
     to = oracle.build(P)
     rp = reference.particles(P)
