@@ -128,6 +128,9 @@ _PROTOS = {
                                 C.c_void_p, C.c_void_p]),
     "gz_direct_sum": (C.c_int, [C.c_void_p, C.POINTER(gz_particles), C.c_void_p, C.c_size_t,
                                 C.c_double, C.c_void_p, C.c_void_p, C.c_void_p]),
+    "gz_direct_sum_ex": (C.c_int, [C.c_void_p, C.POINTER(gz_particles), C.c_void_p,
+                                   C.c_size_t, C.c_double, C.c_int, C.c_double, C.c_void_p,
+                                   C.c_void_p, C.c_void_p]),
     "gz_model_create": (C.c_int, [C.c_void_p, _fp, C.POINTER(C.c_void_p)]),
     "gz_model_free": (C.c_int, [C.c_void_p]),
     "gz_model_set_params": (C.c_int, [C.c_void_p, _fp]),

@@ -1075,7 +1075,20 @@ gz_status gz_direct_sum(gz_ctx* ctx, const gz_particles* p, const uint32_t* targ
     GZ_TRY(check_particles(p, "gz_direct_sum"));
     if (n_t > 0 && (!targets || !ax || !ay || !az))
         return fail(GZ_ERR_INVALID, "gz_direct_sum: null target or output array");
-    return gz::launch_direct(p, targets, n_t, G, ax, ay, az, ctx->stream);
+    return gz::launch_direct(p, targets, n_t, G, 1, 0.0, ax, ay, az, ctx->stream);
+}
+
+gz_status gz_direct_sum_ex(gz_ctx* ctx, const gz_particles* p, const uint32_t* targets,
+                           size_t n_t, double G, int fp64, double box, double* ax, double* ay,
+                           double* az) {
+    gz::clear_error();
+    GZ_TRY(check_ctx(ctx));
+    GZ_TRY(check_particles(p, "gz_direct_sum_ex"));
+    if (n_t > 0 && (!targets || !ax || !ay || !az))
+        return fail(GZ_ERR_INVALID, "gz_direct_sum_ex: null target or output array");
+    if (!(box >= 0.0) || std::isinf(box))
+        return fail(GZ_ERR_INVALID, "gz_direct_sum_ex: box must be finite and >= 0");
+    return gz::launch_direct(p, targets, n_t, G, fp64 ? 1 : 0, box, ax, ay, az, ctx->stream);
 }
 
 } // extern "C"

@@ -161,7 +161,8 @@ gz_status launch_walk(const WalkArgs& a, bool fp64, bool count, hipStream_t st);
 gz_status launch_walk_retry(const WalkArgs& a, int n_retry, bool fp64, bool count,
                             hipStream_t st);
 gz_status launch_direct(const gz_particles* p, const uint32_t* targets, size_t nt, double G,
-                        double* ax, double* ay, double* az, hipStream_t st);
+                        int fp64, double box, double* ax, double* ay, double* az,
+                        hipStream_t st);
 
 // ---- learned opening criterion (gz_learned.hip) ----------------------------------------------------
 struct GroupStats;

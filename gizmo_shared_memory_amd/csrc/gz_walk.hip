@@ -980,6 +980,15 @@ __global__ __launch_bounds__(64, TJ ? 3 : (FP64 ? 4 : 5)) void k_walk(WalkArgs a
     }
 }
 
+// Nearest periodic image of a separation in a cubic box of side L (grav_cuda.cu:104-109): one
+// wrap either way, for separations within (-1.5 L, 1.5 L).
+__device__ __forceinline__ double min_image(double d, double L) {
+    const double hb = 0.5 * L;
+    if (d > hb) return d - L;
+    if (d < -hb) return d + L;
+    return d;
+}
+
 // ---- direct sum (tree.cc:946-971; the reference's device form is grav_cuda.cu:65-141) -----------
 // One target per thread, sources staged through LDS 256 at a time in index order, so each target
 // sums its sources in exactly the reference's order with the reference's FP64 expressions.
@@ -988,8 +997,8 @@ constexpr int kDsThreads = 256;
 __global__ __launch_bounds__(kDsThreads) void k_direct_fp64(
     const double* __restrict__ x, const double* __restrict__ y, const double* __restrict__ z,
     const double* __restrict__ m, const double* __restrict__ soft, size_t n,
-    const uint32_t* __restrict__ targets, size_t nt, double G, double* __restrict__ ax,
-    double* __restrict__ ay, double* __restrict__ az) {
+    const uint32_t* __restrict__ targets, size_t nt, double G, double box,
+    double* __restrict__ ax, double* __restrict__ ay, double* __restrict__ az) {
     __shared__ double sx[kDsThreads], sy[kDsThreads], sz[kDsThreads], sm[kDsThreads],
         ss[kDsThreads];
     const size_t t = (size_t)blockIdx.x * kDsThreads + threadIdx.x;
@@ -1018,7 +1027,12 @@ __global__ __launch_bounds__(kDsThreads) void k_direct_fp64(
         if (live) {
             for (int j = 0; j < ntile; ++j) {
                 if (base + (size_t)j == (size_t)p) continue;
-                const double dx = sx[j] - px, dy = sy[j] - py, dz = sz[j] - pz;
+                double dx = sx[j] - px, dy = sy[j] - py, dz = sz[j] - pz;
+                if (box > 0) {
+                    dx = min_image(dx, box);
+                    dy = min_image(dy, box);
+                    dz = min_image(dz, box);
+                }
                 const double r = sqrt(dx * dx + dy * dy + dz * dz);
                 if (r <= 0) continue;
                 const double fac =
@@ -1035,6 +1049,86 @@ __global__ __launch_bounds__(kDsThreads) void k_direct_fp64(
         ax[t] = bad ? nan : G * ax_;
         ay[t] = bad ? nan : G * ay_;
         az[t] = bad ? nan : G * az_;
+    }
+}
+
+// The float-narrowed direct sum, the reference device form's default arithmetic
+// (grav_cuda.cu:65-141 with R = float): each separation is differenced in FP64 and then narrowed,
+// and r, the spline (at the larger of the two softenings, 1e-30 when both are zero), the mass
+// factor and the per-target sums are FP32. Sources are summed in index order, self and r^2 <= 0
+// skipped, the result widened and multiplied by G (the reference leaves G to its caller,
+// grav_cuda.cu:186-192).
+__device__ __forceinline__ float spline_force_over_r_f(float r, float h) {
+    const float h_inv = 1.0f / h;
+    const float h_inv3 = h_inv * h_inv * h_inv;
+    const float u = r * h_inv;
+    if (u >= 1.0f) return 1.0f / (r * r * r);
+    if (u < 0.5f) return h_inv3 * (10.666666666666667f + u * u * (32.0f * u - 38.4f));
+    return h_inv3 * (21.333333333333332f - 48.0f * u + 38.4f * u * u -
+                     10.666666666666667f * u * u * u - 0.06666666666666667f / (u * u * u));
+}
+
+__global__ __launch_bounds__(kDsThreads) void k_direct_f32(
+    const double* __restrict__ x, const double* __restrict__ y, const double* __restrict__ z,
+    const double* __restrict__ m, const double* __restrict__ soft, size_t n,
+    const uint32_t* __restrict__ targets, size_t nt, double G, double box,
+    double* __restrict__ ax, double* __restrict__ ay, double* __restrict__ az) {
+    __shared__ double sx[kDsThreads], sy[kDsThreads], sz[kDsThreads];
+    __shared__ float sm[kDsThreads], ss[kDsThreads];
+    const size_t t = (size_t)blockIdx.x * kDsThreads + threadIdx.x;
+    const bool live = t < nt;
+    uint32_t p = 0;
+    double px = 0, py = 0, pz = 0;
+    float ep = 1.0f;
+    bool bad = false;
+    if (live) {
+        p = targets[t];
+        if (p >= n) {
+            bad = true;
+            p = 0;
+        }
+        px = x[p]; py = y[p]; pz = z[p];
+        ep = soft ? (float)soft[p] : 0.0f;
+    }
+    float ax_ = 0.0f, ay_ = 0.0f, az_ = 0.0f;
+    for (size_t base = 0; base < n; base += kDsThreads) {
+        const size_t k = base + threadIdx.x;
+        if (k < n) {
+            sx[threadIdx.x] = x[k]; sy[threadIdx.x] = y[k]; sz[threadIdx.x] = z[k];
+            sm[threadIdx.x] = (float)m[k];
+            ss[threadIdx.x] = soft ? (float)soft[k] : 0.0f;
+        }
+        __syncthreads();
+        const int ntile = (int)min((size_t)kDsThreads, n - base);
+        if (live) {
+            for (int j = 0; j < ntile; ++j) {
+                if (base + (size_t)j == (size_t)p) continue;
+                double dxd = sx[j] - px, dyd = sy[j] - py, dzd = sz[j] - pz;
+                if (box > 0) {
+                    dxd = min_image(dxd, box);
+                    dyd = min_image(dyd, box);
+                    dzd = min_image(dzd, box);
+                }
+                const float dx = (float)dxd, dy = (float)dyd, dz = (float)dzd;
+                const float r2 = dx * dx + dy * dy + dz * dz;
+                if (r2 <= 0.0f) continue;
+                const float r = __builtin_sqrtf(r2);
+                const float es = ss[j];
+                float e = ep > es ? ep : es;
+                if (!(e > 0.0f)) e = 1e-30f;
+                const float fac = sm[j] * spline_force_over_r_f(r, e);
+                ax_ += fac * dx;
+                ay_ += fac * dy;
+                az_ += fac * dz;
+            }
+        }
+        __syncthreads();
+    }
+    if (live) {
+        const double nan = __builtin_nan("");
+        ax[t] = bad ? nan : G * (double)ax_;
+        ay[t] = bad ? nan : G * (double)ay_;
+        az[t] = bad ? nan : G * (double)az_;
     }
 }
 
@@ -1152,10 +1246,16 @@ gz_status launch_group_pack(int ngroups, size_t nt, int group_size, const int* g
 }
 
 gz_status launch_direct(const gz_particles* p, const uint32_t* targets, size_t nt, double G,
-                        double* ax, double* ay, double* az, hipStream_t st) {
+                        int fp64, double box, double* ax, double* ay, double* az,
+                        hipStream_t st) {
     if (nt == 0) return GZ_OK;
-    k_direct_fp64<<<(unsigned)((nt + kDsThreads - 1) / kDsThreads), kDsThreads, 0, st>>>(
-        p->x, p->y, p->z, p->m, p->soft, p->n, targets, nt, G, ax, ay, az);
+    const unsigned grid = (unsigned)((nt + kDsThreads - 1) / kDsThreads);
+    if (fp64)
+        k_direct_fp64<<<grid, kDsThreads, 0, st>>>(p->x, p->y, p->z, p->m, p->soft, p->n,
+                                                   targets, nt, G, box, ax, ay, az);
+    else
+        k_direct_f32<<<grid, kDsThreads, 0, st>>>(p->x, p->y, p->z, p->m, p->soft, p->n,
+                                                  targets, nt, G, box, ax, ay, az);
     GZ_HIP(hipGetLastError());
     return GZ_OK;
 }

@@ -625,6 +625,23 @@ def accel_brute(P: Particles, targets, G: float = 1.0, ctx: Context | None = Non
     return ax, ay, az
 
 
+def direct_sum(P: Particles, targets, G: float = 1.0, fp64: bool = False, box: float = 0.0,
+               ctx: Context | None = None):
+    """shmem_cuda_accel_bruteforce (grav_cuda.cu:65-247): the device direct sum in the reference's
+    default float-narrowed arithmetic (fp64=False: separations differenced in FP64, the rest in
+    FP32) or its SHMEM_CUDA_FP64 form; box > 0 = nearest periodic image. Includes G."""
+    ctx = ctx or Context.get(P.device.index or 0)
+    tg = _dev_u32(targets, P.device)
+    nt = int(tg.shape[0])
+    ax, ay, az = _outputs(nt, P.device)
+    ctx.enter()
+    L.check(L.load().gz_direct_sum_ex(ctx.handle, P.cstruct(), _ptr(tg), nt, float(G),
+                                      1 if fp64 else 0, float(box), _ptr(ax), _ptr(ay),
+                                      _ptr(az)), "gz_direct_sum_ex")
+    ctx.synchronize()
+    return ax, ay, az
+
+
 def order_targets(T: Tree, active=None) -> torch.Tensor:
     """Active particles in the tree's Morton order (mfm.cc:698-718). None = all active."""
     n_out = C.c_size_t(0)

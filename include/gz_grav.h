@@ -292,6 +292,16 @@ gz_status gz_tree_potential(gz_ctx* ctx, const gz_tree* tree, const gz_particles
  * device counterpart is shmem_cuda_accel_bruteforce (grav_cuda.cu:186-192), which omits G. */
 gz_status gz_direct_sum(gz_ctx* ctx, const gz_particles* p, const uint32_t* targets, size_t n_t,
                         double G, double* ax, double* ay, double* az);
+/* The reference's device direct sum, shmem_cuda_accel_bruteforce (grav_cuda.cu:65-247), in both
+ * of its arithmetics. fp64 = 0 is its default float-narrowed form (R = float): each separation is
+ * differenced in FP64 and narrowed, r, the spline at max(eps_target, eps_source) (1e-30 when both
+ * are zero) and the per-target sums are FP32. fp64 != 0 is its SHMEM_CUDA_FP64 form, here the
+ * same arithmetic as gz_direct_sum. box > 0 takes the nearest periodic image of each separation
+ * in a cube of that side (grav_cuda.cu:104-109); 0 = open boundaries. Unlike the reference, the
+ * result includes G, targets and particles are device arrays, and errors are returned. */
+gz_status gz_direct_sum_ex(gz_ctx* ctx, const gz_particles* p, const uint32_t* targets,
+                           size_t n_t, double G, int fp64, double box, double* ax, double* ay,
+                           double* az);
 
 /* ---- learned opening criterion --------------------------------------------------------------
  * No counterpart in the reference, whose opening parameter is one global tolerance

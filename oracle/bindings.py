@@ -213,6 +213,10 @@ class Oracle(_Lib):
                                 C.c_double, C.c_double, _dp, _dp, _dp, _dp]
         L.gzo_accel_brute.argtypes = [C.POINTER(_GzoParticles), _up, C.c_size_t, C.c_double,
                                       _dp, _dp, _dp]
+        L.gzo_direct_sum_f32.argtypes = [C.POINTER(_GzoParticles), _up, C.c_size_t,
+                                         C.c_double, C.c_double, _dp, _dp, _dp]
+        L.gzo_direct_sum_box.argtypes = [C.POINTER(_GzoParticles), _up, C.c_size_t,
+                                         C.c_double, C.c_double, _dp, _dp, _dp]
         L.gzo_accel_grouped_tj.argtypes = [C.c_void_p, C.POINTER(_GzoParticles), _up,
                                            C.c_size_t, C.c_double, C.c_double, C.c_int, _dp,
                                            _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _llp]
@@ -415,6 +419,17 @@ class Oracle(_Lib):
         cp = self._cp(P)
         self.lib.gzo_accel_brute(C.byref(cp), _ptr(tg, _up), nt, G, _ptr(ax, _dp),
                                  _ptr(ay, _dp), _ptr(az, _dp))
+        return ax, ay, az
+
+    def direct_sum(self, P: HostParticles, targets, G=1.0, fp64=False, box=0.0):
+        """grav_cuda.cu's direct sum: float-narrowed (fp64=False) or FP64, optional box."""
+        tg = np.ascontiguousarray(targets, dtype=np.uint32)
+        nt = tg.shape[0]
+        ax, ay, az = (np.zeros(nt) for _ in range(3))
+        cp = self._cp(P)
+        fn = self.lib.gzo_direct_sum_box if fp64 else self.lib.gzo_direct_sum_f32
+        fn(C.byref(cp), _ptr(tg, _up), nt, G, float(box), _ptr(ax, _dp), _ptr(ay, _dp),
+           _ptr(az, _dp))
         return ax, ay, az
 
 

@@ -962,3 +962,84 @@ void gzo_accel_brute(const gzo_particles* P, const uint32_t* targets, size_t nt,
         ax[t] = G * sx; ay[t] = G * sy; az[t] = G * sz;
     }
 }
+
+/* The reference's device direct sum, shmem_cuda_accel_bruteforce (grav_cuda.cu:65-141), in its
+ * default float arithmetic (R = float): the separation is differenced in double and narrowed;
+ * r, the spline at the larger softening (1e-30 when both are zero), the mass factor and the sums
+ * are float, sources in index order, self and r^2 <= 0 skipped. box > 0: nearest periodic image
+ * (grav_cuda.cu:104-109). The result is multiplied by G in double (the reference leaves G to its
+ * caller). */
+static float spline_force_over_r_f(float r, float h) {
+    const float h_inv = 1.0f / h;
+    const float h_inv3 = h_inv * h_inv * h_inv;
+    const float u = r * h_inv;
+    if (u >= 1.0f) return 1.0f / (r * r * r);
+    if (u < 0.5f) return h_inv3 * (10.666666666666667f + u * u * (32.0f * u - 38.4f));
+    return h_inv3 * (21.333333333333332f - 48.0f * u + 38.4f * u * u -
+                     10.666666666666667f * u * u * u - 0.06666666666666667f / (u * u * u));
+}
+
+static double min_image(double d, double L) {
+    const double hb = 0.5 * L;
+    if (d > hb) return d - L;
+    if (d < -hb) return d + L;
+    return d;
+}
+
+void gzo_direct_sum_f32(const gzo_particles* P, const uint32_t* targets, size_t nt, double G,
+                        double box, double* ax, double* ay, double* az) {
+#pragma omp parallel for schedule(static)
+    for (long long t = 0; t < (long long)nt; ++t) {
+        const uint32_t p = targets[t];
+        const double px = P->x[p], py = P->y[p], pz = P->z[p];
+        const float ep = (float)soft_of(P, p);
+        float sx = 0.0f, sy = 0.0f, sz = 0.0f;
+        for (size_t q = 0; q < P->n; ++q) {
+            if (q == p) continue;
+            double dxd = P->x[q] - px, dyd = P->y[q] - py, dzd = P->z[q] - pz;
+            if (box > 0) {
+                dxd = min_image(dxd, box);
+                dyd = min_image(dyd, box);
+                dzd = min_image(dzd, box);
+            }
+            const float dx = (float)dxd, dy = (float)dyd, dz = (float)dzd;
+            const float r2 = dx * dx + dy * dy + dz * dz;
+            if (r2 <= 0.0f) continue;
+            const float r = sqrtf(r2);
+            const float es = (float)soft_of(P, q);
+            float e = ep > es ? ep : es;
+            if (!(e > 0.0f)) e = 1e-30f;
+            const float fac = (float)P->m[q] * spline_force_over_r_f(r, e);
+            sx += fac * dx; sy += fac * dy; sz += fac * dz;
+        }
+        ax[t] = G * (double)sx; ay[t] = G * (double)sy; az[t] = G * (double)sz;
+    }
+}
+
+/* accel_brute's FP64 pair rule (tree.cc:946-971) with the nearest periodic image when box > 0;
+ * box = 0 is gzo_accel_brute. */
+void gzo_direct_sum_box(const gzo_particles* P, const uint32_t* targets, size_t nt, double G,
+                        double box, double* ax, double* ay, double* az) {
+#pragma omp parallel for schedule(static)
+    for (long long t = 0; t < (long long)nt; ++t) {
+        const uint32_t p = targets[t];
+        const double px = P->x[p], py = P->y[p], pz = P->z[p];
+        const double ep = soft_of(P, p);
+        double sx = 0, sy = 0, sz = 0;
+        for (size_t q = 0; q < P->n; ++q) {
+            if (q == p) continue;
+            double dx = P->x[q] - px, dy = P->y[q] - py, dz = P->z[q] - pz;
+            if (box > 0) {
+                dx = min_image(dx, box);
+                dy = min_image(dy, box);
+                dz = min_image(dz, box);
+            }
+            const double r = sqrt(dx * dx + dy * dy + dz * dz);
+            if (r <= 0) continue;
+            const double fac =
+                P->m[q] * gzo_spline_force_over_r(r, dmax(dmax(ep, soft_of(P, q)), 1e-300));
+            sx += dx * fac; sy += dy * fac; sz += dz * fac;
+        }
+        ax[t] = G * sx; ay[t] = G * sy; az[t] = G * sz;
+    }
+}

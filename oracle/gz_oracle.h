@@ -28,6 +28,10 @@
  *                         the direct sum of the spline potential
  *   gzo_accel_brute       bit for bit against the reference accel_brute, Newton's third law, and
  *                         a two-body closed form
+ *   gzo_direct_sum_f32    within float rounding of gzo_accel_brute, a two-body closed form in
+ *                         float, and periodic images equal to explicitly shifted copies
+ *   gzo_direct_sum_box    box = 0 bit for bit against gzo_accel_brute; a periodic box equals the
+ *                         open sum over the nearest images placed explicitly
  *   gzo_build_vel, gzo_kick_nodes, gzo_refresh_walk_nodes, gzo_raise_soft
  *                         bit for bit against the reference (one thread) through vmax, vcom, the
  *                         incremental moments and the refreshed walk nodes; the refreshed COM
@@ -127,6 +131,12 @@ void gzo_potential(void* t, const gzo_particles* P, const uint32_t* targets, siz
                    double theta, double G, const double* aold, double* phi);
 void gzo_accel_brute(const gzo_particles* P, const uint32_t* targets, size_t nt, double G,
                      double* ax, double* ay, double* az);
+/* The reference's device direct sum (grav_cuda.cu:65-141) in its default float arithmetic, and
+ * accel_brute's FP64 form, both with an optional periodic box (box = 0: open). */
+void gzo_direct_sum_f32(const gzo_particles* P, const uint32_t* targets, size_t nt, double G,
+                        double box, double* ax, double* ay, double* az);
+void gzo_direct_sum_box(const gzo_particles* P, const uint32_t* targets, size_t nt, double G,
+                        double box, double* ax, double* ay, double* az);
 
 #ifdef __cplusplus
 }
