@@ -24,6 +24,11 @@ import os
 import sys
 import time
 
+# The cores this process may run on, read before anything loads an OpenMP runtime: with
+# OMP_PROC_BIND set, libgomp binds the initial thread to one place when it starts, after which
+# the affinity mask shows that single core.
+_AFFINITY_CPUS = len(os.sched_getaffinity(0))
+
 # The reference's run recipe (shmem/README.md:49-57) binds OpenMP threads one per core, spread
 # over the sockets. OpenMP reads these once, when its runtime is first loaded, so they are set
 # before anything that links it is imported; a value already in the environment is kept.
@@ -102,7 +107,7 @@ def host_cores():
     """Threads for the CPU baseline and a description of the host. The count is the cores this
     process may run on (its affinity mask), capped by OMP_NUM_THREADS when that is set: a GPU
     box grants each GPU a share of its cores and says how many there."""
-    avail = len(os.sched_getaffinity(0))
+    avail = _AFFINITY_CPUS
     env = os.environ.get("OMP_NUM_THREADS", "")
     threads = min(avail, int(env)) if env.isdigit() and int(env) > 0 else avail
     model = "unknown"

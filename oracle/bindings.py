@@ -79,6 +79,15 @@ class _GzoParticles(C.Structure):
                 ("soft", _dp)]
 
 
+class _GzoTreeView(C.Structure):
+    _fields_ = [("nnodes", C.c_size_t), ("n_live", C.c_size_t), ("root", C.c_int),
+                ("first", _ip), ("next", _ip), ("parent", _ip), ("plo", _ip), ("phi", _ip),
+                ("mass", _dp), ("cx", _dp), ("cy", _dp), ("cz", _dp),
+                ("size", _dp), ("gcx", _dp), ("gcy", _dp), ("gcz", _dp),
+                ("wn_soft", _fp), ("vmax", _fp), ("xpad", _fp), ("tau", C.c_double),
+                ("orderbuf", _up), ("leaf_of", _ip)]
+
+
 @dataclass
 class TreeArrays:
     """Node and particle arrays of one built tree, as numpy arrays."""
@@ -213,6 +222,9 @@ class Oracle(_Lib):
                                 C.c_double, C.c_double, _dp, _dp, _dp, _dp]
         L.gzo_accel_brute.argtypes = [C.POINTER(_GzoParticles), _up, C.c_size_t, C.c_double,
                                       _dp, _dp, _dp]
+        L.gzo_audit_tree.restype = C.c_longlong
+        L.gzo_audit_tree.argtypes = [C.POINTER(_GzoTreeView), C.POINTER(_GzoParticles), _dp,
+                                     _dp, _dp, C.c_int, C.c_char_p, C.c_size_t]
         L.gzo_direct_sum_f32.argtypes = [C.POINTER(_GzoParticles), _up, C.c_size_t,
                                          C.c_double, C.c_double, _dp, _dp, _dp]
         L.gzo_direct_sum_box.argtypes = [C.POINTER(_GzoParticles), _up, C.c_size_t,
@@ -420,6 +432,32 @@ class Oracle(_Lib):
         self.lib.gzo_accel_brute(C.byref(cp), _ptr(tg, _up), nt, G, _ptr(ax, _dp),
                                  _ptr(ay, _dp), _ptr(az, _dp))
         return ax, ay, az
+
+    def audit_tree(self, T, P: HostParticles, vel=None, vmax=None, tau=0.0, check_com=True):
+        """audit_tree (tree.cc:973-1119) over tree arrays T (a TreeArrays, or the dict a device
+        tree downloads to): (violations, first message)."""
+        g = (lambda k: T[k]) if isinstance(T, dict) else (lambda k: getattr(T, k))
+        keep = {}
+
+        def arr(k, dt, ptype):
+            a = np.ascontiguousarray(g(k), dtype=dt)
+            keep[k] = a
+            return _ptr(a, ptype)
+
+        vm = None if vmax is None else np.ascontiguousarray(vmax, dtype=np.float32)
+        view = _GzoTreeView(
+            int(g("nnodes")), int(g("n_live")), int(g("root")),
+            *(arr(k, np.int32, _ip) for k in ("first", "next", "parent", "plo", "phi")),
+            *(arr(k, np.float64, _dp) for k in ("mass", "cx", "cy", "cz", "size", "gcx", "gcy",
+                                                 "gcz")),
+            arr("wn_soft", np.float32, _fp), _ptr(vm, _fp), None, float(tau),
+            arr("orderbuf", np.uint32, _up), arr("leaf_of", np.int32, _ip))
+        v = (None, None, None) if vel is None else tuple(_f64(a) for a in vel)
+        buf = C.create_string_buffer(512)
+        cp = self._cp(P)
+        bad = self.lib.gzo_audit_tree(C.byref(view), C.byref(cp), *(_ptr(a, _dp) for a in v),
+                                      1 if check_com else 0, buf, 512)
+        return int(bad), buf.value.decode()
 
     def direct_sum(self, P: HostParticles, targets, G=1.0, fp64=False, box=0.0):
         """grav_cuda.cu's direct sum: float-narrowed (fp64=False) or FP64, optional box."""

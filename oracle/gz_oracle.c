@@ -10,6 +10,7 @@
 
 #include <float.h>
 #include <math.h>
+#include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
 
@@ -1042,4 +1043,155 @@ void gzo_direct_sum_box(const gzo_particles* P, const uint32_t* targets, size_t 
         }
         ax[t] = G * sx; ay[t] = G * sy; az[t] = G * sz;
     }
+}
+
+/* audit_tree (tree.cc:973-1119), gravity-only arguments (no h, no box, no lazy drift), over a
+ * tree given as plain arrays, so that a device tree downloaded to the host is audited by the same
+ * code as the oracle's own. Counts violations and describes the first in msg:
+ *   1. threading: from the root, every child chain first[X] .. next[X] ends, no node or particle
+ *      is reached twice, parent[] and leaf_of[] agree with the threading, and every particle with
+ *      m > 0 is reached;
+ *   2. every node's mass equals the sum of its particles' masses (1e-10 relative) and, when
+ *      check_com, its centre of mass lies within 1e-6 of its reach (size + 2 (vmax tau + xpad))
+ *      of the particles' centre of mass;
+ *   3. every ancestor of a particle covers it: softening bound >= its softening, speed bound >=
+ *      |v| (when vmax and v are given), and its position within the search node's box (float
+ *      geometric centre, half side with the pack's safety epsilon, tree.cc:496-506) widened by
+ *      vmax tau + xpad. */
+long long gzo_audit_tree(const gzo_tree_view* T, const gzo_particles* P, const double* vx,
+                         const double* vy, const double* vz, int check_com, char* msg,
+                         size_t msglen) {
+    long long bad = 0;
+#define AUDIT_FAIL(...)                                                                          \
+    do {                                                                                         \
+        if (bad++ == 0 && msg && msglen) snprintf(msg, msglen, __VA_ARGS__);                    \
+    } while (0)
+    const size_t N = P->n, nn = T->nnodes;
+    if (msg && msglen) msg[0] = 0;
+    if (nn == 0) return 0;
+    if (T->root < 0 || (size_t)T->root >= nn) {
+        AUDIT_FAIL("root %d out of range %zu", T->root, nn);
+        return bad;
+    }
+    unsigned char* seen_node = calloc(nn, 1);
+    unsigned char* seen_part = calloc(N ? N : 1, 1);
+    int* stack = malloc((nn + 1) * sizeof(int));
+    size_t sp = 0, steps = 0;
+    if (T->parent[T->root] != -1) AUDIT_FAIL("root %d has parent %d", T->root, T->parent[T->root]);
+    stack[sp++] = T->root;
+    while (sp) {
+        const int X = stack[--sp];
+        if (X < 0 || (size_t)X >= nn) {
+            AUDIT_FAIL("node index %d out of range %zu", X, nn);
+            continue;
+        }
+        if (seen_node[X]++) {
+            AUDIT_FAIL("node %d reached twice", X);
+            continue;
+        }
+        if (T->first[X] < 0) {
+            for (int slot = T->plo[X]; slot < T->phi[X]; ++slot) {
+                if (slot < 0 || (size_t)slot >= T->n_live) {
+                    AUDIT_FAIL("leaf %d slot %d outside [0, %zu)", X, slot, T->n_live);
+                    break;
+                }
+                const uint32_t q = T->orderbuf[slot];
+                if (q >= N) {
+                    AUDIT_FAIL("leaf %d holds index %u", X, q);
+                    continue;
+                }
+                if (seen_part[q]++) AUDIT_FAIL("particle %u reached twice, leaf %d", q, X);
+                if (T->leaf_of[q] != X)
+                    AUDIT_FAIL("leaf_of[%u]=%d, but threading reaches it in leaf %d", q,
+                               T->leaf_of[q], X);
+            }
+            continue;
+        }
+        for (int c = T->first[X]; c != T->next[X]; c = T->next[c]) {
+            if (c < 0 || (size_t)c >= nn || ++steps > 2 * nn) {
+                AUDIT_FAIL("child chain of node %d never reaches next[X]=%d", X, T->next[X]);
+                break;
+            }
+            if (T->parent[c] != X) AUDIT_FAIL("parent[%d]=%d, threading says %d", c, T->parent[c], X);
+            if (sp < nn + 1) stack[sp++] = c;
+        }
+    }
+    for (size_t q = 0; q < N; ++q)
+        if (!seen_part[q] && P->m[q] > 0) {
+            AUDIT_FAIL("particle %zu unreachable from the root", q);
+            break;
+        }
+    free(stack);
+    free(seen_node);
+    free(seen_part);
+    if (bad) return bad;
+
+    const double tau = T->tau;
+    double* msum = calloc(nn, sizeof(double));
+    double* mdx = calloc(3 * nn, sizeof(double));
+    for (size_t q = 0; q < N; ++q) {
+        if (T->leaf_of[q] < 0 || P->m[q] <= 0) continue;
+        for (int no = T->leaf_of[q]; no >= 0; no = T->parent[no]) {
+            msum[no] += P->m[q];
+            mdx[3 * no + 0] += (P->x[q] - T->cx[no]) * P->m[q];
+            mdx[3 * no + 1] += (P->y[q] - T->cy[no]) * P->m[q];
+            mdx[3 * no + 2] += (P->z[q] - T->cz[no]) * P->m[q];
+        }
+    }
+    for (size_t no = 0; no < nn; ++no) {
+        const double M = T->mass[no];
+        const double tol = 1e-10 * fmax(msum[no], fabs(M)) + 1e-300;
+        if (fabs(msum[no] - M) > tol) {
+            AUDIT_FAIL("node %zu mass %.17g, particles sum to %.17g", no, M, msum[no]);
+            break;
+        }
+        if (check_com && msum[no] > 0) {
+            const double ox = mdx[3 * no] / msum[no], oy = mdx[3 * no + 1] / msum[no],
+                         oz = mdx[3 * no + 2] / msum[no];
+            const double off = sqrt(ox * ox + oy * oy + oz * oz);
+            const double vm = T->vmax ? (double)T->vmax[no] : 0.0;
+            const double xp = T->xpad ? (double)T->xpad[no] : 0.0;
+            const double reach = T->size[no] + 2.0 * (vm * tau + xp);
+            if (off > 1e-6 * reach) {
+                AUDIT_FAIL("node %zu tracked COM is %.9g off the particles' (node reach %.9g)", no,
+                           off, reach);
+                break;
+            }
+        }
+    }
+    free(msum);
+    free(mdx);
+
+    const double snode_eps = 1e-6 * T->size[T->root];
+    for (size_t q = 0; q < N && bad < 1000; ++q) {
+        if (T->leaf_of[q] < 0 || P->m[q] <= 0) continue;
+        const double sq = P->soft ? P->soft[q] : 0.0;
+        const double v = (vx && vy && vz) ? sqrt(vx[q] * vx[q] + vy[q] * vy[q] + vz[q] * vz[q]) : 0.0;
+        for (int no = T->leaf_of[q]; no >= 0; no = T->parent[no]) {
+            if ((double)T->wn_soft[no] * (1.0 + 1e-6) < sq)
+                AUDIT_FAIL("node %d soft %.9g below soft=%.9g of particle %zu", no,
+                           (double)T->wn_soft[no], sq, q);
+            const double vm = T->vmax ? (double)T->vmax[no] : 0.0;
+            if (T->vmax && vx && vm * (1.0 + 1e-6) < v)
+                AUDIT_FAIL("vmax[%d]=%.9g below |v|=%.9g of particle %zu", no, vm, v, q);
+            /* the search node's box (tree.cc:496-506): float centre, half side widened by 1e-6
+             * of the root side plus 4 float ulps of the centre's largest coordinate */
+            const float fcx = (float)T->gcx[no], fcy = (float)T->gcy[no], fcz = (float)T->gcz[no];
+            const double cmax = fmax(fabs(T->gcx[no]), fmax(fabs(T->gcy[no]), fabs(T->gcz[no])));
+            const double half =
+                (double)(float)(0.5 * T->size[no] + snode_eps + 4.0 * FLT_EPSILON * cmax);
+            const double xp = T->xpad ? (double)T->xpad[no] : 0.0;
+            const double reach = half + vm * tau + xp + 1e-6 * half;
+            const double d[3] = {P->x[q] - (double)fcx, P->y[q] - (double)fcy,
+                                 P->z[q] - (double)fcz};
+            for (int a = 0; a < 3; ++a)
+                if (fabs(d[a]) > reach) {
+                    AUDIT_FAIL("particle %zu outside node %d box: offset %.9g > reach %.9g", q, no,
+                               fabs(d[a]), reach);
+                    break;
+                }
+        }
+    }
+#undef AUDIT_FAIL
+    return bad;
 }

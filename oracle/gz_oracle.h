@@ -28,6 +28,9 @@
  *                         the direct sum of the spline potential
  *   gzo_accel_brute       bit for bit against the reference accel_brute, Newton's third law, and
  *                         a two-body closed form
+ *   gzo_audit_tree        zero violations wherever the reference's audit_tree finds none (the same
+ *                         particle sets), and each invariant caught when a mistake is planted
+ *                         in a copy of the tree arrays on the CPU
  *   gzo_direct_sum_f32    within float rounding of gzo_accel_brute, a two-body closed form in
  *                         float, and periodic images equal to explicitly shifted copies
  *   gzo_direct_sum_box    box = 0 bit for bit against gzo_accel_brute; a periodic box equals the
@@ -137,6 +140,26 @@ void gzo_direct_sum_f32(const gzo_particles* P, const uint32_t* targets, size_t 
                         double box, double* ax, double* ay, double* az);
 void gzo_direct_sum_box(const gzo_particles* P, const uint32_t* targets, size_t nt, double G,
                         double box, double* ax, double* ay, double* az);
+
+/* A tree as plain host arrays (node ids any numbering, children threaded by first/next as in
+ * tree.cc:340-348), for gzo_audit_tree. vmax and xpad may be NULL (zero). */
+typedef struct {
+    size_t nnodes, n_live;
+    int root;
+    const int *first, *next, *parent, *plo, *phi;
+    const double *mass, *cx, *cy, *cz; /* walk-node mass and centre of mass at tau       */
+    const double *size, *gcx, *gcy, *gcz; /* geometric side and centre                 */
+    const float* wn_soft;                 /* node softening bound                        */
+    const float *vmax, *xpad;
+    double tau;                           /* time since the build                        */
+    const uint32_t* orderbuf;
+    const int* leaf_of;
+} gzo_tree_view;
+/* audit_tree (tree.cc:973-1119) without h, box or lazy drift: the number of violated invariants,
+ * the first described in msg. */
+long long gzo_audit_tree(const gzo_tree_view* T, const gzo_particles* P, const double* vx,
+                         const double* vy, const double* vz, int check_com, char* msg,
+                         size_t msglen);
 
 #ifdef __cplusplus
 }
