@@ -160,6 +160,8 @@ _PROTOS = {
     "gz_graph_free": (C.c_int, [C.c_void_p]),
     "gz_order_targets": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t, C.c_void_p,
                                    C.POINTER(C.c_size_t)]),
+    "gz_order_targets_noalloc": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t,
+                                           C.c_void_p, C.POINTER(C.c_size_t)]),
     "gz_aold_from_accel": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t,
                                      C.c_double, C.c_double, C.c_void_p]),
     "gz_scatter_accel": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t, C.c_void_p, C.c_void_p,

@@ -330,6 +330,7 @@ gz_status gz_ctx_create(int device, void* stream, gz_ctx** out) {
     if (device < 0 || device >= ndev)
         return fail(GZ_ERR_INVALID, "gz_ctx_create: device %d of %d", device, ndev);
     GZ_HIP(hipSetDevice(device));
+    GZ_TRY(gz::resolve_noalloc_scratch(device));
     gz_ctx* c = new (std::nothrow) gz_ctx();
     if (!c) return fail(GZ_ERR_NOMEM, "gz_ctx_create: out of host memory");
     c->device = device;

@@ -4,10 +4,45 @@
 // kick (mfm.cc:3749-3767).
 #include <algorithm>
 #include <cmath>
+#include <mutex>
 
 #include "gz_internal.h"
 
 namespace gz {
+
+// ---- fixed scratch for gz_order_targets_noalloc ------------------------------------------------
+// The flags, their exclusive scan and the scan's block partials for up to kNoAllocSlots live
+// particles, in a file-scope device array: every context on a GPU shares that GPU's copy, which
+// keeps the last call's contents. Host code never passes the array's name to a kernel (on the host
+// it is only a stand-in for the symbol): it uses the address hipGetSymbolAddress returns for the
+// current device, resolved once per device when a context is created there.
+constexpr size_t kNoAllocSlots = size_t(1) << 23;
+constexpr size_t kNoAllocFlagBytes = ((kNoAllocSlots + 64) * sizeof(uint32_t) + 255) & ~size_t(255);
+constexpr size_t kNoAllocScanBytes = size_t(1) << 20;
+constexpr size_t kNoAllocBytes = 2 * kNoAllocFlagBytes + kNoAllocScanBytes;
+constexpr int kNoAllocMaxDevices = 64;
+
+static __device__ __align__(256) uint8_t scratch[kNoAllocBytes];
+
+static std::mutex g_scratch_mu;
+static uint8_t* g_scratch_ptr[kNoAllocMaxDevices] = {};
+
+gz_status resolve_noalloc_scratch(int device) {
+    if (device < 0 || device >= kNoAllocMaxDevices) return GZ_OK; // the entry point refuses it
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (g_scratch_ptr[device]) return GZ_OK;
+    void* p = nullptr;
+    GZ_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    if (!p) return fail(GZ_ERR_HIP, "hipGetSymbolAddress(scratch) returned null");
+    g_scratch_ptr[device] = static_cast<uint8_t*>(p);
+    return GZ_OK;
+}
+
+static uint8_t* noalloc_scratch(int device) {
+    if (device < 0 || device >= kNoAllocMaxDevices) return nullptr;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    return g_scratch_ptr[device];
+}
 
 constexpr int kT = 256;
 static inline unsigned blocks(size_t n) { return (unsigned)((n + kT - 1) / kT); }
@@ -86,30 +121,14 @@ using gz::fail;
 
 extern "C" {
 
-gz_status gz_order_targets(gz_ctx* ctx, const gz_tree* tree, const uint32_t* active,
-                           size_t n_active, uint32_t* out_targets, size_t* n_out) {
-    gz::clear_error();
-    if (!ctx || !tree || !n_out) return fail(GZ_ERR_INVALID, "gz_order_targets: null argument");
-    GZ_HIP(hipSetDevice(ctx->device));
+// The subset case of target ordering: flag the active slots of the Morton order and compact
+// orderbuf by the flags, the reference's filtering scan (mfm.cc:710-717); particles outside the
+// tree are dropped there. b holds 2 * fb + scan_tmp_bytes(nl) bytes.
+static gz_status order_subset(gz_ctx* ctx, const gz_tree* tree, const uint32_t* active,
+                              size_t n_active, uint32_t* out_targets, size_t* n_out, uint8_t* b,
+                              size_t fb) {
     hipStream_t st = ctx->stream;
-    *n_out = 0;
     const size_t nl = tree->n_live;
-    if (nl == 0) return GZ_OK;
-    if (!out_targets) return fail(GZ_ERR_INVALID, "gz_order_targets: null output");
-    if (!active || n_active >= tree->n) {
-        // everything active: the Morton order is orderbuf (mfm.cc:700-702)
-        GZ_HIP(hipMemcpyAsync(out_targets, tree->orderbuf, nl * sizeof(uint32_t),
-                              hipMemcpyDeviceToDevice, st));
-        GZ_HIP(hipStreamSynchronize(st));
-        *n_out = nl;
-        return GZ_OK;
-    }
-    // A subset: flag the active slots of the Morton order and compact orderbuf by the flags, the
-    // reference's filtering scan (mfm.cc:710-717). Particles outside the tree are dropped there.
-    gz::DevMem& s = ctx->scratch[7];
-    const size_t fb = ((nl + 64) * sizeof(uint32_t) + 255) & ~size_t(255);
-    GZ_TRY(s.reserve(2 * fb + gz::scan_tmp_bytes(nl) + 256));
-    uint8_t* b = s.as<uint8_t>();
     uint32_t* flag = reinterpret_cast<uint32_t*>(b);
     uint32_t* pos = reinterpret_cast<uint32_t*>(b + fb);
     uint32_t* total = pos + nl;
@@ -128,6 +147,59 @@ gz_status gz_order_targets(gz_ctx* ctx, const gz_tree* tree, const uint32_t* act
     GZ_HIP(hipStreamSynchronize(st));
     *n_out = h;
     return GZ_OK;
+}
+
+// Everything active: the Morton order is orderbuf (mfm.cc:700-702).
+static gz_status order_all(gz_ctx* ctx, const gz_tree* tree, uint32_t* out_targets,
+                           size_t* n_out) {
+    const size_t nl = tree->n_live;
+    GZ_HIP(hipMemcpyAsync(out_targets, tree->orderbuf, nl * sizeof(uint32_t),
+                          hipMemcpyDeviceToDevice, ctx->stream));
+    GZ_HIP(hipStreamSynchronize(ctx->stream));
+    *n_out = nl;
+    return GZ_OK;
+}
+
+gz_status gz_order_targets(gz_ctx* ctx, const gz_tree* tree, const uint32_t* active,
+                           size_t n_active, uint32_t* out_targets, size_t* n_out) {
+    gz::clear_error();
+    if (!ctx || !tree || !n_out) return fail(GZ_ERR_INVALID, "gz_order_targets: null argument");
+    GZ_HIP(hipSetDevice(ctx->device));
+    *n_out = 0;
+    const size_t nl = tree->n_live;
+    if (nl == 0) return GZ_OK;
+    if (!out_targets) return fail(GZ_ERR_INVALID, "gz_order_targets: null output");
+    if (!active || n_active >= tree->n) return order_all(ctx, tree, out_targets, n_out);
+    gz::DevMem& s = ctx->scratch[7];
+    const size_t fb = ((nl + 64) * sizeof(uint32_t) + 255) & ~size_t(255);
+    GZ_TRY(s.reserve(2 * fb + gz::scan_tmp_bytes(nl) + 256));
+    return order_subset(ctx, tree, active, n_active, out_targets, n_out, s.as<uint8_t>(), fb);
+}
+
+gz_status gz_order_targets_noalloc(gz_ctx* ctx, const gz_tree* tree, const uint32_t* active,
+                                   size_t n_active, uint32_t* out_targets, size_t* n_out) {
+    gz::clear_error();
+    if (!ctx || !tree || !n_out)
+        return fail(GZ_ERR_INVALID, "gz_order_targets_noalloc: null argument");
+    *n_out = 0;
+    const size_t nl = tree->n_live;
+    if (nl > gz::kNoAllocSlots)
+        return fail(GZ_ERR_INVALID,
+                    "gz_order_targets_noalloc: %zu particles in the tree exceed its fixed scratch "
+                    "of %zu; use gz_order_targets",
+                    nl, gz::kNoAllocSlots);
+    GZ_HIP(hipSetDevice(ctx->device));
+    if (nl == 0) return GZ_OK;
+    if (!out_targets) return fail(GZ_ERR_INVALID, "gz_order_targets_noalloc: null output");
+    if (!active || n_active >= tree->n) return order_all(ctx, tree, out_targets, n_out);
+    uint8_t* b = gz::noalloc_scratch(ctx->device);
+    if (!b)
+        return fail(GZ_ERR_HIP, "gz_order_targets_noalloc: the fixed scratch of device %d was not "
+                                "resolved when its context was created", ctx->device);
+    const size_t fb = ((nl + 64) * sizeof(uint32_t) + 255) & ~size_t(255);
+    if (2 * fb > 2 * gz::kNoAllocFlagBytes || gz::scan_tmp_bytes(nl) > gz::kNoAllocScanBytes)
+        return fail(GZ_ERR_INTERNAL, "gz_order_targets_noalloc: scratch layout exceeded");
+    return order_subset(ctx, tree, active, n_active, out_targets, n_out, b, fb);
 }
 
 gz_status gz_aold_from_accel(gz_ctx* ctx, const double* a_prev, const uint32_t* targets,

@@ -205,6 +205,10 @@ struct gz_ctx {
 };
 
 namespace gz {
+// The fixed scratch of gz_order_targets_noalloc (gz_caller.hip): resolve its address on `device`,
+// which must be current. Called once per device, when a context is created there.
+gz_status resolve_noalloc_scratch(int device);
+
 // Tree reuse (gz_reuse.hip). status bit 0: an index outside the particle set.
 gz_status launch_kick_nodes(gz_tree* t, const uint32_t* active, size_t na, const double* vx,
                             const double* vy, const double* vz, const double* m, double* vlx,

@@ -356,6 +356,15 @@ gz_status gz_tree_group_features(gz_ctx* ctx, const gz_tree* tree, const gz_part
  * any order (NULL with n_active == N means all active). out_targets: room for n_active. */
 gz_status gz_order_targets(gz_ctx* ctx, const gz_tree* tree, const uint32_t* active,
                            size_t n_active, uint32_t* out_targets, size_t* n_out);
+/* gz_order_targets with no device allocation. No counterpart in the reference: it is for callers
+ * that cannot allocate device memory (code under hipGraph capture, tight serving loops, harnesses
+ * that forbid hipMalloc). Its flags and scan live in a fixed 65 MiB array per GPU, resolved when
+ * the first context on that GPU is created and held until the process ends; a tree of more than
+ * 8,388,608 (2^23) live particles is refused with GZ_ERR_INVALID. Every call on a GPU shares that
+ * array, so calls that overlap on one GPU (several host threads or streams) race: serialise them.
+ * Like gz_order_targets it synchronises the context's stream to return *n_out. */
+gz_status gz_order_targets_noalloc(gz_ctx* ctx, const gz_tree* tree, const uint32_t* active,
+                                   size_t n_active, uint32_t* out_targets, size_t* n_out);
 /* aold[k] = err_tol_force_acc * |a_prev[targets[k]]| / G (mfm.cc:743-751); a_prev is N x 3
  * interleaved (Vec3d per particle, the engine's a_grav). */
 gz_status gz_aold_from_accel(gz_ctx* ctx, const double* a_prev, const uint32_t* targets,
