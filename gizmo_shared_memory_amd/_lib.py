@@ -166,6 +166,8 @@ _PROTOS = {
                                      C.c_double, C.c_double, C.c_void_p]),
     "gz_scatter_accel": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t, C.c_void_p, C.c_void_p,
                                    C.c_void_p, C.c_void_p]),
+    "gz_drift": (C.c_int, [C.c_void_p, C.c_size_t, C.c_double, C.c_void_p, C.c_void_p,
+                           C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
     "gz_kick": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t, C.c_void_p, C.c_void_p,
                           C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
 }

@@ -115,6 +115,21 @@ __global__ __launch_bounds__(kT) void k_kick(const uint32_t* __restrict__ active
     pending[i] = open_dt;
 }
 
+// The drift (drift_particle_to, mfm.cc:1287-1295, gravity part, no periodic fold): x += v dt for
+// every particle with m > 0; a tombstone stays parked where it is.
+__global__ __launch_bounds__(kT) void k_drift(size_t n, double dt, const double* __restrict__ vx,
+                                              const double* __restrict__ vy,
+                                              const double* __restrict__ vz,
+                                              const double* __restrict__ m,
+                                              double* __restrict__ x, double* __restrict__ y,
+                                              double* __restrict__ z) {
+    const size_t i = (size_t)blockIdx.x * kT + threadIdx.x;
+    if (i >= n || !(m[i] > 0.0)) return;
+    x[i] = x[i] + vx[i] * dt;
+    y[i] = y[i] + vy[i] * dt;
+    z[i] = z[i] + vz[i] * dt;
+}
+
 } // namespace gz
 
 using gz::fail;
@@ -227,6 +242,20 @@ gz_status gz_scatter_accel(gz_ctx* ctx, const uint32_t* targets, size_t n_t, con
     if (n_t)
         gz::k_scatter<<<gz::blocks(n_t), gz::kT, 0, ctx->stream>>>(targets, n_t, ax, ay, az,
                                                                   a_grav);
+    GZ_HIP(hipGetLastError());
+    return GZ_OK;
+}
+
+gz_status gz_drift(gz_ctx* ctx, size_t n, double dt, const double* vx, const double* vy,
+                   const double* vz, const double* m, double* x, double* y, double* z) {
+    gz::clear_error();
+    if (!ctx) return fail(GZ_ERR_INVALID, "gz_drift: null context");
+    if (n && (!vx || !vy || !vz || !m || !x || !y || !z))
+        return fail(GZ_ERR_INVALID, "gz_drift: null array");
+    if (!std::isfinite(dt)) return fail(GZ_ERR_INVALID, "gz_drift: dt is not finite");
+    GZ_HIP(hipSetDevice(ctx->device));
+    if (n)
+        gz::k_drift<<<gz::blocks(n), gz::kT, 0, ctx->stream>>>(n, dt, vx, vy, vz, m, x, y, z);
     GZ_HIP(hipGetLastError());
     return GZ_OK;
 }

@@ -81,6 +81,35 @@ def plummer(n=65536, a=1.0, rmax_over_a=20.0, eps=0.01, seed=SEEDS["plummer_64k"
     return _pack(x, y, z, eps)
 
 
+def plummer_equilibrium(n=32768, a=1.0, rmax_over_a=130.5, eps=0.28, seed=42):
+    """A Plummer sphere in equilibrium, G = M = 1: positions as plummer() (truncated at
+    rmax_over_a, 100 half-mass radii by default, as test/plummer/make_plummer_ics.py truncates),
+    speeds from the isotropic distribution function. The speed in units of the local escape
+    speed sqrt(2 / sqrt(r^2 + a^2)) has density proportional to q^2 (1 - q^2)^(7/2) on [0, 1],
+    drawn by rejection under its maximum (about 0.092 at q^2 = 2/9; Aarseth, Henon & Wielen
+    1974, A&A 37, 183). Returns (x, y, z, vx, vy, vz, m, soft) with the centre of mass and its
+    velocity at the origin. eps is the kernel extent h (0.28 = 2.8 x the Plummer-equivalent 0.1
+    of the reference's plummer.params)."""
+    rng = np.random.default_rng(seed)
+    r = plummer_radii(rng, n, a, rmax_over_a)
+    x, y, z = _isotropic(rng, r)
+    q = np.empty(n)
+    todo = np.arange(n)
+    while todo.size:
+        qq = rng.uniform(0.0, 1.0, todo.size)
+        yy = rng.uniform(0.0, 0.1, todo.size)
+        ok = yy < qq * qq * (1.0 - qq * qq) ** 3.5
+        q[todo[ok]] = qq[ok]
+        todo = todo[~ok]
+    vesc = np.sqrt(2.0 / np.sqrt(r * r + a * a))
+    vx, vy, vz = _isotropic(rng, q * vesc)
+    m = np.full(n, 1.0 / n)
+    out = []
+    for c in (x, y, z, vx, vy, vz):
+        out.append(np.ascontiguousarray(c - np.sum(m * c)))
+    return (*out, m, np.full(n, float(eps)))
+
+
 def hernquist(n=4194304, a=1.0, rmax_over_a=50.0, eps=0.005, seed=SEEDS["hernquist_4m"]):
     """Hernquist sphere, M=1, truncated at rmax_over_a * a (the 4M plain-GPU workload)."""
     rng = np.random.default_rng(seed)

@@ -685,6 +685,20 @@ def scatter_accel(targets, ax, ay, az, a_grav: torch.Tensor, ctx: Context | None
     ctx.synchronize()
 
 
+def drift(P: Particles, v, dt: float, ctx: Context | None = None):
+    """The drift (mfm.cc:1287-1295, gravity part): P's positions += v dt in place for every
+    particle with m > 0. v: three float64 device tensors by particle."""
+    ctx = ctx or Context.get(P.device.index or 0)
+    for t in v:
+        if t.dtype != torch.float64 or not t.is_contiguous() or t.device != P.device \
+                or int(t.shape[0]) != P.n:
+            raise ValueError("drift: v must be three contiguous float64 tensors, one per particle")
+    ctx.enter()
+    L.check(L.load().gz_drift(ctx.handle, P.n, float(dt), *(_ptr(t) for t in v), _ptr(P.m),
+                              _ptr(P.x), _ptr(P.y), _ptr(P.z)), "gz_drift")
+    ctx.synchronize()
+
+
 def kick(active, a_grav: torch.Tensor, dt: torch.Tensor, pending: torch.Tensor, v,
          a_extra: torch.Tensor | None = None, ctx: Context | None = None):
     """The gravity part of the KDK kick (mfm.cc:3749-3767) for the particles in active: with

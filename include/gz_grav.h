@@ -381,6 +381,12 @@ gz_status gz_kick(gz_ctx* ctx, const uint32_t* active, size_t n_active, const do
                   const double* a_extra, const double* dt, double* pending_half_kick, double* vx,
                   double* vy, double* vz);
 
+/* The drift (drift_particle_to, mfm.cc:1287-1295, gravity part, open boundaries): for every
+ * particle with m > 0, x += v * dt; massless tombstones stay where they are. All arrays device, by
+ * particle. A tree built before the drift needs gz_tree_update_sources before its next walk. */
+gz_status gz_drift(gz_ctx* ctx, size_t n, double dt, const double* vx, const double* vy,
+                   const double* vz, const double* m, double* x, double* y, double* z);
+
 #ifdef __cplusplus
 }
 #endif
