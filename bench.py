@@ -294,18 +294,15 @@ def other_configs(gz, ics, args):
         pt = tg.view(torch.int32)[torch.as_tensor(probes.astype(np.int64), device="cuda:0")]
         ex = [v.cpu().numpy() for v in gz.accel_brute(P, pt.view(torch.uint32), 1.0)]
         out = {"workload": name, "build_ms": T.times()["total"]}
-        # the FP32 walk's group size: the fastest of 16, 32 and 64 on two timed steps (small
-        # systems have too few 64-target groups to fill the GPU)
-        best = None
-        for gs in (16, 32, 64):
-            dt = time_walk(gz, T, P, tg, aold, 2, 1, precision="fp32", group_size=gs)[1]
-            if best is None or dt < best[1]:
-                best = (gs, dt)
-        for label, prec, gs in (("fp64", "fp64", 0), ("fp32", "fp32", best[0])):
+        # the FP32 walk's launch parameters from the library's autotuner (small systems have
+        # too few 64-target groups to fill the GPU)
+        ch32, _ = gz.autotune(T, P, tg, 0.0, 1.0, aold, precision="fp32")
+        ch64 = {"group_size": 0, "stack_limit": 0, "waves_per_cu": 0}
+        for label, prec, ch in (("fp64", "fp64", ch64), ("fp32", "fp32", ch32)):
             (ax, ay, az, _), dt, kms = time_walk(gz, T, P, tg, aold, args.steps, args.warmup,
-                                                 precision=prec, group_size=gs)
+                                                 precision=prec, **ch)
             e = rel_err(tuple(v.cpu().numpy()[probes] for v in (ax, ay, az)), ex)
-            out[label] = {"group_size": gs, "ms_per_step": dt * 1e3, "kernel_ms": kms,
+            out[label] = {**ch, "ms_per_step": dt * 1e3, "kernel_ms": kms,
                           "evals_per_s": n / dt, "err": err_stats(e)}
             del ax, ay, az
         if n <= 65536 and not args.no_cpu:
@@ -332,7 +329,9 @@ def main():
                     help="CPU work per timed baseline walk (one warm-up and three timed walks)")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--group-sizes", default="0,16,32,64")
+    ap.add_argument("--group-size", type=int, default=-1,
+                    help="target-group size of the FP32 walks; -1 (default): the library's "
+                         "autotuner picks group size, node-stack depth and launch width")
     ap.add_argument("--variants", default="fp64,fp32,learned,global",
                     help="walks to measure (the headline is the learned walk when measured, else "
                          "the FP32 walk); a profiler run can keep to one walk kernel")
@@ -389,15 +388,15 @@ def main():
     ex = [v.cpu().numpy() for v in ex]
     log(f"direct sum on {len(probes)} probes: {time.perf_counter() - t0:.2f} s")
 
-    def measure(label, prec, gs, steps, warmup, scale=1.0, model=None):
+    def measure(label, prec, choice, steps, warmup, scale=1.0, model=None):
         ao = aold if scale == 1.0 else aold * scale
         (ax, ay, az, c), dt, kms = time_walk(gz, T, P, targets, ao, steps, warmup, precision=prec,
-                                             group_size=gs, model=model)
+                                             model=model, **choice)
         cnt = gz.accel_grouped(T, P, targets, 0.0, 1.0, 0, aold=ao, precision=prec,
-                               group_size=gs, counts=True, model=model)[3]
+                               group_size=choice["group_size"], counts=True, model=model)[3]
         inter = cnt["multipole"] + cnt["pairs"]
         e = rel_err(tuple(v.cpu().numpy()[probes] for v in (ax, ay, az)), ex)
-        r = {"group_size": gs, "ms_per_step": dt * 1e3, "kernel_ms": kms,
+        r = {**choice, "ms_per_step": dt * 1e3, "kernel_ms": kms,
              "evals_per_s": nt / dt, "interactions_per_target": inter / nt,
              "multipole_per_target": cnt["multipole"] / nt, "pairs_per_target": cnt["pairs"] / nt,
              "nodes_visited_per_target": cnt["nodes_visited"] / nt, "groups": cnt["groups"],
@@ -412,29 +411,29 @@ def main():
 
     results = {}
     variants = set(args.variants.split(","))
-    sizes = [int(v) for v in args.group_sizes.split(",")]
+    fixed_gs = int(args.group_size)
 
-    def tune(label, model=None):
-        """The group size with the lowest time per step for this walk (two timed steps each)."""
-        if len(sizes) == 1:
-            return sizes[0]
-        best = None
-        for gs in sizes:
-            (_, dt, kms) = time_walk(gz, T, P, targets, aold, 2, 1, precision="fp32",
-                                     group_size=gs, model=model)
-            log(f"tune {label} group_size={gs}: {dt * 1e3:.2f} ms/step, kernel {kms:.2f} ms")
-            if best is None or dt < best[1]:
-                best = (gs, dt)
-        return best[0]
+    def tune(label, prec="fp32", model=None):
+        """The launch parameters for this walk: the library's autotuner (gz_walk_autotune:
+        group size, then node-stack depth and launch width, each timed on this workload), or
+        the one group size given with --group-size."""
+        if fixed_gs >= 0:
+            return {"group_size": fixed_gs, "stack_limit": 0, "waves_per_cu": 0}
+        choice, rep = gz.autotune(T, P, targets, 0.0, 1.0, aold, precision=prec, model=model)
+        log(f"autotune {label}: {choice} from {rep['points']}")
+        return choice
 
-    # plain GPU walks with the reference criterion
+    # plain GPU walks with the reference criterion: the parity walk with the reference's own
+    # grouping (one group per leaf), and the FP32 walk
     if "fp64" in variants:
-        results["fp64"] = measure("fp64", "fp64", 0, args.steps, args.warmup)
-    gsz = sizes[0]
+        results["fp64"] = measure("fp64", "fp64",
+                                  {"group_size": 0, "stack_limit": 0, "waves_per_cu": 0},
+                                  args.steps, args.warmup)
+    ch32 = None
     if "fp32" in variants or "global" in variants:
-        gsz = tune("fp32")
+        ch32 = tune("fp32")
     if "fp32" in variants:
-        results["fp32"] = measure("fp32", "fp32", gsz, args.steps, args.warmup)
+        results["fp32"] = measure("fp32", "fp32", ch32, args.steps, args.warmup)
 
     # learned criterion (FP32 walk), its group size tuned on its own
     model = None
@@ -443,8 +442,8 @@ def main():
     elif os.path.exists(args.model):
         from gizmo_shared_memory_amd import learned
         model = learned.load_model(args.model)
-        gsl = tune("learned", model)
-        results["learned_fp32"] = measure("learned_fp32", "fp32", gsl, args.steps, args.warmup,
+        chl = tune("learned", model=model)
+        results["learned_fp32"] = measure("learned_fp32", "fp32", chl, args.steps, args.warmup,
                                           model=model)
     else:
         log(f"no learned model at {args.model}")
@@ -452,17 +451,18 @@ def main():
         # the method without the network: one global tolerance for every group, the largest
         # on a grid whose p99 error stays within the learned walk's
         goal = max(results["learned_fp32"]["err"]["p99"], 1e-12)
-        gsl = results["learned_fp32"]["group_size"]
+        chg = {k: results["learned_fp32"][k] for k in ("group_size", "stack_limit",
+                                                        "waves_per_cu")}
         glob = None
         for k in range(0, 11):
             sc = 2.0 ** (k / 2.0)
-            r = measure(f"global_tol_x{sc:.2f}", "fp32", gsl, 1, 1, scale=sc)
+            r = measure(f"global_tol_x{sc:.2f}", "fp32", chg, 1, 1, scale=sc)
             if r["err"]["p99"] <= goal:
                 glob = sc
             else:
                 break
         if glob is not None:  # the chosen tolerance, timed like the other walks
-            glob = measure(f"global_tol_x{glob:.2f}", "fp32", gsl, args.steps, args.warmup,
+            glob = measure(f"global_tol_x{glob:.2f}", "fp32", chg, args.steps, args.warmup,
                            scale=glob)
         results["global_tolerance_fp32"] = glob
 
@@ -514,7 +514,8 @@ def main():
                      "fallback" if "learned_fp32" in results else
                      f"FP32 group-relative, reference criterion (relative {ERR_TOL_FORCE_ACC}, "
                      f"aold from theta={THETA_BOOT} bootstrap)") +
-                    f", group_size={head['group_size']}",
+                    f", group_size={head['group_size']}, stack_limit={head['stack_limit']}, "
+                    f"waves_per_cu={head['waves_per_cu']} (autotuned)",
             "learned_criterion": (os.path.relpath(args.model, ROOT)
                                   if "learned_fp32" in results else None),
             "n_probes": len(probes),

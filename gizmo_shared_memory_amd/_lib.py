@@ -68,7 +68,21 @@ class gz_tree_info(C.Structure):
 class gz_walk_opts(C.Structure):
     _fields_ = [("precision", C.c_int), ("criterion", C.c_int), ("group_size", C.c_int),
                 ("launch", C.c_int), ("count", C.c_int), ("stack_limit", C.c_int),
-                ("model", C.c_void_p)]
+                ("model", C.c_void_p), ("waves_per_cu", C.c_int)]
+
+
+TUNE_RETUNE = 1
+TUNE_MAX_POINTS = 16
+
+
+class gz_tune_point(C.Structure):
+    _fields_ = [("group_size", C.c_int), ("stack_limit", C.c_int), ("waves_per_cu", C.c_int),
+                ("ms", C.c_double)]
+
+
+class gz_tune_report(C.Structure):
+    _fields_ = [("n_points", C.c_int), ("from_cache", C.c_int),
+                ("points", gz_tune_point * TUNE_MAX_POINTS), ("best", gz_tune_point)]
 
 
 class gz_walk_counts(C.Structure):
@@ -114,6 +128,7 @@ _PROTOS = {
     "gz_tree_kick_nodes": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t, C.c_void_p,
                                      C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
                                      C.c_void_p, C.c_double]),
+    "gz_tree_drift": (C.c_int, [C.c_void_p, C.c_void_p, C.c_double]),
     "gz_tree_refresh_walk_nodes": (C.c_int, [C.c_void_p, C.c_void_p, C.c_double]),
     "gz_tree_raise_soft": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t,
                                      C.c_void_p]),
@@ -126,6 +141,10 @@ _PROTOS = {
     "gz_tree_accel": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(gz_particles), C.c_void_p,
                                 C.c_size_t, C.c_double, C.c_double, C.c_void_p, C.c_void_p,
                                 C.c_void_p, C.c_void_p]),
+    "gz_walk_autotune": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(gz_particles), C.c_void_p,
+                                   C.c_size_t, C.c_double, C.c_double, C.c_void_p,
+                                   C.POINTER(gz_walk_opts), C.c_int, C.c_int,
+                                   C.POINTER(gz_walk_opts), C.POINTER(gz_tune_report)]),
     "gz_direct_sum": (C.c_int, [C.c_void_p, C.POINTER(gz_particles), C.c_void_p, C.c_size_t,
                                 C.c_double, C.c_void_p, C.c_void_p, C.c_void_p]),
     "gz_direct_sum_ex": (C.c_int, [C.c_void_p, C.POINTER(gz_particles), C.c_void_p,

@@ -195,6 +195,132 @@ def substeps_reuse(n=1 << 23, n_levels=5, precision="fp32", group_size=32, model
     }
 
 
+def _p(t):
+    import ctypes as C
+    return C.c_void_p(t.data_ptr())
+
+
+class Schedule:
+    """One cycle of 2^(n_levels-1) substeps of a kick-drift-kick integrator on one reused tree, as
+    the reference runs between rebuilds (mfm.cc mfm_step, :3440-3870): at substep s everyone
+    drifts by dt_sub (gz_drift on the particle arrays, gz_tree_drift on the tree's sorted copy),
+    the walk nodes are refreshed to the tree's age (gz_tree_refresh_walk_nodes), and the active
+    set of substep s gets aold from its previous accelerations (gz_aold_from_accel), its walk (a
+    plan formed once, with the launch parameters gz_walk_autotune picked for its size), the
+    write-back (gz_scatter_accel), the kick with dt from its bin (gz_kick) and the kick feed
+    (gz_tree_kick_nodes). Nothing in a cycle synchronises with the host, so a cycle can be
+    captured into one hipGraph and replayed."""
+
+    def __init__(self, n=1 << 23, n_levels=5, precision="fp32", model=None, dt_sub=1e-4,
+                 sigma_v=0.3, seed=11, tune=True, group_size=32):
+        from . import _lib as L
+        self.L = L
+        x, y, z, m, soft = ics.halo_disk(n=n)
+        rng = np.random.default_rng(seed)
+        dev = "cuda:0"
+        self.v = [torch.as_tensor(rng.normal(scale=sigma_v, size=n), device=dev)
+                  for _ in range(3)]
+        self.P = P = gz.Particles(x, y, z, m, soft)
+        self.T = T = gz.build(P, vel=self.v)
+        self.ctx = T.ctx
+        self.vlast = [a.clone() for a in self.v]
+        tg = T.device_orderbuf()
+        boot = gz.accel_grouped(T, P, tg, THETA_BOOT, 1.0, 0)
+        self.a_grav = torch.zeros((n, 3), dtype=torch.float64, device=dev)
+        gz.scatter_accel(tg, *boot, self.a_grav)
+        del boot
+        level = ics.timestep_bins(x, y, z, n_levels=n_levels)
+        self.nsub = 2 ** (n_levels - 1)
+        self.dt_sub = dt_sub
+        # a particle of level b steps 2^(n_levels - 1 - b) substeps
+        self.dt = torch.as_tensor(dt_sub * 2.0 ** (n_levels - 1 - level.astype(np.float64)),
+                                  device=dev)
+        self.pending = 0.5 * self.dt.clone()
+        self.lists = [gz.order_targets(T, ics.active_for_substep(level, s, n_levels))
+                      for s in range(self.nsub)]
+        self.aolds = [torch.empty(int(t.shape[0]), dtype=torch.float64, device=dev)
+                      for t in self.lists]
+        self.out = [gz._outputs(int(t.shape[0]), P.device) for t in self.lists]
+        self.choices = []
+        self.plans = []
+        for s, t in enumerate(self.lists):
+            aold = gz.aold_from_accel(T, self.a_grav, t, ALPHA_REF, 1.0)
+            if tune:
+                ch, rep = gz.autotune(T, P, t, 0.0, 1.0, aold, precision=precision, model=model)
+            else:
+                ch, rep = {"group_size": group_size, "stack_limit": 0, "waves_per_cu": 0}, None
+            self.choices.append(ch)
+            self.plans.append(gz.WalkPlan(T, P, t, precision=precision, model=model, **ch))
+        self.counts = [int(t.shape[0]) for t in self.lists]
+
+    def enqueue_cycle(self):
+        """The cycle's calls on the library stream, with no host synchronisation."""
+        L, lib, h = self.L, self.L.load(), self.ctx.handle
+        P, T = self.P, self.T
+        for s in range(self.nsub):
+            L.check(lib.gz_drift(h, P.n, self.dt_sub, *(_p(a) for a in self.v), _p(P.m),
+                                 _p(P.x), _p(P.y), _p(P.z)), "gz_drift")
+            # the tree's age advances on the device (gz_tree_drift); tau = -1 reads it there, so
+            # every replay of a captured cycle refreshes at the right age
+            L.check(lib.gz_tree_drift(h, T.handle, self.dt_sub), "gz_tree_drift")
+            L.check(lib.gz_tree_refresh_walk_nodes(h, T.handle, -1.0),
+                    "gz_tree_refresh_walk_nodes")
+            t, aold, (ax, ay, az) = self.lists[s], self.aolds[s], self.out[s]
+            nt = int(t.shape[0])
+            L.check(lib.gz_aold_from_accel(h, _p(self.a_grav), _p(t), nt, ALPHA_REF, 1.0,
+                                           _p(aold)), "gz_aold_from_accel")
+            L.check(lib.gz_walk_plan_launch(h, self.plans[s].handle, 0.0, 1.0, _p(aold),
+                                            _p(ax), _p(ay), _p(az)), "gz_walk_plan_launch")
+            L.check(lib.gz_scatter_accel(h, _p(t), nt, _p(ax), _p(ay), _p(az),
+                                         _p(self.a_grav)), "gz_scatter_accel")
+            L.check(lib.gz_kick(h, _p(t), nt, _p(self.a_grav), None, _p(self.dt),
+                                _p(self.pending), *(_p(a) for a in self.v)), "gz_kick")
+            L.check(lib.gz_tree_kick_nodes(h, T.handle, _p(t), nt, *(_p(a) for a in self.v),
+                                           _p(P.m), *(_p(a) for a in self.vlast), -1.0),
+                    "gz_tree_kick_nodes")
+
+    def state(self):
+        return [a.clone() for a in (self.P.x, self.P.y, self.P.z, *self.v, self.a_grav,
+                                    self.pending)]
+
+
+def substeps_schedule(n=1 << 23, n_levels=5, precision="fp32", model=None, cycles=3):
+    """Config 4a as an integrator runs it: each cycle's 16 substeps in sequence (drift, tree
+    drift, refresh, walk, write-back, kick, kick feed), eager and as one captured hipGraph per
+    cycle. Evals/s count the active targets walked per cycle over the whole cycle's time."""
+    S = Schedule(n=n, n_levels=n_levels, precision=precision, model=model)
+    ctx = S.ctx
+    total = sum(S.counts)
+    # eager cycles (the first also loads every kernel)
+    t_eager = []
+    for _ in range(cycles):
+        _sync(ctx)
+        t0 = time.perf_counter()
+        S.enqueue_cycle()
+        _sync(ctx)
+        t_eager.append(time.perf_counter() - t0)
+    # one cycle captured, replayed
+    g = gz.Graph(ctx)
+    g.begin()
+    S.enqueue_cycle()
+    g.end()
+    t_graph = []
+    for _ in range(cycles):
+        _sync(ctx)
+        t0 = time.perf_counter()
+        g.launch()
+        _sync(ctx)
+        t_graph.append(time.perf_counter() - t0)
+    return {
+        "workload": f"halo+disk N={n}, {S.nsub} sequential substeps of {n_levels} power-of-two "
+                    f"bins on one reused tree, kick-drift-kick",
+        "precision": precision, "learned": model is not None,
+        "tuned": S.choices, "active_per_substep": S.counts,
+        "eager_cycle_ms": min(t_eager) * 1e3, "eager_evals_per_s": total / min(t_eager),
+        "graph_cycle_ms": min(t_graph) * 1e3, "graph_evals_per_s": total / min(t_graph),
+    }
+
+
 def spheres(n_spheres=512, precision="fp32", group_size=32, theta=0.7, n_probe_spheres=16,
             reps=3):
     (x, y, z, m, soft), starts = ics.spheres(n_spheres=n_spheres)
@@ -251,6 +377,7 @@ def main():
     if args.learned:
         from . import learned
         model = learned.load_model()
+    print(json.dumps(substeps_schedule(model=model)), flush=True)
     print(json.dumps(substeps(model=model)), flush=True)
     print(json.dumps(substeps_reuse(model=model)), flush=True)
     print(json.dumps(spheres()), flush=True)

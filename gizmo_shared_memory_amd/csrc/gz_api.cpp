@@ -62,6 +62,7 @@ struct WalkExtra {
     double* tidal = nullptr;       // tidal tensor out (6 x n_t), FP64 walks
     double* jerk = nullptr;        // jerk out (3 x n_t), with vx/vy/vz
     const double *vx = nullptr, *vy = nullptr, *vz = nullptr;
+    int waves_per_cu = 0;          // persistent launch width (0 = the most)
 };
 
 // The walk shared by gz_tree_accel_grouped, gz_tree_accel and gz_tree_group_features.
@@ -258,6 +259,7 @@ gz_status walk(gz_ctx* ctx, const gz_tree* tree, const gz_particles* p, const ui
     a.gstack = nullptr;
     a.gstack_cap = 0;
     a.stack_limit = stack_limit;
+    a.waves_per_cu = ex.waves_per_cu;
     if (persistent) GZ_TRY(gz::ctx_spill(ctx, &a.spill, &a.spill_cap));
     a.G = G;
     a.ax = ax; a.ay = ay; a.az = az;
@@ -488,6 +490,11 @@ gz_status gz_tree_kick_nodes(gz_ctx* ctx, gz_tree* tree, const uint32_t* active,
     GZ_TRY(gz::fill_zero_u32(reinterpret_cast<uint32_t*>(status), 1, st));
     GZ_TRY(gz::launch_kick_nodes(tree, active, n_active, vx, vy, vz, m, vlast_x, vlast_y, vlast_z,
                                  tau, status, st));
+    // Under stream capture the index check cannot be read back: the kernel still skips an index
+    // outside the particle set, but the call cannot report it.
+    hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
+    GZ_HIP(hipStreamIsCapturing(st, &cap));
+    if (cap != hipStreamCaptureStatusNone) return GZ_OK;
     int h = 0;
     GZ_HIP(hipMemcpyAsync(&h, status, sizeof h, hipMemcpyDeviceToHost, st));
     GZ_HIP(hipStreamSynchronize(st));
@@ -501,6 +508,16 @@ gz_status gz_tree_refresh_walk_nodes(gz_ctx* ctx, gz_tree* tree, double tau) {
     GZ_TRY(check_ctx(ctx));
     if (!tree) return fail(GZ_ERR_INVALID, "gz_tree_refresh_walk_nodes: null tree");
     return gz::launch_refresh(tree, tau, ctx->stream);
+}
+
+gz_status gz_tree_drift(gz_ctx* ctx, gz_tree* tree, double dt) {
+    gz::clear_error();
+    GZ_TRY(check_ctx(ctx));
+    if (!tree) return fail(GZ_ERR_INVALID, "gz_tree_drift: null tree");
+    if (!tree->has_vel || !tree->svel)
+        return fail(GZ_ERR_INVALID, "gz_tree_drift: the tree was built without velocities");
+    if (!std::isfinite(dt)) return fail(GZ_ERR_INVALID, "gz_tree_drift: dt is not finite");
+    return gz::launch_tree_drift(tree, dt, ctx->stream);
 }
 
 gz_status gz_tree_raise_soft(gz_ctx* ctx, gz_tree* tree, const uint32_t* idx, size_t n,
@@ -603,6 +620,10 @@ gz_status gz_tree_accel_grouped(gz_ctx* ctx, const gz_tree* tree, const gz_parti
         persistent = opts->launch == GZ_LAUNCH_QUEUE;
         count = opts->count;
         stack_limit = opts->stack_limit;
+        if (opts->waves_per_cu < 0 || opts->waves_per_cu > 20)
+            return fail(GZ_ERR_INVALID, "gz_tree_accel_grouped: waves_per_cu %d out of [0, 20]",
+                        opts->waves_per_cu);
+        ex.waves_per_cu = opts->waves_per_cu;
         if (opts->criterion == GZ_CRIT_LEARNED) {
             if (!opts->model)
                 return fail(GZ_ERR_INVALID, "gz_tree_accel_grouped: learned criterion without a model");
@@ -616,6 +637,140 @@ gz_status gz_tree_accel_grouped(gz_ctx* ctx, const gz_tree* tree, const gz_parti
     return walk(ctx, tree, p, targets, n_t, theta, G, batch, group_size, aold, precision,
                 persistent, 0, count, stack_limit, 1, ax, ay, az, counts,
                 "gz_tree_accel_grouped", ex);
+}
+
+// ---- walk autotuner ----------------------------------------------------------------------------
+namespace {
+
+int ilog2_bucket(long long v) {
+    int b = 0;
+    while (v > 1) {
+        v >>= 1;
+        ++b;
+    }
+    return b;
+}
+
+// The workload a tuned choice is kept for: what decides which launch parameters win.
+std::vector<long long> tune_key(const gz_tree* tree, size_t n_t, double theta,
+                                const gz_walk_opts& o) {
+    return {o.precision, o.criterion, (long long)reinterpret_cast<uintptr_t>(o.model),
+            o.launch, theta > 0.0 ? 1 : 0, ilog2_bucket((long long)n_t),
+            ilog2_bucket((long long)tree->n_live), ilog2_bucket((long long)tree->nnodes),
+            tree->n_clusters > 1 ? 1 : 0};
+}
+
+} // namespace
+
+gz_status gz_walk_autotune(gz_ctx* ctx, const gz_tree* tree, const gz_particles* p,
+                           const uint32_t* targets, size_t n_t, double theta, double G,
+                           const double* aold, const gz_walk_opts* base, int reps, int flags,
+                           gz_walk_opts* best, gz_tune_report* report) {
+    gz::clear_error();
+    GZ_TRY(check_ctx(ctx));
+    if (!tree || !p || !best) return fail(GZ_ERR_INVALID, "gz_walk_autotune: null argument");
+    if (n_t > 0 && !targets) return fail(GZ_ERR_INVALID, "gz_walk_autotune: null targets");
+    gz_walk_opts b0{};
+    if (base) b0 = *base;
+    b0.count = 0;
+    if (b0.criterion == GZ_CRIT_LEARNED && !aold)
+        return fail(GZ_ERR_INVALID, "gz_walk_autotune: the learned criterion needs aold");
+    if (reps <= 0) reps = 2;
+    if (report) std::memset(report, 0, sizeof *report);
+    const std::vector<long long> key = tune_key(tree, n_t, theta, b0);
+    if (!(flags & GZ_TUNE_RETUNE)) {
+        for (const auto& e : ctx->tuned)
+            if (e.first == key) {
+                *best = e.second;
+                best->model = b0.model;
+                if (report) {
+                    report->from_cache = 1;
+                    report->best = {best->group_size, best->stack_limit, best->waves_per_cu, 0.0};
+                }
+                return GZ_OK;
+            }
+    }
+    *best = b0;
+    if (n_t == 0) return GZ_OK;
+    // scratch outputs: the caller's arrays are left alone
+    gz::DevMem& out = ctx->scratch[12];
+    GZ_TRY(out.reserve(3 * n_t * sizeof(double)));
+    double* ax = out.as<double>();
+    int np = 0;
+    gz_tune_point pts[GZ_TUNE_MAX_POINTS];
+    auto time_one = [&](const gz_walk_opts& o, double* ms) -> gz_status {
+        gz_walk_counts c;
+        double sum = 0.0;
+        for (int r = 0; r <= reps; ++r) {
+            GZ_TRY(gz_tree_accel_grouped(ctx, tree, p, targets, n_t, theta, G, 0, aold, ax,
+                                         ax + n_t, ax + 2 * n_t, &o, &c));
+            if (r > 0) sum += c.walk_ms + c.prep_ms;
+        }
+        *ms = sum / reps;
+        if (np < GZ_TUNE_MAX_POINTS)
+            pts[np++] = {o.group_size, o.stack_limit, o.waves_per_cu, *ms};
+        return GZ_OK;
+    };
+    gz_walk_opts cur = b0;
+    cur.stack_limit = 0;
+    cur.waves_per_cu = 0;
+    double best_ms = 1e300;
+    // 1. target-group size at the default stack and width
+    std::vector<int> sizes = {16, 32, 64};
+    if (b0.precision == GZ_PREC_FP64) sizes.insert(sizes.begin(), 0);
+    int best_gs = sizes[0];
+    for (int gs : sizes) {
+        gz_walk_opts o = cur;
+        o.group_size = gs;
+        double ms = 0;
+        GZ_TRY(time_one(o, &ms));
+        if (ms < best_ms) {
+            best_ms = ms;
+            best_gs = gs;
+        }
+    }
+    cur.group_size = best_gs;
+    // 2. the on-chip node-stack depth, then 3. the persistent launch width (queue launches)
+    if (cur.launch == GZ_LAUNCH_QUEUE) {
+        int best_sl = 0;
+        for (int sl : {512, 640}) {
+            gz_walk_opts o = cur;
+            o.stack_limit = sl;
+            double ms = 0;
+            GZ_TRY(time_one(o, &ms));
+            if (ms < best_ms) {
+                best_ms = ms;
+                best_sl = sl;
+            }
+        }
+        cur.stack_limit = best_sl;
+        int best_w = 0;
+        for (int w : {10, 15}) {
+            gz_walk_opts o = cur;
+            o.waves_per_cu = w;
+            double ms = 0;
+            GZ_TRY(time_one(o, &ms));
+            if (ms < best_ms) {
+                best_ms = ms;
+                best_w = w;
+            }
+        }
+        cur.waves_per_cu = best_w;
+    }
+    *best = cur;
+    bool stored = false;
+    for (auto& e : ctx->tuned)
+        if (e.first == key) {
+            e.second = cur;
+            stored = true;
+        }
+    if (!stored) ctx->tuned.emplace_back(key, cur);
+    if (report) {
+        report->n_points = np;
+        std::memcpy(report->points, pts, (size_t)np * sizeof(gz_tune_point));
+        report->best = {cur.group_size, cur.stack_limit, cur.waves_per_cu, best_ms};
+    }
+    return GZ_OK;
 }
 
 gz_status gz_tree_group_features(gz_ctx* ctx, const gz_tree* tree, const gz_particles* p,
@@ -651,6 +806,9 @@ gz_status gz_walk_plan_create(gz_ctx* ctx, const gz_tree* tree, const gz_particl
         group_size = opts->group_size;
         persistent = opts->launch == GZ_LAUNCH_QUEUE;
         if (opts->count) return fail(GZ_ERR_UNSUPPORTED, "gz_walk_plan_create: counting");
+        if (opts->waves_per_cu < 0 || opts->waves_per_cu > 20 || opts->stack_limit < 0)
+            return fail(GZ_ERR_INVALID, "gz_walk_plan_create: waves_per_cu or stack_limit out of "
+                                        "range");
         if (opts->criterion == GZ_CRIT_LEARNED) {
             if (!opts->model) return fail(GZ_ERR_INVALID, "gz_walk_plan_create: no model");
             model = static_cast<gz_model*>(const_cast<void*>(opts->model));
@@ -673,6 +831,8 @@ gz_status gz_walk_plan_create(gz_ctx* ctx, const gz_tree* tree, const gz_particl
     P->precision = precision;
     P->persistent = persistent;
     P->model = model;
+    P->stack_limit = opts ? opts->stack_limit : 0;
+    P->waves_per_cu = opts ? opts->waves_per_cu : 0;
     // device memory: group starts (n_t + 1), counters (16 x i64), retry list (n_t), segment
     // offsets, and the global stacks of the re-walk
     P->gstack_cap = std::max(1024, std::min(tree->nnodes, 1 << 18));
@@ -768,6 +928,8 @@ gz_status gz_walk_plan_launch(gz_ctx* ctx, gz_walk_plan* P, double theta, double
     a.nclusters = tree->n_clusters;
     a.persistent = P->persistent;
     a.leaf_always_open = 0;
+    a.stack_limit = P->stack_limit;
+    a.waves_per_cu = P->waves_per_cu;
     a.queue = reinterpret_cast<int*>(P->misc + 9); // eight ints: misc[9..12]
     a.status = reinterpret_cast<int*>(P->misc + 6);
     a.counts = P->misc;

@@ -251,6 +251,18 @@ __global__ __launch_bounds__(kThreads) void k_gather_sorted(
     rank[p] = (uint32_t)r;
 }
 
+// Sorted copy of the build-time velocities.
+__global__ __launch_bounds__(kThreads) void k_gather_vel(const uint32_t* __restrict__ orderbuf,
+                                                         size_t nl, const double* __restrict__ vx,
+                                                         const double* __restrict__ vy,
+                                                         const double* __restrict__ vz,
+                                                         double4* __restrict__ svel) {
+    const size_t r = (size_t)blockIdx.x * kThreads + threadIdx.x;
+    if (r >= nl) return;
+    const uint32_t p = orderbuf[r];
+    svel[r] = make_double4(vx[p], vy[p], vz[p], 0.0);
+}
+
 __global__ __launch_bounds__(kThreads) void k_fill_u32(uint32_t* __restrict__ a, size_t n,
                                                        uint32_t v) {
     const size_t i = (size_t)blockIdx.x * kThreads + threadIdx.x;
@@ -523,7 +535,7 @@ void tree_release(gz_tree* t) {
     void* ptrs[] = {t->skey, t->orderbuf, t->rank, t->leaf_of, t->slot_leaf, t->src, t->src_soft,
                     t->wn, t->next, t->parent, t->nlo, t->nhi, t->gcx, t->gcy, t->gcz, t->size,
                     t->delta, t->soft, t->level_begin_dev, t->cluster_start_dev, t->reuse_mem,
-                    t->hmax};
+                    t->hmax, t->svel, t->age};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
 }
@@ -758,10 +770,21 @@ gz_status build_tree(gz_ctx* ctx, const gz_particles* p, const gz_build_opts* op
     // --- reuse state (tree.h:250-317): vmax and xpad always, the velocity moments with velocities
     NodeVel nv{};
     GZ_TRY(alloc_reuse(t, nnodes, opts && opts->vx && opts->vy && opts->vz));
+    if (!t->age) GZ_TRY(dalloc(reinterpret_cast<void**>(&t->age), sizeof(double)));
+    k_fill_u32<<<1, kThreads, 0, st>>>(reinterpret_cast<uint32_t*>(t->age), 2, 0u);
     nv.nn = nnodes;
     nv.vmax = t->vmax;
     nv.xpad = t->xpad;
     if (t->has_vel) {
+        if (t->svel) (void)hipFree(t->svel);
+        t->svel = nullptr;
+        GZ_TRY(dalloc(reinterpret_cast<void**>(&t->svel),
+                      std::max<size_t>(t->n_live, 1) * sizeof(double4)));
+        if (t->n_live)
+            k_gather_vel<<<grid_for(t->n_live), kThreads, 0, st>>>(t->orderbuf, t->n_live,
+                                                                   opts->vx, opts->vy, opts->vz,
+                                                                   t->svel);
+        GZ_HIP(hipGetLastError());
         nv.vx = opts->vx;
         nv.vy = opts->vy;
         nv.vz = opts->vz;

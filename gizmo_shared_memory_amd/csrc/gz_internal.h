@@ -5,6 +5,7 @@
 
 #include <cstdint>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "gz_grav.h"
@@ -91,6 +92,7 @@ struct WalkArgs {
     int gstack_cap;
     int* spill;             // persistent launches: spill_cap stack entries per wave, or NULL
     int spill_cap;
+    int waves_per_cu;       // persistent launches: wavefronts per CU (0 = the most, 20)
     int stack_limit;        // > 0: use at most this many LDS stack entries (exercises the spill
                             // tier, and the retry when below the 512 entries one push can need)
     double G;
@@ -196,7 +198,9 @@ struct gz_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
     bool own_stream = false;
-    gz::DevMem scratch[12];
+    gz::DevMem scratch[13];
+    // the walk autotuner's choices, one per workload key (gz_walk_autotune)
+    std::vector<std::pair<std::vector<long long>, gz_walk_opts>> tuned;
     hipEvent_t ev[16] = {};
     // the walk's node-stack spill slices: allocated once, at a fixed size, on the first walk, so
     // that captured graphs can keep the pointer (walks on one context never overlap)
@@ -214,6 +218,7 @@ gz_status launch_kick_nodes(gz_tree* t, const uint32_t* active, size_t na, const
                             const double* vy, const double* vz, const double* m, double* vlx,
                             double* vly, double* vlz, double tau, int* status, hipStream_t st);
 gz_status launch_refresh(gz_tree* t, double tau, hipStream_t st);
+gz_status launch_tree_drift(gz_tree* t, double dt, hipStream_t st);
 gz_status launch_node_vel(const gz_tree* t, double* out, hipStream_t st);
 gz_status launch_raise_soft(gz_tree* t, const uint32_t* idx, size_t n_idx, const double* soft,
                             int* status, hipStream_t st);
@@ -233,6 +238,7 @@ struct gz_walk_plan {
     int ngroups = 0;
     int precision = 0;
     int persistent = 1;
+    int stack_limit = 0, waves_per_cu = 0;
     gz_model* model = nullptr;
     int gstack_cap = 0, retry_waves = 0;
     void* mem = nullptr; // gstart, counters, retry list, global stacks, learned group scales
@@ -303,6 +309,12 @@ struct gz_tree {
     float* xpad = nullptr;
     bool has_vel = false;
     double *vcom = nullptr, *m0 = nullptr, *dp = nullptr, *dq = nullptr, *dmass = nullptr;
+    // with build-time velocities: each sorted slot's velocity (vx, vy, vz, 0), as the build and
+    // the kick feed last saw it, from which gz_tree_drift drifts the sorted copy
+    double4* svel = nullptr;
+    // the tree's age on the device (t_since_build, tree.h:250), zero at the build and advanced
+    // by gz_tree_drift, for refreshes and kick feeds enqueued without the host (tau < 0)
+    double* age = nullptr;
     // largest gas kernel radius below each node (tree.h:269-272), set by gz_tree_init_hmax
     float* hmax = nullptr;
 };

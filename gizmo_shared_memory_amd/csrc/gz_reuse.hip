@@ -8,6 +8,7 @@
 // here one thread per particle does the same climb with device atomics. Maxima (vmax, softening)
 // come out the same in any order; the momentum sums are FP64 atomic additions, whose rounding
 // depends on the order, as the reference's OpenMP atomics do.
+#include <algorithm>
 #include <cmath>
 
 #include "gz_internal.h"
@@ -39,7 +40,9 @@ __global__ __launch_bounds__(256) void k_kick_nodes(
     const double* __restrict__ vx, const double* __restrict__ vy, const double* __restrict__ vz,
     const double* __restrict__ m, double* __restrict__ vlx, double* __restrict__ vly,
     double* __restrict__ vlz, double tau, uint32_t* __restrict__ vmax, double* __restrict__ dp,
-    double* __restrict__ dq, size_t nn, int* __restrict__ status) {
+    double* __restrict__ dq, size_t nn, const uint32_t* __restrict__ rank,
+    double4* __restrict__ svel, const double* __restrict__ age, int* __restrict__ status) {
+    if (age) tau = *age; // the tree's own age, kept on the device
     const size_t k = (size_t)blockIdx.x * 256 + threadIdx.x;
     const int lane = threadIdx.x & 63;
     int leaf = -1;
@@ -52,6 +55,7 @@ __global__ __launch_bounds__(256) void k_kick_nodes(
     double ux = 0, uy = 0, uz = 0;
     if (leaf >= 0) {
         ux = vx[i]; uy = vy[i]; uz = vz[i];
+        if (svel) svel[rank[i]] = make_double4(ux, uy, uz, 0.0); // the sorted velocity copy
         const float speed = (float)sqrt(ux * ux + uy * uy + uz * uz);
         const uint32_t want = __float_as_uint(speed);
         for (int no = leaf; no >= 0; no = parent[no])
@@ -116,9 +120,11 @@ __global__ __launch_bounds__(256) void k_refresh(int nn, WNode* __restrict__ wn,
                                                  const double* __restrict__ m0,
                                                  const double* __restrict__ dp,
                                                  const double* __restrict__ dq,
-                                                 const double* __restrict__ dmass, double tau) {
+                                                 const double* __restrict__ dmass, double tau,
+                                                 const double* __restrict__ age) {
     const int i = blockIdx.x * 256 + threadIdx.x;
     if (i >= nn) return;
+    if (age) tau = *age; // the tree's own age, kept on the device
     const size_t n = (size_t)nn;
     WNode w = wn[i];
     if (m0) {
@@ -206,7 +212,33 @@ gz_status launch_kick_nodes(gz_tree* t, const uint32_t* active, size_t na, const
     k_kick_nodes<<<(unsigned)((na + 255) / 256), 256, 0, st>>>(
         active, na, t->n, t->leaf_of, t->parent, t->level_begin_dev, vx, vy, vz, m, vlx, vly,
         vlz, tau,
-        reinterpret_cast<uint32_t*>(t->vmax), t->dp, t->dq, (size_t)t->nnodes, status);
+        reinterpret_cast<uint32_t*>(t->vmax), t->dp, t->dq, (size_t)t->nnodes, t->rank, t->svel,
+        tau < 0.0 ? t->age : nullptr, status);
+    GZ_HIP(hipGetLastError());
+    return GZ_OK;
+}
+
+// The drift applied to the tree's sorted copy of the sources, from the sorted velocity copy: for
+// every slot, x += v dt with the same operands (and so the same rounding) as gz_drift followed by
+// a regather, but streamed in slot order instead of gathered through orderbuf.
+__global__ __launch_bounds__(256) void k_tree_drift(size_t nl, double dt,
+                                                    const double4* __restrict__ svel,
+                                                    double4* __restrict__ src,
+                                                    double* __restrict__ age) {
+    const size_t s = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (s == 0) *age += dt; // read by later kernels on the stream, never by this one
+    if (s >= nl) return;
+    const double4 v = svel[s];
+    double4 q = src[s];
+    q.x = q.x + v.x * dt;
+    q.y = q.y + v.y * dt;
+    q.z = q.z + v.z * dt;
+    src[s] = q;
+}
+
+gz_status launch_tree_drift(gz_tree* t, double dt, hipStream_t st) {
+    k_tree_drift<<<(unsigned)((std::max<size_t>(t->n_live, 1) + 255) / 256), 256, 0, st>>>(
+        t->n_live, dt, t->svel, t->src, t->age);
     GZ_HIP(hipGetLastError());
     return GZ_OK;
 }
@@ -215,7 +247,7 @@ gz_status launch_refresh(gz_tree* t, double tau, hipStream_t st) {
     if (t->nnodes == 0) return GZ_OK;
     k_refresh<<<(unsigned)((t->nnodes + 255) / 256), 256, 0, st>>>(
         t->nnodes, t->wn, t->size, t->delta, t->vmax, t->xpad, t->vcom, t->m0, t->dp, t->dq,
-        t->dmass, tau);
+        t->dmass, tau, tau < 0.0 ? t->age : nullptr);
     GZ_HIP(hipGetLastError());
     return GZ_OK;
 }

@@ -1168,7 +1168,8 @@ gz_status launch_walk(const WalkArgs& a, bool fp64, bool count, hipStream_t st) 
     if (a.persistent) {
         int waves = 0;
         GZ_TRY(persistent_waves(&waves));
-        grid = (unsigned)std::min<long long>(waves, a.ngroups);
+        if (a.waves_per_cu > 0 && a.waves_per_cu < 20) waves = waves / 20 * a.waves_per_cu;
+        grid = (unsigned)std::min<long long>(std::max(waves, 1), a.ngroups);
     } else {
         grid = (unsigned)a.ngroups;
     }

@@ -201,6 +201,14 @@ class Tree:
                 "gz_tree_kick_nodes")
         self.ctx.synchronize()
 
+    def drift(self, dt: float):
+        """gz_tree_drift: the sorted copy of the sources drifted by its sorted velocities (a tree
+        built with vel=, kicks reported through kick_nodes)."""
+        self.ctx.enter()
+        L.check(L.load().gz_tree_drift(self.ctx.handle, self.handle, float(dt)),
+                "gz_tree_drift")
+        self.ctx.synchronize()
+
     def refresh_walk_nodes(self, tau: float):
         """Tree::refresh_walk_nodes (tree.cc:260-274)."""
         self.ctx.enter()
@@ -395,7 +403,8 @@ def group_features(T: Tree, P: Particles, targets, aold, group_size: int = 0):
 def accel_grouped(T: Tree, P: Particles, targets, theta: float, G: float = 1.0, batch: int = 0,
                   aold=None, *, precision: str = "fp64", group_size: int = 0,
                   persistent: bool = True, counts: bool = False, timing: bool = False,
-                  stack_limit: int = 0, model: Model | None = None, out=None):
+                  stack_limit: int = 0, model: Model | None = None, out=None,
+                  waves_per_cu: int = 0):
     """shmem::accel_grouped (tree.h:428-440) without tidal/jerk/lazy/sink terms.
 
     precision "fp64" is the parity walk; "fp32" evaluates interactions in FP32 on positions
@@ -415,7 +424,8 @@ def accel_grouped(T: Tree, P: Particles, targets, theta: float, G: float = 1.0, 
     crit = L.CRIT_REFERENCE if model is None else L.CRIT_LEARNED
     launch = L.LAUNCH_QUEUE if persistent else L.LAUNCH_PER_GROUP
     opts = L.gz_walk_opts(prec, crit, int(group_size), launch, int(counts),
-                          int(stack_limit), None if model is None else model.handle)
+                          int(stack_limit), None if model is None else model.handle,
+                          int(waves_per_cu))
     cnt = L.gz_walk_counts()
     want = counts or timing
     T.ctx.enter()
@@ -429,20 +439,50 @@ def accel_grouped(T: Tree, P: Particles, targets, theta: float, G: float = 1.0, 
     return ax, ay, az
 
 
+def autotune(T: Tree, P: Particles, targets, theta: float, G: float = 1.0, aold=None, *,
+             precision: str = "fp32", model: Model | None = None, reps: int = 2,
+             retune: bool = False):
+    """gz_walk_autotune: the fastest launch parameters for this workload, timed on the device
+    (group size, then node-stack depth and persistent launch width) and cached per workload in
+    the context. Returns (choice, report): choice holds group_size, stack_limit and waves_per_cu,
+    the keyword arguments of accel_grouped and WalkPlan; report lists the timed candidates."""
+    tg = _dev_u32(targets, P.device)
+    nt = int(tg.shape[0])
+    ao = None if aold is None else _dev_f64(aold, P.device)
+    prec = {"fp64": L.PREC_FP64, "fp32": L.PREC_FP32_REL}[precision]
+    crit = L.CRIT_REFERENCE if model is None else L.CRIT_LEARNED
+    base = L.gz_walk_opts(prec, crit, 0, L.LAUNCH_QUEUE, 0, 0,
+                          None if model is None else model.handle, 0)
+    best = L.gz_walk_opts()
+    rep = L.gz_tune_report()
+    T.ctx.enter()
+    L.check(L.load().gz_walk_autotune(T.ctx.handle, T.handle, P.cstruct(), _ptr(tg), nt,
+                                      float(theta), float(G), _ptr(ao), C.byref(base), int(reps),
+                                      L.TUNE_RETUNE if retune else 0, C.byref(best),
+                                      C.byref(rep)), "gz_walk_autotune")
+    choice = {"group_size": best.group_size, "stack_limit": best.stack_limit,
+              "waves_per_cu": best.waves_per_cu}
+    pts = [{"group_size": rep.points[k].group_size, "stack_limit": rep.points[k].stack_limit,
+            "waves_per_cu": rep.points[k].waves_per_cu, "ms": rep.points[k].ms}
+           for k in range(rep.n_points)]
+    return choice, {"from_cache": bool(rep.from_cache), "points": pts, "best_ms": rep.best.ms}
+
+
 class WalkPlan:
     """Groups of a fixed target list formed once (gz_walk_plan); launch() only enqueues the walk,
     so launches can be captured into a graph. segments: slot offsets (length n_seg + 1) of
     independent target lists walked together, e.g. the active sets of several substeps."""
 
     def __init__(self, T: Tree, P: Particles, targets, *, segments=None, precision="fp32",
-                 group_size=32, model: Model | None = None):
+                 group_size=32, model: Model | None = None, stack_limit: int = 0,
+                 waves_per_cu: int = 0):
         self.T, self.P = T, P
         self.targets = _dev_u32(targets, P.device)
         nt = int(self.targets.shape[0])
         prec = {"fp64": L.PREC_FP64, "fp32": L.PREC_FP32_REL}[precision]
         crit = L.CRIT_REFERENCE if model is None else L.CRIT_LEARNED
-        opts = L.gz_walk_opts(prec, crit, int(group_size), L.LAUNCH_QUEUE, 0, 0,
-                              None if model is None else model.handle)
+        opts = L.gz_walk_opts(prec, crit, int(group_size), L.LAUNCH_QUEUE, 0, int(stack_limit),
+                              None if model is None else model.handle, int(waves_per_cu))
         self.model = model
         seg = None
         nseg = 0

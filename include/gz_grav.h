@@ -131,15 +131,29 @@ gz_status gz_tree_update_sources(gz_ctx* ctx, gz_tree* tree, const gz_particles*
  * ancestor's speed bound to |v| (Tree::raise_vmax, tree.cc:276-285) and, when v_last is given,
  * add m (v - v_last) and its product with tau to every ancestor's incremental momentum and first
  * moment (Tree::kick_node, tree.cc:211-227), then set v_last = v. The momentum feed needs a tree
- * built with velocities; without them only the speed bound is raised. v_last may be NULL. */
+ * built with velocities; without them only the speed bound is raised. v_last may be NULL. A tree
+ * built with velocities also records v in its sorted velocity copy (for gz_tree_drift); tau < 0
+ * takes the tree's age on the device. Under
+ * hipGraph capture the call enqueues its kernel only: an index outside the particle set is
+ * skipped but not reported. */
 gz_status gz_tree_kick_nodes(gz_ctx* ctx, gz_tree* tree, const uint32_t* active, size_t n_active,
                              const double* vx, const double* vy, const double* vz,
                              const double* m, double* vlast_x, double* vlast_y, double* vlast_z,
                              double tau);
+/* The drift of the tree's own sorted copy of the sources: every slot's position += v dt, with v the
+ * particle's velocity as the build (with velocities) and gz_tree_kick_nodes last saw it. After
+ * gz_drift with the same dt and velocities it leaves the sorted copy exactly as
+ * gz_tree_update_sources would (the same operations on the same operands), but it streams the
+ * copy in slot order instead of gathering every particle through orderbuf. Needs a tree built with
+ * velocities, and velocity changes reported through gz_tree_kick_nodes. It also advances the
+ * tree's age kept on the device (zero at the build), which gz_tree_refresh_walk_nodes and
+ * gz_tree_kick_nodes use when passed tau < 0: a sequence of steps enqueued without the host (a
+ * captured graph) then refreshes at the right age on every replay. */
+gz_status gz_tree_drift(gz_ctx* ctx, gz_tree* tree, double dt);
 /* Tree::refresh_walk_nodes (tree.cc:260-274): every walk node's mass and centre of mass at tau
  * (time since the build) from the incremental moments, and its opening radius and side widened
  * by 2 (vmax tau + xpad). The identity on mass and centre of mass for a tree built without
- * velocities. */
+ * velocities. tau < 0: the tree's age on the device (see gz_tree_drift). */
 gz_status gz_tree_refresh_walk_nodes(gz_ctx* ctx, gz_tree* tree, double tau);
 /* Tree::raise_soft (tree.cc:311-319) for the particles idx (device, n): raise every ancestor's
  * softening bound to soft[i] (soft: device, indexed by particle). */
@@ -178,6 +192,8 @@ typedef struct {
     int stack_limit; /* 0 = default. > 0 caps the on-chip node stack at this many entries, so
                         groups spill to the global-memory re-walk (for tests of that path)    */
     const void* model; /* gz_model* when criterion == GZ_CRIT_LEARNED                          */
+    int waves_per_cu;  /* persistent launches: wavefronts per compute unit, 1..20 (0 = 20, the
+                          most the walk's registers and LDS allow)                              */
 } gz_walk_opts;
 
 /* SHMEM_COUNT_WALK (tree.cc:671-674, :891-895): nodes tested, multipole interactions, pair
@@ -229,6 +245,35 @@ gz_status gz_graph_begin(gz_ctx* ctx);
 gz_status gz_graph_end(gz_ctx* ctx, gz_graph** out);
 gz_status gz_graph_launch(gz_ctx* ctx, gz_graph* graph);
 gz_status gz_graph_free(gz_graph* graph);
+
+/* ---- autotuning the walk ---------------------------------------------------------------------
+ * No counterpart in the reference, whose batch size is a measured constant (tree.h:411-413).
+ * Times gz_tree_accel_grouped (grouping and walk kernels, HIP events) for candidate launch
+ * parameters on the caller's own workload and returns the fastest as *best (base with group_size,
+ * stack_limit and waves_per_cu replaced): target-group sizes first (0 and 16/32/64 for FP64,
+ * 16/32/64 for FP32), then, at the best of those, the on-chip node-stack depth (stack_limit
+ * 512/640/768 entries: a shallower stack spills to global memory sooner) and the wavefronts per
+ * compute unit of the persistent launch (10/15/20). Every candidate is walked once untimed and then `reps` times (0 = 2).
+ * The choice is cached in the context per workload (precision, criterion and model, theta = 0 or
+ * not, and the power-of-two sizes of the target list, of the tree and of its node count); a later
+ * call for the same workload returns it without timing unless flags & GZ_TUNE_RETUNE. The walk's
+ * outputs go to scratch: the caller's arrays are not written. report may be NULL. */
+#define GZ_TUNE_RETUNE 1
+#define GZ_TUNE_MAX_POINTS 16
+typedef struct {
+    int group_size, stack_limit, waves_per_cu;
+    double ms; /* mean of the timed walks, grouping included */
+} gz_tune_point;
+typedef struct {
+    int n_points;   /* candidates timed this call (0 when the choice came from the cache) */
+    int from_cache; /* 1: the cached choice for this workload was returned                 */
+    gz_tune_point points[GZ_TUNE_MAX_POINTS];
+    gz_tune_point best;
+} gz_tune_report;
+gz_status gz_walk_autotune(gz_ctx* ctx, const gz_tree* tree, const gz_particles* p,
+                           const uint32_t* targets, size_t n_t, double theta, double G,
+                           const double* aold, const gz_walk_opts* base, int reps, int flags,
+                           gz_walk_opts* best, gz_tune_report* report);
 
 /* shmem::accel (tree.h:392-394, tree.cc:528-582): per-target walk, a reached leaf is always
  * direct-summed. */

@@ -182,7 +182,7 @@ int main() {
     expect(e < 1e-12, fmt("walk on drifted particles equals the oracle's", e));
 
     // the FP32 group-relative walk through the same call
-    gz_walk_opts wo{GZ_PREC_FP32_REL, GZ_CRIT_REFERENCE, 32, GZ_LAUNCH_QUEUE, 0, 0, nullptr};
+    gz_walk_opts wo{GZ_PREC_FP32_REL, GZ_CRIT_REFERENCE, 32, GZ_LAUNCH_QUEUE, 0, 0, nullptr, 0};
     gz_shmem::set_walk_options(&wo);
     gz_shmem::accel_grouped(T, P, tg, 0.0, G, 0, ax, ay, az, nullptr, aold.data());
     wo.precision = GZ_PREC_FP64;
