@@ -1,9 +1,11 @@
 """The pieces of an integrator cycle enqueued without the host, and the walk autotuner.
 
-* gz_tree_drift on a tree built with velocities, after kicks fed through gz_tree_kick_nodes,
+* gz_tree_drift on a tree built with velocities, after kicks reported through gz_tree_kick_nodes,
   leaves the sorted copy exactly as gz_drift + gz_tree_update_sources do: the walks on the two
-  trees are bit for bit the same; the tree's device age equals the summed drifts, and a refresh
-  at tau < 0 (the device age) equals one at that tau;
+  trees are bit for bit the same (the kick feed here raises speed bounds only: the momentum feed's
+  FP64 atomic sums round in the order the atomics land, which would make the two trees' refreshed
+  centres of mass differ in their last bits); the tree's device age equals the summed drifts, and a
+  refresh at tau < 0 (the device age) equals one at that tau on the same tree;
 * a cycle of 16 substeps on a reused tree (drift, tree drift, refresh, aold, walk plan, write-back,
   kick, kick feed) captured into one hipGraph: a replay from a saved state gives what the same
   cycle enqueued eagerly gives from that state (to the rounding of the FP64 atomic sums in the
@@ -62,13 +64,17 @@ def test_tree_drift_equals_drift_and_regather(gz):
             ai = torch.as_tensor(act.astype(np.int64), device="cuda:0")
             for c, d in zip(vd, dv):
                 c[ai] += torch.as_tensor(d, device="cuda:0")
-            T.kick_nodes(act, vd, vlast=vl, tau=tau)
+            T.kick_nodes(act, vd, tau=tau)
         (P0, T0, _, _), (P1, T1, _, _) = trees
         T0.refresh_walk_nodes(tau)
-        T1.refresh_walk_nodes(-1.0)  # the device age, advanced by the tree drifts
+        T1.refresh_walk_nodes(tau)
         d0, d1 = T0.download(), T1.download()
         for f in ("mass", "cx", "cy", "cz", "wn_s", "wn_len"):
             assert np.array_equal(d0[f], d1[f]), f"step {step}: refreshed {f} differs"
+        T1.refresh_walk_nodes(-1.0)  # the device age, advanced by the tree drifts
+        d2 = T1.download()
+        for f in ("mass", "cx", "cy", "cz", "wn_s", "wn_len"):
+            assert np.array_equal(d1[f], d2[f]), f"step {step}: the device age differs from tau"
         tg = T0.device_orderbuf()
         a0 = gz.accel_grouped(T0, P0, tg, 0.7, 1.0, 0, precision="fp32", group_size=32)
         a1 = gz.accel_grouped(T1, P1, tg, 0.7, 1.0, 0, precision="fp32", group_size=32)
