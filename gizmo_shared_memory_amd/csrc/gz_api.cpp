@@ -214,6 +214,7 @@ gz_status walk(gz_ctx* ctx, const gz_tree* tree, const gz_particles* p, const ui
     a.wn = tree->wn;
     a.src = tree->src;
     a.src_soft = tree->src_soft;
+    a.node_soft = tree->soft;
     a.targets = targets;
     a.tslot = tslot;
     a.x = p->x; a.y = p->y; a.z = p->z; a.soft = p->soft;
@@ -733,7 +734,7 @@ gz_status gz_walk_autotune(gz_ctx* ctx, const gz_tree* tree, const gz_particles*
     // 2. the on-chip node-stack depth, then 3. the persistent launch width (queue launches)
     if (cur.launch == GZ_LAUNCH_QUEUE) {
         int best_sl = 0;
-        for (int sl : {512, 640}) {
+        for (int sl : {384, 448}) {
             gz_walk_opts o = cur;
             o.stack_limit = sl;
             double ms = 0;
@@ -913,6 +914,7 @@ gz_status gz_walk_plan_launch(gz_ctx* ctx, gz_walk_plan* P, double theta, double
     a.wn = tree->wn;
     a.src = tree->src;
     a.src_soft = tree->src_soft;
+    a.node_soft = tree->soft;
     a.targets = P->targets;
     a.tslot = P->tslot;
     a.x = P->p.x; a.y = P->p.y; a.z = P->p.z; a.soft = P->p.soft;

@@ -56,6 +56,7 @@ struct WalkArgs {
     const WNode* wn;
     const double4* src;     // (x, y, z, m) per sorted slot
     const double* src_soft; // softening per sorted slot
+    const double* node_soft; // FP64 largest softening below each node
     const uint32_t* targets;
     const int* tslot;        // sorted slot of each target (its data in src), -1 if not in the tree
     const double *x, *y, *z, *soft; // target data by particle, for targets outside the tree

@@ -31,7 +31,7 @@
 
 namespace gz {
 
-constexpr int kWalkStack = 768;   // node ids per wave in LDS
+constexpr int kWalkStack = 512;   // node ids per wave in LDS
 constexpr int kSpillKeep = 256;   // LDS entries kept when the bottom of the stack is spilled
 constexpr int kXcdChunk = 64;     // groups per chunk of the persistent launch's XCD interleave
 constexpr int kMonoF = 120;       // monopole entries per wave, FP32 (16 B each)
@@ -40,6 +40,11 @@ constexpr int kMonoD = 96;        // monopole entries per wave, FP64 (32 B each)
 constexpr int kPairD = 64;        // particle entries per wave, FP64 (40 B each)
 constexpr int kMaxChunk = 64;     // targets evaluated together (one wave)
 constexpr int kPadF = 16;         // FP32 lists: zero-mass entries after the last (two slice steps)
+// A batch appends up to 64 entries to a list right after flushing it (64 accepted nodes, or 64
+// leaf particles), so every list holds at least 64; the stack takes one batch's 512 children.
+static_assert(kMonoF >= 64 && kPairF >= 64 && kMonoD >= 64 && kPairD >= 64,
+              "a list must take one batch of 64 entries after a flush");
+static_assert(kWalkStack >= 512, "the LDS stack must take the 512 children of one batch");
 
 // std::min / std::max of two numbers that are never NaN: one v_min_f64 / v_max_f64 (a compare
 // and two selects otherwise); the two forms differ only in the sign of a zero result.
@@ -340,9 +345,8 @@ __device__ __forceinline__ int lanes_below(uint64_t m) {
 // given r^2, inv = 1/r and g3 = 1/r^3 from the Newtonian path, and h^2. Both branches are
 // evaluated and one selected, with the hardware 1/sqrt(h^2) in place of the division; the outer
 // branch's -1/(15 u^3) term times h^-3 is -1/(15 r^3) = -g3/15, so it needs no second reciprocal.
-__device__ __forceinline__ float spline_inside_f(float r2, float inv, float g3, float h2) {
-    const float hinv = __builtin_amdgcn_rsqf(h2);
-    const float hinv3 = hinv * hinv * hinv;
+__device__ __forceinline__ float spline_inside_h(float r2, float inv, float g3, float hinv,
+                                                 float hinv3) {
     const float u = (r2 * inv) * hinv;
     const float u2 = u * u;
     // 32/3 + u^2 (32 u - 38.4) and 64/3 - 48 u + 38.4 u^2 - 32/3 u^3, in Horner form
@@ -352,6 +356,13 @@ __device__ __forceinline__ float spline_inside_f(float r2, float inv, float g3, 
         21.333333333333332f);
     const bool inner = u < 0.5f;
     return __builtin_fmaf(-0.06666666666666667f, inner ? 0.0f : g3, hinv3 * (inner ? lo : hi));
+}
+
+// The same for a kernel extent h given as h^2 (hinv = 1/h from the hardware reciprocal square
+// root, hinv3 = hinv^3).
+__device__ __forceinline__ float spline_inside_f(float r2, float inv, float g3, float h2) {
+    const float hinv = __builtin_amdgcn_rsqf(h2);
+    return spline_inside_h(r2, inv, g3, hinv, hinv * hinv * hinv);
 }
 
 // The FP32 walk runs five waves per SIMD: at most 96 VGPRs (the wave-uniform group data lives in
@@ -484,9 +495,11 @@ __global__ __launch_bounds__(64, TJ ? 3 : (FP64 ? 4 : 5)) void k_walk(WalkArgs a
             const bool tvalid = sub < L;
             const int jstart = tvalid ? sub : (1 << 30);
             real tx = 0, ty = 0, tz = 0, te = 0;
+            double pe_t = 1e300; // the target's softening (FP64), for the uniform-kernel test
             if (tvalid) {
                 double px, py, pz, pe;
                 target_data(a, tlo + t, px, py, pz, pe);
+                pe_t = pe;
                 if (FP64) {
                     tx = (real)px; ty = (real)py; tz = (real)pz;
                 } else {
@@ -503,6 +516,11 @@ __global__ __launch_bounds__(64, TJ ? 3 : (FP64 ? 4 : 5)) void k_walk(WalkArgs a
             }
             int mcount = 0, pcount = 0;
             bool overflow = false;
+            // FP32: when no particle below the group's root is softer than the chunk's least
+            // softened target, every pair's kernel extent max(h_t, h_s) is the target's own, so
+            // the particle loop takes h = h_t and its reciprocal powers once per lane
+            const int groot = root_of(a.targets[gi.tlo], a.cluster_start, a.nclusters, a.root);
+            const bool uni = !FP64 && a.node_soft[groot] <= wave_min_d(pe_t);
 
             // Evaluate both lists against every target of the chunk.
             auto flush = [&]() {
@@ -645,9 +663,50 @@ __global__ __launch_bounds__(64, TJ ? 3 : (FP64 ? 4 : 5)) void k_walk(WalkArgs a
                         fy = __builtin_fmaf(dy, f, fy);
                         fz = __builtin_fmaf(dz, f, fz);
                     };
+                    // uniform kernel extent: h = the target's, 1/h and 1/h^3 once per lane
+                    const float hinvt = __builtin_amdgcn_rsqf(te2);
+                    const float hinv3t = hinvt * hinvt * hinvt;
+                    auto pair_u = [&](const real4 s, int j) {
+                        const float dx = (float)s.x - txf, dy = (float)s.y - tyf,
+                                    dz = (float)s.z - tzf;
+                        const float r2 = __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, dz * dz));
+                        const float inv = __builtin_amdgcn_rsqf(r2);
+                        float g3 = inv * inv * inv;
+                        if (r2 <= te2) {
+                            const float sp = spline_inside_h(r2, inv, g3, hinvt, hinv3t);
+                            g3 = (r2 > 0.f) ? sp : 0.f;
+                        }
+                        if (COUNT && tvalid && j < pcount && r2 > 0.f) ++g_pair;
+                        const float f = (float)s.w * g3;
+                        fx = __builtin_fmaf(dx, f, fx);
+                        fy = __builtin_fmaf(dy, f, fy);
+                        fz = __builtin_fmaf(dz, f, fz);
+                    };
                     // two slice steps per iteration, so the LDS reads of two entries overlap;
                     // the padding covers the second step past the end
-                    if (L == 1) {
+                    if (uni && L == 1) {
+                        for (int j0 = 0; j0 < mcount; j0 += 2) {
+                            const real4 s0 = s_mono[j0], s1 = s_mono[j0 + 1];
+                            mono(s0);
+                            mono(s1);
+                        }
+                        for (int j0 = 0; j0 < pcount; j0 += 2) {
+                            const real4 s0 = s_pair[j0], s1 = s_pair[j0 + 1];
+                            pair_u(s0, j0);
+                            pair_u(s1, j0 + 1);
+                        }
+                    } else if (uni) {
+                        for (int j0 = 0; j0 < mcount; j0 += 2 * L) {
+                            const real4 s0 = s_mono[j0 + js], s1 = s_mono[j0 + L + js];
+                            mono(s0);
+                            mono(s1);
+                        }
+                        for (int j0 = 0; j0 < pcount; j0 += 2 * L) {
+                            const int j1 = j0 + L;
+                            pair_u(s_pair[j0 + js], j0 + js);
+                            pair_u(s_pair[j1 + js], j1 + js);
+                        }
+                    } else if (L == 1) {
                         // one slice (more than 32 targets): every lane reads the same entries,
                         // the second at a fixed offset from the first
                         for (int j0 = 0; j0 < mcount; j0 += 2) {
@@ -682,14 +741,12 @@ __global__ __launch_bounds__(64, TJ ? 3 : (FP64 ? 4 : 5)) void k_walk(WalkArgs a
                 pcount = 0;
             };
 
-            // Node batches are loaded one iteration ahead: the loads for the next batch are in
-            // flight while this batch's leaves are copied and the list is evaluated.
+            // Node batches: the next batch is popped from the stack while this batch's lists
+            // are built, and its nodes are loaded once they are.
             int sp = 0;
             int gsp = 0; // entries below the LDS stack, in the spill slice
             int k = 1;
-            int node = (lane == 0)
-                           ? root_of(a.targets[gi.tlo], a.cluster_start, a.nclusters, a.root)
-                           : -1;
+            int node = (lane == 0) ? groot : -1;
             WNode w;
             if (node >= 0) w = a.wn[node];
             while (k > 0 && !overflow) {
@@ -779,8 +836,6 @@ __global__ __launch_bounds__(64, TJ ? 3 : (FP64 ? 4 : 5)) void k_walk(WalkArgs a
                 const int base = sp - k;
                 const int next_node = (lane < k) ? stk[base + lane] : -1;
                 sp = base;
-                WNode wn_next;
-                if (next_node >= 0) wn_next = a.wn[next_node];
 
                 const uint64_t mb = __ballot(accept);
                 const int nm = __popcll(mb);
@@ -898,8 +953,11 @@ __global__ __launch_bounds__(64, TJ ? 3 : (FP64 ? 4 : 5)) void k_walk(WalkArgs a
                     }
                 }
                 __syncthreads();
+                // the next batch's nodes are loaded here, after this batch's lists are built:
+                // loading them earlier keeps 16 more registers live through the list building
+                // and the flushes (measured 2% slower), and other waves hide the latency
                 node = next_node;
-                w = wn_next;
+                if (node >= 0) w = a.wn[node];
             }
             if (overflow) {
                 if (!GSTACK && lane == 0) {

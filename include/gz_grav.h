@@ -252,7 +252,7 @@ gz_status gz_graph_free(gz_graph* graph);
  * parameters on the caller's own workload and returns the fastest as *best (base with group_size,
  * stack_limit and waves_per_cu replaced): target-group sizes first (0 and 16/32/64 for FP64,
  * 16/32/64 for FP32), then, at the best of those, the on-chip node-stack depth (stack_limit
- * 512/640/768 entries: a shallower stack spills to global memory sooner) and the wavefronts per
+ * 384/448/512 entries: a shallower stack spills to global memory sooner) and the wavefronts per
  * compute unit of the persistent launch (10/15/20). Every candidate is walked once untimed and then `reps` times (0 = 2).
  * The choice is cached in the context per workload (precision, criterion and model, theta = 0 or
  * not, and the power-of-two sizes of the target list, of the tree and of its node count); a later
