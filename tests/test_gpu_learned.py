@@ -189,13 +189,13 @@ def test_learned_walk_tolerance_scales_the_reference_test(gz):
 
 
 def test_trained_network_against_the_method_without_it(gz):
-    """What the trained criterion (learned_criterion.json) gains on a 1M-particle halo+disk, judged
-    by the FP64 direct sum on 8,192 probes: against the reference criterion (ErrTolForceAcc
-    0.0025) it does at least 30% fewer interactions at a p99 error under 1.5e-2; against one
-    global tolerance tuned to the same p99 (interpolated between the grid points that bracket
-    it) it is no worse than 3%. Measured: 37% fewer than the reference, 2% fewer than the
-    matched global tolerance, p99 1.16e-2 (8.5e-3 at 8M) -- the network adds little over a tuned
-    constant."""
+    """What the trained criterion (learned_criterion.json) gains on a 1M-particle halo+disk, a
+    distribution it never saw, judged by the FP64 direct sum on 8,192 probes: against the
+    reference criterion (ErrTolForceAcc 0.0025) it does at least 30% fewer interactions at a p99
+    error within 1e-2; against one global tolerance tuned to the same p99 (interpolated between
+    the grid points that bracket it) it is no worse than 3%. Measured: 34% fewer than the
+    reference, 3% fewer than the matched global tolerance, p99 9.3e-3 (6.7e-3 at 8M) -- the
+    network adds little over a tuned constant."""
     from gizmo_shared_memory_amd import ics, learned
     x, y, z, m, soft = ics.halo_disk(n=1 << 20)
     P = gz.Particles(x, y, z, m, soft)
@@ -219,7 +219,7 @@ def test_trained_network_against_the_method_without_it(gz):
     grid = [run(2.0 ** (k / 2.0)) for k in range(2, 12)]
     print(f"reference: {plain_n:.0f} interactions/target, p99 {plain_p99:.3g}; learned: "
           f"{lrn_n:.0f}, p99 {lrn_p99:.3g}; global grid {[(round(a), round(b, 5)) for a, b in grid]}")
-    assert lrn_p99 < 1.5e-2
+    assert lrn_p99 <= 1e-2
     assert lrn_n < 0.7 * plain_n
     lo = [g for g in grid if g[1] <= lrn_p99]
     hi = [g for g in grid if g[1] > lrn_p99]
