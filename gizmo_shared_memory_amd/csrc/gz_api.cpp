@@ -284,6 +284,7 @@ gz_status walk(gz_ctx* ctx, const gz_tree* tree, const gz_particles* p, const ui
         r.gstack = gs.as<int>();
         r.gstack_cap = cap;
         GZ_TRY(gz::launch_walk_retry(r, std::min(per_launch, retries), fp64, cnt, st));
+        GZ_HIP(hipEventRecord(ev[10], st)); // the walk's time includes the re-walk
         GZ_HIP(hipMemcpyAsync(host_misc, misc, sizeof host_misc, hipMemcpyDeviceToHost, st));
         GZ_HIP(hipStreamSynchronize(st));
         if (host_misc[6] & 1)
