@@ -134,6 +134,15 @@ _PROTOS = {
                                      C.c_void_p]),
     "gz_tree_download_reuse": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
                                          C.c_void_p]),
+    "gz_tree_mass_events": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t]
+                            + [C.c_void_p] * 7 + [C.c_double, C.c_double]),
+    "gz_tree_shift_nodes": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t]
+                            + [C.c_void_p] * 4),
+    "gz_tree_raise_pad": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t,
+                                    C.c_void_p]),
+    "gz_tree_raise_vmax": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t,
+                                     C.c_void_p]),
+    "gz_tree_download_pads": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
     "gz_tree_accel_grouped": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(gz_particles),
                                         C.c_void_p, C.c_size_t, C.c_double, C.c_double, C.c_int,
                                         C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,

@@ -542,6 +542,120 @@ gz_status gz_tree_raise_soft(gz_ctx* ctx, gz_tree* tree, const uint32_t* idx, si
     return GZ_OK;
 }
 
+namespace {
+// The per-call status word of the reuse kernels: cleared before the launch, read back after it
+// unless the stream is being captured (then the kernels still skip bad entries, unreported).
+struct ReuseStatus {
+    int* dev = nullptr;
+    gz_status begin(gz_ctx* ctx) {
+        gz::DevMem& sm = ctx->scratch[8];
+        GZ_TRY(sm.reserve(256));
+        dev = sm.as<int>();
+        return gz::fill_zero_u32(reinterpret_cast<uint32_t*>(dev), 1, ctx->stream);
+    }
+    gz_status end(gz_ctx* ctx, int* bits) {
+        *bits = 0;
+        hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
+        GZ_HIP(hipStreamIsCapturing(ctx->stream, &cap));
+        if (cap != hipStreamCaptureStatusNone) return GZ_OK;
+        GZ_HIP(hipMemcpyAsync(bits, dev, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+        GZ_HIP(hipStreamSynchronize(ctx->stream));
+        return GZ_OK;
+    }
+};
+
+gz_status node_events(const char* name, gz_ctx* ctx, gz_tree* tree, int kind,
+                      const uint32_t* idx, size_t n, const double* v, const double* x,
+                      const double* y, const double* z, const double* vx, const double* vy,
+                      const double* vz, double tau, double box) {
+    ReuseStatus rs;
+    GZ_TRY(rs.begin(ctx));
+    GZ_TRY(gz::launch_node_events(tree, kind, idx, n, v, x, y, z, vx, vy, vz, tau, box, rs.dev,
+                                  ctx->stream));
+    int bits = 0;
+    GZ_TRY(rs.end(ctx, &bits));
+    if (bits & 1)
+        return fail(GZ_ERR_INVALID, "%s: an index is outside the particle set", name);
+    return GZ_OK;
+}
+} // namespace
+
+gz_status gz_tree_mass_events(gz_ctx* ctx, gz_tree* tree, const uint32_t* idx, size_t n,
+                              const double* dm, const double* x, const double* y, const double* z,
+                              const double* vx, const double* vy, const double* vz, double tau,
+                              double box) {
+    gz::clear_error();
+    GZ_TRY(check_ctx(ctx));
+    if (!tree) return fail(GZ_ERR_INVALID, "gz_tree_mass_events: null tree");
+    if (n == 0 || !tree->has_vel) return GZ_OK; // the reference's no-op without vcom
+    if (!idx || !dm || !x || !y || !z || !vx || !vy || !vz)
+        return fail(GZ_ERR_INVALID, "gz_tree_mass_events: null array");
+    if (!std::isfinite(box)) return fail(GZ_ERR_INVALID, "gz_tree_mass_events: box is not finite");
+    return node_events("gz_tree_mass_events", ctx, tree, gz::kNodeEventMass, idx, n, dm, x, y, z,
+                       vx, vy, vz, tau, box);
+}
+
+gz_status gz_tree_shift_nodes(gz_ctx* ctx, gz_tree* tree, const uint32_t* idx, size_t n,
+                              const double* m, const double* dx, const double* dy,
+                              const double* dz) {
+    gz::clear_error();
+    GZ_TRY(check_ctx(ctx));
+    if (!tree) return fail(GZ_ERR_INVALID, "gz_tree_shift_nodes: null tree");
+    if (n == 0 || !tree->has_vel) return GZ_OK; // the reference's no-op without vcom
+    if (!idx || !m || !dx || !dy || !dz)
+        return fail(GZ_ERR_INVALID, "gz_tree_shift_nodes: null array");
+    return node_events("gz_tree_shift_nodes", ctx, tree, gz::kNodeEventShift, idx, n, m, dx, dy,
+                       dz, nullptr, nullptr, nullptr, 0.0, 0.0);
+}
+
+gz_status gz_tree_raise_pad(gz_ctx* ctx, gz_tree* tree, const uint32_t* idx, size_t n,
+                            const double* d) {
+    gz::clear_error();
+    GZ_TRY(check_ctx(ctx));
+    if (!tree) return fail(GZ_ERR_INVALID, "gz_tree_raise_pad: null tree");
+    if (n == 0) return GZ_OK;
+    if (!idx || !d) return fail(GZ_ERR_INVALID, "gz_tree_raise_pad: null array");
+    return node_events("gz_tree_raise_pad", ctx, tree, gz::kNodeEventPad, idx, n, d, nullptr,
+                       nullptr, nullptr, nullptr, nullptr, nullptr, 0.0, 0.0);
+}
+
+gz_status gz_tree_raise_vmax(gz_ctx* ctx, gz_tree* tree, const uint32_t* idx, size_t n,
+                             const float* speed) {
+    gz::clear_error();
+    GZ_TRY(check_ctx(ctx));
+    if (!tree) return fail(GZ_ERR_INVALID, "gz_tree_raise_vmax: null tree");
+    if (n == 0) return GZ_OK;
+    if (!idx || !speed) return fail(GZ_ERR_INVALID, "gz_tree_raise_vmax: null array");
+    ReuseStatus rs;
+    GZ_TRY(rs.begin(ctx));
+    GZ_TRY(gz::launch_raise_vmax(tree, idx, n, speed, rs.dev, ctx->stream));
+    int bits = 0;
+    GZ_TRY(rs.end(ctx, &bits));
+    if (bits & 1)
+        return fail(GZ_ERR_INVALID, "gz_tree_raise_vmax: an index is outside the particle set");
+    if (bits & 2) return fail(GZ_ERR_INVALID, "gz_tree_raise_vmax: a speed is negative or NaN");
+    return GZ_OK;
+}
+
+gz_status gz_tree_download_pads(const gz_tree* tree, float* xpad, double* dmass) {
+    if (!tree) return fail(GZ_ERR_INVALID, "gz_tree_download_pads: null tree");
+    GZ_TRY(check_ctx(tree->ctx));
+    hipStream_t st = tree->ctx->stream;
+    const size_t nn = (size_t)tree->nnodes;
+    if (nn == 0) return GZ_OK;
+    if (xpad)
+        GZ_HIP(hipMemcpyAsync(xpad, tree->xpad, nn * sizeof(float), hipMemcpyDeviceToHost, st));
+    if (dmass) {
+        if (tree->dmass)
+            GZ_HIP(hipMemcpyAsync(dmass, tree->dmass, nn * sizeof(double), hipMemcpyDeviceToHost,
+                                  st));
+        else
+            std::memset(dmass, 0, nn * sizeof(double));
+    }
+    GZ_HIP(hipStreamSynchronize(st));
+    return GZ_OK;
+}
+
 gz_status gz_tree_download_reuse(const gz_tree* tree, float* vmax, double* vcom, double* dp,
                                  double* dq) {
     if (!tree) return fail(GZ_ERR_INVALID, "gz_tree_download_reuse: null tree");

@@ -223,6 +223,16 @@ gz_status launch_tree_drift(gz_tree* t, double dt, hipStream_t st);
 gz_status launch_node_vel(const gz_tree* t, double* out, hipStream_t st);
 gz_status launch_raise_soft(gz_tree* t, const uint32_t* idx, size_t n_idx, const double* soft,
                             int* status, hipStream_t st);
+// status bit 1: a negative or NaN speed.
+gz_status launch_raise_vmax(gz_tree* t, const uint32_t* idx, size_t n_ev, const float* speed,
+                            int* status, hipStream_t st);
+// Mass events (v = dm, x/y/z the position, vx/vy/vz the velocity), shifts (v = m, x/y/z the
+// displacement) or pads (v = d), applied in the order of the events.
+enum { kNodeEventMass = 0, kNodeEventShift = 1, kNodeEventPad = 2 };
+gz_status launch_node_events(gz_tree* t, int kind, const uint32_t* idx, size_t n_ev,
+                             const double* v, const double* x, const double* y, const double* z,
+                             const double* vx, const double* vy, const double* vz, double tau,
+                             double box, int* status, hipStream_t st);
 
 // The spill slices of ctx (allocating them on first use) and their size per wave.
 gz_status ctx_spill(gz_ctx* ctx, int** spill, int* cap);

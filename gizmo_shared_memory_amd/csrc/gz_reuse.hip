@@ -1,13 +1,16 @@
 // Reusing a tree across steps instead of rebuilding it (shmem/tree.cc:211-319, tree.h:250-331):
 // the kick feed that raises each node's speed bound and accumulates its momentum change
 // (mfm.cc:3798-3811: Tree::raise_vmax and Tree::kick_node), the refresh of the walk nodes from the
-// incremental moments (Tree::refresh_walk_nodes), and raising a node's softening bound
-// (Tree::raise_soft).
+// incremental moments (Tree::refresh_walk_nodes), raising a node's softening and speed bounds
+// (Tree::raise_soft, Tree::raise_vmax), and the events that are neither a drift nor a kick
+// (Tree::mass_event, Tree::shift_node, Tree::raise_pad).
 //
 // The reference climbs the parent chain of each active particle's leaf with atomics on the CPU;
 // here one thread per particle does the same climb with device atomics. Maxima (vmax, softening)
 // come out the same in any order; the momentum sums are FP64 atomic additions, whose rounding
-// depends on the order, as the reference's OpenMP atomics do.
+// depends on the order, as the reference's OpenMP atomics do. The reference applies the other
+// events one at a time in serial loops; here they are applied in the caller's order too, so their
+// sums round as the reference's do.
 #include <algorithm>
 #include <cmath>
 
@@ -257,6 +260,197 @@ gz_status launch_raise_soft(gz_tree* t, const uint32_t* idx, size_t n_idx, const
     if (n_idx == 0 || t->nnodes == 0) return GZ_OK;
     k_raise_soft<<<(unsigned)((n_idx + 255) / 256), 256, 0, st>>>(
         idx, n_idx, t->n, t->leaf_of, t->parent, soft, t->wn, t->soft, status);
+    GZ_HIP(hipGetLastError());
+    return GZ_OK;
+}
+
+// raise_vmax (tree.cc:276-285) with the caller's speeds, one thread per event: the same climb as
+// the kick feed's, stopping at the first ancestor that already covers the speed. A maximum, so
+// the order of the events does not matter. Negative or NaN speeds are skipped and flagged.
+__global__ __launch_bounds__(256) void k_raise_vmax(const uint32_t* __restrict__ idx, size_t n_ev,
+                                                    size_t n, const int* __restrict__ leaf_of,
+                                                    const int* __restrict__ parent,
+                                                    const float* __restrict__ speed,
+                                                    uint32_t* __restrict__ vmax,
+                                                    int* __restrict__ status) {
+    const size_t k = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (k >= n_ev) return;
+    const uint32_t i = idx[k];
+    if (i >= n) {
+        atomicOr(status, 1);
+        return;
+    }
+    const float s = speed[k];
+    if (!(s >= 0.0f)) {
+        atomicOr(status, 2);
+        return;
+    }
+    const int leaf = leaf_of[i];
+    const uint32_t want = __float_as_uint(s);
+    for (int no = leaf; no >= 0; no = parent[no])
+        if (atomicMax(&vmax[no], want) >= want) break;
+}
+
+gz_status launch_raise_vmax(gz_tree* t, const uint32_t* idx, size_t n_ev, const float* speed,
+                            int* status, hipStream_t st) {
+    if (n_ev == 0 || t->nnodes == 0) return GZ_OK;
+    k_raise_vmax<<<(unsigned)((n_ev + 255) / 256), 256, 0, st>>>(
+        idx, n_ev, t->n, t->leaf_of, t->parent, speed, reinterpret_cast<uint32_t*>(t->vmax),
+        status);
+    GZ_HIP(hipGetLastError());
+    return GZ_OK;
+}
+
+// The minimum-image separation (shmem/vec.h:25-30): folded into [-box/2, box/2] when box > 0.
+__device__ __forceinline__ double min_image_sep(double d, double box) {
+    if (box > 0) {
+        if (d > 0.5 * box) d -= box;
+        else if (d < -0.5 * box) d += box;
+    }
+    return d;
+}
+
+// Events between rebuilds that are not a drift or a kick, in the caller's order (the reference
+// calls them one at a time from serial loops): Tree::mass_event (tree.cc:239-251),
+// Tree::shift_node (tree.cc:253-258) and Tree::raise_pad (tree.cc:323-330). Every event climbs
+// from its leaf to the root, and a mass event reads each ancestor's current centre of mass (to
+// unwrap its position), so the events at one node must be applied in order. One wave does it:
+// lane L owns the ancestor at level L (node ids are level-major, so a node's level is its depth),
+// which makes every node's updates those of the serial loop, in the same order and with the same
+// operations, while the ~20 levels of one event proceed together. Per 64 events, each lane first
+// climbs one event's parent chain into LDS.
+struct NodeEventArgs {
+    const uint32_t* idx;
+    size_t n_ev, n, nn;
+    const int* leaf_of;
+    const int* parent;
+    const int* level_begin;
+    const double* a;                      // dm (mass event), m (shift), d (pad)
+    const double *x, *y, *z;              // position (mass event) or displacement (shift)
+    const double *vx, *vy, *vz;           // velocity (mass event)
+    double tau, box;
+    const double* age;                    // the tree's age on the device, used when non-null
+    const double* m0;                     // build moments: mass, then COM x, y, z (stride nn)
+    const double* vcom;                   // build COM velocity (stride nn)
+    double *dp, *dq, *dmass;
+    float* xpad;
+    int* status;
+};
+
+template <int KIND>
+__global__ __launch_bounds__(64) void k_node_events(NodeEventArgs a) {
+    __shared__ int chain[64][kMaxLevel + 1];
+    __shared__ int depth[64];
+    const int lane = threadIdx.x;
+    const double tau = a.age ? *a.age : a.tau;
+    const size_t nn = a.nn;
+    for (size_t base = 0; base < a.n_ev; base += 64) {
+        const size_t k0 = base + lane;
+        int lvl = -1;
+        if (k0 < a.n_ev) {
+            const uint32_t i = a.idx[k0];
+            int leaf = -1;
+            if (i >= a.n) atomicOr(a.status, 1);
+            else leaf = a.leaf_of[i];
+            bool skip = leaf < 0;
+            if (KIND == kNodeEventMass) skip = skip || a.a[k0] == 0.0;
+            if (KIND == kNodeEventPad) skip = skip || !(a.a[k0] > 0);
+            if (!skip) {
+                lvl = 0;
+                while (lvl < kMaxLevel && a.level_begin[lvl + 1] <= leaf) ++lvl;
+                int no = leaf;
+                for (int L = lvl; L >= 0; --L) {
+                    chain[lane][L] = no;
+                    no = a.parent[no];
+                }
+            }
+        }
+        depth[lane] = lvl;
+        __syncthreads();
+        const int cnt = (int)min((size_t)64, a.n_ev - base);
+        for (int e = 0; e < cnt; ++e) {
+            const int le = depth[e]; // the same for every lane
+            if (le < 0) continue;
+            const size_t k = base + e;
+            const int node = (lane <= le) ? chain[e][lane] : -1;
+            if (KIND == kNodeEventPad) {
+                // the leaf's pad grows by d (rounded up to float); the ancestors take it as a
+                // maximum: the reference's climb stops at the first that already covers it, and
+                // every node above that one covers it too
+                float pad = 0.0f;
+                if (lane == le) pad = soft_as_bound((double)a.xpad[node] + a.a[k]);
+                pad = __shfl(pad, le);
+                if (node >= 0 && a.xpad[node] < pad) a.xpad[node] = pad;
+            } else if (KIND == kNodeEventShift) {
+                if (node >= 0) {
+                    const double m = a.a[k];
+                    a.dq[node] -= m * a.x[k];
+                    a.dq[nn + node] -= m * a.y[k];
+                    a.dq[2 * nn + node] -= m * a.z[k];
+                }
+            } else if (node >= 0) {
+                // node_com (tree.cc:229-237) at tau, then the moments of a particle of mass dm at
+                // x (unwrapped against that centre) moving at v
+                const double mass = a.m0[node];
+                const double M = mass + a.dmass[node];
+                double c[3];
+#pragma unroll
+                for (int d = 0; d < 3; ++d) {
+                    c[d] = (M > 0) ? (mass * a.m0[(d + 1) * nn + node] +
+                                      (mass * a.vcom[d * nn + node] + a.dp[d * nn + node]) * tau -
+                                      a.dq[d * nn + node]) / M
+                                   : a.m0[(d + 1) * nn + node];
+                }
+                const double dm = a.a[k];
+                const double xs[3] = {a.x[k], a.y[k], a.z[k]};
+                const double vs[3] = {a.vx[k], a.vy[k], a.vz[k]};
+                a.dmass[node] += dm;
+#pragma unroll
+                for (int d = 0; d < 3; ++d) a.dp[d * nn + node] += dm * vs[d];
+#pragma unroll
+                for (int d = 0; d < 3; ++d) {
+                    const double xr = c[d] + min_image_sep(xs[d] - c[d], a.box);
+                    a.dq[d * nn + node] -= dm * (xr - vs[d] * tau);
+                }
+            }
+        }
+        __syncthreads();
+    }
+}
+
+gz_status launch_node_events(gz_tree* t, int kind, const uint32_t* idx, size_t n_ev,
+                             const double* v, const double* x, const double* y, const double* z,
+                             const double* vx, const double* vy, const double* vz, double tau,
+                             double box, int* status, hipStream_t st) {
+    if (n_ev == 0 || t->nnodes == 0) return GZ_OK;
+    NodeEventArgs a{};
+    a.idx = idx;
+    a.n_ev = n_ev;
+    a.n = t->n;
+    a.nn = (size_t)t->nnodes;
+    a.leaf_of = t->leaf_of;
+    a.parent = t->parent;
+    a.level_begin = t->level_begin_dev;
+    a.a = v;
+    a.x = x;
+    a.y = y;
+    a.z = z;
+    a.vx = vx;
+    a.vy = vy;
+    a.vz = vz;
+    a.tau = tau;
+    a.box = box;
+    a.age = tau < 0.0 ? t->age : nullptr;
+    a.m0 = t->m0;
+    a.vcom = t->vcom;
+    a.dp = t->dp;
+    a.dq = t->dq;
+    a.dmass = t->dmass;
+    a.xpad = t->xpad;
+    a.status = status;
+    if (kind == kNodeEventMass) k_node_events<kNodeEventMass><<<1, 64, 0, st>>>(a);
+    else if (kind == kNodeEventShift) k_node_events<kNodeEventShift><<<1, 64, 0, st>>>(a);
+    else k_node_events<kNodeEventPad><<<1, 64, 0, st>>>(a);
     GZ_HIP(hipGetLastError());
     return GZ_OK;
 }

@@ -247,6 +247,62 @@ class Tree:
                 "gz_tree_raise_soft")
         self.ctx.synchronize()
 
+    def mass_events(self, idx, dm, x, v, tau: float, box: float = 0.0):
+        """Tree::mass_event (tree.cc:239-251) for each event k in order: a particle of mass dm[k]
+        (negative removes one) at x[:, k] moving at v[:, k], appearing at tau since the build
+        (tau < 0: the tree's age), in the build-time leaf of particle idx[k]."""
+        dev = self.P.device
+        ix = _dev_u32(idx, dev)
+        arrs = [_dev_f64(dm, dev)] + [_dev_f64(a, dev) for a in x] + [_dev_f64(a, dev) for a in v]
+        self.ctx.enter()
+        L.check(L.load().gz_tree_mass_events(self.ctx.handle, self.handle, _ptr(ix),
+                                             int(ix.shape[0]), *(a.data_ptr() for a in arrs),
+                                             float(tau), float(box)),
+                "gz_tree_mass_events")
+        self.ctx.synchronize()
+
+    def shift_nodes(self, idx, m, dx):
+        """Tree::shift_node (tree.cc:253-258) for each event k in order: a particle of mass m[k]
+        in the build-time leaf of particle idx[k] displaced by dx[:, k]."""
+        dev = self.P.device
+        ix = _dev_u32(idx, dev)
+        arrs = [_dev_f64(m, dev)] + [_dev_f64(a, dev) for a in dx]
+        self.ctx.enter()
+        L.check(L.load().gz_tree_shift_nodes(self.ctx.handle, self.handle, _ptr(ix),
+                                             int(ix.shape[0]), *(a.data_ptr() for a in arrs)),
+                "gz_tree_shift_nodes")
+        self.ctx.synchronize()
+
+    def raise_pad(self, idx, d):
+        """Tree::raise_pad (tree.cc:323-330) for each event k in order: a displacement d[k] of a
+        particle in the build-time leaf of particle idx[k]."""
+        ix = _dev_u32(idx, self.P.device)
+        dt = _dev_f64(d, self.P.device)
+        self.ctx.enter()
+        L.check(L.load().gz_tree_raise_pad(self.ctx.handle, self.handle, _ptr(ix),
+                                           int(ix.shape[0]), dt.data_ptr()),
+                "gz_tree_raise_pad")
+        self.ctx.synchronize()
+
+    def raise_vmax(self, idx, speed):
+        """Tree::raise_vmax (tree.cc:276-285) to speed[k] (float32, >= 0) for the build-time leaf
+        of particle idx[k]."""
+        ix = _dev_u32(idx, self.P.device)
+        sp = torch.as_tensor(np.ascontiguousarray(speed, dtype=np.float32), device=self.P.device)
+        self.ctx.enter()
+        L.check(L.load().gz_tree_raise_vmax(self.ctx.handle, self.handle, _ptr(ix),
+                                            int(ix.shape[0]), sp.data_ptr()),
+                "gz_tree_raise_vmax")
+        self.ctx.synchronize()
+
+    def download_pads(self):
+        """Host copies of xpad (nnodes float32) and dmass (nnodes float64)."""
+        nn = self.nnodes
+        xpad, dmass = np.zeros(nn, np.float32), np.zeros(nn)
+        L.check(L.load().gz_tree_download_pads(self.handle, xpad.ctypes.data, dmass.ctypes.data),
+                "gz_tree_download_pads")
+        return xpad, dmass
+
     def download_reuse(self) -> dict:
         """Host copies of vmax (nnodes) and vcom, dp, dq (nnodes x 3)."""
         nn = self.nnodes

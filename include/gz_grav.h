@@ -159,6 +159,44 @@ gz_status gz_tree_refresh_walk_nodes(gz_ctx* ctx, gz_tree* tree, double tau);
  * softening bound to soft[i] (soft: device, indexed by particle). */
 gz_status gz_tree_raise_soft(gz_ctx* ctx, gz_tree* tree, const uint32_t* idx, size_t n,
                              const double* soft);
+/* Events between rebuilds that are neither a drift nor a kick, as the reference's serial callers
+ * report them (accretion merges and removals, mfm.cc:1676-1687 and :1895-1901; the Hermite
+ * corrector, mfm.cc:2350-2356). Each event k names the particle idx[k] whose build-time leaf
+ * receives it (particles not in the tree are skipped); the per-event values are device arrays of
+ * n, and events are applied in the order of k, so every node's sums round as the reference's
+ * serial calls do. An index outside the particle set is an error (unreported under capture).
+ *
+ * Tree::mass_event (tree.h:321-323, tree.cc:239-251): a particle of mass dm[k] (negative removes
+ * one) at (x, y, z)[k] moving at (vx, vy, vz)[k] appears at tau since the build (tau < 0: the
+ * tree's age on the device); every ancestor's mass, momentum and first moment take it, with the
+ * position unwrapped against that ancestor's current centre of mass in a periodic box of side
+ * box (box <= 0: open). dm[k] == 0 is skipped. A no-op on a tree built without velocities, as the
+ * reference's is. */
+gz_status gz_tree_mass_events(gz_ctx* ctx, gz_tree* tree, const uint32_t* idx, size_t n,
+                              const double* dm, const double* x, const double* y, const double* z,
+                              const double* vx, const double* vy, const double* vz, double tau,
+                              double box);
+/* Tree::shift_node (tree.h:324-325, tree.cc:253-258): a particle of mass m[k] displaced by
+ * (dx, dy, dz)[k] without a change of velocity; every ancestor's first moment takes it. A no-op
+ * on a tree built without velocities. */
+gz_status gz_tree_shift_nodes(gz_ctx* ctx, gz_tree* tree, const uint32_t* idx, size_t n,
+                              const double* m, const double* dx, const double* dy,
+                              const double* dz);
+/* Tree::raise_pad (tree.h:316-317, tree.cc:323-330): a displacement of d[k] not explained by
+ * motion at vmax; the leaf's pad grows by d[k] (rounded up to float) and every ancestor's pad
+ * becomes at least the leaf's. d[k] <= 0 (or NaN) is skipped. gz_tree_refresh_walk_nodes widens
+ * each node's opening radius and side by 2 xpad, and the neighbour searches widen their node
+ * test by it. */
+gz_status gz_tree_raise_pad(gz_ctx* ctx, gz_tree* tree, const uint32_t* idx, size_t n,
+                            const double* d);
+/* Tree::raise_vmax (tree.h:303-309, tree.cc:276-285) to speed[k] (device floats, >= 0; a negative
+ * or NaN speed is an error). A maximum, so the order does not matter. */
+gz_status gz_tree_raise_vmax(gz_ctx* ctx, gz_tree* tree, const uint32_t* idx, size_t n,
+                             const float* speed);
+/* Per node, to HOST buffers (either may be NULL): the pad xpad (nnodes floats) and the mass change
+ * since the build dmass (nnodes doubles, zero for a tree built without velocities). */
+gz_status gz_tree_download_pads(const gz_tree* tree, float* xpad, double* dmass);
+
 /* The reuse state to HOST buffers (any may be NULL): vmax (nnodes floats), and vcom, dp, dq
  * (3 x nnodes doubles each, x y z of node i at 3i..3i+2; zero for a tree built without
  * velocities). The refreshed walk-node fields come from gz_tree_download_nodes. */

@@ -201,7 +201,43 @@ class _Lib:
                           leaf_of=None, **ints, **dbls, **flts)
 
 
-class Oracle(_Lib):
+class _EventsMixin:
+    """Tree::mass_event, shift_node, raise_pad and raise_vmax (tree.cc:239-258, 276-285,
+    323-330) for the leaf of each listed particle, in order, with per-event values; the same
+    entry points in the oracle and the reference adapter (prefix gzo_ / gzref_)."""
+
+    def _fn(self, name):
+        return getattr(self.lib, self.prefix + name)
+
+    def mass_events(self, t, idx, dm, x, v, tau, box=0.0):
+        ix = np.ascontiguousarray(idx, dtype=np.uint32)
+        arrs = [_f64(dm)] + [_f64(a) for a in x] + [_f64(a) for a in v]
+        self._fn("mass_events")(t, _ptr(ix, _up), ix.shape[0], *(_ptr(a, _dp) for a in arrs),
+                                tau, box)
+
+    def shift_nodes(self, t, idx, m, dx):
+        ix = np.ascontiguousarray(idx, dtype=np.uint32)
+        arrs = [_f64(m)] + [_f64(a) for a in dx]
+        self._fn("shift_nodes")(t, _ptr(ix, _up), ix.shape[0], *(_ptr(a, _dp) for a in arrs))
+
+    def raise_pad(self, t, idx, d):
+        ix = np.ascontiguousarray(idx, dtype=np.uint32)
+        self._fn("raise_pad")(t, _ptr(ix, _up), ix.shape[0], _ptr(_f64(d), _dp))
+
+    def raise_vmax(self, t, idx, speed):
+        ix = np.ascontiguousarray(idx, dtype=np.uint32)
+        sp = np.ascontiguousarray(speed, dtype=np.float32)
+        self._fn("raise_vmax")(t, _ptr(ix, _up), ix.shape[0], _ptr(sp, _fp))
+
+    def pads(self, t):
+        """(xpad float32, dmass float64) per node."""
+        nn = self._tree_arrays(t).nnodes
+        xpad, dmass = np.zeros(nn, np.float32), np.zeros(nn)
+        self._fn("tree_pads")(t, _ptr(xpad, _fp), _ptr(dmass, _dp))
+        return xpad, dmass
+
+
+class Oracle(_EventsMixin, _Lib):
     """The C restatement, ``liboracle.so``."""
 
     prefix = "gzo_"
@@ -253,6 +289,12 @@ class Oracle(_Lib):
         L.gzo_refresh_walk_nodes.argtypes = [C.c_void_p, C.c_double]
         L.gzo_raise_soft.argtypes = [C.c_void_p, _up, C.c_size_t, _dp]
         L.gzo_tree_reuse.argtypes = [C.c_void_p, _fp, _dp, _dp, _dp, _dp, _dp]
+        L.gzo_mass_events.argtypes = [C.c_void_p, _up, C.c_size_t] + [_dp] * 7 + [C.c_double,
+                                                                               C.c_double]
+        L.gzo_shift_nodes.argtypes = [C.c_void_p, _up, C.c_size_t, _dp, _dp, _dp, _dp]
+        L.gzo_raise_pad.argtypes = [C.c_void_p, _up, C.c_size_t, _dp]
+        L.gzo_raise_vmax.argtypes = [C.c_void_p, _up, C.c_size_t, _fp]
+        L.gzo_tree_pads.argtypes = [C.c_void_p, _fp, _dp]
 
     @staticmethod
     def _cp(P: HostParticles):
@@ -471,7 +513,7 @@ class Oracle(_Lib):
         return ax, ay, az
 
 
-class Reference(_Lib):
+class Reference(_EventsMixin, _Lib):
     """The reference ``shmem/tree.cc``, compiled into ``_ref/libgzref.so``."""
 
     prefix = "gzref_"
@@ -516,6 +558,12 @@ class Reference(_Lib):
         L.gzref_refresh_walk_nodes.argtypes = [C.c_void_p, C.c_double]
         L.gzref_raise_soft.argtypes = [C.c_void_p, _up, C.c_size_t, _dp]
         L.gzref_tree_reuse.argtypes = [C.c_void_p, _fp, _dp, _dp, _dp, _dp, _dp, _dp, _fp, _fp]
+        L.gzref_mass_events.argtypes = [C.c_void_p, _up, C.c_size_t] + [_dp] * 7 + [
+            C.c_double, C.c_double]
+        L.gzref_shift_nodes.argtypes = [C.c_void_p, _up, C.c_size_t, _dp, _dp, _dp, _dp]
+        L.gzref_raise_pad.argtypes = [C.c_void_p, _up, C.c_size_t, _dp]
+        L.gzref_raise_vmax.argtypes = [C.c_void_p, _up, C.c_size_t, _fp]
+        L.gzref_tree_pads.argtypes = [C.c_void_p, _fp, _dp]
         L.gzref_max_threads.restype = C.c_int
         L.gzref_set_threads.argtypes = [C.c_int]
         self._handles = {}

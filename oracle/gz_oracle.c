@@ -453,6 +453,106 @@ void gzo_raise_soft(void* h, const uint32_t* idx, size_t n, const double* soft) 
     }
 }
 
+/* The minimum-image separation of vec.h:25-30: folded into [-box/2, box/2] when box > 0. */
+static double min_image_sep(double d, double box) {
+    if (box > 0) {
+        if (d > 0.5 * box) d -= box;
+        else if (d < -0.5 * box) d += box;
+    }
+    return d;
+}
+
+/* node_com (tree.cc:229-237) at tau from the incremental moments: the build COM when the node
+ * has no mass left. Only called on trees built with velocities. */
+static void node_com_at(const otree* T, int no, double tau, double* c) {
+    const double M = T->mass[no] + T->dmass[no];
+    if (M <= 0) { c[0] = T->cx[no]; c[1] = T->cy[no]; c[2] = T->cz[no]; return; }
+    const double m0 = T->mass[no];
+    c[0] = (m0 * T->cx[no] + (m0 * T->vcom_x[no] + T->dp_x[no]) * tau - T->dq_x[no]) / M;
+    c[1] = (m0 * T->cy[no] + (m0 * T->vcom_y[no] + T->dp_y[no]) * tau - T->dq_y[no]) / M;
+    c[2] = (m0 * T->cz[no] + (m0 * T->vcom_z[no] + T->dp_z[no]) * tau - T->dq_z[no]) / M;
+}
+
+/* mass_event (tree.cc:239-251) for each event in order: every ancestor of the leaf takes dm, its
+ * momentum dm v and, in its first moment, dm (x_r - v tau), where x_r is x unwrapped against that
+ * ancestor's COM at the moment of the event. */
+void gzo_mass_events(void* h, const uint32_t* idx, size_t n, const double* dm, const double* x,
+                     const double* y, const double* z, const double* vx, const double* vy,
+                     const double* vz, double tau, double box) {
+    otree* T = (otree*)h;
+    if (!T->dp_x) return;
+    for (size_t k = 0; k < n; ++k) {
+        const int leaf = T->leaf_of[idx[k]];
+        if (leaf < 0 || dm[k] == 0.0) continue;
+        for (int no = leaf; no >= 0; no = T->parent[no]) {
+            double c[3];
+            node_com_at(T, no, tau, c);
+            const double xr0 = c[0] + min_image_sep(x[k] - c[0], box);
+            const double xr1 = c[1] + min_image_sep(y[k] - c[1], box);
+            const double xr2 = c[2] + min_image_sep(z[k] - c[2], box);
+            T->dmass[no] += dm[k];
+            T->dp_x[no] += dm[k] * vx[k];
+            T->dp_y[no] += dm[k] * vy[k];
+            T->dp_z[no] += dm[k] * vz[k];
+            T->dq_x[no] -= dm[k] * (xr0 - vx[k] * tau);
+            T->dq_y[no] -= dm[k] * (xr1 - vy[k] * tau);
+            T->dq_z[no] -= dm[k] * (xr2 - vz[k] * tau);
+        }
+    }
+}
+
+/* shift_node (tree.cc:253-258): every ancestor's first moment loses m dx. */
+void gzo_shift_nodes(void* h, const uint32_t* idx, size_t n, const double* m, const double* dx,
+                     const double* dy, const double* dz) {
+    otree* T = (otree*)h;
+    if (!T->dp_x) return;
+    for (size_t k = 0; k < n; ++k) {
+        const int leaf = T->leaf_of[idx[k]];
+        if (leaf < 0) continue;
+        for (int no = leaf; no >= 0; no = T->parent[no]) {
+            T->dq_x[no] -= m[k] * dx[k];
+            T->dq_y[no] -= m[k] * dy[k];
+            T->dq_z[no] -= m[k] * dz[k];
+        }
+    }
+}
+
+/* raise_pad (tree.cc:323-330): the leaf's pad grows by d (rounded up to float) and every ancestor
+ * takes it as a maximum, stopping at the first that already covers it. */
+void gzo_raise_pad(void* h, const uint32_t* idx, size_t n, const double* d) {
+    otree* T = (otree*)h;
+    for (size_t k = 0; k < n; ++k) {
+        const int leaf = T->leaf_of[idx[k]];
+        if (leaf < 0 || !(d[k] > 0)) continue;
+        const float pad = soft_as_bound((double)T->xpad[leaf] + d[k]);
+        for (int no = leaf; no >= 0; no = T->parent[no]) {
+            if (T->xpad[no] >= pad) break;
+            T->xpad[no] = pad;
+        }
+    }
+}
+
+/* raise_vmax (tree.cc:276-285) to speed[k], stopping at the first ancestor that covers it. */
+void gzo_raise_vmax(void* h, const uint32_t* idx, size_t n, const float* speed) {
+    otree* T = (otree*)h;
+    for (size_t k = 0; k < n; ++k) {
+        const int leaf = T->leaf_of[idx[k]];
+        if (leaf < 0) continue;
+        for (int no = leaf; no >= 0; no = T->parent[no]) {
+            if (T->vmax[no] >= speed[k]) break;
+            T->vmax[no] = speed[k];
+        }
+    }
+}
+
+void gzo_tree_pads(void* h, float* xpad, double* dmass) {
+    const otree* T = (const otree*)h;
+    for (int i = 0; i < T->nnodes; ++i) {
+        if (xpad) xpad[i] = T->xpad[i];
+        if (dmass) dmass[i] = T->dmass ? T->dmass[i] : 0.0;
+    }
+}
+
 /* The reuse state per node; any pointer may be NULL (vcom, dp and dq are zero without
  * velocities at build time). */
 void gzo_tree_reuse(void* h, float* vmax, double* vcom, double* dp, double* dq, double* wcom,

@@ -40,6 +40,14 @@
  *                         incremental moments and the refreshed walk nodes; the refreshed COM
  *                         against the centre of mass of the drifted particles (ballistic motion
  *                         between kicks makes it exact); vmax against max |v| below each node
+ *   gzo_mass_events, gzo_shift_nodes, gzo_raise_pad, gzo_raise_vmax
+ *                         bit for bit against the reference's Tree::mass_event, shift_node,
+ *                         raise_pad and raise_vmax called in the same order; the refreshed mass
+ *                         and COM against the mass and centre of mass of the particle set with
+ *                         the event applied (a particle added, removed or displaced); an event
+ *                         and its negation cancelling; a leaf's pad against the float-rounded-up
+ *                         running sum of its displacements and every node's pad against the
+ *                         maximum over the leaves below it
  */
 #ifndef GZ_ORACLE_H
 #define GZ_ORACLE_H
@@ -96,6 +104,25 @@ void gzo_refresh_walk_nodes(void* t, double tau);
 void gzo_raise_soft(void* t, const uint32_t* idx, size_t n, const double* soft);
 void gzo_tree_reuse(void* t, float* vmax, double* vcom, double* dp, double* dq, double* wcom,
                     double* wmass);
+/* Events between rebuilds that are not a drift or a kick, each for the leaf that held particle
+ * idx[k] at the build (particles not in the tree are skipped), applied in the order of k:
+ * Tree::mass_event (tree.cc:239-251): a particle of mass dm[k] (negative removes one) at x, v,
+ * appearing at tau since the build, with x unwrapped against each ancestor's current COM in a
+ * periodic box of side box (box <= 0: open); needs a tree built with velocities.
+ * Tree::shift_node (tree.cc:253-258): a particle of mass m[k] displaced by dx without a velocity
+ * change; needs velocities at build. Tree::raise_pad (tree.cc:323-330): a non-ballistic
+ * displacement of d[k] > 0. Tree::raise_vmax (tree.cc:276-285): the speed bound raised to
+ * speed[k]. */
+void gzo_mass_events(void* t, const uint32_t* idx, size_t n, const double* dm, const double* x,
+                     const double* y, const double* z, const double* vx, const double* vy,
+                     const double* vz, double tau, double box);
+void gzo_shift_nodes(void* t, const uint32_t* idx, size_t n, const double* m, const double* dx,
+                     const double* dy, const double* dz);
+void gzo_raise_pad(void* t, const uint32_t* idx, size_t n, const double* d);
+void gzo_raise_vmax(void* t, const uint32_t* idx, size_t n, const float* speed);
+/* Per node: the displacement pad (float) and the mass change since the build (0 without
+ * velocities at build); either may be NULL. */
+void gzo_tree_pads(void* t, float* xpad, double* dmass);
 /* sorted keys (n_live) */
 void gzo_tree_skeys(void* t, uint64_t* skey);
 

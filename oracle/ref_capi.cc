@@ -253,6 +253,50 @@ void gzref_raise_soft(void* t, const uint32_t* idx, size_t n, const double* soft
     for (size_t k = 0; k < n; ++k) T.raise_soft(T.leaf_of[idx[k]], soft[idx[k]]);
 }
 
+// Tree::mass_event (tree.cc:239-251) for the leaf of each listed particle, in order, with the
+// event's own mass, position and velocity.
+void gzref_mass_events(void* t, const uint32_t* idx, size_t n, const double* dm, const double* x,
+                       const double* y, const double* z, const double* vx, const double* vy,
+                       const double* vz, double tau, double box) {
+    Tree& T = static_cast<RefTree*>(t)->T;
+    for (size_t k = 0; k < n; ++k)
+        T.mass_event(T.leaf_of[idx[k]], dm[k], Vec3d{x[k], y[k], z[k]},
+                     Vec3d{vx[k], vy[k], vz[k]}, tau, box);
+}
+
+// Tree::shift_node (tree.cc:253-258) for the leaf of each listed particle, in order.
+void gzref_shift_nodes(void* t, const uint32_t* idx, size_t n, const double* m, const double* dx,
+                       const double* dy, const double* dz) {
+    Tree& T = static_cast<RefTree*>(t)->T;
+    for (size_t k = 0; k < n; ++k)
+        T.shift_node(T.leaf_of[idx[k]], m[k], Vec3d{dx[k], dy[k], dz[k]});
+}
+
+// Tree::raise_pad (tree.cc:323-330) for the leaf of each listed particle, in order.
+void gzref_raise_pad(void* t, const uint32_t* idx, size_t n, const double* d) {
+    Tree& T = static_cast<RefTree*>(t)->T;
+    for (size_t k = 0; k < n; ++k) T.raise_pad(T.leaf_of[idx[k]], d[k]);
+}
+
+// Tree::raise_vmax (tree.cc:276-285) for the leaf of each listed particle, in order.
+void gzref_raise_vmax(void* t, const uint32_t* idx, size_t n, const float* speed) {
+    Tree& T = static_cast<RefTree*>(t)->T;
+    for (size_t k = 0; k < n; ++k) {
+        const int leaf = T.leaf_of[idx[k]];
+        if (leaf >= 0) T.raise_vmax(leaf, speed[k]);
+    }
+}
+
+// Per node: xpad and dmass (tree.h:273-276, :296-306); either may be NULL; dmass is 0 for a tree
+// built without velocities.
+void gzref_tree_pads(void* t, float* xpad, double* dmass) {
+    const Tree& T = static_cast<RefTree*>(t)->T;
+    for (size_t i = 0; i < T.nnodes(); ++i) {
+        if (xpad) xpad[i] = T.xpad[i];
+        if (dmass) dmass[i] = T.dmass.empty() ? 0.0 : T.dmass[i];
+    }
+}
+
 // Reuse state per node (tree.h:266-317) and the walk node view (wn): any pointer may be NULL;
 // vectors hold 3 values per node.
 void gzref_tree_reuse(void* t, float* vmax, double* vcom, double* dp, double* dq, double* wcom,
