@@ -199,7 +199,9 @@ def widened_rows(gz, ics, args):
     theta=0.7, every particle a target; the reference on every 512th particle, all host
     threads), and the forward and reverse neighbour searches (every 4th particle in Morton order
     as a query, radius and kernel radius = the side of the particle's leaf; the reference's
-    per-centre calls, one thread, on every 64th query)."""
+    per-centre calls, one thread, on every 64th query), and the density / smoothing-length solve
+    (every particle a gas target, DesNumNgb 32, tolerance 1, first guesses from the mean density
+    of the particle's leaf; the reference's density() on all host threads on every 64th target)."""
     import torch
     from oracle.bindings import HostParticles, Reference, have_reference
     x, y, z, m, soft = ics.hernquist()
@@ -236,6 +238,16 @@ def widened_rows(gz, ics, args):
     (s, _), dt = timed(lambda: gz.ngb_search_reverse(T, side, n, c, r))
     out["ngb_search_reverse"] = {"queries": nq, "per_query": int(s[-1]) / nq,
                                  "ms_per_call": dt * 1e3, "queries_per_s": nq / dt}
+    # first guesses: DesNumNgb particles at the mean density of the particle's leaf
+    leaf_n = (td["phi"] - td["plo"])[td["leaf_of"]].astype(np.float64)
+    h_leaf = side * np.cbrt(3.0 * 32.0 / (4.0 * np.pi * leaf_n))
+    h0 = h_leaf[order]
+    (_, rho_d, nngb_d, it_d), dt = timed(lambda: gz.density(T, P, tg, 32.0, 1.0, h_start=h0))
+    it_h = it_d.cpu().numpy()
+    out["density"] = {"targets": n, "ms_per_call": dt * 1e3, "targets_per_s": n / dt,
+                      "mean_iterations": float(it_h.mean()),
+                      "at_iteration_cap": int((it_h == 100).sum()),
+                      "mean_neighbours_inside_h": float(nngb_d.float().mean())}
     if not args.no_cpu and have_reference():
         ref = Reference()
         threads = args.cpu_threads or host_cores()[0]
@@ -261,6 +273,11 @@ def widened_rows(gz, ics, args):
             t0 = time.perf_counter()
             ref.ngb_search_reverse(tr, rp, side, n, cs, rs)
             out["ngb_search_reverse"]["cpu_reference_queries_per_s_1thread"] = cs.shape[0] / (
+                time.perf_counter() - t0)
+            ds = order[::64].astype(np.uint32)
+            t0 = time.perf_counter()
+            ref.density(tr, rp, ds, 32.0, 1.0, h_start=h_leaf[ds])
+            out["density"]["cpu_reference_targets_per_s"] = ds.shape[0] / (
                 time.perf_counter() - t0)
             out["cpu_threads_walks"] = threads
         finally:

@@ -143,6 +143,10 @@ _PROTOS = {
     "gz_tree_raise_vmax": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t,
                                      C.c_void_p]),
     "gz_tree_download_pads": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
+    "gz_tree_density": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(gz_particles), C.c_void_p,
+                                  C.c_size_t, C.c_double, C.c_double, C.c_void_p, C.c_int,
+                                  C.c_void_p, C.c_double, C.c_void_p, C.c_void_p, C.c_void_p,
+                                  C.c_void_p]),
     "gz_tree_accel_grouped": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(gz_particles),
                                         C.c_void_p, C.c_size_t, C.c_double, C.c_double, C.c_int,
                                         C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,

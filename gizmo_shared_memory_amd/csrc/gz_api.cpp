@@ -756,6 +756,7 @@ gz_status gz_tree_accel_grouped(gz_ctx* ctx, const gz_tree* tree, const gz_parti
 }
 
 // ---- walk autotuner ----------------------------------------------------------------------------
+extern "C++" { // helpers with C++ types, inside the C-linkage block
 namespace {
 
 int ilog2_bucket(long long v) {
@@ -777,6 +778,7 @@ std::vector<long long> tune_key(const gz_tree* tree, size_t n_t, double theta,
 }
 
 } // namespace
+} // extern "C++"
 
 gz_status gz_walk_autotune(gz_ctx* ctx, const gz_tree* tree, const gz_particles* p,
                            const uint32_t* targets, size_t n_t, double theta, double G,

@@ -1,0 +1,486 @@
+// The density / smoothing-length solve of the shared-memory hydro (shmem::density, hydro.h:115-119,
+// hydro.cc:150-327), non-periodic and without the neighbour cache or lazy drift.
+//
+// The reference iterates each target privately: search the tree within h, sum the kernel over the
+// gas neighbours, and take a bracket-guarded Newton step on N_eff(h) = ball_vol(h) sum_j W until
+// N_eff is within the (widening) tolerance of the neighbour target. Here every still-unconverged
+// target advances one step per round: one neighbour search for all of them (the forward search's
+// CSR lists, each in the reference's traversal order), then one thread per target sums its list
+// in that order and applies the reference's update, then the targets that are done leave the
+// active list (a stable compaction, so the searches keep their Morton locality). Each target's
+// sums and steps are the reference's operations in the reference's order; the one difference is
+// pow(dx, 2) in the 3D face-closure test, taken here as dx * dx (correctly rounded; glibc's pow
+// is within half an ulp of it, not always equal).
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+#include "gz_internal.h"
+
+namespace gz {
+namespace {
+
+constexpr double kPi = 3.14159265358979323846;
+constexpr int kMaxIter = 100; // hydro.cc:201
+constexpr size_t kMaxPairs = size_t(1) << 28; // neighbour pairs listed by one search
+
+// kernel_norm, ball_vol, kernel_w, kernel_dwdh (hydro.h:29-65): cubic spline of support h in dim
+// dimensions, operand for operand.
+__device__ __forceinline__ double knorm(double h, int dim) {
+    if (dim == 1) return (4.0 / 3.0) / h;
+    if (dim == 2) return 40.0 / (7.0 * kPi * h * h);
+    return 8.0 / (kPi * h * h * h);
+}
+__device__ __forceinline__ double ball_vol(double h, int dim) {
+    if (dim == 1) return 2.0 * h;
+    if (dim == 2) return kPi * h * h;
+    return 4.0 * kPi / 3.0 * h * h * h;
+}
+__device__ __forceinline__ double kernel_w(double r, double h, int dim) {
+    const double q = r / h;
+    if (q >= 1.0) return 0.0;
+    const double norm = knorm(h, dim);
+    if (q < 0.5) return norm * (1.0 - 6.0 * q * q + 6.0 * q * q * q);
+    const double u = 1.0 - q;
+    return norm * 2.0 * u * u * u;
+}
+__device__ __forceinline__ double kernel_dwdh(double r, double h, int dim) {
+    const double q = r / h;
+    if (q >= 1.0) return 0.0;
+    const double norm = knorm(h, dim);
+    double f, fp;
+    if (q < 0.5) {
+        f = 1.0 - 6.0 * q * q + 6.0 * q * q * q;
+        fp = -12.0 * q + 18.0 * q * q;
+    } else {
+        const double u = 1.0 - q;
+        f = 2.0 * u * u * u;
+        fp = -6.0 * u * u;
+    }
+    return -((double)dim / h) * norm * f - (q / h) * norm * fp;
+}
+
+// invert_moments (shmem/vec.h:43-63) of the symmetric moments e (xx yy zz xy yz xz) into b
+// (row-major 3x3, zero outside the live block); false when degenerate. 3D is Mat3::invert
+// (math_types/mat3.h:41-67): cofactors times 1 / det, singular when det is 0 or NaN.
+__device__ __forceinline__ bool invert_moments(const double* e, double* b, int dim) {
+#pragma unroll
+    for (int k = 0; k < 9; ++k) b[k] = 0.0;
+    if (dim == 1) {
+        if (fabs(e[0]) < 1e-300) return false;
+        b[0] = 1.0 / e[0];
+        return true;
+    }
+    if (dim == 2) {
+        const double det = e[0] * e[1] - e[3] * e[3];
+        if (fabs(det) < 1e-300) return false;
+        b[0] = e[1] / det;
+        b[1] = -e[3] / det;
+        b[3] = -e[3] / det;
+        b[4] = e[0] / det;
+        return true;
+    }
+    const double m00 = e[0], m01 = e[3], m02 = e[5], m10 = e[3], m11 = e[1], m12 = e[4],
+                 m20 = e[5], m21 = e[4], m22 = e[2];
+    const double d = m00 * (m11 * m22 - m12 * m21) - m01 * (m10 * m22 - m12 * m20) +
+                     m02 * (m10 * m21 - m11 * m20);
+    if (d == 0.0 || d != d) return false;
+    const double id = 1.0 / d;
+    b[0] = (m11 * m22 - m12 * m21) * id;
+    b[1] = (m02 * m21 - m01 * m22) * id;
+    b[2] = (m01 * m12 - m02 * m11) * id;
+    b[3] = (m12 * m20 - m10 * m22) * id;
+    b[4] = (m00 * m22 - m02 * m20) * id;
+    b[5] = (m02 * m10 - m00 * m12) * id;
+    b[6] = (m10 * m21 - m11 * m20) * id;
+    b[7] = (m01 * m20 - m00 * m21) * id;
+    b[8] = (m00 * m11 - m01 * m10) * id;
+    return true;
+}
+
+struct DensArgs {
+    const uint32_t* act; // active targets (positions in the target list)
+    int n_act;
+    const uint32_t* targets;
+    const double *x, *y, *z, *m;
+    const uint8_t* type;  // null: all gas
+    const size_t* start;  // CSR of the active targets' neighbours (particle indices)
+    const uint32_t* found;
+    double des_ngb, ngb_tol;
+    int n_dims;
+    const double* cond_prev; // by particle, or null
+    double *h, *h_lo, *h_hi, *rho;
+    int *iter, *nngb, *iters;
+    uint32_t* keep; // 1: the target stays active
+};
+
+// One step of the solve for every active target (hydro.cc:198-296), one thread each.
+__global__ __launch_bounds__(256) void k_density_step(DensArgs a) {
+    const int k = blockIdx.x * 256 + threadIdx.x;
+    if (k >= a.n_act) return;
+    const uint32_t t = a.act[k];
+    const uint32_t p = a.targets[t];
+    const double p0 = a.x[p], p1 = a.y[p], p2 = a.z[p];
+    const double h = a.h[t];
+    const int it = a.iter[t];
+    const int dim = a.n_dims;
+    const size_t b0 = a.start[k], b1 = a.start[k + 1];
+    if (it >= kMaxIter) {
+        // out of iterations: rho and the count at the last h (hydro.cc:285-295)
+        double rho = 0.0;
+        int n_in = 0;
+        for (size_t q = b0; q < b1; ++q) {
+            const uint32_t j = a.found[q];
+            if (a.type && a.type[j] != 0) continue; // gas neighbours only
+            const double d0 = a.x[j] - p0, d1 = a.y[j] - p1, d2 = a.z[j] - p2;
+            const double r = sqrt(d0 * d0 + d1 * d1 + d2 * d2);
+            if (r < h) ++n_in;
+            rho += a.m[j] * kernel_w(r, h, dim);
+        }
+        a.rho[t] = rho;
+        a.nngb[t] = n_in;
+        a.iters[t] = kMaxIter;
+        a.keep[k] = 0;
+        return;
+    }
+    double weight_sum = 0.0, dweight_dh = 0.0, rho = 0.0;
+    double mom[6] = {0, 0, 0, 0, 0, 0}, face[3] = {0, 0, 0};
+    int n_inside = 0;
+    for (size_t q = b0; q < b1; ++q) {
+        const uint32_t j = a.found[q];
+        if (a.type && a.type[j] != 0) continue;
+        const double d0 = a.x[j] - p0, d1 = a.y[j] - p1, d2 = a.z[j] - p2;
+        const double r = sqrt(d0 * d0 + d1 * d1 + d2 * d2);
+        const double w = kernel_w(r, h, dim);
+        weight_sum += w;
+        dweight_dh += kernel_dwdh(r, h, dim);
+        rho += a.m[j] * w;
+        mom[0] += d0 * d0 * w;
+        mom[1] += d1 * d1 * w;
+        mom[2] += d2 * d2 * w;
+        mom[3] += d0 * d1 * w;
+        mom[4] += d1 * d2 * w;
+        mom[5] += d0 * d2 * w;
+        face[0] += d0 * w;
+        face[1] += d1 * w;
+        face[2] += d2 * w;
+        if (r < h) ++n_inside;
+    }
+    const double n_eff = ball_vol(h, dim) * weight_sum;
+    // the condition-number and face-closure corrections widen the neighbour target
+    // (hydro.cc:220-262)
+    double ncorr = 1.0;
+    if (a.cond_prev) {
+        const double cn = a.cond_prev[p];
+        if (cn > 100.0) ncorr = fmin(sqrt(1.0 + (cn - 100.0) / 1000.0), 2.0);
+    }
+    double des_eff = a.des_ngb * ncorr, tol_eff = a.ngb_tol * ncorr;
+    if (weight_sum > 0) {
+        double bm[9];
+        if (invert_moments(mom, bm, dim)) {
+            const double vol = 1.0 / weight_sum;
+            const double dx_i = sqrt(vol * (mom[0] + mom[1] + mom[2]));
+            double sum_abs = 0.0;
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                const double lk =
+                    (bm[3 * c] * face[0] + bm[3 * c + 1] * face[1] + bm[3 * c + 2] * face[2]) *
+                    (2.0 * vol);
+                sum_abs += fabs(lk) / dim;
+            }
+            const double pw = (dim == 3) ? dx_i * dx_i : (dim == 2 ? dx_i : 1.0);
+            const double denom = 2.0 * dim * pw;
+            if (denom > 0) {
+                const double fce = sum_abs / denom;
+                if (fce > 0.35) ncorr = fmax(ncorr, fmin(fce / 0.35, 2.0));
+                des_eff = a.des_ngb * ncorr;
+                tol_eff = a.ngb_tol * ncorr;
+            }
+        }
+    }
+    // the tolerance widens as the solve iterates (hydro.cc:263-267)
+    if (it > 1)
+        tol_eff = fmin(0.25 * des_eff, tol_eff * exp(0.1 * log(des_eff / (16.0 * tol_eff)) * it));
+    const double residual = n_eff - des_eff;
+    if (fabs(residual) < tol_eff) {
+        a.rho[t] = rho;
+        a.nngb[t] = n_inside;
+        a.iters[t] = it;
+        a.keep[k] = 0;
+        return;
+    }
+    double h_lo = a.h_lo[t], h_hi = a.h_hi[t];
+    if (residual > 0) h_hi = h;
+    else h_lo = h;
+    // Newton on N_eff(h), guarded by the bracket (hydro.cc:271-281)
+    const double dneff_dh = ball_vol(h, dim) * ((double)dim / h * weight_sum + dweight_dh);
+    double h_next = (dneff_dh > 0) ? h - residual / dneff_dh : 0.0;
+    if (h_next <= h_lo || (h_hi > 0 && h_next >= h_hi) || h_next <= 0) {
+        if (h_hi > 0) h_next = (h_lo > 0) ? sqrt(h_lo * h_hi) : 0.5 * h_hi;
+        else h_next = h * ((residual > 0) ? 0.7 : 1.6);
+    }
+    a.h[t] = h_next;
+    a.h_lo[t] = h_lo;
+    a.h_hi[t] = h_hi;
+    a.iter[t] = it + 1;
+    a.keep[k] = 1;
+}
+
+// Start state: every target active at its first guess; a target outside the particle set flags
+// status bit 0 (the call then fails before any search).
+__global__ __launch_bounds__(256) void k_density_init(int nt, const uint32_t* __restrict__ targets,
+                                                      size_t n, const double* __restrict__ h_start,
+                                                      double h_default, double* __restrict__ h,
+                                                      double* __restrict__ h_lo,
+                                                      double* __restrict__ h_hi,
+                                                      int* __restrict__ iter,
+                                                      uint32_t* __restrict__ act,
+                                                      int* __restrict__ status) {
+    const int t = blockIdx.x * 256 + threadIdx.x;
+    if (t >= nt) return;
+    if (targets[t] >= n) atomicOr(status, 1);
+    h[t] = h_start ? h_start[t] : h_default;
+    h_lo[t] = 0.0;
+    h_hi[t] = 0.0;
+    iter[t] = 0;
+    act[t] = (uint32_t)t;
+}
+
+// The active targets' centres and search radii.
+__global__ __launch_bounds__(256) void k_density_centres(int na, const uint32_t* __restrict__ act,
+                                                         const uint32_t* __restrict__ targets,
+                                                         const double* __restrict__ x,
+                                                         const double* __restrict__ y,
+                                                         const double* __restrict__ z,
+                                                         const double* __restrict__ h,
+                                                         double* __restrict__ cx,
+                                                         double* __restrict__ cy,
+                                                         double* __restrict__ cz,
+                                                         double* __restrict__ r) {
+    const int k = blockIdx.x * 256 + threadIdx.x;
+    if (k >= na) return;
+    const uint32_t t = act[k];
+    const uint32_t p = targets[t];
+    cx[k] = x[p];
+    cy[k] = y[p];
+    cz[k] = z[p];
+    r[k] = h[t];
+}
+
+// next[j] = cur[pos[j]] for the kept entries (pos from the flags' scan).
+__global__ __launch_bounds__(256) void k_density_compact(int nk, const int* __restrict__ pos,
+                                                         const uint32_t* __restrict__ cur,
+                                                         uint32_t* __restrict__ next) {
+    const int j = blockIdx.x * 256 + threadIdx.x;
+    if (j < nk) next[j] = cur[pos[j]];
+}
+
+// The bounding box of all particles (hydro.cc:172-178), as order-preserving 64-bit keys of the
+// doubles: min x y z in box[0..2], max in box[3..5] (the max stored complemented so that both
+// are atomicMin).
+__device__ __forceinline__ unsigned long long dkey(double v) {
+    const unsigned long long b = (unsigned long long)__double_as_longlong(v);
+    return (b >> 63) ? ~b : (b | 0x8000000000000000ull);
+}
+__global__ __launch_bounds__(256) void k_density_bbox(size_t n, const double* __restrict__ x,
+                                                      const double* __restrict__ y,
+                                                      const double* __restrict__ z,
+                                                      unsigned long long* __restrict__ box) {
+    unsigned long long lo[3] = {~0ull, ~0ull, ~0ull}, hi[3] = {~0ull, ~0ull, ~0ull};
+    for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (size_t)gridDim.x * 256) {
+        const double v[3] = {x[i], y[i], z[i]};
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            const unsigned long long kk = dkey(v[d]);
+            lo[d] = min(lo[d], kk);
+            hi[d] = min(hi[d], ~kk);
+        }
+    }
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        for (int s = 32; s >= 1; s >>= 1) {
+            lo[d] = min(lo[d], (unsigned long long)__shfl_xor((long long)lo[d], s));
+            hi[d] = min(hi[d], (unsigned long long)__shfl_xor((long long)hi[d], s));
+        }
+    }
+    if ((threadIdx.x & 63) == 0) {
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            atomicMin(&box[d], lo[d]);
+            atomicMin(&box[3 + d], hi[d]);
+        }
+    }
+}
+
+double key_to_double(unsigned long long k) {
+    const unsigned long long b = (k >> 63) ? (k & 0x7FFFFFFFFFFFFFFFull) : ~k;
+    double v;
+    std::memcpy(&v, &b, sizeof v);
+    return v;
+}
+
+} // namespace
+} // namespace gz
+
+using namespace gz;
+
+extern "C" gz_status gz_tree_density(gz_ctx* ctx, const gz_tree* tree, const gz_particles* p,
+                                     const uint32_t* targets, size_t n_t, double des_ngb,
+                                     double ngb_tol, const double* h_start, int n_dims,
+                                     const double* cond_prev, double tau, double* h, double* rho,
+                                     int* nngb, int* iters) {
+    clear_error();
+    if (!ctx || !tree || !p) return fail(GZ_ERR_INVALID, "gz_tree_density: null argument");
+    GZ_HIP(hipSetDevice(ctx->device));
+    if (n_t == 0) return GZ_OK;
+    if (!targets || !h || !rho || !nngb || !iters || !p->x || !p->y || !p->z || !p->m)
+        return fail(GZ_ERR_INVALID, "gz_tree_density: null array");
+    if (n_dims < 1 || n_dims > 3)
+        return fail(GZ_ERR_INVALID, "gz_tree_density: n_dims=%d (1, 2 or 3)", n_dims);
+    if (!(des_ngb > 0) || !(ngb_tol >= 0))
+        return fail(GZ_ERR_INVALID, "gz_tree_density: des_ngb must be > 0 and ngb_tol >= 0");
+    if (n_t >= 0x7FFFFFFFull) return fail(GZ_ERR_INVALID, "gz_tree_density: n_t too large");
+    if (p->n != tree->n)
+        return fail(GZ_ERR_INVALID, "gz_tree_density: %zu particles, the tree holds %zu", p->n,
+                    (size_t)tree->n);
+    hipStream_t st = ctx->stream;
+    const int nt = (int)n_t;
+
+    // state: h_lo, h_hi, the active centres and radii (doubles); the CSR offsets; iter, the
+    // active list, the compaction's output, the keep flags, their scan and the kept positions;
+    // the box keys, the scan total and the status word; the scan's temporary space
+    size_t off = 0;
+    auto take = [&](size_t bytes) {
+        const size_t o = off;
+        off += (bytes + 255) & ~size_t(255);
+        return o;
+    };
+    const size_t o_lo = take(n_t * 8), o_hi = take(n_t * 8), o_cx = take(n_t * 8),
+                 o_cy = take(n_t * 8), o_cz = take(n_t * 8), o_r = take(n_t * 8),
+                 o_start = take((n_t + 1) * sizeof(size_t)), o_iter = take(n_t * 4),
+                 o_act = take(n_t * 4), o_nxt = take(n_t * 4), o_keep = take((n_t + 1) * 4),
+                 o_scan = take((n_t + 1) * 4), o_kept = take((n_t + 1) * 4), o_misc = take(256),
+                 o_tmp = take(scan_tmp_bytes(n_t + 1));
+    DevMem& mem = ctx->scratch[13];
+    GZ_TRY(mem.reserve(off));
+    uint8_t* base = mem.as<uint8_t>();
+    double* d_lo = reinterpret_cast<double*>(base + o_lo);
+    double* d_hi = reinterpret_cast<double*>(base + o_hi);
+    double* d_cx = reinterpret_cast<double*>(base + o_cx);
+    double* d_cy = reinterpret_cast<double*>(base + o_cy);
+    double* d_cz = reinterpret_cast<double*>(base + o_cz);
+    double* d_r = reinterpret_cast<double*>(base + o_r);
+    size_t* d_start = reinterpret_cast<size_t*>(base + o_start);
+    int* d_iter = reinterpret_cast<int*>(base + o_iter);
+    uint32_t* d_act = reinterpret_cast<uint32_t*>(base + o_act);
+    uint32_t* d_nxt = reinterpret_cast<uint32_t*>(base + o_nxt);
+    uint32_t* d_keep = reinterpret_cast<uint32_t*>(base + o_keep);
+    uint32_t* d_scan = reinterpret_cast<uint32_t*>(base + o_scan);
+    int* d_kept = reinterpret_cast<int*>(base + o_kept);
+    unsigned long long* d_box = reinterpret_cast<unsigned long long*>(base + o_misc);
+    uint32_t* d_total = reinterpret_cast<uint32_t*>(base + o_misc + 64);
+    int* d_status = reinterpret_cast<int*>(base + o_misc + 128);
+    GZ_TRY(fill_zero_u32(reinterpret_cast<uint32_t*>(base + o_misc), 64, st));
+
+    // the default first guess: des_ngb particles at the mean density of the bounding box of all
+    // particles (hydro.cc:168-186), the box reduced on the device and the guess formed here
+    double h_default = 0.0;
+    if (!h_start) {
+        if (p->n == 0) return fail(GZ_ERR_INVALID, "gz_tree_density: no particles");
+        const unsigned long long init[6] = {~0ull, ~0ull, ~0ull, ~0ull, ~0ull, ~0ull};
+        GZ_HIP(hipMemcpyAsync(d_box, init, sizeof init, hipMemcpyHostToDevice, st));
+        const unsigned gb = (unsigned)std::min<size_t>(1024, (p->n + 255) / 256);
+        k_density_bbox<<<gb, 256, 0, st>>>(p->n, p->x, p->y, p->z, d_box);
+        GZ_HIP(hipGetLastError());
+        unsigned long long kb[6];
+        GZ_HIP(hipMemcpyAsync(kb, d_box, sizeof kb, hipMemcpyDeviceToHost, st));
+        GZ_HIP(hipStreamSynchronize(st));
+        double lo[3], hi[3];
+        for (int d = 0; d < 3; ++d) {
+            lo[d] = key_to_double(kb[d]);
+            hi[d] = key_to_double(~kb[3 + d]);
+        }
+        const double bbox_volume = (hi[0] - lo[0]) * (hi[1] - lo[1]) * (hi[2] - lo[2]);
+        const double np = (double)p->n;
+        h_default = std::cbrt(3.0 * des_ngb * bbox_volume / (4.0 * kPi * np));
+        if (n_dims == 1) h_default = 0.5 * des_ngb * (hi[0] - lo[0]) / np;
+        if (n_dims == 2)
+            h_default = std::sqrt(des_ngb * (hi[0] - lo[0]) * (hi[1] - lo[1]) / (kPi * np));
+    }
+    k_density_init<<<(unsigned)((n_t + 255) / 256), 256, 0, st>>>(
+        nt, targets, p->n, h_start, h_default, h, d_lo, d_hi, d_iter, d_act, d_status);
+    GZ_HIP(hipGetLastError());
+    int status = 0;
+    GZ_HIP(hipMemcpyAsync(&status, d_status, sizeof status, hipMemcpyDeviceToHost, st));
+    GZ_HIP(hipStreamSynchronize(st));
+    if (status & 1)
+        return fail(GZ_ERR_INVALID, "gz_tree_density: a target is outside the particle set");
+
+    // rounds: a target takes at most 100 steps and one last search
+    uint32_t n_act = (uint32_t)n_t;
+    for (int round = 0; n_act > 0; ++round) {
+        if (round > kMaxIter + 1)
+            return fail(GZ_ERR_INTERNAL, "gz_tree_density: the solve did not end");
+        uint32_t chunk = n_act;
+        const unsigned ga = (n_act + 255) / 256;
+        k_density_centres<<<ga, 256, 0, st>>>((int)n_act, d_act, targets, p->x, p->y, p->z, h,
+                                              d_cx, d_cy, d_cz, d_r);
+        GZ_HIP(hipGetLastError());
+        // the active targets in chunks whose neighbour lists hold at most kMaxPairs pairs (a GiB
+        // of indices): all at once unless a chunk's lists would hold more, then halves until
+        // they do not
+        for (uint32_t c0 = 0; c0 < n_act;) {
+            const uint32_t c1 = std::min<uint32_t>(n_act, c0 + chunk);
+            uint32_t* found = nullptr;
+            size_t total = 0;
+            const gz_status sr = ngb_search_grow(ctx, tree, d_cx + c0, d_cy + c0, d_cz + c0,
+                                                 d_r + c0, c1 - c0, tau, d_start,
+                                                 ctx->scratch[14], &found, &total, kMaxPairs);
+            if (sr == GZ_ERR_UNSUPPORTED && c1 - c0 > 1) {
+                chunk = std::max<uint32_t>(1, (c1 - c0) / 2);
+                continue;
+            }
+            GZ_TRY(sr);
+            DensArgs a{};
+            a.act = d_act + c0;
+            a.n_act = (int)(c1 - c0);
+            a.targets = targets;
+            a.x = p->x;
+            a.y = p->y;
+            a.z = p->z;
+            a.m = p->m;
+            a.type = p->type;
+            a.start = d_start;
+            a.found = found;
+            a.des_ngb = des_ngb;
+            a.ngb_tol = ngb_tol;
+            a.n_dims = n_dims;
+            a.cond_prev = cond_prev;
+            a.h = h;
+            a.h_lo = d_lo;
+            a.h_hi = d_hi;
+            a.rho = rho;
+            a.iter = d_iter;
+            a.nngb = nngb;
+            a.iters = iters;
+            a.keep = d_keep + c0;
+            k_density_step<<<(c1 - c0 + 255) / 256, 256, 0, st>>>(a);
+            GZ_HIP(hipGetLastError());
+            c0 = c1;
+        }
+        // the targets still active, in their order: kept positions, then their target indices
+        GZ_TRY(exclusive_scan_u32(d_keep, d_scan, n_act, d_total, base + o_tmp, st, d_kept));
+        uint32_t n_next = 0;
+        GZ_HIP(hipMemcpyAsync(&n_next, d_total, sizeof n_next, hipMemcpyDeviceToHost, st));
+        GZ_HIP(hipStreamSynchronize(st));
+        if (n_next > 0) {
+            k_density_compact<<<(n_next + 255) / 256, 256, 0, st>>>((int)n_next, d_kept, d_act,
+                                                                     d_nxt);
+            GZ_HIP(hipGetLastError());
+            std::swap(d_act, d_nxt);
+        }
+        n_act = n_next;
+    }
+    GZ_HIP(hipStreamSynchronize(st));
+    return GZ_OK;
+}

@@ -199,7 +199,7 @@ struct gz_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
     bool own_stream = false;
-    gz::DevMem scratch[13];
+    gz::DevMem scratch[16];
     // the walk autotuner's choices, one per workload key (gz_walk_autotune)
     std::vector<std::pair<std::vector<long long>, gz_walk_opts>> tuned;
     hipEvent_t ev[16] = {};
@@ -233,6 +233,14 @@ gz_status launch_node_events(gz_tree* t, int kind, const uint32_t* idx, size_t n
                              const double* v, const double* x, const double* y, const double* z,
                              const double* vx, const double* vy, const double* vz, double tau,
                              double box, int* status, hipStream_t st);
+
+// The forward neighbour search (gz_ngb.hip) with its lists in `found`, reserved to their total
+// (the same CSR as gz_tree_ngb_search); *found_ptr is null when there are none. A total above
+// max_total returns GZ_ERR_UNSUPPORTED, with no message and nothing filled.
+gz_status ngb_search_grow(gz_ctx* ctx, const gz_tree* tree, const double* cx, const double* cy,
+                          const double* cz, const double* radius, size_t n_q, double tau,
+                          size_t* start, DevMem& found, uint32_t** found_ptr, size_t* total,
+                          size_t max_total);
 
 // The spill slices of ctx (allocating them on first use) and their size per wave.
 gz_status ctx_spill(gz_ctx* ctx, int** spill, int* cap);

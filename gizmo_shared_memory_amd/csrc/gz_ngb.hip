@@ -243,16 +243,8 @@ __global__ __launch_bounds__(64) void k_ngb(NgbArgs a) {
     }
 }
 
-// Each query's neighbours in slot (Morton) order, then as particle indices. Lists are short (a
-// kernel's worth of neighbours); a Shell sort keeps the rare long one tolerable.
-__global__ __launch_bounds__(256) void k_ngb_finish(int nq, const size_t* __restrict__ start,
-                                                    uint32_t* __restrict__ slots,
-                                                    const uint32_t* __restrict__ orderbuf,
-                                                    uint32_t* __restrict__ found) {
-    const int q = blockIdx.x * 256 + threadIdx.x;
-    if (q >= nq) return;
-    uint32_t* s = slots + start[q];
-    const long long n = (long long)(start[q + 1] - start[q]);
+// The per-query lists of more than kFinishLds neighbours: a Shell sort in place.
+__device__ void shell_sort_u32(uint32_t* s, long long n) {
     for (long long gap = n / 2; gap > 0; gap = (gap == 2) ? 1 : gap * 5 / 11) {
         for (long long i = gap; i < n; ++i) {
             const uint32_t v = s[i];
@@ -264,9 +256,75 @@ __global__ __launch_bounds__(256) void k_ngb_finish(int nq, const size_t* __rest
             s[j] = v;
         }
     }
-    uint32_t* f = found + start[q];
-    for (long long i = 0; i < n; ++i) f[i] = orderbuf[s[i]];
 }
+
+// Each query's neighbours in slot (Morton) order, then as particle indices: one wavefront per
+// query (a grid-stride loop over the queries). Up to 64 neighbours: a bitonic sort across the
+// lanes' registers; up to kFinishLds: a bitonic sort in LDS, each lane taking every 64th element;
+// longer lists (rare: a radius far above the local spacing): a Shell sort by one lane.
+constexpr int kFinishLds = 1024;
+
+__device__ __forceinline__ uint32_t bitonic_lanes(uint32_t v, int lane) {
+#pragma unroll
+    for (int k = 2; k <= 64; k <<= 1) {
+#pragma unroll
+        for (int j = k >> 1; j > 0; j >>= 1) {
+            const uint32_t o = (uint32_t)__shfl_xor((int)v, j);
+            const bool up = ((lane & k) == 0);
+            const bool lower = ((lane & j) == 0);
+            // the lower lane of a pair keeps the smaller value in an ascending block
+            v = (lower == up) ? min(v, o) : max(v, o);
+        }
+    }
+    return v;
+}
+
+__global__ __launch_bounds__(64) void k_ngb_finish(int nq, const size_t* __restrict__ start,
+                                                   uint32_t* __restrict__ slots,
+                                                   const uint32_t* __restrict__ orderbuf,
+                                                   uint32_t* __restrict__ found) {
+    __shared__ uint32_t buf[kFinishLds];
+    const int lane = threadIdx.x;
+    for (int q = blockIdx.x; q < nq; q += gridDim.x) {
+        const size_t b = start[q];
+        const int n = (int)(start[q + 1] - b);
+        uint32_t* s = slots + b;
+        uint32_t* f = found + b;
+        if (n <= 64) {
+            uint32_t v = (lane < n) ? s[lane] : 0xFFFFFFFFu;
+            v = bitonic_lanes(v, lane);
+            if (lane < n) f[lane] = orderbuf[v];
+        } else if (n <= kFinishLds) {
+            int m = 128;
+            while (m < n) m <<= 1;
+            for (int i = lane; i < m; i += 64) buf[i] = (i < n) ? s[i] : 0xFFFFFFFFu;
+            __syncthreads();
+            for (int k = 2; k <= m; k <<= 1) {
+                for (int j = k >> 1; j > 0; j >>= 1) {
+                    // each pair (i, i ^ j) with i < i ^ j is handled by one lane
+                    for (int t = lane; t < (m >> 1); t += 64) {
+                        const int i = ((t & ~(j - 1)) << 1) | (t & (j - 1));
+                        const int l = i | j;
+                        const uint32_t x = buf[i], y = buf[l];
+                        const bool up = ((i & k) == 0);
+                        if ((x > y) == up) {
+                            buf[i] = y;
+                            buf[l] = x;
+                        }
+                    }
+                    __syncthreads();
+                }
+            }
+            for (int i = lane; i < n; i += 64) f[i] = orderbuf[buf[i]];
+            __syncthreads();
+        } else if (lane == 0) {
+            shell_sort_u32(s, n);
+            for (int i = 0; i < n; ++i) f[i] = orderbuf[s[i]];
+        }
+    }
+}
+
+
 
 __global__ void k_u32_to_size(int n, const uint32_t* __restrict__ in, size_t* __restrict__ out) {
     const int i = blockIdx.x * 256 + threadIdx.x;
@@ -362,10 +420,14 @@ gz_status ngb_pass(NgbArgs& a, bool fill, bool rev, int waves, hipStream_t st) {
 }
 } // namespace
 
+// Both searches; with `grow`, the lists go to that buffer (reserved to the total) and *found_out
+// points at them, unless the total exceeds grow_max: then GZ_ERR_UNSUPPORTED with no message
+// and nothing filled.
 static gz_status ngb_run(gz_ctx* ctx, const gz_tree* tree, const double* cx, const double* cy,
                          const double* cz, const double* radius, size_t n_q, double tau,
                          size_t* start, uint32_t* found, size_t cap, size_t* total,
-                         const double* h, size_t n_gas, bool rev) {
+                         const double* h, size_t n_gas, bool rev, DevMem* grow = nullptr,
+                         uint32_t** found_out = nullptr, size_t grow_max = 0) {
     const char* who = rev ? "gz_tree_ngb_search_reverse" : "gz_tree_ngb_search";
     clear_error();
     if (!ctx || !tree || !total) return fail(GZ_ERR_INVALID, "%s: null argument", who);
@@ -454,11 +516,18 @@ static gz_status ngb_run(gz_ctx* ctx, const gz_tree* tree, const double* cx, con
     if (h_status & 1)
         return fail(GZ_ERR_INTERNAL, "%s: a search stack outgrew %d entries", who,
                     stack_cap);
+    if (grow && h_total > grow_max) return GZ_ERR_UNSUPPORTED;
     if (h_total > 0xFFFFFFFFull)
         return fail(GZ_ERR_UNSUPPORTED,
                     "%s: %llu neighbour pairs exceed the 2^32 - 1 one call can return; split the "
                     "queries", who, h_total);
     *total = tot32;
+    if (grow && tot32 > 0) {
+        GZ_TRY(grow->reserve((size_t)tot32 * sizeof(uint32_t)));
+        found = grow->as<uint32_t>();
+        cap = tot32;
+        *found_out = found;
+    }
     if (!found || cap < (size_t)tot32 || tot32 == 0) return GZ_OK;
     // pass 2: the neighbours' slots, then sorted and mapped to particle indices
     DevMem& sl = ctx->scratch[11];
@@ -466,8 +535,8 @@ static gz_status ngb_run(gz_ctx* ctx, const gz_tree* tree, const double* cx, con
     a.start = start;
     a.slots = sl.as<uint32_t>();
     GZ_TRY(ngb_pass(a, true, rev, waves, st));
-    k_ngb_finish<<<(unsigned)((n_q + 255) / 256), 256, 0, st>>>((int)n_q, start, a.slots,
-                                                                 tree->orderbuf, found);
+    k_ngb_finish<<<(unsigned)std::min<size_t>(n_q, 65536), 64, 0, st>>>((int)n_q, start, a.slots,
+                                                                         tree->orderbuf, found);
     GZ_HIP(hipGetLastError());
     GZ_HIP(hipMemcpyAsync(&h_status, a.status, sizeof h_status, hipMemcpyDeviceToHost, st));
     GZ_HIP(hipStreamSynchronize(st));
@@ -476,6 +545,17 @@ static gz_status ngb_run(gz_ctx* ctx, const gz_tree* tree, const double* cx, con
                     stack_cap);
     return GZ_OK;
 }
+
+namespace gz {
+gz_status ngb_search_grow(gz_ctx* ctx, const gz_tree* tree, const double* cx, const double* cy,
+                          const double* cz, const double* radius, size_t n_q, double tau,
+                          size_t* start, DevMem& found, uint32_t** found_ptr, size_t* total,
+                          size_t max_total) {
+    *found_ptr = nullptr;
+    return ngb_run(ctx, tree, cx, cy, cz, radius, n_q, tau, start, nullptr, 0, total, nullptr, 0,
+                   false, &found, found_ptr, std::min<size_t>(max_total, 0xFFFFFFFFull));
+}
+} // namespace gz
 
 extern "C" gz_status gz_tree_ngb_search(gz_ctx* ctx, const gz_tree* tree, const double* cx,
                                         const double* cy, const double* cz, const double* radius,

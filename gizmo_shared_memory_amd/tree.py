@@ -625,6 +625,37 @@ def accel_grouped_tidal(T: Tree, P: Particles, targets, theta: float, G: float =
     return ax, ay, az, tid, jk
 
 
+def density(T: Tree, P: Particles, targets, des_ngb: float = 32.0, ngb_tol: float = 1.0,
+            h_start=None, n_dims: int = 3, type=None, cond_prev=None, tau: float = 0.0):
+    """shmem::density (hydro.cc:150-327) for the target particles: (h, rho, nngb, iters) device
+    tensors. type: GIZMO types by particle (0 = gas; None: all gas); cond_prev: condition numbers
+    by particle; h_start: first guesses by target (None: the reference's own)."""
+    dev = P.device
+    tg = _dev_u32(targets, dev)
+    nt = int(tg.shape[0])
+    h = torch.empty(max(nt, 1), dtype=torch.float64, device=dev)
+    rho = torch.empty_like(h)
+    nngb = torch.empty(max(nt, 1), dtype=torch.int32, device=dev)
+    iters = torch.empty_like(nngb)
+    hs = None if h_start is None else _dev_f64(h_start, dev)
+    cp = None if cond_prev is None else _dev_f64(cond_prev, dev)
+    ty = None
+    if type is not None:
+        ty = torch.as_tensor(np.ascontiguousarray(type, dtype=np.uint8), device=dev)
+    pc = L.gz_particles(P.n, P.x.data_ptr(), P.y.data_ptr(), P.z.data_ptr(), P.m.data_ptr(),
+                        None if P.soft is None else P.soft.data_ptr(), None,
+                        None if ty is None else ty.data_ptr())
+    T.ctx.enter()
+    L.check(L.load().gz_tree_density(T.ctx.handle, T.handle, C.byref(pc), _ptr(tg), nt,
+                                     float(des_ngb), float(ngb_tol),
+                                     None if hs is None else hs.data_ptr(), int(n_dims),
+                                     None if cp is None else cp.data_ptr(), float(tau),
+                                     h.data_ptr(), rho.data_ptr(), nngb.data_ptr(),
+                                     iters.data_ptr()), "gz_tree_density")
+    T.ctx.synchronize()
+    return h[:nt], rho[:nt], nngb[:nt], iters[:nt]
+
+
 def ngb_search(T: Tree, centres, radius, tau: float = 0.0):
     """shmem::ngb_search (hydro.h:73-75) for many centres: (start, found) device tensors, CSR —
     the particles within radius[q] of centres[q] (n_q x 3), each list in Morton order."""

@@ -364,6 +364,21 @@ gz_status gz_tree_ngb_search_reverse(gz_ctx* ctx, const gz_tree* tree, const dou
                                      const double* cz, const double* h_q, size_t n_q, double tau,
                                      size_t* start, uint32_t* found, size_t cap, size_t* total);
 
+/* shmem::density (hydro.h:115-119, hydro.cc:150-327), non-periodic, the tree aged tau, without the
+ * neighbour cache or lazy drift: for each target particle targets[t] the support radius h[t]
+ * solving N_eff(h) = ball_vol(h) sum_j W(r_j, h) = des_ngb (the cubic spline of support h,
+ * hydro.h:29-65, in n_dims = 1, 2 or 3 dimensions; the target widened by the face-closure and, with
+ * cond_prev (by particle, may be NULL), the condition-number corrections) to within ngb_tol, by
+ * bracket-guarded Newton from h_start[t] (NULL: the reference's guess, des_ngb particles at the
+ * mean density of the bounding box of all particles), with gas neighbours only (p->type NULL: all
+ * gas). Outputs (device, n_t each): h, rho = sum m_j W(r_j, h), the neighbours inside h, and the
+ * iterations taken (100: not converged, rho and the count then at the last h). The positions are
+ * read from p (the tree's sorted copy must match them: gz_tree_update_sources after a drift). */
+gz_status gz_tree_density(gz_ctx* ctx, const gz_tree* tree, const gz_particles* p,
+                          const uint32_t* targets, size_t n_t, double des_ngb, double ngb_tol,
+                          const double* h_start, int n_dims, const double* cond_prev, double tau,
+                          double* h, double* rho, int* nngb, int* iters);
+
 /* shmem::potential (tree.h:446-447, tree.cc:899-944): the spline-softened potential at each
  * target, G included, walked per target like gz_tree_accel (leaves always summed), FP64. */
 gz_status gz_tree_potential(gz_ctx* ctx, const gz_tree* tree, const gz_particles* p,

@@ -295,6 +295,13 @@ class Oracle(_EventsMixin, _Lib):
         L.gzo_raise_pad.argtypes = [C.c_void_p, _up, C.c_size_t, _dp]
         L.gzo_raise_vmax.argtypes = [C.c_void_p, _up, C.c_size_t, _fp]
         L.gzo_tree_pads.argtypes = [C.c_void_p, _fp, _dp]
+        L.gzo_density.argtypes = [C.c_void_p, C.POINTER(_GzoParticles), C.c_void_p, _up,
+                                  C.c_size_t, C.c_double, C.c_double, _dp, C.c_int, _dp,
+                                  C.c_double, _dp, _dp, _ip, _ip]
+        L.gzo_kernel_w.restype = C.c_double
+        L.gzo_kernel_w.argtypes = [C.c_double, C.c_double, C.c_int]
+        L.gzo_kernel_dwdh.restype = C.c_double
+        L.gzo_kernel_dwdh.argtypes = [C.c_double, C.c_double, C.c_int]
 
     @staticmethod
     def _cp(P: HostParticles):
@@ -427,6 +434,28 @@ class Oracle(_EventsMixin, _Lib):
         self.lib.gzo_potential(t, C.byref(cp), _ptr(tg, _up), tg.shape[0], theta, G,
                                _ptr(ao, _dp), _ptr(out, _dp))
         return out
+
+    def density(self, t, P: HostParticles, targets, des_ngb=32.0, ngb_tol=1.0, h_start=None,
+                n_dims=3, type=None, cond_prev=None, tau=0.0):
+        """shmem::density (hydro.cc:150-327): (h, rho, nngb, iters) per target."""
+        tg = np.ascontiguousarray(targets, dtype=np.uint32)
+        nt = tg.shape[0]
+        out = (np.zeros(nt), np.zeros(nt), np.zeros(nt, np.int32), np.zeros(nt, np.int32))
+        ty = None if type is None else np.ascontiguousarray(type, dtype=np.uint8)
+        cp = self._cp(P)
+        self.lib.gzo_density(t, C.byref(cp), None if ty is None else ty.ctypes.data_as(C.c_void_p),
+                             _ptr(tg, _up), nt, des_ngb, ngb_tol,
+                             None if h_start is None else _ptr(_f64(h_start), _dp), n_dims,
+                             None if cond_prev is None else _ptr(_f64(cond_prev), _dp), tau,
+                             _ptr(out[0], _dp), _ptr(out[1], _dp), _ptr(out[2], _ip),
+                             _ptr(out[3], _ip))
+        return out
+
+    def kernel_w(self, r, h, dim=3):
+        return self.lib.gzo_kernel_w(r, h, dim)
+
+    def kernel_dwdh(self, r, h, dim=3):
+        return self.lib.gzo_kernel_dwdh(r, h, dim)
 
     def build_vel(self, P: HostParticles, v, seed: int = -1):
         cp = self._cp(P)
@@ -564,6 +593,9 @@ class Reference(_EventsMixin, _Lib):
         L.gzref_raise_pad.argtypes = [C.c_void_p, _up, C.c_size_t, _dp]
         L.gzref_raise_vmax.argtypes = [C.c_void_p, _up, C.c_size_t, _fp]
         L.gzref_tree_pads.argtypes = [C.c_void_p, _fp, _dp]
+        L.gzref_density.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, _up, C.c_size_t,
+                                    C.c_double, C.c_double, _dp, C.c_int, _dp, C.c_double, _dp,
+                                    _dp, _ip, _ip]
         L.gzref_max_threads.restype = C.c_int
         L.gzref_set_threads.argtypes = [C.c_int]
         self._handles = {}
@@ -580,6 +612,21 @@ class Reference(_EventsMixin, _Lib):
 
     def free(self, t):
         self.lib.gzref_tree_free(t)
+
+    def density(self, t, rp, targets, des_ngb=32.0, ngb_tol=1.0, h_start=None, n_dims=3,
+                type=None, cond_prev=None, tau=0.0):
+        """shmem::density (hydro.cc:150-327): (h, rho, nngb, iters) per target."""
+        tg = np.ascontiguousarray(targets, dtype=np.uint32)
+        nt = tg.shape[0]
+        out = (np.zeros(nt), np.zeros(nt), np.zeros(nt, np.int32), np.zeros(nt, np.int32))
+        ty = None if type is None else np.ascontiguousarray(type, dtype=np.uint8)
+        self.lib.gzref_density(t, rp, None if ty is None else ty.ctypes.data_as(C.c_void_p),
+                               _ptr(tg, _up), nt, des_ngb, ngb_tol,
+                               None if h_start is None else _ptr(_f64(h_start), _dp), n_dims,
+                               None if cond_prev is None else _ptr(_f64(cond_prev), _dp), tau,
+                               _ptr(out[0], _dp), _ptr(out[1], _dp), _ptr(out[2], _ip),
+                               _ptr(out[3], _ip))
+        return out
 
     def build_vel(self, rp, v, seed: int = -1):
         v = tuple(_f64(a) for a in v)

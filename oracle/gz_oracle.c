@@ -751,6 +751,250 @@ size_t gzo_ngb_search_reverse(void* t, const gzo_particles* P, const double* h, 
 }
 
 /* ---------------------------------------------------------------------------------------------
+ * The density / smoothing-length solve (hydro.h:24-65 kernels, hydro.cc:150-327), non-periodic,
+ * without the neighbour cache or lazy drift
+ * ------------------------------------------------------------------------------------------- */
+
+static const double kPiD = 3.14159265358979323846;
+
+/* kernel_norm, ball_vol, kernel_w, kernel_dwdh (hydro.h:29-65), support radius h, dimension dim */
+static double knorm(double h, int dim) {
+    if (dim == 1) return (4.0 / 3.0) / h;
+    if (dim == 2) return 40.0 / (7.0 * kPiD * h * h);
+    return 8.0 / (kPiD * h * h * h);
+}
+static double ball_vol(double h, int dim) {
+    if (dim == 1) return 2.0 * h;
+    if (dim == 2) return kPiD * h * h;
+    return 4.0 * kPiD / 3.0 * h * h * h;
+}
+double gzo_kernel_w(double r, double h, int dim) {
+    const double q = r / h;
+    if (q >= 1.0) return 0.0;
+    const double norm = knorm(h, dim);
+    if (q < 0.5) return norm * (1.0 - 6.0 * q * q + 6.0 * q * q * q);
+    const double u = 1.0 - q;
+    return norm * 2.0 * u * u * u;
+}
+double gzo_kernel_dwdh(double r, double h, int dim) {
+    const double q = r / h;
+    if (q >= 1.0) return 0.0;
+    const double norm = knorm(h, dim);
+    double f, fp;
+    if (q < 0.5) {
+        f = 1.0 - 6.0 * q * q + 6.0 * q * q * q;
+        fp = -12.0 * q + 18.0 * q * q;
+    } else {
+        const double u = 1.0 - q;
+        f = 2.0 * u * u * u;
+        fp = -6.0 * u * u;
+    }
+    return -((double)dim / h) * norm * f - (q / h) * norm * fp;
+}
+
+/* invert_moments (vec.h:43-63): the rank-dim inverse of the symmetric moments (xx yy zz xy yz xz)
+ * into b (row-major 3x3, zero outside the live block); 0 when degenerate. 3D is Mat3::invert
+ * (math_types/mat3.h:41-67): cofactors times 1 / det, singular when det is 0 or NaN. */
+static int invert_moments(const double* e, double* b, int dim) {
+    for (int k = 0; k < 9; ++k) b[k] = 0.0;
+    if (dim == 1) {
+        if (fabs(e[0]) < 1e-300) return 0;
+        b[0] = 1.0 / e[0];
+        return 1;
+    }
+    if (dim == 2) {
+        const double det = e[0] * e[1] - e[3] * e[3];
+        if (fabs(det) < 1e-300) return 0;
+        b[0] = e[1] / det; b[1] = -e[3] / det;
+        b[3] = -e[3] / det; b[4] = e[0] / det;
+        return 1;
+    }
+    const double m[3][3] = {{e[0], e[3], e[5]}, {e[3], e[1], e[4]}, {e[5], e[4], e[2]}};
+    const double d = m[0][0] * (m[1][1] * m[2][2] - m[1][2] * m[2][1]) -
+                     m[0][1] * (m[1][0] * m[2][2] - m[1][2] * m[2][0]) +
+                     m[0][2] * (m[1][0] * m[2][1] - m[1][1] * m[2][0]);
+    if (d == 0.0 || d != d) return 0;
+    const double id = 1.0 / d;
+    b[0] = (m[1][1] * m[2][2] - m[1][2] * m[2][1]) * id;
+    b[1] = (m[0][2] * m[2][1] - m[0][1] * m[2][2]) * id;
+    b[2] = (m[0][1] * m[1][2] - m[0][2] * m[1][1]) * id;
+    b[3] = (m[1][2] * m[2][0] - m[1][0] * m[2][2]) * id;
+    b[4] = (m[0][0] * m[2][2] - m[0][2] * m[2][0]) * id;
+    b[5] = (m[0][2] * m[1][0] - m[0][0] * m[1][2]) * id;
+    b[6] = (m[1][0] * m[2][1] - m[1][1] * m[2][0]) * id;
+    b[7] = (m[0][1] * m[2][0] - m[0][0] * m[2][1]) * id;
+    b[8] = (m[0][0] * m[1][1] - m[0][1] * m[1][0]) * id;
+    return 1;
+}
+
+/* The particles within radius of c (ngb_search, hydro.cc:37-108, tree aged tau), in traversal order,
+ * into *buf (grown as needed); returns the count. sc: the search nodes as gzo_ngb_search packs
+ * them. */
+static size_t ngb_one(const otree* T, const gzo_particles* P, const float* sc, double c0,
+                      double c1, double c2, double radius, double tau, uint32_t** buf,
+                      size_t* cap) {
+    size_t n = 0;
+    const double r2max = radius * radius;
+    int node = T->nnodes ? T->root : -1;
+    while (node >= 0) {
+        const float* sn = sc + 4 * node;
+        double dist = radius + (double)sn[3] + (double)T->xpad[node];
+        if (tau > 0) dist += (double)T->vmax[node] * tau;
+        const double dx = (double)sn[0] - c0;
+        if (dx > dist || -dx > dist) { node = T->next[node]; continue; }
+        const double dy = (double)sn[1] - c1;
+        if (dy > dist || -dy > dist) { node = T->next[node]; continue; }
+        const double dz = (double)sn[2] - c2;
+        if (dz > dist || -dz > dist) { node = T->next[node]; continue; }
+        dist += 0.7320508075688772 * (double)sn[3];
+        if (dx * dx + dy * dy + dz * dz > dist * dist) { node = T->next[node]; continue; }
+        if (T->first[node] < 0) {
+            for (int k = T->plo[node]; k < T->phi[node]; ++k) {
+                const uint32_t j = T->orderbuf[k];
+                const double ex = P->x[j] - c0, ey = P->y[j] - c1, ez = P->z[j] - c2;
+                if (ex * ex + ey * ey + ez * ez < r2max) {
+                    if (n == *cap) {
+                        *cap = *cap ? 2 * *cap : 256;
+                        *buf = (uint32_t*)realloc(*buf, *cap * sizeof(uint32_t));
+                    }
+                    (*buf)[n++] = j;
+                }
+            }
+            node = T->next[node];
+        } else {
+            node = T->first[node];
+        }
+    }
+    return n;
+}
+
+void gzo_density(void* t, const gzo_particles* P, const uint8_t* type, const uint32_t* targets,
+                 size_t nt, double des_ngb, double ngb_tol, const double* h_start, int n_dims,
+                 const double* cond_prev, double tau, double* h_out, double* rho_out,
+                 int* nngb_out, int* iters_out) {
+    const otree* T = (const otree*)t;
+    const int nn = T->nnodes;
+    const double snode_eps = 1e-6 * (nn ? T->size[T->root] : 1.0);
+    float* sc = (float*)malloc((size_t)(nn ? nn : 1) * 4 * sizeof(float));
+    for (int i = 0; i < nn; ++i) {
+        sc[4 * i] = (float)T->gcx[i]; sc[4 * i + 1] = (float)T->gcy[i];
+        sc[4 * i + 2] = (float)T->gcz[i];
+        const double cmax = fmax(fabs(T->gcx[i]), fmax(fabs(T->gcy[i]), fabs(T->gcz[i])));
+        sc[4 * i + 3] = (float)(0.5 * T->size[i] + snode_eps + 4.0 * (double)FLT_EPSILON * cmax);
+    }
+    /* default guess: des_ngb particles at the mean density of the bounding box (hydro.cc:170-186) */
+    double h_default = 0.0;
+    if (!h_start) {
+        double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+        for (size_t i = 0; i < P->n; ++i) {
+            lo[0] = fmin(lo[0], P->x[i]); hi[0] = fmax(hi[0], P->x[i]);
+            lo[1] = fmin(lo[1], P->y[i]); hi[1] = fmax(hi[1], P->y[i]);
+            lo[2] = fmin(lo[2], P->z[i]); hi[2] = fmax(hi[2], P->z[i]);
+        }
+        const double bbox_volume = (hi[0] - lo[0]) * (hi[1] - lo[1]) * (hi[2] - lo[2]);
+        const double np = (double)P->n;
+        h_default = cbrt(3.0 * des_ngb * bbox_volume / (4.0 * kPiD * np));
+        if (n_dims == 1) h_default = 0.5 * des_ngb * (hi[0] - lo[0]) / np;
+        if (n_dims == 2) h_default = sqrt(des_ngb * (hi[0] - lo[0]) * (hi[1] - lo[1]) / (kPiD * np));
+    }
+    uint32_t* nb = NULL;
+    size_t cap = 0;
+    for (size_t ti = 0; ti < nt; ++ti) {
+        const uint32_t tp = targets[ti];
+        const double p0 = P->x[tp], p1 = P->y[tp], p2 = P->z[tp];
+        double h = h_start ? h_start[ti] : h_default;
+        double h_lo = 0.0, h_hi = 0.0;
+        int iter = 0, converged = 0, n_inside = 0;
+        double rho = 0.0;
+        for (; iter < 100; ++iter) {
+            const size_t n = ngb_one(T, P, sc, p0, p1, p2, h, tau, &nb, &cap);
+            double weight_sum = 0.0, dweight_dh = 0.0;
+            double mom[6] = {0, 0, 0, 0, 0, 0}, face[3] = {0, 0, 0};
+            rho = 0.0;
+            n_inside = 0;
+            for (size_t k = 0; k < n; ++k) {
+                const uint32_t j = nb[k];
+                if (type && type[j] != 0) continue; /* gas neighbours only */
+                const double d0 = P->x[j] - p0, d1 = P->y[j] - p1, d2 = P->z[j] - p2;
+                const double r = sqrt(d0 * d0 + d1 * d1 + d2 * d2);
+                const double w = gzo_kernel_w(r, h, n_dims);
+                weight_sum += w;
+                dweight_dh += gzo_kernel_dwdh(r, h, n_dims);
+                rho += P->m[j] * w;
+                mom[0] += d0 * d0 * w; mom[1] += d1 * d1 * w; mom[2] += d2 * d2 * w;
+                mom[3] += d0 * d1 * w; mom[4] += d1 * d2 * w; mom[5] += d0 * d2 * w;
+                face[0] += d0 * w; face[1] += d1 * w; face[2] += d2 * w;
+                if (r < h) ++n_inside;
+            }
+            const double n_eff = ball_vol(h, n_dims) * weight_sum;
+            /* the condition-number and face-closure corrections (hydro.cc:220-262) */
+            double ncorr = 1.0;
+            if (cond_prev) {
+                const double cn = cond_prev[tp];
+                if (cn > 100.0) ncorr = fmin(sqrt(1.0 + (cn - 100.0) / 1000.0), 2.0);
+            }
+            double des_eff = des_ngb * ncorr, tol_eff = ngb_tol * ncorr;
+            if (weight_sum > 0) {
+                double b[9];
+                if (invert_moments(mom, b, n_dims)) {
+                    const double vol = 1.0 / weight_sum;
+                    const double dx_i = sqrt(vol * (mom[0] + mom[1] + mom[2]));
+                    double sum_abs = 0.0;
+                    for (int c = 0; c < 3; ++c) {
+                        const double lk = (b[3 * c] * face[0] + b[3 * c + 1] * face[1] +
+                                           b[3 * c + 2] * face[2]) * (2.0 * vol);
+                        sum_abs += fabs(lk) / n_dims;
+                    }
+                    const double denom = 2.0 * n_dims * pow(dx_i, n_dims - 1);
+                    if (denom > 0) {
+                        const double fce = sum_abs / denom;
+                        if (fce > 0.35) ncorr = fmax(ncorr, fmin(fce / 0.35, 2.0));
+                        des_eff = des_ngb * ncorr;
+                        tol_eff = ngb_tol * ncorr;
+                    }
+                }
+            }
+            /* the tolerance widens as the solve iterates (hydro.cc:263-267) */
+            if (iter > 1)
+                tol_eff = fmin(0.25 * des_eff,
+                               tol_eff * exp(0.1 * log(des_eff / (16.0 * tol_eff)) * iter));
+            const double residual = n_eff - des_eff;
+            if (fabs(residual) < tol_eff) { converged = 1; break; }
+            if (residual > 0) h_hi = h;
+            else h_lo = h;
+            /* Newton on N_eff(h), guarded by the bracket (hydro.cc:271-281) */
+            const double dneff_dh =
+                ball_vol(h, n_dims) * ((double)n_dims / h * weight_sum + dweight_dh);
+            double h_next = (dneff_dh > 0) ? h - residual / dneff_dh : 0.0;
+            if (h_next <= h_lo || (h_hi > 0 && h_next >= h_hi) || h_next <= 0) {
+                if (h_hi > 0) h_next = (h_lo > 0) ? sqrt(h_lo * h_hi) : 0.5 * h_hi;
+                else h_next = h * ((residual > 0) ? 0.7 : 1.6);
+            }
+            h = h_next;
+        }
+        if (!converged) { /* rho and the count at the last h (hydro.cc:285-295) */
+            const size_t n = ngb_one(T, P, sc, p0, p1, p2, h, tau, &nb, &cap);
+            rho = 0.0;
+            n_inside = 0;
+            for (size_t k = 0; k < n; ++k) {
+                const uint32_t j = nb[k];
+                if (type && type[j] != 0) continue;
+                const double d0 = P->x[j] - p0, d1 = P->y[j] - p1, d2 = P->z[j] - p2;
+                const double r = sqrt(d0 * d0 + d1 * d1 + d2 * d2);
+                if (r < h) ++n_inside;
+                rho += P->m[j] * gzo_kernel_w(r, h, n_dims);
+            }
+        }
+        h_out[ti] = h;
+        rho_out[ti] = rho;
+        nngb_out[ti] = n_inside;
+        iters_out[ti] = iter;
+    }
+    free(nb);
+    free(sc);
+}
+
+/* ---------------------------------------------------------------------------------------------
  * Walks (tree.cc:528-582, 626-897) and direct summation (tree.cc:946-971)
  * ------------------------------------------------------------------------------------------- */
 

@@ -19,6 +19,11 @@
  *                         along ballistic motion)
  *   gzo_ngb_search        lists equal to the reference's ngb_search (same order) and, as sets,
  *                         to a brute-force distance test
+ *   gzo_density           bit for bit against the reference's shmem::density (h, rho, neighbour
+ *                         count, iterations) in 1, 2 and 3 dimensions, with and without gas-only
+ *                         neighbours and condition numbers; the kernel normalised (its integral
+ *                         over the ball is 1) and dW/dh equal to finite differences of W; the
+ *                         converged h meeting its own N_eff condition
  *   gzo_init_hmax, gzo_raise_hmax, gzo_ngb_search_reverse
  *                         hmax bit for bit against the reference's init_hmax and against the
  *                         maximum of the rounded-up h below each node; lists equal to the
@@ -157,6 +162,20 @@ size_t gzo_ngb_search_reverse(void* t, const gzo_particles* P, const double* h, 
                               const double* cx, const double* cy, const double* cz,
                               const double* hq, size_t nq, double tau, size_t* start,
                               uint32_t* found, size_t cap);
+/* The cubic-spline hydro kernel W(r, h) and dW/dh at fixed r (hydro.h:38-65), support h. */
+double gzo_kernel_w(double r, double h, int dim);
+double gzo_kernel_dwdh(double r, double h, int dim);
+/* shmem::density (hydro.cc:150-327), non-periodic, tree aged tau, without the neighbour cache or
+ * lazy drift: for each target (a particle index) the support radius h solving
+ * N_eff(h) = ball_vol(h) sum_j W(r_j, h) = des_ngb (widened by the face-closure and, with
+ * cond_prev (by particle, may be NULL), the condition-number corrections) to within ngb_tol, by
+ * bracket-guarded Newton from h_start[t] (NULL: the mean-density guess over the bounding box of
+ * all particles), counting gas neighbours only (type NULL: all gas). Outputs h, rho = sum m_j W,
+ * the neighbours inside h and the iterations taken (100: not converged). */
+void gzo_density(void* t, const gzo_particles* P, const uint8_t* type, const uint32_t* targets,
+                 size_t nt, double des_ngb, double ngb_tol, const double* h_start, int n_dims,
+                 const double* cond_prev, double tau, double* h_out, double* rho_out,
+                 int* nngb_out, int* iters_out);
 void gzo_potential(void* t, const gzo_particles* P, const uint32_t* targets, size_t nt,
                    double theta, double G, const double* aold, double* phi);
 void gzo_accel_brute(const gzo_particles* P, const uint32_t* targets, size_t nt, double G,

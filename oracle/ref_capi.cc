@@ -287,6 +287,31 @@ void gzref_raise_vmax(void* t, const uint32_t* idx, size_t n, const float* speed
     }
 }
 
+// shmem::density (hydro.h:115-119, hydro.cc:150-327), non-periodic, no neighbour cache, the tree
+// aged tau. type (may be NULL: all gas) is set on the particles for the call; h_start NULL means
+// the reference's own default guess; cond_prev (by particle) may be NULL.
+void gzref_density(void* t, void* particles, const uint8_t* type, const uint32_t* targets,
+                   size_t nt, double des_ngb, double ngb_tol, const double* h_start, int n_dims,
+                   const double* cond_prev, double tau, double* h, double* rho, int* nngb,
+                   int* iters) {
+    auto* rt = static_cast<RefTree*>(t);
+    auto* rp = static_cast<RefParticles*>(particles);
+    rt->T.t_since_build = tau;
+    if (type) rp->P.type.assign(type, type + rp->P.size());
+    const std::vector<uint32_t> tg(targets, targets + nt);
+    const std::vector<double> hs = h_start ? std::vector<double>(h_start, h_start + nt)
+                                           : std::vector<double>();
+    const DensityResult r = density(rt->T, rp->P, tg, des_ngb, ngb_tol, hs, 0.0, n_dims, nullptr,
+                                    nullptr, cond_prev);
+    rp->P.type.clear();
+    for (size_t k = 0; k < nt; ++k) {
+        h[k] = r.h[k];
+        rho[k] = r.rho[k];
+        nngb[k] = r.nngb[k];
+        iters[k] = r.iters[k];
+    }
+}
+
 // Per node: xpad and dmass (tree.h:273-276, :296-306); either may be NULL; dmass is 0 for a tree
 // built without velocities.
 void gzref_tree_pads(void* t, float* xpad, double* dmass) {
