@@ -112,60 +112,59 @@ struct DensArgs {
     double *h, *h_lo, *h_hi, *rho;
     int *iter, *nngb, *iters;
     uint32_t* keep; // 1: the target stays active
+    int* long_list; // positions (in the active list) of the targets with long neighbour lists
+    int* n_long;
 };
 
-// One step of the solve for every active target (hydro.cc:198-296), one thread each.
-__global__ __launch_bounds__(256) void k_density_step(DensArgs a) {
-    const int k = blockIdx.x * 256 + threadIdx.x;
-    if (k >= a.n_act) return;
-    const uint32_t t = a.act[k];
-    const uint32_t p = a.targets[t];
-    const double p0 = a.x[p], p1 = a.y[p], p2 = a.z[p];
-    const double h = a.h[t];
-    const int it = a.iter[t];
+// The terms one gas neighbour j adds to a target at (p0, p1, p2) with support h (hydro.cc:206-217):
+// v = {W, dW/dh, m W, dx dx W, dy dy W, dz dz W, dx dy W, dy dz W, dx dz W, dx W, dy W, dz W}, and
+// whether r < h. Non-gas neighbours add nothing (returns false).
+constexpr int kTerms = 12;
+__device__ __forceinline__ bool density_terms(const DensArgs& a, uint32_t j, double p0, double p1,
+                                              double p2, double h, int dim, double* v,
+                                              bool* inside) {
+    if (a.type && a.type[j] != 0) return false; // gas neighbours only
+    const double d0 = a.x[j] - p0, d1 = a.y[j] - p1, d2 = a.z[j] - p2;
+    const double r = sqrt(d0 * d0 + d1 * d1 + d2 * d2);
+    const double w = kernel_w(r, h, dim);
+    v[0] = w;
+    v[1] = kernel_dwdh(r, h, dim);
+    v[2] = a.m[j] * w;
+    v[3] = d0 * d0 * w;
+    v[4] = d1 * d1 * w;
+    v[5] = d2 * d2 * w;
+    v[6] = d0 * d1 * w;
+    v[7] = d1 * d2 * w;
+    v[8] = d0 * d2 * w;
+    v[9] = d0 * w;
+    v[10] = d1 * w;
+    v[11] = d2 * w;
+    *inside = r < h;
+    return true;
+}
+
+// A target's sums over its neighbours, added in list (traversal) order.
+struct DensSums {
+    double s[kTerms];
+    int n_inside;
+};
+
+// The rest of one step (hydro.cc:218-296) from the target's sums: the final pass's outputs when
+// out of iterations, else the corrections, the convergence test and the Newton / bisection step.
+__device__ void density_update(const DensArgs& a, int k, uint32_t t, uint32_t p, double h, int it,
+                               const DensSums& S) {
     const int dim = a.n_dims;
-    const size_t b0 = a.start[k], b1 = a.start[k + 1];
     if (it >= kMaxIter) {
         // out of iterations: rho and the count at the last h (hydro.cc:285-295)
-        double rho = 0.0;
-        int n_in = 0;
-        for (size_t q = b0; q < b1; ++q) {
-            const uint32_t j = a.found[q];
-            if (a.type && a.type[j] != 0) continue; // gas neighbours only
-            const double d0 = a.x[j] - p0, d1 = a.y[j] - p1, d2 = a.z[j] - p2;
-            const double r = sqrt(d0 * d0 + d1 * d1 + d2 * d2);
-            if (r < h) ++n_in;
-            rho += a.m[j] * kernel_w(r, h, dim);
-        }
-        a.rho[t] = rho;
-        a.nngb[t] = n_in;
+        a.rho[t] = S.s[2];
+        a.nngb[t] = S.n_inside;
         a.iters[t] = kMaxIter;
         a.keep[k] = 0;
         return;
     }
-    double weight_sum = 0.0, dweight_dh = 0.0, rho = 0.0;
-    double mom[6] = {0, 0, 0, 0, 0, 0}, face[3] = {0, 0, 0};
-    int n_inside = 0;
-    for (size_t q = b0; q < b1; ++q) {
-        const uint32_t j = a.found[q];
-        if (a.type && a.type[j] != 0) continue;
-        const double d0 = a.x[j] - p0, d1 = a.y[j] - p1, d2 = a.z[j] - p2;
-        const double r = sqrt(d0 * d0 + d1 * d1 + d2 * d2);
-        const double w = kernel_w(r, h, dim);
-        weight_sum += w;
-        dweight_dh += kernel_dwdh(r, h, dim);
-        rho += a.m[j] * w;
-        mom[0] += d0 * d0 * w;
-        mom[1] += d1 * d1 * w;
-        mom[2] += d2 * d2 * w;
-        mom[3] += d0 * d1 * w;
-        mom[4] += d1 * d2 * w;
-        mom[5] += d0 * d2 * w;
-        face[0] += d0 * w;
-        face[1] += d1 * w;
-        face[2] += d2 * w;
-        if (r < h) ++n_inside;
-    }
+    const double weight_sum = S.s[0], dweight_dh = S.s[1];
+    const double* mom = S.s + 3;
+    const double* face = S.s + 9;
     const double n_eff = ball_vol(h, dim) * weight_sum;
     // the condition-number and face-closure corrections widen the neighbour target
     // (hydro.cc:220-262)
@@ -203,8 +202,8 @@ __global__ __launch_bounds__(256) void k_density_step(DensArgs a) {
         tol_eff = fmin(0.25 * des_eff, tol_eff * exp(0.1 * log(des_eff / (16.0 * tol_eff)) * it));
     const double residual = n_eff - des_eff;
     if (fabs(residual) < tol_eff) {
-        a.rho[t] = rho;
-        a.nngb[t] = n_inside;
+        a.rho[t] = S.s[2];
+        a.nngb[t] = S.n_inside;
         a.iters[t] = it;
         a.keep[k] = 0;
         return;
@@ -224,6 +223,86 @@ __global__ __launch_bounds__(256) void k_density_step(DensArgs a) {
     a.h_hi[t] = h_hi;
     a.iter[t] = it + 1;
     a.keep[k] = 1;
+}
+
+// Lists longer than this go to k_density_step_long (one wave each).
+constexpr int kLongList = 192;
+
+// One step for every active target with a short list, one thread each; long lists are queued.
+__global__ __launch_bounds__(256) void k_density_step(DensArgs a) {
+    const int k = blockIdx.x * 256 + threadIdx.x;
+    if (k >= a.n_act) return;
+    const size_t b0 = a.start[k], b1 = a.start[k + 1];
+    if (b1 - b0 > (size_t)kLongList) {
+        a.long_list[atomicAdd(a.n_long, 1)] = k;
+        return;
+    }
+    const uint32_t t = a.act[k];
+    const uint32_t p = a.targets[t];
+    const double p0 = a.x[p], p1 = a.y[p], p2 = a.z[p];
+    const double h = a.h[t];
+    const int it = a.iter[t];
+    DensSums S;
+#pragma unroll
+    for (int c = 0; c < kTerms; ++c) S.s[c] = 0.0;
+    S.n_inside = 0;
+    for (size_t q = b0; q < b1; ++q) {
+        double v[kTerms];
+        bool in = false;
+        if (!density_terms(a, a.found[q], p0, p1, p2, h, a.n_dims, v, &in)) continue;
+#pragma unroll
+        for (int c = 0; c < kTerms; ++c) S.s[c] += v[c];
+        if (in) ++S.n_inside;
+    }
+    density_update(a, k, t, p, h, it, S);
+}
+
+// The queued long lists, one wave per target (a grid-stride loop over the queue): the lanes
+// compute 64 neighbours' terms at a time into LDS, and lane 0 adds them in list order, so the
+// sums are those of the one-thread loop.
+__global__ __launch_bounds__(64) void k_density_step_long(DensArgs a) {
+    __shared__ double s_v[kTerms][64];
+    __shared__ int s_in[64];
+    const int lane = threadIdx.x;
+    const int nl = *a.n_long;
+    for (int e = blockIdx.x; e < nl; e += gridDim.x) {
+        const int k = a.long_list[e];
+        const uint32_t t = a.act[k];
+        const uint32_t p = a.targets[t];
+        const double p0 = a.x[p], p1 = a.y[p], p2 = a.z[p];
+        const double h = a.h[t];
+        const int it = a.iter[t];
+        const size_t b0 = a.start[k], b1 = a.start[k + 1];
+        DensSums S;
+#pragma unroll
+        for (int c = 0; c < kTerms; ++c) S.s[c] = 0.0;
+        S.n_inside = 0;
+        for (size_t c0 = b0; c0 < b1; c0 += 64) {
+            const size_t q = c0 + lane;
+            double v[kTerms];
+            bool in = false;
+            int flag = 0; // bit 0: adds its terms; bit 1: inside h
+            if (q < b1 && density_terms(a, a.found[q], p0, p1, p2, h, a.n_dims, v, &in)) {
+                flag = 1 | (in ? 2 : 0);
+#pragma unroll
+                for (int c = 0; c < kTerms; ++c) s_v[c][lane] = v[c];
+            }
+            s_in[lane] = flag;
+            __syncthreads();
+            if (lane == 0) {
+                const int cnt = (int)min((size_t)64, b1 - c0);
+                for (int i = 0; i < cnt; ++i) {
+                    const int f = s_in[i];
+                    if (!(f & 1)) continue;
+#pragma unroll
+                    for (int c = 0; c < kTerms; ++c) S.s[c] += s_v[c][i];
+                    if (f & 2) ++S.n_inside;
+                }
+            }
+            __syncthreads();
+        }
+        if (lane == 0) density_update(a, k, t, p, h, it, S);
+    }
 }
 
 // Start state: every target active at its first guess; a target outside the particle set flags
@@ -359,7 +438,8 @@ extern "C" gz_status gz_tree_density(gz_ctx* ctx, const gz_tree* tree, const gz_
                  o_cy = take(n_t * 8), o_cz = take(n_t * 8), o_r = take(n_t * 8),
                  o_start = take((n_t + 1) * sizeof(size_t)), o_iter = take(n_t * 4),
                  o_act = take(n_t * 4), o_nxt = take(n_t * 4), o_keep = take((n_t + 1) * 4),
-                 o_scan = take((n_t + 1) * 4), o_kept = take((n_t + 1) * 4), o_misc = take(256),
+                 o_scan = take((n_t + 1) * 4), o_kept = take((n_t + 1) * 4),
+                 o_long = take(n_t * 4), o_misc = take(256),
                  o_tmp = take(scan_tmp_bytes(n_t + 1));
     DevMem& mem = ctx->scratch[13];
     GZ_TRY(mem.reserve(off));
@@ -380,6 +460,8 @@ extern "C" gz_status gz_tree_density(gz_ctx* ctx, const gz_tree* tree, const gz_
     unsigned long long* d_box = reinterpret_cast<unsigned long long*>(base + o_misc);
     uint32_t* d_total = reinterpret_cast<uint32_t*>(base + o_misc + 64);
     int* d_status = reinterpret_cast<int*>(base + o_misc + 128);
+    int* d_long = reinterpret_cast<int*>(base + o_long);
+    int* d_nlong = reinterpret_cast<int*>(base + o_misc + 192);
     GZ_TRY(fill_zero_u32(reinterpret_cast<uint32_t*>(base + o_misc), 64, st));
 
     // the default first guess: des_ngb particles at the mean density of the bounding box of all
@@ -464,7 +546,12 @@ extern "C" gz_status gz_tree_density(gz_ctx* ctx, const gz_tree* tree, const gz_
             a.nngb = nngb;
             a.iters = iters;
             a.keep = d_keep + c0;
+            a.long_list = d_long;
+            a.n_long = d_nlong;
+            GZ_TRY(fill_zero_u32(reinterpret_cast<uint32_t*>(d_nlong), 1, st));
             k_density_step<<<(c1 - c0 + 255) / 256, 256, 0, st>>>(a);
+            GZ_HIP(hipGetLastError());
+            k_density_step_long<<<(unsigned)std::min<uint32_t>(c1 - c0, 4096), 64, 0, st>>>(a);
             GZ_HIP(hipGetLastError());
             c0 = c1;
         }

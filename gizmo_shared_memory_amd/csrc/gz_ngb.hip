@@ -34,6 +34,7 @@ struct NgbArgs {
     const uint32_t* orderbuf;
     const double *cx, *cy, *cz, *radius;
     int nq, ngroups, root, nclusters;
+    int qpw; // queries per wave (a power of two, at most 64)
     double snode_eps, tau;
     int* queue;
     int* stack; // stack_cap ints per wave
@@ -70,8 +71,8 @@ __global__ __launch_bounds__(64) void k_ngb(NgbArgs a) {
         if (lane == 0) v = atomicAdd(a.queue, 1);
         const int g = __shfl(v, 0);
         if (g >= a.ngroups) break;
-        const int q = g * 64 + lane;
-        const bool live = q < a.nq;
+        const int q = g * a.qpw + lane;
+        const bool live = lane < a.qpw && q < a.nq;
         double qx = 0, qy = 0, qz = 0, r = 0;
         if (live) {
             qx = a.cx[q]; qy = a.cy[q]; qz = a.cz[q];
@@ -326,6 +327,30 @@ __global__ __launch_bounds__(64) void k_ngb_finish(int nq, const size_t* __restr
 
 
 
+// Many pairs: every list sorted at once by a device radix sort of (query << 32 | slot) keys, which
+// keeps each query's range of the CSR and does not slow down on long lists. One wave per query
+// writes its keys.
+constexpr size_t kRadixFinishMin = size_t(1) << 20; // pairs
+constexpr size_t kRadixFinishMax = size_t(1) << 28; // 24 B of keys and values per pair
+__global__ __launch_bounds__(64) void k_ngb_keys(int nq, const size_t* __restrict__ start,
+                                                 const uint32_t* __restrict__ slots,
+                                                 uint64_t* __restrict__ keys,
+                                                 uint32_t* __restrict__ vals) {
+    for (int q = blockIdx.x; q < nq; q += gridDim.x) {
+        const size_t b = start[q], e = start[q + 1];
+        for (size_t i = b + threadIdx.x; i < e; i += 64) {
+            keys[i] = ((uint64_t)q << 32) | slots[i];
+            vals[i] = 0;
+        }
+    }
+}
+__global__ __launch_bounds__(256) void k_ngb_map(size_t n, const uint64_t* __restrict__ keys,
+                                                 const uint32_t* __restrict__ orderbuf,
+                                                 uint32_t* __restrict__ found) {
+    for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (size_t)gridDim.x * 256)
+        found[i] = orderbuf[(uint32_t)keys[i]];
+}
+
 __global__ void k_u32_to_size(int n, const uint32_t* __restrict__ in, size_t* __restrict__ out) {
     const int i = blockIdx.x * 256 + threadIdx.x;
     if (i < n) out[i] = in[i];
@@ -475,7 +500,12 @@ static gz_status ngb_run(gz_ctx* ctx, const gz_tree* tree, const double* cx, con
     a.orderbuf = tree->orderbuf;
     a.cx = cx; a.cy = cy; a.cz = cz; a.radius = radius;
     a.nq = (int)n_q;
-    a.ngroups = (int)((n_q + 63) / 64);
+    // 64 consecutive queries per wave share its traversal; with too few queries to occupy the
+    // GPU that way (and often scattered, as the last unconverged targets of a density solve),
+    // fewer per wave, down to one, so that more waves walk at once
+    a.qpw = 64;
+    while (a.qpw > 1 && (n_q + a.qpw - 1) / a.qpw < (size_t)waves) a.qpw >>= 1;
+    a.ngroups = (int)((n_q + a.qpw - 1) / a.qpw);
     a.root = tree->root;
     a.nclusters = tree->n_clusters;
     a.snode_eps = 1e-6 * tree->root_box[3];
@@ -535,9 +565,29 @@ static gz_status ngb_run(gz_ctx* ctx, const gz_tree* tree, const double* cx, con
     a.start = start;
     a.slots = sl.as<uint32_t>();
     GZ_TRY(ngb_pass(a, true, rev, waves, st));
-    k_ngb_finish<<<(unsigned)std::min<size_t>(n_q, 65536), 64, 0, st>>>((int)n_q, start, a.slots,
-                                                                         tree->orderbuf, found);
-    GZ_HIP(hipGetLastError());
+    if ((size_t)tot32 >= kRadixFinishMin && (size_t)tot32 <= kRadixFinishMax) {
+        DevMem& kb = ctx->scratch[15];
+        const size_t nk = tot32;
+        GZ_TRY(kb.reserve(nk * (2 * sizeof(uint64_t) + 2 * sizeof(uint32_t))));
+        uint64_t* keys = kb.as<uint64_t>();
+        uint64_t* keys_alt = keys + nk;
+        uint32_t* vals = reinterpret_cast<uint32_t*>(keys_alt + nk);
+        uint32_t* vals_alt = vals + nk;
+        k_ngb_keys<<<(unsigned)std::min<size_t>(n_q, 65536), 64, 0, st>>>((int)n_q, start,
+                                                                        a.slots, keys, vals);
+        GZ_HIP(hipGetLastError());
+        int qbits = 1;
+        while (qbits < 31 && (size_t(1) << qbits) < n_q) ++qbits;
+        GZ_TRY(radix_sort_pairs(keys, vals, keys_alt, vals_alt, nk, 32 + qbits, ctx->scratch[3],
+                                st));
+        k_ngb_map<<<(unsigned)std::min<size_t>((nk + 255) / 256, 65536), 256, 0, st>>>(
+            nk, keys, tree->orderbuf, found);
+        GZ_HIP(hipGetLastError());
+    } else {
+        k_ngb_finish<<<(unsigned)std::min<size_t>(n_q, 65536), 64, 0, st>>>(
+            (int)n_q, start, a.slots, tree->orderbuf, found);
+        GZ_HIP(hipGetLastError());
+    }
     GZ_HIP(hipMemcpyAsync(&h_status, a.status, sizeof h_status, hipMemcpyDeviceToHost, st));
     GZ_HIP(hipStreamSynchronize(st));
     if (h_status & 1)
