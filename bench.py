@@ -358,6 +358,9 @@ def main():
                     help="also time the path's other calls (potential, tidal tensor, forward and "
                          "reverse neighbour search) on the Hernquist 4M beside the reference's "
                          "CPU code, logged to stderr; the stdout line is unchanged")
+    ap.add_argument("--rows-only", action="store_true",
+                    help="time only the path's other calls (as --rows) and print nothing on "
+                         "stdout: a short run for profiling those kernels")
     ap.add_argument("--configs", action="store_true",
                     help="also time the other named workloads (Plummer 64k, Hernquist 4M, "
                          "clustered 16M: build, FP64 and FP32 walks, errors against the direct "
@@ -378,6 +381,9 @@ def main():
     from gizmo_shared_memory_amd import tree as gz
 
     assert torch.cuda.is_available(), "bench.py needs a GPU"
+    if args.rows_only:
+        widened_rows(gz, ics, args)
+        return
     t0 = time.perf_counter()
     x, y, z, m, soft = ics.halo_disk(n=args.n)
     log(f"generated halo+disk n={args.n} in {time.perf_counter() - t0:.1f} s")

@@ -327,9 +327,10 @@ __global__ __launch_bounds__(64) void k_ngb_finish(int nq, const size_t* __restr
 
 
 
-// Many pairs: every list sorted at once by a device radix sort of (query << 32 | slot) keys, which
-// keeps each query's range of the CSR and does not slow down on long lists. One wave per query
-// writes its keys.
+// Many pairs with some lists too long for the LDS sort: every list sorted at once by a device
+// radix sort of (query << 32 | slot) keys, which keeps each query's range of the CSR and does not
+// slow down on long lists (short lists sort faster in k_ngb_finish). One wave per query writes
+// its keys.
 constexpr size_t kRadixFinishMin = size_t(1) << 20; // pairs
 constexpr size_t kRadixFinishMax = size_t(1) << 28; // 24 B of keys and values per pair
 __global__ __launch_bounds__(64) void k_ngb_keys(int nq, const size_t* __restrict__ start,
@@ -358,12 +359,24 @@ __global__ void k_u32_to_size(int n, const uint32_t* __restrict__ in, size_t* __
 
 // The sum of the per-query counts in 64 bits (the 32-bit scan that makes the offsets would wrap
 // past 2^32 - 1 pairs): a grid-stride sum per thread, a wave sum, one atomic per wave.
+// The largest count too, which picks how the lists are sorted.
 __global__ __launch_bounds__(256) void k_sum_counts(int n, const uint32_t* __restrict__ in,
-                                                    unsigned long long* __restrict__ out) {
+                                                    unsigned long long* __restrict__ out,
+                                                    uint32_t* __restrict__ max_out) {
     long long s = 0;
-    for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) s += in[i];
-    for (int d = 32; d >= 1; d >>= 1) s += __shfl_xor(s, d);
-    if ((threadIdx.x & 63) == 0 && s) atomicAdd(out, (unsigned long long)s);
+    uint32_t mx = 0;
+    for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) {
+        s += in[i];
+        mx = max(mx, in[i]);
+    }
+    for (int d = 32; d >= 1; d >>= 1) {
+        s += __shfl_xor(s, d);
+        mx = max(mx, (uint32_t)__shfl_xor((int)mx, d));
+    }
+    if ((threadIdx.x & 63) == 0) {
+        if (s) atomicAdd(out, (unsigned long long)s);
+        if (mx) atomicMax(max_out, mx);
+    }
 }
 
 // h rounded up to float, so that the float bound stays one-sided (tree.cc:286-290).
@@ -520,10 +533,11 @@ static gz_status ngb_run(gz_ctx* ctx, const gz_tree* tree, const double* cx, con
     a.n_gas = rev ? n_gas : 0;
     GZ_TRY(fill_zero_u32(count, n_q + 1, st));
     unsigned long long* total64 = reinterpret_cast<unsigned long long*>(misc + 4);
-    GZ_TRY(fill_zero_u32(reinterpret_cast<uint32_t*>(total64), 2, st));
+    uint32_t* max_count = reinterpret_cast<uint32_t*>(misc + 6);
+    GZ_TRY(fill_zero_u32(reinterpret_cast<uint32_t*>(total64), 3, st));
     GZ_TRY(ngb_pass(a, false, rev, waves, st));
     k_sum_counts<<<(unsigned)std::min<size_t>(1024, (n_q + 255) / 256), 256, 0, st>>>(
-        (int)n_q, count, total64);
+        (int)n_q, count, total64, max_count);
     GZ_HIP(hipGetLastError());
     // offsets: exclusive scan of the counts (count[n_q] = 0 gives the total at start[n_q])
     uint32_t tot32 = 0;
@@ -540,6 +554,8 @@ static gz_status ngb_run(gz_ctx* ctx, const gz_tree* tree, const double* cx, con
     }
     int h_status = 0;
     unsigned long long h_total = 0;
+    uint32_t h_max = 0;
+    GZ_HIP(hipMemcpyAsync(&h_max, max_count, sizeof h_max, hipMemcpyDeviceToHost, st));
     GZ_HIP(hipMemcpyAsync(&h_status, a.status, sizeof h_status, hipMemcpyDeviceToHost, st));
     GZ_HIP(hipMemcpyAsync(&h_total, total64, sizeof h_total, hipMemcpyDeviceToHost, st));
     GZ_HIP(hipStreamSynchronize(st));
@@ -565,7 +581,8 @@ static gz_status ngb_run(gz_ctx* ctx, const gz_tree* tree, const double* cx, con
     a.start = start;
     a.slots = sl.as<uint32_t>();
     GZ_TRY(ngb_pass(a, true, rev, waves, st));
-    if ((size_t)tot32 >= kRadixFinishMin && (size_t)tot32 <= kRadixFinishMax) {
+    if (h_max > (uint32_t)kFinishLds && (size_t)tot32 >= kRadixFinishMin &&
+        (size_t)tot32 <= kRadixFinishMax) {
         DevMem& kb = ctx->scratch[15];
         const size_t nk = tot32;
         GZ_TRY(kb.reserve(nk * (2 * sizeof(uint64_t) + 2 * sizeof(uint32_t))));

@@ -281,6 +281,76 @@ class Tree {
                                             hipMemcpyDeviceToHost), "copy to host");
     }
 
+    // Tree::mass_event (tree.h:321-323, tree.cc:239-251) for each event k in order: a particle
+    // of mass dm[k] (negative removes one) at x[k] moving at v[k], appearing at tau, in the
+    // build-time leaf of particle particles[k] (where the reference takes leaf_of[i]); x is
+    // unwrapped against each ancestor's centre of mass in a box of side box (0: open).
+    void mass_events(const std::vector<uint32_t>& particles, const std::vector<double>& dm,
+                     const std::vector<Vec3d>& x, const std::vector<Vec3d>& v, double tau,
+                     double box = 0.0) {
+        const size_t ne = particles.size();
+        if (dm.size() != ne || x.size() != ne || v.size() != ne)
+            throw Error("mass_events: one mass, position and velocity per event");
+        if (ne == 0) return;
+        std::vector<double> buf(7 * ne);
+        for (size_t k = 0; k < ne; ++k) {
+            buf[k] = dm[k];
+            for (int d = 0; d < 3; ++d) {
+                buf[(1 + d) * ne + k] = x[k].data[d];
+                buf[(4 + d) * ne + k] = v[k].data[d];
+            }
+        }
+        auto& w = detail::ws();
+        const uint32_t* d_idx = w.idx.upload(particles);
+        const double* b = w.extra.upload(buf);
+        detail::check(gz_tree_mass_events(detail::ctx(), t_.get(), d_idx, ne, b, b + ne,
+                                          b + 2 * ne, b + 3 * ne, b + 4 * ne, b + 5 * ne,
+                                          b + 6 * ne, tau, box),
+                      "gz_tree_mass_events");
+    }
+    // Tree::shift_node (tree.h:324-325, tree.cc:253-258) for each event k: a particle of mass
+    // m[k] displaced by dx[k] without a change of velocity.
+    void shift_nodes(const std::vector<uint32_t>& particles, const std::vector<double>& m,
+                     const std::vector<Vec3d>& dx) {
+        const size_t ne = particles.size();
+        if (m.size() != ne || dx.size() != ne)
+            throw Error("shift_nodes: one mass and displacement per event");
+        if (ne == 0) return;
+        std::vector<double> buf(4 * ne);
+        for (size_t k = 0; k < ne; ++k) {
+            buf[k] = m[k];
+            for (int d = 0; d < 3; ++d) buf[(1 + d) * ne + k] = dx[k].data[d];
+        }
+        auto& w = detail::ws();
+        const uint32_t* d_idx = w.idx.upload(particles);
+        const double* b = w.extra.upload(buf);
+        detail::check(gz_tree_shift_nodes(detail::ctx(), t_.get(), d_idx, ne, b, b + ne,
+                                          b + 2 * ne, b + 3 * ne),
+                      "gz_tree_shift_nodes");
+    }
+    // Tree::raise_pad (tree.h:316-317, tree.cc:323-330) for each event k: a displacement d[k].
+    void raise_pad(const std::vector<uint32_t>& particles, const std::vector<double>& d) {
+        if (d.size() != particles.size()) throw Error("raise_pad: one displacement per event");
+        if (d.empty()) return;
+        auto& w = detail::ws();
+        const uint32_t* d_idx = w.idx.upload(particles);
+        const double* d_d = w.extra.upload(d);
+        detail::check(gz_tree_raise_pad(detail::ctx(), t_.get(), d_idx, d.size(), d_d),
+                      "gz_tree_raise_pad");
+    }
+    // Tree::raise_vmax (tree.h:303-309, tree.cc:276-285) for each event k: speed[k] >= 0.
+    void raise_vmax(const std::vector<uint32_t>& particles, const std::vector<float>& speed) {
+        if (speed.size() != particles.size()) throw Error("raise_vmax: one speed per event");
+        if (speed.empty()) return;
+        auto& w = detail::ws();
+        const uint32_t* d_idx = w.idx.upload(particles);
+        detail::DevArray<float> d_s;
+        const float* ds = d_s.upload(speed);
+        detail::check(gz_tree_raise_vmax(detail::ctx(), t_.get(), d_idx, speed.size(), ds),
+                      "gz_tree_raise_vmax");
+        detail::sync();
+    }
+
   private:
     friend Tree build(const Particles&, BuildTimes*, const double* const*, bool, long long);
     friend struct WalkAccess;
@@ -554,6 +624,78 @@ inline void ngb_search_reverse(const Tree& T, const Particles& P, const double* 
     ngb_search_reverse_many(T, P, h, n_gas, std::vector<Vec3d>{centre}, std::vector<double>{h_i},
                             f);
     found.insert(found.end(), f[0].begin(), f[0].end());
+}
+
+// shmem::DensityResult and shmem::NeighborCache (hydro.h:86-108).
+struct DensityResult {
+    std::vector<double> h, rho;
+    std::vector<int> nngb, iters;
+};
+struct NeighborCache {
+    std::vector<uint32_t> flat;
+    std::vector<size_t> start;
+    bool valid = false;
+    void clear() {
+        flat.clear();
+        start.clear();
+        valid = false;
+    }
+};
+
+// shmem::density (hydro.h:115-119, hydro.cc:150-327): h and rho for every target, with the
+// gas neighbours of Particles::type. Periodic boxes and lazy drift are not supported. With a
+// cache, each target's neighbour list at its final h (the reference's last search) is filled by
+// one more search.
+inline DensityResult density(const Tree& T, const Particles& P,
+                             const std::vector<uint32_t>& targets, double des_ngb, double ngb_tol,
+                             const std::vector<double>& h_start, double box = 0.0, int n_dims = 3,
+                             NeighborCache* cache = nullptr, const LazyDrift* lazy = nullptr,
+                             const double* cond_prev = nullptr) {
+    if (box != 0.0) throw Error("density: periodic boxes are not supported");
+    if (lazy) throw Error("density: lazy drift is not supported");
+    const size_t nt = targets.size();
+    if (!h_start.empty() && h_start.size() != nt) throw Error("density: one h_start per target");
+    DensityResult r;
+    r.h.resize(nt);
+    r.rho.resize(nt);
+    r.nngb.resize(nt);
+    r.iters.resize(nt);
+    if (cache) cache->clear();
+    if (nt > 0) {
+        const gz_particles p = WalkAccess::stage(T, P, "density");
+        auto& w = detail::ws();
+        const uint32_t* d_t = detail::stage_targets(targets);
+        const double* d_hs = h_start.empty() ? nullptr : w.aold.upload(h_start);
+        detail::DevArray<double> d_cond, d_out;
+        const double* d_c = cond_prev ? d_cond.upload(cond_prev, P.size()) : nullptr;
+        double* o = d_out.reserve(3 * nt);
+        int* oi = reinterpret_cast<int*>(o + 2 * nt);
+        detail::check(gz_tree_density(detail::ctx(), WalkAccess::tree(T), &p, d_t, nt, des_ngb,
+                                      ngb_tol, d_hs, n_dims, d_c, T.t_since_build, o, o + nt, oi,
+                                      oi + nt),
+                      "gz_tree_density");
+        detail::sync();
+        d_out.download(r.h.data(), nt);
+        detail::check_hip(hipMemcpy(r.rho.data(), o + nt, nt * sizeof(double),
+                                    hipMemcpyDeviceToHost), "copy");
+        detail::check_hip(hipMemcpy(r.nngb.data(), oi, nt * sizeof(int), hipMemcpyDeviceToHost),
+                          "copy");
+        detail::check_hip(hipMemcpy(r.iters.data(), oi + nt, nt * sizeof(int),
+                                    hipMemcpyDeviceToHost), "copy");
+    }
+    if (cache) {
+        std::vector<Vec3d> c(nt);
+        for (size_t k = 0; k < nt; ++k) c[k] = Vec3d{{P.x[targets[k]], P.y[targets[k]], P.z[targets[k]]}};
+        std::vector<std::vector<uint32_t>> f;
+        ngb_search_many(T, P, c, r.h, f);
+        cache->start.assign(nt + 1, 0);
+        for (size_t k = 0; k < nt; ++k) {
+            cache->start[k + 1] = cache->start[k] + f[k].size();
+            cache->flat.insert(cache->flat.end(), f[k].begin(), f[k].end());
+        }
+        cache->valid = true;
+    }
+    return r;
 }
 
 // shmem::accel_brute (tree.h:450-451): the exact FP64 sum over all particles, G included.

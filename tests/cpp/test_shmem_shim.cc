@@ -359,6 +359,53 @@ int main() {
         gzo_tree_free(tq);
     }
 
+    // ---- the density solve (hydro.cc:150-327) against the oracle's (pinned to the reference),
+    // with some neighbours not gas, and its neighbour cache against the oracle's search at the
+    // final h
+    {
+        gz_shmem::Particles D = plummer(6000, 1.0, 0.01, 777u);
+        D.type.assign(D.size(), 0);
+        for (size_t i = 0; i < D.size(); i += 9) D.type[i] = 4;
+        const gzo_particles hd = view(D);
+        void* td = gzo_build(&hd, -1);
+        gz_shmem::Tree TD = gz_shmem::build(D);
+        std::vector<uint32_t> dtg;
+        for (uint32_t i = 0; i < (uint32_t)D.size(); i += 3) dtg.push_back(i);
+        const size_t nd = dtg.size();
+        gz_shmem::NeighborCache cache;
+        const gz_shmem::DensityResult r =
+            gz_shmem::density(TD, D, dtg, 32.0, 0.5, std::vector<double>(), 0.0, 3, &cache);
+        std::vector<double> oh(nd), orho(nd);
+        std::vector<int> on(nd), oit(nd);
+        gzo_density(td, &hd, D.type.data(), dtg.data(), nd, 32.0, 0.5, nullptr, 3, nullptr, 0.0,
+                    oh.data(), orho.data(), on.data(), oit.data());
+        double eh = 0.0;
+        bool counts = true;
+        for (size_t k = 0; k < nd; ++k) {
+            eh = std::fmax(eh, std::fabs(r.h[k] - oh[k]) / oh[k]);
+            eh = std::fmax(eh, std::fabs(r.rho[k] - orho[k]) / orho[k]);
+            counts = counts && r.nngb[k] == on[k] && r.iters[k] == oit[k];
+        }
+        expect(counts && eh <= 1e-12,
+               fmt("density: iterations and counts equal the oracle's, h and rho within", eh));
+        std::vector<double> cx(nd), cy(nd), cz(nd);
+        for (size_t k = 0; k < nd; ++k) {
+            cx[k] = D.x[dtg[k]];
+            cy[k] = D.y[dtg[k]];
+            cz[k] = D.z[dtg[k]];
+        }
+        std::vector<size_t> ostart(nd + 1);
+        const size_t tot = gzo_ngb_search(td, &hd, cx.data(), cy.data(), cz.data(), oh.data(), nd,
+                                          0.0, ostart.data(), nullptr, 0);
+        std::vector<uint32_t> oflat(tot);
+        gzo_ngb_search(td, &hd, cx.data(), cy.data(), cz.data(), oh.data(), nd, 0.0,
+                       ostart.data(), oflat.data(), tot);
+        expect(cache.valid && cache.start == ostart && cache.flat == oflat,
+               "density: the neighbour cache equals the oracle's search at the final h (" +
+                   std::to_string(tot) + " pairs)");
+        gzo_tree_free(td);
+    }
+
     std::printf("%s: %d failure(s)\n", g_failures ? "shim FAILED" : "shim ok", g_failures);
     return g_failures ? 1 : 0;
 }
