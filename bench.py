@@ -195,9 +195,9 @@ def cpu_baseline(x, y, z, m, soft, orderbuf, leaf_of, aold_all, seconds, threads
 
 def widened_rows(gz, ics, args):
     """The path's other calls on the Hernquist 4M (a=1, eps=0.005), each beside the reference's
-    own CPU code (oracle/_ref) on a sample: the potential and the tidal-tensor walks (FP64,
-    theta=0.7, every particle a target; the reference on every 512th particle, all host
-    threads), and the forward and reverse neighbour searches (every 4th particle in Morton order
+    own CPU code (oracle/_ref) on a sample: the potential, per-target acceleration and
+    tidal-tensor walks (FP64, theta=0.7, every particle a target; the reference on every 512th
+    particle, all host threads), and the forward and reverse neighbour searches (every 4th particle in Morton order
     as a query, radius and kernel radius = the side of the particle's leaf; the reference's
     per-centre calls, one thread, on every 64th query), and the density / smoothing-length solve
     (every particle a gas target, DesNumNgb 32, tolerance 1, first guesses from the mean density
@@ -223,6 +223,8 @@ def widened_rows(gz, ics, args):
 
     _, dt = timed(lambda: gz.potential(T, P, tg, THETA_BOOT))
     out["potential_fp64"] = {"ms_per_call": dt * 1e3, "evals_per_s": n / dt}
+    _, dt = timed(lambda: gz.accel(T, P, tg, THETA_BOOT))
+    out["accel_per_target_fp64"] = {"ms_per_call": dt * 1e3, "evals_per_s": n / dt}
     _, dt = timed(lambda: gz.accel_grouped_tidal(T, P, tg, THETA_BOOT))
     out["tidal_fp64"] = {"ms_per_call": dt * 1e3, "evals_per_s": n / dt}
     order = td["orderbuf"].astype(np.int64)
@@ -259,6 +261,10 @@ def widened_rows(gz, ics, args):
             t0 = time.perf_counter()
             ref.potential(tr, rp, smp, THETA_BOOT)
             out["potential_fp64"]["cpu_reference_evals_per_s"] = smp.shape[0] / (
+                time.perf_counter() - t0)
+            t0 = time.perf_counter()
+            ref.accel(tr, rp, smp, THETA_BOOT)
+            out["accel_per_target_fp64"]["cpu_reference_evals_per_s"] = smp.shape[0] / (
                 time.perf_counter() - t0)
             t0 = time.perf_counter()
             ref.accel_grouped_tj(tr, rp, smp, THETA_BOOT)

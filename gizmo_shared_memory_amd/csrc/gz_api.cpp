@@ -58,7 +58,6 @@ struct WalkExtra {
     bool groups_only = false;   // stop after the grouping (walk plans)
     const uint32_t* seg = nullptr; // device: nseg + 1 slot offsets of independent target lists
     int nseg = 0;
-    bool potential = false;        // sum the potential into ax (ay, az unused)
     double* tidal = nullptr;       // tidal tensor out (6 x n_t), FP64 walks
     double* jerk = nullptr;        // jerk out (3 x n_t), with vx/vy/vz
     const double *vx = nullptr, *vy = nullptr, *vz = nullptr;
@@ -68,7 +67,7 @@ struct WalkExtra {
 // The walk shared by gz_tree_accel_grouped, gz_tree_accel and gz_tree_group_features.
 gz_status walk(gz_ctx* ctx, const gz_tree* tree, const gz_particles* p, const uint32_t* targets,
                size_t n_t, double theta, double G, int batch, int group_size, const double* aold,
-               int precision, int persistent, int leaf_always_open,
+               int precision, int persistent,
                int count, int stack_limit, int pack, double* ax, double* ay, double* az,
                gz_walk_counts* counts, const char* who, const WalkExtra& ex = WalkExtra()) {
     const bool features_only = ex.feats_out != nullptr;
@@ -79,7 +78,7 @@ gz_status walk(gz_ctx* ctx, const gz_tree* tree, const gz_particles* p, const ui
     if (p->n != tree->n)
         return fail(GZ_ERR_INVALID, "%s: particle count %zu differs from the tree's %zu", who,
                     p->n, tree->n);
-    const bool need_xyz = !features_only && !groups_only && !ex.potential;
+    const bool need_xyz = !features_only && !groups_only;
     if (n_t > 0 && (!targets || (need_xyz && (!ay || !az)) ||
                     (!features_only && !groups_only && !ax)))
         return fail(GZ_ERR_INVALID, "%s: null target or output array", who);
@@ -230,8 +229,6 @@ gz_status walk(gz_ctx* ctx, const gz_tree* tree, const gz_particles* p, const ui
     a.cluster_start = tree->cluster_start_dev;
     a.nclusters = tree->n_clusters;
     a.persistent = persistent;
-    a.leaf_always_open = leaf_always_open;
-    a.potential = ex.potential ? 1 : 0;
     if (ex.tidal) {
         if (precision != GZ_PREC_FP64)
             return fail(GZ_ERR_UNSUPPORTED, "%s: the tidal tensor is an FP64 walk", who);
@@ -304,6 +301,75 @@ gz_status walk(gz_ctx* ctx, const gz_tree* tree, const gz_particles* p, const ui
         counts->walk_ms = ev_ms(ev[9], ev[10]);
     }
     return GZ_OK;
+}
+
+// shmem::accel and shmem::potential (tree.cc:528-582, :899-944): every target walked on its own,
+// 64 Morton-neighbouring targets per wavefront (gz_walk_pt.hip). A target list that is not in
+// the tree's slot order is walked in that order (sorted on the device); outputs stay by list
+// position.
+gz_status walk_per_target(gz_ctx* ctx, const gz_tree* tree, const gz_particles* p,
+                          const uint32_t* targets, size_t n_t, double theta, double G,
+                          const double* aold, bool potential, double* ax, double* ay, double* az,
+                          const char* who) {
+    GZ_TRY(check_ctx(ctx));
+    if (!tree) return fail(GZ_ERR_INVALID, "%s: null tree", who);
+    GZ_TRY(check_particles(p, who));
+    if (p->n != tree->n)
+        return fail(GZ_ERR_INVALID, "%s: particle count %zu differs from the tree's %zu", who,
+                    p->n, tree->n);
+    if (n_t > 0 && (!targets || !ax || (!potential && (!ay || !az))))
+        return fail(GZ_ERR_INVALID, "%s: null target or output array", who);
+    if (n_t >= 0x7FFFFFFFull) return fail(GZ_ERR_INVALID, "%s: n_t=%zu too large", who, n_t);
+    if (n_t == 0) return GZ_OK;
+    if (tree->depth + 2 > gz::kPtDepth)
+        return fail(GZ_ERR_INTERNAL, "%s: tree depth %d exceeds the walk's stack", who,
+                    tree->depth);
+    hipStream_t st = ctx->stream;
+    Carver c;
+    const size_t o_slot = c.add<int>(n_t), o_k0 = c.add<uint64_t>(n_t),
+                 o_k1 = c.add<uint64_t>(n_t), o_v0 = c.add<uint32_t>(n_t),
+                 o_v1 = c.add<uint32_t>(n_t), o_misc = c.add<int>(64);
+    gz::DevMem& s = ctx->scratch[5];
+    GZ_TRY(s.reserve(c.used));
+    uint8_t* b = s.as<uint8_t>();
+    int* tslot = reinterpret_cast<int*>(b + o_slot);
+    uint64_t* k0 = reinterpret_cast<uint64_t*>(b + o_k0);
+    uint64_t* k1 = reinterpret_cast<uint64_t*>(b + o_k1);
+    uint32_t* v0 = reinterpret_cast<uint32_t*>(b + o_v0);
+    uint32_t* v1 = reinterpret_cast<uint32_t*>(b + o_v1);
+    int* misc = reinterpret_cast<int*>(b + o_misc);
+    GZ_TRY(gz::fill_zero_u32(reinterpret_cast<uint32_t*>(misc), 4, st));
+    GZ_TRY(gz::launch_pt_keys(targets, n_t, tree, p->n, tslot, k0, v0, misc, st));
+    int head[2] = {0, 0};
+    GZ_HIP(hipMemcpyAsync(head, misc, sizeof head, hipMemcpyDeviceToHost, st));
+    GZ_HIP(hipStreamSynchronize(st));
+    if (head[0] & 2)
+        return fail(GZ_ERR_INVALID, "%s: a target index is outside the particle set (n=%zu)", who,
+                    p->n);
+    const uint32_t* order = nullptr;
+    if (head[1]) {
+        int bits = 1;
+        while (bits < 40 && (1ull << bits) <= (unsigned long long)tree->n_live) ++bits;
+        GZ_TRY(gz::radix_sort_pairs(k0, v0, k1, v1, n_t, bits, ctx->scratch[3], st));
+        order = v0;
+    }
+    gz::PtArgs a{};
+    a.wn = tree->wn;
+    a.src = tree->src;
+    a.src_soft = tree->src_soft;
+    a.targets = targets;
+    a.tslot = tslot;
+    a.order = order;
+    a.x = p->x; a.y = p->y; a.z = p->z; a.soft = p->soft;
+    a.aold = aold;
+    a.theta_sq = theta * theta;
+    a.nt = (int)n_t;
+    a.root = tree->root;
+    a.cluster_start = tree->cluster_start_dev;
+    a.nclusters = tree->n_clusters;
+    a.G = G;
+    a.ax = ax; a.ay = ay; a.az = az;
+    return gz::launch_walk_pt(a, potential, st);
 }
 
 } // namespace
@@ -751,7 +817,7 @@ gz_status gz_tree_accel_grouped(gz_ctx* ctx, const gz_tree* tree, const gz_parti
     if (batch > 0 && group_size > 0)
         return fail(GZ_ERR_INVALID, "gz_tree_accel_grouped: batch > 0 and group_size > 0");
     return walk(ctx, tree, p, targets, n_t, theta, G, batch, group_size, aold, precision,
-                persistent, 0, count, stack_limit, 1, ax, ay, az, counts,
+                persistent, count, stack_limit, 1, ax, ay, az, counts,
                 "gz_tree_accel_grouped", ex);
 }
 
@@ -903,7 +969,7 @@ gz_status gz_tree_group_features(gz_ctx* ctx, const gz_tree* tree, const gz_part
     ex.gstart_out = group_start;
     ex.ngroups_out = n_groups;
     return walk(ctx, tree, p, targets, n_t, 0.0, 1.0, 0, group_size, aold, GZ_PREC_FP32_REL,
-                0, 0, 0, 0, 1, nullptr, nullptr, nullptr, nullptr, "gz_tree_group_features", ex);
+                0, 0, 0, 1, nullptr, nullptr, nullptr, nullptr, "gz_tree_group_features", ex);
 }
 
 // ---- walk plans ---------------------------------------------------------------------------------
@@ -997,7 +1063,7 @@ gz_status gz_walk_plan_create(gz_ctx* ctx, const gz_tree* tree, const gz_particl
         ex.seg = seg;
         ex.nseg = n_seg;
         s = walk(ctx, tree, p, targets, n_t, 0.0, 1.0, 0, group_size, nullptr, precision,
-                 persistent, 0, 0, 0, 1, nullptr, nullptr, nullptr, nullptr,
+                 persistent, 0, 0, 1, nullptr, nullptr, nullptr, nullptr,
                  "gz_walk_plan_create", ex);
         P->ngroups = (int)ng;
     }
@@ -1046,7 +1112,6 @@ gz_status gz_walk_plan_launch(gz_ctx* ctx, gz_walk_plan* P, double theta, double
     a.cluster_start = tree->cluster_start_dev;
     a.nclusters = tree->n_clusters;
     a.persistent = P->persistent;
-    a.leaf_always_open = 0;
     a.stack_limit = P->stack_limit;
     a.waves_per_cu = P->waves_per_cu;
     a.queue = reinterpret_cast<int*>(P->misc + 9); // eight ints: misc[9..12]
@@ -1314,10 +1379,8 @@ gz_status gz_tree_accel(gz_ctx* ctx, const gz_tree* tree, const gz_particles* p,
                         const uint32_t* targets, size_t n_t, double theta, double G,
                         const double* aold, double* ax, double* ay, double* az) {
     gz::clear_error();
-    // one group per target: the group box is the target's position, so the opening distance is
-    // |COM - x| per axis exactly as shmem::accel computes it (tree.cc:552-557)
-    return walk(ctx, tree, p, targets, n_t, theta, G, 1, 0, aold, GZ_PREC_FP64, 1, 1, 0,
-                0, 0, ax, ay, az, nullptr, "gz_tree_accel");
+    return walk_per_target(ctx, tree, p, targets, n_t, theta, G, aold, false, ax, ay, az,
+                           "gz_tree_accel");
 }
 
 gz_status gz_tree_accel_grouped_tidal(gz_ctx* ctx, const gz_tree* tree, const gz_particles* p,
@@ -1333,7 +1396,7 @@ gz_status gz_tree_accel_grouped_tidal(gz_ctx* ctx, const gz_tree* tree, const gz
     ex.vx = vx;
     ex.vy = vy;
     ex.vz = vz;
-    return walk(ctx, tree, p, targets, n_t, theta, G, batch, 0, aold, GZ_PREC_FP64, 1, 0, 0, 0, 0,
+    return walk(ctx, tree, p, targets, n_t, theta, G, batch, 0, aold, GZ_PREC_FP64, 1, 0, 0, 0,
                 ax, ay, az, nullptr, "gz_tree_accel_grouped_tidal", ex);
 }
 
@@ -1342,12 +1405,8 @@ gz_status gz_tree_potential(gz_ctx* ctx, const gz_tree* tree, const gz_particles
                             const double* aold, double* phi) {
     gz::clear_error();
     if (n_t > 0 && !phi) return fail(GZ_ERR_INVALID, "gz_tree_potential: null output");
-    // shmem::potential walks each target alone, as shmem::accel does: one group per target,
-    // every reached leaf summed particle by particle
-    WalkExtra ex;
-    ex.potential = true;
-    return walk(ctx, tree, p, targets, n_t, theta, G, 1, 0, aold, GZ_PREC_FP64, 1, 1, 0, 0, 0,
-                phi, nullptr, nullptr, nullptr, "gz_tree_potential", ex);
+    return walk_per_target(ctx, tree, p, targets, n_t, theta, G, aold, true, phi, nullptr,
+                           nullptr, "gz_tree_potential");
 }
 
 gz_status gz_direct_sum(gz_ctx* ctx, const gz_particles* p, const uint32_t* targets, size_t n_t,

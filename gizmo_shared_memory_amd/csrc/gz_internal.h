@@ -74,8 +74,6 @@ struct WalkArgs {
     const uint32_t* cluster_start; // forest: particle offsets of the clusters; NULL: one tree
     int nclusters;
     int persistent;
-    int leaf_always_open;   // 1: a reached leaf is always summed pair by pair (shmem::accel)
-    int potential;          // 1: sum the spline potential into ax instead (shmem::potential, FP64)
     // tidal tensor (6 x nt, no G) and jerk (3 x nt), FP64 only; NULL when not wanted. The jerk
     // needs particle velocities (pv*, by particle) and node velocities (nvel, 3 x nnodes).
     double* tidal;
@@ -98,6 +96,27 @@ struct WalkArgs {
                             // tier, and the retry when below the 512 entries one push can need)
     double G;
     double *ax, *ay, *az;
+};
+
+// Everything the per-target walk (shmem::accel, shmem::potential) reads. Passed by value.
+// kPtDepth: the walk's stack of sibling ranges, one entry per tree level below the root.
+constexpr int kPtDepth = 64;
+struct PtArgs {
+    const WNode* wn;
+    const double4* src;      // (x, y, z, m) per sorted slot
+    const double* src_soft;  // softening per sorted slot
+    const uint32_t* targets;
+    const int* tslot;        // sorted slot of each target, -1 if not in the tree
+    const uint32_t* order;   // list positions in walk (slot) order, or NULL: the list's own order
+    const double *x, *y, *z, *soft; // target data by particle, for targets outside the tree
+    const double* aold;      // per list position, or NULL
+    double theta_sq;
+    int nt;
+    int root;
+    const uint32_t* cluster_start; // forest: particle offsets of the clusters; NULL: one tree
+    int nclusters;
+    double G;
+    double *ax, *ay, *az;    // potential: ax only
 };
 
 // ---- errors ---------------------------------------------------------------------------------
@@ -163,6 +182,12 @@ gz_status launch_group_pack(int ngroups, size_t nt, int group_size, const int* g
 gz_status launch_walk(const WalkArgs& a, bool fp64, bool count, hipStream_t st);
 gz_status launch_walk_retry(const WalkArgs& a, int n_retry, bool fp64, bool count,
                             hipStream_t st);
+// ---- per-target walks (gz_walk_pt.hip) ---------------------------------------------------------
+// tslot, and (slot, list position) sort keys of the targets; misc[0] bit 1: a target index outside
+// the particle set, misc[1] != 0: the list is not in slot order. misc must start at zero.
+gz_status launch_pt_keys(const uint32_t* targets, size_t nt, const gz_tree* t, size_t n,
+                         int* tslot, uint64_t* keys, uint32_t* vals, int* misc, hipStream_t st);
+gz_status launch_walk_pt(const PtArgs& a, bool potential, hipStream_t st);
 gz_status launch_direct(const gz_particles* p, const uint32_t* targets, size_t nt, double G,
                         int fp64, double box, double* ax, double* ay, double* az,
                         hipStream_t st);

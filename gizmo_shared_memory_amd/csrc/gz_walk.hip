@@ -2,8 +2,9 @@
 // with an LDS interaction list (FP64 and FP32 group-relative), and the exact direct sum.
 //
 // Semantics follow shmem::accel_grouped (shmem/tree.cc:626-897) without lazy drift or a
-// sink-direct radius; the tidal tensor and jerk (tree.cc:811-871, the TJ instantiation) and
-// shmem::potential (tree.cc:899-944, per-target groups) are the same walk with other sums:
+// sink-direct radius; the tidal tensor and jerk (tree.cc:811-871, the TJ instantiation) are the
+// same walk with other sums (the per-target walks, shmem::accel and shmem::potential, are in
+// gz_walk_pt.hip):
 //   * groups are runs of Morton-ordered targets that share a leaf (tree.cc:656-667), or fixed
 //     windows of `batch` targets, or (group_size > 0) runs under the highest node that holds at most
 //     group_size particles;
@@ -27,6 +28,7 @@
 #include <cstdio>
 
 #include "gz_features.h"
+#include "gz_grav_math.h"
 #include "gz_wave.h"
 
 namespace gz {
@@ -45,86 +47,6 @@ constexpr int kPadF = 16;         // FP32 lists: zero-mass entries after the las
 static_assert(kMonoF >= 64 && kPairF >= 64 && kMonoD >= 64 && kPairD >= 64,
               "a list must take one batch of 64 entries after a flush");
 static_assert(kWalkStack >= 512, "the LDS stack must take the 512 children of one batch");
-
-// std::min / std::max of two numbers that are never NaN: one v_min_f64 / v_max_f64 (a compare
-// and two selects otherwise); the two forms differ only in the sign of a zero result.
-__device__ __forceinline__ double dmin_w(double a, double b) { return __builtin_fmin(a, b); }
-__device__ __forceinline__ double dmax_w(double a, double b) { return __builtin_fmax(a, b); }
-
-// ---- kernels of the pair force (tree.h:59-67), FP64 as the reference writes them --------------
-__device__ __forceinline__ double spline_force_over_r_d(double r, double h) {
-    const double h_inv = 1.0 / h;
-    const double h_inv3 = h_inv * h_inv * h_inv;
-    const double u = r * h_inv;
-    if (u >= 1.0) return 1.0 / (r * r * r);
-    if (u < 0.5) return h_inv3 * (10.666666666666667 + u * u * (32.0 * u - 38.4));
-    return h_inv3 * (21.333333333333332 - 48.0 * u + 38.4 * u * u - 10.666666666666667 * u * u * u -
-                     0.06666666666666667 / (u * u * u));
-}
-
-// 1/sqrt(x) for x > 0 in FP64 to about an ulp: the hardware estimate y (measured: within 2^-24
-// relative) and one third-order step y (1 + e/2 + 3 e^2/8), e = 1 - x y^2, which takes the error
-// from d to about 5 d^3 / 2 (measured: within 1.4e-16 relative over 2^-60..2^60, against 2.8e-16
-// for two Newton steps, in five operations instead of eight).
-__device__ __forceinline__ double rsqrt_d(double x) {
-    const double y = __builtin_amdgcn_rsq(x);
-    const double e = __builtin_fma(-(x * y), y, 1.0);
-    return __builtin_fma(y, e * __builtin_fma(e, 0.375, 0.5), y);
-}
-
-// spline_force_over_r from r^2, with r and 1/r^3 from rsqrt_d (a few ulp from the reference's
-// sqrt and division) and the spline's 1/u^3 as (h/r)^3; the branches and polynomials are the
-// reference's (tree.h:59-67).
-__device__ __forceinline__ double spline_force_over_r_r2(double r2, double h) {
-    const double inv = rsqrt_d(r2);
-    const double r = r2 * inv;
-    const double h_inv = 1.0 / h;
-    const double u = r * h_inv;
-    if (u >= 1.0) return inv * inv * inv;
-    const double h_inv3 = h_inv * h_inv * h_inv;
-    if (u < 0.5) return h_inv3 * (10.666666666666667 + u * u * (32.0 * u - 38.4));
-    const double hu = h * inv; // 1/u
-    return h_inv3 * (21.333333333333332 - 48.0 * u + 38.4 * u * u - 10.666666666666667 * u * u * u -
-                     0.06666666666666667 * (hu * hu * hu));
-}
-
-// spline_potential (tree.h:72-80), operand for operand: exactly -1/r beyond h.
-__device__ __forceinline__ double spline_potential_d(double r, double h) {
-    const double h_inv = 1.0 / h;
-    const double u = r * h_inv;
-    if (u >= 1.0) return -1.0 / r;
-    if (u < 0.5) return h_inv * (-2.8 + u * u * (5.333333333333333 + u * u * (6.4 * u - 9.6)));
-    return h_inv * (-3.2 + 0.06666666666666667 / u +
-                    u * u * (10.666666666666667 + u * (-16.0 + u * (9.6 - 2.1333333333333333 * u))));
-}
-
-// grav_tidal_factor (tree.h:86-96), operand for operand: the second kernel derivative, 3/r^5
-// beyond h.
-__device__ __forceinline__ double tidal_factor_d(double r, double h) {
-    const double h_inv = 1.0 / h;
-    const double u = r * h_inv;
-    if (u >= 1.0) return 3.0 / (r * r * r * r * r);
-    const double h_inv5 = h_inv * h_inv * h_inv * h_inv * h_inv;
-    double wk;
-    if (u < 0.5) wk = 76.8 - 96.0 * u;
-    else wk = -0.2 / (u * u * u * u * u) + 48.0 / u - 76.8 + 32.0 * u;
-    return wk * h_inv5;
-}
-
-// open_node (tree.h:109-144), operand for operand. The reference returns at the first arm that
-// holds; every arm is a side-effect-free predicate, so their OR is the same decision, and
-// evaluating all of them keeps the wave on one path (no divergent branches per node).
-__device__ __forceinline__ bool open_node_d(double r_sq, double len, double l_open, double mass,
-                                            double node_soft, double soft_t, double aold,
-                                            double theta_sq, double adx, double ady, double adz) {
-    const double grow_t = soft_t + 0.6 * len, grow_n = node_soft + 0.6 * len;
-    const bool soft_arm = (r_sq < grow_t * grow_t) | (r_sq < grow_n * grow_n);
-    const bool theta_arm = (theta_sq > 0.0) & (l_open * l_open >= theta_sq * r_sq);
-    const bool rel_arm = (aold > 0) ? (mass * len * len > r_sq * r_sq * aold) : (theta_sq <= 0.0);
-    const double box = 0.6 * len;
-    const bool inside = (adx < box) & (ady < box) & (adz < box);
-    return soft_arm | theta_arm | rel_arm | inside;
-}
 
 // ---- grouping ------------------------------------------------------------------------------------
 // Group key of each target slot: its leaf (batch <= 0, group_size == 0), the highest ancestor of the
@@ -580,27 +502,6 @@ __global__ __launch_bounds__(64, TJ ? 3 : ((FP64 && COUNT) ? 4 : 5)) void k_walk
                         tj(dx, dy, dz, (double)s.w * spline_force_over_r_d(r, h),
                            (double)s.w * tidal_factor_d(r, h), s_pext[j]);
                     }
-                } else if (FP64 && a.potential) {
-                    // shmem::potential (tree.cc:899-944): an accepted node lies outside both
-                    // softenings (open_node's first test), where the spline is exactly -1/r
-                    for (int j = jstart; j < mcount; j += L) {
-                        const real4 s = s_mono[j];
-                        const double dx = (double)s.x - (double)tx, dy = (double)s.y - (double)ty,
-                                     dz = (double)s.z - (double)tz;
-                        const double r = sqrt(dx * dx + dy * dy + dz * dz);
-                        if (r > 0) accx += (double)s.w * (-1.0 / r);
-                    }
-                    for (int j = jstart; j < pcount; j += L) {
-                        const real4 s = s_pair[j];
-                        const double es = (double)s_psoft[j];
-                        const double dx = (double)s.x - (double)tx, dy = (double)s.y - (double)ty,
-                                     dz = (double)s.z - (double)tz;
-                        const double r = sqrt(dx * dx + dy * dy + dz * dz);
-                        if (r <= 0) continue;
-                        if (COUNT && tvalid) ++g_pair;
-                        accx += (double)s.w *
-                                spline_potential_d(r, dmax_w(dmax_w((double)te, es), 1e-300));
-                    }
                 } else if (FP64) {
                     for (int j = jstart; j < mcount; j += L) {
                         // u >= 1 for every accepted node, so the spline is 1/r^3 (tree.h:63),
@@ -765,7 +666,6 @@ __global__ __launch_bounds__(64, TJ ? 3 : ((FP64 && COUNT) ? 4 : 5)) void k_walk
                     accept = !open_node_d(rmin2, (double)w.len, w.s, w.mass, (double)w.soft,
                                           gi.emax, gi.amin, gi.theta_sq, dx, dy, dz);
                     first = w.first;
-                    if (a.leaf_always_open && first < 0) accept = false; // tree.cc:554-570
                     if (!accept) {
                         if (first < 0) {
                             open_leaf = true;
@@ -1010,11 +910,9 @@ __global__ __launch_bounds__(64, TJ ? 3 : ((FP64 && COUNT) ? 4 : 5)) void k_walk
                 }
             }
             if (tvalid && sub == 0) {
-                a.ax[tlo + t] = a.G * accx; // the potential, in potential walks
-                if (!a.potential) {
-                    a.ay[tlo + t] = a.G * accy;
-                    a.az[tlo + t] = a.G * accz;
-                }
+                a.ax[tlo + t] = a.G * accx;
+                a.ay[tlo + t] = a.G * accy;
+                a.az[tlo + t] = a.G * accz;
             }
         }
         if (COUNT) {

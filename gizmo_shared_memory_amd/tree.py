@@ -736,6 +736,7 @@ def accel(T: Tree, P: Particles, targets, theta: float, G: float = 1.0, aold=Non
     L.check(L.load().gz_tree_accel(T.ctx.handle, T.handle, P.cstruct(), _ptr(tg), nt,
                                    float(theta), float(G), _ptr(ao), _ptr(ax), _ptr(ay),
                                    _ptr(az)), "gz_tree_accel")
+    T.ctx.synchronize()
     return ax, ay, az
 
 
