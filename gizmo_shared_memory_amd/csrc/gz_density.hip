@@ -4,13 +4,28 @@
 // The reference iterates each target privately: search the tree within h, sum the kernel over the
 // gas neighbours, and take a bracket-guarded Newton step on N_eff(h) = ball_vol(h) sum_j W until
 // N_eff is within the (widening) tolerance of the neighbour target. Here every still-unconverged
-// target advances one step per round: one neighbour search for all of them (the forward search's
-// CSR lists, each in the reference's traversal order), then one thread per target sums its list
-// in that order and applies the reference's update, then the targets that are done leave the
-// active list (a stable compaction, so the searches keep their Morton locality). Each target's
-// sums and steps are the reference's operations in the reference's order; the one difference is
-// pow(dx, 2) in the 3D face-closure test, taken here as dx * dx (correctly rounded; glibc's pow
-// is within half an ulp of it, not always equal).
+// target advances one step per round, and a round is one kernel (k_density_round): each target
+// has a lane, a wavefront holds up to 64 targets that are neighbours in Morton order, and the wave
+// walks the union of their own searches depth first, one node at a time (the nodes' search boxes
+// packed into 32-byte records per call). A lane waits on a node only while the reference's node
+// test (hydro.cc:55-72, on the float search nodes) has admitted its sphere at every ancestor; at
+// a reached leaf, each waiting lane tests every particle against its sphere (hydro.cc:80-87) and
+// notes a neighbour in its column of an LDS buffer; a lane adds its noted neighbours' kernel
+// terms whenever some column fills, and at the end. Depth first with children in octant order is
+// the tree's slot order, which is the order of the reference's neighbour list, so each lane adds
+// the reference's terms in the reference's order without a stored list. The lane then applies
+// the reference's update, and the targets that are done leave the active list (a stable
+// compaction, so the waves keep their Morton locality).
+//
+// Two cases leave this path. A target whose radius exceeds kFarLeaf times its leaf's side would
+// make its wave walk a large part of the tree one node at a time: it takes the list path instead
+// (the forward search's CSR lists, sorted into the reference's order, then one thread, or for a
+// long list one wave, per target). And once kLoopTargets or fewer targets remain, one launch
+// iterates each of them to the end instead of a round (and a host round trip) per step.
+//
+// The one difference from the reference's arithmetic is pow(dx, 2) in the 3D face-closure test,
+// taken here as dx * dx (correctly rounded; glibc's pow is within half an ulp of it, not always
+// equal).
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -22,6 +37,10 @@ namespace {
 
 constexpr double kPi = 3.14159265358979323846;
 constexpr int kMaxIter = 100; // hydro.cc:201
+constexpr int kWavesWanted = 2048; // a round's targets are spread thinner below this many waves
+constexpr int kNbBuf = 32;          // neighbours each lane holds before their terms are added
+constexpr double kFarLeaf = 4.0;    // radius over leaf side beyond which a target takes the lists
+constexpr uint32_t kLoopTargets = 2048; // active targets few enough to iterate in one launch
 constexpr size_t kMaxPairs = size_t(1) << 28; // neighbour pairs listed by one search
 
 // kernel_norm, ball_vol, kernel_w, kernel_dwdh (hydro.h:29-65): cubic spline of support h in dim
@@ -98,22 +117,49 @@ __device__ __forceinline__ bool invert_moments(const double* e, double* b, int d
     return true;
 }
 
+// One search node as the fused rounds read it: one 32-byte scalar load per visit. The centre and
+// half side are the float search node's (tree.cc:496-506); xpad and vmax as the tree holds them;
+// link is the first child, or ~plo for a leaf, and span the child count, or phi for a leaf.
+struct alignas(32) SRec {
+    float cx, cy, cz, half;
+    float xpad, vmax;
+    int link, span;
+};
+
 struct DensArgs {
-    const uint32_t* act; // active targets (positions in the target list)
+    const uint32_t* act; // active targets (positions in the target list), in slot order
     int n_act;
+    int tpw;             // targets per wavefront (a power of two, at most 64)
     const uint32_t* targets;
     const double *x, *y, *z, *m;
-    const uint8_t* type;  // null: all gas
-    const size_t* start;  // CSR of the active targets' neighbours (particle indices)
-    const uint32_t* found;
+    const uint8_t* type; // null: all gas
     double des_ngb, ngb_tol;
     int n_dims;
     const double* cond_prev; // by particle, or null
     double *h, *h_lo, *h_hi, *rho;
     int *iter, *nngb, *iters;
     uint32_t* keep; // 1: the target stays active
-    int* long_list; // positions (in the active list) of the targets with long neighbour lists
+    // the list path (targets whose sphere reaches far beyond their own leaf): the CSR of their
+    // neighbours (particle indices), and the long lists queued for a wave each
+    const size_t* start;
+    const uint32_t* found;
+    int* long_list;
     int* n_long;
+    // the fused rounds: a target whose radius exceeds kFarLeaf times its leaf's side goes to the
+    // list path instead (appended to far_list)
+    const int* tslot;     // sorted slot per target (list position), -1 outside the tree
+    const int* slot_leaf;
+    uint32_t* far_list;
+    uint32_t* n_far;
+    // the tree: its search nodes (built per call), the side of each node (the far test), the
+    // sorted copy of the particles and their indices, the roots, and the time since the build
+    // (the vmax pad)
+    const SRec* srec;
+    const double* size;
+    const double4* src;
+    const uint32_t* orderbuf;
+    int root, nclusters;
+    double tau;
 };
 
 // The terms one gas neighbour j adds to a target at (p0, p1, p2) with support h (hydro.cc:206-217):
@@ -258,10 +304,11 @@ __global__ __launch_bounds__(256) void k_density_step(DensArgs a) {
 }
 
 // The queued long lists, one wave per target (a grid-stride loop over the queue): the lanes
-// compute 64 neighbours' terms at a time into LDS, and lane 0 adds them in list order, so the
-// sums are those of the one-thread loop.
+// compute 64 neighbours' terms at a time into LDS, then lane c (c < 12) adds term c of each of
+// them in list order and lane 12 counts those inside h, so every sum is that of the one-thread
+// loop, term by term.
 __global__ __launch_bounds__(64) void k_density_step_long(DensArgs a) {
-    __shared__ double s_v[kTerms][64];
+    __shared__ double s_v[kTerms][65]; // rows padded so the 12 adding lanes hit distinct banks
     __shared__ int s_in[64];
     const int lane = threadIdx.x;
     const int nl = *a.n_long;
@@ -273,10 +320,8 @@ __global__ __launch_bounds__(64) void k_density_step_long(DensArgs a) {
         const double h = a.h[t];
         const int it = a.iter[t];
         const size_t b0 = a.start[k], b1 = a.start[k + 1];
-        DensSums S;
-#pragma unroll
-        for (int c = 0; c < kTerms; ++c) S.s[c] = 0.0;
-        S.n_inside = 0;
+        double acc = 0.0; // lane c < 12: term c's sum
+        int n_in = 0;     // lane 12: the count inside h
         for (size_t c0 = b0; c0 < b1; c0 += 64) {
             const size_t q = c0 + lane;
             double v[kTerms];
@@ -289,20 +334,180 @@ __global__ __launch_bounds__(64) void k_density_step_long(DensArgs a) {
             }
             s_in[lane] = flag;
             __syncthreads();
-            if (lane == 0) {
-                const int cnt = (int)min((size_t)64, b1 - c0);
-                for (int i = 0; i < cnt; ++i) {
-                    const int f = s_in[i];
-                    if (!(f & 1)) continue;
-#pragma unroll
-                    for (int c = 0; c < kTerms; ++c) S.s[c] += s_v[c][i];
-                    if (f & 2) ++S.n_inside;
-                }
+            const int cnt = (int)min((size_t)64, b1 - c0);
+            if (lane < kTerms) {
+                for (int i = 0; i < cnt; ++i)
+                    if (s_in[i] & 1) acc += s_v[lane][i];
+            } else if (lane == kTerms) {
+                for (int i = 0; i < cnt; ++i)
+                    if (s_in[i] & 2) ++n_in;
             }
             __syncthreads();
         }
+        DensSums S;
+#pragma unroll
+        for (int c = 0; c < kTerms; ++c) S.s[c] = __shfl(acc, c);
+        S.n_inside = __shfl(n_in, kTerms);
         if (lane == 0) density_update(a, k, t, p, h, it, S);
     }
+}
+
+// One step for every active target (see the top of the file): tpw targets per wavefront, lane l
+// holding active entry wave * tpw + l. A target whose radius reaches far beyond its own leaf would
+// make its wave walk a large part of the tree one node at a time; it is queued for the list path
+// (whose search tests 64 nodes at once) and takes no part here. With LOOP, each lane repeats its
+// steps until its target is done.
+template <bool LOOP>
+__global__ __launch_bounds__(64) void k_density_round(DensArgs a) {
+    const int lane = threadIdx.x;
+    // Block b runs on XCD b % 8: each XCD gets one contiguous share of the Morton-ordered waves.
+    const int b = blockIdx.x, nb = gridDim.x;
+    const int xcd = b & 7, per = nb >> 3, rem = nb & 7;
+    const int wv = xcd * per + min(xcd, rem) + (b >> 3);
+    const int k = wv * a.tpw + lane;
+    const bool live = lane < a.tpw && k < a.n_act;
+    const int dim = a.n_dims;
+    uint32_t t = 0, p = 0;
+    double qx = 0, qy = 0, qz = 0, h = 0;
+    int it = 0;
+    bool mine = false; // this lane steps its target here
+    if (live) {
+        t = a.act[k];
+        p = a.targets[t];
+        qx = a.x[p];
+        qy = a.y[p];
+        qz = a.z[p];
+        h = a.h[t];
+        it = a.iter[t];
+        mine = true;
+        const int ts = a.tslot[t];
+        if (!LOOP && ts >= 0 && h > kFarLeaf * a.size[a.slot_leaf[ts]]) {
+            a.far_list[atomicAdd(a.n_far, 1u)] = (uint32_t)k;
+            mine = false;
+        }
+    }
+    // Each lane's neighbours (particle indices, in traversal order) wait in its column of s_nb
+    // until some column is full or the walk ends; then every lane adds its waiting neighbours'
+    // terms, in order. A lane reads back only what it wrote itself, so no barrier is needed.
+    __shared__ uint32_t s_nb[kNbBuf][64];
+    DensSums S;
+    int nb_n = 0;
+    auto flush = [&]() {
+        for (int e = 0; __ballot(e < nb_n) != 0; ++e) {
+            if (e < nb_n) {
+                double v[kTerms];
+                bool inside = false;
+                if (density_terms(a, s_nb[e][lane], qx, qy, qz, h, dim, v, &inside)) {
+#pragma unroll
+                    for (int c = 0; c < kTerms; ++c) S.s[c] += v[c];
+                    if (inside) ++S.n_inside;
+                }
+            }
+        }
+        nb_n = 0;
+    };
+
+    for (;;) {
+        const uint64_t lanes = __ballot(mine);
+        if (lanes == 0) break;
+        const double r2 = h * h; // the search's radius_sq (hydro.cc:50)
+#pragma unroll
+        for (int c = 0; c < kTerms; ++c) S.s[c] = 0.0;
+        S.n_inside = 0;
+        // The current sibling range (next node, nodes left, lanes waiting on it) and the
+        // ranges above it, entry e in lane e of stf/stc/stl/sth: one per tree level, below the
+        // roots' range (a forest's roots are nodes 0..nc-1, searched in that order).
+        int cf = (a.nclusters > 1) ? 0 : a.root;
+        int cc = (a.nclusters > 1) ? a.nclusters : 1;
+        uint64_t cm = lanes;
+        int sp = 0;
+        int stf = 0, stc = 0, stl = 0, sth = 0;
+        for (;;) {
+            if (cc == 0) {
+                if (sp == 0) break;
+                --sp;
+                cf = __builtin_amdgcn_readlane(stf, sp);
+                cc = __builtin_amdgcn_readlane(stc, sp);
+                cm = ((uint64_t)(uint32_t)__builtin_amdgcn_readlane(sth, sp) << 32) |
+                     (uint64_t)(uint32_t)__builtin_amdgcn_readlane(stl, sp);
+            }
+            const int node = cf;
+            ++cf;
+            --cc;
+            const bool wait = (cm >> lane) & 1ull;
+            // the node test of the forward search, per lane (hydro.cc:55-72), exactly as the
+            // device search evaluates it
+            const SRec nr = a.srec[node];
+            double dist = h + (double)nr.half + (double)nr.xpad;
+            if (a.tau > 0) dist += (double)nr.vmax * a.tau;
+            const double dx = (double)nr.cx - qx, dy = (double)nr.cy - qy, dz = (double)nr.cz - qz;
+            bool hit = false;
+            if (wait && !(dx > dist || -dx > dist) && !(dy > dist || -dy > dist) &&
+                !(dz > dist || -dz > dist)) {
+                const double ds = dist + 0.7320508075688772 * (double)nr.half;
+                hit = !(dx * dx + dy * dy + dz * dz > ds * ds);
+            }
+            const uint64_t om = __ballot(hit);
+            if (om == 0) continue;
+            if (nr.link < 0) {
+                // the leaf's particles in slot order: a neighbour when within the radius (the
+                // tree's copy of the position, as the search tests it)
+                for (int s = ~nr.link; s < nr.span; ++s) {
+                    const double4 q = a.src[s];
+                    const double ex = q.x - qx, ey = q.y - qy, ez = q.z - qz;
+                    const bool in = hit && ex * ex + ey * ey + ez * ez < r2;
+                    if (__ballot(in) == 0) continue;
+                    if (__ballot(nb_n == kNbBuf) != 0) flush();
+                    if (in) s_nb[nb_n++][lane] = a.orderbuf[s];
+                }
+                continue;
+            }
+            // descend: the node's remaining siblings (if any) wait on the stack below its
+            // children
+            const bool top = lane == sp;
+            stf = top ? cf : stf;
+            stc = top ? cc : stc;
+            stl = top ? (int)(uint32_t)cm : stl;
+            sth = top ? (int)(uint32_t)(cm >> 32) : sth;
+            sp += (cc > 0) ? 1 : 0;
+            cf = nr.link;
+            cc = nr.span;
+            cm = om;
+        }
+        flush();
+        if (mine) density_update(a, k, t, p, h, it, S);
+        if (!LOOP) break;
+        if (mine && a.keep[k]) { // the next step, from the state the update left
+            h = a.h[t];
+            it = a.iter[t];
+        } else {
+            mine = false;
+        }
+    }
+}
+
+// The search nodes of the fused rounds, one per tree node.
+__global__ __launch_bounds__(256) void k_density_srec(int nn, const WNode* __restrict__ wn,
+                                                      const double* __restrict__ gcx,
+                                                      const double* __restrict__ gcy,
+                                                      const double* __restrict__ gcz,
+                                                      const double* __restrict__ size,
+                                                      const float* __restrict__ xpad,
+                                                      const float* __restrict__ vmax,
+                                                      double snode_eps, SRec* __restrict__ out) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= nn) return;
+    const WNode w = wn[i];
+    SRec r;
+    r.cx = (float)gcx[i];
+    r.cy = (float)gcy[i];
+    r.cz = (float)gcz[i];
+    r.half = snode_half_of(gcx[i], gcy[i], gcz[i], size[i], snode_eps);
+    r.xpad = xpad[i];
+    r.vmax = vmax[i];
+    r.link = (w.first >= 0) ? w.first : ~w.plo;
+    r.span = (w.first >= 0) ? w.nchild : w.phi;
+    out[i] = r;
 }
 
 // Start state: every target active at its first guess; a target outside the particle set flags
@@ -344,6 +549,22 @@ __global__ __launch_bounds__(256) void k_density_centres(int na, const uint32_t*
     cy[k] = y[p];
     cz[k] = z[p];
     r[k] = h[t];
+}
+
+// The far targets' entries in the active list: their target positions, for the list path.
+__global__ __launch_bounds__(256) void k_density_far_act(int nf, const uint32_t* __restrict__ far,
+                                                         const uint32_t* __restrict__ act,
+                                                         uint32_t* __restrict__ far_act) {
+    const int j = blockIdx.x * 256 + threadIdx.x;
+    if (j < nf) far_act[j] = act[far[j]];
+}
+
+// The list path's keep flags back into the active list's.
+__global__ __launch_bounds__(256) void k_density_far_keep(int nf, const uint32_t* __restrict__ far,
+                                                          const uint32_t* __restrict__ far_keep,
+                                                          uint32_t* __restrict__ keep) {
+    const int j = blockIdx.x * 256 + threadIdx.x;
+    if (j < nf) keep[far[j]] = far_keep[j];
 }
 
 // next[j] = cur[pos[j]] for the kept entries (pos from the flags' scan).
@@ -425,32 +646,31 @@ extern "C" gz_status gz_tree_density(gz_ctx* ctx, const gz_tree* tree, const gz_
     hipStream_t st = ctx->stream;
     const int nt = (int)n_t;
 
-    // state: h_lo, h_hi, the active centres and radii (doubles); the CSR offsets; iter, the
-    // active list, the compaction's output, the keep flags, their scan and the kept positions;
-    // the box keys, the scan total and the status word; the scan's temporary space
+    // state: h_lo, h_hi (doubles); iter, the active list, the compaction's output, the keep
+    // flags, their scan and the kept positions; the targets' slots and the sort's keys and
+    // values (twice); the box keys, the scan total and the status words; the scan's temporary
+    // space
     size_t off = 0;
     auto take = [&](size_t bytes) {
         const size_t o = off;
         off += (bytes + 255) & ~size_t(255);
         return o;
     };
-    const size_t o_lo = take(n_t * 8), o_hi = take(n_t * 8), o_cx = take(n_t * 8),
-                 o_cy = take(n_t * 8), o_cz = take(n_t * 8), o_r = take(n_t * 8),
-                 o_start = take((n_t + 1) * sizeof(size_t)), o_iter = take(n_t * 4),
+    const size_t o_lo = take(n_t * 8), o_hi = take(n_t * 8), o_iter = take(n_t * 4),
                  o_act = take(n_t * 4), o_nxt = take(n_t * 4), o_keep = take((n_t + 1) * 4),
                  o_scan = take((n_t + 1) * 4), o_kept = take((n_t + 1) * 4),
-                 o_long = take(n_t * 4), o_misc = take(256),
-                 o_tmp = take(scan_tmp_bytes(n_t + 1));
+                 o_slot = take(n_t * 4), o_k0 = take(n_t * 8), o_k1 = take(n_t * 8),
+                 o_v0 = take(n_t * 4), o_v1 = take(n_t * 4), o_far = take(n_t * 4),
+                 o_fact = take(n_t * 4), o_fkeep = take(n_t * 4), o_long = take(n_t * 4),
+                 o_cx = take(n_t * 8), o_cy = take(n_t * 8), o_cz = take(n_t * 8),
+                 o_r = take(n_t * 8), o_start = take((n_t + 1) * sizeof(size_t)),
+                 o_srec = take((size_t)std::max(tree->nnodes, 1) * sizeof(SRec)),
+                 o_misc = take(256), o_tmp = take(scan_tmp_bytes(n_t + 1));
     DevMem& mem = ctx->scratch[13];
     GZ_TRY(mem.reserve(off));
     uint8_t* base = mem.as<uint8_t>();
     double* d_lo = reinterpret_cast<double*>(base + o_lo);
     double* d_hi = reinterpret_cast<double*>(base + o_hi);
-    double* d_cx = reinterpret_cast<double*>(base + o_cx);
-    double* d_cy = reinterpret_cast<double*>(base + o_cy);
-    double* d_cz = reinterpret_cast<double*>(base + o_cz);
-    double* d_r = reinterpret_cast<double*>(base + o_r);
-    size_t* d_start = reinterpret_cast<size_t*>(base + o_start);
     int* d_iter = reinterpret_cast<int*>(base + o_iter);
     uint32_t* d_act = reinterpret_cast<uint32_t*>(base + o_act);
     uint32_t* d_nxt = reinterpret_cast<uint32_t*>(base + o_nxt);
@@ -460,8 +680,17 @@ extern "C" gz_status gz_tree_density(gz_ctx* ctx, const gz_tree* tree, const gz_
     unsigned long long* d_box = reinterpret_cast<unsigned long long*>(base + o_misc);
     uint32_t* d_total = reinterpret_cast<uint32_t*>(base + o_misc + 64);
     int* d_status = reinterpret_cast<int*>(base + o_misc + 128);
-    int* d_long = reinterpret_cast<int*>(base + o_long);
-    int* d_nlong = reinterpret_cast<int*>(base + o_misc + 192);
+    int* d_sort = reinterpret_cast<int*>(base + o_misc + 192); // bit 1: bad index; [1]: unsorted
+    uint32_t* d_nfar = reinterpret_cast<uint32_t*>(base + o_misc + 224);
+    int* d_nlong = reinterpret_cast<int*>(base + o_misc + 240);
+    uint32_t* d_far = reinterpret_cast<uint32_t*>(base + o_far);
+    uint32_t* d_fact = reinterpret_cast<uint32_t*>(base + o_fact);
+    uint32_t* d_fkeep = reinterpret_cast<uint32_t*>(base + o_fkeep);
+    double* d_cx = reinterpret_cast<double*>(base + o_cx);
+    double* d_cy = reinterpret_cast<double*>(base + o_cy);
+    double* d_cz = reinterpret_cast<double*>(base + o_cz);
+    double* d_r = reinterpret_cast<double*>(base + o_r);
+    size_t* d_start = reinterpret_cast<size_t*>(base + o_start);
     GZ_TRY(fill_zero_u32(reinterpret_cast<uint32_t*>(base + o_misc), 64, st));
 
     // the default first guess: des_ngb particles at the mean density of the bounding box of all
@@ -492,68 +721,130 @@ extern "C" gz_status gz_tree_density(gz_ctx* ctx, const gz_tree* tree, const gz_
     k_density_init<<<(unsigned)((n_t + 255) / 256), 256, 0, st>>>(
         nt, targets, p->n, h_start, h_default, h, d_lo, d_hi, d_iter, d_act, d_status);
     GZ_HIP(hipGetLastError());
-    int status = 0;
-    GZ_HIP(hipMemcpyAsync(&status, d_status, sizeof status, hipMemcpyDeviceToHost, st));
+    // the targets' sorted slots: a list out of slot order is walked in slot order (the waves'
+    // targets are then neighbours), the results staying by list position
+    uint64_t* k0 = reinterpret_cast<uint64_t*>(base + o_k0);
+    uint64_t* k1 = reinterpret_cast<uint64_t*>(base + o_k1);
+    uint32_t* v0 = reinterpret_cast<uint32_t*>(base + o_v0);
+    uint32_t* v1 = reinterpret_cast<uint32_t*>(base + o_v1);
+    GZ_TRY(launch_pt_keys(targets, n_t, tree, p->n, reinterpret_cast<int*>(base + o_slot), k0, v0,
+                          d_sort, st));
+    int head[3] = {0, 0, 0};
+    GZ_HIP(hipMemcpyAsync(head, d_status, sizeof(int), hipMemcpyDeviceToHost, st));
+    GZ_HIP(hipMemcpyAsync(head + 1, d_sort, 2 * sizeof(int), hipMemcpyDeviceToHost, st));
     GZ_HIP(hipStreamSynchronize(st));
-    if (status & 1)
+    if ((head[0] & 1) || (head[1] & 2))
         return fail(GZ_ERR_INVALID, "gz_tree_density: a target is outside the particle set");
+    if (head[2]) {
+        int bits = 1;
+        while (bits < 40 && (1ull << bits) <= (unsigned long long)tree->n_live) ++bits;
+        GZ_TRY(radix_sort_pairs(k0, v0, k1, v1, n_t, bits, ctx->scratch[3], st));
+        GZ_HIP(hipMemcpyAsync(d_act, v0, n_t * sizeof(uint32_t), hipMemcpyDeviceToDevice, st));
+    }
+    if (tree->depth + 2 > kPtDepth)
+        return fail(GZ_ERR_INTERNAL, "gz_tree_density: tree depth %d exceeds the walk's stack",
+                    tree->depth);
+
+    DensArgs a{};
+    a.targets = targets;
+    a.x = p->x;
+    a.y = p->y;
+    a.z = p->z;
+    a.m = p->m;
+    a.type = p->type;
+    a.des_ngb = des_ngb;
+    a.ngb_tol = ngb_tol;
+    a.n_dims = n_dims;
+    a.cond_prev = cond_prev;
+    a.h = h;
+    a.h_lo = d_lo;
+    a.h_hi = d_hi;
+    a.rho = rho;
+    a.iter = d_iter;
+    a.nngb = nngb;
+    a.iters = iters;
+    a.keep = d_keep;
+    a.tslot = reinterpret_cast<const int*>(base + o_slot);
+    a.slot_leaf = tree->slot_leaf;
+    a.far_list = d_far;
+    a.n_far = d_nfar;
+    a.long_list = reinterpret_cast<int*>(base + o_long);
+    a.n_long = d_nlong;
+    a.srec = reinterpret_cast<const SRec*>(base + o_srec);
+    if (tree->nnodes > 0) {
+        k_density_srec<<<(unsigned)((tree->nnodes + 255) / 256), 256, 0, st>>>(
+            tree->nnodes, tree->wn, tree->gcx, tree->gcy, tree->gcz, tree->size, tree->xpad,
+            tree->vmax, 1e-6 * tree->root_box[3], reinterpret_cast<SRec*>(base + o_srec));
+        GZ_HIP(hipGetLastError());
+    }
+    a.size = tree->size;
+    a.src = tree->src;
+    a.orderbuf = tree->orderbuf;
+    a.root = tree->root;
+    a.nclusters = tree->n_clusters;
+    a.tau = tau;
 
     // rounds: a target takes at most 100 steps and one last search
     uint32_t n_act = (uint32_t)n_t;
     for (int round = 0; n_act > 0; ++round) {
         if (round > kMaxIter + 1)
             return fail(GZ_ERR_INTERNAL, "gz_tree_density: the solve did not end");
-        uint32_t chunk = n_act;
-        const unsigned ga = (n_act + 255) / 256;
-        k_density_centres<<<ga, 256, 0, st>>>((int)n_act, d_act, targets, p->x, p->y, p->z, h,
-                                              d_cx, d_cy, d_cz, d_r);
+        // 64 targets per wave while that gives enough waves; fewer (down to one) when the active
+        // targets are too few to occupy the GPU that way (and often far apart, as the last
+        // unconverged targets are). The last few iterate to the end in one launch.
+        int tpw = 64;
+        while (tpw > 1 && (n_act + tpw - 1) / tpw < (uint32_t)kWavesWanted) tpw >>= 1;
+        a.act = d_act;
+        a.n_act = (int)n_act;
+        a.tpw = tpw;
+        const bool loop = n_act <= kLoopTargets;
+        GZ_TRY(fill_zero_u32(d_nfar, 1, st));
+        if (loop) k_density_round<true><<<(n_act + tpw - 1) / tpw, 64, 0, st>>>(a);
+        else k_density_round<false><<<(n_act + tpw - 1) / tpw, 64, 0, st>>>(a);
         GZ_HIP(hipGetLastError());
-        // the active targets in chunks whose neighbour lists hold at most kMaxPairs pairs (a GiB
-        // of indices): all at once unless a chunk's lists would hold more, then halves until
-        // they do not
-        for (uint32_t c0 = 0; c0 < n_act;) {
-            const uint32_t c1 = std::min<uint32_t>(n_act, c0 + chunk);
-            uint32_t* found = nullptr;
-            size_t total = 0;
-            const gz_status sr = ngb_search_grow(ctx, tree, d_cx + c0, d_cy + c0, d_cz + c0,
-                                                 d_r + c0, c1 - c0, tau, d_start,
-                                                 ctx->scratch[14], &found, &total, kMaxPairs);
-            if (sr == GZ_ERR_UNSUPPORTED && c1 - c0 > 1) {
-                chunk = std::max<uint32_t>(1, (c1 - c0) / 2);
-                continue;
+        uint32_t n_far = 0;
+        if (!loop) {
+            GZ_HIP(hipMemcpyAsync(&n_far, d_nfar, sizeof n_far, hipMemcpyDeviceToHost, st));
+            GZ_HIP(hipStreamSynchronize(st));
+        }
+        if (n_far > 0) {
+            // the far targets: one search for all of them (lists in the reference's order),
+            // then one thread (or, for a long list, one wave) per target sums its list
+            const unsigned gf = (n_far + 255) / 256;
+            k_density_far_act<<<gf, 256, 0, st>>>((int)n_far, d_far, d_act, d_fact);
+            k_density_centres<<<gf, 256, 0, st>>>((int)n_far, d_fact, targets, p->x, p->y, p->z,
+                                                  h, d_cx, d_cy, d_cz, d_r);
+            GZ_HIP(hipGetLastError());
+            // in chunks whose lists hold at most kMaxPairs pairs (a GiB of indices): all at once
+            // unless a chunk's lists would hold more, then halves until they do not
+            uint32_t chunk = n_far;
+            for (uint32_t c0 = 0; c0 < n_far;) {
+                const uint32_t c1 = std::min<uint32_t>(n_far, c0 + chunk);
+                uint32_t* found = nullptr;
+                size_t total = 0;
+                const gz_status sr = ngb_search_grow(ctx, tree, d_cx + c0, d_cy + c0, d_cz + c0,
+                                                     d_r + c0, c1 - c0, tau, d_start,
+                                                     ctx->scratch[14], &found, &total, kMaxPairs);
+                if (sr == GZ_ERR_UNSUPPORTED && c1 - c0 > 1) {
+                    chunk = std::max<uint32_t>(1, (c1 - c0) / 2);
+                    continue;
+                }
+                GZ_TRY(sr);
+                DensArgs f = a;
+                f.act = d_fact + c0;
+                f.n_act = (int)(c1 - c0);
+                f.start = d_start;
+                f.found = found;
+                f.keep = d_fkeep + c0;
+                GZ_TRY(fill_zero_u32(reinterpret_cast<uint32_t*>(d_nlong), 1, st));
+                k_density_step<<<(c1 - c0 + 255) / 256, 256, 0, st>>>(f);
+                GZ_HIP(hipGetLastError());
+                k_density_step_long<<<(unsigned)std::min<uint32_t>(c1 - c0, 4096), 64, 0, st>>>(f);
+                GZ_HIP(hipGetLastError());
+                c0 = c1;
             }
-            GZ_TRY(sr);
-            DensArgs a{};
-            a.act = d_act + c0;
-            a.n_act = (int)(c1 - c0);
-            a.targets = targets;
-            a.x = p->x;
-            a.y = p->y;
-            a.z = p->z;
-            a.m = p->m;
-            a.type = p->type;
-            a.start = d_start;
-            a.found = found;
-            a.des_ngb = des_ngb;
-            a.ngb_tol = ngb_tol;
-            a.n_dims = n_dims;
-            a.cond_prev = cond_prev;
-            a.h = h;
-            a.h_lo = d_lo;
-            a.h_hi = d_hi;
-            a.rho = rho;
-            a.iter = d_iter;
-            a.nngb = nngb;
-            a.iters = iters;
-            a.keep = d_keep + c0;
-            a.long_list = d_long;
-            a.n_long = d_nlong;
-            GZ_TRY(fill_zero_u32(reinterpret_cast<uint32_t*>(d_nlong), 1, st));
-            k_density_step<<<(c1 - c0 + 255) / 256, 256, 0, st>>>(a);
+            k_density_far_keep<<<gf, 256, 0, st>>>((int)n_far, d_far, d_fkeep, d_keep);
             GZ_HIP(hipGetLastError());
-            k_density_step_long<<<(unsigned)std::min<uint32_t>(c1 - c0, 4096), 64, 0, st>>>(a);
-            GZ_HIP(hipGetLastError());
-            c0 = c1;
         }
         // the targets still active, in their order: kept positions, then their target indices
         GZ_TRY(exclusive_scan_u32(d_keep, d_scan, n_act, d_total, base + o_tmp, st, d_kept));

@@ -3,6 +3,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <cfloat>
 #include <cstdint>
 #include <string>
 #include <utility>
@@ -97,6 +98,14 @@ struct WalkArgs {
     double G;
     double *ax, *ay, *az;
 };
+
+// The search node's half side (tree.cc:496-506) from the node's geometric centre and side: the
+// float the reference stores, widened so that it is never smaller than the true one.
+__device__ __forceinline__ float snode_half_of(double gx, double gy, double gz, double size,
+                                               double snode_eps) {
+    const double cmax = fmax(fabs(gx), fmax(fabs(gy), fabs(gz)));
+    return (float)(0.5 * size + snode_eps + 4.0 * (double)FLT_EPSILON * cmax);
+}
 
 // Everything the per-target walk (shmem::accel, shmem::potential) reads. Passed by value.
 // kPtDepth: the walk's stack of sibling ranges, one entry per tree level below the root.

@@ -51,8 +51,7 @@ struct NgbArgs {
 
 // The search node's half side (tree.cc:496-506): float-safe, never smaller than the true one.
 __device__ __forceinline__ float snode_half(const NgbArgs& a, int i) {
-    const double cmax = fmax(fabs(a.gcx[i]), fmax(fabs(a.gcy[i]), fabs(a.gcz[i])));
-    return (float)(0.5 * a.size[i] + a.snode_eps + 4.0 * (double)FLT_EPSILON * cmax);
+    return snode_half_of(a.gcx[i], a.gcy[i], a.gcz[i], a.size[i], a.snode_eps);
 }
 
 // REV: the reverse search (hydro.cc:110-148). radius[q] is then the query's own kernel radius h_q,
@@ -327,11 +326,11 @@ __global__ __launch_bounds__(64) void k_ngb_finish(int nq, const size_t* __restr
 
 
 
-// Many pairs with some lists too long for the LDS sort: every list sorted at once by a device
-// radix sort of (query << 32 | slot) keys, which keeps each query's range of the CSR and does not
-// slow down on long lists (short lists sort faster in k_ngb_finish). One wave per query writes
-// its keys.
-constexpr size_t kRadixFinishMin = size_t(1) << 20; // pairs
+// Some lists too long for the LDS sort: every list sorted at once by a device radix sort of
+// (query << 32 | slot) keys, which keeps each query's range of the CSR and does not slow down on
+// long lists (lists that all fit the LDS sort are faster in k_ngb_finish; past kFinishLds it
+// falls back to a one-lane Shell sort, hundreds of times slower on lists of 10^4-10^5). One wave
+// per query writes its keys.
 constexpr size_t kRadixFinishMax = size_t(1) << 28; // 24 B of keys and values per pair
 __global__ __launch_bounds__(64) void k_ngb_keys(int nq, const size_t* __restrict__ start,
                                                  const uint32_t* __restrict__ slots,
@@ -581,8 +580,7 @@ static gz_status ngb_run(gz_ctx* ctx, const gz_tree* tree, const double* cx, con
     a.start = start;
     a.slots = sl.as<uint32_t>();
     GZ_TRY(ngb_pass(a, true, rev, waves, st));
-    if (h_max > (uint32_t)kFinishLds && (size_t)tot32 >= kRadixFinishMin &&
-        (size_t)tot32 <= kRadixFinishMax) {
+    if (h_max > (uint32_t)kFinishLds && (size_t)tot32 <= kRadixFinishMax) {
         DevMem& kb = ctx->scratch[15];
         const size_t nk = tot32;
         GZ_TRY(kb.reserve(nk * (2 * sizeof(uint64_t) + 2 * sizeof(uint32_t))));
