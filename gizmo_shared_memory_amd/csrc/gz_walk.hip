@@ -478,29 +478,38 @@ __global__ __launch_bounds__(64, TJ ? 3 : ((FP64 && COUNT) ? 4 : 5)) void k_walk
                             jj[2] += g1 * dvz - vdotr * g2 * dz;
                         }
                     };
+                    // an accepted node lies outside both softenings (u >= 1), where the two
+                    // factors are 1/r^3 and 3/r^5 (tree.h:63, :90): both from rsqrt_d, a few ulp
+                    // from the reference's square root and divisions
                     for (int j = jstart; j < mcount; j += L) {
                         const real4 s = s_mono[j];
                         const double4 v = s_mext[j];
                         const double dx = (double)s.x - (double)tx, dy = (double)s.y - (double)ty,
                                      dz = (double)s.z - (double)tz;
-                        const double r2 = dx * dx + dy * dy + dz * dz;
+                        const double r2 = __builtin_fma(dx, dx, __builtin_fma(dy, dy, dz * dz));
                         if (r2 <= 0) continue;
-                        const double r = sqrt(r2);
-                        const double es = dmax_w(dmax_w((double)te, v.w), 1e-300);
-                        tj(dx, dy, dz, (double)s.w * spline_force_over_r_d(r, es),
-                           (double)s.w * tidal_factor_d(r, es), v);
+                        const double inv = rsqrt_d(r2), inv2 = inv * inv;
+                        const double g1 = (double)s.w * (inv2 * inv);
+                        tj(dx, dy, dz, g1, 3.0 * g1 * inv2, v);
                     }
+                    // particles: the same beyond the kernel (r^2 >= h^2), the reference's
+                    // spline factors inside it (r from r^2 and 1/r)
                     for (int j = jstart; j < pcount; j += L) {
                         const real4 s = s_pair[j];
                         const double es = (double)s_psoft[j];
                         const double dx = (double)s.x - (double)tx, dy = (double)s.y - (double)ty,
                                      dz = (double)s.z - (double)tz;
-                        const double r2 = dx * dx + dy * dy + dz * dz;
+                        const double r2 = __builtin_fma(dx, dx, __builtin_fma(dy, dy, dz * dz));
                         if (r2 <= 0) continue;
-                        const double r = sqrt(r2);
                         const double h = dmax_w(dmax_w((double)te, es), 1e-300);
-                        tj(dx, dy, dz, (double)s.w * spline_force_over_r_d(r, h),
-                           (double)s.w * tidal_factor_d(r, h), s_pext[j]);
+                        const double inv = rsqrt_d(r2), inv2 = inv * inv;
+                        double f1 = inv2 * inv, f2 = 3.0 * f1 * inv2;
+                        if (r2 < h * h) {
+                            const double r = r2 * inv;
+                            f1 = spline_force_over_r_d(r, h);
+                            f2 = tidal_factor_d(r, h);
+                        }
+                        tj(dx, dy, dz, (double)s.w * f1, (double)s.w * f2, s_pext[j]);
                     }
                 } else if (FP64) {
                     for (int j = jstart; j < mcount; j += L) {
