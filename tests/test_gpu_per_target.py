@@ -120,3 +120,36 @@ def test_per_target_errors(gz):
         gz.potential(T, P, [5, 1 << 20], 0.7)
     assert gz.potential(T, P, [], 0.7).numel() == 0
     assert gz.accel(T, P, [], 0.7)[0].numel() == 0
+
+
+def test_hernquist_1m_sampled(gz, oracle):
+    """Both per-target walks on a 1M Hernquist sphere, every particle a target (65,536 waves'
+    worth of unions), checked on a random sample against the oracle in both regimes."""
+    from oracle.bindings import HostParticles
+    x, y, z, m, soft = ics.hernquist(n=1 << 20, eps=0.005, seed=31)
+    n = x.shape[0]
+    print(f"[per-target] hernquist n={n}", flush=True)
+    P = gz.Particles(x, y, z, m, soft)
+    T = gz.build(P)
+    tg = T.device_orderbuf()
+    pick = np.sort(np.random.default_rng(6).choice(n, 2000, replace=False))
+    tg_np = tg.view(torch.int32).cpu().numpy().astype(np.uint32)
+    hp = HostParticles(x, y, z, m, soft)
+    to = oracle.build(hp)
+    try:
+        # aold for every target (by list position) from the grouped walk at theta = 0.7
+        boot = gz.accel_grouped(T, P, tg, 0.7, 1.0, 0)
+        aold = 0.0025 * np.linalg.norm(np.stack([_np(v) for v in boot], 1), axis=1)
+        assert np.all(aold > 0)
+        aold_s = aold[pick]
+        for theta, ao, ao_s in ((0.7, None, None), (0.0, aold, aold_s)):
+            got = gz.accel(T, P, tg, theta, 1.0, ao)
+            g = np.stack([_np(v)[pick] for v in got], 1)
+            o = np.stack(oracle.accel(to, hp, tg_np[pick], theta, 1.0, ao_s), 1)
+            e = np.linalg.norm(g - o, axis=1) / np.linalg.norm(o, axis=1)
+            assert e.max() < 1e-12, (theta, e.max())
+            phi = _np(gz.potential(T, P, tg, theta, 1.0, ao))[pick]
+            po = oracle.potential(to, hp, tg_np[pick], theta, 1.0, ao_s)
+            assert np.all(np.abs(phi - po) <= 1e-12 * np.abs(po)), theta
+    finally:
+        oracle.free(to)
