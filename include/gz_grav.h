@@ -314,7 +314,9 @@ gz_status gz_walk_autotune(gz_ctx* ctx, const gz_tree* tree, const gz_particles*
                            gz_walk_opts* best, gz_tune_report* report);
 
 /* shmem::accel (tree.h:392-394, tree.cc:528-582): per-target walk, a reached leaf is always
- * direct-summed. */
+ * direct-summed. Targets may come in any order and repeat (a list out of the tree's slot order is
+ * walked in slot order); outputs and aold are by list position. Asynchronous on the context's
+ * stream once the target list has been checked. */
 gz_status gz_tree_accel(gz_ctx* ctx, const gz_tree* tree, const gz_particles* p,
                         const uint32_t* targets, size_t n_t, double theta, double G,
                         const double* aold, double* ax, double* ay, double* az);
@@ -380,7 +382,8 @@ gz_status gz_tree_density(gz_ctx* ctx, const gz_tree* tree, const gz_particles* 
                           double* h, double* rho, int* nngb, int* iters);
 
 /* shmem::potential (tree.h:446-447, tree.cc:899-944): the spline-softened potential at each
- * target, G included, walked per target like gz_tree_accel (leaves always summed), FP64. */
+ * target, G included, walked per target like gz_tree_accel (leaves always summed), FP64, with the
+ * same target-order rules. */
 gz_status gz_tree_potential(gz_ctx* ctx, const gz_tree* tree, const gz_particles* p,
                             const uint32_t* targets, size_t n_t, double theta, double G,
                             const double* aold, double* phi);
