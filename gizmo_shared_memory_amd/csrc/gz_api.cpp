@@ -364,6 +364,10 @@ gz_status walk_per_target(gz_ctx* ctx, const gz_tree* tree, const gz_particles* 
     a.aold = aold;
     a.theta_sq = theta * theta;
     a.nt = (int)n_t;
+    // 64 targets per wave while that gives 2,048 waves or more; fewer (down to one) for small
+    // target sets, whose targets are often far apart
+    a.tpw = 64;
+    while (a.tpw > 1 && (n_t + a.tpw - 1) / a.tpw < 2048) a.tpw >>= 1;
     a.root = tree->root;
     a.cluster_start = tree->cluster_start_dev;
     a.nclusters = tree->n_clusters;

@@ -121,6 +121,7 @@ struct PtArgs {
     const double* aold;      // per list position, or NULL
     double theta_sq;
     int nt;
+    int tpw;                 // targets per wavefront (a power of two, at most 64)
     int root;
     const uint32_t* cluster_start; // forest: particle offsets of the clusters; NULL: one tree
     int nclusters;

@@ -5,8 +5,8 @@
 // (leaves are always opened in these two walks) is summed particle by particle, the target itself
 // skipped.
 //
-// How it runs on CDNA: one wavefront takes 64 targets that are neighbours in Morton order, one
-// per lane, and walks the union of their 64 traversals depth first. The wave visits one node at a
+// How it runs on CDNA: one wavefront takes 64 targets that are neighbours in Morton order (fewer,
+// down to one, when there are too few targets to occupy the GPU that way), one per lane, and walks the union of their 64 traversals depth first. The wave visits one node at a
 // time; the node's index is wave-uniform, so its 64-byte record comes through the scalar cache
 // and every lane still waiting on it tests it against its own target. Lanes that accept it add
 // its monopole; the lanes that open it are the mask under which its children are visited. Each
@@ -67,8 +67,8 @@ __global__ __launch_bounds__(64) void k_walk_pt(PtArgs a) {
     const int b = blockIdx.x, nb = gridDim.x;
     const int xcd = b & 7, per = nb >> 3, rem = nb & 7;
     const int wv = xcd * per + min(xcd, rem) + (b >> 3);
-    const int k = wv * 64 + lane;
-    const bool valid = k < a.nt;
+    const int k = wv * a.tpw + lane;
+    const bool valid = lane < a.tpw && k < a.nt;
     int i = 0, my = -1, croot = 0;
     double px = 0, py = 0, pz = 0, pe = 0, ao = 0;
     if (valid) {
@@ -214,7 +214,7 @@ gz_status launch_pt_keys(const uint32_t* targets, size_t nt, const gz_tree* t, s
 
 gz_status launch_walk_pt(const PtArgs& a, bool potential, hipStream_t st) {
     if (a.nt <= 0) return GZ_OK;
-    const unsigned grid = (unsigned)((a.nt + 63) / 64);
+    const unsigned grid = (unsigned)((a.nt + a.tpw - 1) / a.tpw);
     const bool rel = a.aold != nullptr;
     if (potential && rel) k_walk_pt<true, true><<<grid, 64, 0, st>>>(a);
     else if (potential) k_walk_pt<true, false><<<grid, 64, 0, st>>>(a);
