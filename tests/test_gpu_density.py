@@ -136,3 +136,40 @@ def test_density_errors(gz):
         gz.density(T, P, [0, 1], des_ngb=0.0)
     h, rho, nngb, iters = gz.density(T, P, [])
     assert h.numel() == 0
+
+
+def test_density_on_a_forest(gz):
+    """A forest's search covers every cluster's tree in root order (as the neighbour search
+    does: the forest is one particle set). Each lane's sums depend only on its own search, so
+    the targets of all clusters mixed in one call give exactly what each cluster's targets give
+    alone. Against a tree built over one cluster alone the results agree except where a search
+    box differs: the search nodes' half sides are widened by 1e-6 of the root cell's side
+    (tree.cc:496), which for a forest is the cell around all of its clusters."""
+    from gizmo_shared_memory_amd import ics
+    (x, y, z, m, soft), starts = ics.spheres(n_spheres=4, lo_log2=10.0, hi_log2=12.0, eps=0.01,
+                                             spacing=1e6, seed=9)
+    n = x.shape[0]
+    print(f"[density] forest of 4, n={n}", flush=True)
+    P = gz.Particles(x, y, z, m, soft)
+    F = gz.build(P, clusters=starts)
+    tg = np.random.default_rng(2).permutation(n).astype(np.uint32)  # clusters mixed in waves
+    h0 = np.full(n, 0.05)
+    mixed = [t.cpu().numpy() for t in gz.density(F, P, tg, des_ngb=32.0, ngb_tol=0.5, h_start=h0)]
+    pos = np.empty(n, dtype=np.int64)
+    pos[tg] = np.arange(n)
+    differ = 0
+    for c in range(4):
+        lo, hi = int(starts[c]), int(starts[c + 1])
+        cl = np.arange(lo, hi, dtype=np.uint32)
+        alone = [t.cpu().numpy() for t in gz.density(F, P, cl, des_ngb=32.0, ngb_tol=0.5,
+                                                     h_start=h0[lo:hi])]
+        for u, v in zip(mixed, alone):
+            assert np.array_equal(u[pos[lo:hi]], v), c
+        Pc = gz.Particles(x[lo:hi], y[lo:hi], z[lo:hi], m[lo:hi], soft[lo:hi])
+        Tc = gz.build(Pc)
+        own = [t.cpu().numpy() for t in gz.density(Tc, Pc, cl - lo, des_ngb=32.0, ngb_tol=0.5,
+                                                   h_start=h0[lo:hi])]
+        differ += int(np.count_nonzero((alone[0] != own[0]) | (alone[3] != own[3])))
+    print(f"[density] {differ} of {n} targets differ from the clusters' own trees", flush=True)
+    assert differ <= n // 100
+    assert np.all(mixed[0] > 0)
