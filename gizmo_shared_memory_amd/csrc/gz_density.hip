@@ -17,11 +17,12 @@
 // the reference's update, and the targets that are done leave the active list (a stable
 // compaction, so the waves keep their Morton locality).
 //
-// Two cases leave this path. A target whose radius exceeds kFarLeaf times its leaf's side would
-// make its wave walk a large part of the tree one node at a time: it takes the list path instead
-// (the forward search's CSR lists, sorted into the reference's order, then one thread, or for a
-// long list one wave, per target). And once kLoopTargets or fewer targets remain, one launch
-// iterates each of them to the end instead of a round (and a host round trip) per step.
+// Two cases leave this path for k_density_far, where a wavefront takes a single target, tests a
+// node's children together and a leaf's particles together, and adds the terms in the same order.
+// A target whose radius exceeds kFarLeaf times its leaf's side would make its wave of 64 walk a
+// large part of the tree one node and one particle at a time: it takes a wave of its own after the
+// round. And once kLoopTargets or fewer targets remain, one launch gives each a wave that iterates
+// it to the end, instead of a round (and a host round trip) per step.
 //
 // The one difference from the reference's arithmetic is pow(dx, 2) in the 3D face-closure test,
 // taken here as dx * dx (correctly rounded; glibc's pow is within half an ulp of it, not always
@@ -40,8 +41,7 @@ constexpr int kMaxIter = 100; // hydro.cc:201
 constexpr int kWavesWanted = 2048; // a round's targets are spread thinner below this many waves
 constexpr int kNbBuf = 32;          // neighbours each lane holds before their terms are added
 constexpr double kFarLeaf = 4.0;    // radius over leaf side beyond which a target takes the lists
-constexpr uint32_t kLoopTargets = 2048; // active targets few enough to iterate in one launch
-constexpr size_t kMaxPairs = size_t(1) << 28; // neighbour pairs listed by one search
+constexpr uint32_t kLoopTargets = 131072; // active targets few enough to iterate in one launch
 
 // kernel_norm, ball_vol, kernel_w, kernel_dwdh (hydro.h:29-65): cubic spline of support h in dim
 // dimensions, operand for operand.
@@ -126,6 +126,21 @@ struct alignas(32) SRec {
     int link, span;
 };
 
+// A read of memory no kernel of the solve writes (the tree's records), through the constant
+// address space: at a wave-uniform address it is a scalar load, served by the scalar cache.
+template <class T>
+__device__ __forceinline__ T ld_const(const T* p) {
+    static_assert(sizeof(T) % 4 == 0, "read as whole dwords");
+    const __attribute__((address_space(4))) uint32_t* q =
+        (const __attribute__((address_space(4))) uint32_t*)p;
+    uint32_t w[sizeof(T) / 4];
+#pragma unroll
+    for (unsigned i = 0; i < sizeof(T) / 4; ++i) w[i] = q[i];
+    T v;
+    __builtin_memcpy(&v, w, sizeof(T));
+    return v;
+}
+
 struct DensArgs {
     const uint32_t* act; // active targets (positions in the target list), in slot order
     int n_act;
@@ -139,14 +154,8 @@ struct DensArgs {
     double *h, *h_lo, *h_hi, *rho;
     int *iter, *nngb, *iters;
     uint32_t* keep; // 1: the target stays active
-    // the list path (targets whose sphere reaches far beyond their own leaf): the CSR of their
-    // neighbours (particle indices), and the long lists queued for a wave each
-    const size_t* start;
-    const uint32_t* found;
-    int* long_list;
-    int* n_long;
-    // the fused rounds: a target whose radius exceeds kFarLeaf times its leaf's side goes to the
-    // list path instead (appended to far_list)
+    // the rounds: a target whose radius exceeds kFarLeaf times its leaf's side is left to a wave
+    // of its own (appended to far_list)
     const int* tslot;     // sorted slot per target (list position), -1 outside the tree
     const int* slot_leaf;
     uint32_t* far_list;
@@ -155,6 +164,7 @@ struct DensArgs {
     // sorted copy of the particles and their indices, the roots, and the time since the build
     // (the vmax pad)
     const SRec* srec;
+    const WNode* wn;
     const double* size;
     const double4* src;
     const uint32_t* orderbuf;
@@ -271,93 +281,204 @@ __device__ void density_update(const DensArgs& a, int k, uint32_t t, uint32_t p,
     a.keep[k] = 1;
 }
 
-// Lists longer than this go to k_density_step_long (one wave each).
-constexpr int kLongList = 192;
-
-// One step for every active target with a short list, one thread each; long lists are queued.
-__global__ __launch_bounds__(256) void k_density_step(DensArgs a) {
-    const int k = blockIdx.x * 256 + threadIdx.x;
-    if (k >= a.n_act) return;
-    const size_t b0 = a.start[k], b1 = a.start[k + 1];
-    if (b1 - b0 > (size_t)kLongList) {
-        a.long_list[atomicAdd(a.n_long, 1)] = k;
-        return;
-    }
-    const uint32_t t = a.act[k];
-    const uint32_t p = a.targets[t];
-    const double p0 = a.x[p], p1 = a.y[p], p2 = a.z[p];
-    const double h = a.h[t];
-    const int it = a.iter[t];
-    DensSums S;
-#pragma unroll
-    for (int c = 0; c < kTerms; ++c) S.s[c] = 0.0;
-    S.n_inside = 0;
-    for (size_t q = b0; q < b1; ++q) {
-        double v[kTerms];
-        bool in = false;
-        if (!density_terms(a, a.found[q], p0, p1, p2, h, a.n_dims, v, &in)) continue;
-#pragma unroll
-        for (int c = 0; c < kTerms; ++c) S.s[c] += v[c];
-        if (in) ++S.n_inside;
-    }
-    density_update(a, k, t, p, h, it, S);
-}
-
-// The queued long lists, one wave per target (a grid-stride loop over the queue): the lanes
-// compute 64 neighbours' terms at a time into LDS, then lane c (c < 12) adds term c of each of
-// them in list order and lane 12 counts those inside h, so every sum is that of the one-thread
-// loop, term by term.
-__global__ __launch_bounds__(64) void k_density_step_long(DensArgs a) {
+// One step for each of its targets (a.act[e] for block e), one wavefront per target walking that
+// target's own search depth first. The lanes test a node's children together (a forest's roots up
+// to 64 at a time), so the wave takes one step per admitted interior node; the admitted leaves
+// among a node's children up to its next admitted interior child are taken together, their
+// particles (contiguous slot ranges) spread over the lanes in slot order, tested against the
+// sphere, and the neighbours' terms computed into LDS in that order. Lane c (c < 12) then adds
+// term c of each, in order, and lane 12 counts those inside h, so every sum is the round's, term
+// by term. For the targets whose spheres reach far beyond their leaf, and for the last few
+// targets: with LOOP, the wave repeats its target's steps until the target is done.
+template <bool LOOP>
+__global__ __launch_bounds__(64) void k_density_far(DensArgs a) {
     __shared__ double s_v[kTerms][65]; // rows padded so the 12 adding lanes hit distinct banks
     __shared__ int s_in[64];
     const int lane = threadIdx.x;
-    const int nl = *a.n_long;
-    for (int e = blockIdx.x; e < nl; e += gridDim.x) {
-        const int k = a.long_list[e];
-        const uint32_t t = a.act[k];
-        const uint32_t p = a.targets[t];
-        const double p0 = a.x[p], p1 = a.y[p], p2 = a.z[p];
+    const int e = blockIdx.x;
+    const int dim = a.n_dims;
+    const uint32_t t = a.act[e];
+    const uint32_t p = a.targets[t];
+    const double qx = a.x[p], qy = a.y[p], qz = a.z[p];
+    const int r_first = (a.nclusters > 1) ? 0 : a.root;
+    const int r_count = (a.nclusters > 1) ? a.nclusters : 1;
+    for (;;) {
         const double h = a.h[t];
         const int it = a.iter[t];
-        const size_t b0 = a.start[k], b1 = a.start[k + 1];
+        const double r2 = h * h; // the search's radius_sq (hydro.cc:50)
+        // the node test of the forward search (hydro.cc:55-72), as the rounds evaluate it
+        auto admits = [&](const SRec& nr) -> bool {
+            double dist = h + (double)nr.half + (double)nr.xpad;
+            if (a.tau > 0) dist += (double)nr.vmax * a.tau;
+            const double dx = (double)nr.cx - qx, dy = (double)nr.cy - qy,
+                         dz = (double)nr.cz - qz;
+            if ((dx > dist || -dx > dist) || (dy > dist || -dy > dist) ||
+                (dz > dist || -dz > dist))
+                return false;
+            const double ds = dist + 0.7320508075688772 * (double)nr.half;
+            return !(dx * dx + dy * dy + dz * dz > ds * ds);
+        };
         double acc = 0.0; // lane c < 12: term c's sum
         int n_in = 0;     // lane 12: the count inside h
-        for (size_t c0 = b0; c0 < b1; c0 += 64) {
-            const size_t q = c0 + lane;
-            double v[kTerms];
-            bool in = false;
-            int flag = 0; // bit 0: adds its terms; bit 1: inside h
-            if (q < b1 && density_terms(a, a.found[q], p0, p1, p2, h, a.n_dims, v, &in)) {
-                flag = 1 | (in ? 2 : 0);
-#pragma unroll
-                for (int c = 0; c < kTerms; ++c) s_v[c][lane] = v[c];
-            }
-            s_in[lane] = flag;
+        int nbuf = 0;     // neighbours waiting in s_v (the same in every lane)
+        auto flush = [&]() {
             __syncthreads();
-            const int cnt = (int)min((size_t)64, b1 - c0);
             if (lane < kTerms) {
-                for (int i = 0; i < cnt; ++i)
-                    if (s_in[i] & 1) acc += s_v[lane][i];
+                for (int i = 0; i < nbuf; ++i) acc += s_v[lane][i];
             } else if (lane == kTerms) {
-                for (int i = 0; i < cnt; ++i)
-                    if (s_in[i] & 2) ++n_in;
+                for (int i = 0; i < nbuf; ++i) n_in += s_in[i];
             }
             __syncthreads();
+            nbuf = 0;
+        };
+        // one particle per lane (slot s, or none for s < 0): a neighbour when within the radius
+        // (the tree's copy of the position, as the search tests it), its terms packed into s_v
+        // in lane order
+        auto take = [&](int s) {
+            double v[kTerms];
+            bool inside = false, ok = false;
+            if (s >= 0) {
+                const double4 q = a.src[s];
+                const double ex = q.x - qx, ey = q.y - qy, ez = q.z - qz;
+                if (ex * ex + ey * ey + ez * ez < r2)
+                    ok = density_terms(a, a.orderbuf[s], qx, qy, qz, h, dim, v, &inside);
+            }
+            const uint64_t om = __ballot(ok);
+            if (om == 0) return;
+            const int n_ok = __builtin_popcountll(om);
+            if (nbuf + n_ok > 64) flush();
+            if (ok) {
+                const int pos = nbuf + __builtin_popcountll(om & ((1ull << lane) - 1));
+#pragma unroll
+                for (int c = 0; c < kTerms; ++c) s_v[c][pos] = v[c];
+                s_in[pos] = inside ? 1 : 0;
+            }
+            nbuf += n_ok;
+        };
+        // the admitted leaves run (bit j: child cf + j), their particles in slot order
+        auto leaves = [&](int cf, uint64_t run) {
+            int plo = 0, cnt = 0, off = 0, total = 0;
+            if ((run >> lane) & 1ull) {
+                const SRec c = a.srec[cf + lane];
+                plo = ~c.link;
+                cnt = c.span - plo;
+            }
+            for (uint64_t r = run; r != 0; r &= r - 1) {
+                const int j = (int)__builtin_ctzll(r);
+                if (lane == j) off = total;
+                total += __builtin_amdgcn_readlane(cnt, j);
+            }
+            for (int g0 = 0; g0 < total; g0 += 64) {
+                const int g = g0 + lane;
+                int s = -1;
+                for (uint64_t r = run; r != 0; r &= r - 1) {
+                    const int j = (int)__builtin_ctzll(r);
+                    const int o = __builtin_amdgcn_readlane(off, j);
+                    if (g >= o && g < o + __builtin_amdgcn_readlane(cnt, j))
+                        s = __builtin_amdgcn_readlane(plo, j) + (g - o);
+                }
+                take(s);
+            }
+        };
+        // A node whose box (its centre and half side as the node test reads them, so the cells
+        // below it too) lies inside the sphere, by a margin above the float rounding of node
+        // centres: every node below it is admitted, so the walk would test each of its particles
+        // in slot order. They are a contiguous slot range, taken 64 at a time instead.
+        auto inside_sphere = [&](const SRec& nr) -> bool {
+            const double hf = (double)nr.half;
+            const double ax = fabs((double)nr.cx - qx) + hf, ay = fabs((double)nr.cy - qy) + hf,
+                         az = fabs((double)nr.cz - qz) + hf;
+            const double m = 1e-6 * (fabs((double)nr.cx) + fabs((double)nr.cy) +
+                                     fabs((double)nr.cz));
+            const double hm = h * (1.0 - 1e-12) - m;
+            return hm > 0 && ax * ax + ay * ay + az * az < hm * hm;
+        };
+        for (int r0 = 0; r0 < r_count; r0 += 64) {
+            // the children of the node being walked (first index; lane masks of those admitted
+            // and not yet visited, and of those that are leaves) and the stack of the same above
+            // it, entry d in lane d
+            int cf = r_first + r0;
+            uint64_t cm, lm;
+            {
+                bool hit = false, leaf = false;
+                if (lane < min(64, r_count - r0)) {
+                    const SRec c = a.srec[cf + lane];
+                    hit = admits(c);
+                    leaf = c.link < 0;
+                }
+                cm = __ballot(hit);
+                lm = __ballot(hit && leaf);
+            }
+            int sp = 0, stf = 0, sml = 0, smh = 0, sll = 0, slh = 0;
+            for (;;) {
+                if (cm == 0) {
+                    if (sp == 0) break;
+                    --sp;
+                    cf = __builtin_amdgcn_readlane(stf, sp);
+                    cm = ((uint64_t)(uint32_t)__builtin_amdgcn_readlane(smh, sp) << 32) |
+                         (uint64_t)(uint32_t)__builtin_amdgcn_readlane(sml, sp);
+                    lm = ((uint64_t)(uint32_t)__builtin_amdgcn_readlane(slh, sp) << 32) |
+                         (uint64_t)(uint32_t)__builtin_amdgcn_readlane(sll, sp);
+                    continue;
+                }
+                const uint64_t inner = cm & ~lm;
+                if ((cm & (0 - cm) & lm) != 0) {
+                    // the next child is a leaf: it and the leaves after it up to the next
+                    // interior child
+                    const uint64_t run = inner ? (cm & ((inner & (0 - inner)) - 1)) : cm;
+                    cm &= ~run;
+                    leaves(cf, run);
+                    continue;
+                }
+                // an interior child: its admitted children are walked before its remaining
+                // siblings, which wait on the stack
+                const int node = cf + (int)__builtin_ctzll(cm);
+                cm &= cm - 1;
+                const SRec nr = ld_const(a.srec + node);
+                if (inside_sphere(nr)) {
+                    const WNode w = ld_const(a.wn + node);
+                    for (int s0 = w.plo; s0 < w.phi; s0 += 64)
+                        take(s0 + lane < w.phi ? s0 + lane : -1);
+                    continue;
+                }
+                bool hit = false, leaf = false;
+                if (lane < nr.span) {
+                    const SRec c = a.srec[nr.link + lane];
+                    hit = admits(c);
+                    leaf = c.link < 0;
+                }
+                const uint64_t om = __ballot(hit);
+                if (om == 0) continue;
+                if (cm != 0) {
+                    const bool top = lane == sp;
+                    stf = top ? cf : stf;
+                    sml = top ? (int)(uint32_t)cm : sml;
+                    smh = top ? (int)(uint32_t)(cm >> 32) : smh;
+                    sll = top ? (int)(uint32_t)lm : sll;
+                    slh = top ? (int)(uint32_t)(lm >> 32) : slh;
+                    ++sp;
+                }
+                cf = nr.link;
+                cm = om;
+                lm = __ballot(hit && leaf);
+            }
         }
+        flush();
         DensSums S;
 #pragma unroll
         for (int c = 0; c < kTerms; ++c) S.s[c] = __shfl(acc, c);
         S.n_inside = __shfl(n_in, kTerms);
-        if (lane == 0) density_update(a, k, t, p, h, it, S);
+        if (lane == 0) density_update(a, e, t, p, h, it, S);
+        if (!LOOP) break;
+        __syncthreads(); // the update's state, written by lane 0, for every lane
+        if (a.keep[e] == 0) break;
     }
 }
 
 // One step for every active target (see the top of the file): tpw targets per wavefront, lane l
 // holding active entry wave * tpw + l. A target whose radius reaches far beyond its own leaf would
-// make its wave walk a large part of the tree one node at a time; it is queued for the list path
-// (whose search tests 64 nodes at once) and takes no part here. With LOOP, each lane repeats its
-// steps until its target is done.
-template <bool LOOP>
+// make its wave walk a large part of the tree one node at a time; it is queued for a wave of its
+// own (k_density_far) and takes no part here.
 __global__ __launch_bounds__(64) void k_density_round(DensArgs a) {
     const int lane = threadIdx.x;
     // Block b runs on XCD b % 8: each XCD gets one contiguous share of the Morton-ordered waves.
@@ -371,6 +492,7 @@ __global__ __launch_bounds__(64) void k_density_round(DensArgs a) {
     double qx = 0, qy = 0, qz = 0, h = 0;
     int it = 0;
     bool mine = false; // this lane steps its target here
+    bool far = false;  // this lane's target is left to a wave of its own
     if (live) {
         t = a.act[k];
         p = a.targets[t];
@@ -381,8 +503,8 @@ __global__ __launch_bounds__(64) void k_density_round(DensArgs a) {
         it = a.iter[t];
         mine = true;
         const int ts = a.tslot[t];
-        if (!LOOP && ts >= 0 && h > kFarLeaf * a.size[a.slot_leaf[ts]]) {
-            a.far_list[atomicAdd(a.n_far, 1u)] = (uint32_t)k;
+        if (ts >= 0 && h > kFarLeaf * a.size[a.slot_leaf[ts]]) {
+            far = true;
             mine = false;
         }
     }
@@ -407,83 +529,79 @@ __global__ __launch_bounds__(64) void k_density_round(DensArgs a) {
         nb_n = 0;
     };
 
-    for (;;) {
-        const uint64_t lanes = __ballot(mine);
-        if (lanes == 0) break;
-        const double r2 = h * h; // the search's radius_sq (hydro.cc:50)
-#pragma unroll
-        for (int c = 0; c < kTerms; ++c) S.s[c] = 0.0;
-        S.n_inside = 0;
-        // The current sibling range (next node, nodes left, lanes waiting on it) and the
-        // ranges above it, entry e in lane e of stf/stc/stl/sth: one per tree level, below the
-        // roots' range (a forest's roots are nodes 0..nc-1, searched in that order).
-        int cf = (a.nclusters > 1) ? 0 : a.root;
-        int cc = (a.nclusters > 1) ? a.nclusters : 1;
-        uint64_t cm = lanes;
-        int sp = 0;
-        int stf = 0, stc = 0, stl = 0, sth = 0;
-        for (;;) {
-            if (cc == 0) {
-                if (sp == 0) break;
-                --sp;
-                cf = __builtin_amdgcn_readlane(stf, sp);
-                cc = __builtin_amdgcn_readlane(stc, sp);
-                cm = ((uint64_t)(uint32_t)__builtin_amdgcn_readlane(sth, sp) << 32) |
-                     (uint64_t)(uint32_t)__builtin_amdgcn_readlane(stl, sp);
-            }
-            const int node = cf;
-            ++cf;
-            --cc;
-            const bool wait = (cm >> lane) & 1ull;
-            // the node test of the forward search, per lane (hydro.cc:55-72), exactly as the
-            // device search evaluates it
-            const SRec nr = a.srec[node];
-            double dist = h + (double)nr.half + (double)nr.xpad;
-            if (a.tau > 0) dist += (double)nr.vmax * a.tau;
-            const double dx = (double)nr.cx - qx, dy = (double)nr.cy - qy, dz = (double)nr.cz - qz;
-            bool hit = false;
-            if (wait && !(dx > dist || -dx > dist) && !(dy > dist || -dy > dist) &&
-                !(dz > dist || -dz > dist)) {
-                const double ds = dist + 0.7320508075688772 * (double)nr.half;
-                hit = !(dx * dx + dy * dy + dz * dz > ds * ds);
-            }
-            const uint64_t om = __ballot(hit);
-            if (om == 0) continue;
-            if (nr.link < 0) {
-                // the leaf's particles in slot order: a neighbour when within the radius (the
-                // tree's copy of the position, as the search tests it)
-                for (int s = ~nr.link; s < nr.span; ++s) {
-                    const double4 q = a.src[s];
-                    const double ex = q.x - qx, ey = q.y - qy, ez = q.z - qz;
-                    const bool in = hit && ex * ex + ey * ey + ez * ez < r2;
-                    if (__ballot(in) == 0) continue;
-                    if (__ballot(nb_n == kNbBuf) != 0) flush();
-                    if (in) s_nb[nb_n++][lane] = a.orderbuf[s];
-                }
-                continue;
-            }
-            // descend: the node's remaining siblings (if any) wait on the stack below its
-            // children
-            const bool top = lane == sp;
-            stf = top ? cf : stf;
-            stc = top ? cc : stc;
-            stl = top ? (int)(uint32_t)cm : stl;
-            sth = top ? (int)(uint32_t)(cm >> 32) : sth;
-            sp += (cc > 0) ? 1 : 0;
-            cf = nr.link;
-            cc = nr.span;
-            cm = om;
-        }
-        flush();
-        if (mine) density_update(a, k, t, p, h, it, S);
-        if (!LOOP) break;
-        if (mine && a.keep[k]) { // the next step, from the state the update left
-            h = a.h[t];
-            it = a.iter[t];
-        } else {
-            mine = false;
-        }
+    // (the far targets are listed at the end: no store precedes the walk's reads)
+    const uint64_t lanes = __ballot(mine);
+    if (lanes == 0) {
+        if (far) a.far_list[atomicAdd(a.n_far, 1u)] = (uint32_t)k;
+        return;
     }
+    const double r2 = h * h; // the search's radius_sq (hydro.cc:50)
+#pragma unroll
+    for (int c = 0; c < kTerms; ++c) S.s[c] = 0.0;
+    S.n_inside = 0;
+    // The current sibling range (next node, nodes left, lanes waiting on it) and the
+    // ranges above it, entry e in lane e of stf/stc/stl/sth: one per tree level, below the
+    // roots' range (a forest's roots are nodes 0..nc-1, searched in that order).
+    int cf = (a.nclusters > 1) ? 0 : a.root;
+    int cc = (a.nclusters > 1) ? a.nclusters : 1;
+    uint64_t cm = lanes;
+    int sp = 0;
+    int stf = 0, stc = 0, stl = 0, sth = 0;
+    for (;;) {
+        if (cc == 0) {
+            if (sp == 0) break;
+            --sp;
+            cf = __builtin_amdgcn_readlane(stf, sp);
+            cc = __builtin_amdgcn_readlane(stc, sp);
+            cm = ((uint64_t)(uint32_t)__builtin_amdgcn_readlane(sth, sp) << 32) |
+                 (uint64_t)(uint32_t)__builtin_amdgcn_readlane(stl, sp);
+        }
+        const int node = cf;
+        ++cf;
+        --cc;
+        const bool wait = (cm >> lane) & 1ull;
+        // the node test of the forward search, per lane (hydro.cc:55-72), exactly as the
+        // device search evaluates it
+        const SRec nr = ld_const(a.srec + node);
+        double dist = h + (double)nr.half + (double)nr.xpad;
+        if (a.tau > 0) dist += (double)nr.vmax * a.tau;
+        const double dx = (double)nr.cx - qx, dy = (double)nr.cy - qy, dz = (double)nr.cz - qz;
+        bool hit = false;
+        if (wait && !(dx > dist || -dx > dist) && !(dy > dist || -dy > dist) &&
+            !(dz > dist || -dz > dist)) {
+            const double ds = dist + 0.7320508075688772 * (double)nr.half;
+            hit = !(dx * dx + dy * dy + dz * dz > ds * ds);
+        }
+        const uint64_t om = __ballot(hit);
+        if (om == 0) continue;
+        if (nr.link < 0) {
+            // the leaf's particles in slot order: a neighbour when within the radius (the
+            // tree's copy of the position, as the search tests it)
+            for (int s = ~nr.link; s < nr.span; ++s) {
+                const double4 q = ld_const(a.src + s);
+                const double ex = q.x - qx, ey = q.y - qy, ez = q.z - qz;
+                const bool in = hit && ex * ex + ey * ey + ez * ez < r2;
+                if (__ballot(in) == 0) continue;
+                if (__ballot(nb_n == kNbBuf) != 0) flush();
+                if (in) s_nb[nb_n++][lane] = ld_const(a.orderbuf + s);
+            }
+            continue;
+        }
+        // descend: the node's remaining siblings (if any) wait on the stack below its
+        // children
+        const bool top = lane == sp;
+        stf = top ? cf : stf;
+        stc = top ? cc : stc;
+        stl = top ? (int)(uint32_t)cm : stl;
+        sth = top ? (int)(uint32_t)(cm >> 32) : sth;
+        sp += (cc > 0) ? 1 : 0;
+        cf = nr.link;
+        cc = nr.span;
+        cm = om;
+    }
+    flush();
+    if (mine) density_update(a, k, t, p, h, it, S);
+    if (far) a.far_list[atomicAdd(a.n_far, 1u)] = (uint32_t)k;
 }
 
 // The search nodes of the fused rounds, one per tree node.
@@ -528,27 +646,6 @@ __global__ __launch_bounds__(256) void k_density_init(int nt, const uint32_t* __
     h_hi[t] = 0.0;
     iter[t] = 0;
     act[t] = (uint32_t)t;
-}
-
-// The active targets' centres and search radii.
-__global__ __launch_bounds__(256) void k_density_centres(int na, const uint32_t* __restrict__ act,
-                                                         const uint32_t* __restrict__ targets,
-                                                         const double* __restrict__ x,
-                                                         const double* __restrict__ y,
-                                                         const double* __restrict__ z,
-                                                         const double* __restrict__ h,
-                                                         double* __restrict__ cx,
-                                                         double* __restrict__ cy,
-                                                         double* __restrict__ cz,
-                                                         double* __restrict__ r) {
-    const int k = blockIdx.x * 256 + threadIdx.x;
-    if (k >= na) return;
-    const uint32_t t = act[k];
-    const uint32_t p = targets[t];
-    cx[k] = x[p];
-    cy[k] = y[p];
-    cz[k] = z[p];
-    r[k] = h[t];
 }
 
 // The far targets' entries in the active list: their target positions, for the list path.
@@ -661,9 +758,7 @@ extern "C" gz_status gz_tree_density(gz_ctx* ctx, const gz_tree* tree, const gz_
                  o_scan = take((n_t + 1) * 4), o_kept = take((n_t + 1) * 4),
                  o_slot = take(n_t * 4), o_k0 = take(n_t * 8), o_k1 = take(n_t * 8),
                  o_v0 = take(n_t * 4), o_v1 = take(n_t * 4), o_far = take(n_t * 4),
-                 o_fact = take(n_t * 4), o_fkeep = take(n_t * 4), o_long = take(n_t * 4),
-                 o_cx = take(n_t * 8), o_cy = take(n_t * 8), o_cz = take(n_t * 8),
-                 o_r = take(n_t * 8), o_start = take((n_t + 1) * sizeof(size_t)),
+                 o_fact = take(n_t * 4), o_fkeep = take(n_t * 4),
                  o_srec = take((size_t)std::max(tree->nnodes, 1) * sizeof(SRec)),
                  o_misc = take(256), o_tmp = take(scan_tmp_bytes(n_t + 1));
     DevMem& mem = ctx->scratch[13];
@@ -682,15 +777,9 @@ extern "C" gz_status gz_tree_density(gz_ctx* ctx, const gz_tree* tree, const gz_
     int* d_status = reinterpret_cast<int*>(base + o_misc + 128);
     int* d_sort = reinterpret_cast<int*>(base + o_misc + 192); // bit 1: bad index; [1]: unsorted
     uint32_t* d_nfar = reinterpret_cast<uint32_t*>(base + o_misc + 224);
-    int* d_nlong = reinterpret_cast<int*>(base + o_misc + 240);
     uint32_t* d_far = reinterpret_cast<uint32_t*>(base + o_far);
     uint32_t* d_fact = reinterpret_cast<uint32_t*>(base + o_fact);
     uint32_t* d_fkeep = reinterpret_cast<uint32_t*>(base + o_fkeep);
-    double* d_cx = reinterpret_cast<double*>(base + o_cx);
-    double* d_cy = reinterpret_cast<double*>(base + o_cy);
-    double* d_cz = reinterpret_cast<double*>(base + o_cz);
-    double* d_r = reinterpret_cast<double*>(base + o_r);
-    size_t* d_start = reinterpret_cast<size_t*>(base + o_start);
     GZ_TRY(fill_zero_u32(reinterpret_cast<uint32_t*>(base + o_misc), 64, st));
 
     // the default first guess: des_ngb particles at the mean density of the bounding box of all
@@ -768,8 +857,6 @@ extern "C" gz_status gz_tree_density(gz_ctx* ctx, const gz_tree* tree, const gz_
     a.slot_leaf = tree->slot_leaf;
     a.far_list = d_far;
     a.n_far = d_nfar;
-    a.long_list = reinterpret_cast<int*>(base + o_long);
-    a.n_long = d_nlong;
     a.srec = reinterpret_cast<const SRec*>(base + o_srec);
     if (tree->nnodes > 0) {
         k_density_srec<<<(unsigned)((tree->nnodes + 255) / 256), 256, 0, st>>>(
@@ -777,6 +864,7 @@ extern "C" gz_status gz_tree_density(gz_ctx* ctx, const gz_tree* tree, const gz_
             tree->vmax, 1e-6 * tree->root_box[3], reinterpret_cast<SRec*>(base + o_srec));
         GZ_HIP(hipGetLastError());
     }
+    a.wn = tree->wn;
     a.size = tree->size;
     a.src = tree->src;
     a.orderbuf = tree->orderbuf;
@@ -799,8 +887,8 @@ extern "C" gz_status gz_tree_density(gz_ctx* ctx, const gz_tree* tree, const gz_
         a.tpw = tpw;
         const bool loop = n_act <= kLoopTargets;
         GZ_TRY(fill_zero_u32(d_nfar, 1, st));
-        if (loop) k_density_round<true><<<(n_act + tpw - 1) / tpw, 64, 0, st>>>(a);
-        else k_density_round<false><<<(n_act + tpw - 1) / tpw, 64, 0, st>>>(a);
+        if (loop) k_density_far<true><<<n_act, 64, 0, st>>>(a);
+        else k_density_round<<<(n_act + tpw - 1) / tpw, 64, 0, st>>>(a);
         GZ_HIP(hipGetLastError());
         uint32_t n_far = 0;
         if (!loop) {
@@ -808,41 +896,17 @@ extern "C" gz_status gz_tree_density(gz_ctx* ctx, const gz_tree* tree, const gz_
             GZ_HIP(hipStreamSynchronize(st));
         }
         if (n_far > 0) {
-            // the far targets: one search for all of them (lists in the reference's order),
-            // then one thread (or, for a long list, one wave) per target sums its list
+            // the far targets: a wave each, in the round's order (their keep flags are then
+            // moved to the round's positions)
             const unsigned gf = (n_far + 255) / 256;
             k_density_far_act<<<gf, 256, 0, st>>>((int)n_far, d_far, d_act, d_fact);
-            k_density_centres<<<gf, 256, 0, st>>>((int)n_far, d_fact, targets, p->x, p->y, p->z,
-                                                  h, d_cx, d_cy, d_cz, d_r);
             GZ_HIP(hipGetLastError());
-            // in chunks whose lists hold at most kMaxPairs pairs (a GiB of indices): all at once
-            // unless a chunk's lists would hold more, then halves until they do not
-            uint32_t chunk = n_far;
-            for (uint32_t c0 = 0; c0 < n_far;) {
-                const uint32_t c1 = std::min<uint32_t>(n_far, c0 + chunk);
-                uint32_t* found = nullptr;
-                size_t total = 0;
-                const gz_status sr = ngb_search_grow(ctx, tree, d_cx + c0, d_cy + c0, d_cz + c0,
-                                                     d_r + c0, c1 - c0, tau, d_start,
-                                                     ctx->scratch[14], &found, &total, kMaxPairs);
-                if (sr == GZ_ERR_UNSUPPORTED && c1 - c0 > 1) {
-                    chunk = std::max<uint32_t>(1, (c1 - c0) / 2);
-                    continue;
-                }
-                GZ_TRY(sr);
-                DensArgs f = a;
-                f.act = d_fact + c0;
-                f.n_act = (int)(c1 - c0);
-                f.start = d_start;
-                f.found = found;
-                f.keep = d_fkeep + c0;
-                GZ_TRY(fill_zero_u32(reinterpret_cast<uint32_t*>(d_nlong), 1, st));
-                k_density_step<<<(c1 - c0 + 255) / 256, 256, 0, st>>>(f);
-                GZ_HIP(hipGetLastError());
-                k_density_step_long<<<(unsigned)std::min<uint32_t>(c1 - c0, 4096), 64, 0, st>>>(f);
-                GZ_HIP(hipGetLastError());
-                c0 = c1;
-            }
+            DensArgs f = a;
+            f.act = d_fact;
+            f.n_act = (int)n_far;
+            f.keep = d_fkeep;
+            k_density_far<false><<<n_far, 64, 0, st>>>(f);
+            GZ_HIP(hipGetLastError());
             k_density_far_keep<<<gf, 256, 0, st>>>((int)n_far, d_far, d_fkeep, d_keep);
             GZ_HIP(hipGetLastError());
         }

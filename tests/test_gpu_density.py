@@ -103,9 +103,10 @@ def test_density_hernquist_1m_sampled(gz, oracle):
     _compare(tuple(t[torch.as_tensor(pick, device=t.device)] for t in (h, rho, nngb, iters)), ora)
 
 
-def test_density_splits_searches_that_exceed_one_call(gz, oracle):
-    # the reference's default guess on a sphere truncated at 50 scale radii: the first rounds'
-    # lists hold more pairs than the solve lists at once (2^28), so it splits its active set
+def test_density_default_guess_on_a_truncated_sphere(gz, oracle):
+    # the reference's default guess (the mean density of the bounding box) on a sphere truncated
+    # at 50 scale radii: the first steps' spheres reach far beyond the targets' leaves, most of
+    # the sphere for the central targets, so whole subtrees lie inside them
     from gizmo_shared_memory_amd import ics
     from oracle.bindings import HostParticles
     x, y, z, m, soft = ics.hernquist(n=1 << 16, eps=0.005, seed=13)
@@ -122,6 +123,51 @@ def test_density_splits_searches_that_exceed_one_call(gz, oracle):
     finally:
         oracle.free(to)
     _compare(tuple(t[torch.as_tensor(pick, device=t.device)] for t in dev), ora)
+
+
+def test_density_on_an_aged_tree(gz, oracle):
+    """A tree aged by tau (built at the start positions with velocities, its sources moved on):
+    the node tests widen by vmax * tau, as the reference's do."""
+    from oracle.bindings import HostParticles
+    from tests.test_oracle_reuse import _scenario
+    pos, v0, _, _ = _scenario(n=5000, seed=31)
+    tau = 0.05
+    xt = [pos[d] + v0[d] * tau for d in range(3)]
+    print("[density] aged tree, n=5000", flush=True)
+    T = gz.build(gz.Particles(*pos), vel=v0)
+    Pt = gz.Particles(xt[0], xt[1], xt[2], pos[3], pos[4])
+    T.update_sources(Pt)
+    tg = np.arange(0, 5000, 2, dtype=np.uint32)
+    dev = gz.density(T, Pt, tg, des_ngb=32.0, ngb_tol=0.5, tau=tau)
+    to = oracle.build_vel(HostParticles(*pos), v0)
+    try:
+        ora = oracle.density(to, HostParticles(xt[0], xt[1], xt[2], pos[3], pos[4]), tg,
+                             des_ngb=32.0, ngb_tol=0.5, tau=tau)
+    finally:
+        oracle.free(to)
+    _compare(dev, ora)
+
+
+def test_density_from_spheres_holding_the_whole_tree(gz, oracle):
+    """First guesses far above the particles' extent: the first step's every search takes the
+    whole tree, node by node at first and then as whole subtrees inside the sphere."""
+    from oracle.bindings import HostParticles
+    x, y, z, m, soft = _gas(3000, 3, seed=12)
+    ext = max(np.ptp(x), np.ptp(y), np.ptp(z))
+    tg = np.arange(0, 3000, 7, dtype=np.uint32)
+    h0 = np.random.default_rng(5).uniform(2.0, 4.0, tg.shape[0]) * ext
+    print("[density] whole-tree spheres, n=3000", flush=True)
+    P = gz.Particles(x, y, z, m, soft)
+    T = gz.build(P)
+    dev = gz.density(T, P, tg, des_ngb=32.0, ngb_tol=0.5, h_start=h0)
+    hp = HostParticles(x, y, z, m, soft)
+    to = oracle.build(hp)
+    try:
+        ora = oracle.density(to, hp, tg, des_ngb=32.0, ngb_tol=0.5, h_start=h0)
+    finally:
+        oracle.free(to)
+    _compare(dev, ora)
+    print(f"[density] iterations {ora[3].min()}..{ora[3].max()}", flush=True)
 
 
 def test_density_errors(gz):
