@@ -577,13 +577,21 @@ __global__ __launch_bounds__(64) void k_density_round(DensArgs a) {
         if (nr.link < 0) {
             // the leaf's particles in slot order: a neighbour when within the radius (the
             // tree's copy of the position, as the search tests it)
-            for (int s = ~nr.link; s < nr.span; ++s) {
-                const double4 q = ld_const(a.src + s);
-                const double ex = q.x - qx, ey = q.y - qy, ez = q.z - qz;
-                const bool in = hit && ex * ex + ey * ey + ez * ez < r2;
-                if (__ballot(in) == 0) continue;
-                if (__ballot(nb_n == kNbBuf) != 0) flush();
-                if (in) s_nb[nb_n++][lane] = ld_const(a.orderbuf + s);
+            // (four records loaded together, then noted one by one in slot order)
+            for (int s0 = ~nr.link; s0 < nr.span; s0 += 4) {
+                double4 q[4];
+#pragma unroll
+                for (int j = 0; j < 4; ++j)
+                    if (s0 + j < nr.span) q[j] = ld_const(a.src + s0 + j);
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    if (s0 + j >= nr.span) break;
+                    const double ex = q[j].x - qx, ey = q[j].y - qy, ez = q[j].z - qz;
+                    const bool in = hit && ex * ex + ey * ey + ez * ez < r2;
+                    if (__ballot(in) == 0) continue;
+                    if (__ballot(nb_n == kNbBuf) != 0) flush();
+                    if (in) s_nb[nb_n++][lane] = ld_const(a.orderbuf + s0 + j);
+                }
             }
             continue;
         }

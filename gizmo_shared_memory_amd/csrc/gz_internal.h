@@ -269,14 +269,6 @@ gz_status launch_node_events(gz_tree* t, int kind, const uint32_t* idx, size_t n
                              const double* vx, const double* vy, const double* vz, double tau,
                              double box, int* status, hipStream_t st);
 
-// The forward neighbour search (gz_ngb.hip) with its lists in `found`, reserved to their total
-// (the same CSR as gz_tree_ngb_search); *found_ptr is null when there are none. A total above
-// max_total returns GZ_ERR_UNSUPPORTED, with no message and nothing filled.
-gz_status ngb_search_grow(gz_ctx* ctx, const gz_tree* tree, const double* cx, const double* cy,
-                          const double* cz, const double* radius, size_t n_q, double tau,
-                          size_t* start, DevMem& found, uint32_t** found_ptr, size_t* total,
-                          size_t max_total);
-
 // The spill slices of ctx (allocating them on first use) and their size per wave.
 gz_status ctx_spill(gz_ctx* ctx, int** spill, int* cap);
 } // namespace gz

@@ -457,14 +457,11 @@ gz_status ngb_pass(NgbArgs& a, bool fill, bool rev, int waves, hipStream_t st) {
 }
 } // namespace
 
-// Both searches; with `grow`, the lists go to that buffer (reserved to the total) and *found_out
-// points at them, unless the total exceeds grow_max: then GZ_ERR_UNSUPPORTED with no message
-// and nothing filled.
+// Both searches.
 static gz_status ngb_run(gz_ctx* ctx, const gz_tree* tree, const double* cx, const double* cy,
                          const double* cz, const double* radius, size_t n_q, double tau,
                          size_t* start, uint32_t* found, size_t cap, size_t* total,
-                         const double* h, size_t n_gas, bool rev, DevMem* grow = nullptr,
-                         uint32_t** found_out = nullptr, size_t grow_max = 0) {
+                         const double* h, size_t n_gas, bool rev) {
     const char* who = rev ? "gz_tree_ngb_search_reverse" : "gz_tree_ngb_search";
     clear_error();
     if (!ctx || !tree || !total) return fail(GZ_ERR_INVALID, "%s: null argument", who);
@@ -513,8 +510,8 @@ static gz_status ngb_run(gz_ctx* ctx, const gz_tree* tree, const double* cx, con
     a.cx = cx; a.cy = cy; a.cz = cz; a.radius = radius;
     a.nq = (int)n_q;
     // 64 consecutive queries per wave share its traversal; with too few queries to occupy the
-    // GPU that way (and often scattered, as the last unconverged targets of a density solve),
-    // fewer per wave, down to one, so that more waves walk at once
+    // GPU that way (a small set is often scattered too), fewer per wave, down to one, so that
+    // more waves walk at once
     a.qpw = 64;
     while (a.qpw > 1 && (n_q + a.qpw - 1) / a.qpw < (size_t)waves) a.qpw >>= 1;
     a.ngroups = (int)((n_q + a.qpw - 1) / a.qpw);
@@ -561,18 +558,11 @@ static gz_status ngb_run(gz_ctx* ctx, const gz_tree* tree, const double* cx, con
     if (h_status & 1)
         return fail(GZ_ERR_INTERNAL, "%s: a search stack outgrew %d entries", who,
                     stack_cap);
-    if (grow && h_total > grow_max) return GZ_ERR_UNSUPPORTED;
     if (h_total > 0xFFFFFFFFull)
         return fail(GZ_ERR_UNSUPPORTED,
                     "%s: %llu neighbour pairs exceed the 2^32 - 1 one call can return; split the "
                     "queries", who, h_total);
     *total = tot32;
-    if (grow && tot32 > 0) {
-        GZ_TRY(grow->reserve((size_t)tot32 * sizeof(uint32_t)));
-        found = grow->as<uint32_t>();
-        cap = tot32;
-        *found_out = found;
-    }
     if (!found || cap < (size_t)tot32 || tot32 == 0) return GZ_OK;
     // pass 2: the neighbours' slots, then sorted and mapped to particle indices
     DevMem& sl = ctx->scratch[11];
@@ -610,17 +600,6 @@ static gz_status ngb_run(gz_ctx* ctx, const gz_tree* tree, const double* cx, con
                     stack_cap);
     return GZ_OK;
 }
-
-namespace gz {
-gz_status ngb_search_grow(gz_ctx* ctx, const gz_tree* tree, const double* cx, const double* cy,
-                          const double* cz, const double* radius, size_t n_q, double tau,
-                          size_t* start, DevMem& found, uint32_t** found_ptr, size_t* total,
-                          size_t max_total) {
-    *found_ptr = nullptr;
-    return ngb_run(ctx, tree, cx, cy, cz, radius, n_q, tau, start, nullptr, 0, total, nullptr, 0,
-                   false, &found, found_ptr, std::min<size_t>(max_total, 0xFFFFFFFFull));
-}
-} // namespace gz
 
 extern "C" gz_status gz_tree_ngb_search(gz_ctx* ctx, const gz_tree* tree, const double* cx,
                                         const double* cy, const double* cz, const double* radius,
