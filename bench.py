@@ -232,12 +232,16 @@ def widened_rows(gz, ics, args):
     q = order[::4]
     c = np.stack([x[q], y[q], z[q]], 1)
     r = side[q]
-    (s, _), dt = timed(lambda: gz.ngb_search(T, c, r))
+    # the searches' inputs resident in HBM, as the walks' are
+    c_d = torch.as_tensor(c, device=P.device)
+    r_d = torch.as_tensor(r, device=P.device)
+    side_d = torch.as_tensor(side, device=P.device)
+    (s, _), dt = timed(lambda: gz.ngb_search(T, c_d, r_d))
     nq = q.shape[0]
     out["ngb_search"] = {"queries": nq, "per_query": int(s[-1]) / nq, "ms_per_call": dt * 1e3,
                          "queries_per_s": nq / dt}
-    T.init_hmax(side, n)
-    (s, _), dt = timed(lambda: gz.ngb_search_reverse(T, side, n, c, r))
+    T.init_hmax(side_d, n)
+    (s, _), dt = timed(lambda: gz.ngb_search_reverse(T, side_d, n, c_d, r_d))
     out["ngb_search_reverse"] = {"queries": nq, "per_query": int(s[-1]) / nq,
                                  "ms_per_call": dt * 1e3, "queries_per_s": nq / dt}
     # first guesses: DesNumNgb particles at the mean density of the particle's leaf

@@ -656,6 +656,24 @@ def density(T: Tree, P: Particles, targets, des_ngb: float = 32.0, ngb_tol: floa
     return h[:nt], rho[:nt], nngb[:nt], iters[:nt]
 
 
+def _ngb_lists(T: Tree, call, who: str, nq: int):
+    """The lists of a search whose C call ends in (found, cap, &total): first into a buffer sized
+    from the tree's last search (at least 32 entries per query), so that one pass of counting
+    usually serves; a second call only when the lists need more."""
+    dev = T.P.device
+    total = C.c_size_t(0)
+    cap = max(getattr(T, "_ngb_cap", 0), 32 * nq, 1)
+    found = torch.empty(cap, dtype=torch.int32, device=dev)
+    L.check(call(found.data_ptr(), cap, C.byref(total)), who)
+    if total.value > cap:
+        cap = int(total.value)
+        found = torch.empty(cap, dtype=torch.int32, device=dev)
+        L.check(call(found.data_ptr(), cap, C.byref(total)), who)
+    T._ngb_cap = int(total.value)
+    T.ctx.synchronize()
+    return found[:int(total.value)].view(torch.uint32)
+
+
 def ngb_search(T: Tree, centres, radius, tau: float = 0.0):
     """shmem::ngb_search (hydro.h:73-75) for many centres: (start, found) device tensors, CSR —
     the particles within radius[q] of centres[q] (n_q x 3), each list in Morton order."""
@@ -666,20 +684,13 @@ def ngb_search(T: Tree, centres, radius, tau: float = 0.0):
     r = _dev_f64(radius, dev)
     nq = int(r.shape[0])
     start = torch.zeros(nq + 1, dtype=torch.int64, device=dev)
-    total = C.c_size_t(0)
     lib = L.load()
     T.ctx.enter()
-    L.check(lib.gz_tree_ngb_search(T.ctx.handle, T.handle, *(a.data_ptr() for a in cs),
-                                   r.data_ptr(), nq, float(tau), start.data_ptr(), None, 0,
-                                   C.byref(total)), "gz_tree_ngb_search")
-    found = torch.empty(max(int(total.value), 1), dtype=torch.int32, device=dev)
-    if total.value:
-        L.check(lib.gz_tree_ngb_search(T.ctx.handle, T.handle, *(a.data_ptr() for a in cs),
-                                       r.data_ptr(), nq, float(tau), start.data_ptr(),
-                                       found.data_ptr(), int(total.value), C.byref(total)),
-                "gz_tree_ngb_search")
-    T.ctx.synchronize()
-    return start, found[:int(total.value)].view(torch.uint32)
+    args = (T.ctx.handle, T.handle, *(a.data_ptr() for a in cs), r.data_ptr(), nq, float(tau),
+            start.data_ptr())
+    found = _ngb_lists(T, lambda *out: lib.gz_tree_ngb_search(*args, *out),
+                       "gz_tree_ngb_search", nq)
+    return start, found
 
 
 def ngb_search_reverse(T: Tree, h, n_gas: int, centres, h_q, tau: float = 0.0):
@@ -697,19 +708,13 @@ def ngb_search_reverse(T: Tree, h, n_gas: int, centres, h_q, tau: float = 0.0):
     r = _dev_f64(h_q, dev)
     nq = int(r.shape[0])
     start = torch.zeros(nq + 1, dtype=torch.int64, device=dev)
-    total = C.c_size_t(0)
     lib = L.load()
     T.ctx.enter()
     args = (T.ctx.handle, T.handle, hh.data_ptr(), int(n_gas), *(a.data_ptr() for a in cs),
             r.data_ptr(), nq, float(tau), start.data_ptr())
-    L.check(lib.gz_tree_ngb_search_reverse(*args, None, 0, C.byref(total)),
-            "gz_tree_ngb_search_reverse")
-    found = torch.empty(max(int(total.value), 1), dtype=torch.int32, device=dev)
-    if total.value:
-        L.check(lib.gz_tree_ngb_search_reverse(*args, found.data_ptr(), int(total.value),
-                                               C.byref(total)), "gz_tree_ngb_search_reverse")
-    T.ctx.synchronize()
-    return start, found[:int(total.value)].view(torch.uint32)
+    found = _ngb_lists(T, lambda *out: lib.gz_tree_ngb_search_reverse(*args, *out),
+                       "gz_tree_ngb_search_reverse", nq)
+    return start, found
 
 
 def potential(T: Tree, P: Particles, targets, theta: float, G: float = 1.0, aold=None):
