@@ -23,6 +23,7 @@
 
 #include <hip/hip_runtime_api.h>
 
+#include <algorithm>
 #include <cstddef>
 #include <cstdint>
 #include <memory>
@@ -539,17 +540,18 @@ inline void ngb_search_many(const Tree& T, const Particles& P, const std::vector
     const double* d_r = w.aold.upload(radius);
     detail::DevArray<size_t> start;
     start.reserve(nq + 1);
-    size_t total = 0;
-    detail::check(gz_tree_ngb_search(detail::ctx(), WalkAccess::tree(T), d_c, d_c + nq,
-                                     d_c + 2 * nq, d_r, nq, T.t_since_build, start.get(), nullptr,
-                                     0, &total),
-                  "gz_tree_ngb_search");
+    // the lists into a buffer of 32 entries per query, searched again only if they need more
+    size_t total = 0, cap = std::max<size_t>(32 * nq, 1);
     detail::DevArray<uint32_t> hits;
-    hits.reserve(total + 1);
-    detail::check(gz_tree_ngb_search(detail::ctx(), WalkAccess::tree(T), d_c, d_c + nq,
-                                     d_c + 2 * nq, d_r, nq, T.t_since_build, start.get(),
-                                     hits.get(), total, &total),
-                  "gz_tree_ngb_search");
+    for (;;) {
+        hits.reserve(cap);
+        detail::check(gz_tree_ngb_search(detail::ctx(), WalkAccess::tree(T), d_c, d_c + nq,
+                                         d_c + 2 * nq, d_r, nq, T.t_since_build, start.get(),
+                                         hits.get(), cap, &total),
+                      "gz_tree_ngb_search");
+        if (total <= cap) break;
+        cap = total;
+    }
     detail::sync();
     std::vector<size_t> hs(nq + 1);
     std::vector<uint32_t> hh(total);
@@ -593,18 +595,19 @@ inline void ngb_search_reverse_many(const Tree& T, const Particles& P, const dou
     const double* d_h = w.extra.upload(std::vector<double>(h, h + n_gas));
     detail::DevArray<size_t> start;
     start.reserve(nq + 1);
-    size_t total = 0;
-    detail::check(gz_tree_ngb_search_reverse(detail::ctx(), WalkAccess::tree(T), d_h, n_gas,
-                                             d_c, d_c + nq, d_c + 2 * nq, d_r, nq,
-                                             T.t_since_build, start.get(), nullptr, 0, &total),
-                  "gz_tree_ngb_search_reverse");
+    // the lists into a buffer of 32 entries per query, searched again only if they need more
+    size_t total = 0, cap = std::max<size_t>(32 * nq, 1);
     detail::DevArray<uint32_t> hits;
-    hits.reserve(total + 1);
-    detail::check(gz_tree_ngb_search_reverse(detail::ctx(), WalkAccess::tree(T), d_h, n_gas,
-                                             d_c, d_c + nq, d_c + 2 * nq, d_r, nq,
-                                             T.t_since_build, start.get(), hits.get(), total,
-                                             &total),
-                  "gz_tree_ngb_search_reverse");
+    for (;;) {
+        hits.reserve(cap);
+        detail::check(gz_tree_ngb_search_reverse(detail::ctx(), WalkAccess::tree(T), d_h, n_gas,
+                                                 d_c, d_c + nq, d_c + 2 * nq, d_r, nq,
+                                                 T.t_since_build, start.get(), hits.get(), cap,
+                                                 &total),
+                      "gz_tree_ngb_search_reverse");
+        if (total <= cap) break;
+        cap = total;
+    }
     detail::sync();
     std::vector<size_t> hs(nq + 1);
     std::vector<uint32_t> hh(total);
