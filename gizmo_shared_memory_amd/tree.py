@@ -658,18 +658,20 @@ def density(T: Tree, P: Particles, targets, des_ngb: float = 32.0, ngb_tol: floa
 
 def _ngb_lists(T: Tree, call, who: str, nq: int):
     """The lists of a search whose C call ends in (found, cap, &total): first into a buffer sized
-    from the tree's last search (at least 32 entries per query), so that one pass of counting
-    usually serves; a second call only when the lists need more."""
+    from the tree's last search (its mean list length and an eighth more, at least 32 entries per
+    query, at most 2^28 entries in all), so that one pass of counting usually serves; a second
+    call only when the lists need more."""
     dev = T.P.device
     total = C.c_size_t(0)
-    cap = max(getattr(T, "_ngb_cap", 0), 32 * nq, 1)
+    per_query = max(32.0, 1.125 * getattr(T, "_ngb_per_query", 0.0))
+    cap = max(1, min(int(per_query * nq) + 1, 1 << 28))
     found = torch.empty(cap, dtype=torch.int32, device=dev)
     L.check(call(found.data_ptr(), cap, C.byref(total)), who)
     if total.value > cap:
         cap = int(total.value)
         found = torch.empty(cap, dtype=torch.int32, device=dev)
         L.check(call(found.data_ptr(), cap, C.byref(total)), who)
-    T._ngb_cap = int(total.value)
+    T._ngb_per_query = int(total.value) / max(nq, 1)
     T.ctx.synchronize()
     return found[:int(total.value)].view(torch.uint32)
 
