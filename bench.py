@@ -248,7 +248,8 @@ def widened_rows(gz, ics, args):
     leaf_n = (td["phi"] - td["plo"])[td["leaf_of"]].astype(np.float64)
     h_leaf = side * np.cbrt(3.0 * 32.0 / (4.0 * np.pi * leaf_n))
     h0 = h_leaf[order]
-    (_, rho_d, nngb_d, it_d), dt = timed(lambda: gz.density(T, P, tg, 32.0, 1.0, h_start=h0))
+    h0_d = torch.as_tensor(h0, device=P.device)
+    (_, rho_d, nngb_d, it_d), dt = timed(lambda: gz.density(T, P, tg, 32.0, 1.0, h_start=h0_d))
     it_h = it_d.cpu().numpy()
     out["density"] = {"targets": n, "ms_per_call": dt * 1e3, "targets_per_s": n / dt,
                       "mean_iterations": float(it_h.mean()),
